@@ -1,0 +1,227 @@
+/*
+ * tdx.h -- C ABI of the MI355X residual/loss engine for TEDEouS-style solvers.
+ *
+ * The library replaces, for the 'NN' and 'mat' solution modes, the per-step
+ * objective of the reference:
+ *
+ *   Solution.evaluate()             tedeous/solution.py:129-168
+ *     Operator.operator_compute()   tedeous/eval.py:167-193, 223-232
+ *       Derivative_NN.take_derivative   tedeous/derivative.py:30-58
+ *       Derivative_mat.take_derivative  tedeous/derivative.py:293-323
+ *     Bounds.apply_bcs()            tedeous/eval.py:283-461 (+ dict_to_matrix 55-87)
+ *     Losses._default_loss()        tedeous/losses.py:84-135
+ *   loss.backward()                 tedeous/optimizers/closure.py:60
+ *   torch.optim.Adam.step()         tedeous/optimizers/optimizer.py:44-45, 63-66
+ *
+ * Conventions (all entry points):
+ *   - Plain pointers and sizes only.  Every device buffer is owned by the
+ *     caller: parameters, gradients, outputs and the workspace.  A plan owns
+ *     only immutable tables uploaded by its create call.
+ *   - All work is enqueued on the given stream; nothing synchronises the
+ *     host, nothing allocates, so a step can be captured in a HIP graph.
+ *   - Return 0 on success, a negative TDX_E* code otherwise; the message of
+ *     the last error on the calling thread is returned by tdx_last_error().
+ *   - dtype 0 = float32 (the product path), 1 = float64 (the same kernels
+ *     instantiated in double precision, used for tight parity checks).
+ */
+#ifndef TDX_H
+#define TDX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDX_OK 0
+#define TDX_EINVAL (-1)
+#define TDX_EUNSUPPORTED (-2)
+#define TDX_ELAUNCH (-3)
+#define TDX_ENOMEM (-4)
+
+typedef void *tdx_stream_t; /* a hipStream_t; NULL = the null stream */
+
+/* ABI version; bumped on any layout change of the structs below. */
+int tdx_version(void);
+/* Message of the last failure on the calling thread ("" if none). */
+const char *tdx_last_error(void);
+
+/* ===================================================================== */
+/* 'NN' mode                                                              */
+/* ===================================================================== */
+/*
+ * The network is the reference's tanh MLP (nn.Sequential of Linear/Tanh):
+ * d_in -> width -> ... -> width -> n_out with n_layers Linear layers.
+ * Parameters of one instance are stored exactly as torch's
+ * net.parameters() flattened: W1[width][d_in], b1, W2[width][width], b2,
+ * ..., W_L[n_out][width], b_L.  n_inst instances of the same architecture
+ * are stored back to back (stride tdx_nn_param_count()).
+ *
+ * The residual program is a list of groups (see torch_de_solver_amd/
+ * compiler.py).  Group g has n_entries entries and n_slots slots; the
+ * network input of (slot s, entry e) is
+ *     data[slot_base[s] + e*d_in + a] + reals[slot_off + s*d_in + a]
+ * i.e. the reference's shifted grid x + fl(shift*h) (tedeous/
+ * input_preprocessing.py:239-264), generated in the kernel.
+ * Tables (int32 records in `ints`, working-precision values in `reals`
+ * and `data`):
+ *   group   [TDX_GROUP_INTS]: inst, n_entries, n_slots, slot_base_i,
+ *           slot_off_r, n_eq, eq_i, row_base
+ *   slot_base_i -> n_slots ints: data offset of entry 0 of the slot's base
+ *   eq      [TDX_EQ_INTS]: n_terms, term_i, target_off (-1: zero target),
+ *           resid_off (-1: none), lam_r (lambda/denominator),
+ *           invden_r (1/denominator)
+ *   term    [TDX_TERM_INTS]: n_factors, factor_i, coeff_kind
+ *           (0: reals[coeff_idx], 1: data[coeff_idx + entry]), coeff_idx
+ *   factor  [TDX_FACTOR_INTS]: var, pow_kind (0: integer pow_int,
+ *           1: real reals[pow_r]), pow_int, n_stencil, stencil_i, pow_r
+ *   stencil [TDX_STENCIL_INTS]: slot, weight_r
+ * Seed rows: row (group g, slot s, entry e) = row_base + s*n_entries + e.
+ */
+#define TDX_GROUP_INTS 8
+#define TDX_EQ_INTS 6
+#define TDX_TERM_INTS 4
+#define TDX_FACTOR_INTS 6
+#define TDX_STENCIL_INTS 2
+
+typedef struct tdx_nn_desc {
+  int dtype;
+  int d_in, n_out, width, n_layers;
+  int n_inst;
+  int n_groups;
+  const int32_t *groups; /* n_groups * TDX_GROUP_INTS */
+  int64_t n_ints;
+  const int32_t *ints;
+  int64_t n_reals;
+  const void *reals; /* dtype values */
+  int64_t n_data;
+  const void *data; /* dtype values: coordinates, coefficients, targets */
+  int64_t n_resid;  /* length of the optional residual output */
+  int tile_rows;    /* rows per backward tile (0: default) */
+  int chunk_tiles;  /* tiles per backward work item (0: default) */
+} tdx_nn_desc;
+
+typedef struct tdx_nn_plan tdx_nn_plan;
+
+/* Upload the tables and build the work lists.  Replaces the compile step
+ * Solution.__init__ (tedeous/solution.py:23-88) for 'NN' mode. */
+int tdx_nn_plan_create(const tdx_nn_desc *desc, tdx_nn_plan **out);
+void tdx_nn_plan_destroy(tdx_nn_plan *plan);
+/* Parameters per instance (torch layout). */
+int64_t tdx_nn_param_count(const tdx_nn_plan *plan);
+/* Bytes of caller-provided device workspace for every call below. */
+size_t tdx_nn_workspace_bytes(const tdx_nn_plan *plan);
+
+/* Residual + boundary + loss, forward.  Writes loss[2*i] (lambda-weighted)
+ * and loss[2*i+1] (normalized, all lambdas 1) for every instance i, and the
+ * residual values to resid (may be NULL).  Replaces Solution.evaluate()
+ * (tedeous/solution.py:129-168) in 'NN' mode. */
+int tdx_nn_forward(const tdx_nn_plan *plan, const void *params, void *loss,
+                   void *resid, void *workspace, tdx_stream_t stream);
+/* Adjoint of tdx_nn_forward: dparams = dloss[0] * d(sum_i loss_i)/dparams.
+ * dloss is a device pointer to one value.  Must follow a tdx_nn_forward with
+ * the same params and workspace (it reads the seeds that call stored).
+ * Replaces loss.backward() (tedeous/optimizers/closure.py:60). */
+int tdx_nn_backward(const tdx_nn_plan *plan, const void *params,
+                    const void *dloss, void *dparams, void *workspace,
+                    tdx_stream_t stream);
+/* One fused training step: forward, adjoint and torch.optim.Adam update
+ * (no weight decay, amsgrad off) of all instances.  step is a device int64
+ * holding the number of completed steps; it is incremented by the call.
+ * loss receives the pre-update losses as in tdx_nn_forward. */
+int tdx_nn_train_step(const tdx_nn_plan *plan, void *params, void *exp_avg,
+                      void *exp_avg_sq, int64_t *step, double lr,
+                      double beta1, double beta2, double eps, void *loss,
+                      void *workspace, tdx_stream_t stream);
+
+/* ===================================================================== */
+/* 'mat' mode                                                             */
+/* ===================================================================== */
+/*
+ * The unknown is a grid tensor u[n_eq][n_0][n_1] (1-D grids use n_0 = 1;
+ * the single axis is axis 1).  D_a is the reference's per-axis stencil
+ * (tedeous/derivative.py:199-291): interior (sum_b alpha_back[b] u[i+b-dp+1]
+ * + sum_f alpha_forw[f] u[i+f]) / (2 h_a), first dp-1 points forw/h_a,
+ * last dp-1 points back/h_a.
+ * Tables:
+ *   eq      [TDX_MEQ_INTS]: n_terms, term_i, lam_r (lambda/N)
+ *   term    [TDX_MTERM_INTS]: n_factors, factor_i, coeff_kind
+ *           (0: reals[coeff_idx], 1: data[coeff_idx + flat point]), coeff_idx
+ *   factor  [TDX_MFACTOR_INTS]: var, pow_kind, pow_int, pow_r, n_axes,
+ *           axis0, axis1, axis2 (in application order)
+ *   bc entry [TDX_MBC_INTS]: n_sets, pos_i (ints: n_sets flat positions),
+ *           n_terms (0: the plain value u[var]), term_i, var, target_r
+ *           (data index of the padded target), lam_r (lambda/max_len),
+ *           invden_r (1/max_len), seed_i (first seed slot of this entry)
+ *   bc adjoint CSR over touched grid points: bct_ptr[n_touch+1],
+ *           bct_pos[n_touch] (flat index into u incl. the n_eq plane),
+ *           bct_seed[nnz], bct_w_r[nnz] (reals index of the weight)
+ */
+#define TDX_MEQ_INTS 3
+#define TDX_MTERM_INTS 4
+#define TDX_MFACTOR_INTS 8
+#define TDX_MBC_INTS 10
+
+typedef struct tdx_mat_desc {
+  int dtype;
+  int n_eq;
+  int n0, n1; /* grid shape; 1-D grids: n0 = 1 */
+  int dp;     /* derivative_points */
+  double h[2];
+  const double *alpha_back; /* dp values */
+  const double *alpha_forw; /* dp values */
+  int eq_i;                 /* ints offset of the n_eq equation records */
+  int n_bc;                 /* boundary entries */
+  int bc_i;                 /* ints offset of the bc entry records */
+  int n_bc_seeds;           /* seed slots over all bc entries */
+  int n_touch;              /* grid points touched by bc adjoints */
+  int bct_ptr_i, bct_pos_i, bct_seed_i, bct_w_i; /* ints offsets */
+  int64_t n_ints;
+  const int32_t *ints;
+  int64_t n_reals;
+  const void *reals;
+  int64_t n_data;
+  const void *data;
+  int exact_div; /* 1: IEEE division by 2h and h (bit-identical fields) */
+} tdx_mat_desc;
+
+typedef struct tdx_mat_plan tdx_mat_plan;
+
+int tdx_mat_plan_create(const tdx_mat_desc *desc, tdx_mat_plan **out);
+void tdx_mat_plan_destroy(tdx_mat_plan *plan);
+size_t tdx_mat_workspace_bytes(const tdx_mat_plan *plan);
+
+/* Forward: loss[0] (lambda-weighted), loss[1] (normalized); op receives the
+ * residual field [n_eq][n0*n1] (may be NULL); bval receives the boundary
+ * values, one per bc entry (may be NULL).  Replaces Solution.evaluate()
+ * (tedeous/solution.py:129-168) in 'mat' mode. */
+int tdx_mat_forward(const tdx_mat_plan *plan, const void *u, void *loss,
+                    void *op, void *bval, void *workspace, tdx_stream_t stream);
+/* grad = dloss[0] * dloss/du (dloss: device pointer to one value).
+ * Replaces loss.backward() for the grid tensor. */
+int tdx_mat_backward(const tdx_mat_plan *plan, const void *u,
+                     const void *dloss, void *grad, void *workspace,
+                     tdx_stream_t stream);
+/* Fused step: gradient and torch.optim.Adam update of u (see
+ * tdx_nn_train_step for step and loss). */
+int tdx_mat_train_step(const tdx_mat_plan *plan, void *u, void *exp_avg,
+                       void *exp_avg_sq, int64_t *step, double lr,
+                       double beta1, double beta2, double eps, void *loss,
+                       void *workspace, tdx_stream_t stream);
+
+/* ===================================================================== */
+/* Optimizer                                                              */
+/* ===================================================================== */
+/* torch.optim.Adam update of n values (weight_decay 0, amsgrad off,
+ * maximize off): step is a device int64 of completed steps, read as t and
+ * not modified; pass a separate tdx_step_increment after the update. */
+int tdx_adam(int dtype, int64_t n, void *param, const void *grad,
+             void *exp_avg, void *exp_avg_sq, const int64_t *step, double lr,
+             double beta1, double beta2, double eps, tdx_stream_t stream);
+int tdx_step_increment(int64_t *step, tdx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TDX_H */

@@ -1,0 +1,539 @@
+"""CPU oracle: an op-for-op restatement of the reference's residual and loss.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product imports this module; it is
+used by tests/, by ``__graft_entry__.smoke()`` and by the CPU-baseline leg of
+bench.py, always as the checker (never as the thing measured or shipped).
+
+It restates, in torch CPU ops (the arithmetic library the reference itself
+computes with), the reference's 'NN' and 'mat' evaluation path:
+
+* grids and conditions        tedeous/data.py:41-107, 241-323
+* point typing (qhull)        tedeous/points_type.py:21-125
+* stencil schemes             tedeous/finite_diffs.py:9-268
+* operator/bc preparation     tedeous/input_preprocessing.py:51-81, 173-408,
+                              511-594
+* derivatives                 tedeous/derivative.py:18-58, 135-323
+* residual / bcs              tedeous/eval.py:55-87, 143-193, 283-461
+* loss                        tedeous/losses.py:84-135
+* training loop               tedeous/model.py:154-188,
+                              tedeous/optimizers/closure.py:49-64
+
+The structure follows the reference's data flow (materialised shifted grids,
+flattened ``torch.roll`` stencils, padded boundary matrices) so that, in
+float32, the residual fields come out with the same rounding as the
+reference's.  Parity checks of this module against the reference itself
+live in tests/test_oracle.py (golden vectors written by
+tests/golden/make_golden.py, closed forms, brute force).
+"""
+
+from copy import copy, deepcopy
+from typing import Callable, Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+
+# ----------------------------------------------------------------------------
+# problem definition
+# ----------------------------------------------------------------------------
+
+def axis_linspace(a: float, b: float, n_points: int, dtype=torch.float32) -> torch.Tensor:
+    """``Domain.variable(name, [a, b], n)``: n + 1 points (tedeous/data.py:62-67)."""
+    return torch.linspace(a, b, n_points + 1, dtype=dtype)
+
+
+def grid_points(axes: List[torch.Tensor]) -> torch.Tensor:
+    """'NN' grid ``[N, d]`` (tedeous/data.py:83-88)."""
+    if len(axes) == 1:
+        return axes[0].reshape(-1, 1)
+    return torch.cartesian_prod(*axes)
+
+
+def grid_mesh(axes: List[torch.Tensor]) -> torch.Tensor:
+    """'mat' grid ``[d, n1, ...]`` (tedeous/data.py:102-104)."""
+    m = np.meshgrid(*[a.numpy() for a in axes], indexing='ij')
+    return torch.tensor(np.array(m))
+
+
+def condition_points(bnd, axes: Dict[str, torch.Tensor]) -> torch.Tensor:
+    """Point set of one condition (tedeous/data.py:241-286)."""
+    dtype = next(iter(axes.values())).dtype
+    if isinstance(bnd, torch.Tensor):
+        out = bnd.to(dtype)
+    else:
+        parts = []
+        for name, ax in axes.items():
+            spec = bnd[name]
+            if isinstance(spec, torch.Tensor):
+                parts.append(spec.to(dtype))
+            elif isinstance(spec, (int, float)):
+                parts.append(torch.tensor([spec]).to(dtype))
+            else:
+                lo, hi = spec
+                parts.append(ax[(ax >= lo) & (ax <= hi)].to(dtype))
+        out = torch.cartesian_prod(*parts).to(dtype)
+    return out.reshape(-1, 1) if out.dim() == 1 else out
+
+
+def unify(op: Optional[dict]) -> Optional[dict]:
+    """Per-factor list form of every term (tedeous/input_preprocessing.py:51-81)."""
+    if op is None:
+        return None
+    op = deepcopy(op)
+    for term in op.values():
+        key = list(term.keys())[1]
+        scalar = isinstance(term['pow'], (int, float)) or callable(term['pow'])
+        if 'var' not in term:
+            if scalar:
+                term[key], term['pow'], term['var'] = [term[key]], [term['pow']], [0]
+            else:
+                term['var'] = [0] * len(term['pow'])
+        elif scalar:
+            term[key], term['pow'], term['var'] = [term[key]], [term['pow']], [term['var']]
+    return op
+
+
+def build_conditions(conds: List[dict], axes: Dict[str, torch.Tensor]) -> List[dict]:
+    """Resolve point sets and values (tedeous/data.py:288-323)."""
+    dtype = next(iter(axes.values())).dtype
+    out = []
+    for c in conds:
+        c = dict(c)
+        c['bop'] = unify(c.get('bop'))
+        if c['type'] == 'periodic':
+            c['bnd'] = [condition_points(b, axes) for b in c['bnd']]
+        else:
+            c['bnd'] = condition_points(c['bnd'], axes)
+        v = c['bval']
+        if isinstance(v, torch.Tensor):
+            c['bval'] = v.to(dtype)
+        elif isinstance(v, (int, float)):
+            c['bval'] = (torch.ones_like(c['bnd'][:, 0]) * v).to(dtype)
+        else:
+            c['bval'] = v(c['bnd']).to(dtype)
+        out.append(c)
+    return out
+
+
+# ----------------------------------------------------------------------------
+# 'NN' mode
+# ----------------------------------------------------------------------------
+
+def point_types(grid: torch.Tensor) -> List[str]:
+    """Per-point type via the qhull hull test (tedeous/points_type.py:39-106)."""
+    from scipy.spatial import Delaunay
+    n, d = grid.shape
+    tests = []
+    tri = Delaunay(grid.numpy()) if d > 1 else None
+    for ax in range(d):
+        for sign in (1, -1):
+            p = grid.clone()
+            p[:, ax] = grid[:, ax] + sign * 0.0001
+            if d > 1:
+                tests.append(tri.find_simplex(p.numpy()) >= 0)
+            else:
+                tests.append(((p <= grid.max()) & (p >= grid.min())).numpy().reshape(-1))
+    tests = np.array(tests).T
+    out = []
+    for i in range(n):
+        if d == 1 or tests[i].all():
+            out.append('central')
+        else:
+            out.append(''.join('f' if tests[i, 2 * a] else 'b' for a in range(d)))
+    return out
+
+
+def scheme(term_axes, nvars: int, ptype: str, label: str, h: float):
+    """(shift vectors, weights) of one factor (tedeous/finite_diffs.py:9-268)."""
+    if term_axes == [None]:
+        return [None], [1]
+    if label == '1':
+        dirs = ['central'] * len(term_axes) if ptype == 'central' else [ptype[a] for a in term_axes]
+        shifts = [[0] * nvars]
+        for a, dr in zip(term_axes, dirs):
+            nxt = []
+            for s in shifts:
+                sp, sm = copy(s), copy(s)
+                if dr == 'central':
+                    sp[a] += 1
+                    sm[a] -= 1
+                elif dr == 'f':
+                    sp[a] += 1
+                else:
+                    sm[a] -= 1
+                nxt += [sp, sm]
+            shifts = nxt
+        signs = [1]
+        central = np.unique(dirs)[0] == 'central'
+        for _ in term_axes:
+            nxt = []
+            for sg in signs:
+                nxt += ([sg * (1 / (2 * h)), -sg * (1 / (2 * h))] if central
+                        else [sg / h, -sg / h])
+            signs = nxt
+        return shifts, signs
+    dirs = [ptype[a] for a in term_axes]
+    shifts = [[0] * nvars]
+    for a, dr in zip(term_axes, dirs):
+        nxt = []
+        for s in shifts:
+            s3, s2, s1 = copy(s), copy(s), copy(s)
+            step = 1 if dr == 'f' else -1
+            s3[a] += 2 * step
+            s2[a] += step
+            nxt += [s3, s2, s1]
+        shifts = nxt
+    signs = [1]
+    for dr in dirs:
+        nxt = []
+        for sg in signs:
+            if dr == 'f':
+                nxt += [3 * (1 / (2 * h)) * sg, -4 * (1 / (2 * h)) * sg, (1 / (2 * h)) * sg]
+            else:
+                nxt += [-3 * (1 / (2 * h)) * sg, 4 * (1 / (2 * h)) * sg, -(1 / (2 * h)) * sg]
+        signs = nxt
+    return shifts, signs
+
+
+def _shift(grid: torch.Tensor, axis: int, amount: float) -> torch.Tensor:
+    out = grid.clone()
+    out[:, axis] = grid[:, axis] + amount
+    return out
+
+
+def _positions(grid: torch.Tensor, pts: torch.Tensor) -> List[int]:
+    """Grid positions of points (tedeous/input_preprocessing.py:84-170)."""
+    g = grid.double()
+    out = []
+    for p in pts.double():
+        m = torch.where(torch.all(torch.isclose(g, p), dim=1))[0]
+        if m.numel() == 1:
+            out.append(int(m))
+        else:
+            out.append(int(torch.argmin(torch.linalg.norm(g - p, dim=1))))
+    return out
+
+
+def nn_prepare_operator(op: dict, grid: torch.Tensor, pts: torch.Tensor, ptype: str,
+                        h: float, inner: str, boundary: str) -> dict:
+    """Materialise the shifted grids of every factor
+    (tedeous/input_preprocessing.py:208-345)."""
+    op = unify(op)
+    label = inner if ptype == 'central' else boundary
+    nvars = grid.shape[-1]
+    for term in op.values():
+        key = list(term.keys())[1]
+        c = term['coeff']
+        if isinstance(c, (int, float)):
+            pass
+        elif callable(c):
+            term['coeff'] = (c, pts)
+        elif isinstance(c, torch.Tensor):
+            term['coeff'] = c[_positions(grid, pts)].reshape(-1, 1)
+        schemes, signs = [], []
+        for axes in term[key]:
+            sh, sg = scheme(axes, nvars, ptype, label, h)
+            grids = []
+            for s in sh:
+                if s is None:
+                    grids.append(pts)
+                    continue
+                x = pts
+                for ax, k in enumerate(s):
+                    x = _shift(x, ax, k * h)
+                grids.append(x)
+            schemes.append(grids)
+            signs.append(sg)
+        term[key] = [schemes, signs]
+    return op
+
+
+def nn_term_value(net, term: dict) -> torch.Tensor:
+    """One term (tedeous/derivative.py:30-58)."""
+    key = list(term.keys())[1]
+    c = term['coeff']
+    if isinstance(c, tuple):
+        c = c[0](c[1]).reshape(-1, 1)
+    der = 1.
+    for j, grids in enumerate(term[key][0]):
+        s = 0.
+        for k, x in enumerate(grids):
+            s += net(x)[:, term['var'][j]].reshape(-1, 1) * term[key][1][j][k]
+        p = term['pow'][j]
+        der = der * s ** p if isinstance(p, (int, float)) else p(der * s)
+    return c * der
+
+
+def apply_operator(value_fn, op: dict) -> torch.Tensor:
+    """In-order sum of term values (tedeous/eval.py:143-165)."""
+    total = None
+    for term in op.values():
+        v = value_fn(term)
+        total = v if total is None else total + v
+    return total
+
+
+def pad_matrix(bval: Dict[str, torch.Tensor], true: Dict[str, torch.Tensor]):
+    """Columns per type, zero-padded to the longest (tedeous/eval.py:55-87,
+    tedeous/utils.py:276-290)."""
+    keys = list(bval.keys())
+    max_len = max(len(v) for v in bval.values())
+
+    def pad(x):
+        return torch.nn.functional.pad(x, (0, max_len - x.size(-1))) if x.size(-1) < max_len else x
+
+    b = torch.hstack([pad(bval[k]).reshape(-1, 1) for k in keys])
+    t = torch.hstack([pad(true[k]).reshape(-1, 1) for k in keys])
+    return b, t, keys, [len(bval[k]) for k in keys]
+
+
+def lam_row(val: torch.Tensor, lam) -> torch.Tensor:
+    """(tedeous/input_preprocessing.py:14-43)"""
+    if isinstance(lam, torch.Tensor):
+        return lam
+    if isinstance(lam, (int, float)):
+        return (torch.ones(val.shape[-1]) * lam).reshape(1, -1)
+    return torch.tensor(lam).reshape(1, -1)
+
+
+def default_loss(op, bval, true, lam_op, lam_bnd):
+    """(tedeous/losses.py:84-135)"""
+    mse_op = torch.mean((op - torch.zeros(op.shape, dtype=op.dtype)) ** 2, 0)
+    mse_b = torch.mean((bval - true) ** 2, 0)
+    loss = mse_op @ lam_op.T + mse_b @ lam_bnd.T
+    with torch.no_grad():
+        norm = mse_op @ torch.ones_like(lam_op).T + mse_b @ torch.ones_like(lam_bnd).T
+    return loss, norm
+
+
+class NNOracle:
+    """Reference-semantics evaluation of an 'NN' problem on the CPU."""
+
+    def __init__(self, axes: Dict[str, torch.Tensor], operators: List[dict], conds: List[dict],
+                 h: float = 0.001, inner_order: str = '1', boundary_order: str = '2'):
+        self.grid = grid_points(list(axes.values()))
+        self.types = point_types(self.grid)
+        cidx = [i for i, t in enumerate(self.types) if t == 'central']
+        self.central = self.grid[cidx]
+        self.ops = [nn_prepare_operator(op, self.grid, self.central, 'central', h,
+                                        inner_order, boundary_order) for op in operators]
+        self.conds = build_conditions(conds, axes)
+        for c in self.conds:
+            if c['bop'] is not None and c['type'] == 'operator':
+                # per point type of the condition's points (tedeous/input_preprocessing.py:390-408)
+                prepared = []
+                lut = {tuple(r): i for i, r in enumerate(self.grid.tolist())}
+                typ = [self.types[lut[tuple(p)]] for p in c['bnd'].tolist()]
+                for t in dict.fromkeys(typ):
+                    sel = [i for i, tt in enumerate(typ) if tt == t]
+                    prepared.append((sel, nn_prepare_operator(c['bop'], self.grid, c['bnd'][sel], t,
+                                                              h, inner_order, boundary_order)))
+                c['prepared'] = prepared
+
+    def residual(self, net) -> torch.Tensor:
+        cols = [apply_operator(lambda t: nn_term_value(net, t), op).reshape(-1, 1) for op in self.ops]
+        return cols[0] if len(cols) == 1 else torch.cat(cols, 1)
+
+    def boundary(self, net):
+        bval, true = {}, {}
+        for c in self.conds:
+            if c['type'] == 'dirichlet':
+                v = net(c['bnd'])[:, c['var']].reshape(-1, 1)
+            elif c['type'] == 'periodic' and c['bop'] is None:
+                v = net(c['bnd'][0])[:, c['var']].reshape(-1, 1)
+                for b in c['bnd'][1:]:
+                    v = v - net(b)[:, c['var']].reshape(-1, 1)
+            elif c['type'] == 'operator':
+                # each point keeps its own target (see the compiler's note)
+                v = torch.zeros(len(c['bnd']), 1, dtype=self.grid.dtype)
+                parts = []
+                for sel, op in c['prepared']:
+                    parts.append((sel, apply_operator(lambda t: nn_term_value(net, t), op)))
+                v = torch.cat([p for _, p in parts]).reshape(-1, 1)
+                order = [i for sel, _ in parts for i in sel]
+                inv = torch.empty(len(order), dtype=torch.long)
+                inv[torch.tensor(order)] = torch.arange(len(order))
+                v = v[inv]
+            else:
+                raise NotImplementedError(c['type'])
+            k = c['type']
+            if k in bval:
+                bval[k] = torch.cat((bval[k], v.reshape(-1)))
+                true[k] = torch.cat((true[k], c['bval'].reshape(-1)))
+            else:
+                bval[k] = v.reshape(-1)
+                true[k] = c['bval'].reshape(-1)
+        return pad_matrix(bval, true)
+
+    def evaluate(self, net, lam_op=1, lam_bnd=1):
+        op = self.residual(net)
+        b, t, keys, lens = self.boundary(net)
+        loss, norm = default_loss(op, b, t, lam_row(op, lam_op).to(op.dtype),
+                                  lam_row(b, lam_bnd).to(op.dtype))
+        return loss, norm, op, b, t, keys, lens
+
+
+# ----------------------------------------------------------------------------
+# 'mat' mode
+# ----------------------------------------------------------------------------
+
+def mat_alphas(dp: int):
+    """Vandermonde solve for the one-sided coefficients (tedeous/derivative.py:157-197)."""
+    from scipy import linalg
+    back = list(range(-dp + 1, 1))
+    forw = list(range(dp))
+
+    def solve(labels):
+        lab = np.array(labels)
+        A = np.array([lab ** i for i in range(len(labels))])
+        b = np.zeros_like(lab)
+        b[1] = 1
+        return linalg.solve(A, b)
+
+    return back, forw, solve(back), solve(forw)
+
+
+class MatDerivative:
+    """Per-axis stencil through flattened rolls (tedeous/derivative.py:199-323)."""
+
+    def __init__(self, dp: int):
+        self.back, self.forw, self.ab, self.af = mat_alphas(dp)
+        n = len(self.back) - 1
+        self.back_idx = [-i for i in range(1, n + 1)]
+        self.forw_idx = list(range(n))
+
+    def _combine(self, u):
+        b, f = 0, 0
+        for i, (sb, sf) in enumerate(zip(self.back, self.forw)):
+            b += torch.roll(u, -sb) * self.ab[i]
+            f += torch.roll(u, -sf) * self.af[i]
+        return b, f
+
+    @staticmethod
+    def step(grid: torch.Tensor, axis: int):
+        flat = torch.vstack([grid[i].reshape(-1) for i in range(grid.shape[0])]).T.float()
+        v = torch.unique(flat[:, axis])
+        return abs(v[1] - v[0])
+
+    def d(self, u: torch.Tensor, h, axis: int) -> torch.Tensor:
+        if u.dim() == 1 or u.shape[0] == 1:
+            shape = u.shape
+            x = u.reshape(-1)
+            b, f = self._combine(x)
+            du = (b + f) / (2 * h)
+            du[self.back_idx] = b[self.back_idx] / h
+            du[self.forw_idx] = f[self.forw_idx] / h
+            return du.reshape(shape)
+        pos = u.dim() - 1
+        x = torch.transpose(u, pos, axis)
+        b, f = self._combine(x)
+        du = (b + f) / (2 * h)
+        if pos == 1:
+            du[:, self.back_idx] = b[:, self.back_idx] / h
+            du[:, self.forw_idx] = f[:, self.forw_idx] / h
+        elif pos == 2:
+            du[:, :, self.back_idx] = b[:, :, self.back_idx] / h
+            du[:, :, self.forw_idx] = f[:, :, self.forw_idx] / h
+        return torch.transpose(du, pos, axis)
+
+    def term(self, model: torch.Tensor, term: dict, grid: torch.Tensor) -> torch.Tensor:
+        key = list(term.keys())[1]
+        der = torch.zeros_like(model) + 1
+        for j, axes in enumerate(term[key]):
+            prod = model[term['var'][j]]
+            if axes != [None]:
+                for ax in axes:
+                    if ax is None:
+                        continue
+                    prod = self.d(prod, self.step(grid, ax), ax)
+            p = term['pow'][j]
+            der = der * prod ** p if isinstance(p, (int, float)) else p(der * prod)
+        c = term['coeff']
+        return c(grid) * der if callable(c) else c * der
+
+
+class MatOracle:
+    """Reference-semantics evaluation of a 'mat' problem on the CPU."""
+
+    def __init__(self, axes: Dict[str, torch.Tensor], operators: List[dict], conds: List[dict],
+                 derivative_points: int = 2):
+        self.grid = grid_mesh(list(axes.values()))
+        self.D = MatDerivative(derivative_points)
+        self.ops = [unify(op) for op in operators]
+        self.conds = build_conditions(conds, axes)
+        for c in self.conds:
+            sets = c['bnd'] if c['type'] == 'periodic' else [c['bnd']]
+            c['pos'] = [self._positions(s) for s in sets]
+
+    def _positions(self, pts):
+        """(tedeous/input_preprocessing.py:553-575)"""
+        out = []
+        for p in pts:
+            if self.grid.shape[0] == 1:
+                out.append((torch.tensor(_positions(self.grid.reshape(-1, 1), p.reshape(1, -1))),))
+                continue
+            prod = torch.ones_like(self.grid[0], dtype=torch.bool)
+            for ax in range(self.grid.shape[0]):
+                prod &= torch.isclose(p[ax].float(), self.grid[ax].float())
+            out.append(torch.where(prod))
+        return out
+
+    def field(self, model, op):
+        return apply_operator(lambda t: self.D.term(model, t, self.grid), op)
+
+    def residual(self, model):
+        cols = [self.field(model, op).reshape(-1, 1) for op in self.ops]
+        return cols[0] if len(cols) == 1 else torch.cat(cols, 1)
+
+    def boundary(self, model):
+        bval, true = {}, {}
+        for c in self.conds:
+            if c['bop'] is None:
+                vals = [torch.cat([model[c['var']][p] for p in pos]).reshape(-1, 1) for pos in c['pos']]
+            else:
+                var = c['bop'][list(c['bop'].keys())[0]]['var'][0]
+                f = self.field(model, c['bop'])
+                vals = [torch.cat([f[var][p] for p in pos]).reshape(-1, 1) for pos in c['pos']]
+            v = vals[0]
+            for w in vals[1:]:
+                v = v - w
+            k = c['type']
+            if k in bval:
+                bval[k] = torch.cat((bval[k], v.reshape(-1)))
+                true[k] = torch.cat((true[k], c['bval'].reshape(-1)))
+            else:
+                bval[k] = v.reshape(-1)
+                true[k] = c['bval'].reshape(-1)
+        return pad_matrix(bval, true)
+
+    def evaluate(self, model, lam_op=1, lam_bnd=1):
+        op = self.residual(model)
+        b, t, keys, lens = self.boundary(model)
+        loss, norm = default_loss(op, b, t, lam_row(op, lam_op).to(op.dtype),
+                                  lam_row(b, lam_bnd).to(op.dtype))
+        return loss, norm, op, b, t, keys, lens
+
+
+# ----------------------------------------------------------------------------
+# training loop
+# ----------------------------------------------------------------------------
+
+def train_adam(params: List[torch.Tensor], loss_fn: Callable[[], torch.Tensor], lr: float,
+               epochs: int, betas=(0.9, 0.999), eps=1e-8) -> List[float]:
+    """``Model.train(Optimizer('Adam', {'lr': lr}), epochs)``: one initial
+    evaluation, then ``epochs - 1`` closure steps (tedeous/model.py:154-188,
+    tedeous/optimizers/closure.py:49-64).  Returns the pre-step losses."""
+    opt = torch.optim.Adam(params, lr=lr, betas=betas, eps=eps)
+    history = []
+
+    def closure():
+        opt.zero_grad()
+        loss = loss_fn()
+        loss.backward()
+        history.append(float(loss))
+        return loss
+
+    for _ in range(1, epochs):
+        opt.zero_grad()
+        opt.step(closure)
+    return history

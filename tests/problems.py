@@ -1,0 +1,199 @@
+"""Small problem definitions shared by the golden-vector generator, the oracle
+tests and the GPU parity tests.
+
+Each builder takes a namespace ``api`` providing ``Domain``, ``Conditions``
+and ``Equation`` (the reference's classes or this package's, which have the
+same interface) and returns ``(domain, equation, conditions, compile_kwargs)``.
+Network / grid-tensor initialisation is seeded and independent of ``api``.
+"""
+
+import math
+
+import numpy as np
+import torch
+
+MU = 0.01 / math.pi
+
+
+def legendre_eq(grid_coeffs=None, mode='NN'):
+    """(1 - x^2) u'' - 2 x u' + 12 u = 0 (n = 3)."""
+    if mode == 'NN':
+        c1, c2 = grid_coeffs
+    else:
+        c1 = lambda g: 1 - g ** 2  # noqa: E731
+        c2 = lambda g: -2 * g      # noqa: E731
+    return {
+        '(1-t^2)*d2u/dt2': {'coeff': c1, 'd2u/dt2': [0, 0], 'pow': 1, 'var': 0},
+        '-2t*du/dt': {'coeff': c2, 'du/dt': [0], 'pow': 1, 'var': 0},
+        'n*(n+1)*u': {'coeff': 12, 'u': [None], 'pow': 1, 'var': 0},
+    }
+
+
+def burgers_eq(mu=MU):
+    return {
+        'du/dt': {'coeff': 1., 'du/dt': [1], 'pow': 1, 'var': 0},
+        '+u*du/dx': {'coeff': 1, 'u*du/dx': [[None], [0]], 'pow': [1, 1], 'var': [0, 0]},
+        '-mu*d2u/dx2': {'coeff': -mu, 'd2u/dx2': [0, 0], 'pow': 1, 'var': 0},
+    }
+
+
+def kdv_eq():
+    return {
+        'du/dt': {'coeff': 1, 'du/dt': [1], 'pow': 1, 'var': 0},
+        '6u*du/dx': {'coeff': 6, 'u*du/dx': [[None], [0]], 'pow': [1, 1], 'var': [0, 0]},
+        'd3u/dx3': {'coeff': 1, 'd3u/dx3': [0, 0, 0], 'pow': 1, 'var': 0},
+    }
+
+
+def wave_eq():
+    return {
+        'd2u/dt2': {'coeff': 1, 'd2u/dt2': [1, 1], 'pow': 1},
+        '-4*d2u/dx2': {'coeff': -4, 'd2u/dx2': [0, 0], 'pow': 1},
+    }
+
+
+def legendre_nn(api, n=20, dtype='float32'):
+    d = api.Domain()
+    d.variable('x', [-1, 1], n, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'x': -1}, value=-1)
+    b.dirichlet({'x': 1}, value=1)
+    grid = d.build('NN')
+    eq = api.Equation()
+    eq.add(legendre_eq((1 - grid ** 2, -2 * grid), 'NN'))
+    return d, eq, b, dict(mode='NN', lambda_operator=1, lambda_bound=10)
+
+
+def legendre_nn_opbc(api, n=20, dtype='float32'):
+    """t in [0, 1], u(0) = P3(0) = 0, u'(1) = P3'(1) = 6 (an operator condition)."""
+    d = api.Domain()
+    d.variable('t', [0, 1], n, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'t': 0}, value=0.0)
+    bop = {'du/dt': {'coeff': 1, 'du/dt': [0], 'pow': 1, 'var': 0}}
+    b.operator({'t': 1}, operator=bop, value=6.0)
+    grid = d.build('NN')
+    eq = api.Equation()
+    eq.add(legendre_eq((1 - grid ** 2, -2 * grid), 'NN'))
+    return d, eq, b, dict(mode='NN', lambda_operator=1, lambda_bound=10)
+
+
+def burgers_nn(api, nx=15, nt=9, dtype='float32', mu=MU):
+    d = api.Domain()
+    d.variable('x', [-1, 1], nx, dtype=dtype)
+    d.variable('t', [0, 1], nt, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'x': [-1, 1], 't': 0}, value=lambda g: -torch.sin(np.pi * g[:, 0]))
+    b.dirichlet({'x': -1, 't': [0, 1]}, value=0)
+    b.dirichlet({'x': 1, 't': [0, 1]}, value=0)
+    eq = api.Equation()
+    eq.add(burgers_eq(mu))
+    return d, eq, b, dict(mode='NN', lambda_operator=1, lambda_bound=10)
+
+
+def kdv_nn(api, nx=24, nt=7, dtype='float32', L=20.0):
+    d = api.Domain()
+    tdt = torch.float32 if dtype == 'float32' else torch.float64
+    x = L * torch.arange(nx, dtype=tdt) / nx
+    d.variable('x', x, None, dtype=dtype)
+    d.variable('t', [0, 1], nt, dtype=dtype)
+    b = api.Conditions()
+    b.periodic([{'x': 0, 't': [0, 1]}, {'x': L, 't': [0, 1]}])
+    u0 = 0.5 / torch.cosh(0.5 * (x - L / 2)) ** 2
+    b.dirichlet({'x': [0, L], 't': 0}, value=u0)
+    eq = api.Equation()
+    eq.add(kdv_eq())
+    return d, eq, b, dict(mode='NN', lambda_operator=1, lambda_bound=100, h=0.01)
+
+
+def wave_mat(api, n=11, dtype='float32', dp=2):
+    d = api.Domain()
+    d.variable('x', [0, 1], n, dtype=dtype)
+    d.variable('t', [0, 1], n, dtype=dtype)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=torch.sin(np.pi * x))
+    b.dirichlet({'x': 0, 't': [0, 1]}, value=0)
+    b.dirichlet({'x': 1, 't': [0, 1]}, value=0)
+    bop = {'du/dt': {'coeff': 1, 'du/dt': [1], 'pow': 1}}
+    b.operator({'x': [0, 1], 't': 0}, operator=bop, value=0)
+    eq = api.Equation()
+    eq.add(wave_eq())
+    return d, eq, b, dict(mode='mat', lambda_operator=1, lambda_bound=100, derivative_points=dp)
+
+
+def legendre_mat(api, n=20, dtype='float32'):
+    d = api.Domain()
+    d.variable('x', [-1, 1], n, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'x': -1}, value=-1)
+    bop = {'du/dx': {'coeff': 1, 'du/dx': [0], 'pow': 1, 'var': 0}}
+    b.operator({'x': 1}, operator=bop, value=6.0)
+    eq = api.Equation()
+    eq.add(legendre_eq(mode='mat'))
+    return d, eq, b, dict(mode='mat', lambda_operator=1, lambda_bound=10)
+
+
+def nonlinear_mat(api, n=14, dtype='float32'):
+    """A 2-D 'mat' problem with a product term and a squared factor:
+    u_t + u*u_x - 0.1*(u_xx)^2 = 0, Dirichlet on t = 0 and a periodic pair."""
+    d = api.Domain()
+    d.variable('x', [0, 1], n, dtype=dtype)
+    d.variable('t', [0, 0.5], n - 3, dtype=dtype)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=torch.cos(np.pi * x))
+    b.periodic([{'x': 0, 't': [0, 0.5]}, {'x': 1, 't': [0, 0.5]}])
+    eq = api.Equation()
+    eq.add({
+        'u_t': {'coeff': 1, 'u_t': [1], 'pow': 1, 'var': 0},
+        'u*u_x': {'coeff': 1, 'uux': [[None], [0]], 'pow': [1, 1], 'var': [0, 0]},
+        'uxx^2': {'coeff': -0.1, 'uxx': [0, 0], 'pow': 2, 'var': 0},
+    })
+    return d, eq, b, dict(mode='mat', lambda_operator=1, lambda_bound=10)
+
+
+def make_mlp(layers, seed=0, dtype=torch.float32):
+    """Linear/Tanh stack with Xavier-normal weights and zero biases."""
+    torch.manual_seed(seed)
+    mods = []
+    for i in range(len(layers) - 1):
+        lin = torch.nn.Linear(layers[i], layers[i + 1])
+        torch.nn.init.xavier_normal_(lin.weight)
+        torch.nn.init.zeros_(lin.bias)
+        mods.append(lin)
+        if i < len(layers) - 2:
+            mods.append(torch.nn.Tanh())
+    return torch.nn.Sequential(*mods).to(dtype)
+
+
+def randomize_biases(net, seed=1, scale=0.1):
+    """Give the biases non-zero values so that their gradients are exercised."""
+    g = torch.Generator().manual_seed(seed)
+    with torch.no_grad():
+        for m in net:
+            if isinstance(m, torch.nn.Linear):
+                m.bias.copy_(scale * torch.randn(m.bias.shape, generator=g, dtype=torch.float64)
+                             .to(m.bias.dtype))
+    return net
+
+
+NN_CASES = {
+    # name: (builder, mlp layers)
+    'legendre_nn': (legendre_nn, [1, 16, 16, 1]),
+    'legendre_nn_opbc': (legendre_nn_opbc, [1, 16, 16, 1]),
+    'burgers_nn': (burgers_nn, [2, 12, 12, 12, 1]),
+    'kdv_nn': (kdv_nn, [2, 10, 10, 10, 1]),
+}
+
+MAT_CASES = {
+    'wave_mat': wave_mat,
+    'wave_mat_dp3': lambda api, dtype='float32': wave_mat(api, dtype=dtype, dp=3),
+    'legendre_mat': legendre_mat,
+    'nonlinear_mat': nonlinear_mat,
+}
+
+
+def mat_init(shape, seed=0, dtype=torch.float32, std=0.01):
+    g = torch.Generator().manual_seed(seed)
+    return (std * torch.randn(shape, generator=g, dtype=torch.float64)).to(dtype)

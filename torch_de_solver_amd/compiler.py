@@ -1,0 +1,483 @@
+"""Lowering of a problem (grid, operator dicts, conditions) into flat tables.
+
+This is the compile step of the reference (``Model.compile`` ->
+``Solution.__init__`` -> ``Equation_NN/Equation_mat.operator_prepare`` and
+``bnd_prepare``, tedeous/model.py:96-113, tedeous/solution.py:23-88,
+tedeous/input_preprocessing.py:319-408, 534-594), restated as a table
+builder.  Nothing here runs per training step.
+
+'NN' mode lowers to *residual groups*.  A group has ``n_entries`` entries
+(collocation points, or boundary points) and ``n_slots`` evaluation slots;
+slot ``s`` of entry ``e`` is the network input
+``base_s[e] + offset_s`` (the reference's shifted grid, computed in the
+working precision as ``x + fl(shift * h)`` per axis,
+tedeous/points_type.py:35-37, tedeous/input_preprocessing.py:255-264).
+Each group carries one or more equations; an equation is the in-order sum
+of terms ``coeff * prod_j (sum_k w_jk u_{slot_jk}[var_j]) ** pow_j``
+(tedeous/derivative.py:30-58, tedeous/eval.py:158-165).  A periodic
+condition ``u(set0) - u(set1) - ...`` (tedeous/eval.py:342-345) is one
+factor whose stencil has weights ``+1, -1, ...`` over per-set slots.
+
+The interior group's entries are the ``'central'`` points only
+(tedeous/input_preprocessing.py:356).  Identical shift vectors are
+evaluated once: the reference evaluates duplicates separately but gets the
+same values, so the residual is unchanged.
+
+'mat' mode lowers to a term table over the whole grid tensor plus boundary
+entries given by flat grid positions (tedeous/input_preprocessing.py:553-594).
+"""
+
+from dataclasses import dataclass, field
+from typing import Any, Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from .equations import Factor, Term, parse_operator
+from .finite_diffs import mat_alphas, nn_scheme
+from .points_type import point_types
+
+BC_TYPES = ('dirichlet', 'operator', 'periodic', 'robin', 'data')
+
+
+# ----------------------------------------------------------------------------
+# shared helpers
+# ----------------------------------------------------------------------------
+
+def _lambda_list(lam, n: int, what: str) -> List[float]:
+    if isinstance(lam, torch.Tensor):
+        lam = lam.detach().cpu().reshape(-1).tolist()
+    if isinstance(lam, (int, float)):
+        return [float(lam)] * n
+    lam = [float(x) for x in lam]
+    if len(lam) != n:
+        raise ValueError(f"{what}: expected {n} weights, got {len(lam)}")
+    return lam
+
+
+def _pow_code(p) -> Tuple[int, float]:
+    """(kind, value): kind 0 = integer power, 1 = real power."""
+    if callable(p):
+        raise NotImplementedError(
+            "a callable 'pow' is not supported by the native engine")
+    if float(p) == int(p):
+        return 0, float(int(p))
+    return 1, float(p)
+
+
+# ----------------------------------------------------------------------------
+# 'NN' mode
+# ----------------------------------------------------------------------------
+
+@dataclass
+class NNFactor:
+    var: int
+    pow_kind: int
+    pow_val: float
+    stencil: List[Tuple[int, float]]          # (slot, weight as double)
+
+
+@dataclass
+class NNTerm:
+    coeff_scalar: Optional[float]
+    coeff_array: Optional[np.ndarray]          # per entry, or None
+    factors: List[NNFactor]
+
+
+@dataclass
+class NNEquation:
+    terms: List[NNTerm]                        # summed in order
+    lam: float                                 # lambda of its loss column
+    denom: float                               # N (operator) or max_len (bc)
+    target: Optional[np.ndarray]               # per entry, None -> zeros
+    column: int = -1                           # bc column (-1: operator eq)
+    col_pos: Optional[np.ndarray] = None       # entry positions in column
+
+
+@dataclass
+class NNGroup:
+    kind: str                                  # 'operator' or a bc type
+    n_entries: int
+    slot_base: List[int]                       # index of base array per slot
+    slot_offset: List[List[float]]             # per slot, per axis (double)
+    bases: List[np.ndarray]                    # base coordinate arrays [n, d]
+    equations: List[NNEquation]
+
+
+@dataclass
+class NNProblem:
+    d_in: int
+    n_out: int
+    groups: List[NNGroup]
+    n_central: int
+    central_index: np.ndarray                  # grid indices of entries
+    bc_keys: List[str]                         # column types, first-seen order
+    bc_lengths: List[int]
+    bc_max_len: int
+    dtype: torch.dtype
+    h: float
+
+
+def _coeff_for_entries(coeff, grid: torch.Tensor, entry_idx: np.ndarray,
+                       entry_pts: torch.Tensor):
+    """Coefficient at the entries (tedeous/input_preprocessing.py:266-295).
+
+    Scalars stay scalars.  A tensor is indexed by grid position (the
+    reference looks positions up with ``bndpos``).  A callable is evaluated
+    at the entry points once, here; the reference re-evaluates it every
+    step on the same points, which gives the same values for a function of
+    the coordinates.
+    """
+    if isinstance(coeff, torch.nn.Parameter):
+        raise NotImplementedError("trainable coefficients are not supported")
+    if isinstance(coeff, (int, float)):
+        return float(coeff), None
+    if isinstance(coeff, torch.Tensor):
+        flat = coeff.detach().cpu().reshape(-1)
+        if flat.numel() == 1:
+            return float(flat[0]), None
+        if flat.numel() != grid.shape[0]:
+            raise ValueError("a tensor coefficient must have one value per "
+                             "grid point")
+        return None, flat.numpy()[entry_idx]
+    if callable(coeff):
+        vals = coeff(entry_pts)
+        if isinstance(vals, torch.Tensor):
+            vals = vals.detach().cpu().numpy()
+        vals = np.asarray(vals).reshape(-1)
+        if vals.size == 1:
+            vals = np.full(len(entry_idx), float(vals[0]))
+        return None, vals
+    raise NameError('"coeff" should be: torch.Tensor or callable or int or float!')
+
+
+def _nn_terms(terms: List[Term], nvars: int, point_type: str, order: str,
+              h: float, grid: torch.Tensor, entry_idx: np.ndarray,
+              entry_pts: torch.Tensor, slot_index: Dict, shifts: List):
+    """Lower one operator on one point set; extends the shared slot table."""
+    out_terms = []
+    for t in terms:
+        cs, ca = _coeff_for_entries(t.coeff, grid, entry_idx, entry_pts)
+        factors = []
+        for f in t.factors:
+            sch = nn_scheme(f.axes, nvars, point_type, order, h)
+            st = []
+            for shift, w in sch:
+                if shift not in slot_index:
+                    slot_index[shift] = len(shifts)
+                    shifts.append(shift)
+                st.append((slot_index[shift], w))
+            kind, pv = _pow_code(f.pow)
+            factors.append(NNFactor(var=f.var, pow_kind=kind, pow_val=pv,
+                                    stencil=st))
+        out_terms.append(NNTerm(coeff_scalar=cs, coeff_array=ca,
+                                factors=factors))
+    return out_terms
+
+
+def _match_grid_rows(grid_np: np.ndarray, pts: np.ndarray) -> np.ndarray:
+    """Index of the grid row equal to each point, -1 if none."""
+    lut = {tuple(r): i for i, r in enumerate(grid_np.tolist())}
+    return np.array([lut.get(tuple(p), -1) for p in pts.tolist()], dtype=np.int64)
+
+
+def compile_nn(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[dict]],
+               n_out: int, h: float = 0.001, inner_order: str = '1',
+               boundary_order: str = '2', lambda_operator=1,
+               lambda_bound=1) -> NNProblem:
+    """Lower an 'NN' problem (tedeous/input_preprocessing.py:173-408,
+    tedeous/eval.py:90-461, tedeous/losses.py:84-135)."""
+    if inner_order == '2':
+        nn_scheme([0], 1, 'central', '2', h)  # raises with the reason
+    grid_cpu = grid.detach().cpu()
+    dtype = grid_cpu.dtype
+    grid_np = grid_cpu.numpy()
+    n, d = grid_np.shape
+    types = point_types(grid_np)
+    central_idx = np.nonzero(types == 'central')[0].astype(np.int64)
+    central_pts = grid_cpu[torch.from_numpy(central_idx)]
+
+    if isinstance(operators, dict):
+        operators = [operators]
+    n_eq = len(operators)
+    lam_op = _lambda_list(lambda_operator, n_eq, 'lambda_operator')
+
+    slot_index: Dict = {}
+    shifts: List = []
+    eqs = []
+    for i, op in enumerate(operators):
+        terms = _nn_terms(parse_operator(op), d, 'central', inner_order, h, grid_cpu,
+                          central_idx, central_pts, slot_index, shifts)
+        eqs.append(NNEquation(terms=terms, lam=lam_op[i],
+                              denom=float(len(central_idx)), target=None))
+    interior = NNGroup(kind='operator', n_entries=len(central_idx),
+                       slot_base=[0] * len(shifts),
+                       slot_offset=[[s * h for s in sh] for sh in shifts],
+                       bases=[central_pts.numpy()], equations=eqs)
+    groups = [interior]
+
+    # ---- boundary conditions: columns by type, first-seen order
+    # (tedeous/eval.py:448-459, 55-87)
+    keys: List[str] = []
+    col_vals: Dict[str, List[np.ndarray]] = {}
+    col_len: Dict[str, int] = {}
+    pending = []   # (group, eq, column key, start position, count)
+    for bc in (bconds or []):
+        typ = bc['type']
+        if typ not in keys:
+            keys.append(typ)
+            col_vals[typ] = []
+            col_len[typ] = 0
+        true = bc['bval'].detach().cpu().reshape(-1).numpy()
+        col_vals[typ].append(true)
+        if typ == 'dirichlet' or (typ == 'periodic' and bc['bop'] is None):
+            sets = [bc['bnd']] if typ == 'dirichlet' else list(bc['bnd'])
+            pts = [s.detach().cpu().to(dtype).numpy() for s in sets]
+            cnt = pts[0].shape[0]
+            if any(p.shape[0] != cnt for p in pts):
+                raise ValueError("periodic point sets must have equal sizes")
+            # value u(set0) - u(set1) - ... as one +-1 stencil: the same
+            # float operations as tedeous/eval.py:342-345
+            stencil = [(k, 1.0 if k == 0 else -1.0) for k in range(len(pts))]
+            fac = NNFactor(var=int(bc['var']), pow_kind=0, pow_val=1.0,
+                           stencil=stencil)
+            eq = NNEquation(terms=[NNTerm(1.0, None, [fac])], lam=0.0, denom=1.0, target=None)
+            g = NNGroup(kind=typ, n_entries=cnt,
+                        slot_base=list(range(len(pts))),
+                        slot_offset=[[0.0] * d for _ in pts], bases=pts,
+                        equations=[eq])
+            groups.append(g)
+            pending.append((eq, typ, col_len[typ], np.arange(cnt)))
+            col_len[typ] += cnt
+        elif typ == 'operator':
+            bnd = bc['bnd'].detach().cpu().to(dtype)
+            rows = _match_grid_rows(grid_np, bnd.numpy())
+            if np.any(rows < 0):
+                raise ValueError("operator-condition points must be grid points "
+                                 "in 'NN' mode")
+            btypes = types[rows]
+            terms = parse_operator(bc['bop'])
+            start = col_len[typ]
+            # one group per point type; each entry keeps its own target
+            for pt in dict.fromkeys(btypes.tolist()):
+                sel = np.nonzero(btypes == pt)[0]
+                order = inner_order if pt == 'central' else boundary_order
+                s_index: Dict = {}
+                s_list: List = []
+                pts_t = bnd[torch.from_numpy(sel)]
+                tl = _nn_terms(terms, d, pt, order, h, grid_cpu, rows[sel],
+                               pts_t, s_index, s_list)
+                eq = NNEquation(terms=tl, lam=0.0, denom=1.0, target=None)
+                g = NNGroup(kind=typ, n_entries=len(sel),
+                            slot_base=[0] * len(s_list),
+                            slot_offset=[[s * h for s in sh] for sh in s_list],
+                            bases=[pts_t.numpy()], equations=[eq])
+                groups.append(g)
+                pending.append((eq, typ, start, sel))
+            col_len[typ] += bnd.shape[0]
+        else:
+            raise NotImplementedError(
+                f"boundary type {typ!r}{' with an operator' if bc['bop'] is not None else ''}"
+                " is not supported by the native 'NN' engine")
+
+    max_len = max(col_len.values()) if keys else 0
+    lam_b = _lambda_list(lambda_bound, len(keys), 'lambda_bound') if keys else []
+    # padded target columns (PadTransform, tedeous/utils.py:257-290)
+    padded = {}
+    for k in keys:
+        t = np.concatenate(col_vals[k]) if col_vals[k] else np.zeros(0)
+        col = np.zeros(max_len, dtype=np.float64)
+        m = min(len(t), max_len)
+        col[:m] = t[:m]
+        padded[k] = col
+    for eq, typ, start, sel in pending:
+        ci = keys.index(typ)
+        eq.column = ci
+        eq.lam = lam_b[ci]
+        eq.denom = float(max_len)
+        pos = start + np.asarray(sel, dtype=np.int64)
+        eq.col_pos = pos
+        eq.target = padded[typ][pos]
+
+    return NNProblem(d_in=d, n_out=n_out, groups=groups,
+                     n_central=len(central_idx), central_index=central_idx,
+                     bc_keys=keys, bc_lengths=[col_len[k] for k in keys],
+                     bc_max_len=max_len, dtype=dtype, h=h)
+
+
+# ----------------------------------------------------------------------------
+# 'mat' mode
+# ----------------------------------------------------------------------------
+
+@dataclass
+class MatFactor:
+    var: int
+    axes: List[int]                            # [] for the undifferentiated field
+    pow_kind: int
+    pow_val: float
+
+
+@dataclass
+class MatTerm:
+    coeff_scalar: Optional[float]
+    coeff_field: Optional[np.ndarray]          # full-grid field or None
+    factors: List[MatFactor]
+
+
+@dataclass
+class MatBC:
+    typ: str
+    var: int
+    positions: List[np.ndarray]                # per set: flat grid indices
+    terms: Optional[List[MatTerm]]             # None: plain value u[var]
+    column: int = -1
+    col_start: int = 0
+
+
+@dataclass
+class MatProblem:
+    shape: Tuple[int, ...]                     # grid shape (n1, ..., nd)
+    n_eq: int
+    h: List[float]                             # per-axis step (working precision)
+    dp: int
+    alpha_back: np.ndarray
+    alpha_forw: np.ndarray
+    equations: List[List[MatTerm]]
+    lam_op: List[float]
+    bcs: List[MatBC]
+    bc_keys: List[str]
+    bc_lengths: List[int]
+    bc_max_len: int
+    lam_bnd: List[float]
+    targets: np.ndarray                        # [max_len, n_cols] padded
+    dtype: torch.dtype
+
+
+def _mat_terms(operator: dict, grid: torch.Tensor, shape, n_eq: int = 1) -> List[MatTerm]:
+    out = []
+    for t in parse_operator(operator):
+        c = t.coeff
+        if isinstance(c, torch.nn.Parameter):
+            raise NotImplementedError("trainable coefficients are not supported")
+        if isinstance(c, (int, float)):
+            cs, cf = float(c), None
+        else:
+            # tedeous/derivative.py:318-321: ``coeff(grid) * der_term`` or
+            # ``coeff * der_term`` broadcast against the [n_eq, *grid] field
+            val = c(grid) if callable(c) else c
+            if not isinstance(val, torch.Tensor):
+                val = torch.as_tensor(val)
+            val = val.detach().cpu()
+            if val.numel() == 1:
+                cs, cf = float(val.reshape(-1)[0]), None
+            else:
+                full = torch.broadcast_to(val, (n_eq,) + tuple(shape))
+                if n_eq > 1 and not bool(torch.all(full == full[0:1])):
+                    raise NotImplementedError("a coefficient field that differs between "
+                                              "equation planes is not supported")
+                cs, cf = None, full[0].numpy().copy()
+        facs = []
+        for f in t.factors:
+            kind, pv = _pow_code(f.pow)
+            facs.append(MatFactor(var=f.var, axes=list(f.axes or []),
+                                  pow_kind=kind, pow_val=pv))
+        out.append(MatTerm(coeff_scalar=cs, coeff_field=cf, factors=facs))
+    return out
+
+
+def _mat_positions(grid: torch.Tensor, pts: torch.Tensor) -> np.ndarray:
+    """Flat positions of boundary points (tedeous/input_preprocessing.py:553-575).
+
+    In d>1 every position whose coordinates are all ``isclose`` to the point
+    is taken (zero or several are possible, as in the reference).  In 1-D a
+    single ``isclose`` match is taken, else the nearest point
+    (tedeous/input_preprocessing.py:84-150).
+    """
+    g = grid.detach().cpu()
+    d = g.shape[0]
+    flat = [g[a].reshape(-1).to(torch.float32) for a in range(d)]
+    out = []
+    for p in pts.detach().cpu():
+        if d == 1:
+            gd = g.reshape(-1).double()
+            m = torch.nonzero(torch.isclose(gd, p.double().reshape(-1)[0])).reshape(-1)
+            if m.numel() == 1:
+                out.append(int(m[0]))
+            else:
+                out.append(int(torch.argmin((gd - p.double().reshape(-1)[0]).abs())))
+            continue
+        mask = torch.ones_like(flat[0], dtype=torch.bool)
+        for a in range(d):
+            mask &= torch.isclose(p[a].float(), flat[a])
+        out.extend(torch.nonzero(mask).reshape(-1).tolist())
+    return np.asarray(out, dtype=np.int64)
+
+
+def compile_mat(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[dict]],
+                n_eq: int, derivative_points: int = 2, lambda_operator=1,
+                lambda_bound=1) -> MatProblem:
+    """Lower a 'mat' problem (tedeous/derivative.py:135-323,
+    tedeous/eval.py:283-461, tedeous/losses.py:84-135)."""
+    g = grid.detach().cpu()
+    d = g.shape[0]
+    if d > 3:
+        raise NotImplementedError("'mat' mode supports grids of up to 3 axes")
+    shape = tuple(g.shape[1:])
+    # step per axis: |v1 - v0| of the sorted unique axis values, in the grid
+    # dtype (tedeous/derivative.py:229-247)
+    hs = []
+    for a in range(d):
+        u = torch.unique(g[a].reshape(-1).float())
+        hs.append(float(torch.abs(u[1] - u[0])) if u.numel() > 1 else 1.0)
+    ab, af, _, _ = mat_alphas(derivative_points)
+    if isinstance(operators, dict):
+        operators = [operators]
+    eqs = [_mat_terms(op, g, shape, n_eq) for op in operators]
+    lam_op = _lambda_list(lambda_operator, len(eqs), 'lambda_operator')
+
+    bcs: List[MatBC] = []
+    keys: List[str] = []
+    col_len: Dict[str, int] = {}
+    col_true: Dict[str, List[np.ndarray]] = {}
+    for bc in (bconds or []):
+        typ = bc['type']
+        if typ not in keys:
+            keys.append(typ)
+            col_len[typ] = 0
+            col_true[typ] = []
+        col_true[typ].append(bc['bval'].detach().cpu().reshape(-1).numpy())
+        if typ == 'dirichlet':
+            pos = [_mat_positions(g, bc['bnd'])]
+            terms = None
+            var = int(bc['var'])
+        elif typ == 'operator':
+            pos = [_mat_positions(g, bc['bnd'])]
+            terms = _mat_terms(bc['bop'], g, shape, n_eq)
+            # value read from the bop's first term variable
+            # (tedeous/eval.py:320-326)
+            var = terms[0].factors[0].var
+        elif typ == 'periodic':
+            pos = [_mat_positions(g, b) for b in bc['bnd']]
+            terms = _mat_terms(bc['bop'], g, shape, n_eq) if bc['bop'] is not None else None
+            var = int(bc['var']) if terms is None else terms[0].factors[0].var
+            if any(len(p) != len(pos[0]) for p in pos):
+                raise ValueError("periodic point sets must have equal sizes")
+        else:
+            raise NotImplementedError(
+                f"boundary type {typ!r} is not supported by the native 'mat' engine")
+        m = MatBC(typ=typ, var=var, positions=pos, terms=terms,
+                  column=keys.index(typ), col_start=col_len[typ])
+        col_len[typ] += len(pos[0])
+        bcs.append(m)
+    max_len = max(col_len.values()) if keys else 0
+    targets = np.zeros((max_len, len(keys)), dtype=np.float64)
+    for ci, k in enumerate(keys):
+        t = np.concatenate(col_true[k])
+        m = min(len(t), max_len)
+        targets[:m, ci] = t[:m]
+    lam_b = _lambda_list(lambda_bound, len(keys), 'lambda_bound') if keys else []
+    return MatProblem(shape=shape, n_eq=n_eq, h=hs, dp=int(derivative_points),
+                      alpha_back=ab, alpha_forw=af, equations=eqs, lam_op=lam_op,
+                      bcs=bcs, bc_keys=keys, bc_lengths=[col_len[k] for k in keys],
+                      bc_max_len=max_len, lam_bnd=lam_b, targets=targets,
+                      dtype=g.dtype)

@@ -1,0 +1,1034 @@
+// 'NN' mode: residual/loss forward and adjoint of a tanh MLP evaluated at
+// finite-difference-shifted copies of the collocation points.
+//
+// What the reference does per step (tedeous/derivative.py:30-58,
+// tedeous/eval.py:143-193, 283-461, tedeous/losses.py:84-135): for every
+// factor of every term it evaluates the network on a materialised shifted
+// copy of the grid, combines the copies with the stencil weights, multiplies
+// factors and coefficient, sums terms, squares and averages; autograd then
+// differentiates the whole graph.
+//
+// Here:
+//   k_pack      packs hidden-layer weights into MFMA operand order.
+//   k_forward   one wavefront per 16 entries of a group: for each slot, the
+//               shifted inputs are formed in registers and pushed through
+//               the network (hidden layers on 16x16x4 MFMA, tanh on VALU);
+//               the residual program is then evaluated per entry, squared
+//               into the loss, and the seeds dL/du(slot) are stored.
+//   k_backward  tiles of 16*NW rows (row = one (slot, entry) pair): the
+//               forward is recomputed with activations kept in LDS, the
+//               seeds are propagated back through the network, and the
+//               weight gradients are accumulated with MFMA (K = rows) into
+//               a per-work-item partial; no atomics.
+//   k_reduce    sums the partials of each instance in a fixed order and
+//               either stores dparams or applies the Adam update.
+//   k_loss      sums per-workgroup loss partials in a fixed order.
+//
+// Register layout of activations (hidden width padded to HP = 16*NB): lane
+// l of a wavefront holds row l%16; quarter q = l/16 holds the features
+// f = 16*mb + crow(q, i) in register [mb][i].  A hidden layer is computed
+// transposed, Z^T = W * A^T, so that the accumulator of one layer is
+// directly the B operand of the next (k-step (mb, i) contracts over the
+// features held in register [mb][i]); the weights are packed to match.
+
+#include "tdx_common.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace tdx {
+
+constexpr int MAXD = 4;   // network inputs
+constexpr int MAXO = 4;   // network outputs
+constexpr int MAXF = 8;   // factors per term
+constexpr int K1_WAVES = 4;
+
+template <typename T>
+struct NNArgs {
+  // architecture
+  int d_in, n_out, H, NH, P;  // NH = hidden H x H layers
+  // tables
+  const int32_t *groups;
+  const int32_t *ints;
+  const T *reals;
+  const T *data;
+  // work lists
+  const int2 *k1_items;       // (group, entry start) per forward workgroup
+  const int4 *k2_items;       // (group, row start, row end, inst) per backward workgroup
+  const int32_t *inst_k1_ptr, *inst_k1_idx;
+  const int32_t *inst_k2_ptr, *inst_k2_idx;
+  int n_inst;
+  // buffers
+  const T *params;            // n_inst * P
+  T *packed;                  // n_inst * NH * 2 * HP*HP
+  T *seeds;                   // n_rows * n_out
+  T *lp;                      // 2 per forward workgroup
+  T *partial;                 // P per backward workgroup
+  T *resid;
+  const T *dloss;
+  int acc_lds;                // backward accumulates in LDS
+  int tile_rows;
+};
+
+template <typename T>
+struct Layout {
+  int d_in, n_out, H, NH;
+  __device__ __host__ int oW1() const { return 0; }
+  __device__ __host__ int ob1() const { return H * d_in; }
+  __device__ __host__ int oWh(int j) const { return H * d_in + H + j * (H * H + H); }
+  __device__ __host__ int obh(int j) const { return oWh(j) + H * H; }
+  __device__ __host__ int oWL() const { return H * d_in + H + NH * (H * H + H); }
+  __device__ __host__ int obL() const { return oWL() + n_out * H; }
+  __device__ __host__ int count() const { return obL() + n_out; }
+};
+
+// ---------------------------------------------------------------------------
+// weight packing
+// ---------------------------------------------------------------------------
+// Wf[j][ob][mb][i][lane] = W_j[16*ob + lane%16][16*mb + crow(lane/16, i)]
+// Wb[j][ib][mb][i][lane] = W_j[16*mb + crow(lane/16, i)][16*ib + lane%16]
+template <typename T, int NB>
+__global__ void k_pack(NNArgs<T> a) {
+  constexpr int HP = 16 * NB;
+  const int64_t per_layer = (int64_t)HP * HP;
+  const int64_t total = (int64_t)a.n_inst * a.NH * 2 * per_layer;
+  Layout<T> L{a.d_in, a.n_out, a.H, a.NH};
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = t;
+    const int lane = r % 64; r /= 64;
+    const int i = r % 4; r /= 4;
+    const int mb = r % NB; r /= NB;
+    const int xb = r % NB; r /= NB;
+    const int which = r % 2; r /= 2;
+    const int j = r % a.NH; r /= a.NH;
+    const int inst = (int)r;
+    const T *W = a.params + (int64_t)inst * a.P + L.oWh(j);
+    int out, in;
+    if (which == 0) {
+      out = 16 * xb + (lane & 15);
+      in = 16 * mb + crow<T>(lane >> 4, i);
+    } else {
+      out = 16 * mb + crow<T>(lane >> 4, i);
+      in = 16 * xb + (lane & 15);
+    }
+    T v = (out < a.H && in < a.H) ? W[out * a.H + in] : T(0);
+    a.packed[t] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// forward of 16 rows per wavefront
+// ---------------------------------------------------------------------------
+template <typename T, int NB>
+struct Net {
+  const T *W1, *b1, *WL, *bL, *params;
+  const T *Wf, *Wb;
+  Layout<T> L;
+};
+
+template <typename T, int NB>
+__device__ __forceinline__ Net<T, NB> make_net(const NNArgs<T> &a, int inst) {
+  constexpr int HP = 16 * NB;
+  Net<T, NB> n;
+  n.L = Layout<T>{a.d_in, a.n_out, a.H, a.NH};
+  n.params = a.params + (int64_t)inst * a.P;
+  n.W1 = n.params + n.L.oW1();
+  n.b1 = n.params + n.L.ob1();
+  n.WL = n.params + n.L.oWL();
+  n.bL = n.params + n.L.obL();
+  n.Wf = a.packed + (int64_t)inst * a.NH * 2 * HP * HP;
+  n.Wb = n.Wf + (int64_t)HP * HP;  // layer j: Wf at 2j, Wb at 2j+1
+  return n;
+}
+
+// Layer 1 (VALU): act[mb][i] = tanh(b1[f] + sum_a x[a] W1[f][a]).
+template <typename T, int NB>
+__device__ __forceinline__ void layer_in(const Net<T, NB> &n, const T *x, int q,
+                                         T (&act)[NB][4]) {
+  const int H = n.L.H, d = n.L.d_in;
+#pragma unroll
+  for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int f = 16 * mb + crow<T>(q, i);
+      T z = T(0);
+      if (f < H) {
+        z = n.b1[f];
+        for (int k = 0; k < d; ++k) z = fma(x[k], n.W1[f * d + k], z);
+        z = dtanh(z);
+      }
+      act[mb][i] = z;
+    }
+}
+
+// Hidden layer j (MFMA): act <- tanh(W_j act + b_j).
+template <typename T, int NB>
+__device__ __forceinline__ void layer_hidden(const Net<T, NB> &n, int j, int lane,
+                                             T (&act)[NB][4]) {
+  constexpr int HP = 16 * NB;
+  typedef typename V4<T>::type v4;
+  const int H = n.L.H, q = lane >> 4;
+  const T *wp = n.Wf + (int64_t)j * 2 * HP * HP + lane;
+  v4 acc[NB];
+#pragma unroll
+  for (int ob = 0; ob < NB; ++ob) acc[ob] = v4{0, 0, 0, 0};
+#pragma unroll
+  for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const T b = act[mb][i];
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob)
+        acc[ob] = mma16(wp[((ob * NB + mb) * 4 + i) * 64], b, acc[ob]);
+    }
+  const T *bias = n.params + n.L.obh(j);
+#pragma unroll
+  for (int ob = 0; ob < NB; ++ob)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int f = 16 * ob + crow<T>(q, i);
+      act[ob][i] = (f < H) ? dtanh(acc[ob][i] + bias[f]) : T(0);
+    }
+}
+
+// Output layer (VALU + cross-quarter shuffle): u[o] for the lane's row.
+template <typename T, int NB>
+__device__ __forceinline__ T layer_out(const Net<T, NB> &n, int o, int q,
+                                       const T (&act)[NB][4]) {
+  const int H = n.L.H;
+  T s = T(0);
+#pragma unroll
+  for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int f = 16 * mb + crow<T>(q, i);
+      if (f < H) s = fma(act[mb][i], n.WL[o * H + f], s);
+    }
+  s += __shfl_xor(s, 16);
+  s += __shfl_xor(s, 32);
+  return s + n.bL[o];
+}
+
+template <typename T>
+__device__ __forceinline__ T coeff_of(const NNArgs<T> &a, const int32_t *Tm, int e) {
+  return Tm[2] == 0 ? a.reals[Tm[3]] : a.data[Tm[3] + e];
+}
+
+template <typename T>
+__device__ __forceinline__ T fpow(const NNArgs<T> &a, const int32_t *F, T x) {
+  return F[1] == 0 ? ipow(x, F[2]) : dpow(x, a.reals[F[5]]);
+}
+
+// d(x**p)/dx
+template <typename T>
+__device__ __forceinline__ T dfpow(const NNArgs<T> &a, const int32_t *F, T x) {
+  if (F[1] == 0) {
+    const int p = F[2];
+    return p == 0 ? T(0) : T(p) * ipow(x, p - 1);
+  }
+  const T p = a.reals[F[5]];
+  return p * dpow(x, p - T(1));
+}
+
+// ---------------------------------------------------------------------------
+// k_forward
+// ---------------------------------------------------------------------------
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  const int2 item = a.k1_items[blockIdx.x];
+  const int32_t *G = a.groups + item.x * TDX_GROUP_INTS;
+  const int inst = G[0], n_entries = G[1], n_slots = G[2];
+  const int slot_base_i = G[3], slot_off_r = G[4], n_eq = G[5], eq_i = G[6];
+  const int64_t row_base = G[7];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = lane & 15, q = lane >> 4;
+  const int e = item.y + w * 16 + row;
+  const bool valid = e < n_entries;
+  const int ec = valid ? e : n_entries - 1;
+  const int SO = n_slots * a.n_out;
+  T *u_lds = smem + (w * 16 + row) * SO;
+  T *sd_lds = smem + K1_WAVES * 16 * SO + (w * 16 + row) * SO;
+  T *red = smem + 2 * K1_WAVES * 16 * SO;
+
+  const Net<T, NB> net = make_net<T, NB>(a, inst);
+  T act[NB][4];
+  for (int s = 0; s < n_slots; ++s) {
+    T x[MAXD];
+    const int64_t base = a.ints[slot_base_i + s];
+#pragma unroll
+    for (int k = 0; k < MAXD; ++k)
+      x[k] = (k < a.d_in) ? a.data[base + (int64_t)ec * a.d_in + k] +
+                                a.reals[slot_off_r + s * a.d_in + k]
+                          : T(0);
+    layer_in<T, NB>(net, x, q, act);
+    for (int j = 0; j < a.NH; ++j) layer_hidden<T, NB>(net, j, lane, act);
+    for (int o = 0; o < a.n_out; ++o) {
+      const T u = layer_out<T, NB>(net, o, q, act);
+      if (q == 0) u_lds[s * a.n_out + o] = u;
+    }
+  }
+  __syncthreads();
+
+  T lsum = T(0), lnsum = T(0);
+  if (q == 0) {
+    for (int k = 0; k < SO; ++k) sd_lds[k] = T(0);
+    if (valid) {
+      for (int eq = 0; eq < n_eq; ++eq) {
+        const int32_t *E = a.ints + eq_i + eq * TDX_EQ_INTS;
+        const int n_terms = E[0], term_i = E[1];
+        T r = T(0);
+        for (int t = 0; t < n_terms; ++t) {
+          const int32_t *Tm = a.ints + term_i + t * TDX_TERM_INTS;
+          T der = T(1);
+          for (int f = 0; f < Tm[0]; ++f) {
+            const int32_t *F = a.ints + Tm[1] + f * TDX_FACTOR_INTS;
+            T gs = T(0);
+            for (int k = 0; k < F[3]; ++k) {
+              const int32_t *S = a.ints + F[4] + k * TDX_STENCIL_INTS;
+              gs = gs + u_lds[S[0] * a.n_out + F[0]] * a.reals[S[1]];
+            }
+            der = der * fpow(a, F, gs);
+          }
+          const T tv = coeff_of(a, Tm, e) * der;
+          r = (t == 0) ? tv : r + tv;
+        }
+        const T target = E[2] >= 0 ? a.data[E[2] + e] : T(0);
+        if (E[3] >= 0 && a.resid) a.resid[E[3] + e] = r;
+        const T diff = r - target;
+        const T sq = diff * diff;
+        const T lam_d = a.reals[E[4]];
+        lsum += lam_d * sq;
+        lnsum += a.reals[E[5]] * sq;
+        const T dr = T(2) * lam_d * diff;
+        // seeds: d r / d u(slot, var), through every factor of every term
+        for (int t = 0; t < n_terms; ++t) {
+          const int32_t *Tm = a.ints + term_i + t * TDX_TERM_INTS;
+          const int nf = Tm[0];
+          T gsv[MAXF];
+#pragma unroll
+          for (int f = 0; f < MAXF; ++f) {
+            gsv[f] = T(0);
+            if (f < nf) {
+              const int32_t *F = a.ints + Tm[1] + f * TDX_FACTOR_INTS;
+              T gs = T(0);
+              for (int k = 0; k < F[3]; ++k) {
+                const int32_t *S = a.ints + F[4] + k * TDX_STENCIL_INTS;
+                gs = gs + u_lds[S[0] * a.n_out + F[0]] * a.reals[S[1]];
+              }
+              gsv[f] = gs;
+            }
+          }
+          const T cdr = coeff_of(a, Tm, e) * dr;
+#pragma unroll
+          for (int f = 0; f < MAXF; ++f) {
+            if (f < nf) {
+              const int32_t *F = a.ints + Tm[1] + f * TDX_FACTOR_INTS;
+              T g = cdr * dfpow(a, F, gsv[f]);
+#pragma unroll
+              for (int f2 = 0; f2 < MAXF; ++f2)
+                if (f2 < nf && f2 != f)
+                  g *= fpow(a, a.ints + Tm[1] + f2 * TDX_FACTOR_INTS, gsv[f2]);
+              for (int k = 0; k < F[3]; ++k) {
+                const int32_t *S = a.ints + F[4] + k * TDX_STENCIL_INTS;
+                sd_lds[S[0] * a.n_out + F[0]] += g * a.reals[S[1]];
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // store seeds: row (slot s, entry e) = row_base + s*n_entries + e
+  if (valid && q == 0) {
+    for (int s = 0; s < n_slots; ++s)
+      for (int o = 0; o < a.n_out; ++o)
+        a.seeds[(row_base + (int64_t)s * n_entries + e) * a.n_out + o] =
+            sd_lds[s * a.n_out + o];
+  }
+  // deterministic workgroup sum of the loss partials
+  red[threadIdx.x] = lsum;
+  red[256 + threadIdx.x] = lnsum;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      red[threadIdx.x] += red[threadIdx.x + st];
+      red[256 + threadIdx.x] += red[256 + threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a.lp[2 * blockIdx.x] = red[0];
+    a.lp[2 * blockIdx.x + 1] = red[256];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_backward
+// ---------------------------------------------------------------------------
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void k_backward(NNArgs<T> a) {
+  constexpr int HP = 16 * NB;
+  constexpr int LS = HP + 4;  // LDS row stride
+  typedef typename V4<T>::type v4;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  const int4 item = a.k2_items[blockIdx.x];
+  const int32_t *G = a.groups + item.x * TDX_GROUP_INTS;
+  const int n_entries = G[1];
+  const int slot_base_i = G[3], slot_off_r = G[4];
+  const int64_t row_base = G[7];
+  const int inst = item.w;
+  const int NW = blockDim.x >> 6;
+  const int TR = 16 * NW;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = lane & 15, q = lane >> 4;
+  const int trow = w * 16 + row;
+  const int NACT = a.NH + 1;
+  const int H = a.H, d_in = a.d_in, n_out = a.n_out;
+
+  T *act_l = smem;                         // [NACT][TR][LS]
+  T *dst = act_l + (int64_t)NACT * TR * LS;  // [TR][LS]
+  T *xs = dst + TR * LS;                     // [TR][MAXD]
+  T *ss = xs + TR * MAXD;                    // [TR][MAXO]
+  T *acc = a.acc_lds ? ss + TR * MAXO : a.partial + (int64_t)blockIdx.x * a.P;
+
+  for (int k = threadIdx.x; k < a.P; k += blockDim.x) acc[k] = T(0);
+  __syncthreads();
+
+  const Net<T, NB> net = make_net<T, NB>(a, inst);
+  const Layout<T> &L = net.L;
+  const T dl = *a.dloss;
+  const int64_t r_begin = item.y, r_end = item.z;
+
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += TR) {
+    const int64_t r = r0 + trow;
+    const bool valid = r < r_end;
+    const int64_t rc = valid ? r : r_begin;
+    const int64_t lr = rc - row_base;
+    const int s = (int)(lr / n_entries);
+    const int e = (int)(lr - (int64_t)s * n_entries);
+    T x[MAXD], so[MAXO];
+    const int64_t base = a.ints[slot_base_i + s];
+#pragma unroll
+    for (int k = 0; k < MAXD; ++k)
+      x[k] = (k < d_in) ? a.data[base + (int64_t)e * d_in + k] +
+                              a.reals[slot_off_r + s * d_in + k]
+                        : T(0);
+#pragma unroll
+    for (int o = 0; o < MAXO; ++o)
+      so[o] = (valid && o < n_out) ? a.seeds[rc * n_out + o] * dl : T(0);
+    if (q == 0) {
+#pragma unroll
+      for (int k = 0; k < MAXD; ++k) xs[trow * MAXD + k] = x[k];
+#pragma unroll
+      for (int o = 0; o < MAXO; ++o) ss[trow * MAXO + o] = so[o];
+    }
+
+    // forward recompute, activations to LDS
+    T act[NB][4];
+    layer_in<T, NB>(net, x, q, act);
+    for (int j = 0; j <= a.NH; ++j) {
+      if (j > 0) layer_hidden<T, NB>(net, j - 1, lane, act);
+      T *dstl = act_l + ((int64_t)j * TR + trow) * LS;
+#pragma unroll
+      for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dstl[16 * mb + crow<T>(q, i)] = act[mb][i];
+    }
+    __syncthreads();
+
+    // output layer: dW_L, db_L (VALU over rows); delta of the last hidden act
+    {
+      const T *aN = act_l + (int64_t)a.NH * TR * LS;
+      for (int t = threadIdx.x; t < n_out * H; t += blockDim.x) {
+        const int o = t / H, k = t - o * H;
+        T sum = T(0);
+        for (int rr = 0; rr < TR; ++rr) sum = fma(ss[rr * MAXO + o], aN[rr * LS + k], sum);
+        acc[L.oWL() + t] += sum;
+      }
+      for (int o = threadIdx.x; o < n_out; o += blockDim.x) {
+        T sum = T(0);
+        for (int rr = 0; rr < TR; ++rr) sum += ss[rr * MAXO + o];
+        acc[L.obL() + o] += sum;
+      }
+    }
+    T dl_r[NB][4];
+    {
+      const T *aN = act_l + ((int64_t)a.NH * TR + trow) * LS;
+#pragma unroll
+      for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int f = 16 * mb + crow<T>(q, i);
+          T g = T(0);
+          if (f < H) {
+            for (int o = 0; o < n_out; ++o) g = fma(net.WL[o * H + f], so[o], g);
+            const T av = aN[f];
+            g *= (T(1) - av * av);
+          }
+          dl_r[mb][i] = g;
+          dst[trow * LS + f] = g;
+        }
+    }
+    __syncthreads();
+
+    for (int j = a.NH; j >= 1; --j) {
+      const int jj = j - 1;  // hidden weight layer mapping act[j-1] -> act[j]
+      const T *ain = act_l + (int64_t)(j - 1) * TR * LS;
+      // dW_jj = delta_j^T act_{j-1}, blocks owned round-robin by wavefronts
+      for (int blk = w; blk < NB * NB; blk += NW) {
+        const int ob = blk / NB, ib = blk - ob * NB;
+        v4 c = v4{0, 0, 0, 0};
+        for (int kb = 0; kb < TR / 4; ++kb) {
+          const int rr = 4 * kb + q;
+          c = mma16(dst[rr * LS + 16 * ob + row], ain[rr * LS + 16 * ib + row], c);
+        }
+        const int in = 16 * ib + row;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int out = 16 * ob + crow<T>(q, i);
+          if (out < H && in < H) acc[L.oWh(jj) + out * H + in] += c[i];
+        }
+      }
+      for (int k = threadIdx.x; k < H; k += blockDim.x) {
+        T sum = T(0);
+        for (int rr = 0; rr < TR; ++rr) sum += dst[rr * LS + k];
+        acc[L.obh(jj) + k] += sum;
+      }
+      // delta_{j-1} = (W_jj^T delta_j) * (1 - act_{j-1}^2)
+      T dn[NB][4];
+      {
+        const T *wp = net.Wb + (int64_t)jj * 2 * HP * HP + lane;
+        v4 c[NB];
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) c[ib] = v4{0, 0, 0, 0};
+#pragma unroll
+        for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const T b = dl_r[mb][i];
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib)
+              c[ib] = mma16(wp[((ib * NB + mb) * 4 + i) * 64], b, c[ib]);
+          }
+        const T *arow = ain + trow * LS;
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int f = 16 * ib + crow<T>(q, i);
+            const T av = arow[f];
+            dn[ib][i] = (f < H) ? c[ib][i] * (T(1) - av * av) : T(0);
+          }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          dl_r[mb][i] = dn[mb][i];
+          dst[trow * LS + 16 * mb + crow<T>(q, i)] = dn[mb][i];
+        }
+      __syncthreads();
+    }
+    // first layer: dW_1 = delta_0^T x, db_1
+    for (int t = threadIdx.x; t < H * d_in; t += blockDim.x) {
+      const int k = t / d_in, c = t - k * d_in;
+      T sum = T(0);
+      for (int rr = 0; rr < TR; ++rr) sum = fma(dst[rr * LS + k], xs[rr * MAXD + c], sum);
+      acc[L.oW1() + t] += sum;
+    }
+    for (int k = threadIdx.x; k < H; k += blockDim.x) {
+      T sum = T(0);
+      for (int rr = 0; rr < TR; ++rr) sum += dst[rr * LS + k];
+      acc[L.ob1() + k] += sum;
+    }
+    __syncthreads();
+  }
+  if (a.acc_lds) {
+    T *out = a.partial + (int64_t)blockIdx.x * a.P;
+    for (int k = threadIdx.x; k < a.P; k += blockDim.x) out[k] = acc[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reductions and the optimizer
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_loss(NNArgs<T> a, T *loss) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_inst) return;
+  T s0 = T(0), s1 = T(0);
+  for (int k = a.inst_k1_ptr[i]; k < a.inst_k1_ptr[i + 1]; ++k) {
+    const int wg = a.inst_k1_idx[k];
+    s0 += a.lp[2 * wg];
+    s1 += a.lp[2 * wg + 1];
+  }
+  loss[2 * i] = s0;
+  loss[2 * i + 1] = s1;
+}
+
+struct AdamHyper {
+  double lr, b1, b2, eps;
+};
+
+// torch.optim.Adam (single-tensor form): m = b1 m + (1-b1) g;
+// v = b2 v + (1-b2) g^2; p -= lr/(1-b1^t) * m / (sqrt(v)/sqrt(1-b2^t) + eps)
+template <typename T>
+__device__ __forceinline__ void adam_update(T &p, T g, T &m, T &v, int64_t t,
+                                            const AdamHyper &h) {
+  const T b2 = T(h.b2);
+  const T w1 = T(1.0 - h.b1), w2 = T(1.0 - h.b2);
+  m = m + w1 * (g - m);
+  v = v * b2 + w2 * g * g;
+  const double bc1 = 1.0 - pow(h.b1, (double)t);
+  const double bc2 = 1.0 - pow(h.b2, (double)t);
+  const T step_size = T(h.lr / bc1);
+  const T bc2s = T(sqrt(bc2));
+  const T denom = sqrt(v) / bc2s + T(h.eps);
+  p = p - step_size * (m / denom);
+}
+
+// dparams (mode 0) or the Adam update (mode 1) from the per-item partials
+template <typename T>
+__global__ void k_reduce(NNArgs<T> a, int mode, T *dparams, T *params, T *m,
+                         T *v, const int64_t *step, AdamHyper h) {
+  const int64_t tot = (int64_t)a.n_inst * a.P;
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= tot) return;
+  const int inst = (int)(idx / a.P);
+  const int p = (int)(idx - (int64_t)inst * a.P);
+  T g = T(0);
+  for (int k = a.inst_k2_ptr[inst]; k < a.inst_k2_ptr[inst + 1]; ++k)
+    g += a.partial[(int64_t)a.inst_k2_idx[k] * a.P + p];
+  if (mode == 0) {
+    dparams[idx] = g;
+  } else {
+    T pv = params[idx], mv = m[idx], vv = v[idx];
+    adam_update(pv, g, mv, vv, *step + 1, h);
+    params[idx] = pv;
+    m[idx] = mv;
+    v[idx] = vv;
+  }
+}
+
+template <typename T>
+__global__ void k_adam(int64_t n, T *p, const T *g, T *m, T *v, const int64_t *step,
+                       AdamHyper h) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T pv = p[i], mv = m[i], vv = v[i];
+  adam_update(pv, g[i], mv, vv, *step + 1, h);
+  p[i] = pv;
+  m[i] = mv;
+  v[i] = vv;
+}
+
+__global__ void k_step_inc(int64_t *step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+template <typename T>
+__global__ void k_set_one(T *x) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *x = T(1);
+}
+
+}  // namespace tdx
+
+// ===========================================================================
+// host side
+// ===========================================================================
+using namespace tdx;
+
+struct tdx_nn_plan {
+  int dtype, d_in, n_out, H, n_layers, NH, NB, HP, P, n_inst;
+  int n_groups;
+  int64_t n_rows;
+  int n_k1, n_k2;
+  int k2_waves;
+  int acc_lds;
+  size_t k1_lds, k2_lds;
+  DevTable groups, ints, reals, data, k1_items, k2_items, ik1p, ik1i, ik2p, ik2i;
+  // workspace layout (byte offsets)
+  size_t off_packed, off_seeds, off_lp, off_partial, off_one, ws_bytes;
+};
+
+namespace {
+
+template <typename T>
+NNArgs<T> make_args(const tdx_nn_plan *p, const void *params, void *ws) {
+  NNArgs<T> a;
+  a.d_in = p->d_in;
+  a.n_out = p->n_out;
+  a.H = p->H;
+  a.NH = p->NH;
+  a.P = p->P;
+  a.groups = (const int32_t *)p->groups.ptr;
+  a.ints = (const int32_t *)p->ints.ptr;
+  a.reals = (const T *)p->reals.ptr;
+  a.data = (const T *)p->data.ptr;
+  a.k1_items = (const int2 *)p->k1_items.ptr;
+  a.k2_items = (const int4 *)p->k2_items.ptr;
+  a.inst_k1_ptr = (const int32_t *)p->ik1p.ptr;
+  a.inst_k1_idx = (const int32_t *)p->ik1i.ptr;
+  a.inst_k2_ptr = (const int32_t *)p->ik2p.ptr;
+  a.inst_k2_idx = (const int32_t *)p->ik2i.ptr;
+  a.n_inst = p->n_inst;
+  char *w = (char *)ws;
+  a.params = (const T *)params;
+  a.packed = (T *)(w + p->off_packed);
+  a.seeds = (T *)(w + p->off_seeds);
+  a.lp = (T *)(w + p->off_lp);
+  a.partial = (T *)(w + p->off_partial);
+  a.resid = nullptr;
+  a.dloss = (const T *)(w + p->off_one);
+  a.acc_lds = p->acc_lds;
+  a.tile_rows = 16 * p->k2_waves;
+  return a;
+}
+
+template <typename T, int NB>
+int launch_pack(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
+  if (p->NH == 0) return TDX_OK;
+  const int64_t total = (int64_t)p->n_inst * p->NH * 2 * p->HP * p->HP;
+  const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65535);
+  hipLaunchKernelGGL((k_pack<T, NB>), dim3(blocks), dim3(256), 0, s, a);
+  TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}
+
+template <typename T, int NB>
+int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStream_t s) {
+  int rc = launch_pack<T, NB>(p, a, s);
+  if (rc) return rc;
+  a.resid = resid;
+  hipLaunchKernelGGL((k_forward<T, NB>), dim3(p->n_k1), dim3(256), p->k1_lds, s, a);
+  TDX_TRY_HIP(hipGetLastError());
+  if (loss) {
+    hipLaunchKernelGGL((k_loss<T>), dim3((p->n_inst + 127) / 128), dim3(128), 0, s, a, loss);
+    TDX_TRY_HIP(hipGetLastError());
+  }
+  return TDX_OK;
+}
+
+template <typename T, int NB>
+int launch_backward(const tdx_nn_plan *p, NNArgs<T> a, hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void *)k_backward<T, NB>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_backward<T, NB>), dim3(p->n_k2), dim3(64 * p->k2_waves), p->k2_lds,
+                     s, a);
+  TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}
+
+template <typename T>
+int launch_reduce(const tdx_nn_plan *p, const NNArgs<T> &a, int mode, T *dparams, T *params,
+                  T *m, T *v, const int64_t *step, AdamHyper h, hipStream_t s) {
+  const int64_t tot = (int64_t)p->n_inst * p->P;
+  hipLaunchKernelGGL((k_reduce<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, a,
+                     mode, dparams, params, m, v, step, h);
+  TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}
+
+template <typename T>
+int nn_dispatch(const tdx_nn_plan *p, int op, const void *params, void *params_mut, void *loss,
+                void *resid, const void *dloss, void *dparams, void *m, void *v,
+                int64_t *step, AdamHyper h, void *ws, hipStream_t s) {
+  NNArgs<T> a = make_args<T>(p, params ? params : params_mut, ws);
+#define TDX_NB_CASE(NBV)                                                          \
+  case NBV: {                                                                     \
+    int rc = TDX_OK;                                                              \
+    if (op == 0) {                                                                \
+      rc = launch_forward<T, NBV>(p, a, (T *)loss, (T *)resid, s);                \
+    } else if (op == 1) {                                                         \
+      a.dloss = (const T *)dloss;                                                 \
+      rc = launch_pack<T, NBV>(p, a, s);                                          \
+      if (!rc) rc = launch_backward<T, NBV>(p, a, s);                             \
+      if (!rc) rc = launch_reduce<T>(p, a, 0, (T *)dparams, nullptr, nullptr,     \
+                                     nullptr, nullptr, h, s);                     \
+    } else {                                                                      \
+      rc = launch_forward<T, NBV>(p, a, (T *)loss, nullptr, s);                   \
+      if (!rc) {                                                                  \
+        hipLaunchKernelGGL((k_set_one<T>), dim3(1), dim3(64), 0, s,               \
+                           (T *)((char *)ws + p->off_one));                       \
+        rc = launch_backward<T, NBV>(p, a, s);                                    \
+      }                                                                           \
+      if (!rc) rc = launch_reduce<T>(p, a, 1, nullptr, (T *)params_mut, (T *)m,  \
+                                     (T *)v, step, h, s);                         \
+      if (!rc) {                                                                  \
+        hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, s, step);            \
+        TDX_TRY_HIP(hipGetLastError());                                           \
+      }                                                                           \
+    }                                                                             \
+    return rc;                                                                    \
+  }
+  switch (p->NB) {
+    TDX_NB_CASE(1)
+    TDX_NB_CASE(2)
+    TDX_NB_CASE(3)
+    TDX_NB_CASE(4)
+    TDX_NB_CASE(5)
+    TDX_NB_CASE(6)
+    TDX_NB_CASE(7)
+    TDX_NB_CASE(8)
+    default:
+      set_error("unsupported width");
+      return TDX_EUNSUPPORTED;
+  }
+#undef TDX_NB_CASE
+}
+
+// Host-side validation of every table index, so that a malformed table is an
+// error code and never an out-of-bounds access on the device.
+int validate_nn(const tdx_nn_desc *d, int64_t *n_rows_out) {
+  auto in_ints = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_ints; };
+  auto in_reals = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_reals; };
+  auto in_data = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_data; };
+  int64_t n_rows = 0;
+  for (int g = 0; g < d->n_groups; ++g) {
+    const int32_t *G = d->groups + (int64_t)g * TDX_GROUP_INTS;
+    const int inst = G[0], ne = G[1], ns = G[2];
+    TDX_REQUIRE(inst >= 0 && inst < d->n_inst, "group %d: bad instance", g);
+    TDX_REQUIRE(ne >= 1, "group %d: no entries", g);
+    TDX_REQUIRE(ns >= 1 && ns * d->n_out <= 256, "group %d: bad slot count %d", g, ns);
+    TDX_REQUIRE(in_ints(G[3], ns), "group %d: slot table out of range", g);
+    TDX_REQUIRE(in_reals(G[4], (int64_t)ns * d->d_in), "group %d: offsets out of range", g);
+    for (int s = 0; s < ns; ++s)
+      TDX_REQUIRE(in_data(d->ints[G[3] + s], (int64_t)ne * d->d_in),
+                  "group %d slot %d: coordinates out of range", g, s);
+    TDX_REQUIRE(G[5] >= 1 && in_ints(G[6], (int64_t)G[5] * TDX_EQ_INTS), "group %d: eq table", g);
+    TDX_REQUIRE(G[7] == n_rows, "group %d: row base %d, expected %lld", g, G[7], (long long)n_rows);
+    n_rows += (int64_t)ns * ne;
+    for (int q = 0; q < G[5]; ++q) {
+      const int32_t *E = d->ints + G[6] + q * TDX_EQ_INTS;
+      TDX_REQUIRE(E[0] >= 1 && in_ints(E[1], (int64_t)E[0] * TDX_TERM_INTS), "eq table");
+      TDX_REQUIRE(E[2] < 0 || in_data(E[2], ne), "target out of range");
+      TDX_REQUIRE(E[3] < 0 || (E[3] + (int64_t)ne <= d->n_resid), "residual slot out of range");
+      TDX_REQUIRE(in_reals(E[4], 1) && in_reals(E[5], 1), "eq weights out of range");
+      for (int t = 0; t < E[0]; ++t) {
+        const int32_t *Tm = d->ints + E[1] + t * TDX_TERM_INTS;
+        TDX_REQUIRE(Tm[0] >= 1 && Tm[0] <= MAXF, "a term needs 1..%d factors", MAXF);
+        TDX_REQUIRE(in_ints(Tm[1], (int64_t)Tm[0] * TDX_FACTOR_INTS), "factor table");
+        TDX_REQUIRE((Tm[2] == 0 && in_reals(Tm[3], 1)) || (Tm[2] == 1 && in_data(Tm[3], ne)),
+                    "coefficient out of range");
+        for (int f = 0; f < Tm[0]; ++f) {
+          const int32_t *F = d->ints + Tm[1] + f * TDX_FACTOR_INTS;
+          TDX_REQUIRE(F[0] >= 0 && F[0] < d->n_out, "factor variable out of range");
+          TDX_REQUIRE(F[1] == 0 || in_reals(F[5], 1), "power out of range");
+          TDX_REQUIRE(F[3] >= 1 && in_ints(F[4], (int64_t)F[3] * TDX_STENCIL_INTS), "stencil");
+          for (int k = 0; k < F[3]; ++k) {
+            const int32_t *S = d->ints + F[4] + k * TDX_STENCIL_INTS;
+            TDX_REQUIRE(S[0] >= 0 && S[0] < ns && in_reals(S[1], 1), "stencil entry");
+          }
+        }
+      }
+    }
+  }
+  *n_rows_out = n_rows;
+  return TDX_OK;
+}
+
+}  // namespace
+
+extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
+  TDX_REQUIRE(d && out, "null argument");
+  *out = nullptr;
+  TDX_REQUIRE(d->dtype == 0 || d->dtype == 1, "dtype must be 0 (f32) or 1 (f64)");
+  TDX_REQUIRE(d->d_in >= 1 && d->d_in <= MAXD, "d_in must be 1..%d", MAXD);
+  TDX_REQUIRE(d->n_out >= 1 && d->n_out <= MAXO, "n_out must be 1..%d", MAXO);
+  TDX_REQUIRE(d->width >= 1 && d->width <= 128, "width must be 1..128");
+  TDX_REQUIRE(d->n_layers >= 2, "at least two Linear layers");
+  TDX_REQUIRE(d->n_inst >= 1 && d->n_groups >= 1, "empty problem");
+  int64_t n_rows = 0;
+  int rc = validate_nn(d, &n_rows);
+  if (rc) return rc;
+
+  TDX_REQUIRE(n_rows < (1LL << 31), "too many rows");
+  tdx_nn_plan *p = new tdx_nn_plan();
+  p->dtype = d->dtype;
+  p->d_in = d->d_in;
+  p->n_out = d->n_out;
+  p->H = d->width;
+  p->n_layers = d->n_layers;
+  p->NH = d->n_layers - 2;
+  p->NB = (d->width + 15) / 16;
+  p->HP = 16 * p->NB;
+  p->n_inst = d->n_inst;
+  p->n_groups = d->n_groups;
+  p->n_rows = n_rows;
+  const int H = p->H;
+  p->P = H * d->d_in + H + p->NH * (H * H + H) + d->n_out * H + d->n_out;
+  const size_t ts = d->dtype == 0 ? 4 : 8;
+
+  // forward work: 64 entries per workgroup
+  std::vector<int2> k1;
+  std::vector<std::vector<int>> inst_k1(d->n_inst), inst_k2(d->n_inst);
+  for (int g = 0; g < d->n_groups; ++g) {
+    const int32_t *G = d->groups + (int64_t)g * TDX_GROUP_INTS;
+    for (int e = 0; e < G[1]; e += 16 * K1_WAVES) {
+      inst_k1[G[0]].push_back((int)k1.size());
+      k1.push_back(make_int2(g, e));
+    }
+  }
+  int max_slots = 1;
+  for (int g = 0; g < d->n_groups; ++g)
+    max_slots = std::max(max_slots, (int)d->groups[(int64_t)g * TDX_GROUP_INTS + 2]);
+  p->k1_lds = (2 * K1_WAVES * 16 * max_slots * d->n_out + 512) * ts;
+
+  // backward work: tiles of 16*waves rows, chunk_tiles tiles per workgroup
+  const int LS = p->HP + 4;
+  const int NACT = p->NH + 1;
+  int waves = d->tile_rows > 0 ? d->tile_rows / 16 : 4;
+  waves = std::max(1, std::min(4, waves));
+  auto lds_of = [&](int wv, bool with_acc) {
+    const int TR = 16 * wv;
+    size_t n = (size_t)NACT * TR * LS + (size_t)TR * LS + (size_t)TR * MAXD + (size_t)TR * MAXO;
+    if (with_acc) n += p->P;
+    return n * ts;
+  };
+  const size_t LDS_MAX = 160 * 1024;
+  while (waves > 1 && lds_of(waves, false) > LDS_MAX) --waves;
+  if (lds_of(waves, false) > LDS_MAX) {
+    delete p;
+    set_error("network too large for the backward tile (%zu bytes of LDS)", lds_of(waves, false));
+    return TDX_EUNSUPPORTED;
+  }
+  p->k2_waves = waves;
+  p->acc_lds = lds_of(waves, true) <= LDS_MAX ? 1 : 0;
+  p->k2_lds = lds_of(waves, p->acc_lds != 0);
+  const int TR = 16 * waves;
+  const int chunk_tiles = d->chunk_tiles > 0 ? d->chunk_tiles : (p->acc_lds ? 16 : 4);
+  std::vector<int4> k2;
+  for (int g = 0; g < d->n_groups; ++g) {
+    const int32_t *G = d->groups + (int64_t)g * TDX_GROUP_INTS;
+    const int64_t rb = G[7], nr = (int64_t)G[1] * G[2];
+    for (int64_t r = 0; r < nr; r += (int64_t)TR * chunk_tiles) {
+      const int64_t re = std::min(nr, r + (int64_t)TR * chunk_tiles);
+      inst_k2[G[0]].push_back((int)k2.size());
+      k2.push_back(make_int4(g, (int)(rb + r), (int)(rb + re), G[0]));
+    }
+  }
+  p->n_k1 = (int)k1.size();
+  p->n_k2 = (int)k2.size();
+  std::vector<int32_t> ik1p(1, 0), ik1i, ik2p(1, 0), ik2i;
+  for (int i = 0; i < d->n_inst; ++i) {
+    for (int x : inst_k1[i]) ik1i.push_back(x);
+    ik1p.push_back((int32_t)ik1i.size());
+    for (int x : inst_k2[i]) ik2i.push_back(x);
+    ik2p.push_back((int32_t)ik2i.size());
+  }
+  if (ik1i.empty()) ik1i.push_back(0);
+  if (ik2i.empty()) ik2i.push_back(0);
+
+  int r2 = TDX_OK;
+  if (!r2) r2 = p->groups.upload(d->groups, (size_t)d->n_groups * TDX_GROUP_INTS * 4);
+  if (!r2) r2 = p->ints.upload(d->ints, (size_t)std::max<int64_t>(d->n_ints, 1) * 4);
+  if (!r2) r2 = p->reals.upload(d->reals, (size_t)d->n_reals * ts);
+  if (!r2) r2 = p->data.upload(d->data, (size_t)d->n_data * ts);
+  if (!r2) r2 = p->k1_items.upload(k1.data(), k1.size() * sizeof(int2));
+  if (!r2) r2 = p->k2_items.upload(k2.data(), k2.size() * sizeof(int4));
+  if (!r2) r2 = p->ik1p.upload(ik1p.data(), ik1p.size() * 4);
+  if (!r2) r2 = p->ik1i.upload(ik1i.data(), ik1i.size() * 4);
+  if (!r2) r2 = p->ik2p.upload(ik2p.data(), ik2p.size() * 4);
+  if (!r2) r2 = p->ik2i.upload(ik2i.data(), ik2i.size() * 4);
+  if (r2) {
+    tdx_nn_plan_destroy(p);
+    return r2;
+  }
+  size_t off = 0;
+  p->off_packed = off;
+  off = align_up(off + (size_t)p->n_inst * std::max(p->NH, 0) * 2 * p->HP * p->HP * ts, 256);
+  p->off_seeds = off;
+  off = align_up(off + (size_t)n_rows * p->n_out * ts, 256);
+  p->off_lp = off;
+  off = align_up(off + (size_t)p->n_k1 * 2 * ts, 256);
+  p->off_partial = off;
+  off = align_up(off + (size_t)p->n_k2 * p->P * ts, 256);
+  p->off_one = off;
+  off = align_up(off + ts, 256);
+  p->ws_bytes = off;
+  *out = p;
+  return TDX_OK;
+}
+
+extern "C" void tdx_nn_plan_destroy(tdx_nn_plan *p) {
+  if (!p) return;
+  DevTable *t[] = {&p->groups, &p->ints, &p->reals, &p->data, &p->k1_items,
+                   &p->k2_items, &p->ik1p, &p->ik1i, &p->ik2p, &p->ik2i};
+  for (DevTable *x : t) x->release();
+  delete p;
+}
+
+extern "C" int64_t tdx_nn_param_count(const tdx_nn_plan *p) { return p ? p->P : -1; }
+extern "C" size_t tdx_nn_workspace_bytes(const tdx_nn_plan *p) { return p ? p->ws_bytes : 0; }
+
+extern "C" int tdx_nn_forward(const tdx_nn_plan *p, const void *params, void *loss, void *resid,
+                              void *ws, tdx_stream_t stream) {
+  TDX_REQUIRE(p && params && ws, "null argument");
+  AdamHyper h{0, 0, 0, 0};
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == 0)
+    return nn_dispatch<float>(p, 0, params, nullptr, loss, resid, nullptr, nullptr, nullptr,
+                              nullptr, nullptr, h, ws, s);
+  return nn_dispatch<double>(p, 0, params, nullptr, loss, resid, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, h, ws, s);
+}
+
+extern "C" int tdx_nn_backward(const tdx_nn_plan *p, const void *params, const void *dloss,
+                               void *dparams, void *ws, tdx_stream_t stream) {
+  TDX_REQUIRE(p && params && dloss && dparams && ws, "null argument");
+  AdamHyper h{0, 0, 0, 0};
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == 0)
+    return nn_dispatch<float>(p, 1, params, nullptr, nullptr, nullptr, dloss, dparams, nullptr,
+                              nullptr, nullptr, h, ws, s);
+  return nn_dispatch<double>(p, 1, params, nullptr, nullptr, nullptr, dloss, dparams, nullptr,
+                             nullptr, nullptr, h, ws, s);
+}
+
+extern "C" int tdx_nn_train_step(const tdx_nn_plan *p, void *params, void *m, void *v,
+                                 int64_t *step, double lr, double b1, double b2, double eps,
+                                 void *loss, void *ws, tdx_stream_t stream) {
+  TDX_REQUIRE(p && params && m && v && step && ws, "null argument");
+  AdamHyper h{lr, b1, b2, eps};
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == 0)
+    return nn_dispatch<float>(p, 2, nullptr, params, loss, nullptr, nullptr, nullptr, m, v,
+                              step, h, ws, s);
+  return nn_dispatch<double>(p, 2, nullptr, params, loss, nullptr, nullptr, nullptr, m, v,
+                             step, h, ws, s);
+}
+
+extern "C" int tdx_adam(int dtype, int64_t n, void *param, const void *grad, void *m, void *v,
+                        const int64_t *step, double lr, double b1, double b2, double eps,
+                        tdx_stream_t stream) {
+  TDX_REQUIRE(param && grad && m && v && step && n >= 0, "null argument");
+  if (n == 0) return TDX_OK;
+  AdamHyper h{lr, b1, b2, eps};
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (dtype == 0)
+    hipLaunchKernelGGL((k_adam<float>), dim3(blocks), dim3(256), 0, s, n, (float *)param,
+                       (const float *)grad, (float *)m, (float *)v, step, h);
+  else
+    hipLaunchKernelGGL((k_adam<double>), dim3(blocks), dim3(256), 0, s, n, (double *)param,
+                       (const double *)grad, (double *)m, (double *)v, step, h);
+  TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}
+
+extern "C" int tdx_step_increment(int64_t *step, tdx_stream_t stream) {
+  TDX_REQUIRE(step, "null argument");
+  hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, (hipStream_t)stream, step);
+  TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}

@@ -1,0 +1,142 @@
+"""ctypes binding of ``libtdx.so`` (declared in include/tdx.h).
+
+The library is required: there is no fallback.  If it is missing or fails to
+load, every native entry point raises ``NativeUnavailable``.
+"""
+
+import ctypes
+import os
+from typing import Optional
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libtdx.so')
+
+TDX_GROUP_INTS = 8
+TDX_EQ_INTS = 6
+TDX_TERM_INTS = 4
+TDX_FACTOR_INTS = 6
+TDX_STENCIL_INTS = 2
+TDX_MEQ_INTS = 3
+TDX_MTERM_INTS = 4
+TDX_MFACTOR_INTS = 8
+TDX_MBC_INTS = 10
+
+EXPORTS = [
+    'tdx_version', 'tdx_last_error',
+    'tdx_nn_plan_create', 'tdx_nn_plan_destroy', 'tdx_nn_param_count',
+    'tdx_nn_workspace_bytes', 'tdx_nn_forward', 'tdx_nn_backward',
+    'tdx_nn_train_step',
+    'tdx_mat_plan_create', 'tdx_mat_plan_destroy', 'tdx_mat_workspace_bytes',
+    'tdx_mat_forward', 'tdx_mat_backward', 'tdx_mat_train_step',
+    'tdx_adam', 'tdx_step_increment',
+]
+
+
+class NativeUnavailable(RuntimeError):
+    pass
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class NNDesc(ctypes.Structure):
+    _fields_ = [
+        ('dtype', ctypes.c_int), ('d_in', ctypes.c_int), ('n_out', ctypes.c_int),
+        ('width', ctypes.c_int), ('n_layers', ctypes.c_int), ('n_inst', ctypes.c_int),
+        ('n_groups', ctypes.c_int), ('groups', c_i32p),
+        ('n_ints', ctypes.c_int64), ('ints', c_i32p),
+        ('n_reals', ctypes.c_int64), ('reals', ctypes.c_void_p),
+        ('n_data', ctypes.c_int64), ('data', ctypes.c_void_p),
+        ('n_resid', ctypes.c_int64),
+        ('tile_rows', ctypes.c_int), ('chunk_tiles', ctypes.c_int),
+    ]
+
+
+class MatDesc(ctypes.Structure):
+    _fields_ = [
+        ('dtype', ctypes.c_int), ('n_eq', ctypes.c_int), ('n0', ctypes.c_int),
+        ('n1', ctypes.c_int), ('dp', ctypes.c_int), ('h', ctypes.c_double * 2),
+        ('alpha_back', c_dp), ('alpha_forw', c_dp),
+        ('eq_i', ctypes.c_int), ('n_bc', ctypes.c_int), ('bc_i', ctypes.c_int),
+        ('n_bc_seeds', ctypes.c_int), ('n_touch', ctypes.c_int),
+        ('bct_ptr_i', ctypes.c_int), ('bct_pos_i', ctypes.c_int),
+        ('bct_seed_i', ctypes.c_int), ('bct_w_i', ctypes.c_int),
+        ('n_ints', ctypes.c_int64), ('ints', c_i32p),
+        ('n_reals', ctypes.c_int64), ('reals', ctypes.c_void_p),
+        ('n_data', ctypes.c_int64), ('data', ctypes.c_void_p),
+        ('exact_div', ctypes.c_int),
+    ]
+
+
+_lib: Optional[ctypes.CDLL] = None
+_load_error: Optional[str] = None
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    vp, i64, dbl, sz = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_size_t
+    lib.tdx_version.restype = ctypes.c_int
+    lib.tdx_last_error.restype = ctypes.c_char_p
+    lib.tdx_nn_plan_create.argtypes = [ctypes.POINTER(NNDesc), ctypes.POINTER(vp)]
+    lib.tdx_nn_plan_destroy.argtypes = [vp]
+    lib.tdx_nn_plan_destroy.restype = None
+    lib.tdx_nn_param_count.argtypes = [vp]
+    lib.tdx_nn_param_count.restype = i64
+    lib.tdx_nn_workspace_bytes.argtypes = [vp]
+    lib.tdx_nn_workspace_bytes.restype = sz
+    lib.tdx_nn_forward.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.tdx_nn_backward.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.tdx_nn_train_step.argtypes = [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]
+    lib.tdx_mat_plan_create.argtypes = [ctypes.POINTER(MatDesc), ctypes.POINTER(vp)]
+    lib.tdx_mat_plan_destroy.argtypes = [vp]
+    lib.tdx_mat_plan_destroy.restype = None
+    lib.tdx_mat_workspace_bytes.argtypes = [vp]
+    lib.tdx_mat_workspace_bytes.restype = sz
+    lib.tdx_mat_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.tdx_mat_backward.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.tdx_mat_train_step.argtypes = [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]
+    lib.tdx_adam.argtypes = [ctypes.c_int, i64, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp]
+    lib.tdx_step_increment.argtypes = [vp, vp]
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises ``NativeUnavailable`` if it cannot load."""
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeUnavailable(
+            f"{LIB_PATH} is missing: build it with "
+            "`python -m torch_de_solver_amd.build`")
+    try:
+        l = ctypes.CDLL(LIB_PATH)
+    except OSError as exc:
+        raise NativeUnavailable(f"cannot load {LIB_PATH}: {exc}") from exc
+    _declare(l)
+    _lib = l
+    return l
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().tdx_last_error().decode(errors='replace')
+        raise NativeError(f"tdx error {rc}: {msg}")
+
+
+def ptr(t) -> int:
+    """Device (or host) address of a tensor or numpy array."""
+    if t is None:
+        return None
+    if hasattr(t, 'data_ptr'):
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+def stream_ptr(stream=None) -> int:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream

@@ -1,0 +1,89 @@
+"""``Optimizer`` and ``Closure`` (tedeous/optimizers/optimizer.py:11-73,
+tedeous/optimizers/closure.py:5-140), for the optimizers this engine serves.
+
+The swarm, natural-gradient and Nystrom-Newton optimizers of the reference
+(PSO, CSO, NGD, NNCG) are not provided.
+"""
+
+from typing import Union
+
+import torch
+from torch.optim.lr_scheduler import CosineAnnealingWarmRestarts, ExponentialLR
+
+_TORCH = {
+    'Adam': torch.optim.Adam,
+    'AdamW': torch.optim.AdamW,
+    'SGD': torch.optim.SGD,
+    'LBFGS': torch.optim.LBFGS,
+    'RMSprop': torch.optim.RMSprop,
+}
+
+
+class Optimizer:
+    def __init__(self, optimizer: Union[str, dict], params: dict,
+                 gamma: Union[float, None] = None, decay_every: Union[int, None] = None,
+                 cosine_scheduler_patience: Union[float, None] = None):
+        self.optimizer = optimizer
+        self.params = params
+        self.gamma = gamma
+        self.decay_every = decay_every
+        self.cosine_scheduler_patience = cosine_scheduler_patience
+        self.scheduler = None
+
+    def optimizer_choice(self, mode: str, model):
+        """torch optimizer over the network parameters ('NN') or the grid
+        tensor ('mat') (tedeous/optimizers/optimizer.py:25-73)."""
+        if self.optimizer not in _TORCH:
+            raise NotImplementedError(f"optimizer {self.optimizer!r} is not provided")
+        cls = _TORCH[self.optimizer]
+        if mode == 'mat':
+            opt = cls([model.requires_grad_()], **self.params)
+        else:
+            opt = cls(model.parameters(), **self.params)
+        if self.gamma is not None:
+            self.scheduler = ExponentialLR(opt, gamma=self.gamma)
+        if self.cosine_scheduler_patience is not None:
+            self.scheduler = CosineAnnealingWarmRestarts(opt, self.cosine_scheduler_patience)
+        return opt
+
+    def fusable_adam(self):
+        """(lr, beta1, beta2, eps) when the settings are exactly what the fused
+        native step implements (plain Adam, no decay/amsgrad/schedule)."""
+        if self.optimizer != 'Adam' or self.gamma is not None or \
+                self.cosine_scheduler_patience is not None:
+            return None
+        p = dict(self.params)
+        lr = p.pop('lr', 1e-3)
+        betas = p.pop('betas', (0.9, 0.999))
+        eps = p.pop('eps', 1e-8)
+        if p.pop('weight_decay', 0) != 0 or p.pop('amsgrad', False) or \
+                p.pop('maximize', False):
+            return None
+        for k in ('foreach', 'fused', 'capturable', 'differentiable'):
+            p.pop(k, None)
+        if p:
+            return None
+        return float(lr), float(betas[0]), float(betas[1]), float(eps)
+
+
+class Closure:
+    """zero_grad -> evaluate -> backward (tedeous/optimizers/closure.py:49-64)."""
+
+    def __init__(self, mixed_precision: bool, model):
+        if mixed_precision:
+            raise NotImplementedError("mixed precision is not served: stencils stay in FP32")
+        self.model = model
+        self.optimizer = model.optimizer
+        self.normalized_loss_stop = model.normalized_loss_stop
+
+    def _closure(self):
+        self.optimizer.zero_grad()
+        loss, loss_normalized = self.model.solution_cls.evaluate()
+        loss.backward()
+        self.model.cur_loss = loss_normalized if self.normalized_loss_stop else loss
+        return loss
+
+    def get_closure(self, _type: str):
+        if _type in ('PSO', 'CSO', 'NGD', 'NNCG'):
+            raise NotImplementedError(f"optimizer {_type} is not provided")
+        return self._closure

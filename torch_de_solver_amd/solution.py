@@ -1,0 +1,175 @@
+"""``Solution``: the per-step objective, served by the native engines.
+
+Drop-in for tedeous/solution.py:19-168.  ``evaluate()`` returns
+``(loss, loss_normalized)`` with shape ``[1]`` each, like the reference
+(tedeous/losses.py:119-135), and differentiates through the native adjoint.
+The reference's side outputs ``op``, ``bval``, ``true_bval``, ``bval_keys``
+and ``bval_length`` (read by callbacks, tedeous/callbacks/adaptive_lambda.py:
+90-105) are computed on demand from the native residual output.
+"""
+
+from typing import List, Optional, Tuple, Union
+
+import torch
+
+from .compiler import MatProblem, NNProblem, compile_mat, compile_nn
+from .device import check_device
+from .mat_engine import MatEngine
+from .nn_engine import MLPArch, NNEngine, mlp_arch
+
+
+class Operator_bcond_preproc:
+    """Holds the compile inputs (tedeous/input_preprocessing.py:597-644).
+
+    ``set_strategy(mode)`` returns the holder itself tagged with the mode;
+    the lowering to kernel tables happens in ``Solution``.
+    """
+
+    def __init__(self, grid: torch.Tensor, operator, bconds, h: float = 0.001,
+                 inner_order: str = '1', boundary_order: str = '2'):
+        self.grid = check_device(grid)
+        self.operator = operator
+        self.bconds = bconds
+        self.h = h
+        self.inner_order = inner_order
+        self.boundary_order = boundary_order
+        self.mode = None
+
+    def set_strategy(self, strategy: str) -> 'Operator_bcond_preproc':
+        if strategy not in ('NN', 'mat'):
+            raise NotImplementedError(
+                f"mode {strategy!r} is not served by this engine (only 'NN' and 'mat')")
+        self.mode = strategy
+        return self
+
+
+def _operators(op) -> List[dict]:
+    if isinstance(op, dict):
+        return [op]
+    return list(op)
+
+
+class Solution:
+    """Objective of one problem (tedeous/solution.py:19-168)."""
+
+    def __init__(self, grid: torch.Tensor, equal_cls: Operator_bcond_preproc,
+                 model: Union[torch.nn.Module, torch.Tensor], mode: str,
+                 weak_form=None, lambda_operator=1, lambda_bound=1, tol: float = 0,
+                 derivative_points: int = 2, batch_size: int = None,
+                 exact_div: bool = True):
+        if weak_form:
+            raise NotImplementedError("weak-form losses are not served by this engine")
+        if tol:
+            raise NotImplementedError("causal losses (tol != 0) are not served by this engine")
+        if batch_size is not None:
+            raise NotImplementedError("mini-batching is not served by this engine")
+        if mode not in ('NN', 'mat'):
+            raise NotImplementedError(f"mode {mode!r} is not served by this engine")
+        self.grid = grid
+        self.mode = mode
+        self.model = model
+        self.lambda_operator = lambda_operator
+        self.lambda_bound = lambda_bound
+        self.derivative_points = derivative_points
+        ops = _operators(equal_cls.operator)
+        device = torch.device('cuda', torch.cuda.current_device()) if torch.cuda.is_available() \
+            else torch.device('cpu')
+        if mode == 'NN':
+            self.arch: MLPArch = mlp_arch(model)
+            self.problem: NNProblem = compile_nn(
+                grid, ops, equal_cls.bconds, n_out=self.arch.n_out, h=equal_cls.h,
+                inner_order=equal_cls.inner_order, boundary_order=equal_cls.boundary_order,
+                lambda_operator=lambda_operator, lambda_bound=lambda_bound)
+            self.engine = NNEngine([self.problem], self.arch, device)
+        else:
+            self.problem: MatProblem = compile_mat(
+                grid, ops, equal_cls.bconds, n_eq=len(ops), derivative_points=derivative_points,
+                lambda_operator=lambda_operator, lambda_bound=lambda_bound)
+            self.engine = MatEngine(self.problem, device, exact_div=exact_div)
+        self.loss = None
+        self.loss_normalized = None
+
+    # -- per step ------------------------------------------------------------
+    def _flat_params(self) -> torch.Tensor:
+        params = list(self.model.parameters())
+        return torch.cat([p.reshape(-1) for p in params])
+
+    def evaluate(self, save_graph: bool = True) -> Tuple[torch.Tensor, torch.Tensor]:
+        """``(loss, loss_normalized)``, each of shape ``[1]``."""
+        if self.mode == 'NN':
+            pair = self.engine.loss(self._flat_params())[0]
+        else:
+            u = self.model
+            if not u.is_contiguous():
+                u = u.contiguous()
+            pair = self.engine.loss(u)
+        loss = pair[0:1]
+        if not save_graph:
+            loss = loss.detach()
+        self.loss = loss
+        self.loss_normalized = pair[1:2].detach()
+        return self.loss, self.loss_normalized
+
+    # -- side outputs, on demand ----------------------------------------------
+    def _side(self):
+        if self.mode == 'NN':
+            return self._side_nn()
+        return self._side_mat()
+
+    def _side_nn(self):
+        eng: NNEngine = self.engine
+        with torch.no_grad():
+            resid = eng.residuals(self._flat_params().detach())
+        prob = self.problem
+        # interior equations carry no boundary column
+        cols = [resid[r.offset:r.offset + r.count] for r in eng.tables.resid if r.column < 0]
+        n_eq = len(cols)
+        op = torch.stack(cols, 1) if n_eq > 1 else cols[0].reshape(-1, 1)
+        ncol = len(prob.bc_keys)
+        bval = torch.zeros(prob.bc_max_len, max(ncol, 1), dtype=resid.dtype, device=resid.device)
+        true = torch.zeros_like(bval)
+        for r in eng.tables.resid:
+            if r.column < 0:
+                continue
+            pos = torch.as_tensor(r.col_pos, device=resid.device)
+            bval[pos, r.column] = resid[r.offset:r.offset + r.count]
+            g = prob.groups[r.group]
+            t = g.equations[r.eq].target
+            true[pos, r.column] = torch.as_tensor(t, dtype=resid.dtype, device=resid.device)
+        return op, bval[:, :ncol], true[:, :ncol], list(prob.bc_keys), list(prob.bc_lengths)
+
+    def _side_mat(self):
+        eng: MatEngine = self.engine
+        _, op, bv = eng.fields(self.model)
+        prob = self.problem
+        N = op.shape[1]
+        # the reference stacks per-equation residuals broadcast over n_eq
+        # copies (tedeous/derivative.py:306, tedeous/eval.py:183-193)
+        cols = [op[q].repeat(prob.n_eq).reshape(-1, 1) for q in range(prob.n_eq)]
+        opm = cols[0] if len(cols) == 1 else torch.cat(cols, 1)
+        ncol = len(prob.bc_keys)
+        bval = torch.zeros(prob.bc_max_len, max(ncol, 1), dtype=op.dtype, device=op.device)
+        for k, (bi, col, pos) in enumerate(eng.tables.bc_entries):
+            bval[pos, col] = bv[k]
+        true = torch.as_tensor(prob.targets, dtype=op.dtype, device=op.device)
+        return opm, bval[:, :ncol], true, list(prob.bc_keys), list(prob.bc_lengths)
+
+    @property
+    def op(self):
+        return self._side()[0]
+
+    @property
+    def bval(self):
+        return self._side()[1]
+
+    @property
+    def true_bval(self):
+        return self._side()[2]
+
+    @property
+    def bval_keys(self):
+        return list(self.problem.bc_keys)
+
+    @property
+    def bval_length(self):
+        return list(self.problem.bc_lengths)
