@@ -1,0 +1,352 @@
+#!/usr/bin/env python3
+"""Training-throughput benchmark (one JSON line on stdout).
+
+Default workload: the sweep of 256 independent Burgers solves (each with its
+own 2-32-32-32-1 tanh network, nu = 10^U(-3,-1), nx in [64, 512],
+nt in [32, 128]) trained together by the fused native step (forward,
+hand-written adjoint and Adam of every instance in one grouped launch
+sequence, replayed from a HIP graph).  A step is one Adam step of every
+instance.  Reported next to it, measured in the same run:
+
+* plain_gpu: the same sweep trained by the reference's algorithm in stock
+  torch ops on the same GPU, one instance after the other (the per-instance
+  path a reference user runs);
+* cpu_baseline: the CPU oracle (an op-for-op restatement of the reference in
+  torch CPU ops) timed on the host cores on a bounded sample of instances,
+  scaled to the full sweep by collocation points;
+* roofline: FP32-MFMA roofline of the dominant kernel (the backward kernel),
+  from HIP-event timing of its launches during a timed pass.
+
+Other workloads (--workload): burgers, kdv, legendre ('NN', one instance),
+wave (the 128 x 128 'mat' problem) and wave16k (the wave operator on a
+16384 x 16384 field, the HBM-roofline case of the stencil kernels).
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_MFMA_PEAK = 157.3e12     # MI355X dense FP32 matrix, FLOP/s
+HBM_PEAK = 8.0e12             # MI355X HBM3E, B/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def mlp_flops_per_row(arch) -> int:
+    """2 * multiply-adds of one network evaluation."""
+    H, d, o, L = arch.width, arch.d_in, arch.n_out, arch.n_layers
+    return 2 * (d * H + (L - 2) * H * H + H * o)
+
+
+def time_cuda(fn, n):
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(n):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / n
+
+
+def cpu_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+# ---------------------------------------------------------------------------
+# 'NN' workloads
+# ---------------------------------------------------------------------------
+
+def nn_problems(workload):
+    from torch_de_solver_amd import workloads as W
+    if workload == 'sweep':
+        insts = W.sweep_instances(256)
+        return [W.sweep_problem(i) for i in insts], insts
+    if workload == 'burgers':
+        return [W.burgers()], None
+    if workload == 'kdv':
+        return [W.kdv()], None
+    if workload == 'legendre':
+        return [W.legendre()], None
+    raise ValueError(workload)
+
+
+def plain_nn_step_ms(problems, nets0, n_steps, max_instances, lr):
+    """Per-instance plain path on the GPU: ms of one sweep step (sum over
+    instances), plus the sample description."""
+    from torch_de_solver_amd.compiler import compile_nn
+    from torch_de_solver_amd.plain import PlainNN
+    dev = torch.device('cuda')
+    total_ms, pts_done, pts_all = 0.0, 0, 0
+    sel = list(range(len(problems)))[:max_instances]
+    comp = []
+    for i, (dom, eq, conds, net, kw) in enumerate(problems):
+        prob = compile_nn(dom.build('NN'), eq.equation_lst, conds.build(dom.variable_dict),
+                          1, h=kw.get('h', 1e-3), lambda_operator=kw['lambda_operator'],
+                          lambda_bound=kw['lambda_bound'])
+        pts_all += prob.n_central
+        comp.append(prob)
+    for i in sel:
+        net = nets0[i].to(dev)
+        pl = PlainNN(comp[i], net, dev)
+        opt = torch.optim.Adam(net.parameters(), lr=lr)
+
+        def step():
+            opt.zero_grad()
+            loss = pl.loss()
+            loss.backward()
+            opt.step()
+        step()
+        total_ms += time_cuda(step, n_steps)
+        pts_done += comp[i].n_central
+    scale = pts_all / max(pts_done, 1)
+    return total_ms * scale, len(sel), scale
+
+
+def cpu_nn_step_ms(problems, nets0, budget_s, lr):
+    """CPU oracle step time for the sweep, from a bounded sample of
+    instances scaled by collocation points."""
+    from oracle import tedeous_oracle as O
+    torch.set_num_threads(cpu_cores())
+    spent, pts_done, n_inst, steps_done, ms_sum = 0.0, 0, 0, 0, 0.0
+    pts_all = 0
+    from torch_de_solver_amd.workloads import collocation_points
+    for dom, eq, conds, net, kw in problems:
+        axes = list(dom.variable_dict.values())
+        pts_all += collocation_points(len(axes[0]), len(axes[1])) if len(axes) == 2 else len(axes[0])
+    t_start = time.time()
+    for i, (dom, eq, conds, net, kw) in enumerate(problems):
+        if time.time() - t_start > budget_s and n_inst > 0:
+            break
+        orc = O.NNOracle(dict(dom.variable_dict), eq.equation_lst,
+                         [dict(c) for c in conds.conditions_lst], h=kw.get('h', 1e-3))
+        netc = nets0[i].to('cpu')
+        opt = torch.optim.Adam(netc.parameters(), lr=lr)
+        t0 = time.perf_counter()
+        opt.zero_grad()
+        loss = orc.evaluate(netc, kw['lambda_operator'], kw['lambda_bound'])[0]
+        loss.backward()
+        opt.step()
+        ms_sum += (time.perf_counter() - t0) * 1e3
+        pts_done += len(orc.central)
+        n_inst += 1
+    return ms_sum * pts_all / pts_done, n_inst, pts_done, pts_all
+
+
+def bench_nn(args):
+    from torch_de_solver_amd.nn_engine import mlp_arch
+    from torch_de_solver_amd.sweep import SweepSolver
+    problems, insts = nn_problems(args.workload)
+    import copy
+    nets0 = [copy.deepcopy(p[3]) for p in problems]
+    t0 = time.time()
+    solver = SweepSolver(problems)
+    log(f"[bench] compiled {solver.n_inst} instance(s) in {time.time() - t0:.1f}s")
+    eng = solver.engine
+    arch = solver.arch
+    n_rows = eng.tables.n_rows
+    colloc = solver.collocation_points()
+    lr = 1e-3
+    # warm-up: one eager step, capture, W replays
+    solver.capture(lr)
+    solver.replay(args.warmup)
+    torch.cuda.synchronize()
+    log(f"[bench] warm-up done ({args.warmup} replays)")
+    ms = time_cuda(lambda: solver.replay(1), args.steps)
+    loss_now = solver.loss_buf.view(-1, 2)[:, 0].sum().item()
+    log(f"[bench] fused: {ms:.3f} ms/step")
+    # roofline pass: eager steps with event timing of every kernel launch
+    eng.profile(args.steps)
+    for _ in range(args.steps):
+        solver.step(lr)
+    torch.cuda.synchronize()
+    prof = eng.profile_read()
+    eng.profile(0)
+    F = mlp_flops_per_row(arch)
+    bwd_ms = prof['backward'][0] / max(prof['backward'][1], 1)
+    fwd_ms = prof['forward'][0] / max(prof['forward'][1], 1)
+    upd_ms = prof['update'][0] / max(prof['update'][1], 1)
+    bwd_flops = 2.0 * F * n_rows
+    fwd_flops = 1.0 * F * n_rows
+    achieved = bwd_flops / (bwd_ms * 1e-3)
+    log(f"[bench] kernels: forward {fwd_ms:.3f} ms, backward {bwd_ms:.3f} ms, update {upd_ms:.3f} ms")
+    out = {
+        "metric": "train steps/s & collocation pts/s, fused+adjoint vs plain GPU, at equal rel-L2",
+        "value": 1e3 / ms,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload if args.workload != 'sweep' else
+                   "sweep: 256 Burgers solves, nu=10^U(-3,-1), nx~U[64,512], nt~U[32,128], "
+                   "MLP 2-32-32-32-1 tanh each, Adam lr 1e-3, h=1e-3",
+                   "instances": solver.n_inst, "collocation_points": colloc,
+                   "network_rows_per_step": n_rows, "mlp": f"{arch.d_in}-{arch.width}x{arch.n_layers - 1}-{arch.n_out}",
+                   "parallelism": "single GPU"},
+        "colloc_pts_per_s": colloc * 1e3 / ms,
+        "loss_sum_after": loss_now,
+        "roofline": {"bound": "mfma", "kernel": "k_backward (recompute + adjoint + weight gradients)",
+                     "achieved": achieved / 1e12, "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_MFMA_PEAK, "traffic": None,
+                     "algorithmic_flops_per_launch": bwd_flops,
+                     "avg_launch_ms": bwd_ms},
+        "kernel_ms": {"forward": fwd_ms, "backward": bwd_ms, "reduce_adam": upd_ms,
+                      "forward_tflops": fwd_flops / (fwd_ms * 1e-3) / 1e12},
+    }
+    if not args.no_plain:
+        plain_ms, n_sel, scale = plain_nn_step_ms(problems, nets0, args.plain_steps,
+                                                  args.plain_instances, lr)
+        out["plain_gpu"] = {"value": 1e3 / plain_ms, "unit": "steps/s", "ms_per_step": plain_ms,
+                            "instances_timed": n_sel, "scaled_by_points": scale,
+                            "steps_per_instance": args.plain_steps}
+        out["speedup_vs_plain_gpu"] = plain_ms / ms
+        log(f"[bench] plain GPU: {plain_ms:.1f} ms/step ({n_sel} instances timed)")
+    if not args.no_cpu:
+        cms, n_inst, pts, pts_all = cpu_nn_step_ms(problems, nets0, args.cpu_seconds, lr)
+        out["cpu_baseline"] = {"value": 1e3 / cms, "unit": "steps/s", "cores": cpu_cores(),
+                               "kind": "port",
+                               "sample": f"{n_inst} instance(s) x 1 Adam step of the CPU oracle "
+                                         f"({pts} of {pts_all} collocation points), scaled by points"}
+        log(f"[bench] cpu oracle: {cms:.0f} ms/step (scaled)")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# 'mat' workloads
+# ---------------------------------------------------------------------------
+
+def bench_mat(args):
+    import numpy as np
+    from torch_de_solver_amd import workloads as W
+    from torch_de_solver_amd.compiler import compile_mat
+    from torch_de_solver_amd.mat_engine import MatEngine
+    from torch_de_solver_amd.plain import PlainMat
+    n = 16384 if args.workload == 'wave16k' else 128
+    dom, eq, conds, u0, kw = W.wave(n=n)
+    t0 = time.time()
+    grid = dom.build('mat')
+    prob = compile_mat(grid, eq.equation_lst, conds.build(dom.variable_dict), 1,
+                       lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
+    dev = torch.device('cuda')
+    eng = MatEngine(prob, dev, exact_div=not args.fast_div)
+    log(f"[bench] compiled {n}x{n} in {time.time() - t0:.1f}s")
+    u = u0.to(dev).contiguous()
+    m, v = torch.zeros_like(u), torch.zeros_like(u)
+    step = torch.zeros(1, dtype=torch.int64, device=dev)
+    loss = torch.zeros(2, device=dev)
+    lr = 1e-3
+
+    def one():
+        eng.train_step(u, m, v, step, lr, 0.9, 0.999, 1e-8, loss)
+    one()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        one()
+    for _ in range(args.warmup):
+        g.replay()
+    ms = time_cuda(g.replay, args.steps)
+    N = n * n
+    # roofline of the fused residual+adjoint kernel: 8 B/point (read u, write grad)
+    dl = torch.ones(1, device=dev)
+    grad = torch.empty_like(u)
+    bwd_ms = time_cuda(lambda: eng.backward(u, dl, grad), args.steps)
+    achieved = 8.0 * N / (bwd_ms * 1e-3)
+    out = {
+        "metric": "train steps/s & collocation pts/s, fused+adjoint vs plain GPU, at equal rel-L2",
+        "value": 1e3 / ms, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"wave u_tt=4u_xx 'mat' {n}x{n}, Adam lr 1e-3",
+                   "collocation_points": N, "parallelism": "single GPU"},
+        "colloc_pts_per_s": N * 1e3 / ms,
+        "roofline": {"bound": "hbm", "kernel": "k_mat_tile (residual + loss + adjoint)",
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "algorithmic_bytes_per_launch": 8.0 * N, "avg_launch_ms": bwd_ms},
+    }
+    if not args.no_plain and n <= 4096:
+        pl = PlainMat(prob, grid.to(dev))
+        up = u0.to(dev).clone().requires_grad_()
+        opt = torch.optim.Adam([up], lr=lr)
+
+        def pstep():
+            opt.zero_grad()
+            l = pl.loss(up)
+            l.backward()
+            opt.step()
+        pstep()
+        pms = time_cuda(pstep, max(args.plain_steps, 3))
+        out["plain_gpu"] = {"value": 1e3 / pms, "unit": "steps/s", "ms_per_step": pms}
+        out["speedup_vs_plain_gpu"] = pms / ms
+    if not args.no_cpu and n <= 4096:
+        from oracle import tedeous_oracle as O
+        torch.set_num_threads(cpu_cores())
+        orc = O.MatOracle(dict(dom.variable_dict), eq.equation_lst,
+                          [dict(c) for c in conds.conditions_lst])
+        uc = u0.clone().requires_grad_()
+        opt = torch.optim.Adam([uc], lr=lr)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10) or k < 2:
+            opt.zero_grad()
+            l = orc.evaluate(uc, kw['lambda_operator'], kw['lambda_bound'])[0]
+            l.backward()
+            opt.step()
+            k += 1
+        cms = (time.perf_counter() - t0) * 1e3 / k
+        out["cpu_baseline"] = {"value": 1e3 / cms, "unit": "steps/s", "cores": cpu_cores(),
+                               "kind": "port", "sample": f"{k} Adam steps of the CPU oracle"}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--workload', default='sweep',
+                    choices=['sweep', 'burgers', 'kdv', 'legendre', 'wave', 'wave16k'])
+    ap.add_argument('--plain-steps', type=int, default=2)
+    ap.add_argument('--plain-instances', type=int, default=256)
+    ap.add_argument('--cpu-seconds', type=float, default=20.0)
+    ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--fast-div', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(2)
+    if args.workload in ('wave', 'wave16k'):
+        out = bench_mat(args)
+    else:
+        out = bench_nn(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -135,6 +135,17 @@ int tdx_nn_train_step(const tdx_nn_plan *plan, void *params, void *exp_avg,
                       double beta1, double beta2, double eps, void *loss,
                       void *workspace, tdx_stream_t stream);
 
+/* Launch timing of the kernels of a plan, for measurement: with capacity
+ * > 0 every following launch of the forward (kind 0), backward (kind 1) and
+ * reduce/update (kind 2) kernels is bracketed by a pair of HIP events on its
+ * stream, up to capacity launches per kind; tdx_nn_profile_read waits for
+ * them and returns the summed milliseconds and launch counts per kind (and
+ * resets the counts).  capacity 0 turns timing off.  Not for use under
+ * graph capture.  No counterpart in the reference (measurement only). */
+#define TDX_NN_KERNEL_KINDS 3
+int tdx_nn_profile(tdx_nn_plan *plan, int capacity);
+int tdx_nn_profile_read(tdx_nn_plan *plan, double *ms, int *count);
+
 /* ===================================================================== */
 /* 'mat' mode                                                             */
 /* ===================================================================== */

@@ -1,0 +1,171 @@
+"""CPU tests of the host side: the library loads and exports every symbol of
+include/tdx.h, the compiler's tables against closed forms and against the
+oracle's own preparation, and plan validation (error codes, no launches)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import problems
+from oracle import tedeous_oracle as O
+from torch_de_solver_amd import data as api
+from torch_de_solver_amd import native
+from torch_de_solver_amd.compiler import compile_mat, compile_nn
+from torch_de_solver_amd.finite_diffs import mat_alphas, nn_scheme
+from torch_de_solver_amd.points_type import point_types, types_by_hull
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, 'include', 'tdx.h')).read()
+    return sorted(set(re.findall(r'\b(tdx_[a-z0-9_]+)\s*\(', text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = native.lib()
+    syms = header_symbols()
+    assert syms, "no declarations found"
+    for s in syms:
+        assert hasattr(lib, s), f"missing export {s}"
+    assert set(syms) == set(native.EXPORTS)
+    assert lib.tdx_version() == 1
+
+
+def test_plan_validation_rejects_bad_tables():
+    """Malformed tables give an error code and a message, never a launch."""
+    lib = native.lib()
+    d = native.NNDesc()
+    d.dtype, d.d_in, d.n_out, d.width, d.n_layers, d.n_inst = 0, 2, 1, 16, 3, 1
+    groups = np.array([0, 10, 1, 0, 0, 1, 1, 0], dtype=np.int32)  # slot table ok
+    ints = np.array([999999], dtype=np.int32)                    # coordinates out of range
+    reals = np.zeros(4, dtype=np.float32)
+    data = np.zeros(4, dtype=np.float32)
+    d.n_groups = 1
+    d.groups = groups.ctypes.data_as(native.c_i32p)
+    d.n_ints, d.ints = ints.size, ints.ctypes.data_as(native.c_i32p)
+    d.n_reals, d.reals = reals.size, reals.ctypes.data
+    d.n_data, d.data = data.size, data.ctypes.data
+    h = ctypes.c_void_p()
+    rc = lib.tdx_nn_plan_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == -1 and not h.value
+    assert b'out of range' in lib.tdx_last_error()
+    d.width = 1000
+    assert lib.tdx_nn_plan_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert lib.tdx_nn_plan_create(None, ctypes.byref(h)) == -1
+
+
+@pytest.mark.parametrize('term,ptype,label', [
+    ([0], 'central', '1'), ([0, 0], 'central', '1'), ([0, 0, 0], 'central', '1'),
+    ([0, 1], 'central', '1'), ([1], 'fb', '1'), ([0], 'fb', '1'), ([0, 1], 'bf', '2'),
+    ([1, 1], 'ff', '2'), ([0], 'bb', '2'),
+])
+def test_nn_scheme_matches_oracle(term, ptype, label):
+    h = 1e-3
+    mine = nn_scheme(term, 2, ptype, label, h)
+    sh, sg = O.scheme(term, 2, ptype, label, h)
+    assert [tuple(s) for s in sh] == [s for s, _ in mine]
+    assert [w for _, w in mine] == list(sg)   # same double arithmetic: identical
+
+
+def test_inner_order_2_is_rejected():
+    with pytest.raises(ValueError):
+        nn_scheme([0], 1, 'central', '2', 1e-3)
+
+
+@pytest.mark.parametrize('dp', [2, 3, 5])
+def test_mat_alphas_match_oracle(dp):
+    ab, af, back, forw = mat_alphas(dp)
+    b2, f2, ab2, af2 = O.mat_alphas(dp)
+    np.testing.assert_allclose(ab, ab2, rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(af, af2, rtol=1e-14, atol=1e-15)
+
+
+def test_point_types_closed_form_and_hull():
+    x = torch.linspace(0, 1, 7)
+    t = torch.linspace(0, 2, 5)
+    g = torch.cartesian_prod(x, t).numpy()
+    box = point_types(g)
+    hull = types_by_hull(g)
+    assert list(box) == list(hull)
+    central = np.nonzero(box == 'central')[0]
+    assert len(central) == 5 * 3                 # strict interior
+    ref = O.point_types(torch.from_numpy(g))
+    assert list(box) == ref
+    # 1-D: every point is central
+    assert set(point_types(np.linspace(0, 1, 9).reshape(-1, 1))) == {'central'}
+
+
+@pytest.mark.parametrize('name', list(problems.NN_CASES))
+def test_nn_tables_reproduce_oracle_shifted_grids(name):
+    """Every (factor, stencil entry) of the compiled interior group points at
+    the slot whose input equals the oracle's materialised shifted grid."""
+    builder, layers = problems.NN_CASES[name]
+    d, eq, b, kw = builder(api)
+    grid = d.build('NN')
+    prob = compile_nn(grid, eq.equation_lst, b.build(d.variable_dict), 1, h=kw.get('h', 1e-3),
+                      lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
+    orc = O.NNOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst],
+                     h=kw.get('h', 1e-3))
+    g0 = prob.groups[0]
+    base = torch.from_numpy(g0.bases[0])
+    assert torch.equal(base, orc.central)
+    op = orc.ops[0]
+    for t_mine, term in zip(g0.equations[0].terms, op.values()):
+        key = list(term.keys())[1]
+        for f_mine, grids, signs in zip(t_mine.factors, term[key][0], term[key][1]):
+            assert len(f_mine.stencil) == len(grids)
+            for (slot, w), x_ref, w_ref in zip(f_mine.stencil, grids, signs):
+                off = torch.tensor(g0.slot_offset[slot], dtype=torch.float64)
+                x = base.clone()
+                for a in range(x.shape[1]):
+                    x[:, a] = x[:, a] + float(off[a])
+                assert torch.equal(x, x_ref)
+                assert np.float32(w) == np.float32(w_ref)
+
+
+def test_bc_columns_padding_and_targets(golden):
+    """Column order, lengths and padded targets equal the reference's."""
+    for name in ('kdv_nn', 'legendre_nn_opbc', 'burgers_nn'):
+        g = golden(f'{name}_float32')
+        builder, layers = problems.NN_CASES[name]
+        d, eq, b, kw = builder(api)
+        prob = compile_nn(d.build('NN'), eq.equation_lst, b.build(d.variable_dict), 1,
+                          h=kw.get('h', 1e-3), lambda_bound=kw['lambda_bound'])
+        assert prob.bc_keys == list(g['keys'])
+        assert prob.bc_lengths == list(g['lens'])
+        true = np.zeros_like(g['true_bval'])
+        for grp in prob.groups[1:]:
+            for eq_ in grp.equations:
+                true[eq_.col_pos, eq_.column] = eq_.target
+        np.testing.assert_array_equal(true, g['true_bval'])
+
+
+def test_unsupported_features_raise():
+    d, eq, b, kw = problems.burgers_nn(api)
+    grid = d.build('NN')
+    bc = b.build(d.variable_dict)
+    bad = [dict(eq.equation_lst[0])]
+    bad[0] = {k: dict(v) for k, v in bad[0].items()}
+    bad[0]['du/dt']['pow'] = lambda x: x
+    with pytest.raises(NotImplementedError):
+        compile_nn(grid, bad, bc, 1)
+    with pytest.raises(NotImplementedError):
+        compile_nn(grid, eq.equation_lst, bc + [{'type': 'robin', 'bnd': bc[0]['bnd'],
+                                                 'bop': None, 'bval': bc[0]['bval'], 'var': 0}], 1)
+
+
+def test_native_forward_requires_device_tensors():
+    """The product path has no CPU fallback: without a HIP device the
+    engines refuse to build."""
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    from torch_de_solver_amd.nn_engine import NNEngine, mlp_arch
+    d, eq, b, kw = problems.burgers_nn(api)
+    net = problems.make_mlp([2, 12, 12, 12, 1])
+    prob = compile_nn(d.build('NN'), eq.equation_lst, b.build(d.variable_dict), 1)
+    with pytest.raises(native.NativeUnavailable):
+        NNEngine([prob], mlp_arch(net), torch.device('cpu'))

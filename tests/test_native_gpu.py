@@ -359,3 +359,28 @@ def test_model_api_mat_trains():
     model.train(tdx.Optimizer('Adam', {'lr': 1e-3}), 200)
     l1 = float(model.solution_cls.evaluate()[0])
     assert l1 < 0.5 * l0
+
+
+@pytest.mark.parametrize('name', ['burgers_nn', 'kdv_nn', 'legendre_nn_opbc'])
+def test_nn_backward_kernels_agree(name, golden):
+    """The wavefront-private backward (narrow nets) and the workgroup-
+    cooperative backward (forced with tile_rows=-1) give the reference's
+    float64 gradient."""
+    from torch_de_solver_amd.compiler import compile_nn
+    from torch_de_solver_amd.nn_engine import NNEngine, mlp_arch
+    g = golden(f'{name}_float64')
+    builder, layers = problems.NN_CASES[name]
+    d, eq, b, kw = builder(api, dtype='float64')
+    net = problems.make_mlp(layers, dtype=torch.float64)
+    prob = compile_nn(d.build('NN'), eq.equation_lst, b.build(d.variable_dict), 1,
+                      h=kw.get('h', 1e-3), lambda_operator=kw['lambda_operator'],
+                      lambda_bound=kw['lambda_bound'])
+    flat = torch.from_numpy(g['params']).to(DEV)
+    grads = []
+    for tile_rows, chunk in ((0, 0), (-1, 0), (-1, 3), (0, 1)):
+        eng = NNEngine([prob], mlp_arch(net), DEV, tile_rows=tile_rows, chunk_tiles=chunk)
+        p = flat.clone().requires_grad_()
+        eng.loss(p)[0, 0].backward()
+        grads.append(p.grad.cpu().numpy())
+    for gr in grads:
+        np.testing.assert_allclose(gr, g['grad'], rtol=1e-8, atol=1e-10 * np.abs(g['grad']).max())

@@ -389,27 +389,40 @@ def _mat_positions(grid: torch.Tensor, pts: torch.Tensor) -> np.ndarray:
     """Flat positions of boundary points (tedeous/input_preprocessing.py:553-575).
 
     In d>1 every position whose coordinates are all ``isclose`` to the point
-    is taken (zero or several are possible, as in the reference).  In 1-D a
-    single ``isclose`` match is taken, else the nearest point
+    is taken (zero or several are possible, as in the reference), in
+    row-major order.  The grid is an ``ij`` meshgrid, so the test factors
+    into one ``isclose`` per axis on the axis values.  In 1-D a single
+    ``isclose`` match is taken, else the nearest point
     (tedeous/input_preprocessing.py:84-150).
     """
     g = grid.detach().cpu()
     d = g.shape[0]
-    flat = [g[a].reshape(-1).to(torch.float32) for a in range(d)]
+    shape = tuple(g.shape[1:])
     out = []
-    for p in pts.detach().cpu():
-        if d == 1:
-            gd = g.reshape(-1).double()
-            m = torch.nonzero(torch.isclose(gd, p.double().reshape(-1)[0])).reshape(-1)
-            if m.numel() == 1:
-                out.append(int(m[0]))
-            else:
-                out.append(int(torch.argmin((gd - p.double().reshape(-1)[0]).abs())))
-            continue
-        mask = torch.ones_like(flat[0], dtype=torch.bool)
-        for a in range(d):
-            mask &= torch.isclose(p[a].float(), flat[a])
-        out.extend(torch.nonzero(mask).reshape(-1).tolist())
+    if d == 1:
+        gd = g.reshape(-1).double()
+        for p in pts.detach().cpu():
+            v = p.double().reshape(-1)[0]
+            m = torch.nonzero(torch.isclose(gd, v)).reshape(-1)
+            out.append(int(m[0]) if m.numel() == 1 else int(torch.argmin((gd - v).abs())))
+        return np.asarray(out, dtype=np.int64)
+    # axis values of the meshgrid: grid[a] varies along axis a only
+    axis_vals = []
+    for a in range(d):
+        idx = [0] * d
+        idx[a] = slice(None)
+        axis_vals.append(g[a][tuple(idx)].reshape(-1).to(torch.float32))
+    strides = np.cumprod((1,) + shape[::-1])[::-1][1:]
+    P = pts.detach().cpu().to(torch.float32)
+    per_axis = []
+    for a in range(d):
+        # [n_pts, n_a] closeness of each point's coordinate to the axis values
+        per_axis.append(torch.isclose(P[:, a:a + 1], axis_vals[a].reshape(1, -1)).numpy())
+    import itertools
+    for i in range(P.shape[0]):
+        lists = [np.nonzero(per_axis[a][i])[0] for a in range(d)]
+        for combo in itertools.product(*lists):
+            out.append(int(sum(int(c) * int(s) for c, s in zip(combo, strides))))
     return np.asarray(out, dtype=np.int64)
 
 
@@ -427,7 +440,11 @@ def compile_mat(grid: torch.Tensor, operators: List[dict], bconds: Optional[List
     # dtype (tedeous/derivative.py:229-247)
     hs = []
     for a in range(d):
-        u = torch.unique(g[a].reshape(-1).float())
+        # the meshgrid holds each axis value along its own axis only, so the
+        # sorted unique values of the field are those of the axis
+        idx = [0] * d
+        idx[a] = slice(None)
+        u = torch.unique(g[a][tuple(idx)].reshape(-1).float())
         hs.append(float(torch.abs(u[1] - u[0])) if u.numel() > 1 else 1.0)
     ab, af, _, _ = mat_alphas(derivative_points)
     if isinstance(operators, dict):

@@ -85,8 +85,12 @@ struct Layout {
 // ---------------------------------------------------------------------------
 // weight packing
 // ---------------------------------------------------------------------------
-// Wf[j][ob][mb][i][lane] = W_j[16*ob + lane%16][16*mb + crow(lane/16, i)]
-// Wb[j][ib][mb][i][lane] = W_j[16*mb + crow(lane/16, i)][16*ib + lane%16]
+// Per hidden layer j, two operand images of W_j (HP*HP values each), laid
+// out so that lane l reads the four values of k-steps i = 0..3 of block
+// (xb, mb) with one 128-bit load:
+//   Wf[j][ob][mb][lane][i] = W_j[16*ob + lane%16][16*mb + crow(lane/16, i)]
+//   Wb[j][ib][mb][lane][i] = W_j[16*mb + crow(lane/16, i)][16*ib + lane%16]
+// Global image of instance n: [n][j][which][xb][mb][lane][i].
 template <typename T, int NB>
 __global__ void k_pack(NNArgs<T> a) {
   constexpr int HP = 16 * NB;
@@ -96,8 +100,8 @@ __global__ void k_pack(NNArgs<T> a) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     int64_t r = t;
-    const int lane = r % 64; r /= 64;
     const int i = r % 4; r /= 4;
+    const int lane = r % 64; r /= 64;
     const int mb = r % NB; r /= NB;
     const int xb = r % NB; r /= NB;
     const int which = r % 2; r /= 2;
@@ -118,13 +122,19 @@ __global__ void k_pack(NNArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
-// forward of 16 rows per wavefront
+// network view: weights in global memory or staged in LDS
 // ---------------------------------------------------------------------------
+// The staged image of one instance (in LDS) is
+//   [Wf: NH*HP^2][Wb: NH*HP^2 (backward only)][W1: H*d_in][b1: H]
+//   [bh: NH*H][WL: n_out*H][bL: n_out]
 template <typename T, int NB>
 struct Net {
-  const T *W1, *b1, *WL, *bL, *params;
+  const T *W1, *b1, *bh, *WL, *bL;
+  int bh_stride;            // distance between consecutive hidden biases
   const T *Wf, *Wb;
+  int w_stride;             // distance between consecutive layers' images
   Layout<T> L;
+  __device__ __forceinline__ const T *bias(int j) const { return bh + j * bh_stride; }
 };
 
 template <typename T, int NB>
@@ -132,13 +142,64 @@ __device__ __forceinline__ Net<T, NB> make_net(const NNArgs<T> &a, int inst) {
   constexpr int HP = 16 * NB;
   Net<T, NB> n;
   n.L = Layout<T>{a.d_in, a.n_out, a.H, a.NH};
-  n.params = a.params + (int64_t)inst * a.P;
-  n.W1 = n.params + n.L.oW1();
-  n.b1 = n.params + n.L.ob1();
-  n.WL = n.params + n.L.oWL();
-  n.bL = n.params + n.L.obL();
+  const T *params = a.params + (int64_t)inst * a.P;
+  n.W1 = params + n.L.oW1();
+  n.b1 = params + n.L.ob1();
+  n.bh = params + n.L.obh(0);
+  n.bh_stride = a.H * a.H + a.H;
+  n.WL = params + n.L.oWL();
+  n.bL = params + n.L.obL();
   n.Wf = a.packed + (int64_t)inst * a.NH * 2 * HP * HP;
-  n.Wb = n.Wf + (int64_t)HP * HP;  // layer j: Wf at 2j, Wb at 2j+1
+  n.Wb = n.Wf + (int64_t)HP * HP;
+  n.w_stride = 2 * HP * HP;
+  return n;
+}
+
+// Values of the staged image (LDS words, in T units).
+__host__ __device__ inline int stage_words(int NH, int HP, int H, int d_in, int n_out, bool with_wb) {
+  return NH * HP * HP * (with_wb ? 2 : 1) + H * d_in + H + NH * H + n_out * H + n_out;
+}
+
+// Cooperative copy of one instance's weights into LDS at `dst`; returns the
+// network view on the copy.  Ends with a workgroup barrier.
+template <typename T, int NB>
+__device__ Net<T, NB> stage_net(const NNArgs<T> &a, int inst, T *dst, bool with_wb) {
+  constexpr int HP = 16 * NB;
+  const Net<T, NB> g = make_net<T, NB>(a, inst);
+  const int H = a.H, d = a.d_in, o = a.n_out, NH = a.NH;
+  const int img = HP * HP;
+  T *Wf = dst, *Wb = dst + NH * img;
+  T *W1 = Wb + (with_wb ? NH * img : 0);
+  T *b1 = W1 + H * d, *bh = b1 + H, *WL = bh + NH * H, *bL = WL + o * H;
+  typedef typename V4<T>::type v4;
+  for (int j = 0; j < NH; ++j) {
+    const v4 *sf = reinterpret_cast<const v4 *>(g.Wf + (int64_t)j * g.w_stride);
+    v4 *df = reinterpret_cast<v4 *>(Wf + j * img);
+    for (int k = threadIdx.x; k < img / 4; k += blockDim.x) df[k] = sf[k];
+    if (with_wb) {
+      const v4 *sb = reinterpret_cast<const v4 *>(g.Wb + (int64_t)j * g.w_stride);
+      v4 *db = reinterpret_cast<v4 *>(Wb + j * img);
+      for (int k = threadIdx.x; k < img / 4; k += blockDim.x) db[k] = sb[k];
+    }
+  }
+  for (int k = threadIdx.x; k < H * d; k += blockDim.x) W1[k] = g.W1[k];
+  for (int k = threadIdx.x; k < H; k += blockDim.x) b1[k] = g.b1[k];
+  for (int k = threadIdx.x; k < NH * H; k += blockDim.x)
+    bh[k] = g.bias(k / H)[k - (k / H) * H];
+  for (int k = threadIdx.x; k < o * H; k += blockDim.x) WL[k] = g.WL[k];
+  for (int k = threadIdx.x; k < o; k += blockDim.x) bL[k] = g.bL[k];
+  __syncthreads();
+  Net<T, NB> n;
+  n.L = g.L;
+  n.W1 = W1;
+  n.b1 = b1;
+  n.bh = bh;
+  n.bh_stride = H;
+  n.WL = WL;
+  n.bL = bL;
+  n.Wf = Wf;
+  n.Wb = Wb;
+  n.w_stride = img;
   return n;
 }
 
@@ -162,27 +223,33 @@ __device__ __forceinline__ void layer_in(const Net<T, NB> &n, const T *x, int q,
     }
 }
 
+// acc[ob] += W_j (rows of block ob) x act, W given as an operand image.
+template <typename T, int NB>
+__device__ __forceinline__ void mma_layer(const T *img, int lane, const T (&b)[NB][4],
+                                          typename V4<T>::type (&acc)[NB]) {
+  typedef typename V4<T>::type v4;
+  const v4 *wp = reinterpret_cast<const v4 *>(img) + lane;
+#pragma unroll
+  for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) {
+      const v4 w = wp[(ob * NB + mb) * 64];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[ob] = mma16(w[i], b[mb][i], acc[ob]);
+    }
+}
+
 // Hidden layer j (MFMA): act <- tanh(W_j act + b_j).
 template <typename T, int NB>
 __device__ __forceinline__ void layer_hidden(const Net<T, NB> &n, int j, int lane,
                                              T (&act)[NB][4]) {
-  constexpr int HP = 16 * NB;
   typedef typename V4<T>::type v4;
   const int H = n.L.H, q = lane >> 4;
-  const T *wp = n.Wf + (int64_t)j * 2 * HP * HP + lane;
   v4 acc[NB];
 #pragma unroll
   for (int ob = 0; ob < NB; ++ob) acc[ob] = v4{0, 0, 0, 0};
-#pragma unroll
-  for (int mb = 0; mb < NB; ++mb)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const T b = act[mb][i];
-#pragma unroll
-      for (int ob = 0; ob < NB; ++ob)
-        acc[ob] = mma16(wp[((ob * NB + mb) * 4 + i) * 64], b, acc[ob]);
-    }
-  const T *bias = n.params + n.L.obh(j);
+  mma_layer<T, NB>(n.Wf + j * n.w_stride, lane, act, acc);
+  const T *bias = n.bias(j);
 #pragma unroll
   for (int ob = 0; ob < NB; ++ob)
 #pragma unroll
@@ -234,7 +301,7 @@ __device__ __forceinline__ T dfpow(const NNArgs<T> &a, const int32_t *F, T x) {
 // ---------------------------------------------------------------------------
 // k_forward
 // ---------------------------------------------------------------------------
-template <typename T, int NB>
+template <typename T, int NB, bool STAGE>
 __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
@@ -253,7 +320,11 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
   T *sd_lds = smem + K1_WAVES * 16 * SO + (w * 16 + row) * SO;
   T *red = smem + 2 * K1_WAVES * 16 * SO;
 
-  const Net<T, NB> net = make_net<T, NB>(a, inst);
+  Net<T, NB> net;
+  if constexpr (STAGE)
+    net = stage_net<T, NB>(a, inst, red + 512, false);
+  else
+    net = make_net<T, NB>(a, inst);
   T act[NB][4];
   for (int s = 0; s < n_slots; ++s) {
     T x[MAXD];
@@ -408,9 +479,9 @@ __global__ __launch_bounds__(256) void k_backward(NNArgs<T> a) {
     const int64_t r = r0 + trow;
     const bool valid = r < r_end;
     const int64_t rc = valid ? r : r_begin;
-    const int64_t lr = rc - row_base;
-    const int s = (int)(lr / n_entries);
-    const int e = (int)(lr - (int64_t)s * n_entries);
+    const int lr = (int)(rc - row_base);
+    const int s = lr / n_entries;
+    const int e = lr - s * n_entries;
     T x[MAXD], so[MAXO];
     const int64_t base = a.ints[slot_base_i + s];
 #pragma unroll
@@ -502,19 +573,10 @@ __global__ __launch_bounds__(256) void k_backward(NNArgs<T> a) {
       // delta_{j-1} = (W_jj^T delta_j) * (1 - act_{j-1}^2)
       T dn[NB][4];
       {
-        const T *wp = net.Wb + (int64_t)jj * 2 * HP * HP + lane;
         v4 c[NB];
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib) c[ib] = v4{0, 0, 0, 0};
-#pragma unroll
-        for (int mb = 0; mb < NB; ++mb)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const T b = dl_r[mb][i];
-#pragma unroll
-            for (int ib = 0; ib < NB; ++ib)
-              c[ib] = mma16(wp[((ib * NB + mb) * 4 + i) * 64], b, c[ib]);
-          }
+        mma_layer<T, NB>(net.Wb + jj * net.w_stride, lane, dl_r, c);
         const T *arow = ain + trow * LS;
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib)
@@ -556,20 +618,240 @@ __global__ __launch_bounds__(256) void k_backward(NNArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// k_backward_w: wavefront-private backward for narrow networks
+// ---------------------------------------------------------------------------
+// Each wavefront walks its own 16-row groups of the work item with no
+// workgroup barrier: activations and deltas stay in registers, the sums over
+// rows of the bias/first/last-layer gradients are butterfly shuffles over the
+// 16 lanes that hold different rows, and the hidden weight gradients are
+// MFMAs with K = the wavefront's 16 rows, after a transpose through a
+// wavefront-private LDS buffer.  All gradient accumulators are registers;
+// at the end the wavefronts are summed in a fixed order (deterministic).
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int HP>
+constexpr int wave_ls() { return HP + ((16 - HP % 64) + 64) % 64; }
+
+template <typename T, int NB, int NH>
+__global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
+  constexpr int HP = 16 * NB;
+  constexpr int LS = wave_ls<HP>();  // operand reads hit 64 distinct banks
+  typedef typename V4<T>::type v4;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  const int4 item = a.k2_items[blockIdx.x];
+  const int32_t *G = a.groups + item.x * TDX_GROUP_INTS;
+  const int n_entries = G[1];
+  const int slot_base_i = G[3], slot_off_r = G[4];
+  const int64_t row_base = G[7];
+  const int NW = blockDim.x >> 6;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = lane & 15, q = lane >> 4;
+  const int H = a.H, d_in = a.d_in, n_out = a.n_out, P = a.P;
+  T *dbuf = smem + w * 2 * 16 * LS;   // wavefront-private transpose buffers
+  T *abuf = dbuf + 16 * LS;
+  // wavefront-private sums of the bias, first- and last-layer gradients, in
+  // the order [W1][b1][bh][WL][bL]
+  const int n_small = H * d_in + H + NH * H + n_out * H + n_out;
+  const int TRW = max(NW * 2 * 16 * LS, P);  // transposes, then the final sum
+  T *wacc = smem + TRW + (int64_t)w * n_small;
+  T *stage = smem + TRW + (int64_t)NW * n_small;
+  const int oW1 = 0, ob1 = H * d_in, obh = ob1 + H, oWL = obh + NH * H, obL = oWL + n_out * H;
+
+  for (int k = lane; k < n_small; k += 64) wacc[k] = T(0);
+  const Net<T, NB> net = stage_net<T, NB>(a, item.w, stage, true);
+  const Layout<T> &L = net.L;
+  const T dl_scale = *a.dloss;
+
+  v4 dwacc[NH > 0 ? NH : 1][NB * NB];
+#pragma unroll
+  for (int j = 0; j < (NH > 0 ? NH : 1); ++j)
+#pragma unroll
+    for (int b = 0; b < NB * NB; ++b) dwacc[j][b] = v4{0, 0, 0, 0};
+
+  const int64_t r_begin = item.y, r_end = item.z;
+  for (int64_t g0 = r_begin + 16 * w; g0 < r_end; g0 += 16 * NW) {
+    const int64_t r = g0 + row;
+    const bool valid = r < r_end;
+    const int64_t rc = valid ? r : r_begin;
+    // (slot, entry) of the row: one division per wavefront iteration for
+    // its first row, then a bounded walk (rows are slot-major)
+    const int lr0 = (int)(g0 - row_base);
+    int s = lr0 / n_entries;
+    int e = lr0 - s * n_entries + (valid ? row : 0);
+    while (e >= n_entries) {
+      e -= n_entries;
+      ++s;
+    }
+    T x[MAXD];
+    const int64_t base = a.ints[slot_base_i + s];
+#pragma unroll
+    for (int k = 0; k < MAXD; ++k)
+      x[k] = (k < d_in) ? a.data[base + (int64_t)e * d_in + k] + a.reals[slot_off_r + s * d_in + k]
+                        : T(0);
+
+    // forward recompute, every activation kept
+    T act[NH + 1][NB][4];
+    layer_in<T, NB>(net, x, q, act[0]);
+#pragma unroll
+    for (int j = 1; j <= NH; ++j) {
+#pragma unroll
+      for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) act[j][mb][i] = act[j - 1][mb][i];
+      layer_hidden<T, NB>(net, j - 1, lane, act[j]);
+    }
+
+    // output layer gradients and the delta of the last hidden activation
+    T dl[NB][4];
+#pragma unroll
+    for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dl[mb][i] = T(0);
+    for (int o = 0; o < n_out; ++o) {
+      const T so = valid ? a.seeds[rc * n_out + o] * dl_scale : T(0);
+      const T sb = rowsum16(so);
+      if (lane == 0) wacc[obL + o] += sb;
+#pragma unroll
+      for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int f = 16 * mb + crow<T>(q, i);
+          const T sw = rowsum16(so * act[NH][mb][i]);
+          if (row == 0 && f < H) wacc[oWL + o * H + f] += sw;
+          if (f < H) dl[mb][i] = fma(net.WL[o * H + f], so, dl[mb][i]);
+        }
+    }
+#pragma unroll
+    for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const T av = act[NH][mb][i];
+        dl[mb][i] *= (T(1) - av * av);
+      }
+
+#pragma unroll
+    for (int j = NH; j >= 1; --j) {
+      const int jj = j - 1;
+#pragma unroll
+      for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int f = 16 * mb + crow<T>(q, i);
+          const T sb = rowsum16(dl[mb][i]);
+          if (row == 0 && f < H) wacc[obh + jj * H + f] += sb;
+          dbuf[row * LS + f] = dl[mb][i];
+          abuf[row * LS + f] = act[j - 1][mb][i];
+        }
+      wave_sync();
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob)
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) {
+          v4 c = dwacc[jj][ob * NB + ib];
+#pragma unroll
+          for (int kb = 0; kb < 4; ++kb)
+            c = mma16(dbuf[(4 * kb + q) * LS + 16 * ob + row], abuf[(4 * kb + q) * LS + 16 * ib + row], c);
+          dwacc[jj][ob * NB + ib] = c;
+        }
+      wave_sync();
+      // delta_{j-1} = (W_jj^T delta_j) * (1 - act_{j-1}^2)
+      v4 c[NB];
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib) c[ib] = v4{0, 0, 0, 0};
+      mma_layer<T, NB>(net.Wb + jj * net.w_stride, lane, dl, c);
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int f = 16 * ib + crow<T>(q, i);
+          const T av = act[j - 1][ib][i];
+          dl[ib][i] = (f < H) ? c[ib][i] * (T(1) - av * av) : T(0);
+        }
+    }
+    // first layer
+#pragma unroll
+    for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int f = 16 * mb + crow<T>(q, i);
+        const T sb = rowsum16(dl[mb][i]);
+        if (row == 0 && f < H) wacc[ob1 + f] += sb;
+        for (int c = 0; c < d_in; ++c) {
+          const T sw = rowsum16(dl[mb][i] * x[c]);
+          if (row == 0 && f < H) wacc[oW1 + f * d_in + c] += sw;
+        }
+      }
+  }
+  // fixed-order sum over the wavefronts into one gradient vector laid out
+  // like the parameters; it reuses the transpose buffers
+  __syncthreads();
+  T *cp = smem;
+  for (int ww = 0; ww < NW; ++ww) {
+    if (w == ww) {
+      const bool first = ww == 0;
+#pragma unroll
+      for (int jj = 0; jj < NH; ++jj)
+#pragma unroll
+        for (int ob = 0; ob < NB; ++ob)
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int out = 16 * ob + crow<T>(q, i), in = 16 * ib + row;
+              if (out < H && in < H) {
+                T &d = cp[L.oWh(jj) + out * H + in];
+                d = first ? dwacc[jj][ob * NB + ib][i] : d + dwacc[jj][ob * NB + ib][i];
+              }
+            }
+      for (int k = lane; k < n_small; k += 64) {
+        int pos;
+        if (k < ob1) pos = L.oW1() + k;
+        else if (k < obh) pos = L.ob1() + (k - ob1);
+        else if (k < oWL) pos = L.obh((k - obh) / H) + (k - obh) % H;
+        else if (k < obL) pos = L.oWL() + (k - oWL);
+        else pos = L.obL() + (k - obL);
+        cp[pos] = first ? wacc[k] : cp[pos] + wacc[k];
+      }
+    }
+    __syncthreads();
+  }
+  T *out = a.partial + (int64_t)blockIdx.x * P;
+  for (int k = threadIdx.x; k < P; k += blockDim.x) out[k] = cp[k];
+}
+
+// ---------------------------------------------------------------------------
 // reductions and the optimizer
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_loss(NNArgs<T> a, T *loss) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n_inst) return;
+__global__ __launch_bounds__(256) void k_loss(NNArgs<T> a, T *loss) {
+  __shared__ T red[512];
+  const int i = blockIdx.x;
   T s0 = T(0), s1 = T(0);
-  for (int k = a.inst_k1_ptr[i]; k < a.inst_k1_ptr[i + 1]; ++k) {
+  for (int k = a.inst_k1_ptr[i] + threadIdx.x; k < a.inst_k1_ptr[i + 1]; k += blockDim.x) {
     const int wg = a.inst_k1_idx[k];
     s0 += a.lp[2 * wg];
     s1 += a.lp[2 * wg + 1];
   }
-  loss[2 * i] = s0;
-  loss[2 * i + 1] = s1;
+  red[threadIdx.x] = s0;
+  red[256 + threadIdx.x] = s1;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      red[threadIdx.x] += red[threadIdx.x + st];
+      red[256 + threadIdx.x] += red[256 + threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    loss[2 * i] = red[0];
+    loss[2 * i + 1] = red[256];
+  }
 }
 
 struct AdamHyper {
@@ -644,18 +926,58 @@ __global__ void k_set_one(T *x) {
 // ===========================================================================
 using namespace tdx;
 
+static inline int wave_ls_host(int HP) { return HP + ((16 - HP % 64) + 64) % 64; }
+struct tdx_nn_plan;
+static size_t wave_private_lds(const tdx_nn_plan *p, const tdx_nn_desc *d);
+
 struct tdx_nn_plan {
   int dtype, d_in, n_out, H, n_layers, NH, NB, HP, P, n_inst;
   int n_groups;
   int64_t n_rows;
   int n_k1, n_k2;
   int k2_waves;
+  int k1_stage;
+  int k2_wave_private;
   int acc_lds;
   size_t k1_lds, k2_lds;
   DevTable groups, ints, reals, data, k1_items, k2_items, ik1p, ik1i, ik2p, ik2i;
   // workspace layout (byte offsets)
   size_t off_packed, off_seeds, off_lp, off_partial, off_one, ws_bytes;
+  // optional launch timing: event pairs per kernel kind
+  int prof_cap = 0;
+  int prof_n[TDX_NN_KERNEL_KINDS] = {0};
+  std::vector<hipEvent_t> prof_ev[TDX_NN_KERNEL_KINDS];
 };
+
+// LDS words of the wavefront-private backward: transpose buffers (reused
+// for the final sum, so at least P words), per-wavefront small-term sums and
+// the staged weights.
+static size_t wave_private_lds(const tdx_nn_plan *p, const tdx_nn_desc *d) {
+  const size_t tr = std::max<size_t>((size_t)4 * 2 * 16 * wave_ls_host(p->HP), (size_t)p->P);
+  const size_t small = (size_t)p->H * d->d_in + p->H + (size_t)p->NH * p->H +
+                       (size_t)d->n_out * p->H + d->n_out;
+  return tr + 4 * small + stage_words(p->NH, p->HP, p->H, d->d_in, d->n_out, true);
+}
+
+namespace {
+// Records an event pair around one launch when timing is on.
+struct ProfScope {
+  tdx_nn_plan *p;
+  int kind;
+  hipStream_t s;
+  hipEvent_t stop = nullptr;
+  ProfScope(const tdx_nn_plan *pc, int k, hipStream_t st) : p(const_cast<tdx_nn_plan *>(pc)), kind(k), s(st) {
+    if (p->prof_cap > 0 && p->prof_n[kind] < p->prof_cap) {
+      const int i = p->prof_n[kind]++;
+      (void)hipEventRecord(p->prof_ev[kind][2 * i], s);
+      stop = p->prof_ev[kind][2 * i + 1];
+    }
+  }
+  ~ProfScope() {
+    if (stop) (void)hipEventRecord(stop, s);
+  }
+};
+}  // namespace
 
 namespace {
 
@@ -706,25 +1028,63 @@ int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStre
   int rc = launch_pack<T, NB>(p, a, s);
   if (rc) return rc;
   a.resid = resid;
-  hipLaunchKernelGGL((k_forward<T, NB>), dim3(p->n_k1), dim3(256), p->k1_lds, s, a);
+  {
+    ProfScope ps(p, 0, s);
+    if (p->k1_stage)
+      hipLaunchKernelGGL((k_forward<T, NB, true>), dim3(p->n_k1), dim3(256), p->k1_lds, s, a);
+    else
+      hipLaunchKernelGGL((k_forward<T, NB, false>), dim3(p->n_k1), dim3(256), p->k1_lds, s, a);
+  }
   TDX_TRY_HIP(hipGetLastError());
   if (loss) {
-    hipLaunchKernelGGL((k_loss<T>), dim3((p->n_inst + 127) / 128), dim3(128), 0, s, a, loss);
+    hipLaunchKernelGGL((k_loss<T>), dim3(p->n_inst), dim3(256), 0, s, a, loss);
     TDX_TRY_HIP(hipGetLastError());
   }
   return TDX_OK;
 }
 
+template <typename T, int NB, int NH>
+int launch_backward_w(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void *)k_backward_w<T, NB, NH>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  ProfScope ps(p, 1, s);
+  hipLaunchKernelGGL((k_backward_w<T, NB, NH>), dim3(p->n_k2), dim3(64 * p->k2_waves), p->k2_lds,
+                     s, a);
+  TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}
+
 template <typename T, int NB>
 int launch_backward(const tdx_nn_plan *p, NNArgs<T> a, hipStream_t s) {
+  if (p->k2_wave_private) {
+    if constexpr (NB <= 2) {
+      switch (p->NH) {
+        case 0: return launch_backward_w<T, NB, 0>(p, a, s);
+        case 1: return launch_backward_w<T, NB, 1>(p, a, s);
+        case 2: return launch_backward_w<T, NB, 2>(p, a, s);
+        case 3: return launch_backward_w<T, NB, 3>(p, a, s);
+        case 4: return launch_backward_w<T, NB, 4>(p, a, s);
+        default: break;
+      }
+    }
+    set_error("no wave-private backward for this shape");
+    return TDX_EUNSUPPORTED;
+  }
   static bool attr_set = false;
   if (!attr_set) {
     (void)hipFuncSetAttribute((const void *)k_backward<T, NB>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_backward<T, NB>), dim3(p->n_k2), dim3(64 * p->k2_waves), p->k2_lds,
-                     s, a);
+  {
+    ProfScope ps(p, 1, s);
+    hipLaunchKernelGGL((k_backward<T, NB>), dim3(p->n_k2), dim3(64 * p->k2_waves), p->k2_lds,
+                       s, a);
+  }
   TDX_TRY_HIP(hipGetLastError());
   return TDX_OK;
 }
@@ -733,6 +1093,7 @@ template <typename T>
 int launch_reduce(const tdx_nn_plan *p, const NNArgs<T> &a, int mode, T *dparams, T *params,
                   T *m, T *v, const int64_t *step, AdamHyper h, hipStream_t s) {
   const int64_t tot = (int64_t)p->n_inst * p->P;
+  ProfScope ps(p, 2, s);
   hipLaunchKernelGGL((k_reduce<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, a,
                      mode, dparams, params, m, v, step, h);
   TDX_TRY_HIP(hipGetLastError());
@@ -883,6 +1244,12 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   for (int g = 0; g < d->n_groups; ++g)
     max_slots = std::max(max_slots, (int)d->groups[(int64_t)g * TDX_GROUP_INTS + 2]);
   p->k1_lds = (2 * K1_WAVES * 16 * max_slots * d->n_out + 512) * ts;
+  {
+    // stage the forward operand images in LDS when they are small
+    const size_t st = (size_t)stage_words(p->NH, p->HP, p->H, d->d_in, d->n_out, false) * ts;
+    p->k1_stage = (p->k1_lds + st <= 64 * 1024) ? 1 : 0;
+    if (p->k1_stage) p->k1_lds += st;
+  }
 
   // backward work: tiles of 16*waves rows, chunk_tiles tiles per workgroup
   const int LS = p->HP + 4;
@@ -905,8 +1272,17 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   p->k2_waves = waves;
   p->acc_lds = lds_of(waves, true) <= LDS_MAX ? 1 : 0;
   p->k2_lds = lds_of(waves, p->acc_lds != 0);
-  const int TR = 16 * waves;
-  const int chunk_tiles = d->chunk_tiles > 0 ? d->chunk_tiles : (p->acc_lds ? 16 : 4);
+  // narrow networks: wavefront-private backward with register accumulators
+  p->k2_wave_private = (p->NB <= 2 && p->NH <= 4 && d->tile_rows >= 0 &&
+                        wave_private_lds(p, d) * ts <= LDS_MAX) ? 1 : 0;
+  if (p->k2_wave_private) {
+    p->k2_waves = 4;
+    p->acc_lds = 1;
+    p->k2_lds = wave_private_lds(p, d) * ts;
+  }
+  const int TR = 16 * p->k2_waves;
+  const int chunk_tiles = d->chunk_tiles > 0 ? d->chunk_tiles
+                        : (p->k2_wave_private ? 256 : (p->acc_lds ? 16 : 4));
   std::vector<int4> k2;
   for (int g = 0; g < d->n_groups; ++g) {
     const int32_t *G = d->groups + (int64_t)g * TDX_GROUP_INTS;
@@ -960,8 +1336,43 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   return TDX_OK;
 }
 
+extern "C" int tdx_nn_profile(tdx_nn_plan *p, int capacity) {
+  TDX_REQUIRE(p && capacity >= 0, "bad argument");
+  for (int k = 0; k < TDX_NN_KERNEL_KINDS; ++k) {
+    for (hipEvent_t e : p->prof_ev[k]) (void)hipEventDestroy(e);
+    p->prof_ev[k].clear();
+    p->prof_n[k] = 0;
+  }
+  p->prof_cap = 0;
+  if (capacity == 0) return TDX_OK;
+  for (int k = 0; k < TDX_NN_KERNEL_KINDS; ++k) {
+    p->prof_ev[k].resize(2 * (size_t)capacity);
+    for (auto &e : p->prof_ev[k]) TDX_TRY_HIP(hipEventCreate(&e));
+  }
+  p->prof_cap = capacity;
+  return TDX_OK;
+}
+
+extern "C" int tdx_nn_profile_read(tdx_nn_plan *p, double *ms, int *count) {
+  TDX_REQUIRE(p && ms && count, "null argument");
+  for (int k = 0; k < TDX_NN_KERNEL_KINDS; ++k) {
+    double sum = 0.0;
+    for (int i = 0; i < p->prof_n[k]; ++i) {
+      TDX_TRY_HIP(hipEventSynchronize(p->prof_ev[k][2 * i + 1]));
+      float e = 0.f;
+      TDX_TRY_HIP(hipEventElapsedTime(&e, p->prof_ev[k][2 * i], p->prof_ev[k][2 * i + 1]));
+      sum += e;
+    }
+    ms[k] = sum;
+    count[k] = p->prof_n[k];
+    p->prof_n[k] = 0;
+  }
+  return TDX_OK;
+}
+
 extern "C" void tdx_nn_plan_destroy(tdx_nn_plan *p) {
   if (!p) return;
+  (void)tdx_nn_profile(p, 0);
   DevTable *t[] = {&p->groups, &p->ints, &p->reals, &p->data, &p->k1_items,
                    &p->k2_items, &p->ik1p, &p->ik1i, &p->ik2p, &p->ik2i};
   for (DevTable *x : t) x->release();

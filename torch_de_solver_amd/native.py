@@ -25,7 +25,7 @@ EXPORTS = [
     'tdx_version', 'tdx_last_error',
     'tdx_nn_plan_create', 'tdx_nn_plan_destroy', 'tdx_nn_param_count',
     'tdx_nn_workspace_bytes', 'tdx_nn_forward', 'tdx_nn_backward',
-    'tdx_nn_train_step',
+    'tdx_nn_train_step', 'tdx_nn_profile', 'tdx_nn_profile_read',
     'tdx_mat_plan_create', 'tdx_mat_plan_destroy', 'tdx_mat_workspace_bytes',
     'tdx_mat_forward', 'tdx_mat_backward', 'tdx_mat_train_step',
     'tdx_adam', 'tdx_step_increment',
@@ -91,6 +91,9 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.tdx_nn_forward.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.tdx_nn_backward.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.tdx_nn_train_step.argtypes = [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]
+    lib.tdx_nn_profile.argtypes = [vp, ctypes.c_int]
+    lib.tdx_nn_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int)]
     lib.tdx_mat_plan_create.argtypes = [ctypes.POINTER(MatDesc), ctypes.POINTER(vp)]
     lib.tdx_mat_plan_destroy.argtypes = [vp]
     lib.tdx_mat_plan_destroy.restype = None

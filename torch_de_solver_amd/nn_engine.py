@@ -248,6 +248,18 @@ class NNEngine:
             native.ptr(loss) if loss is not None else None,
             native.ptr(self.workspace), native.stream_ptr()))
 
+    def profile(self, capacity: int) -> None:
+        """Time up to ``capacity`` launches per kernel kind with HIP events
+        (0 turns timing off)."""
+        native.check(native.lib().tdx_nn_profile(self._plan, int(capacity)))
+
+    def profile_read(self):
+        """``{kind: (total_ms, launches)}`` for 'forward', 'backward', 'update'."""
+        ms = (ctypes.c_double * 3)()
+        cnt = (ctypes.c_int * 3)()
+        native.check(native.lib().tdx_nn_profile_read(self._plan, ms, cnt))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(('forward', 'backward', 'update'))}
+
     def _check_params(self, params: torch.Tensor) -> None:
         if params.device.type != 'cuda' or params.dtype != self.dtype or not params.is_contiguous():
             raise ValueError("params must be a contiguous device tensor of the problem dtype")

@@ -40,10 +40,10 @@ def point_types(grid: np.ndarray) -> np.ndarray:
         p[:, a] = g[:, a] - SHIFT
         minus_in[:, a] = _inside_box(p, lo, hi)
     central = np.all(plus_in & minus_in, axis=1)
-    out = np.empty(n, dtype=object)
+    out = np.full(n, 'central', dtype=object)
     chars = np.where(plus_in, 'f', 'b')
-    for i in range(n):
-        out[i] = 'central' if central[i] else ''.join(chars[i])
+    for i in np.nonzero(~central)[0]:
+        out[i] = ''.join(chars[i])
     return out
 
 
