@@ -40,7 +40,7 @@ namespace tdx {
 
 constexpr int MAXD = 4;   // network inputs
 constexpr int MAXO = 4;   // network outputs
-constexpr int MAXF = 8;   // factors per term
+constexpr int MAXF = 4;   // factors per term
 constexpr int K1_WAVES = 4;
 
 template <typename T>
@@ -216,7 +216,9 @@ __device__ __forceinline__ void layer_in(const Net<T, NB> &n, const T *x, int q,
       T z = T(0);
       if (f < H) {
         z = n.b1[f];
-        for (int k = 0; k < d; ++k) z = fma(x[k], n.W1[f * d + k], z);
+#pragma unroll
+        for (int k = 0; k < MAXD; ++k)
+          if (k < d) z = fma(x[k], n.W1[f * d + k], z);
         z = dtanh(z);
       }
       act[mb][i] = z;
@@ -374,14 +376,16 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
         lsum += lam_d * sq;
         lnsum += a.reals[E[5]] * sq;
         const T dr = T(2) * lam_d * diff;
-        // seeds: d r / d u(slot, var), through every factor of every term
+        // seeds: d r / d u(slot, var), through every factor of every term;
+        // d(prod_j f_j^p_j)/d f_k = p_k f_k^(p_k-1) * prefix_k * suffix_k
         for (int t = 0; t < n_terms; ++t) {
           const int32_t *Tm = a.ints + term_i + t * TDX_TERM_INTS;
           const int nf = Tm[0];
-          T gsv[MAXF];
+          T gsv[MAXF], pw[MAXF], pre[MAXF + 1], suf[MAXF + 1];
 #pragma unroll
           for (int f = 0; f < MAXF; ++f) {
             gsv[f] = T(0);
+            pw[f] = T(1);
             if (f < nf) {
               const int32_t *F = a.ints + Tm[1] + f * TDX_FACTOR_INTS;
               T gs = T(0);
@@ -390,18 +394,21 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
                 gs = gs + u_lds[S[0] * a.n_out + F[0]] * a.reals[S[1]];
               }
               gsv[f] = gs;
+              pw[f] = fpow(a, F, gs);
             }
           }
+          pre[0] = T(1);
+#pragma unroll
+          for (int f = 0; f < MAXF; ++f) pre[f + 1] = pre[f] * pw[f];
+          suf[MAXF] = T(1);
+#pragma unroll
+          for (int f = MAXF - 1; f >= 0; --f) suf[f] = suf[f + 1] * pw[f];
           const T cdr = coeff_of(a, Tm, e) * dr;
 #pragma unroll
           for (int f = 0; f < MAXF; ++f) {
             if (f < nf) {
               const int32_t *F = a.ints + Tm[1] + f * TDX_FACTOR_INTS;
-              T g = cdr * dfpow(a, F, gsv[f]);
-#pragma unroll
-              for (int f2 = 0; f2 < MAXF; ++f2)
-                if (f2 < nf && f2 != f)
-                  g *= fpow(a, a.ints + Tm[1] + f2 * TDX_FACTOR_INTS, gsv[f2]);
+              const T g = cdr * dfpow(a, F, gsv[f]) * pre[f] * suf[f + 1];
               for (int k = 0; k < F[3]; ++k) {
                 const int32_t *S = a.ints + F[4] + k * TDX_STENCIL_INTS;
                 sd_lds[S[0] * a.n_out + F[0]] += g * a.reals[S[1]];
@@ -634,13 +641,18 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Wavefront-private transpose buffers are feature-major, [feature][row],
+// with a row stride of 20 words: the per-lane writes (16 rows x 4 quarters)
+// and the 128-bit operand reads (4 consecutive row positions per lane) both
+// touch 64 distinct banks.
+constexpr int WRS = 20;
 template <int HP>
-constexpr int wave_ls() { return HP + ((16 - HP % 64) + 64) % 64; }
+constexpr int wave_ls() { return WRS * HP / 16; }
 
 template <typename T, int NB, int NH>
 __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
   constexpr int HP = 16 * NB;
-  constexpr int LS = wave_ls<HP>();  // operand reads hit 64 distinct banks
+  constexpr int LS = wave_ls<HP>();  // 16 rows of one buffer, in words
   typedef typename V4<T>::type v4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
@@ -745,20 +757,29 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
           const int f = 16 * mb + crow<T>(q, i);
           const T sb = rowsum16(dl[mb][i]);
           if (row == 0 && f < H) wacc[obh + jj * H + f] += sb;
-          dbuf[row * LS + f] = dl[mb][i];
-          abuf[row * LS + f] = act[j - 1][mb][i];
+          dbuf[f * WRS + row] = dl[mb][i];
+          abuf[f * WRS + row] = act[j - 1][mb][i];
         }
       wave_sync();
+      // dW block (ob, ib) += delta^T act: k-step kb of lane quarter q takes
+      // row position 4q + kb (the same row order for both operands)
+      {
+        v4 av[NB], dv[NB];
 #pragma unroll
-      for (int ob = 0; ob < NB; ++ob)
-#pragma unroll
-        for (int ib = 0; ib < NB; ++ib) {
-          v4 c = dwacc[jj][ob * NB + ib];
-#pragma unroll
-          for (int kb = 0; kb < 4; ++kb)
-            c = mma16(dbuf[(4 * kb + q) * LS + 16 * ob + row], abuf[(4 * kb + q) * LS + 16 * ib + row], c);
-          dwacc[jj][ob * NB + ib] = c;
+        for (int xb = 0; xb < NB; ++xb) {
+          dv[xb] = *reinterpret_cast<const v4 *>(dbuf + (16 * xb + row) * WRS + 4 * q);
+          av[xb] = *reinterpret_cast<const v4 *>(abuf + (16 * xb + row) * WRS + 4 * q);
         }
+#pragma unroll
+        for (int ob = 0; ob < NB; ++ob)
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib) {
+            v4 c = dwacc[jj][ob * NB + ib];
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) c = mma16(dv[ob][kb], av[ib][kb], c);
+            dwacc[jj][ob * NB + ib] = c;
+          }
+      }
       wave_sync();
       // delta_{j-1} = (W_jj^T delta_j) * (1 - act_{j-1}^2)
       v4 c[NB];
@@ -782,9 +803,12 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
         const int f = 16 * mb + crow<T>(q, i);
         const T sb = rowsum16(dl[mb][i]);
         if (row == 0 && f < H) wacc[ob1 + f] += sb;
-        for (int c = 0; c < d_in; ++c) {
-          const T sw = rowsum16(dl[mb][i] * x[c]);
-          if (row == 0 && f < H) wacc[oW1 + f * d_in + c] += sw;
+#pragma unroll
+        for (int c = 0; c < MAXD; ++c) {
+          if (c < d_in) {
+            const T sw = rowsum16(dl[mb][i] * x[c]);
+            if (row == 0 && f < H) wacc[oW1 + f * d_in + c] += sw;
+          }
         }
       }
   }
@@ -926,7 +950,7 @@ __global__ void k_set_one(T *x) {
 // ===========================================================================
 using namespace tdx;
 
-static inline int wave_ls_host(int HP) { return HP + ((16 - HP % 64) + 64) % 64; }
+static inline int wave_ls_host(int HP) { return 20 * HP / 16; }
 struct tdx_nn_plan;
 static size_t wave_private_lds(const tdx_nn_plan *p, const tdx_nn_desc *d);
 
