@@ -652,7 +652,7 @@ constexpr int wave_ls() { return WRS * HP / 16; }
 template <typename T, int NB, int NH>
 __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
   constexpr int HP = 16 * NB;
-  constexpr int LS = wave_ls<HP>();  // 16 rows of one buffer, in words
+  constexpr int FB = WRS * HP;  // words of one feature-major buffer
   typedef typename V4<T>::type v4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
@@ -665,26 +665,39 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row = lane & 15, q = lane >> 4;
   const int H = a.H, d_in = a.d_in, n_out = a.n_out, P = a.P;
-  T *dbuf = smem + w * 2 * 16 * LS;   // wavefront-private transpose buffers
-  T *abuf = dbuf + 16 * LS;
-  // wavefront-private sums of the bias, first- and last-layer gradients, in
-  // the order [W1][b1][bh][WL][bL]
-  const int n_small = H * d_in + H + NH * H + n_out * H + n_out;
-  const int TRW = max(NW * 2 * 16 * LS, P);  // transposes, then the final sum
-  T *wacc = smem + TRW + (int64_t)w * n_small;
-  T *stage = smem + TRW + (int64_t)NW * n_small;
-  const int oW1 = 0, ob1 = H * d_in, obh = ob1 + H, oWL = obh + NH * H, obL = oWL + n_out * H;
+  // wavefront-private feature-major buffers: deltas, activations, and the
+  // seeds/coordinates of the 16 rows ([column][row], stride WRS)
+  const int WB = 2 * FB + 2 * 16 * WRS;
+  T *dbuf = smem + w * WB;
+  T *abuf = dbuf + FB;
+  T *sbuf = abuf + FB;          // seeds, one column per output
+  T *xbuf = sbuf + 16 * WRS;    // coordinates, one column per input
+  const int TRW = max(NW * WB, P);  // buffers, then the final sum
+  T *stage = smem + TRW;
 
-  for (int k = lane; k < n_small; k += 64) wacc[k] = T(0);
   const Net<T, NB> net = stage_net<T, NB>(a, item.w, stage, true);
   const Layout<T> &L = net.L;
   const T dl_scale = *a.dloss;
 
+  // gradient accumulators (MFMA tiles; row m = crow(q, i) is the hidden
+  // feature 16 ob + m, column n = row):
+  //   dwacc[jj][ob*NB+ib]: hidden W ( out, in )
+  //   sacc[ob]: every small term in one tile, by column -- n < d_in first W,
+  //     d_in first bias, d_in + 1 + jj hidden bias jj, d_in + 1 + NH + o
+  //     output W row o.  Each product accumulated into the tile has a B
+  //     operand that is zero outside its own columns, so the columns add
+  //     exactly what separate tiles would.
+  //   blacc: output bias o = q, summed over the row lanes at the end
   v4 dwacc[NH > 0 ? NH : 1][NB * NB];
+  v4 sacc[NB];
 #pragma unroll
   for (int j = 0; j < (NH > 0 ? NH : 1); ++j)
 #pragma unroll
     for (int b = 0; b < NB * NB; ++b) dwacc[j][b] = v4{0, 0, 0, 0};
+#pragma unroll
+  for (int b = 0; b < NB; ++b) sacc[b] = v4{0, 0, 0, 0};
+  T blacc = T(0);
+  const int col_b1 = d_in, col_bh = d_in + 1, col_wl = d_in + 1 + NH;
 
   const int64_t r_begin = item.y, r_end = item.z;
   for (int64_t g0 = r_begin + 16 * w; g0 < r_end; g0 += 16 * NW) {
@@ -706,6 +719,11 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
     for (int k = 0; k < MAXD; ++k)
       x[k] = (k < d_in) ? a.data[base + (int64_t)e * d_in + k] + a.reals[slot_off_r + s * d_in + k]
                         : T(0);
+    if (q == 0) {
+#pragma unroll
+      for (int k = 0; k < MAXD; ++k)
+        if (k < d_in) xbuf[k * WRS + row] = x[k];
+    }
 
     // forward recompute, every activation kept
     T act[NH + 1][NB][4];
@@ -719,25 +737,26 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
       layer_hidden<T, NB>(net, j - 1, lane, act[j]);
     }
 
-    // output layer gradients and the delta of the last hidden activation
+    // seeds; delta of the last hidden activation
     T dl[NB][4];
 #pragma unroll
     for (int mb = 0; mb < NB; ++mb)
 #pragma unroll
       for (int i = 0; i < 4; ++i) dl[mb][i] = T(0);
-    for (int o = 0; o < n_out; ++o) {
-      const T so = valid ? a.seeds[rc * n_out + o] * dl_scale : T(0);
-      const T sb = rowsum16(so);
-      if (lane == 0) wacc[obL + o] += sb;
 #pragma unroll
-      for (int mb = 0; mb < NB; ++mb)
+    for (int o = 0; o < MAXO; ++o) {
+      if (o < n_out) {
+        const T so = valid ? a.seeds[rc * n_out + o] * dl_scale : T(0);
+        if (o == q) blacc += so;
+        if (q == 0) sbuf[o * WRS + row] = so;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int f = 16 * mb + crow<T>(q, i);
-          const T sw = rowsum16(so * act[NH][mb][i]);
-          if (row == 0 && f < H) wacc[oWL + o * H + f] += sw;
-          if (f < H) dl[mb][i] = fma(net.WL[o * H + f], so, dl[mb][i]);
-        }
+        for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int f = 16 * mb + crow<T>(q, i);
+            if (f < H) dl[mb][i] = fma(net.WL[o * H + f], so, dl[mb][i]);
+          }
+      }
     }
 #pragma unroll
     for (int mb = 0; mb < NB; ++mb)
@@ -745,7 +764,23 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
       for (int i = 0; i < 4; ++i) {
         const T av = act[NH][mb][i];
         dl[mb][i] *= (T(1) - av * av);
+        abuf[(16 * mb + crow<T>(q, i)) * WRS + row] = av;
       }
+    wave_sync();
+    // output layer: dW_L[o][f] = sum_rows s_o act_NH[f]; k-step kb of quarter
+    // q takes row position 4q + kb in both operands
+    {
+      const int o = row - col_wl;
+      const v4 sv = (o >= 0 && o < n_out) ? *reinterpret_cast<const v4 *>(sbuf + o * WRS + 4 * q)
+                                          : v4{0, 0, 0, 0};
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob) {
+        const v4 av = *reinterpret_cast<const v4 *>(abuf + (16 * ob + row) * WRS + 4 * q);
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) sacc[ob] = mma16(av[kb], sv[kb], sacc[ob]);
+      }
+    }
+    wave_sync();
 
 #pragma unroll
     for (int j = NH; j >= 1; --j) {
@@ -755,14 +790,10 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int f = 16 * mb + crow<T>(q, i);
-          const T sb = rowsum16(dl[mb][i]);
-          if (row == 0 && f < H) wacc[obh + jj * H + f] += sb;
           dbuf[f * WRS + row] = dl[mb][i];
           abuf[f * WRS + row] = act[j - 1][mb][i];
         }
       wave_sync();
-      // dW block (ob, ib) += delta^T act: k-step kb of lane quarter q takes
-      // row position 4q + kb (the same row order for both operands)
       {
         v4 av[NB], dv[NB];
 #pragma unroll
@@ -770,8 +801,11 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
           dv[xb] = *reinterpret_cast<const v4 *>(dbuf + (16 * xb + row) * WRS + 4 * q);
           av[xb] = *reinterpret_cast<const v4 *>(abuf + (16 * xb + row) * WRS + 4 * q);
         }
+        const T one = row == col_bh + jj ? T(1) : T(0);
 #pragma unroll
-        for (int ob = 0; ob < NB; ++ob)
+        for (int ob = 0; ob < NB; ++ob) {
+#pragma unroll
+          for (int kb = 0; kb < 4; ++kb) sacc[ob] = mma16(dv[ob][kb], one, sacc[ob]);
 #pragma unroll
           for (int ib = 0; ib < NB; ++ib) {
             v4 c = dwacc[jj][ob * NB + ib];
@@ -779,6 +813,7 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
             for (int kb = 0; kb < 4; ++kb) c = mma16(dv[ob][kb], av[ib][kb], c);
             dwacc[jj][ob * NB + ib] = c;
           }
+        }
       }
       wave_sync();
       // delta_{j-1} = (W_jj^T delta_j) * (1 - act_{j-1}^2)
@@ -795,53 +830,54 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
           dl[ib][i] = (f < H) ? c[ib][i] * (T(1) - av * av) : T(0);
         }
     }
-    // first layer
+    // first layer: dW_1[f][c] = sum_rows delta_0[f] x_c, bias in column d_in
 #pragma unroll
     for (int mb = 0; mb < NB; ++mb)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int f = 16 * mb + crow<T>(q, i);
-        const T sb = rowsum16(dl[mb][i]);
-        if (row == 0 && f < H) wacc[ob1 + f] += sb;
+      for (int i = 0; i < 4; ++i) dbuf[(16 * mb + crow<T>(q, i)) * WRS + row] = dl[mb][i];
+    wave_sync();
+    {
+      const v4 xv = row < d_in ? *reinterpret_cast<const v4 *>(xbuf + row * WRS + 4 * q)
+                               : (row == col_b1 ? v4{1, 1, 1, 1} : v4{0, 0, 0, 0});
 #pragma unroll
-        for (int c = 0; c < MAXD; ++c) {
-          if (c < d_in) {
-            const T sw = rowsum16(dl[mb][i] * x[c]);
-            if (row == 0 && f < H) wacc[oW1 + f * d_in + c] += sw;
-          }
-        }
+      for (int ob = 0; ob < NB; ++ob) {
+        const v4 dv = *reinterpret_cast<const v4 *>(dbuf + (16 * ob + row) * WRS + 4 * q);
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) sacc[ob] = mma16(dv[kb], xv[kb], sacc[ob]);
       }
+    }
+    wave_sync();
   }
+
   // fixed-order sum over the wavefronts into one gradient vector laid out
-  // like the parameters; it reuses the transpose buffers
+  // like the parameters; it reuses the wavefront buffers
   __syncthreads();
   T *cp = smem;
   for (int ww = 0; ww < NW; ++ww) {
     if (w == ww) {
       const bool first = ww == 0;
+      auto put = [&](int pos, T v) { cp[pos] = first ? v : cp[pos] + v; };
 #pragma unroll
-      for (int jj = 0; jj < NH; ++jj)
+      for (int ob = 0; ob < NB; ++ob)
 #pragma unroll
-        for (int ob = 0; ob < NB; ++ob)
+        for (int i = 0; i < 4; ++i) {
+          const int out = 16 * ob + crow<T>(q, i);
+          if (out >= H) continue;
 #pragma unroll
-          for (int ib = 0; ib < NB; ++ib)
+          for (int jj = 0; jj < NH; ++jj)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const int out = 16 * ob + crow<T>(q, i), in = 16 * ib + row;
-              if (out < H && in < H) {
-                T &d = cp[L.oWh(jj) + out * H + in];
-                d = first ? dwacc[jj][ob * NB + ib][i] : d + dwacc[jj][ob * NB + ib][i];
-              }
+            for (int ib = 0; ib < NB; ++ib) {
+              const int in = 16 * ib + row;
+              if (in < H) put(L.oWh(jj) + out * H + in, dwacc[jj][ob * NB + ib][i]);
             }
-      for (int k = lane; k < n_small; k += 64) {
-        int pos;
-        if (k < ob1) pos = L.oW1() + k;
-        else if (k < obh) pos = L.ob1() + (k - ob1);
-        else if (k < oWL) pos = L.obh((k - obh) / H) + (k - obh) % H;
-        else if (k < obL) pos = L.oWL() + (k - oWL);
-        else pos = L.obL() + (k - obL);
-        cp[pos] = first ? wacc[k] : cp[pos] + wacc[k];
-      }
+          const T sv = sacc[ob][i];
+          if (row < d_in) put(L.oW1() + out * d_in + row, sv);
+          else if (row == col_b1) put(L.ob1() + out, sv);
+          else if (row < col_wl) put(L.obh(row - col_bh) + out, sv);
+          else if (row < col_wl + n_out) put(L.oWL() + (row - col_wl) * H + out, sv);
+        }
+      const T bl = rowsum16(blacc);
+      if (row == 0 && q < n_out) put(L.obL() + q, bl);
     }
     __syncthreads();
   }
@@ -977,10 +1013,9 @@ struct tdx_nn_plan {
 // for the final sum, so at least P words), per-wavefront small-term sums and
 // the staged weights.
 static size_t wave_private_lds(const tdx_nn_plan *p, const tdx_nn_desc *d) {
-  const size_t tr = std::max<size_t>((size_t)4 * 2 * 16 * wave_ls_host(p->HP), (size_t)p->P);
-  const size_t small = (size_t)p->H * d->d_in + p->H + (size_t)p->NH * p->H +
-                       (size_t)d->n_out * p->H + d->n_out;
-  return tr + 4 * small + stage_words(p->NH, p->HP, p->H, d->d_in, d->n_out, true);
+  const size_t wb = (size_t)2 * 20 * p->HP + 2 * 16 * 20;
+  const size_t tr = std::max<size_t>(4 * wb, (size_t)p->P);
+  return tr + stage_words(p->NH, p->HP, p->H, d->d_in, d->n_out, true);
 }
 
 namespace {
@@ -1298,6 +1333,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   p->k2_lds = lds_of(waves, p->acc_lds != 0);
   // narrow networks: wavefront-private backward with register accumulators
   p->k2_wave_private = (p->NB <= 2 && p->NH <= 4 && d->tile_rows >= 0 &&
+                        d->d_in + 1 + p->NH + d->n_out <= 16 &&
                         wave_private_lds(p, d) * ts <= LDS_MAX) ? 1 : 0;
   if (p->k2_wave_private) {
     p->k2_waves = 4;
