@@ -60,7 +60,7 @@ struct NNArgs {
   int n_inst;
   // buffers
   const T *params;            // n_inst * P
-  T *packed;                  // n_inst * NH * 2 * HP*HP
+  T *packed;                  // n_inst packed network images
   T *seeds;                   // n_rows * n_out
   T *lp;                      // 2 per forward workgroup
   T *partial;                 // P per backward workgroup
@@ -83,146 +83,166 @@ struct Layout {
 };
 
 // ---------------------------------------------------------------------------
-// weight packing
+// packed network image
 // ---------------------------------------------------------------------------
-// Per hidden layer j, two operand images of W_j (HP*HP values each), laid
-// out so that lane l reads the four values of k-steps i = 0..3 of block
-// (xb, mb) with one 128-bit load:
+// One image per instance, in the workspace, rebuilt from the parameters by
+// k_pack at the start of every forward pass (HP = 16 NB):
+//   [Wf: NH x HP*HP][small: SW][Wb: NH x HP*HP]
+// Wf and Wb hold each hidden layer's W_j as MFMA operand images, laid out so
+// that lane l reads the four values of k-steps i = 0..3 of block (xb, mb)
+// with one 128-bit load:
 //   Wf[j][ob][mb][lane][i] = W_j[16*ob + lane%16][16*mb + crow(lane/16, i)]
 //   Wb[j][ib][mb][lane][i] = W_j[16*mb + crow(lane/16, i)][16*ib + lane%16]
-// Global image of instance n: [n][j][which][xb][mb][lane][i].
+// The small arrays are indexed by accumulator position p = 16 mb + 4 q + i,
+// which holds hidden feature f = 16 mb + crow(q, i), zero for f >= H:
+//   b1[HP] | W1[d_in][HP] | bh[NH][HP] | WL[n_out][HP] | bL[n_out, padded to 4]
+// so a lane's four features of one block are one 128-bit load, and padding
+// features evaluate to tanh(0) = 0 without a guard.  The forward kernel
+// stages the prefix without Wb in LDS, the backward kernel all of it.
+__host__ __device__ inline int small_words(int HP, int NH, int d_in, int n_out) {
+  return HP * (1 + d_in + NH + n_out) + ((n_out + 3) & ~3);
+}
+__host__ __device__ inline int image_words(int HP, int NH, int d_in, int n_out) {
+  return 2 * NH * HP * HP + small_words(HP, NH, d_in, n_out);
+}
+// Words of the LDS copy (in T units).
+__host__ __device__ inline int stage_words(int HP, int NH, int d_in, int n_out, bool with_wb) {
+  return NH * HP * HP * (with_wb ? 2 : 1) + small_words(HP, NH, d_in, n_out);
+}
+
+// Hidden feature held at accumulator position p of a block.
+template <typename T>
+__device__ __forceinline__ int feature_at(int p) {
+  return (p & ~15) + crow<T>((p & 15) >> 2, p & 3);
+}
+
 template <typename T, int NB>
 __global__ void k_pack(NNArgs<T> a) {
   constexpr int HP = 16 * NB;
-  const int64_t per_layer = (int64_t)HP * HP;
-  const int64_t total = (int64_t)a.n_inst * a.NH * 2 * per_layer;
-  Layout<T> L{a.d_in, a.n_out, a.H, a.NH};
+  const int NH = a.NH, H = a.H, d = a.d_in, no = a.n_out;
+  const int img = HP * HP;
+  const int SW = small_words(HP, NH, d, no);
+  const int IMG = image_words(HP, NH, d, no);
+  const int64_t total = (int64_t)a.n_inst * IMG;
+  const Layout<T> L{d, no, H, NH};
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r = t;
-    const int i = r % 4; r /= 4;
-    const int lane = r % 64; r /= 64;
-    const int mb = r % NB; r /= NB;
-    const int xb = r % NB; r /= NB;
-    const int which = r % 2; r /= 2;
-    const int j = r % a.NH; r /= a.NH;
-    const int inst = (int)r;
-    const T *W = a.params + (int64_t)inst * a.P + L.oWh(j);
-    int out, in;
-    if (which == 0) {
-      out = 16 * xb + (lane & 15);
-      in = 16 * mb + crow<T>(lane >> 4, i);
+    const int inst = (int)(t / IMG);
+    int off = (int)(t - (int64_t)inst * IMG);
+    const T *W = a.params + (int64_t)inst * a.P;
+    T v = T(0);
+    if (off < NH * img || off >= NH * img + SW) {
+      const int which = off < NH * img ? 0 : 1;
+      if (which) off -= NH * img + SW;
+      const int j = off / img;
+      int r = off - j * img;
+      const int i = r & 3;
+      const int lane = (r >> 2) & 63;
+      r >>= 8;
+      const int mb = r % NB, xb = r / NB;
+      int out, in;
+      if (which == 0) {
+        out = 16 * xb + (lane & 15);
+        in = 16 * mb + crow<T>(lane >> 4, i);
+      } else {
+        out = 16 * mb + crow<T>(lane >> 4, i);
+        in = 16 * xb + (lane & 15);
+      }
+      if (out < H && in < H) v = W[L.oWh(j) + out * H + in];
     } else {
-      out = 16 * mb + crow<T>(lane >> 4, i);
-      in = 16 * xb + (lane & 15);
+      const int sp = off - NH * img;
+      const int blk = sp / HP, f = feature_at<T>(sp - blk * HP);
+      if (blk == 0) {
+        if (f < H) v = W[L.ob1() + f];
+      } else if (blk < 1 + d) {
+        if (f < H) v = W[L.oW1() + f * d + (blk - 1)];
+      } else if (blk < 1 + d + NH) {
+        if (f < H) v = W[L.obh(blk - 1 - d) + f];
+      } else if (blk < 1 + d + NH + no) {
+        if (f < H) v = W[L.oWL() + (blk - 1 - d - NH) * H + f];
+      } else {
+        const int o = sp - HP * (1 + d + NH + no);
+        if (o < no) v = W[L.obL() + o];
+      }
     }
-    T v = (out < a.H && in < a.H) ? W[out * a.H + in] : T(0);
     a.packed[t] = v;
   }
 }
 
 // ---------------------------------------------------------------------------
-// network view: weights in global memory or staged in LDS
+// network view: the packed image in global memory or its copy in LDS
 // ---------------------------------------------------------------------------
-// The staged image of one instance (in LDS) is
-//   [Wf: NH*HP^2][Wb: NH*HP^2 (backward only)][W1: H*d_in][b1: H]
-//   [bh: NH*H][WL: n_out*H][bL: n_out]
 template <typename T, int NB>
 struct Net {
-  const T *W1, *b1, *bh, *WL, *bL;
-  int bh_stride;            // distance between consecutive hidden biases
-  const T *Wf, *Wb;
-  int w_stride;             // distance between consecutive layers' images
+  const T *Wf, *Wb;          // layer j image at Wf + j*HP*HP
+  const T *b1, *W1, *bh, *WL, *bL;  // small arrays, row stride HP
   Layout<T> L;
-  __device__ __forceinline__ const T *bias(int j) const { return bh + j * bh_stride; }
 };
+
+template <typename T, int NB>
+__device__ __forceinline__ Net<T, NB> net_at(const NNArgs<T> &a, const T *base) {
+  constexpr int HP = 16 * NB;
+  Net<T, NB> n;
+  n.L = Layout<T>{a.d_in, a.n_out, a.H, a.NH};
+  n.Wf = base;
+  const T *sm = base + a.NH * HP * HP;
+  n.b1 = sm;
+  n.W1 = sm + HP;
+  n.bh = n.W1 + a.d_in * HP;
+  n.WL = n.bh + a.NH * HP;
+  n.bL = n.WL + a.n_out * HP;
+  n.Wb = sm + small_words(HP, a.NH, a.d_in, a.n_out);
+  return n;
+}
 
 template <typename T, int NB>
 __device__ __forceinline__ Net<T, NB> make_net(const NNArgs<T> &a, int inst) {
   constexpr int HP = 16 * NB;
-  Net<T, NB> n;
-  n.L = Layout<T>{a.d_in, a.n_out, a.H, a.NH};
-  const T *params = a.params + (int64_t)inst * a.P;
-  n.W1 = params + n.L.oW1();
-  n.b1 = params + n.L.ob1();
-  n.bh = params + n.L.obh(0);
-  n.bh_stride = a.H * a.H + a.H;
-  n.WL = params + n.L.oWL();
-  n.bL = params + n.L.obL();
-  n.Wf = a.packed + (int64_t)inst * a.NH * 2 * HP * HP;
-  n.Wb = n.Wf + (int64_t)HP * HP;
-  n.w_stride = 2 * HP * HP;
-  return n;
+  return net_at<T, NB>(a, a.packed + (int64_t)inst * image_words(HP, a.NH, a.d_in, a.n_out));
 }
 
-// Values of the staged image (LDS words, in T units).
-__host__ __device__ inline int stage_words(int NH, int HP, int H, int d_in, int n_out, bool with_wb) {
-  return NH * HP * HP * (with_wb ? 2 : 1) + H * d_in + H + NH * H + n_out * H + n_out;
-}
-
-// Cooperative copy of one instance's weights into LDS at `dst`; returns the
-// network view on the copy.  Ends with a workgroup barrier.
+// Cooperative copy of one instance's image (without Wb unless with_wb) into
+// LDS at `dst`; returns the network view on the copy.  Ends with a
+// workgroup barrier.
 template <typename T, int NB>
 __device__ Net<T, NB> stage_net(const NNArgs<T> &a, int inst, T *dst, bool with_wb) {
   constexpr int HP = 16 * NB;
-  const Net<T, NB> g = make_net<T, NB>(a, inst);
-  const int H = a.H, d = a.d_in, o = a.n_out, NH = a.NH;
-  const int img = HP * HP;
-  T *Wf = dst, *Wb = dst + NH * img;
-  T *W1 = Wb + (with_wb ? NH * img : 0);
-  T *b1 = W1 + H * d, *bh = b1 + H, *WL = bh + NH * H, *bL = WL + o * H;
   typedef typename V4<T>::type v4;
-  for (int j = 0; j < NH; ++j) {
-    const v4 *sf = reinterpret_cast<const v4 *>(g.Wf + (int64_t)j * g.w_stride);
-    v4 *df = reinterpret_cast<v4 *>(Wf + j * img);
-    for (int k = threadIdx.x; k < img / 4; k += blockDim.x) df[k] = sf[k];
-    if (with_wb) {
-      const v4 *sb = reinterpret_cast<const v4 *>(g.Wb + (int64_t)j * g.w_stride);
-      v4 *db = reinterpret_cast<v4 *>(Wb + j * img);
-      for (int k = threadIdx.x; k < img / 4; k += blockDim.x) db[k] = sb[k];
-    }
-  }
-  for (int k = threadIdx.x; k < H * d; k += blockDim.x) W1[k] = g.W1[k];
-  for (int k = threadIdx.x; k < H; k += blockDim.x) b1[k] = g.b1[k];
-  for (int k = threadIdx.x; k < NH * H; k += blockDim.x)
-    bh[k] = g.bias(k / H)[k - (k / H) * H];
-  for (int k = threadIdx.x; k < o * H; k += blockDim.x) WL[k] = g.WL[k];
-  for (int k = threadIdx.x; k < o; k += blockDim.x) bL[k] = g.bL[k];
+  const T *src = a.packed + (int64_t)inst * image_words(HP, a.NH, a.d_in, a.n_out);
+  const int n4 = stage_words(HP, a.NH, a.d_in, a.n_out, with_wb) / 4;
+  const v4 *s4 = reinterpret_cast<const v4 *>(src);
+  v4 *d4 = reinterpret_cast<v4 *>(dst);
+  for (int k = threadIdx.x; k < n4; k += blockDim.x) d4[k] = s4[k];
   __syncthreads();
-  Net<T, NB> n;
-  n.L = g.L;
-  n.W1 = W1;
-  n.b1 = b1;
-  n.bh = bh;
-  n.bh_stride = H;
-  n.WL = WL;
-  n.bL = bL;
-  n.Wf = Wf;
-  n.Wb = Wb;
-  n.w_stride = img;
-  return n;
+  return net_at<T, NB>(a, dst);
+}
+
+// Four consecutive values of a small array at accumulator position p.
+template <typename T>
+__device__ __forceinline__ typename V4<T>::type ld4(const T *p) {
+  return *reinterpret_cast<const typename V4<T>::type *>(p);
 }
 
 // Layer 1 (VALU): act[mb][i] = tanh(b1[f] + sum_a x[a] W1[f][a]).
 template <typename T, int NB>
 __device__ __forceinline__ void layer_in(const Net<T, NB> &n, const T *x, int q,
                                          T (&act)[NB][4]) {
-  const int H = n.L.H, d = n.L.d_in;
+  constexpr int HP = 16 * NB;
+  typedef typename V4<T>::type v4;
+  const int d = n.L.d_in;
 #pragma unroll
-  for (int mb = 0; mb < NB; ++mb)
+  for (int mb = 0; mb < NB; ++mb) {
+    v4 z = ld4(n.b1 + 16 * mb + 4 * q);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int f = 16 * mb + crow<T>(q, i);
-      T z = T(0);
-      if (f < H) {
-        z = n.b1[f];
+    for (int k = 0; k < MAXD; ++k)
+      if (k < d) {
+        const v4 w = ld4(n.W1 + k * HP + 16 * mb + 4 * q);
 #pragma unroll
-        for (int k = 0; k < MAXD; ++k)
-          if (k < d) z = fma(x[k], n.W1[f * d + k], z);
-        z = dtanh(z);
+        for (int i = 0; i < 4; ++i) z[i] = fma(x[k], w[i], z[i]);
       }
-      act[mb][i] = z;
-    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) act[mb][i] = dtanh(z[i]);
+  }
 }
 
 // acc[ob] += W_j (rows of block ob) x act, W given as an operand image.
@@ -241,39 +261,37 @@ __device__ __forceinline__ void mma_layer(const T *img, int lane, const T (&b)[N
     }
 }
 
-// Hidden layer j (MFMA): act <- tanh(W_j act + b_j).
+// Hidden layer j (MFMA, the accumulator starts at the bias):
+// act <- tanh(b_j + W_j act).
 template <typename T, int NB>
 __device__ __forceinline__ void layer_hidden(const Net<T, NB> &n, int j, int lane,
                                              T (&act)[NB][4]) {
+  constexpr int HP = 16 * NB;
   typedef typename V4<T>::type v4;
-  const int H = n.L.H, q = lane >> 4;
+  const int q = lane >> 4;
   v4 acc[NB];
 #pragma unroll
-  for (int ob = 0; ob < NB; ++ob) acc[ob] = v4{0, 0, 0, 0};
-  mma_layer<T, NB>(n.Wf + j * n.w_stride, lane, act, acc);
-  const T *bias = n.bias(j);
+  for (int ob = 0; ob < NB; ++ob) acc[ob] = ld4(n.bh + j * HP + 16 * ob + 4 * q);
+  mma_layer<T, NB>(n.Wf + j * HP * HP, lane, act, acc);
 #pragma unroll
   for (int ob = 0; ob < NB; ++ob)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int f = 16 * ob + crow<T>(q, i);
-      act[ob][i] = (f < H) ? dtanh(acc[ob][i] + bias[f]) : T(0);
-    }
+    for (int i = 0; i < 4; ++i) act[ob][i] = dtanh(acc[ob][i]);
 }
 
 // Output layer (VALU + cross-quarter shuffle): u[o] for the lane's row.
 template <typename T, int NB>
 __device__ __forceinline__ T layer_out(const Net<T, NB> &n, int o, int q,
                                        const T (&act)[NB][4]) {
-  const int H = n.L.H;
+  constexpr int HP = 16 * NB;
+  typedef typename V4<T>::type v4;
   T s = T(0);
 #pragma unroll
-  for (int mb = 0; mb < NB; ++mb)
+  for (int mb = 0; mb < NB; ++mb) {
+    const v4 w = ld4(n.WL + o * HP + 16 * mb + 4 * q);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int f = 16 * mb + crow<T>(q, i);
-      if (f < H) s = fma(act[mb][i], n.WL[o * H + f], s);
-    }
+    for (int i = 0; i < 4; ++i) s = fma(act[mb][i], w[i], s);
+  }
   s += __shfl_xor(s, 16);
   s += __shfl_xor(s, 32);
   return s + n.bL[o];
@@ -544,7 +562,7 @@ __global__ __launch_bounds__(256) void k_backward(NNArgs<T> a) {
           const int f = 16 * mb + crow<T>(q, i);
           T g = T(0);
           if (f < H) {
-            for (int o = 0; o < n_out; ++o) g = fma(net.WL[o * H + f], so[o], g);
+            for (int o = 0; o < n_out; ++o) g = fma(net.WL[o * HP + 16 * mb + 4 * q + i], so[o], g);
             const T av = aN[f];
             g *= (T(1) - av * av);
           }
@@ -583,7 +601,7 @@ __global__ __launch_bounds__(256) void k_backward(NNArgs<T> a) {
         v4 c[NB];
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib) c[ib] = v4{0, 0, 0, 0};
-        mma_layer<T, NB>(net.Wb + jj * net.w_stride, lane, dl_r, c);
+        mma_layer<T, NB>(net.Wb + jj * HP * HP, lane, dl_r, c);
         const T *arow = ain + trow * LS;
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib)
@@ -682,22 +700,28 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
   // gradient accumulators (MFMA tiles; row m = crow(q, i) is the hidden
   // feature 16 ob + m, column n = row):
   //   dwacc[jj][ob*NB+ib]: hidden W ( out, in )
-  //   sacc[ob]: every small term in one tile, by column -- n < d_in first W,
-  //     d_in first bias, d_in + 1 + jj hidden bias jj, d_in + 1 + NH + o
-  //     output W row o.  Each product accumulated into the tile has a B
-  //     operand that is zero outside its own columns, so the columns add
-  //     exactly what separate tiles would.
+  //   sacc[ob]: the first and last layers' terms in one tile, by column --
+  //     n < d_in first W, d_in first bias, d_in + 1 + o output W row o.
+  //     Each product accumulated into the tile has a B operand that is zero
+  //     outside its own columns, so the columns add exactly what separate
+  //     tiles would.
+  //   bhacc[jj][xb]: hidden bias of feature 16 xb + row over the rows
+  //     4q .. 4q+3 of each iteration, summed over the quarters at the end
   //   blacc: output bias o = q, summed over the row lanes at the end
   v4 dwacc[NH > 0 ? NH : 1][NB * NB];
   v4 sacc[NB];
+  T bhacc[NH > 0 ? NH : 1][NB];
 #pragma unroll
-  for (int j = 0; j < (NH > 0 ? NH : 1); ++j)
+  for (int j = 0; j < (NH > 0 ? NH : 1); ++j) {
 #pragma unroll
     for (int b = 0; b < NB * NB; ++b) dwacc[j][b] = v4{0, 0, 0, 0};
 #pragma unroll
+    for (int b = 0; b < NB; ++b) bhacc[j][b] = T(0);
+  }
+#pragma unroll
   for (int b = 0; b < NB; ++b) sacc[b] = v4{0, 0, 0, 0};
   T blacc = T(0);
-  const int col_b1 = d_in, col_bh = d_in + 1, col_wl = d_in + 1 + NH;
+  const int col_b1 = d_in, col_wl = d_in + 1;
 
   const int64_t r_begin = item.y, r_end = item.z;
   for (int64_t g0 = r_begin + 16 * w; g0 < r_end; g0 += 16 * NW) {
@@ -750,12 +774,11 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
         if (o == q) blacc += so;
         if (q == 0) sbuf[o * WRS + row] = so;
 #pragma unroll
-        for (int mb = 0; mb < NB; ++mb)
+        for (int mb = 0; mb < NB; ++mb) {
+          const v4 wl = ld4(net.WL + o * HP + 16 * mb + 4 * q);
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int f = 16 * mb + crow<T>(q, i);
-            if (f < H) dl[mb][i] = fma(net.WL[o * H + f], so, dl[mb][i]);
-          }
+          for (int i = 0; i < 4; ++i) dl[mb][i] = fma(wl[i], so, dl[mb][i]);
+        }
       }
     }
 #pragma unroll
@@ -801,11 +824,9 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
           dv[xb] = *reinterpret_cast<const v4 *>(dbuf + (16 * xb + row) * WRS + 4 * q);
           av[xb] = *reinterpret_cast<const v4 *>(abuf + (16 * xb + row) * WRS + 4 * q);
         }
-        const T one = row == col_bh + jj ? T(1) : T(0);
 #pragma unroll
         for (int ob = 0; ob < NB; ++ob) {
-#pragma unroll
-          for (int kb = 0; kb < 4; ++kb) sacc[ob] = mma16(dv[ob][kb], one, sacc[ob]);
+          bhacc[jj][ob] += (dv[ob][0] + dv[ob][1]) + (dv[ob][2] + dv[ob][3]);
 #pragma unroll
           for (int ib = 0; ib < NB; ++ib) {
             v4 c = dwacc[jj][ob * NB + ib];
@@ -820,7 +841,7 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
       v4 c[NB];
 #pragma unroll
       for (int ib = 0; ib < NB; ++ib) c[ib] = v4{0, 0, 0, 0};
-      mma_layer<T, NB>(net.Wb + jj * net.w_stride, lane, dl, c);
+      mma_layer<T, NB>(net.Wb + jj * HP * HP, lane, dl, c);
 #pragma unroll
       for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
@@ -873,8 +894,17 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
           const T sv = sacc[ob][i];
           if (row < d_in) put(L.oW1() + out * d_in + row, sv);
           else if (row == col_b1) put(L.ob1() + out, sv);
-          else if (row < col_wl) put(L.obh(row - col_bh) + out, sv);
           else if (row < col_wl + n_out) put(L.oWL() + (row - col_wl) * H + out, sv);
+        }
+#pragma unroll
+      for (int jj = 0; jj < NH; ++jj)
+#pragma unroll
+        for (int xb = 0; xb < NB; ++xb) {
+          T v = bhacc[jj][xb];
+          v += __shfl_xor(v, 16);
+          v += __shfl_xor(v, 32);
+          const int f = 16 * xb + row;
+          if (q == 0 && f < H) put(L.obh(jj) + f, v);
         }
       const T bl = rowsum16(blacc);
       if (row == 0 && q < n_out) put(L.obL() + q, bl);
@@ -1015,7 +1045,7 @@ struct tdx_nn_plan {
 static size_t wave_private_lds(const tdx_nn_plan *p, const tdx_nn_desc *d) {
   const size_t wb = (size_t)2 * 20 * p->HP + 2 * 16 * 20;
   const size_t tr = std::max<size_t>(4 * wb, (size_t)p->P);
-  return tr + stage_words(p->NH, p->HP, p->H, d->d_in, d->n_out, true);
+  return tr + stage_words(p->HP, p->NH, d->d_in, d->n_out, true);
 }
 
 namespace {
@@ -1074,8 +1104,7 @@ NNArgs<T> make_args(const tdx_nn_plan *p, const void *params, void *ws) {
 
 template <typename T, int NB>
 int launch_pack(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
-  if (p->NH == 0) return TDX_OK;
-  const int64_t total = (int64_t)p->n_inst * p->NH * 2 * p->HP * p->HP;
+  const int64_t total = (int64_t)p->n_inst * image_words(p->HP, p->NH, p->d_in, p->n_out);
   const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65535);
   hipLaunchKernelGGL((k_pack<T, NB>), dim3(blocks), dim3(256), 0, s, a);
   TDX_TRY_HIP(hipGetLastError());
@@ -1305,7 +1334,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   p->k1_lds = (2 * K1_WAVES * 16 * max_slots * d->n_out + 512) * ts;
   {
     // stage the forward operand images in LDS when they are small
-    const size_t st = (size_t)stage_words(p->NH, p->HP, p->H, d->d_in, d->n_out, false) * ts;
+    const size_t st = (size_t)stage_words(p->HP, p->NH, d->d_in, d->n_out, false) * ts;
     p->k1_stage = (p->k1_lds + st <= 64 * 1024) ? 1 : 0;
     if (p->k1_stage) p->k1_lds += st;
   }
@@ -1333,7 +1362,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   p->k2_lds = lds_of(waves, p->acc_lds != 0);
   // narrow networks: wavefront-private backward with register accumulators
   p->k2_wave_private = (p->NB <= 2 && p->NH <= 4 && d->tile_rows >= 0 &&
-                        d->d_in + 1 + p->NH + d->n_out <= 16 &&
+                        d->d_in + 1 + d->n_out <= 16 &&
                         wave_private_lds(p, d) * ts <= LDS_MAX) ? 1 : 0;
   if (p->k2_wave_private) {
     p->k2_waves = 4;
@@ -1382,7 +1411,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   }
   size_t off = 0;
   p->off_packed = off;
-  off = align_up(off + (size_t)p->n_inst * std::max(p->NH, 0) * 2 * p->HP * p->HP * ts, 256);
+  off = align_up(off + (size_t)p->n_inst * image_words(p->HP, p->NH, d->d_in, d->n_out) * ts, 256);
   p->off_seeds = off;
   off = align_up(off + (size_t)n_rows * p->n_out * ts, 256);
   p->off_lp = off;
