@@ -15,11 +15,22 @@ instance.  Reported next to it, measured in the same run:
   torch CPU ops) timed on the host cores on a bounded sample of instances,
   scaled to the full sweep by collocation points;
 * roofline: FP32-MFMA roofline of the dominant kernel (the backward kernel),
-  from HIP-event timing of its launches during a timed pass.
+  from HIP-event timing of its launches during a timed pass;
+* accuracy: a fresh copy of the workload trained by the fused path and by
+  the plain path from the same initial weights for the same number of Adam
+  steps, each instance's rel-L2 error on its collocation points against the
+  FP64 host solution (Cole-Hopf quadrature for Burgers, closed forms for
+  KdV and Legendre), for a spread of instances.
 
 Other workloads (--workload): burgers, kdv, legendre ('NN', one instance),
 wave (the 128 x 128 'mat' problem) and wave16k (the wave operator on a
 16384 x 16384 field, the HBM-roofline case of the stencil kernels).
+
+--calibrate runs two microbenchmarks instead: a stream copy of 1 GiB
+(out = -in, float32: 1 GiB read + 1 GiB written per launch; its kernel is
+also the known-size reference for scaling rocprofv3's FETCH_SIZE and
+WRITE_SIZE counters) and an FP32 GEMM (torch.mm, 8192^3) for the attainable
+FP32-matrix rate.
 """
 
 import argparse
@@ -147,6 +158,79 @@ def cpu_nn_step_ms(problems, nets0, budget_s, lr):
     return ms_sum * pts_all / pts_done, n_inst, pts_done, pts_all
 
 
+def exact_solution(workload, inst):
+    """FP64 host solution u*(points [n, d]) of an 'NN' workload instance."""
+    from torch_de_solver_amd import exact as X
+    if workload == 'sweep':
+        return lambda p: X.burgers(p[:, 0], p[:, 1], inst.nu)
+    if workload == 'burgers':
+        return lambda p: X.burgers(p[:, 0], p[:, 1], 0.01 / math.pi)
+    if workload == 'kdv':
+        return lambda p: X.kdv_soliton(p[:, 0], p[:, 1])
+    if workload == 'legendre':
+        return lambda p: X.legendre_p3(p[:, 0])
+    raise ValueError(workload)
+
+
+def accuracy_nn(args, lr):
+    """Rel-L2 of fused-trained and plain-trained solutions, same init, same
+    number of Adam steps, against the FP64 host solution."""
+    import numpy as np
+    from torch_de_solver_amd import exact as X
+    from torch_de_solver_amd.plain import PlainNN
+    from torch_de_solver_amd.sweep import SweepSolver
+    problems, insts = nn_problems(args.workload)
+    n = len(problems)
+    sel = sorted(set(int(round(v)) for v in np.linspace(0, n - 1, min(args.acc_instances, n))))
+    solver = SweepSolver(problems)
+    dev, dt = solver.device, solver.params.dtype
+    pts, uref = {}, {}
+    for i in sel:
+        grid = problems[i][0].build('NN')
+        P = grid[torch.as_tensor(solver.problems[i].central_index)]
+        pts[i] = P.to(dev, dt)
+        uref[i] = exact_solution(args.workload, insts[i] if insts else None)(
+            P.double().numpy())
+
+    def rel(net, i):
+        with torch.no_grad():
+            u = net(pts[i])[:, 0].double().cpu().numpy()
+        return X.relative_l2(u, uref[i])
+
+    untrained = [rel(solver.nets[i], i) for i in sel]
+    t0 = time.time()
+    solver.train(args.acc_steps, lr)
+    torch.cuda.synchronize()
+    fused = [rel(solver.nets[i], i) for i in sel]
+    log(f"[bench] accuracy: fused {args.acc_steps} steps in {time.time() - t0:.1f}s")
+    fresh, _ = nn_problems(args.workload)
+    plain = []
+    t0 = time.time()
+    for i in sel:
+        net = fresh[i][3].to(dev, dt)
+        pl = PlainNN(solver.problems[i], net, dev)
+        opt = torch.optim.Adam(net.parameters(), lr=lr)
+        for _ in range(args.acc_steps):
+            opt.zero_grad()
+            pl.loss().backward()
+            opt.step()
+        plain.append(rel(net, i))
+    torch.cuda.synchronize()
+    log(f"[bench] accuracy: plain {len(sel)} x {args.acc_steps} steps in {time.time() - t0:.1f}s")
+    ratio = [f / p for f, p in zip(fused, plain)]
+    return {"steps": args.acc_steps, "instances": sel,
+            "reference_solution": "FP64 host: " + {
+                'sweep': "Cole-Hopf quadrature of viscous Burgers",
+                'burgers': "Cole-Hopf quadrature of viscous Burgers",
+                'kdv': "KdV soliton 0.5 sech^2(0.5(x-t-10)), periodic images",
+                'legendre': "P3(x) = (5x^3-3x)/2"}[args.workload],
+            "points": "collocation ('central') points of each instance",
+            "rel_l2_fused": fused, "rel_l2_plain": plain, "rel_l2_untrained": untrained,
+            "fused_over_plain": ratio,
+            "within_10pct": sum(1 for r in ratio if abs(r - 1.0) <= 0.10),
+            "of": len(sel)}
+
+
 def bench_nn(args):
     from torch_de_solver_amd.nn_engine import mlp_arch
     from torch_de_solver_amd.sweep import SweepSolver
@@ -228,6 +312,8 @@ def bench_nn(args):
                                "sample": f"{n_inst} instance(s) x 1 Adam step of the CPU oracle "
                                          f"({pts} of {pts_all} collocation points), scaled by points"}
         log(f"[bench] cpu oracle: {cms:.0f} ms/step (scaled)")
+    if args.acc_steps > 0:
+        out["accuracy"] = accuracy_nn(args, lr)
     return out
 
 
@@ -320,6 +406,25 @@ def bench_mat(args):
     return out
 
 
+def calibrate(args):
+    dev = torch.device('cuda')
+    n = 1 << 28                      # 2^28 float32 = 1 GiB
+    a = torch.randn(n, device=dev)
+    b = torch.empty_like(a)
+    torch.neg(a, out=b)
+    ms = time_cuda(lambda: torch.neg(a, out=b), args.steps)
+    del a, b
+    m = 8192
+    x = torch.randn(m, m, device=dev)
+    y = torch.randn(m, m, device=dev)
+    z = torch.mm(x, y)
+    gms = time_cuda(lambda: torch.mm(x, y, out=z), max(args.steps // 4, 3))
+    return {"copy": {"bytes_read": 4 * n, "bytes_written": 4 * n, "ms": ms,
+                     "GB_per_s": 8.0 * n / (ms * 1e-3) / 1e9, "peak_GB_per_s": HBM_PEAK / 1e9},
+            "gemm_fp32": {"m": m, "ms": gms, "TFLOP_per_s": 2.0 * m ** 3 / (gms * 1e-3) / 1e12,
+                          "peak_TFLOP_per_s": FP32_MFMA_PEAK / 1e12}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -333,6 +438,11 @@ def main():
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--fast-div', action='store_true')
+    ap.add_argument('--acc-steps', type=int, default=500,
+                    help="Adam steps of the accuracy comparison (0: skip it)")
+    ap.add_argument('--acc-instances', type=int, default=8)
+    ap.add_argument('--calibrate', action='store_true',
+                    help="stream-copy and FP32-GEMM microbenchmarks only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -341,7 +451,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(2)
-    if args.workload in ('wave', 'wave16k'):
+    if args.calibrate:
+        out = calibrate(args)
+    elif args.workload in ('wave', 'wave16k'):
         out = bench_mat(args)
     else:
         out = bench_nn(args)
