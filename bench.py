@@ -238,8 +238,9 @@ def bench_nn(args):
     import copy
     nets0 = [copy.deepcopy(p[3]) for p in problems]
     t0 = time.time()
-    solver = SweepSolver(problems)
-    log(f"[bench] compiled {solver.n_inst} instance(s) in {time.time() - t0:.1f}s")
+    solver = SweepSolver(problems, autotune=not args.no_tune)
+    log(f"[bench] compiled {solver.n_inst} instance(s) in {time.time() - t0:.1f}s"
+        f" (tuning: {solver.tuning})")
     eng = solver.engine
     arch = solver.arch
     n_rows = eng.tables.n_rows
@@ -286,7 +287,8 @@ def bench_nn(args):
                    "MLP 2-32-32-32-1 tanh each, Adam lr 1e-3, h=1e-3",
                    "instances": solver.n_inst, "collocation_points": colloc,
                    "network_rows_per_step": n_rows, "mlp": f"{arch.d_in}-{arch.width}x{arch.n_layers - 1}-{arch.n_out}",
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU",
+                   "autotune": solver.tuning},
         "colloc_pts_per_s": colloc * 1e3 / ms,
         "loss_sum_after": loss_now,
         "roofline": {"bound": "mfma", "kernel": "k_backward (recompute + adjoint + weight gradients)",
@@ -334,9 +336,14 @@ def bench_mat(args):
     prob = compile_mat(grid, eq.equation_lst, conds.build(dom.variable_dict), 1,
                        lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
     dev = torch.device('cuda')
-    eng = MatEngine(prob, dev, exact_div=not args.fast_div)
-    log(f"[bench] compiled {n}x{n} in {time.time() - t0:.1f}s")
     u = u0.to(dev).contiguous()
+    tuning = None
+    tile_rows = args.tile_rows
+    if not args.no_tune and tile_rows == 0:
+        from torch_de_solver_amd.autotune import tune_mat
+        tile_rows, tuning = tune_mat(prob, dev, u, exact_div=not args.fast_div)
+    eng = MatEngine(prob, dev, exact_div=not args.fast_div, tile_rows=tile_rows)
+    log(f"[bench] compiled {n}x{n} in {time.time() - t0:.1f}s (tuning: {tuning})")
     m, v = torch.zeros_like(u), torch.zeros_like(u)
     step = torch.zeros(1, dtype=torch.int64, device=dev)
     loss = torch.zeros(2, device=dev)
@@ -364,7 +371,8 @@ def bench_mat(args):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"wave u_tt=4u_xx 'mat' {n}x{n}, Adam lr 1e-3",
-                   "collocation_points": N, "parallelism": "single GPU"},
+                   "collocation_points": N, "parallelism": "single GPU",
+                   "tile_rows": tile_rows, "autotune": tuning},
         "colloc_pts_per_s": N * 1e3 / ms,
         "roofline": {"bound": "hbm", "kernel": "k_mat_tile (residual + loss + adjoint)",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -441,6 +449,9 @@ def main():
     ap.add_argument('--acc-steps', type=int, default=500,
                     help="Adam steps of the accuracy comparison (0: skip it)")
     ap.add_argument('--acc-instances', type=int, default=8)
+    ap.add_argument('--no-tune', action='store_true',
+                    help="default tile shapes instead of the autotuner's choice")
+    ap.add_argument('--tile-rows', type=int, default=0, help="'mat': output rows per tile")
     ap.add_argument('--calibrate', action='store_true',
                     help="stream-copy and FP32-GEMM microbenchmarks only")
     args = ap.parse_args()

@@ -194,7 +194,8 @@ typedef struct tdx_mat_desc {
   const void *reals;
   int64_t n_data;
   const void *data;
-  int exact_div; /* 1: IEEE division by 2h and h (bit-identical fields) */
+  int exact_div; /* 1: correctly rounded division by 2h and h (bit-identical fields) */
+  int tile_rows; /* output rows per tile workgroup (2-D grids); 0: chosen by the plan */
 } tdx_mat_desc;
 
 typedef struct tdx_mat_plan tdx_mat_plan;

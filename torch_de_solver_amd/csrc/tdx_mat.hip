@@ -26,31 +26,50 @@
 #include "tdx_common.h"
 
 #include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <map>
+#include <mutex>
 
 namespace tdx {
 
 constexpr int MMAX_DP = 8;
 constexpr int MMAX_EQ = 4;
 constexpr int MMAX_FS = 16;   // factor slots over all terms of all equations
+constexpr int MMAX_TF = 4;    // factors of one term
 constexpr int MAT_THREADS = 256;
+constexpr int MFS_INTS = 5;   // factor-slot record: var, n_axes, axis[3]
+// Stencil coefficient table (T values): ab[MMAX_DP] af[MMAX_DP], then per
+// axis 4*MMAX_DP products for the transposed stencils:
+//   ab/(2h) | af/(2h) | ab/h | af/h
+constexpr int MCOEF_AX = 2 * MMAX_DP;
+constexpr int MCOEF_N = 2 * MMAX_DP + 2 * 4 * MMAX_DP;
 
 template <typename T>
 struct MatArgs {
   int n_eq, n0, n1, dp;
   int64_t N;  // n0 * n1
-  T h[2], h2[2], ih[2], ih2[2];
-  T ab[MMAX_DP], af[MMAX_DP];
+  T h0, h1, h20, h21, ih0, ih1, ih20, ih21;  // h, 2h and their reciprocals per axis
+  const T *coef;      // stencil coefficient table
+  const int32_t *fs;  // factor-slot records, n_fs x MFS_INTS
+  // residual program, flattened (copied to LDS by the tile kernel):
+  //   pint: per equation {n_terms, first term}; per term {n_factors,
+  //         coefficient kind, coefficient ref, first factor slot}; per
+  //         factor slot {pow kind, pow int, pow real index}
+  //   preal: lambda per equation, scalar coefficients, real powers
+  const int32_t *pint;
+  const T *preal;
+  int n_pint, n_preal, pterm0, pfac0;
   const int32_t *ints;
   const T *reals;
   const T *data;
   int eq_i;
-  // factor slots (flattened over equations and terms), built on the host
   int n_fs;
-  int fs_var[MMAX_FS], fs_nax[MMAX_FS], fs_ax[MMAX_FS][3];
-  int R0, R1;  // reach per axis
-  int T0, T1;  // tile
-  int tiles1;  // tiles along axis 1
+  int R0, R1;         // reach per axis
+  int T0, T1;         // output tile
+  int E0, E1;         // tile with its halo of 2R per side; E1 is 64 * 2^k
+  int tpr_log;        // log2 of the threads per tile row, min(E1, 256)
+  int tiles1;         // tiles along axis 1
   // bc
   int n_bc, bc_i;
   int n_touch, bct_ptr_i, bct_pos_i, bct_seed_i, bct_w_i;
@@ -60,59 +79,116 @@ struct MatArgs {
   T *op;       // may be null
   T *bval;     // may be null
   T *lp;       // 2 per tile workgroup then 2 per bc workgroup
+  T *lp_mid;   // 2 x 256: first stage of the loss sum
   T *bc_seeds;
   const T *dloss;
   int n_tile_wg;
 };
 
-template <typename T, bool EXACT>
+// x / h by division mode DIV: 0 multiplies by the rounded reciprocal inv
+// (within an ulp); 1 is the IEEE division; 2 (float only) is the
+// correctly rounded quotient from inv = RN(1/h) by one correction step,
+// q = RN(x inv), r = x - q h (exact by fma), RN(q + r inv), which the plan
+// has checked against the correctly rounded quotient for every significand
+// of x in [1, 2); it holds for every x whose quotient scales exactly, so
+// the step is taken for 2^-90 <= |x| <= 2^90 and zero, and the IEEE
+// division otherwise.
+template <typename T, int DIV>
 __device__ __forceinline__ T qdiv(T x, T h, T inv) {
-  if (EXACT) return x / h;
-  return x * inv;
-}
-
-// One application of D along `axis` at local point p of an LDS field whose
-// row stride is E1; g is the global index along the axis, n its length.
-template <typename T, bool EXACT>
-__device__ __forceinline__ T dstencil(const MatArgs<T> &a, const T *src, int p, int stride, int g,
-                                      int n, int axis) {
-  const int dp = a.dp;
-  if (g < dp - 1) {
-    T f = T(0);
-    for (int i = 0; i < dp; ++i) f = f + src[p + i * stride] * a.af[i];
-    return qdiv<T, EXACT>(f, a.h[axis], a.ih[axis]);
+  if constexpr (DIV == 0) {
+    return x * inv;
+  } else if constexpr (DIV == 2 && sizeof(T) == 4) {
+    const float ax = fabsf(x);
+    if (__builtin_expect(ax <= 0x1p90f && (ax >= 0x1p-90f || ax == 0.0f), 1)) {
+      const float q = x * inv;
+      const float r = fmaf(-q, h, x);
+      return fmaf(r, inv, q);
+    }
+    return x / h;
+  } else {
+    return x / h;
   }
-  T b = T(0);
-  for (int i = 0; i < dp; ++i) b = b + src[p + (i - (dp - 1)) * stride] * a.ab[i];
-  if (g >= n - dp + 1) return qdiv<T, EXACT>(b, a.h[axis], a.ih[axis]);
-  T f = T(0);
-  for (int i = 0; i < dp; ++i) f = f + src[p + i * stride] * a.af[i];
-  return qdiv<T, EXACT>(b + f, a.h2[axis], a.ih2[axis]);
 }
 
-// Transpose of D at global index g: sum over the rows k whose stencil reads g.
-template <typename T, bool EXACT>
-__device__ __forceinline__ T dtstencil(const MatArgs<T> &a, const T *src, int p, int stride, int g,
-                                       int n, int axis) {
-  const int dp = a.dp;
+// One application of D at local point p of an LDS field (stride between
+// neighbours along the axis); g is the global index along the axis, n its
+// length, h/h2 the step and twice the step.  DP > 0 fixes the number of
+// derivative points at compile time (the loops unroll and ab/af index
+// registers); DP = 0 takes it from dp.
+template <typename T, int DP, int DIV>
+__device__ __forceinline__ T dstencil(const T *src, int p, int stride, int g, int n, int dp,
+                                      const T *ab, const T *af, T h, T h2, T ih, T ih2) {
+  const int np = DP > 0 ? DP : dp;
+  if (g < np - 1) {
+    T f = src[p] * af[0];
+#pragma unroll
+    for (int i = 1; i < np; ++i) f = f + src[p + i * stride] * af[i];
+    return qdiv<T, DIV>(f, h, ih);
+  }
+  T b = src[p - (np - 1) * stride] * ab[0];
+#pragma unroll
+  for (int i = 1; i < np; ++i) b = b + src[p + (i - (np - 1)) * stride] * ab[i];
+  if (g >= n - np + 1) return qdiv<T, DIV>(b, h, ih);
+  T f = src[p] * af[0];
+#pragma unroll
+  for (int i = 1; i < np; ++i) f = f + src[p + i * stride] * af[i];
+  return qdiv<T, DIV>(b + f, h2, ih2);
+}
+
+// D at a point whose stencil rows are all interior: the third case of
+// dstencil, the same operations in the same order.
+template <typename T, int DP, int DIV>
+__device__ __forceinline__ T dstencil_in(const T *src, int p, int stride, int dp, const T *ab,
+                                         const T *af, T h2, T ih2) {
+  const int np = DP > 0 ? DP : dp;
+  T b = src[p - (np - 1) * stride] * ab[0];
+#pragma unroll
+  for (int i = 1; i < np; ++i) b = b + src[p + (i - (np - 1)) * stride] * ab[i];
+  T f = src[p] * af[0];
+#pragma unroll
+  for (int i = 1; i < np; ++i) f = f + src[p + i * stride] * af[i];
+  return qdiv<T, DIV>(b + f, h2, ih2);
+}
+
+// Transpose of D at global index g: sum over the rows k whose stencil reads
+// g, each weighted by its coefficient over h (one-sided rows) or 2h; c holds
+// ab/(2h), af/(2h), ab/h, af/h at a distance of cs values.
+template <typename T, int DP>
+__device__ __forceinline__ T dtstencil(const T *src, int p, int stride, int g, int n, int dp,
+                                       const T *c, int cs) {
+  const int np = DP > 0 ? DP : dp;
+  const T *cb2 = c, *cf2 = c + cs, *cb1 = c + 2 * cs, *cf1 = c + 3 * cs;
   T acc = T(0);
-  // backward labels: row k = g + (dp-1) - i reads g with ab[i]
-  for (int i = 0; i < dp; ++i) {
-    const int k = g + (dp - 1) - i;
-    if (k < 0 || k >= n || k < dp - 1) continue;  // front rows use forward labels only
-    const T v = src[p + ((dp - 1) - i) * stride] * a.ab[i];
-    acc += (k >= n - dp + 1) ? qdiv<T, EXACT>(v, a.h[axis], a.ih[axis])
-                             : qdiv<T, EXACT>(v, a.h2[axis], a.ih2[axis]);
+  // backward labels: row k = g + (np-1) - i reads g with ab[i]; front rows
+  // (k < np-1) use forward labels only
+#pragma unroll
+  for (int i = 0; i < np; ++i) {
+    const int k = g + (np - 1) - i;
+    if (k < np - 1 || k >= n) continue;
+    acc += src[p + ((np - 1) - i) * stride] * ((k >= n - np + 1) ? cb1[i] : cb2[i]);
   }
-  // forward labels: row k = g - i reads g with af[i]
-  for (int i = 0; i < dp; ++i) {
+  // forward labels: row k = g - i reads g with af[i]; back rows use
+  // backward labels only
+#pragma unroll
+  for (int i = 0; i < np; ++i) {
     const int k = g - i;
-    if (k < 0 || k >= n) continue;
-    if (k >= dp - 1 && k >= n - dp + 1) continue;  // back rows use backward labels only
-    const T v = src[p - i * stride] * a.af[i];
-    acc += (k < dp - 1) ? qdiv<T, EXACT>(v, a.h[axis], a.ih[axis])
-                        : qdiv<T, EXACT>(v, a.h2[axis], a.ih2[axis]);
+    if (k < 0 || (k >= np - 1 && k >= n - np + 1)) continue;
+    acc += src[p - i * stride] * ((k < np - 1) ? cf1[i] : cf2[i]);
   }
+  return acc;
+}
+
+// dtstencil where every row k that reads g is interior.
+template <typename T, int DP>
+__device__ __forceinline__ T dtstencil_in(const T *src, int p, int stride, int dp, const T *c,
+                                          int cs) {
+  const int np = DP > 0 ? DP : dp;
+  const T *cb2 = c, *cf2 = c + cs;
+  T acc = src[p + (np - 1) * stride] * cb2[0];
+#pragma unroll
+  for (int i = 1; i < np; ++i) acc += src[p + ((np - 1) - i) * stride] * cb2[i];
+#pragma unroll
+  for (int i = 0; i < np; ++i) acc += src[p - i * stride] * cf2[i];
   return acc;
 }
 
@@ -130,64 +206,163 @@ __device__ __forceinline__ T mdfpow(const MatArgs<T> &a, const int32_t *F, T x) 
   return p * dpow(x, p - T(1));
 }
 
-struct Box {
-  int lo0, hi0, lo1, hi1;
-};
+// x ** p for a factor record {pow kind, pow int, pow real index} of the
+// flattened program, and its derivative
+template <typename T>
+__device__ __forceinline__ T ppow(const int32_t *Fr, const T *pr, T x) {
+  return Fr[0] == 0 ? ipow(x, Fr[1]) : dpow(x, pr[Fr[2]]);
+}
+template <typename T>
+__device__ __forceinline__ T pdpow(const int32_t *Fr, const T *pr, T x) {
+  if (Fr[0] == 0) {
+    const int p = Fr[1];
+    return p == 0 ? T(0) : T(p) * ipow(x, p - 1);
+  }
+  const T p = pr[Fr[2]];
+  return p * dpow(x, p - T(1));
+}
+
+// Coefficient of a term record of the flattened program: a scalar of the
+// program reals (LDS) or the grid array in global memory at point gp.  The
+// kind is uniform; reading it through readfirstlane keeps the two loads on
+// separate scalar branches instead of one flat load.
+template <typename T>
+__device__ __forceinline__ T term_coeff(const int32_t *Tr, const T *pr, const T *data, int64_t gp) {
+  const int kind = __builtin_amdgcn_readfirstlane(Tr[1]);
+  const int ref = __builtin_amdgcn_readfirstlane(Tr[2]);
+  if (kind == 0) return pr[ref];
+  return data[ref + gp];
+}
+
+// Calls fn(i0, i1) for every point of the box [lo0, hi0) x [lo1, hi1) of an
+// E0 x E1 tile.  A tile row is covered by 2^tpr_log consecutive threads
+// (coalesced along axis 1), 256 >> tpr_log rows per sweep.
+template <typename Fn>
+__device__ __forceinline__ void for_box(int E1, int tpr_log, int lo0, int hi0, int lo1, int hi1,
+                                        Fn fn) {
+  const int tpr = 1 << tpr_log;
+  const int c0 = threadIdx.x & (tpr - 1);
+  for (int i0 = lo0 + (int)(threadIdx.x >> tpr_log); i0 < hi0; i0 += MAT_THREADS >> tpr_log)
+    for (int i1 = c0; i1 < E1; i1 += tpr)
+      if (i1 >= lo1 && i1 < hi1) fn(i0, i1);
+}
 
 // ---------------------------------------------------------------------------
 // tiled residual + loss + adjoint
 // ---------------------------------------------------------------------------
-template <typename T, bool EXACT, bool ADJ>
-__global__ __launch_bounds__(MAT_THREADS) void k_mat_tile(MatArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T *smem = reinterpret_cast<T *>(smem_raw);
-  const int t0 = blockIdx.x / a.tiles1, t1 = blockIdx.x - (blockIdx.x / a.tiles1) * a.tiles1;
-  const int E0 = a.T0 + 4 * a.R0, E1 = a.T1 + 4 * a.R1;
+// LDS: U[n_eq][S] | F[n_fs][S] (factor values, then seeds) | A[S] | B[S] |
+// G[n_eq][S] (adjoint) | 2 x 256 reduction words | coefficient table |
+// program reals | program ints, S = E0 * E1.  DP: number of derivative
+// points, 0 = any (read from dp).
+// INT: the tile with its halo lies inside the grid, so every point exists and
+// every stencil row the tile evaluates or transposes is an interior row.
+template <typename T, int DP, int DIV, bool ADJ, bool INT>
+__device__ __forceinline__ void mat_tile_body(const MatArgs<T> &a, T *smem, int t0, int t1) {
+  const int E0 = a.E0, E1 = a.E1, tl = a.tpr_log;
   const int S = E0 * E1;
-  const int base0 = t0 * a.T0 - 2 * a.R0, base1 = t1 * a.T1 - 2 * a.R1;
-  T *U = smem;                    // n_eq * S
-  T *F = U + a.n_eq * S;          // n_fs * S (values, then seeds)
-  T *A = F + a.n_fs * S;          // scratch
-  T *B = A + S;                   // scratch
-  T *G = B + S;                   // n_eq * S gradient
+  const int R0 = a.R0, R1 = a.R1, T0 = a.T0, T1 = a.T1;
+  const int n0 = a.n0, n1 = a.n1, dp = a.dp;
+  const int64_t N = a.N;
+  const int base0 = t0 * T0 - 2 * R0, base1 = t1 * T1 - 2 * R1;
+  T *U = smem;
+  T *F = U + a.n_eq * S;
+  T *A = F + a.n_fs * S;
+  T *B = A + S;
+  T *G = B + S;
   T *red = G + (ADJ ? a.n_eq * S : 0);
   const int tid = threadIdx.x;
+  auto in_grid = [&](int g0, int g1) {
+    return INT || (g0 >= 0 && g0 < n0 && g1 >= 0 && g1 < n1);
+  };
+  // stencil coefficients: registers when DP is fixed, else an LDS copy
+  constexpr int CN = DP > 0 ? DP : 1;
+  T abr[CN], afr[CN], t0r[4 * CN], t1r[4 * CN];
+  const T *ab, *af, *tc0, *tc1;
+  int cs;
+  if constexpr (DP > 0) {
+#pragma unroll
+    for (int i = 0; i < DP; ++i) {
+      abr[i] = a.coef[i];
+      afr[i] = a.coef[MMAX_DP + i];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        t0r[k * DP + i] = a.coef[MCOEF_AX + k * MMAX_DP + i];
+        t1r[k * DP + i] = a.coef[MCOEF_AX + 4 * MMAX_DP + k * MMAX_DP + i];
+      }
+    }
+    ab = abr;
+    af = afr;
+    tc0 = t0r;
+    tc1 = t1r;
+    cs = DP;
+  } else {
+    T *cl = red + 2 * MAT_THREADS;
+    ab = cl;
+    af = cl + MMAX_DP;
+    tc0 = cl + MCOEF_AX;
+    tc1 = cl + MCOEF_AX + 4 * MMAX_DP;
+    cs = MMAX_DP;
+  }
+  // coefficient table (used when DP = 0) and residual program in LDS
+  T *cl = red + 2 * MAT_THREADS;
+  T *PR = cl + MCOEF_N;
+  int32_t *PI = reinterpret_cast<int32_t *>(PR + a.n_preal);
+  for (int k = tid; k < MCOEF_N; k += MAT_THREADS) cl[k] = a.coef[k];
+  for (int k = tid; k < a.n_preal; k += MAT_THREADS) PR[k] = a.preal[k];
+  for (int k = tid; k < a.n_pint; k += MAT_THREADS) PI[k] = a.pint[k];
 
-  // load u with halo (zero outside the grid)
+  // u with halo (zero outside the grid)
   for (int v = 0; v < a.n_eq; ++v)
-    for (int p = tid; p < S; p += MAT_THREADS) {
-      const int i0 = p / E1, i1 = p - (p / E1) * E1;
+    for_box(E1, tl, 0, E0, 0, E1, [&](int i0, int i1) {
       const int g0 = base0 + i0, g1 = base1 + i1;
       T val = T(0);
-      if (g0 >= 0 && g0 < a.n0 && g1 >= 0 && g1 < a.n1) val = a.u[v * a.N + (int64_t)g0 * a.n1 + g1];
-      U[v * S + p] = val;
-      if (ADJ) G[v * S + p] = T(0);
-    }
+      if (in_grid(g0, g1)) val = a.u[v * N + (int64_t)g0 * n1 + g1];
+      U[v * S + i0 * E1 + i1] = val;
+      if (ADJ) G[v * S + i0 * E1 + i1] = T(0);
+    });
   __syncthreads();
 
-  // forward chains
+  // forward chains: the valid box shrinks by dp-1 per application
   for (int fs = 0; fs < a.n_fs; ++fs) {
-    const int nax = a.fs_nax[fs];
-    const T *src = U + a.fs_var[fs] * S;
-    Box bx{0, E0, 0, E1};
-    if (nax == 0) {
-      for (int p = tid; p < S; p += MAT_THREADS) F[fs * S + p] = src[p];
-    }
+    const int32_t *Rf = a.fs + fs * MFS_INTS;
+    const int nax = Rf[1];
+    const T *src = U + Rf[0] * S;
+    int lo0 = 0, hi0 = E0, lo1 = 0, hi1 = E1;
+    if (nax == 0)
+      for_box(E1, tl, 0, E0, 0, E1, [&](int i0, int i1) {
+        F[fs * S + i0 * E1 + i1] = src[i0 * E1 + i1];
+      });
     for (int j = 0; j < nax; ++j) {
-      const int ax = a.fs_ax[fs][j];
+      const int ax = Rf[2 + j];
       T *dst = (j == nax - 1) ? F + fs * S : ((j & 1) ? B : A);
-      if (ax == 0) { bx.lo0 += a.dp - 1; bx.hi0 -= a.dp - 1; }
-      else { bx.lo1 += a.dp - 1; bx.hi1 -= a.dp - 1; }
-      const int w1 = bx.hi1 - bx.lo1, cnt = (bx.hi0 - bx.lo0) * w1;
-      for (int k = tid; k < cnt; k += MAT_THREADS) {
-        const int i0 = bx.lo0 + k / w1, i1 = bx.lo1 + (k - (k / w1) * w1);
-        const int g0 = base0 + i0, g1 = base1 + i1;
-        const int p = i0 * E1 + i1;
-        T val = T(0);
-        if (g0 >= 0 && g0 < a.n0 && g1 >= 0 && g1 < a.n1)
-          val = (ax == 0) ? dstencil<T, EXACT>(a, src, p, E1, g0, a.n0, 0)
-                          : dstencil<T, EXACT>(a, src, p, 1, g1, a.n1, 1);
-        dst[p] = val;
+      if (ax == 0) {
+        lo0 += dp - 1;
+        hi0 -= dp - 1;
+      } else {
+        lo1 += dp - 1;
+        hi1 -= dp - 1;
+      }
+      if (INT) {
+        const int st = ax == 0 ? E1 : 1;
+        const T h2 = ax == 0 ? a.h20 : a.h21, ih2 = ax == 0 ? a.ih20 : a.ih21;
+        for_box(E1, tl, lo0, hi0, lo1, hi1, [&](int i0, int i1) {
+          const int p = i0 * E1 + i1;
+          dst[p] = dstencil_in<T, DP, DIV>(src, p, st, dp, ab, af, h2, ih2);
+        });
+      } else if (ax == 0) {
+        for_box(E1, tl, lo0, hi0, lo1, hi1, [&](int i0, int i1) {
+          const int g0 = base0 + i0, g1 = base1 + i1, p = i0 * E1 + i1;
+          dst[p] = in_grid(g0, g1)
+                       ? dstencil<T, DP, DIV>(src, p, E1, g0, n0, dp, ab, af, a.h0, a.h20, a.ih0, a.ih20)
+                       : T(0);
+        });
+      } else {
+        for_box(E1, tl, lo0, hi0, lo1, hi1, [&](int i0, int i1) {
+          const int g0 = base0 + i0, g1 = base1 + i1, p = i0 * E1 + i1;
+          dst[p] = in_grid(g0, g1)
+                       ? dstencil<T, DP, DIV>(src, p, 1, g1, n1, dp, ab, af, a.h1, a.h21, a.ih1, a.ih21)
+                       : T(0);
+        });
       }
       __syncthreads();
       src = dst;
@@ -195,121 +370,133 @@ __global__ __launch_bounds__(MAT_THREADS) void k_mat_tile(MatArgs<T> a) {
   }
   __syncthreads();
 
-  // residual, loss, seeds on the tile plus R
+  // residual, loss and seeds on the tile plus R; each point is read and
+  // written by one thread, so a term's seeds overwrite its factor values
   T lsum = T(0), lnsum = T(0);
-  {
-    const Box rb{2 * a.R0 - a.R0, 2 * a.R0 + a.T0 + a.R0, 2 * a.R1 - a.R1, 2 * a.R1 + a.T1 + a.R1};
-    const int w1 = rb.hi1 - rb.lo1, cnt = (rb.hi0 - rb.lo0) * w1;
-    const T dl = ADJ ? *a.dloss : T(0);
-    for (int k = tid; k < cnt; k += MAT_THREADS) {
-      const int i0 = rb.lo0 + k / w1, i1 = rb.lo1 + (k - (k / w1) * w1);
-      const int g0 = base0 + i0, g1 = base1 + i1;
-      const int p = i0 * E1 + i1;
-      const bool in_grid = g0 >= 0 && g0 < a.n0 && g1 >= 0 && g1 < a.n1;
-      const bool in_tile = i0 >= 2 * a.R0 && i0 < 2 * a.R0 + a.T0 && i1 >= 2 * a.R1 &&
-                           i1 < 2 * a.R1 + a.T1;
-      const int64_t gp = (int64_t)g0 * a.n1 + g1;
-      T seed[MMAX_FS];
-#pragma unroll
-      for (int q = 0; q < MMAX_FS; ++q) seed[q] = T(0);
-      if (in_grid) {
-        int fs0 = 0;
-        for (int eq = 0; eq < a.n_eq; ++eq) {
-          const int32_t *E = a.ints + a.eq_i + eq * TDX_MEQ_INTS;
-          const int n_terms = E[0];
-          T r = T(0);
-          int fs = fs0;
-          for (int t = 0; t < n_terms; ++t) {
-            const int32_t *Tm = a.ints + E[1] + t * TDX_MTERM_INTS;
-            T der = T(1);
-            for (int f = 0; f < Tm[0]; ++f, ++fs) {
-              const int32_t *Fc = a.ints + Tm[1] + f * TDX_MFACTOR_INTS;
-              der = der * mfpow(a, Fc, F[fs * S + p]);
-            }
-            const T c = Tm[2] == 0 ? a.reals[Tm[3]] : a.data[Tm[3] + gp];
-            const T tv = c * der;
-            r = (t == 0) ? tv : r + tv;
-          }
-          const T lam_n = a.reals[E[2]];
-          if (in_tile) {
-            lsum += lam_n * (r * r);
-            lnsum += (r * r) / T(a.N);
-            if (a.op) a.op[eq * a.N + gp] = r;
-          }
-          if (ADJ) {
-            const T g = T(2) * lam_n * r * dl;
-            fs = fs0;
-            for (int t = 0; t < n_terms; ++t) {
-              const int32_t *Tm = a.ints + E[1] + t * TDX_MTERM_INTS;
-              const int nf = Tm[0];
-              const T c = Tm[2] == 0 ? a.reals[Tm[3]] : a.data[Tm[3] + gp];
-              for (int f = 0; f < nf; ++f) {
-                const int32_t *Fc = a.ints + Tm[1] + f * TDX_MFACTOR_INTS;
-                T d = c * g * mdfpow(a, Fc, F[(fs + f) * S + p]);
-                for (int f2 = 0; f2 < nf; ++f2)
-                  if (f2 != f)
-                    d *= mfpow(a, a.ints + Tm[1] + f2 * TDX_MFACTOR_INTS, F[(fs + f2) * S + p]);
-#pragma unroll
-                for (int q = 0; q < MMAX_FS; ++q)
-                  if (q == fs + f) seed[q] = d;
-              }
-              fs += nf;
-            }
-          }
-          fs0 = fs;
-        }
+  const T dl = ADJ ? *a.dloss : T(0);
+  for_box(E1, tl, R0, 3 * R0 + T0, R1, 3 * R1 + T1, [&](int i0, int i1) {
+    const int g0 = base0 + i0, g1 = base1 + i1, p = i0 * E1 + i1;
+    const bool ing = in_grid(g0, g1);
+    const bool in_tile = i0 >= 2 * R0 && i0 < 2 * R0 + T0 && i1 >= 2 * R1 && i1 < 2 * R1 + T1;
+    const int64_t gp = ing ? (int64_t)g0 * n1 + g1 : 0;
+    for (int eq = 0; eq < a.n_eq; ++eq) {
+      const int n_terms = PI[2 * eq], term0 = PI[2 * eq + 1];
+      const T lam_n = PR[eq];
+      T r = T(0);
+      for (int t = 0; t < n_terms; ++t) {
+        const int32_t *Tr = PI + a.pterm0 + 4 * (term0 + t);
+        const int nf = Tr[0], fs = Tr[3];
+        T der = T(1);
+        for (int f = 0; f < nf; ++f)
+          der = der * ppow(PI + a.pfac0 + 3 * (fs + f), PR, F[(fs + f) * S + p]);
+        const T c = term_coeff(Tr, PR, a.data, gp);
+        const T tv = c * der;
+        r = (t == 0) ? tv : r + tv;
       }
-      // only this thread reads F at p in this pass, so the seeds can
-      // overwrite the factor values in place
+      if (ing && in_tile) {
+        lsum += lam_n * (r * r);
+        lnsum += (r * r) / T(N);
+        if (a.op) a.op[eq * N + gp] = r;
+      }
       if (ADJ) {
-        for (int q = 0; q < a.n_fs; ++q) F[q * S + p] = seed[q];
+        const T g = T(2) * lam_n * r * dl;
+        for (int t = 0; t < n_terms; ++t) {
+          const int32_t *Tr = PI + a.pterm0 + 4 * (term0 + t);
+          const int nf = Tr[0], fs = Tr[3];
+          const T c = term_coeff(Tr, PR, a.data, gp);
+          T fv[MMAX_TF], pw[MMAX_TF];
+#pragma unroll
+          for (int f = 0; f < MMAX_TF; ++f) {
+            fv[f] = f < nf ? F[(fs + f) * S + p] : T(0);
+            pw[f] = f < nf ? ppow(PI + a.pfac0 + 3 * (fs + f), PR, fv[f]) : T(1);
+          }
+#pragma unroll
+          for (int f = 0; f < MMAX_TF; ++f) {
+            if (f < nf) {
+              T d = c * g * pdpow(PI + a.pfac0 + 3 * (fs + f), PR, fv[f]);
+#pragma unroll
+              for (int f2 = 0; f2 < MMAX_TF; ++f2)
+                if (f2 != f && f2 < nf) d *= pw[f2];
+              F[(fs + f) * S + p] = ing ? d : T(0);
+            }
+          }
+        }
       }
     }
-  }
-  __syncthreads();
+  });
 
   if (ADJ) {
-    // transposed chains back to the tile
+    __syncthreads();
+    // transposed chains from the tile plus R back to the tile; the last
+    // application accumulates into G
     for (int fs = 0; fs < a.n_fs; ++fs) {
-      const int nax = a.fs_nax[fs];
+      const int32_t *Rf = a.fs + fs * MFS_INTS;
+      const int nax = Rf[1];
+      T *Gv = G + Rf[0] * S;
       const T *src = F + fs * S;
-      // seeds are valid on the tile plus R
-      Box bx{a.R0, 3 * a.R0 + a.T0, a.R1, 3 * a.R1 + a.T1};
+      int lo0 = R0, hi0 = 3 * R0 + T0, lo1 = R1, hi1 = 3 * R1 + T1;
+      if (nax == 0)
+        for_box(E1, tl, lo0, hi0, lo1, hi1, [&](int i0, int i1) {
+          Gv[i0 * E1 + i1] += src[i0 * E1 + i1];
+        });
       for (int j = nax - 1; j >= 0; --j) {
-        const int ax = a.fs_ax[fs][j];
+        const int ax = Rf[2 + j];
         T *dst = ((nax - 1 - j) & 1) ? B : A;
-        if (ax == 0) { bx.lo0 += a.dp - 1; bx.hi0 -= a.dp - 1; }
-        else { bx.lo1 += a.dp - 1; bx.hi1 -= a.dp - 1; }
-        const int w1 = bx.hi1 - bx.lo1, cnt = (bx.hi0 - bx.lo0) * w1;
-        for (int k = tid; k < cnt; k += MAT_THREADS) {
-          const int i0 = bx.lo0 + k / w1, i1 = bx.lo1 + (k - (k / w1) * w1);
-          const int g0 = base0 + i0, g1 = base1 + i1;
-          const int p = i0 * E1 + i1;
-          T val = T(0);
-          if (g0 >= 0 && g0 < a.n0 && g1 >= 0 && g1 < a.n1)
-            val = (ax == 0) ? dtstencil<T, EXACT>(a, src, p, E1, g0, a.n0, 0)
-                            : dtstencil<T, EXACT>(a, src, p, 1, g1, a.n1, 1);
-          dst[p] = val;
+        const bool last = j == 0;
+        if (ax == 0) {
+          lo0 += dp - 1;
+          hi0 -= dp - 1;
+        } else {
+          lo1 += dp - 1;
+          hi1 -= dp - 1;
         }
+        if (INT && ax == 0) {
+          for_box(E1, tl, lo0, hi0, lo1, hi1, [&](int i0, int i1) {
+            const int p = i0 * E1 + i1;
+            const T val = dtstencil_in<T, DP>(src, p, E1, dp, tc0, cs);
+            if (last)
+              Gv[p] += val;
+            else
+              dst[p] = val;
+          });
+        } else if (INT) {
+          for_box(E1, tl, lo0, hi0, lo1, hi1, [&](int i0, int i1) {
+            const int p = i0 * E1 + i1;
+            const T val = dtstencil_in<T, DP>(src, p, 1, dp, tc1, cs);
+            if (last)
+              Gv[p] += val;
+            else
+              dst[p] = val;
+          });
+        } else if (ax == 0)
+          for_box(E1, tl, lo0, hi0, lo1, hi1, [&](int i0, int i1) {
+            const int g0 = base0 + i0, g1 = base1 + i1, p = i0 * E1 + i1;
+            const T val = in_grid(g0, g1) ? dtstencil<T, DP>(src, p, E1, g0, n0, dp, tc0, cs) : T(0);
+            if (last)
+              Gv[p] += val;
+            else
+              dst[p] = val;
+          });
+        else
+          for_box(E1, tl, lo0, hi0, lo1, hi1, [&](int i0, int i1) {
+            const int g0 = base0 + i0, g1 = base1 + i1, p = i0 * E1 + i1;
+            const T val = in_grid(g0, g1) ? dtstencil<T, DP>(src, p, 1, g1, n1, dp, tc1, cs) : T(0);
+            if (last)
+              Gv[p] += val;
+            else
+              dst[p] = val;
+          });
         __syncthreads();
         src = dst;
       }
-      // accumulate on the tile
-      T *Gv = G + a.fs_var[fs] * S;
-      for (int k = tid; k < a.T0 * a.T1; k += MAT_THREADS) {
-        const int i0 = 2 * a.R0 + k / a.T1, i1 = 2 * a.R1 + (k - (k / a.T1) * a.T1);
-        const int p = i0 * E1 + i1;
-        Gv[p] += src[p];
-      }
-      __syncthreads();
+      if (nax == 0) __syncthreads();
     }
     for (int v = 0; v < a.n_eq; ++v)
-      for (int k = tid; k < a.T0 * a.T1; k += MAT_THREADS) {
-        const int i0 = 2 * a.R0 + k / a.T1, i1 = 2 * a.R1 + (k - (k / a.T1) * a.T1);
+      for_box(E1, tl, 2 * R0, 2 * R0 + T0, 2 * R1, 2 * R1 + T1, [&](int i0, int i1) {
         const int g0 = base0 + i0, g1 = base1 + i1;
-        if (g0 < a.n0 && g1 < a.n1)
-          a.grad[v * a.N + (int64_t)g0 * a.n1 + g1] = G[v * S + i0 * E1 + i1];
-      }
+        if (INT || (g0 < n0 && g1 < n1))
+          a.grad[v * N + (int64_t)g0 * n1 + g1] = G[v * S + i0 * E1 + i1];
+      });
   }
 
   // deterministic workgroup sum of the loss partials
@@ -329,10 +516,23 @@ __global__ __launch_bounds__(MAT_THREADS) void k_mat_tile(MatArgs<T> a) {
   }
 }
 
+template <typename T, int DP, int DIV, bool ADJ>
+__global__ __launch_bounds__(MAT_THREADS) void k_mat_tile(MatArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  const int t0 = blockIdx.x / a.tiles1, t1 = blockIdx.x - (blockIdx.x / a.tiles1) * a.tiles1;
+  const int base0 = t0 * a.T0 - 2 * a.R0, base1 = t1 * a.T1 - 2 * a.R1;
+  const bool interior = base0 >= 0 && base0 + a.E0 <= a.n0 && base1 >= 0 && base1 + a.E1 <= a.n1;
+  if (interior)
+    mat_tile_body<T, DP, DIV, ADJ, true>(a, smem, t0, t1);
+  else
+    mat_tile_body<T, DP, DIV, ADJ, false>(a, smem, t0, t1);
+}
+
 // ---------------------------------------------------------------------------
 // boundary entries: pointwise derivative chains straight from HBM
 // ---------------------------------------------------------------------------
-template <typename T, bool EXACT>
+template <typename T, int DIV>
 __device__ T point_chain(const MatArgs<T> &a, const T *plane, int g0, int g1, const int *axes,
                          int depth) {
   // D_{axes[depth-1]} ... D_{axes[0]} u at (g0, g1), evaluated recursively
@@ -343,22 +543,25 @@ __device__ T point_chain(const MatArgs<T> &a, const T *plane, int g0, int g1, co
   const int dp = a.dp;
   auto at = [&](int off) {
     const int h0 = ax == 0 ? g0 + off : g0, h1 = ax == 0 ? g1 : g1 + off;
-    return point_chain<T, EXACT>(a, plane, h0, h1, axes, depth - 1);
+    return point_chain<T, DIV>(a, plane, h0, h1, axes, depth - 1);
   };
+  const T *ab = a.coef, *af = a.coef + MMAX_DP;
+  const T h = ax == 0 ? a.h0 : a.h1, ih = ax == 0 ? a.ih0 : a.ih1;
+  const T h2 = ax == 0 ? a.h20 : a.h21, ih2 = ax == 0 ? a.ih20 : a.ih21;
   if (g < dp - 1) {
-    T f = T(0);
-    for (int i = 0; i < dp; ++i) f = f + at(i) * a.af[i];
-    return qdiv<T, EXACT>(f, a.h[ax], a.ih[ax]);
+    T f = at(0) * af[0];
+    for (int i = 1; i < dp; ++i) f = f + at(i) * af[i];
+    return qdiv<T, DIV>(f, h, ih);
   }
-  T b = T(0);
-  for (int i = 0; i < dp; ++i) b = b + at(i - (dp - 1)) * a.ab[i];
-  if (g >= n - dp + 1) return qdiv<T, EXACT>(b, a.h[ax], a.ih[ax]);
-  T f = T(0);
-  for (int i = 0; i < dp; ++i) f = f + at(i) * a.af[i];
-  return qdiv<T, EXACT>(b + f, a.h2[ax], a.ih2[ax]);
+  T b = at(-(dp - 1)) * ab[0];
+  for (int i = 1; i < dp; ++i) b = b + at(i - (dp - 1)) * ab[i];
+  if (g >= n - dp + 1) return qdiv<T, DIV>(b, h, ih);
+  T f = at(0) * af[0];
+  for (int i = 1; i < dp; ++i) f = f + at(i) * af[i];
+  return qdiv<T, DIV>(b + f, h2, ih2);
 }
 
-template <typename T, bool EXACT>
+template <typename T, int DIV>
 __global__ __launch_bounds__(MAT_THREADS) void k_mat_bc(MatArgs<T> a, int adj) {
   __shared__ T red[2 * MAT_THREADS];
   const int tid = threadIdx.x;
@@ -385,7 +588,7 @@ __global__ __launch_bounds__(MAT_THREADS) void k_mat_bc(MatArgs<T> a, int adj) {
           for (int f = 0; f < Tm[0]; ++f) {
             const int32_t *Fc = a.ints + Tm[1] + f * TDX_MFACTOR_INTS;
             const int ax[3] = {Fc[5], Fc[6], Fc[7]};
-            const T fv = point_chain<T, EXACT>(a, a.u + Fc[0] * a.N, g0, g1, ax, Fc[4]);
+            const T fv = point_chain<T, DIV>(a, a.u + Fc[0] * a.N, g0, g1, ax, Fc[4]);
             der = der * mfpow(a, Fc, fv);
           }
           const T c = Tm[2] == 0 ? a.reals[Tm[3]] : a.data[Tm[3] + pos];
@@ -419,12 +622,12 @@ __global__ __launch_bounds__(MAT_THREADS) void k_mat_bc(MatArgs<T> a, int adj) {
             const int32_t *Fc = a.ints + Tm[1] + f * TDX_MFACTOR_INTS;
             const int ax[3] = {Fc[5], Fc[6], Fc[7]};
             T d = c * sg *
-                  mdfpow(a, Fc, point_chain<T, EXACT>(a, a.u + Fc[0] * a.N, g0, g1, ax, Fc[4]));
+                  mdfpow(a, Fc, point_chain<T, DIV>(a, a.u + Fc[0] * a.N, g0, g1, ax, Fc[4]));
             for (int f2 = 0; f2 < nf; ++f2) {
               if (f2 == f) continue;
               const int32_t *F2 = a.ints + Tm[1] + f2 * TDX_MFACTOR_INTS;
               const int ax2[3] = {F2[5], F2[6], F2[7]};
-              d *= mfpow(a, F2, point_chain<T, EXACT>(a, a.u + F2[0] * a.N, g0, g1, ax2, F2[4]));
+              d *= mfpow(a, F2, point_chain<T, DIV>(a, a.u + F2[0] * a.N, g0, g1, ax2, F2[4]));
             }
             a.bc_seeds[sidx++] = d;
           }
@@ -462,11 +665,14 @@ __global__ void k_mat_bc_adj(MatArgs<T> a) {
   a.grad[pos] += s;
 }
 
+// Sum of n (loss, normalized loss) pairs: workgroup b of the grid adds the
+// pairs b, b + G, b + 2G, ... (G = grid size) in a fixed order and writes one
+// pair to out[b]; a second launch with one workgroup adds those.
 template <typename T>
 __global__ void k_mat_loss(const T *lp, int n, T *loss) {
   __shared__ T red[2 * 256];
   T s0 = T(0), s1 = T(0);
-  for (int k = threadIdx.x; k < n; k += 256) {
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += 256 * gridDim.x) {
     s0 += lp[2 * k];
     s1 += lp[2 * k + 1];
   }
@@ -481,8 +687,8 @@ __global__ void k_mat_loss(const T *lp, int n, T *loss) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    loss[0] = red[0];
-    loss[1] = red[256];
+    loss[2 * blockIdx.x] = red[0];
+    loss[2 * blockIdx.x + 1] = red[256];
   }
 }
 
@@ -498,16 +704,49 @@ using namespace tdx;
 struct tdx_mat_plan {
   tdx_mat_desc d;
   int n_fs;
-  int fs_var[MMAX_FS], fs_nax[MMAX_FS], fs_ax[MMAX_FS][3];
-  int R0, R1, T0, T1, tiles0, tiles1, n_tile_wg, n_bc_wg;
+  int R0, R1, T0, T1, E0, E1, tpr_log, tiles0, tiles1, n_tile_wg, n_bc_wg;
   size_t tile_lds, tile_lds_fwd;
-  DevTable ints, reals, data;
-  size_t off_lp, off_seeds, off_one, off_grad, ws_bytes;
+  DevTable ints, reals, data, fs_tab, coef_tab, pint_tab, preal_tab;
+  int n_pint, n_preal, pterm0, pfac0;
+  int div_checked;  // float quotients by the correction step are exact
+  size_t off_lp, off_mid, off_seeds, off_one, off_grad, ws_bytes;
   double ab[MMAX_DP], af[MMAX_DP];
 };
 
 
 namespace {
+
+// True when, for the float step d and y = RN(1/d), RN(RN(x y) + (x - RN(x y) d) y)
+// equals the correctly rounded x / d for every x in [1, 2) (checked one by
+// one).  Scaling x or d by a power of two scales every step exactly, so the
+// result then holds for all x in the range qdiv guards, given 2^-30 <= d <=
+// 2^30.  The reference quotient is computed in double and rounded once to
+// float, which is the correctly rounded float quotient (53 >= 2*24 + 2).
+bool quotient_step_exact(float d) {
+  static std::mutex mu;
+  static std::map<uint32_t, bool> cache;
+  uint32_t key;
+  std::memcpy(&key, &d, 4);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  const float y = (float)(1.0 / (double)d);
+  bool ok = std::isfinite(d) && d >= 0x1p-30f && d <= 0x1p30f;
+  for (uint32_t m = 0; ok && m < (1u << 23); ++m) {
+    const uint32_t bits = 0x3f800000u | m;
+    float x;
+    std::memcpy(&x, &bits, 4);
+    const float q = x * y;
+    const float r = std::fma(-q, d, x);
+    const float q2 = std::fma(r, y, q);
+    ok = q2 == (float)((double)x / (double)d);
+  }
+  std::lock_guard<std::mutex> g(mu);
+  cache[key] = ok;
+  return ok;
+}
 
 template <typename T>
 MatArgs<T> mat_args(const tdx_mat_plan *p, const void *u, void *ws) {
@@ -518,30 +757,34 @@ MatArgs<T> mat_args(const tdx_mat_plan *p, const void *u, void *ws) {
   a.n1 = d.n1;
   a.dp = d.dp;
   a.N = (int64_t)d.n0 * d.n1;
-  for (int k = 0; k < 2; ++k) {
-    a.h[k] = T(d.h[k]);
-    a.h2[k] = T(2.0 * d.h[k]);
-    a.ih[k] = T(1.0 / d.h[k]);
-    a.ih2[k] = T(1.0 / (2.0 * d.h[k]));
-  }
-  for (int i = 0; i < MMAX_DP; ++i) {
-    a.ab[i] = i < d.dp ? T(p->ab[i]) : T(0);
-    a.af[i] = i < d.dp ? T(p->af[i]) : T(0);
-  }
+  a.h0 = T(d.h[0]);
+  a.h1 = T(d.h[1]);
+  a.h20 = T(2.0 * d.h[0]);
+  a.h21 = T(2.0 * d.h[1]);
+  a.ih0 = T(1.0 / d.h[0]);
+  a.ih1 = T(1.0 / d.h[1]);
+  a.ih20 = T(1.0 / (2.0 * d.h[0]));
+  a.ih21 = T(1.0 / (2.0 * d.h[1]));
+  a.coef = (const T *)p->coef_tab.ptr;
+  a.fs = (const int32_t *)p->fs_tab.ptr;
+  a.pint = (const int32_t *)p->pint_tab.ptr;
+  a.preal = (const T *)p->preal_tab.ptr;
+  a.n_pint = p->n_pint;
+  a.n_preal = p->n_preal;
+  a.pterm0 = p->pterm0;
+  a.pfac0 = p->pfac0;
   a.ints = (const int32_t *)p->ints.ptr;
   a.reals = (const T *)p->reals.ptr;
   a.data = (const T *)p->data.ptr;
   a.eq_i = d.eq_i;
   a.n_fs = p->n_fs;
-  for (int q = 0; q < MMAX_FS; ++q) {
-    a.fs_var[q] = p->fs_var[q];
-    a.fs_nax[q] = p->fs_nax[q];
-    for (int k = 0; k < 3; ++k) a.fs_ax[q][k] = p->fs_ax[q][k];
-  }
   a.R0 = p->R0;
   a.R1 = p->R1;
   a.T0 = p->T0;
   a.T1 = p->T1;
+  a.E0 = p->E0;
+  a.E1 = p->E1;
+  a.tpr_log = p->tpr_log;
   a.tiles1 = p->tiles1;
   a.n_bc = d.n_bc;
   a.bc_i = d.bc_i;
@@ -556,31 +799,41 @@ MatArgs<T> mat_args(const tdx_mat_plan *p, const void *u, void *ws) {
   a.bval = nullptr;
   char *w = (char *)ws;
   a.lp = (T *)(w + p->off_lp);
+  a.lp_mid = (T *)(w + p->off_mid);
   a.bc_seeds = (T *)(w + p->off_seeds);
   a.dloss = (const T *)(w + p->off_one);
   a.n_tile_wg = p->n_tile_wg;
   return a;
 }
 
-template <typename T, bool EXACT>
-int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t s) {
+template <typename T, int DP, int DIV>
+int mat_tile_launch(const tdx_mat_plan *p, const MatArgs<T> &a, bool adj, hipStream_t s) {
   static bool attr = false;
   if (!attr) {
-    (void)hipFuncSetAttribute((const void *)k_mat_tile<T, EXACT, true>,
+    (void)hipFuncSetAttribute((const void *)k_mat_tile<T, DP, DIV, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void *)k_mat_tile<T, EXACT, false>,
+    (void)hipFuncSetAttribute((const void *)k_mat_tile<T, DP, DIV, false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr = true;
   }
   if (adj)
-    hipLaunchKernelGGL((k_mat_tile<T, EXACT, true>), dim3(p->n_tile_wg), dim3(MAT_THREADS),
+    hipLaunchKernelGGL((k_mat_tile<T, DP, DIV, true>), dim3(p->n_tile_wg), dim3(MAT_THREADS),
                        p->tile_lds, s, a);
   else
-    hipLaunchKernelGGL((k_mat_tile<T, EXACT, false>), dim3(p->n_tile_wg), dim3(MAT_THREADS),
+    hipLaunchKernelGGL((k_mat_tile<T, DP, DIV, false>), dim3(p->n_tile_wg), dim3(MAT_THREADS),
                        p->tile_lds_fwd, s, a);
   TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}
+
+template <typename T, int DIV>
+int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t s) {
+  int rc = p->d.dp == 2   ? mat_tile_launch<T, 2, DIV>(p, a, adj, s)
+           : p->d.dp == 3 ? mat_tile_launch<T, 3, DIV>(p, a, adj, s)
+                          : mat_tile_launch<T, 0, DIV>(p, a, adj, s);
+  if (rc) return rc;
   if (p->n_bc_wg > 0) {
-    hipLaunchKernelGGL((k_mat_bc<T, EXACT>), dim3(p->n_bc_wg), dim3(MAT_THREADS), 0, s, a,
+    hipLaunchKernelGGL((k_mat_bc<T, DIV>), dim3(p->n_bc_wg), dim3(MAT_THREADS), 0, s, a,
                        adj ? 1 : 0);
     TDX_TRY_HIP(hipGetLastError());
     if (adj && p->d.n_touch > 0) {
@@ -590,8 +843,16 @@ int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t 
     }
   }
   if (loss) {
-    hipLaunchKernelGGL((k_mat_loss<T>), dim3(1), dim3(256), 0, s, a.lp,
-                       p->n_tile_wg + p->n_bc_wg, loss);
+    const int n = p->n_tile_wg + p->n_bc_wg;
+    const int g = std::min(256, (n + 4095) / 4096);
+    T *mid = a.lp_mid;
+    if (g > 1) {
+      hipLaunchKernelGGL((k_mat_loss<T>), dim3(g), dim3(256), 0, s, a.lp, n, mid);
+      TDX_TRY_HIP(hipGetLastError());
+      hipLaunchKernelGGL((k_mat_loss<T>), dim3(1), dim3(256), 0, s, (const T *)mid, g, loss);
+    } else {
+      hipLaunchKernelGGL((k_mat_loss<T>), dim3(1), dim3(256), 0, s, a.lp, n, loss);
+    }
     TDX_TRY_HIP(hipGetLastError());
   }
   return TDX_OK;
@@ -599,8 +860,9 @@ int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t 
 
 template <typename T>
 int mat_dispatch(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t s) {
-  if (p->d.exact_div) return mat_run<T, true>(p, a, adj, loss, s);
-  return mat_run<T, false>(p, a, adj, loss, s);
+  if (!p->d.exact_div) return mat_run<T, 0>(p, a, adj, loss, s);
+  if (sizeof(T) == 4 && p->div_checked) return mat_run<T, 2>(p, a, adj, loss, s);
+  return mat_run<T, 1>(p, a, adj, loss, s);
 }
 
 }  // namespace
@@ -612,6 +874,7 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   TDX_REQUIRE(d->n_eq >= 1 && d->n_eq <= MMAX_EQ, "n_eq must be 1..%d", MMAX_EQ);
   TDX_REQUIRE(d->n0 >= 1 && d->n1 >= 2, "bad grid shape");
   TDX_REQUIRE(d->dp >= 2 && d->dp <= MMAX_DP, "derivative_points must be 2..%d", MMAX_DP);
+  TDX_REQUIRE(d->tile_rows >= 0 && d->tile_rows <= 1024, "tile_rows must be 0..1024");
   auto in_ints = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_ints; };
   auto in_reals = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_reals; };
   auto in_data = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_data; };
@@ -626,22 +889,47 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   }
   int cnt0 = 0, cnt1 = 0;
   p->n_fs = 0;
+  std::vector<int32_t> fs_rec;
+  // flattened residual program (see MatArgs)
+  std::vector<int32_t> prog_i, prog_t, prog_f;
+  std::vector<double> prog_r;
+  auto real_at = [&](int64_t k) {
+    return d->dtype == 0 ? (double)((const float *)d->reals)[k] : ((const double *)d->reals)[k];
+  };
   auto fail = [&](const char *msg) {
     delete p;
     set_error("%s", msg);
     return TDX_EINVAL;
   };
   if (!in_ints(d->eq_i, (int64_t)d->n_eq * TDX_MEQ_INTS)) return fail("eq table out of range");
+  int n_terms_before = 0;
   for (int q = 0; q < d->n_eq; ++q) {
     const int32_t *E = d->ints + d->eq_i + q * TDX_MEQ_INTS;
     if (E[0] < 1 || !in_ints(E[1], (int64_t)E[0] * TDX_MTERM_INTS) || !in_reals(E[2], 1))
       return fail("term table out of range");
+    prog_i.push_back(E[0]);
+    prog_i.push_back(n_terms_before);
+    n_terms_before += E[0];
+    prog_r.push_back(real_at(E[2]));
+  }
+  for (int q = 0; q < d->n_eq; ++q) {
+    const int32_t *E = d->ints + d->eq_i + q * TDX_MEQ_INTS;
     for (int t = 0; t < E[0]; ++t) {
       const int32_t *Tm = d->ints + E[1] + t * TDX_MTERM_INTS;
       if (Tm[0] < 1 || !in_ints(Tm[1], (int64_t)Tm[0] * TDX_MFACTOR_INTS))
         return fail("factor table out of range");
+      if (Tm[0] > MMAX_TF) return fail("at most 4 factors per term");
       if (!((Tm[2] == 0 && in_reals(Tm[3], 1)) || (Tm[2] == 1 && in_data(Tm[3], N))))
         return fail("coefficient out of range");
+      prog_t.push_back(Tm[0]);
+      prog_t.push_back(Tm[2]);
+      if (Tm[2] == 0) {
+        prog_t.push_back((int)prog_r.size());
+        prog_r.push_back(real_at(Tm[3]));
+      } else {
+        prog_t.push_back(Tm[3]);
+      }
+      prog_t.push_back(p->n_fs);
       for (int f = 0; f < Tm[0]; ++f) {
         const int32_t *F = d->ints + Tm[1] + f * TDX_MFACTOR_INTS;
         if (F[0] < 0 || F[0] >= d->n_eq) return fail("factor variable out of range");
@@ -658,9 +946,17 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
         if (c1 > 0 && d->n1 < 2 * d->dp - 1) return fail("axis 1 too short for the stencil");
         cnt0 = std::max(cnt0, c0);
         cnt1 = std::max(cnt1, c1);
-        p->fs_var[p->n_fs] = F[0];
-        p->fs_nax[p->n_fs] = F[4];
-        for (int k = 0; k < 3; ++k) p->fs_ax[p->n_fs][k] = F[5 + k];
+        prog_f.push_back(F[1]);
+        prog_f.push_back(F[2]);
+        if (F[1] != 0) {
+          prog_f.push_back((int)prog_r.size());
+          prog_r.push_back(real_at(F[3]));
+        } else {
+          prog_f.push_back(0);
+        }
+        fs_rec.push_back(F[0]);
+        fs_rec.push_back(F[4]);
+        for (int k = 0; k < 3; ++k) fs_rec.push_back(k < F[4] ? F[5 + k] : 0);
         ++p->n_fs;
       }
     }
@@ -715,32 +1011,90 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
       if (sidx < 0 || sidx >= d->n_bc_seeds || !in_reals(widx, 1)) return fail("bc adjoint entry");
     }
   }
+  // equation records, then term records, then factor records
+  p->pterm0 = (int)prog_i.size();
+  prog_i.insert(prog_i.end(), prog_t.begin(), prog_t.end());
+  p->pfac0 = (int)prog_i.size();
+  prog_i.insert(prog_i.end(), prog_f.begin(), prog_f.end());
+  if (prog_r.size() % 2) prog_r.push_back(0.0);  // keep the ints 8-byte aligned
+  p->n_pint = (int)prog_i.size();
+  p->n_preal = (int)prog_r.size();
   p->R0 = cnt0 * (d->dp - 1);
   p->R1 = cnt1 * (d->dp - 1);
   const size_t ts = d->dtype == 0 ? 4 : 8;
-  // tile shape: 1-D grids use one long row
-  if (d->n0 == 1) {
-    p->T0 = 1;
-    p->T1 = 1024;
-  } else {
-    p->T0 = 16;
-    p->T1 = 64;
-  }
-  auto lds_for = [&](int T0, int T1, bool adj) {
-    const size_t S = (size_t)(T0 + 4 * p->R0) * (T1 + 4 * p->R1);
-    return (S * (d->n_eq + p->n_fs + 2 + (adj ? d->n_eq : 0)) + 2 * MAT_THREADS) * ts;
+  // Tile shape.  A row of the tile with its halo is E1 = 64 * 2^k values
+  // wide (one coalesced wavefront access per 64), at least 32 of them
+  // output; 1-D grids use one row of 1024.  The number of rows is the
+  // largest that keeps the tile within about 48 KB of LDS (several
+  // workgroups per CU), with at least 8 output rows.
+  const size_t words = d->n_eq + p->n_fs + 2 + d->n_eq;  // U, F, A, B, G per point
+  auto lds_for = [&](int E0, int E1, bool adj) {
+    return ((size_t)E0 * E1 * (words - (adj ? 0 : d->n_eq)) + 2 * MAT_THREADS + MCOEF_N +
+            prog_r.size()) * ts + prog_i.size() * 4;
   };
-  while (lds_for(p->T0, p->T1, true) > 160 * 1024 && p->T1 > 16) p->T1 /= 2;
-  while (lds_for(p->T0, p->T1, true) > 160 * 1024 && p->T0 > 1) p->T0 /= 2;
-  if (lds_for(p->T0, p->T1, true) > 160 * 1024) return fail("stencil reach too large for LDS");
-  p->tile_lds = lds_for(p->T0, p->T1, true);
-  p->tile_lds_fwd = lds_for(p->T0, p->T1, false);
+  if (d->n0 == 1) {
+    p->E0 = 1;
+    p->E1 = 1024;
+    if (p->E1 - 4 * p->R1 < 256) p->E1 = 2048;
+  } else {
+    p->E1 = 64;
+    while (p->E1 - 4 * p->R1 < 32) p->E1 *= 2;
+  }
+  p->tpr_log = 0;
+  while ((1 << (p->tpr_log + 1)) <= std::min(p->E1, MAT_THREADS)) ++p->tpr_log;
+  const int rps = MAT_THREADS >> p->tpr_log;
+  if (d->n0 > 1 && d->tile_rows > 0) {
+    // requested output rows, rounded up so the tile rows fill whole sweeps
+    p->E0 = (4 * p->R0 + d->tile_rows + rps - 1) / rps * rps;
+  } else if (d->n0 > 1) {
+    const size_t fixed = (2 * MAT_THREADS + MCOEF_N + prog_r.size()) * ts + prog_i.size() * 4;
+    int e0 = (int)((48 * 1024 - std::min<size_t>(fixed, 32 * 1024)) / ((size_t)p->E1 * words * ts));
+    e0 = std::min(e0, d->n0 + 4 * p->R0);
+    e0 = e0 / rps * rps;
+    if (e0 - 4 * p->R0 < 8) e0 = (4 * p->R0 + 8 + rps - 1) / rps * rps;
+    p->E0 = e0;
+  }
+  p->T0 = p->E0 - 4 * p->R0;
+  p->T1 = p->E1 - 4 * p->R1;
+  if (p->T0 < 1 || p->T1 < 1 || lds_for(p->E0, p->E1, true) > 160 * 1024)
+    return fail("stencil reach too large for LDS");
+  p->div_checked = 0;
+  if (d->dtype == 0 && d->exact_div)
+    p->div_checked = quotient_step_exact((float)d->h[1]) &&
+                     (d->n0 == 1 || quotient_step_exact((float)d->h[0]));
+  p->tile_lds = lds_for(p->E0, p->E1, true);
+  p->tile_lds_fwd = lds_for(p->E0, p->E1, false);
   p->tiles0 = (d->n0 + p->T0 - 1) / p->T0;
   p->tiles1 = (d->n1 + p->T1 - 1) / p->T1;
   p->n_tile_wg = p->tiles0 * p->tiles1;
   p->n_bc_wg = (d->n_bc + MAT_THREADS - 1) / MAT_THREADS;
 
+  // stencil coefficients and their transposed products, in T
+  std::vector<double> coef(MCOEF_N, 0.0);
+  for (int i = 0; i < d->dp; ++i) {
+    coef[i] = p->ab[i];
+    coef[MMAX_DP + i] = p->af[i];
+    for (int ax = 0; ax < 2; ++ax) {
+      double *c = coef.data() + MCOEF_AX + ax * 4 * MMAX_DP;
+      c[i] = p->ab[i] / (2.0 * d->h[ax]);
+      c[MMAX_DP + i] = p->af[i] / (2.0 * d->h[ax]);
+      c[2 * MMAX_DP + i] = p->ab[i] / d->h[ax];
+      c[3 * MMAX_DP + i] = p->af[i] / d->h[ax];
+    }
+  }
+  std::vector<float> coef_f(coef.begin(), coef.end());
+  if (fs_rec.empty()) fs_rec.assign(MFS_INTS, 0);
   int rc = p->ints.upload(d->ints, (size_t)std::max<int64_t>(d->n_ints, 1) * 4);
+  if (!rc) rc = p->fs_tab.upload(fs_rec.data(), fs_rec.size() * 4);
+  if (!rc) rc = p->pint_tab.upload(prog_i.data(), prog_i.size() * 4);
+  if (!rc) {
+    std::vector<float> prf(prog_r.begin(), prog_r.end());
+    rc = d->dtype == 0 ? p->preal_tab.upload(prf.data(), prf.size() * 4)
+                       : p->preal_tab.upload(prog_r.data(), prog_r.size() * 8);
+  }
+  if (!rc)
+    rc = d->dtype == 0 ? p->coef_tab.upload(coef_f.data(), coef_f.size() * 4)
+                       : p->coef_tab.upload(coef.data(), coef.size() * 8);
   if (!rc) rc = p->reals.upload(d->reals, (size_t)d->n_reals * ts);
   if (!rc) rc = p->data.upload(d->data, (size_t)std::max<int64_t>(d->n_data, 1) * ts);
   if (rc) {
@@ -759,6 +1113,8 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   off = align_up(off + (size_t)std::max(d->n_bc_seeds, 1) * ts, 256);
   p->off_one = off;
   off = align_up(off + ts, 256);
+  p->off_mid = off;
+  off = align_up(off + 2 * 256 * ts, 256);
   p->off_grad = off;
   off = align_up(off + (size_t)N * d->n_eq * ts, 256);
   p->ws_bytes = off;
@@ -771,6 +1127,10 @@ extern "C" void tdx_mat_plan_destroy(tdx_mat_plan *p) {
   p->ints.release();
   p->reals.release();
   p->data.release();
+  p->fs_tab.release();
+  p->coef_tab.release();
+  p->pint_tab.release();
+  p->preal_tab.release();
   delete p;
 }
 

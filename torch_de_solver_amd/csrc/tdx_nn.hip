@@ -950,19 +950,36 @@ struct AdamHyper {
 
 // torch.optim.Adam (single-tensor form): m = b1 m + (1-b1) g;
 // v = b2 v + (1-b2) g^2; p -= lr/(1-b1^t) * m / (sqrt(v)/sqrt(1-b2^t) + eps)
+// The per-step scalars (bias corrections in double, as torch computes them
+// in Python floats) depend only on the step t.
+template <typename T>
+struct AdamScalars {
+  T w1, w2, b2, step_size, bc2s, eps;
+};
+template <typename T>
+__device__ __forceinline__ AdamScalars<T> adam_scalars(int64_t t, const AdamHyper &h) {
+  AdamScalars<T> s;
+  s.b2 = T(h.b2);
+  s.w1 = T(1.0 - h.b1);
+  s.w2 = T(1.0 - h.b2);
+  const double bc1 = 1.0 - pow(h.b1, (double)t);
+  const double bc2 = 1.0 - pow(h.b2, (double)t);
+  s.step_size = T(h.lr / bc1);
+  s.bc2s = T(sqrt(bc2));
+  s.eps = T(h.eps);
+  return s;
+}
+template <typename T>
+__device__ __forceinline__ void adam_apply(T &p, T g, T &m, T &v, const AdamScalars<T> &s) {
+  m = m + s.w1 * (g - m);
+  v = v * s.b2 + s.w2 * g * g;
+  const T denom = sqrt(v) / s.bc2s + s.eps;
+  p = p - s.step_size * (m / denom);
+}
 template <typename T>
 __device__ __forceinline__ void adam_update(T &p, T g, T &m, T &v, int64_t t,
                                             const AdamHyper &h) {
-  const T b2 = T(h.b2);
-  const T w1 = T(1.0 - h.b1), w2 = T(1.0 - h.b2);
-  m = m + w1 * (g - m);
-  v = v * b2 + w2 * g * g;
-  const double bc1 = 1.0 - pow(h.b1, (double)t);
-  const double bc2 = 1.0 - pow(h.b2, (double)t);
-  const T step_size = T(h.lr / bc1);
-  const T bc2s = T(sqrt(bc2));
-  const T denom = sqrt(v) / bc2s + T(h.eps);
-  p = p - step_size * (m / denom);
+  adam_apply(p, g, m, v, adam_scalars<T>(t, h));
 }
 
 // dparams (mode 0) or the Adam update (mode 1) from the per-item partials
@@ -989,15 +1006,42 @@ __global__ void k_reduce(NNArgs<T> a, int mode, T *dparams, T *params, T *m,
 }
 
 template <typename T>
-__global__ void k_adam(int64_t n, T *p, const T *g, T *m, T *v, const int64_t *step,
-                       AdamHyper h) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  T pv = p[i], mv = m[i], vv = v[i];
-  adam_update(pv, g[i], mv, vv, *step + 1, h);
-  p[i] = pv;
-  m[i] = mv;
-  v[i] = vv;
+__global__ __launch_bounds__(256) void k_adam(int64_t n, T *p, const T *g, T *m, T *v,
+                                              const int64_t *step, AdamHyper h, int vec) {
+  typedef typename V4<T>::type v4;
+  const AdamScalars<T> sc = adam_scalars<T>(*step + 1, h);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t done = 0;
+  if (vec) {
+    // 4 consecutive values per access; every pointer is 4-value aligned
+    const int64_t n4 = n / 4;
+    for (int64_t i = t0; i < n4; i += stride) {
+      v4 pv = reinterpret_cast<const v4 *>(p)[i];
+      const v4 gv = reinterpret_cast<const v4 *>(g)[i];
+      v4 mv = reinterpret_cast<const v4 *>(m)[i];
+      v4 vv = reinterpret_cast<const v4 *>(v)[i];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        T pk = pv[k], mk = mv[k], vk = vv[k];
+        adam_apply(pk, gv[k], mk, vk, sc);
+        pv[k] = pk;
+        mv[k] = mk;
+        vv[k] = vk;
+      }
+      reinterpret_cast<v4 *>(p)[i] = pv;
+      reinterpret_cast<v4 *>(m)[i] = mv;
+      reinterpret_cast<v4 *>(v)[i] = vv;
+    }
+    done = n4 * 4;
+  }
+  for (int64_t i = done + t0; i < n; i += stride) {
+    T pv = p[i], mv = m[i], vv = v[i];
+    adam_apply(pv, g[i], mv, vv, sc);
+    p[i] = pv;
+    m[i] = mv;
+    v[i] = vv;
+  }
 }
 
 __global__ void k_step_inc(int64_t *step) {
@@ -1513,15 +1557,24 @@ extern "C" int tdx_adam(int dtype, int64_t n, void *param, const void *grad, voi
                         tdx_stream_t stream) {
   TDX_REQUIRE(param && grad && m && v && step && n >= 0, "null argument");
   if (n == 0) return TDX_OK;
+  TDX_REQUIRE(dtype == 0 || dtype == 1, "dtype must be 0 or 1");
   AdamHyper h{lr, b1, b2, eps};
   hipStream_t s = (hipStream_t)stream;
-  const unsigned blocks = (unsigned)((n + 255) / 256);
+  const size_t al = (dtype == 0 ? 4 : 8) * 4;
+  const int vec = ((uintptr_t)param % al == 0 && (uintptr_t)grad % al == 0 &&
+                   (uintptr_t)m % al == 0 && (uintptr_t)v % al == 0) ? 1 : 0;
+  // about 16 vectors per thread at most: the per-step scalars are computed
+  // once per thread
+  const int64_t per_block = 256 * (vec ? 4 : 1);
+  const unsigned blocks =
+      (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + per_block - 1) / per_block,
+                                                       std::max<int64_t>(4096, n / (per_block * 16))));
   if (dtype == 0)
     hipLaunchKernelGGL((k_adam<float>), dim3(blocks), dim3(256), 0, s, n, (float *)param,
-                       (const float *)grad, (float *)m, (float *)v, step, h);
+                       (const float *)grad, (float *)m, (float *)v, step, h, vec);
   else
     hipLaunchKernelGGL((k_adam<double>), dim3(blocks), dim3(256), 0, s, n, (double *)param,
-                       (const double *)grad, (double *)m, (double *)v, step, h);
+                       (const double *)grad, (double *)m, (double *)v, step, h, vec);
   TDX_TRY_HIP(hipGetLastError());
   return TDX_OK;
 }

@@ -179,7 +179,8 @@ class MatTables:
 class MatEngine:
     """Native 'mat' residual/loss engine for one grid tensor."""
 
-    def __init__(self, prob: MatProblem, device: torch.device, exact_div: bool = True):
+    def __init__(self, prob: MatProblem, device: torch.device, exact_div: bool = True,
+                 tile_rows: int = 0):
         if prob.dtype not in (torch.float32, torch.float64):
             raise NotImplementedError(f"dtype {prob.dtype} is not supported")
         self.prob = prob
@@ -207,6 +208,7 @@ class MatEngine:
         d.n_reals, d.reals = t.reals.size, t.reals.ctypes.data
         d.n_data, d.data = t.data.size, t.data.ctypes.data
         d.exact_div = 1 if exact_div else 0
+        d.tile_rows = int(tile_rows)
         handle = ctypes.c_void_p()
         lib = native.lib()
         with torch.cuda.device(self.device):

@@ -70,6 +70,7 @@ class MatDesc(ctypes.Structure):
         ('n_reals', ctypes.c_int64), ('reals', ctypes.c_void_p),
         ('n_data', ctypes.c_int64), ('data', ctypes.c_void_p),
         ('exact_div', ctypes.c_int),
+        ('tile_rows', ctypes.c_int),
     ]
 
 

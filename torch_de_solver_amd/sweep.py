@@ -18,10 +18,11 @@ from .nn_engine import MLPArch, NNEngine, mlp_arch
 
 class SweepSolver:
     def __init__(self, problems: Sequence[Tuple], device=None, tile_rows: int = 0,
-                 chunk_tiles: int = 0):
+                 chunk_tiles: int = 0, autotune: bool = False):
         """``problems``: tuples ``(domain, equation, conditions, net, compile_kwargs)``
         as returned by the builders of ``workloads``; every net must have the
-        same architecture."""
+        same architecture.  ``autotune``: pick ``chunk_tiles`` by timing
+        candidates (``autotune.tune_nn``); the choice is in ``self.tuning``."""
         self.device = torch.device(device or 'cuda')
         self.nets = [p[3] for p in problems]
         self.arch: MLPArch = mlp_arch(self.nets[0])
@@ -39,7 +40,6 @@ class SweepSolver:
                                        lambda_operator=kw.get('lambda_operator', 1),
                                        lambda_bound=kw.get('lambda_bound', 1)))
         self.problems = compiled
-        self.engine = NNEngine(compiled, self.arch, self.device, tile_rows, chunk_tiles)
         P = self.arch.n_params
         dtype = compiled[0].dtype
         self.params = torch.empty(len(problems) * P, dtype=dtype, device=self.device)
@@ -51,6 +51,11 @@ class SweepSolver:
                 self.params[off:off + n].copy_(p.detach().reshape(-1))
                 p.data = self.params[off:off + n].view_as(p)
                 off += n
+        self.tuning = None
+        if autotune and chunk_tiles == 0 and tile_rows == 0:
+            from .autotune import tune_nn
+            chunk_tiles, self.tuning = tune_nn(compiled, self.arch, self.device, self.params)
+        self.engine = NNEngine(compiled, self.arch, self.device, tile_rows, chunk_tiles)
         self.exp_avg = torch.zeros_like(self.params)
         self.exp_avg_sq = torch.zeros_like(self.params)
         self.step_count = torch.zeros(1, dtype=torch.int64, device=self.device)
