@@ -238,7 +238,7 @@ def bench_nn(args):
     import copy
     nets0 = [copy.deepcopy(p[3]) for p in problems]
     t0 = time.time()
-    solver = SweepSolver(problems, autotune=not args.no_tune)
+    solver = SweepSolver(problems, autotune=not args.no_tune, tile_rows=args.nn_tile_rows)
     log(f"[bench] compiled {solver.n_inst} instance(s) in {time.time() - t0:.1f}s"
         f" (tuning: {solver.tuning})")
     eng = solver.engine
@@ -452,6 +452,8 @@ def main():
     ap.add_argument('--no-tune', action='store_true',
                     help="default tile shapes instead of the autotuner's choice")
     ap.add_argument('--tile-rows', type=int, default=0, help="'mat': output rows per tile")
+    ap.add_argument('--nn-tile-rows', type=int, default=0,
+                    help="'NN': backward tile rows (32: two tiles per wavefront iteration)")
     ap.add_argument('--calibrate', action='store_true',
                     help="stream-copy and FP32-GEMM microbenchmarks only")
     args = ap.parse_args()

@@ -98,7 +98,9 @@ typedef struct tdx_nn_desc {
   int64_t n_data;
   const void *data; /* dtype values: coordinates, coefficients, targets */
   int64_t n_resid;  /* length of the optional residual output */
-  int tile_rows;    /* rows per backward tile (0: default) */
+  int tile_rows;    /* backward tile rows (0: default; -1: workgroup-cooperative kernel;
+                       32: two 16-row tiles per wavefront iteration of the
+                       wavefront-private kernel (float), otherwise one) */
   int chunk_tiles;  /* tiles per backward work item (0: default) */
 } tdx_nn_desc;
 

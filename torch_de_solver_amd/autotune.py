@@ -92,11 +92,11 @@ def nn_key(problems, arch, dtype) -> str:
 
 def tune_nn(problems, arch, device, params: torch.Tensor,
             candidates: Sequence[int] = NN_CHUNK_CANDIDATES, reps: int = 3,
-            use_cache: bool = True) -> Tuple[int, dict]:
+            use_cache: bool = True, tile_rows: int = 0) -> Tuple[int, dict]:
     """Pick ``chunk_tiles`` for an 'NN' plan over ``problems`` (compiled) by
     timing forward + backward with ``params``.  Returns (chunk_tiles, report)."""
     from .nn_engine import NNEngine
-    key = nn_key(problems, arch, params.dtype)
+    key = nn_key(problems, arch, params.dtype) + f"|tr{tile_rows}"
     if use_cache:
         hit = _load().get(key)
         if hit and hit.get('chunk_tiles') in candidates:
@@ -106,7 +106,7 @@ def tune_nn(problems, arch, device, params: torch.Tensor,
     one = torch.ones(1, dtype=params.dtype, device=params.device)
     times = {}
     for ct in candidates:
-        eng = NNEngine(problems, arch, device, 0, ct)
+        eng = NNEngine(problems, arch, device, tile_rows, ct)
 
         def run():
             eng.forward(params, loss)

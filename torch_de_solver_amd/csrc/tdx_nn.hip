@@ -667,7 +667,33 @@ constexpr int WRS = 20;
 template <int HP>
 constexpr int wave_ls() { return WRS * HP / 16; }
 
-template <typename T, int NB, int NH>
+// acc[t][ob] += W (rows of block ob) x b[t] for NT independent tiles; each
+// weight operand is loaded once and used by every tile.
+template <typename T, int NB, int NT>
+__device__ __forceinline__ void mma_layer_nt(const T *img, int lane, const T (&b)[NT][NB][4],
+                                             typename V4<T>::type (&acc)[NT][NB]) {
+  typedef typename V4<T>::type v4;
+  const v4 *wp = reinterpret_cast<const v4 *>(img) + lane;
+#pragma unroll
+  for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) {
+      const v4 w = wp[(ob * NB + mb) * 64];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t][ob] = mma16(w[i], b[t][mb][i], acc[t][ob]);
+    }
+}
+
+// Wavefront-private backward: each wavefront walks 16-row tiles of its work
+// item, NT tiles at a time (independent instruction streams, so one tile's
+// tanh and transposes overlap the other's matrix products).  Per tile: the
+// forward recompute keeps every activation in registers; the adjoint runs
+// layer by layer; weight gradients accumulate in MFMA tiles (registers) over
+// the whole item, and the workgroup's wavefronts add theirs in a fixed order
+// at the end.
+template <typename T, int NB, int NH, int NT>
 __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
   constexpr int HP = 16 * NB;
   constexpr int FB = WRS * HP;  // words of one feature-major buffer
@@ -683,14 +709,18 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row = lane & 15, q = lane >> 4;
   const int H = a.H, d_in = a.d_in, n_out = a.n_out, P = a.P;
-  // wavefront-private feature-major buffers: deltas, activations, and the
-  // seeds/coordinates of the 16 rows ([column][row], stride WRS)
+  // wavefront-private feature-major buffers per tile: deltas, activations,
+  // and the seeds/coordinates of the 16 rows ([column][row], stride WRS)
   const int WB = 2 * FB + 2 * 16 * WRS;
-  T *dbuf = smem + w * WB;
-  T *abuf = dbuf + FB;
-  T *sbuf = abuf + FB;          // seeds, one column per output
-  T *xbuf = sbuf + 16 * WRS;    // coordinates, one column per input
-  const int TRW = max(NW * WB, P);  // buffers, then the final sum
+  T *dbuf[NT], *abuf[NT], *sbuf[NT], *xbuf[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    dbuf[t] = smem + (w * NT + t) * WB;
+    abuf[t] = dbuf[t] + FB;
+    sbuf[t] = abuf[t] + FB;          // seeds, one column per output
+    xbuf[t] = sbuf[t] + 16 * WRS;    // coordinates, one column per input
+  }
+  const int TRW = max(NW * NT * WB, P);  // buffers, then the final sum
   T *stage = smem + TRW;
 
   const Net<T, NB> net = stage_net<T, NB>(a, item.w, stage, true);
@@ -706,7 +736,7 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
   //     outside its own columns, so the columns add exactly what separate
   //     tiles would.
   //   bhacc[jj][xb]: hidden bias of feature 16 xb + row over the rows
-  //     4q .. 4q+3 of each iteration, summed over the quarters at the end
+  //     4q .. 4q+3 of each tile, summed over the quarters at the end
   //   blacc: output bias o = q, summed over the row lanes at the end
   v4 dwacc[NH > 0 ? NH : 1][NB * NB];
   v4 sacc[NB];
@@ -724,81 +754,103 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
   const int col_b1 = d_in, col_wl = d_in + 1;
 
   const int64_t r_begin = item.y, r_end = item.z;
-  for (int64_t g0 = r_begin + 16 * w; g0 < r_end; g0 += 16 * NW) {
-    const int64_t r = g0 + row;
-    const bool valid = r < r_end;
-    const int64_t rc = valid ? r : r_begin;
-    // (slot, entry) of the row: one division per wavefront iteration for
-    // its first row, then a bounded walk (rows are slot-major)
-    const int lr0 = (int)(g0 - row_base);
-    int s = lr0 / n_entries;
-    int e = lr0 - s * n_entries + (valid ? row : 0);
-    while (e >= n_entries) {
-      e -= n_entries;
-      ++s;
-    }
-    T x[MAXD];
-    const int64_t base = a.ints[slot_base_i + s];
+  for (int64_t g0 = r_begin + 16 * NT * w; g0 < r_end; g0 += 16 * NT * NW) {
+    // rows g0 + 16 t + row; (slot, entry) of a row: one division per
+    // wavefront iteration, then a bounded walk (rows are slot-major)
+    bool valid[NT];
+    int64_t rc[NT];
+    T x[NT][MAXD];
+    {
+      const int lr0 = (int)(g0 - row_base);
+      int s = lr0 / n_entries;
+      int e = lr0 - s * n_entries;
 #pragma unroll
-    for (int k = 0; k < MAXD; ++k)
-      x[k] = (k < d_in) ? a.data[base + (int64_t)e * d_in + k] + a.reals[slot_off_r + s * d_in + k]
-                        : T(0);
-    if (q == 0) {
+      for (int t = 0; t < NT; ++t) {
+        const int64_t r = g0 + 16 * t + row;
+        valid[t] = r < r_end;
+        rc[t] = valid[t] ? r : r_begin;
+        int et = e + (valid[t] ? 16 * t + row : 0), st = s;
+        while (et >= n_entries) {
+          et -= n_entries;
+          ++st;
+        }
+        const int64_t base = a.ints[slot_base_i + st];
 #pragma unroll
-      for (int k = 0; k < MAXD; ++k)
-        if (k < d_in) xbuf[k * WRS + row] = x[k];
-    }
-
-    // forward recompute, every activation kept
-    T act[NH + 1][NB][4];
-    layer_in<T, NB>(net, x, q, act[0]);
+        for (int k = 0; k < MAXD; ++k)
+          x[t][k] = (k < d_in) ? a.data[base + (int64_t)et * d_in + k] +
+                                     a.reals[slot_off_r + st * d_in + k]
+                               : T(0);
+        if (q == 0) {
 #pragma unroll
-    for (int j = 1; j <= NH; ++j) {
-#pragma unroll
-      for (int mb = 0; mb < NB; ++mb)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) act[j][mb][i] = act[j - 1][mb][i];
-      layer_hidden<T, NB>(net, j - 1, lane, act[j]);
-    }
-
-    // seeds; delta of the last hidden activation
-    T dl[NB][4];
-#pragma unroll
-    for (int mb = 0; mb < NB; ++mb)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) dl[mb][i] = T(0);
-#pragma unroll
-    for (int o = 0; o < MAXO; ++o) {
-      if (o < n_out) {
-        const T so = valid ? a.seeds[rc * n_out + o] * dl_scale : T(0);
-        if (o == q) blacc += so;
-        if (q == 0) sbuf[o * WRS + row] = so;
-#pragma unroll
-        for (int mb = 0; mb < NB; ++mb) {
-          const v4 wl = ld4(net.WL + o * HP + 16 * mb + 4 * q);
-#pragma unroll
-          for (int i = 0; i < 4; ++i) dl[mb][i] = fma(wl[i], so, dl[mb][i]);
+          for (int k = 0; k < MAXD; ++k)
+            if (k < d_in) xbuf[t][k * WRS + row] = x[t][k];
         }
       }
     }
+
+    // forward recompute, every activation kept
+    T act[NH + 1][NT][NB][4];
 #pragma unroll
-    for (int mb = 0; mb < NB; ++mb)
+    for (int t = 0; t < NT; ++t) layer_in<T, NB>(net, x[t], q, act[0][t]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const T av = act[NH][mb][i];
-        dl[mb][i] *= (T(1) - av * av);
-        abuf[(16 * mb + crow<T>(q, i)) * WRS + row] = av;
+    for (int j = 1; j <= NH; ++j) {
+      v4 acc[NT][NB];
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int ob = 0; ob < NB; ++ob) acc[t][ob] = ld4(net.bh + (j - 1) * HP + 16 * ob + 4 * q);
+      mma_layer_nt<T, NB, NT>(net.Wf + (j - 1) * HP * HP, lane, act[j - 1], acc);
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int ob = 0; ob < NB; ++ob)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) act[j][t][ob][i] = dtanh(acc[t][ob][i]);
+    }
+
+    // seeds; delta of the last hidden activation
+    T dl[NT][NB][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+#pragma unroll
+      for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dl[t][mb][i] = T(0);
+#pragma unroll
+      for (int o = 0; o < MAXO; ++o) {
+        if (o < n_out) {
+          const T so = valid[t] ? a.seeds[rc[t] * n_out + o] * dl_scale : T(0);
+          if (o == q) blacc += so;
+          if (q == 0) sbuf[t][o * WRS + row] = so;
+#pragma unroll
+          for (int mb = 0; mb < NB; ++mb) {
+            const v4 wl = ld4(net.WL + o * HP + 16 * mb + 4 * q);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) dl[t][mb][i] = fma(wl[i], so, dl[t][mb][i]);
+          }
+        }
       }
+#pragma unroll
+      for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const T av = act[NH][t][mb][i];
+          dl[t][mb][i] *= (T(1) - av * av);
+          abuf[t][(16 * mb + crow<T>(q, i)) * WRS + row] = av;
+        }
+    }
     wave_sync();
     // output layer: dW_L[o][f] = sum_rows s_o act_NH[f]; k-step kb of quarter
     // q takes row position 4q + kb in both operands
-    {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
       const int o = row - col_wl;
-      const v4 sv = (o >= 0 && o < n_out) ? *reinterpret_cast<const v4 *>(sbuf + o * WRS + 4 * q)
-                                          : v4{0, 0, 0, 0};
+      const v4 sv = (o >= 0 && o < n_out)
+                        ? *reinterpret_cast<const v4 *>(sbuf[t] + o * WRS + 4 * q)
+                        : v4{0, 0, 0, 0};
 #pragma unroll
       for (int ob = 0; ob < NB; ++ob) {
-        const v4 av = *reinterpret_cast<const v4 *>(abuf + (16 * ob + row) * WRS + 4 * q);
+        const v4 av = *reinterpret_cast<const v4 *>(abuf[t] + (16 * ob + row) * WRS + 4 * q);
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) sacc[ob] = mma16(av[kb], sv[kb], sacc[ob]);
       }
@@ -809,20 +861,23 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
     for (int j = NH; j >= 1; --j) {
       const int jj = j - 1;
 #pragma unroll
-      for (int mb = 0; mb < NB; ++mb)
+      for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int f = 16 * mb + crow<T>(q, i);
-          dbuf[f * WRS + row] = dl[mb][i];
-          abuf[f * WRS + row] = act[j - 1][mb][i];
-        }
+        for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int f = 16 * mb + crow<T>(q, i);
+            dbuf[t][f * WRS + row] = dl[t][mb][i];
+            abuf[t][f * WRS + row] = act[j - 1][t][mb][i];
+          }
       wave_sync();
-      {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
         v4 av[NB], dv[NB];
 #pragma unroll
         for (int xb = 0; xb < NB; ++xb) {
-          dv[xb] = *reinterpret_cast<const v4 *>(dbuf + (16 * xb + row) * WRS + 4 * q);
-          av[xb] = *reinterpret_cast<const v4 *>(abuf + (16 * xb + row) * WRS + 4 * q);
+          dv[xb] = *reinterpret_cast<const v4 *>(dbuf[t] + (16 * xb + row) * WRS + 4 * q);
+          av[xb] = *reinterpret_cast<const v4 *>(abuf[t] + (16 * xb + row) * WRS + 4 * q);
         }
 #pragma unroll
         for (int ob = 0; ob < NB; ++ob) {
@@ -838,31 +893,37 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
       }
       wave_sync();
       // delta_{j-1} = (W_jj^T delta_j) * (1 - act_{j-1}^2)
-      v4 c[NB];
+      v4 c[NT][NB];
 #pragma unroll
-      for (int ib = 0; ib < NB; ++ib) c[ib] = v4{0, 0, 0, 0};
-      mma_layer<T, NB>(net.Wb + jj * HP * HP, lane, dl, c);
+      for (int t = 0; t < NT; ++t)
 #pragma unroll
-      for (int ib = 0; ib < NB; ++ib)
+        for (int ib = 0; ib < NB; ++ib) c[t][ib] = v4{0, 0, 0, 0};
+      mma_layer_nt<T, NB, NT>(net.Wb + jj * HP * HP, lane, dl, c);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int f = 16 * ib + crow<T>(q, i);
-          const T av = act[j - 1][ib][i];
-          dl[ib][i] = (f < H) ? c[ib][i] * (T(1) - av * av) : T(0);
-        }
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const T av = act[j - 1][t][ib][i];
+            dl[t][ib][i] = c[t][ib][i] * (T(1) - av * av);
+          }
     }
     // first layer: dW_1[f][c] = sum_rows delta_0[f] x_c, bias in column d_in
 #pragma unroll
-    for (int mb = 0; mb < NB; ++mb)
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) dbuf[(16 * mb + crow<T>(q, i)) * WRS + row] = dl[mb][i];
+      for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dbuf[t][(16 * mb + crow<T>(q, i)) * WRS + row] = dl[t][mb][i];
     wave_sync();
-    {
-      const v4 xv = row < d_in ? *reinterpret_cast<const v4 *>(xbuf + row * WRS + 4 * q)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const v4 xv = row < d_in ? *reinterpret_cast<const v4 *>(xbuf[t] + row * WRS + 4 * q)
                                : (row == col_b1 ? v4{1, 1, 1, 1} : v4{0, 0, 0, 0});
 #pragma unroll
       for (int ob = 0; ob < NB; ++ob) {
-        const v4 dv = *reinterpret_cast<const v4 *>(dbuf + (16 * ob + row) * WRS + 4 * q);
+        const v4 dv = *reinterpret_cast<const v4 *>(dbuf[t] + (16 * ob + row) * WRS + 4 * q);
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) sacc[ob] = mma16(dv[kb], xv[kb], sacc[ob]);
       }
@@ -1072,6 +1133,7 @@ struct tdx_nn_plan {
   int k2_waves;
   int k1_stage;
   int k2_wave_private;
+  int k2_nt;          // tiles per wavefront iteration of the wave-private backward
   int acc_lds;
   size_t k1_lds, k2_lds;
   DevTable groups, ints, reals, data, k1_items, k2_items, ik1p, ik1i, ik2p, ik2i;
@@ -1083,12 +1145,12 @@ struct tdx_nn_plan {
   std::vector<hipEvent_t> prof_ev[TDX_NN_KERNEL_KINDS];
 };
 
-// LDS words of the wavefront-private backward: transpose buffers (reused
-// for the final sum, so at least P words), per-wavefront small-term sums and
-// the staged weights.
+// LDS words of the wavefront-private backward: transpose buffers of every
+// tile of the 4 wavefronts (reused for the final sum, so at least P words)
+// and the staged network image.
 static size_t wave_private_lds(const tdx_nn_plan *p, const tdx_nn_desc *d) {
   const size_t wb = (size_t)2 * 20 * p->HP + 2 * 16 * 20;
-  const size_t tr = std::max<size_t>(4 * wb, (size_t)p->P);
+  const size_t tr = std::max<size_t>(4 * p->k2_nt * wb, (size_t)p->P);
   return tr + stage_words(p->HP, p->NH, d->d_in, d->n_out, true);
 }
 
@@ -1175,19 +1237,27 @@ int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStre
   return TDX_OK;
 }
 
-template <typename T, int NB, int NH>
-int launch_backward_w(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
+template <typename T, int NB, int NH, int NT>
+int launch_backward_wt(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)k_backward_w<T, NB, NH>,
+    (void)hipFuncSetAttribute((const void *)k_backward_w<T, NB, NH, NT>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr_set = true;
   }
   ProfScope ps(p, 1, s);
-  hipLaunchKernelGGL((k_backward_w<T, NB, NH>), dim3(p->n_k2), dim3(64 * p->k2_waves), p->k2_lds,
-                     s, a);
+  hipLaunchKernelGGL((k_backward_w<T, NB, NH, NT>), dim3(p->n_k2), dim3(64 * p->k2_waves),
+                     p->k2_lds, s, a);
   TDX_TRY_HIP(hipGetLastError());
   return TDX_OK;
+}
+
+template <typename T, int NB, int NH>
+int launch_backward_w(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
+  if constexpr (sizeof(T) == 4) {
+    if (p->k2_nt == 2) return launch_backward_wt<T, NB, NH, 2>(p, a, s);
+  }
+  return launch_backward_wt<T, NB, NH, 1>(p, a, s);
 }
 
 template <typename T, int NB>
@@ -1405,6 +1475,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   p->acc_lds = lds_of(waves, true) <= LDS_MAX ? 1 : 0;
   p->k2_lds = lds_of(waves, p->acc_lds != 0);
   // narrow networks: wavefront-private backward with register accumulators
+  p->k2_nt = (d->dtype == 0 && d->tile_rows == 32) ? 2 : 1;
   p->k2_wave_private = (p->NB <= 2 && p->NH <= 4 && d->tile_rows >= 0 &&
                         d->d_in + 1 + d->n_out <= 16 &&
                         wave_private_lds(p, d) * ts <= LDS_MAX) ? 1 : 0;

@@ -52,9 +52,10 @@ class SweepSolver:
                 p.data = self.params[off:off + n].view_as(p)
                 off += n
         self.tuning = None
-        if autotune and chunk_tiles == 0 and tile_rows == 0:
+        if autotune and chunk_tiles == 0:
             from .autotune import tune_nn
-            chunk_tiles, self.tuning = tune_nn(compiled, self.arch, self.device, self.params)
+            chunk_tiles, self.tuning = tune_nn(compiled, self.arch, self.device, self.params,
+                                               tile_rows=tile_rows)
         self.engine = NNEngine(compiled, self.arch, self.device, tile_rows, chunk_tiles)
         self.exp_avg = torch.zeros_like(self.params)
         self.exp_avg_sq = torch.zeros_like(self.params)
