@@ -24,7 +24,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import torch
 
-NN_CHUNK_CANDIDATES = (16, 32, 64, 128, 256)
+NN_CHUNK_CANDIDATES = (1, 2, 4, 8, 16, 32, 64, 128, 256)
 MAT_ROW_CANDIDATES = (4, 8, 12, 16, 20, 28, 36, 44, 60)
 
 _lock = threading.Lock()
