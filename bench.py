@@ -53,6 +53,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(name: str):
+    """HBM bytes per launch of a kernel measured by rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes (each scaled by what it reads for a copy kernel of
+    known size), from the summary ``profiles/<name>_traffic.txt``, whose
+    ``traffic_bytes_per_launch:`` line holds the figure; None when absent."""
+    path = os.path.join(ROOT, 'profiles', name + '_traffic.txt')
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith('traffic_bytes_per_launch:'):
+                    return float(line.split(':', 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def mlp_flops_per_row(arch) -> int:
     """2 * multiply-adds of one network evaluation."""
     H, d, o, L = arch.width, arch.d_in, arch.n_out, arch.n_layers
@@ -238,7 +254,8 @@ def bench_nn(args):
     import copy
     nets0 = [copy.deepcopy(p[3]) for p in problems]
     t0 = time.time()
-    solver = SweepSolver(problems, autotune=not args.no_tune, tile_rows=args.nn_tile_rows)
+    solver = SweepSolver(problems, autotune=not args.no_tune, tile_rows=args.nn_tile_rows,
+                         chunk_tiles=args.chunk_tiles)
     log(f"[bench] compiled {solver.n_inst} instance(s) in {time.time() - t0:.1f}s"
         f" (tuning: {solver.tuning})")
     eng = solver.engine
@@ -293,7 +310,11 @@ def bench_nn(args):
         "loss_sum_after": loss_now,
         "roofline": {"bound": "mfma", "kernel": "k_backward (recompute + adjoint + weight gradients)",
                      "achieved": achieved / 1e12, "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_MFMA_PEAK, "traffic": None,
+                     "frac": achieved / FP32_MFMA_PEAK,
+                     "traffic": profiled_traffic(f"k_backward_w_{args.workload}")
+                     if args.workload == 'sweep' else None,
+                     "traffic_source": "profiles/k_backward_w_sweep_traffic.txt (rocprofv3 "
+                                       "FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                      "algorithmic_flops_per_launch": bwd_flops,
                      "avg_launch_ms": bwd_ms},
         "kernel_ms": {"forward": fwd_ms, "backward": bwd_ms, "reduce_adam": upd_ms,
@@ -376,7 +397,10 @@ def bench_mat(args):
         "colloc_pts_per_s": N * 1e3 / ms,
         "roofline": {"bound": "hbm", "kernel": "k_mat_tile (residual + loss + adjoint)",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": profiled_traffic(f"k_mat_tile_{args.workload}"),
+                     "traffic_source": f"profiles/k_mat_tile_{args.workload}_traffic.txt "
+                                       "(rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                      "algorithmic_bytes_per_launch": 8.0 * N, "avg_launch_ms": bwd_ms},
     }
     if not args.no_plain and n <= 4096:
@@ -452,6 +476,8 @@ def main():
     ap.add_argument('--no-tune', action='store_true',
                     help="default tile shapes instead of the autotuner's choice")
     ap.add_argument('--tile-rows', type=int, default=0, help="'mat': output rows per tile")
+    ap.add_argument('--chunk-tiles', type=int, default=0,
+                    help="'NN': backward tiles per work item (0: autotuned)")
     ap.add_argument('--nn-tile-rows', type=int, default=0,
                     help="'NN': backward tile rows (32: two tiles per wavefront iteration)")
     ap.add_argument('--calibrate', action='store_true',

@@ -90,21 +90,23 @@ struct MatArgs {
 // correctly rounded quotient from inv = RN(1/h) by one correction step,
 // q = RN(x inv), r = x - q h (exact by fma), RN(q + r inv), which the plan
 // has checked against the correctly rounded quotient for every significand
-// of x in [1, 2); it holds for every x whose quotient scales exactly, so
-// the step is taken for 2^-90 <= |x| <= 2^90 and zero, and the IEEE
-// division otherwise.
+// of x in [1, 2) (h > 0).  It holds for every x whose steps scale exactly,
+// 2^-90 <= |x| <= 2^90; the sign of x is copied onto the result, which
+// makes it exact for +-0 too.  Any other x (subnormal, huge, inf, NaN)
+// takes the IEEE division: a wavefront-uniform test keeps the common case
+// free of branches.
 template <typename T, int DIV>
 __device__ __forceinline__ T qdiv(T x, T h, T inv) {
   if constexpr (DIV == 0) {
     return x * inv;
   } else if constexpr (DIV == 2 && sizeof(T) == 4) {
-    const float ax = fabsf(x);
-    if (__builtin_expect(ax <= 0x1p90f && (ax >= 0x1p-90f || ax == 0.0f), 1)) {
-      const float q = x * inv;
-      const float r = fmaf(-q, h, x);
-      return fmaf(r, inv, q);
-    }
-    return x / h;
+    const float q = x * inv;
+    const float r = fmaf(-q, h, x);
+    float res = copysignf(fmaf(r, inv, q), x);
+    const uint32_t ab = __float_as_uint(x) & 0x7fffffffu;
+    const bool slow = ab != 0u && (ab - 0x12800000u) > (0x6C800000u - 0x12800000u);
+    if (__builtin_amdgcn_ballot_w64(slow) != 0) res = slow ? x / h : res;
+    return res;
   } else {
     return x / h;
   }
@@ -210,15 +212,18 @@ __device__ __forceinline__ T mdfpow(const MatArgs<T> &a, const int32_t *F, T x) 
 // flattened program, and its derivative
 template <typename T>
 __device__ __forceinline__ T ppow(const int32_t *Fr, const T *pr, T x) {
-  return Fr[0] == 0 ? ipow(x, Fr[1]) : dpow(x, pr[Fr[2]]);
+  const int kind = __builtin_amdgcn_readfirstlane(Fr[0]);
+  const int pi = __builtin_amdgcn_readfirstlane(Fr[1]);
+  return kind == 0 ? ipow(x, pi) : dpow(x, pr[__builtin_amdgcn_readfirstlane(Fr[2])]);
 }
 template <typename T>
 __device__ __forceinline__ T pdpow(const int32_t *Fr, const T *pr, T x) {
-  if (Fr[0] == 0) {
-    const int p = Fr[1];
+  const int kind = __builtin_amdgcn_readfirstlane(Fr[0]);
+  if (kind == 0) {
+    const int p = __builtin_amdgcn_readfirstlane(Fr[1]);
     return p == 0 ? T(0) : T(p) * ipow(x, p - 1);
   }
-  const T p = pr[Fr[2]];
+  const T p = pr[__builtin_amdgcn_readfirstlane(Fr[2])];
   return p * dpow(x, p - T(1));
 }
 
@@ -242,9 +247,19 @@ __device__ __forceinline__ void for_box(int E1, int tpr_log, int lo0, int hi0, i
                                         Fn fn) {
   const int tpr = 1 << tpr_log;
   const int c0 = threadIdx.x & (tpr - 1);
-  for (int i0 = lo0 + (int)(threadIdx.x >> tpr_log); i0 < hi0; i0 += MAT_THREADS >> tpr_log)
-    for (int i1 = c0; i1 < E1; i1 += tpr)
-      if (i1 >= lo1 && i1 < hi1) fn(i0, i1);
+  const int r0 = lo0 + (int)(threadIdx.x >> tpr_log), rstep = MAT_THREADS >> tpr_log;
+  if (E1 == tpr) {
+    // one column per thread: the column test is made once, and since a row
+    // spans whole wavefronts (E1 >= 64) every lane of a wavefront has the
+    // same rows, which readfirstlane tells the compiler (a scalar loop)
+    const int r0u = __builtin_amdgcn_readfirstlane(r0);
+    if (c0 >= lo1 && c0 < hi1)
+      for (int i0 = r0u; i0 < hi0; i0 += rstep) fn(i0, c0);
+  } else {
+    for (int i0 = r0; i0 < hi0; i0 += rstep)
+      for (int i1 = c0; i1 < E1; i1 += tpr)
+        if (i1 >= lo1 && i1 < hi1) fn(i0, i1);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -380,12 +395,16 @@ __device__ __forceinline__ void mat_tile_body(const MatArgs<T> &a, T *smem, int 
     const bool in_tile = i0 >= 2 * R0 && i0 < 2 * R0 + T0 && i1 >= 2 * R1 && i1 < 2 * R1 + T1;
     const int64_t gp = ing ? (int64_t)g0 * n1 + g1 : 0;
     for (int eq = 0; eq < a.n_eq; ++eq) {
-      const int n_terms = PI[2 * eq], term0 = PI[2 * eq + 1];
+      // program words are the same for every lane: readfirstlane makes the
+      // loops scalar
+      const int n_terms = __builtin_amdgcn_readfirstlane(PI[2 * eq]);
+      const int term0 = __builtin_amdgcn_readfirstlane(PI[2 * eq + 1]);
       const T lam_n = PR[eq];
       T r = T(0);
       for (int t = 0; t < n_terms; ++t) {
         const int32_t *Tr = PI + a.pterm0 + 4 * (term0 + t);
-        const int nf = Tr[0], fs = Tr[3];
+        const int nf = __builtin_amdgcn_readfirstlane(Tr[0]);
+        const int fs = __builtin_amdgcn_readfirstlane(Tr[3]);
         T der = T(1);
         for (int f = 0; f < nf; ++f)
           der = der * ppow(PI + a.pfac0 + 3 * (fs + f), PR, F[(fs + f) * S + p]);
@@ -402,7 +421,8 @@ __device__ __forceinline__ void mat_tile_body(const MatArgs<T> &a, T *smem, int 
         const T g = T(2) * lam_n * r * dl;
         for (int t = 0; t < n_terms; ++t) {
           const int32_t *Tr = PI + a.pterm0 + 4 * (term0 + t);
-          const int nf = Tr[0], fs = Tr[3];
+          const int nf = __builtin_amdgcn_readfirstlane(Tr[0]);
+          const int fs = __builtin_amdgcn_readfirstlane(Tr[3]);
           const T c = term_coeff(Tr, PR, a.data, gp);
           T fv[MMAX_TF], pw[MMAX_TF];
 #pragma unroll
@@ -516,11 +536,23 @@ __device__ __forceinline__ void mat_tile_body(const MatArgs<T> &a, T *smem, int 
   }
 }
 
+// Workgroups are handed to the 8 XCDs round-robin (block b to XCD b % 8),
+// each XCD with its own L2.  Block b takes tile (b % 8) * per + b / 8 for
+// the first 8 * per blocks, so every XCD works through one contiguous band
+// of tiles and the halo rows its neighbouring tiles share stay in its L2.
+__device__ __forceinline__ int xcd_band_tile(int b, int n) {
+  const int full = n & ~7;
+  if (b >= full) return b;
+  const int per = full >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
 template <typename T, int DP, int DIV, bool ADJ>
 __global__ __launch_bounds__(MAT_THREADS) void k_mat_tile(MatArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
-  const int t0 = blockIdx.x / a.tiles1, t1 = blockIdx.x - (blockIdx.x / a.tiles1) * a.tiles1;
+  const int tile = xcd_band_tile((int)blockIdx.x, (int)gridDim.x);
+  const int t0 = tile / a.tiles1, t1 = tile - (tile / a.tiles1) * a.tiles1;
   const int base0 = t0 * a.T0 - 2 * a.R0, base1 = t1 * a.T1 - 2 * a.R1;
   const bool interior = base0 >= 0 && base0 + a.E0 <= a.n0 && base1 >= 0 && base1 + a.E1 <= a.n1;
   if (interior)
