@@ -384,3 +384,36 @@ def test_nn_backward_kernels_agree(name, golden):
         grads.append(p.grad.cpu().numpy())
     for gr in grads:
         np.testing.assert_allclose(gr, g['grad'], rtol=1e-8, atol=1e-10 * np.abs(g['grad']).max())
+
+
+@pytest.mark.parametrize("layers", [[2, 48, 48, 48, 1], [2, 64, 64, 64, 64, 1],
+                                    [2, 100, 100, 100, 100, 1], [1, 100, 100, 1]])
+def test_nn_wide_float32_backward(layers):
+    """Wide float32 networks: the block-owning backward (default), the
+    workgroup-cooperative one (tile_rows=-1) and work lists cut into 1- and
+    3-tile items all give the float32 oracle's gradient (reference op order)
+    to 1e-4 in norm at h = 1e-2."""
+    from torch_de_solver_amd.compiler import compile_nn
+    from torch_de_solver_amd.nn_engine import NNEngine, mlp_arch
+    name = 'burgers_nn' if layers[0] == 2 else 'legendre_nn'
+    builder, _ = problems.NN_CASES[name]
+    d, eq, b, kw = builder(api, dtype='float32')
+    net = problems.randomize_biases(problems.make_mlp(layers, dtype=torch.float32))
+    orc = O.NNOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst],
+                     h=1e-2)
+    lo = orc.evaluate(net, kw['lambda_operator'], kw['lambda_bound'])[0]
+    lo.backward()
+    go = torch.cat([p.grad.reshape(-1) for p in net.parameters()]).numpy()
+    prob = compile_nn(d.build('NN'), eq.equation_lst, b.build(d.variable_dict), 1, h=1e-2,
+                      lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
+    flat = torch.cat([p.detach().reshape(-1) for p in net.parameters()]).to(DEV)
+    for tile_rows, chunk in ((0, 0), (0, 1), (0, 3), (-1, 0)):
+        _print(f"[wide backward {layers} tile_rows={tile_rows} chunk={chunk}]")
+        eng = NNEngine([prob], mlp_arch(net), DEV, tile_rows=tile_rows, chunk_tiles=chunk)
+        p = flat.clone().requires_grad_()
+        ln = eng.loss(p)[0, 0]
+        ln.backward()
+        gn = p.grad.cpu().numpy()
+        assert abs(float(ln) - float(lo)) / abs(float(lo)) < 1e-5, (float(ln), float(lo))
+        err = np.linalg.norm(gn - go) / np.linalg.norm(go)
+        assert err < 1e-4, (tile_rows, chunk, err)

@@ -976,6 +976,289 @@ __global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
   for (int k = threadIdx.x; k < P; k += blockDim.x) out[k] = cp[k];
 }
 
+// Block-owning backward for wide float networks (HP = 16 NB up to 128).
+// A workgroup of 4 wavefronts walks tiles of 64 rows, 16 per wavefront.
+// Each wavefront recomputes the forward pass of its 16 rows and stores every
+// layer's activations of the tile in feature-major LDS buffers
+// ([layer][feature][row], row stride 68: 128-bit operand loads and the
+// row-wise stores both free of bank conflicts); the deltas of the current
+// layer go to one more such buffer.  The weight gradient dW = delta^T act
+// over the 64 rows is computed block by block: wavefront w owns the output
+// blocks ob = w, w + 4, ... of every layer and keeps their sums in
+// accumulator registers over the whole work item, so no gradient is read
+// back or combined between wavefronts until the end.
+constexpr int CTRS = 68;  // row stride of the feature-major tile buffers
+
+// acc[ob] += W (rows of block ob) x b where b is this lane's row trow of a
+// feature-major LDS buffer (stride CTRS); one input block at a time, so only
+// that block's four operands and the NB accumulators are live.
+template <typename T, int NB>
+__device__ __forceinline__ void mma_layer_fm(const T *img, int lane, const T *src, int trow,
+                                             typename V4<T>::type (&acc)[NB]) {
+  typedef typename V4<T>::type v4;
+  const int q = lane >> 4;
+  const v4 *wp = reinterpret_cast<const v4 *>(img) + lane;
+#pragma unroll 1
+  for (int mb = 0; mb < NB; ++mb) {
+    T b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b[i] = src[(16 * mb + crow<T>(q, i)) * CTRS + trow];
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) {
+      const v4 w = wp[(ob * NB + mb) * 64];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[ob] = mma16(w[i], b[i], acc[ob]);
+    }
+  }
+}
+
+__host__ __device__ inline size_t block_lds_words(int HP, int NH) {
+  return (size_t)(NH + 2) * HP * CTRS + (size_t)(MAXD + MAXO) * CTRS + 16;
+}
+
+template <typename T, int NB, int NH>
+__global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
+  constexpr int HP = 16 * NB;
+  constexpr int OBW = (NB + 3) / 4;  // output blocks owned per wavefront
+  constexpr int TR = 64;
+  typedef typename V4<T>::type v4;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  T *actb = smem;                          // [NH+1][HP][CTRS] activations
+  T *dbuf = actb + (NH + 1) * HP * CTRS;   // [HP][CTRS] deltas of one layer
+  T *xbuf = dbuf + HP * CTRS;              // [MAXD][CTRS] coordinates
+  T *sbuf = xbuf + MAXD * CTRS;            // [MAXO][CTRS] seeds
+  T *red = sbuf + MAXO * CTRS;             // 4 wavefront partials of the output bias
+  const int4 item = a.k2_items[blockIdx.x];
+  const int32_t *G = a.groups + item.x * TDX_GROUP_INTS;
+  const int n_entries = G[1];
+  const int slot_base_i = G[3], slot_off_r = G[4];
+  const int64_t row_base = G[7];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = lane & 15, q = lane >> 4;
+  const int trow = 16 * w + row;  // this lane's row within the tile
+  const int H = a.H, d_in = a.d_in, n_out = a.n_out, P = a.P;
+  const Net<T, NB> net = make_net<T, NB>(a, item.w);
+  const Layout<T> &L = net.L;
+  const T dl_scale = *a.dloss;
+  const int col_b1 = d_in, col_wl = d_in + 1;
+
+  v4 dwacc[NH][OBW][NB];
+  v4 sacc[OBW];
+  T bhacc[NH][OBW];
+#pragma unroll
+  for (int j = 0; j < NH; ++j)
+#pragma unroll
+    for (int u = 0; u < OBW; ++u) {
+      bhacc[j][u] = T(0);
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib) dwacc[j][u][ib] = v4{0, 0, 0, 0};
+    }
+#pragma unroll
+  for (int u = 0; u < OBW; ++u) sacc[u] = v4{0, 0, 0, 0};
+  T blacc = T(0);
+
+  auto layer_buf = [&](int j) { return actb + j * HP * CTRS; };
+  // own-row store of a layer's values held as accumulator rows
+  auto store_rows = [&](T *buf, const T (&v)[NB][4]) {
+#pragma unroll
+    for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) buf[(16 * mb + crow<T>(q, i)) * CTRS + trow] = v[mb][i];
+  };
+
+  const int64_t r_begin = item.y, r_end = item.z;
+  for (int64_t t0 = r_begin; t0 < r_end; t0 += TR) {
+    const int64_t r = t0 + trow;
+    const bool valid = r < r_end;
+    const int64_t rc = valid ? r : r_begin;
+    const int lr = (int)(rc - row_base);
+    const int s = lr / n_entries;
+    const int e = lr - s * n_entries;
+    T x[MAXD];
+    const int64_t base = a.ints[slot_base_i + s];
+#pragma unroll
+    for (int k = 0; k < MAXD; ++k)
+      x[k] = (k < d_in) ? a.data[base + (int64_t)e * d_in + k] + a.reals[slot_off_r + s * d_in + k]
+                        : T(0);
+    if (q == 0) {
+#pragma unroll
+      for (int k = 0; k < MAXD; ++k) xbuf[k * CTRS + trow] = x[k];
+    }
+
+    // forward recompute of this wavefront's 16 rows, every layer to LDS
+    T act[NB][4];
+    layer_in<T, NB>(net, x, q, act);
+    store_rows(layer_buf(0), act);
+#pragma unroll 1
+    for (int j = 1; j <= NH; ++j) {
+      v4 acc[NB];
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob) acc[ob] = ld4(net.bh + (j - 1) * HP + 16 * ob + 4 * q);
+      wave_sync();  // the wavefront's own rows of layer j-1 are in LDS
+      mma_layer_fm<T, NB>(net.Wf + (j - 1) * HP * HP, lane, layer_buf(j - 1), trow, acc);
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) act[ob][i] = dtanh(acc[ob][i]);
+      store_rows(layer_buf(j), act);
+    }
+
+    // seeds and the delta of the last hidden activation (act holds it)
+    T dl[NB][4];
+#pragma unroll
+    for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dl[mb][i] = T(0);
+#pragma unroll
+    for (int o = 0; o < MAXO; ++o) {
+      if (o < n_out) {
+        const T so = valid ? a.seeds[rc * n_out + o] * dl_scale : T(0);
+        if (o == q) blacc += so;
+        if (q == 0) sbuf[o * CTRS + trow] = so;
+#pragma unroll
+        for (int mb = 0; mb < NB; ++mb) {
+          const v4 wl = ld4(net.WL + o * HP + 16 * mb + 4 * q);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) dl[mb][i] = fma(wl[i], so, dl[mb][i]);
+        }
+      }
+    }
+#pragma unroll
+    for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dl[mb][i] *= (T(1) - act[mb][i] * act[mb][i]);
+    __syncthreads();
+    // output layer W: columns col_wl + o of the owned small-term tiles,
+    // B = the seed column o (zero elsewhere)
+    {
+      const T *aN = layer_buf(NH);
+      const int o = row - col_wl;
+#pragma unroll
+      for (int u = 0; u < OBW; ++u) {
+        const int ob = w + 4 * u;
+        if (ob < NB) {
+#pragma unroll
+          for (int kk = 0; kk < TR / 16; ++kk) {
+            const v4 av = *reinterpret_cast<const v4 *>(aN + (16 * ob + row) * CTRS + 16 * kk + 4 * q);
+            const v4 sv = (o >= 0 && o < n_out)
+                              ? *reinterpret_cast<const v4 *>(sbuf + o * CTRS + 16 * kk + 4 * q)
+                              : v4{0, 0, 0, 0};
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) sacc[u] = mma16(av[kb], sv[kb], sacc[u]);
+          }
+        }
+      }
+    }
+
+#pragma unroll
+    for (int j = NH; j >= 1; --j) {
+      const int jj = j - 1;
+      const T *ain = layer_buf(j - 1);
+      store_rows(dbuf, dl);
+      __syncthreads();
+      // owned blocks of dW_jj over the 64 rows; hidden bias from the delta
+      // operand (each lane holds 4 rows of one feature per row group)
+#pragma unroll
+      for (int u = 0; u < OBW; ++u) {
+        const int ob = w + 4 * u;
+        if (ob < NB) {
+          // row groups outermost (not unrolled): only one group's operands
+          // are live; each block still sums its rows in the same order
+#pragma unroll 1
+          for (int kk = 0; kk < TR / 16; ++kk) {
+            const v4 dv = *reinterpret_cast<const v4 *>(dbuf + (16 * ob + row) * CTRS + 16 * kk + 4 * q);
+            bhacc[jj][u] += (dv[0] + dv[1]) + (dv[2] + dv[3]);
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+              const v4 av = *reinterpret_cast<const v4 *>(ain + (16 * ib + row) * CTRS + 16 * kk + 4 * q);
+              v4 c = dwacc[jj][u][ib];
+#pragma unroll
+              for (int kb = 0; kb < 4; ++kb) c = mma16(dv[kb], av[kb], c);
+              dwacc[jj][u][ib] = c;
+            }
+          }
+        }
+      }
+      // delta_{j-1} = (W_jj^T delta_j) * (1 - act_{j-1}^2) for this
+      // wavefront's rows; act_{j-1} of the own rows comes back from LDS
+      v4 c[NB];
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib) c[ib] = v4{0, 0, 0, 0};
+      mma_layer_fm<T, NB>(net.Wb + jj * HP * HP, lane, dbuf, trow, c);
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const T av = ain[(16 * ib + crow<T>(q, i)) * CTRS + trow];
+          dl[ib][i] = c[ib][i] * (T(1) - av * av);
+        }
+      __syncthreads();
+    }
+    // first layer: columns c < d_in of the small-term tiles (B = coordinate
+    // c) and column d_in (B = 1)
+    store_rows(dbuf, dl);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < OBW; ++u) {
+      const int ob = w + 4 * u;
+      if (ob < NB) {
+#pragma unroll
+        for (int kk = 0; kk < TR / 16; ++kk) {
+          const v4 dv = *reinterpret_cast<const v4 *>(dbuf + (16 * ob + row) * CTRS + 16 * kk + 4 * q);
+          const v4 xv = row < d_in
+                            ? *reinterpret_cast<const v4 *>(xbuf + row * CTRS + 16 * kk + 4 * q)
+                            : (row == col_b1 ? v4{1, 1, 1, 1} : v4{0, 0, 0, 0});
+#pragma unroll
+          for (int kb = 0; kb < 4; ++kb) sacc[u] = mma16(dv[kb], xv[kb], sacc[u]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // each wavefront writes the blocks it owns; the output bias is the only
+  // sum over wavefronts (fixed order)
+  T *out = a.partial + (int64_t)blockIdx.x * P;
+#pragma unroll
+  for (int u = 0; u < OBW; ++u) {
+    const int ob = w + 4 * u;
+    if (ob < NB) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int of = 16 * ob + crow<T>(q, i);
+        if (of >= H) continue;
+#pragma unroll
+        for (int jj = 0; jj < NH; ++jj)
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib) {
+            const int in = 16 * ib + row;
+            if (in < H) out[L.oWh(jj) + of * H + in] = dwacc[jj][u][ib][i];
+          }
+        const T sv = sacc[u][i];
+        if (row < d_in) out[L.oW1() + of * d_in + row] = sv;
+        else if (row == col_b1) out[L.ob1() + of] = sv;
+        else if (row < col_wl + n_out) out[L.oWL() + (row - col_wl) * H + of] = sv;
+      }
+#pragma unroll
+      for (int jj = 0; jj < NH; ++jj) {
+        T v = bhacc[jj][u];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        const int f = 16 * ob + row;
+        if (q == 0 && f < H) out[L.obh(jj) + f] = v;
+      }
+    }
+  }
+  const T bl = rowsum16(blacc);
+  if (row == 0) red[4 * w + q] = bl;
+  __syncthreads();
+  if ((int)threadIdx.x < n_out) {
+    const int o = threadIdx.x;
+    out[L.obL() + o] = ((red[o] + red[4 + o]) + red[8 + o]) + red[12 + o];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // reductions and the optimizer
 // ---------------------------------------------------------------------------
@@ -1134,6 +1417,7 @@ struct tdx_nn_plan {
   int k1_stage;
   int k2_wave_private;
   int k2_nt;          // tiles per wavefront iteration of the wave-private backward
+  int k2_block;       // block-owning backward (wide float networks)
   int acc_lds;
   size_t k1_lds, k2_lds;
   DevTable groups, ints, reals, data, k1_items, k2_items, ik1p, ik1i, ik2p, ik2i;
@@ -1252,6 +1536,30 @@ int launch_backward_wt(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) 
   return TDX_OK;
 }
 
+// Accumulator registers of the block-owning backward (per lane).
+__host__ __device__ constexpr int block_acc_regs(int NB, int NH) {
+  return NH * ((NB + 3) / 4) * NB * 4;
+}
+
+template <typename T, int NB, int NH>
+int launch_backward_c(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
+  if constexpr (block_acc_regs(NB, NH) <= 192) {
+    static bool attr_set = false;
+    if (!attr_set) {
+      (void)hipFuncSetAttribute((const void *)k_backward_c<T, NB, NH>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      attr_set = true;
+    }
+    ProfScope ps(p, 1, s);
+    hipLaunchKernelGGL((k_backward_c<T, NB, NH>), dim3(p->n_k2), dim3(256), p->k2_lds, s, a);
+    TDX_TRY_HIP(hipGetLastError());
+    return TDX_OK;
+  } else {
+    set_error("block-owning backward: too many accumulators");
+    return TDX_EUNSUPPORTED;
+  }
+}
+
 template <typename T, int NB, int NH>
 int launch_backward_w(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
   if constexpr (sizeof(T) == 4) {
@@ -1274,6 +1582,19 @@ int launch_backward(const tdx_nn_plan *p, NNArgs<T> a, hipStream_t s) {
       }
     }
     set_error("no wave-private backward for this shape");
+    return TDX_EUNSUPPORTED;
+  }
+  if (p->k2_block) {
+    if constexpr (sizeof(T) == 4 && NB >= 3) {
+      switch (p->NH) {
+        case 1: return launch_backward_c<T, NB, 1>(p, a, s);
+        case 2: return launch_backward_c<T, NB, 2>(p, a, s);
+        case 3: return launch_backward_c<T, NB, 3>(p, a, s);
+        case 4: return launch_backward_c<T, NB, 4>(p, a, s);
+        default: break;
+      }
+    }
+    set_error("no block-owning backward for this shape");
     return TDX_EUNSUPPORTED;
   }
   static bool attr_set = false;
@@ -1484,9 +1805,19 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
     p->acc_lds = 1;
     p->k2_lds = wave_private_lds(p, d) * ts;
   }
+  // wide float networks: block-owning backward with register accumulators
+  p->k2_block = (!p->k2_wave_private && d->dtype == 0 && p->NB >= 3 && p->NB <= 8 &&
+                 p->NH >= 1 && p->NH <= 4 && block_acc_regs(p->NB, p->NH) <= 192 &&
+                 d->tile_rows >= 0 && d->d_in + 1 + d->n_out <= 16 &&
+                 block_lds_words(p->HP, p->NH) * ts <= 160 * 1024) ? 1 : 0;
+  if (p->k2_block) {
+    p->k2_waves = 4;
+    p->acc_lds = 1;
+    p->k2_lds = block_lds_words(p->HP, p->NH) * ts;
+  }
   const int TR = 16 * p->k2_waves;
   const int chunk_tiles = d->chunk_tiles > 0 ? d->chunk_tiles
-                        : (p->k2_wave_private ? 256 : (p->acc_lds ? 16 : 4));
+                        : (p->k2_wave_private ? 256 : (p->k2_block ? 8 : (p->acc_lds ? 16 : 4)));
   std::vector<int4> k2;
   for (int g = 0; g < d->n_groups; ++g) {
     const int32_t *G = d->groups + (int64_t)g * TDX_GROUP_INTS;
