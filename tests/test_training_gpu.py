@@ -12,6 +12,12 @@ what the network gains over the grid method.
 * Fused and plain paths from the same initial weights and the same number
   of Adam steps: rel-L2 against the FP64 solution within 10% of each other
   (the plain path runs the reference's algorithm in stock torch ops).
+  Legendre is compared at h = 1e-2: at h = 1e-3 its u'' term (coefficient
+  1 - x^2, divided by h^2) puts a float32 rounding noise of about 1e-3
+  relative into every implementation's gradient (measured against float64:
+  CPU oracle 1.4e-3, plain GPU 1.3e-3, native kernels 1.0e-3 - 1.2e-3), and
+  single float32 trajectories decorrelate within a few hundred Adam steps;
+  at h = 1e-2 all gradients agree with float64 to about 1e-5.
 """
 import math
 
@@ -94,7 +100,9 @@ def test_network_gain_over_grid_method_legendre():
 def test_fused_and_plain_reach_equal_rel_l2(workload):
     steps = 1000
     if workload == 'legendre':
-        build = lambda: W.legendre()  # noqa: E731
+        def build():
+            d, eq, b, net, kw = W.legendre()
+            return d, eq, b, net, dict(kw, h=1e-2)
         sol = lambda p: X.legendre_p3(p[:, 0])  # noqa: E731
     else:
         nu = 0.01 / math.pi
