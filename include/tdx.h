@@ -148,6 +148,14 @@ int tdx_nn_train_step(const tdx_nn_plan *plan, void *params, void *exp_avg,
 int tdx_nn_profile(tdx_nn_plan *plan, int capacity);
 int tdx_nn_profile_read(tdx_nn_plan *plan, double *ms, int *count);
 
+/* Host-side check of the 'mat' plans' division: for the float step h and
+ * y = RN(1/h), counts the x in [1, 2) (all 2^23 significands) for which
+ * mode 0, RN(RN(x y) + (x - RN(x y) h) y) (one correction step), or mode 1,
+ * RN(x y) (the reciprocal product alone), differs from the correctly rounded
+ * x / h.  0 means exact; plans use mode 0 only for steps that give 0.
+ * No counterpart in the reference (diagnostic, no device needed).  */
+int64_t tdx_quotient_check(float h, int mode);
+
 /* ===================================================================== */
 /* 'mat' mode                                                             */
 /* ===================================================================== */

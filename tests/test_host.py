@@ -169,3 +169,33 @@ def test_native_forward_requires_device_tensors():
     prob = compile_nn(d.build('NN'), eq.equation_lst, b.build(d.variable_dict), 1)
     with pytest.raises(native.NativeUnavailable):
         NNEngine([prob], mlp_arch(net), torch.device('cpu'))
+
+
+def test_quotient_step_check():
+    """The 'mat' plans' exact division: the one-step correction of the
+    reciprocal product is the correctly rounded quotient for every float32
+    significand for the grid steps of the configs (wave 1/127, Legendre
+    2/99, 1e-3, 0.02); the reciprocal product alone is not, so the check
+    can fail; bad arguments give -1."""
+    lib = native.lib()
+    for h in (np.float32(1.0 / 127), np.float32(2.0 / 99), np.float32(1e-3), np.float32(0.02)):
+        assert lib.tdx_quotient_check(float(h), 0) == 0, h
+    assert lib.tdx_quotient_check(float(np.float32(1.0 / 127)), 1) > 0
+    assert lib.tdx_quotient_check(0.0, 0) == -1
+    assert lib.tdx_quotient_check(1.0, 7) == -1
+
+
+def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
+    """Tuning choices are stored per plan shape in the JSON cache and read
+    back; TDX_TUNE_CACHE=off keeps nothing."""
+    from torch_de_solver_amd import autotune as A
+    path = tmp_path / 'tuning.json'
+    monkeypatch.setenv('TDX_TUNE_CACHE', str(path))
+    A._store('nn|key', {'chunk_tiles': 32, 'ms': {'32': 1.0}})
+    A._store('mat|key', {'tile_rows': 16, 'ms': {'16': 2.0}})
+    data = A._load()
+    assert data['nn|key']['chunk_tiles'] == 32 and data['mat|key']['tile_rows'] == 16
+    monkeypatch.setenv('TDX_TUNE_CACHE', 'off')
+    assert A._load() == {}
+    A._store('x', {'tile_rows': 4})
+    assert not (tmp_path / 'x').exists()

@@ -754,6 +754,27 @@ namespace {
 // result then holds for all x in the range qdiv guards, given 2^-30 <= d <=
 // 2^30.  The reference quotient is computed in double and rounded once to
 // float, which is the correctly rounded float quotient (53 >= 2*24 + 2).
+int64_t quotient_mismatches(float d, int mode, bool stop_at_first) {
+  const float y = (float)(1.0 / (double)d);
+  int64_t bad = 0;
+  for (uint32_t m = 0; m < (1u << 23); ++m) {
+    const uint32_t bits = 0x3f800000u | m;
+    float x;
+    std::memcpy(&x, &bits, 4);
+    const float q = x * y;
+    float res = q;
+    if (mode == 0) {
+      const float r = std::fma(-q, d, x);
+      res = std::fma(r, y, q);
+    }
+    if (res != (float)((double)x / (double)d)) {
+      ++bad;
+      if (stop_at_first) break;
+    }
+  }
+  return bad;
+}
+
 bool quotient_step_exact(float d) {
   static std::mutex mu;
   static std::map<uint32_t, bool> cache;
@@ -764,17 +785,8 @@ bool quotient_step_exact(float d) {
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
   }
-  const float y = (float)(1.0 / (double)d);
-  bool ok = std::isfinite(d) && d >= 0x1p-30f && d <= 0x1p30f;
-  for (uint32_t m = 0; ok && m < (1u << 23); ++m) {
-    const uint32_t bits = 0x3f800000u | m;
-    float x;
-    std::memcpy(&x, &bits, 4);
-    const float q = x * y;
-    const float r = std::fma(-q, d, x);
-    const float q2 = std::fma(r, y, q);
-    ok = q2 == (float)((double)x / (double)d);
-  }
+  const bool ok = std::isfinite(d) && d >= 0x1p-30f && d <= 0x1p30f &&
+                  quotient_mismatches(d, 0, true) == 0;
   std::lock_guard<std::mutex> g(mu);
   cache[key] = ok;
   return ok;
@@ -1152,6 +1164,14 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   p->ws_bytes = off;
   *out = p;
   return TDX_OK;
+}
+
+extern "C" int64_t tdx_quotient_check(float h, int mode) {
+  if (!(std::isfinite(h) && h > 0.0f) || (mode != 0 && mode != 1)) {
+    set_error("tdx_quotient_check: h must be a positive finite float, mode 0 or 1");
+    return -1;
+  }
+  return quotient_mismatches(h, mode, false);
 }
 
 extern "C" void tdx_mat_plan_destroy(tdx_mat_plan *p) {

@@ -28,7 +28,7 @@ EXPORTS = [
     'tdx_nn_train_step', 'tdx_nn_profile', 'tdx_nn_profile_read',
     'tdx_mat_plan_create', 'tdx_mat_plan_destroy', 'tdx_mat_workspace_bytes',
     'tdx_mat_forward', 'tdx_mat_backward', 'tdx_mat_train_step',
-    'tdx_adam', 'tdx_step_increment',
+    'tdx_adam', 'tdx_step_increment', 'tdx_quotient_check',
 ]
 
 
@@ -95,6 +95,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.tdx_nn_profile.argtypes = [vp, ctypes.c_int]
     lib.tdx_nn_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int)]
+    lib.tdx_quotient_check.argtypes = [ctypes.c_float, ctypes.c_int]
+    lib.tdx_quotient_check.restype = i64
     lib.tdx_mat_plan_create.argtypes = [ctypes.POINTER(MatDesc), ctypes.POINTER(vp)]
     lib.tdx_mat_plan_destroy.argtypes = [vp]
     lib.tdx_mat_plan_destroy.restype = None
