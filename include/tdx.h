@@ -204,8 +204,16 @@ typedef struct tdx_mat_desc {
   const void *reals;
   int64_t n_data;
   const void *data;
-  int exact_div; /* 1: correctly rounded division by 2h and h (bit-identical fields) */
+  int exact_div; /* division by 2h and h: 0 product with the rounded reciprocal;
+                    1 correctly rounded (bit-identical fields), by a checked
+                    correction step where it is exact for the step, else by
+                    IEEE division; 2 correctly rounded by IEEE division */
   int tile_rows; /* output rows per tile workgroup (2-D grids); 0: chosen by the plan */
+  int specialize; /* float 2-D grids: interior points by a kernel generated and
+                     compiled (hipRTC) for this program at plan creation.
+                     0: when the interior holds at least 2^20 points;
+                     1: whenever the grid has an interior (errors are
+                     reported); -1: never */
 } tdx_mat_desc;
 
 typedef struct tdx_mat_plan tdx_mat_plan;
@@ -213,6 +221,17 @@ typedef struct tdx_mat_plan tdx_mat_plan;
 int tdx_mat_plan_create(const tdx_mat_desc *desc, tdx_mat_plan **out);
 void tdx_mat_plan_destroy(tdx_mat_plan *plan);
 size_t tdx_mat_workspace_bytes(const tdx_mat_plan *plan);
+/* Grid points the plan's specialized kernel covers (0: none).  Diagnostic;
+ * no counterpart in the reference. */
+int64_t tdx_mat_specialized_points(const tdx_mat_plan *plan);
+/* Source of the kernel a plan of `desc` generates for its interior (float
+ * 2-D grids), copied into buf (cap bytes, NUL-terminated; buf may be NULL);
+ * with compile != 0 also compiled by hipRTC for gfx950 (no device needed).
+ * Returns the source length, 0 when the program has no specialized kernel
+ * (reason in tdx_last_error) or a negative error code.  Diagnostic; no
+ * counterpart in the reference. */
+int64_t tdx_mat_specialized_source(const tdx_mat_desc *desc, char *buf,
+                                   int64_t cap, int compile);
 
 /* Forward: loss[0] (lambda-weighted), loss[1] (normalized); op receives the
  * residual field [n_eq][n0*n1] (may be NULL); bval receives the boundary

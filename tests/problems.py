@@ -197,3 +197,97 @@ MAT_CASES = {
 def mat_init(shape, seed=0, dtype=torch.float32, std=0.01):
     g = torch.Generator().manual_seed(seed)
     return (std * torch.randn(shape, generator=g, dtype=torch.float64)).to(dtype)
+
+
+# 2-D float 'mat' programs for the kernel a plan generates for its own
+# program: builder(api, nx, nt, dp) -> (domain, equation, conditions, dp,
+# number of equations).
+
+
+def spec_wave(api, nx, nt, dp):
+    d = api.Domain()
+    d.variable('x', [0, 1], nx)
+    d.variable('t', [0, 1], nt)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=torch.sin(np.pi * x))
+    b.dirichlet({'x': 0, 't': [0, 1]}, value=0)
+    b.dirichlet({'x': 1, 't': [0, 1]}, value=0)
+    b.operator({'x': [0, 1], 't': 0}, operator={'du/dt': {'coeff': 1, 'du/dt': [1], 'pow': 1}},
+               value=0)
+    eq = api.Equation()
+    eq.add(wave_eq())
+    return d, eq, b, dp, 1
+
+
+def spec_nonlinear(api, nx, nt, dp):
+    d = api.Domain()
+    d.variable('x', [0, 1], nx)
+    d.variable('t', [0, 0.5], nt)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=torch.cos(np.pi * x))
+    b.periodic([{'x': 0, 't': [0, 0.5]}, {'x': 1, 't': [0, 0.5]}])
+    eq = api.Equation()
+    eq.add({
+        'u_t': {'coeff': 1, 'u_t': [1], 'pow': 1, 'var': 0},
+        'u*u_x': {'coeff': 1, 'uux': [[None], [0]], 'pow': [1, 1], 'var': [0, 0]},
+        'uxx^2': {'coeff': -0.1, 'uxx': [0, 0], 'pow': 2, 'var': 0},
+    })
+    return d, eq, b, dp, 1
+
+
+def spec_mixed(api, nx, nt, dp):
+    """u_t + 0.01 u_xxx + c(x, t) u u_x - 0.5 u^1.5 + 0.2 u_xt + u_tt^3 = 0,
+    c = 1 + x t (a grid field); u near 1 keeps the real power defined."""
+    d = api.Domain()
+    d.variable('x', [0, 1], nx)
+    d.variable('t', [0, 1], nt)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=1 + 0.1 * torch.sin(np.pi * x))
+    b.dirichlet({'x': 0, 't': [0, 1]}, value=1)
+    eq = api.Equation()
+    eq.add({
+        'u_t': {'coeff': 1, 'u_t': [1], 'pow': 1, 'var': 0},
+        'u_xxx': {'coeff': 0.01, 'u_xxx': [0, 0, 0], 'pow': 1, 'var': 0},
+        'c u u_x': {'coeff': lambda g: 1 + g[0] * g[1], 'uux': [[None], [0]], 'pow': [1, 1],
+                    'var': [0, 0]},
+        'u^1.5': {'coeff': -0.5, 'u': [None], 'pow': 1.5, 'var': 0},
+        'u_xt': {'coeff': 0.2, 'u_xt': [0, 1], 'pow': 1, 'var': 0},
+        'u_tt^3': {'coeff': 1e-3, 'u_tt': [1, 1], 'pow': 3, 'var': 0},
+    })
+    return d, eq, b, dp, 1
+
+
+def spec_system(api, nx, nt, dp):
+    """u_t - v_x = 0, v_t - u_x + c(x, t) u = 0 (two planes)."""
+    d = api.Domain()
+    d.variable('x', [0, 1], nx)
+    d.variable('t', [0, 1], nt)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=torch.sin(np.pi * x), var=0)
+    b.dirichlet({'x': [0, 1], 't': 0}, value=0, var=1)
+    eq = api.Equation()
+    eq.add({
+        'u_t': {'coeff': 1, 'u_t': [1], 'pow': 1, 'var': 0},
+        '-v_x': {'coeff': -1, 'v_x': [0], 'pow': 1, 'var': 1},
+    })
+    eq.add({
+        'v_t': {'coeff': 1, 'v_t': [1], 'pow': 1, 'var': 1},
+        '-u_x': {'coeff': -1, 'u_x': [0], 'pow': 1, 'var': 0},
+        'c u': {'coeff': lambda g: 0.5 * g[0] - g[1], 'u': [None], 'pow': 1, 'var': 0},
+    })
+    return d, eq, b, dp, 2
+
+
+SPECIALIZED_CASES = {
+    # name: (builder, nx, nt, dp, initial mean)
+    'wave_dp2': (spec_wave, 300, 262, 2, 0.0),
+    'wave_dp3': (spec_wave, 256, 300, 3, 0.0),
+    'nonlinear_dp2': (spec_nonlinear, 279, 230, 2, 0.0),
+    'mixed_dp2': (spec_mixed, 260, 211, 2, 1.0),
+    'mixed_dp3': (spec_mixed, 199, 240, 3, 1.0),
+    'system_dp2': (spec_system, 230, 250, 2, 0.0),
+}

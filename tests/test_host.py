@@ -199,3 +199,50 @@ def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
     assert A._load() == {}
     A._store('x', {'tile_rows': 4})
     assert not (tmp_path / 'x').exists()
+
+
+def _spec_tables(name, dtype=np.float32):
+    from torch_de_solver_amd.mat_engine import MatTables
+    from torch_de_solver_amd.solution import Operator_bcond_preproc, _operators
+    builder, nx, nt, dp_, _ = problems.SPECIALIZED_CASES[name]
+    d, eq, b, dp, n_eq = builder(api, nx, nt, dp_)
+    grid = d.build('mat')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict)).set_strategy('mat')
+    ops = _operators(pre.operator)
+    prob = compile_mat(grid, ops, pre.bconds, n_eq=len(ops), derivative_points=dp,
+                       lambda_operator=1, lambda_bound=10)
+    return MatTables(prob, dtype)
+
+
+@pytest.mark.parametrize('name', list(problems.SPECIALIZED_CASES))
+def test_specialized_kernel_generates_and_compiles(name):
+    """The kernel a float 2-D 'mat' plan generates for its program compiles
+    for gfx950 (hipRTC, no device), for every division mode; each variable
+    gets one gradient store and each equation one residual store per row."""
+    t = _spec_tables(name)
+    src = t.specialized_source(1, compile=True)
+    assert src
+    n_eq = t.n_eq
+    adj = src[src.index('void tdx_mat_jit_adj'):src.index('void tdx_mat_jit_fwd')]
+    steps = adj.count('const int gr = t - ')
+    assert steps >= 1
+    assert adj.count('a.grad[') == steps * n_eq
+    assert adj.count('a.op[') == steps * n_eq
+    for mode in (0, 2):
+        s2 = t.specialized_source(mode)
+        assert s2 and s2 != src
+
+
+def test_specialized_kernel_not_for_1d_or_double():
+    t = _spec_tables('wave_dp2', np.float64)
+    assert t.specialized_source(1) == ''
+    assert 'float 2-D' in native.lib().tdx_last_error().decode()
+    d, eq, b, kw = problems.legendre_mat(api)
+    from torch_de_solver_amd.mat_engine import MatTables
+    from torch_de_solver_amd.solution import Operator_bcond_preproc, _operators
+    grid = d.build('mat')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict)).set_strategy('mat')
+    ops = _operators(pre.operator)
+    prob = compile_mat(grid, ops, pre.bconds, n_eq=1, derivative_points=2,
+                       lambda_operator=1, lambda_bound=10)
+    assert MatTables(prob, np.float32).specialized_source(1) == ''

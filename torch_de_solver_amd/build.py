@@ -3,7 +3,9 @@
 The library is built in-tree with hipcc; nothing is compiled at import time.
 Every HIP source is compiled with floating-point contraction off (the
 stencil and residual arithmetic must keep the operation order written in the
-kernels) and with packed FP32 instructions disabled.
+kernels) and with packed FP32 instructions disabled.  The library links
+hipRTC, which compiles the 'mat' kernels generated per program (same
+options).
 """
 
 import os
@@ -19,6 +21,7 @@ LIB = os.path.join(HERE, 'libtdx.so')
 ARCH = os.environ.get('TDX_OFFLOAD_ARCH', 'gfx950')
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+ROCM_LIB = '/opt/rocm/lib'
 FLAGS = ['-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
          '-ffp-contract=off',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
@@ -45,7 +48,8 @@ def build(verbose: bool = True) -> str:
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(lambda s: _compile(s, build_dir), srcs))
     tmp = LIB + '.tmp'
-    cmd = [HIPCC, '-shared', f'--offload-arch={ARCH}', '-o', tmp, *objs]
+    cmd = [HIPCC, '-shared', f'--offload-arch={ARCH}', '-o', tmp, *objs,
+           '-L', ROCM_LIB, '-lhiprtc', f'-Wl,-rpath,{ROCM_LIB}']
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")

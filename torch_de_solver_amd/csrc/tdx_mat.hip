@@ -24,6 +24,7 @@
 // products with the rounded reciprocals.
 
 #include "tdx_common.h"
+#include "tdx_mat_jit.h"
 
 #include <algorithm>
 #include <cmath>
@@ -70,6 +71,7 @@ struct MatArgs {
   int E0, E1;         // tile with its halo of 2R per side; E1 is 64 * 2^k
   int tpr_log;        // log2 of the threads per tile row, min(E1, 256)
   int tiles1;         // tiles along axis 1
+  const int32_t *tile_list;  // tiles of this launch; null: every tile
   // bc
   int n_bc, bc_i;
   int n_touch, bct_ptr_i, bct_pos_i, bct_seed_i, bct_w_i;
@@ -551,7 +553,7 @@ template <typename T, int DP, int DIV, bool ADJ>
 __global__ __launch_bounds__(MAT_THREADS) void k_mat_tile(MatArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
-  const int tile = xcd_band_tile((int)blockIdx.x, (int)gridDim.x);
+  const int tile = a.tile_list ? a.tile_list[blockIdx.x] : xcd_band_tile((int)blockIdx.x, (int)gridDim.x);
   const int t0 = tile / a.tiles1, t1 = tile - (tile / a.tiles1) * a.tiles1;
   const int base0 = t0 * a.T0 - 2 * a.R0, base1 = t1 * a.T1 - 2 * a.R1;
   const bool interior = base0 >= 0 && base0 + a.E0 <= a.n0 && base1 >= 0 && base1 + a.E1 <= a.n1;
@@ -740,7 +742,15 @@ struct tdx_mat_plan {
   size_t tile_lds, tile_lds_fwd;
   DevTable ints, reals, data, fs_tab, coef_tab, pint_tab, preal_tab;
   int n_pint, n_preal, pterm0, pfac0;
-  int div_checked;  // float quotients by the correction step are exact
+  int div_mode;     // division mode of the launches (0 product, 1 IEEE, 2 corrected product)
+  // Specialized interior kernel: output rows [jr_lo, jr_hi) x columns
+  // [jc_lo, jc_hi), the union of the interior tiles; the general kernel
+  // takes the n_gen_wg tiles of gen_list (every tile when jit_on is 0).
+  int jit_on;
+  JitKernel jit;
+  int jr_lo, jr_hi, jc_lo, jc_hi, j_hb, j_nstrips, j_nbands, j_waves;
+  DevTable gen_list;
+  int n_gen_wg;
   size_t off_lp, off_mid, off_seeds, off_one, off_grad, ws_bytes;
   double ab[MMAX_DP], af[MMAX_DP];
 };
@@ -830,6 +840,7 @@ MatArgs<T> mat_args(const tdx_mat_plan *p, const void *u, void *ws) {
   a.E1 = p->E1;
   a.tpr_log = p->tpr_log;
   a.tiles1 = p->tiles1;
+  a.tile_list = p->jit_on ? (const int32_t *)p->gen_list.ptr : nullptr;
   a.n_bc = d.n_bc;
   a.bc_i = d.bc_i;
   a.n_touch = d.n_touch;
@@ -846,7 +857,7 @@ MatArgs<T> mat_args(const tdx_mat_plan *p, const void *u, void *ws) {
   a.lp_mid = (T *)(w + p->off_mid);
   a.bc_seeds = (T *)(w + p->off_seeds);
   a.dloss = (const T *)(w + p->off_one);
-  a.n_tile_wg = p->n_tile_wg;
+  a.n_tile_wg = p->n_gen_wg;
   return a;
 }
 
@@ -860,13 +871,45 @@ int mat_tile_launch(const tdx_mat_plan *p, const MatArgs<T> &a, bool adj, hipStr
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr = true;
   }
+  if (p->n_gen_wg == 0) return TDX_OK;
   if (adj)
-    hipLaunchKernelGGL((k_mat_tile<T, DP, DIV, true>), dim3(p->n_tile_wg), dim3(MAT_THREADS),
+    hipLaunchKernelGGL((k_mat_tile<T, DP, DIV, true>), dim3(p->n_gen_wg), dim3(MAT_THREADS),
                        p->tile_lds, s, a);
   else
-    hipLaunchKernelGGL((k_mat_tile<T, DP, DIV, false>), dim3(p->n_tile_wg), dim3(MAT_THREADS),
+    hipLaunchKernelGGL((k_mat_tile<T, DP, DIV, false>), dim3(p->n_gen_wg), dim3(MAT_THREADS),
                        p->tile_lds_fwd, s, a);
   TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}
+
+// The specialized interior kernel: 4 wavefronts per workgroup, one per
+// (strip, band); its loss pairs follow the general tiles' and the boundary
+// workgroups'.
+template <typename T>
+int mat_jit_launch(const tdx_mat_plan *p, const MatArgs<T> &a, bool adj, hipStream_t s) {
+  JitArgs j;
+  j.u = (const float *)a.u;
+  j.grad = (float *)a.grad;
+  j.op = (float *)a.op;
+  j.lp = (float *)a.lp;
+  j.data = (const float *)a.data;
+  j.dloss = (const float *)a.dloss;
+  j.N = a.N;
+  j.n0 = a.n0;
+  j.n1 = a.n1;
+  j.r_lo = p->jr_lo;
+  j.r_hi = p->jr_hi;
+  j.c_lo = p->jc_lo;
+  j.c_hi = p->jc_hi;
+  j.hb = p->j_hb;
+  j.nstrips = p->j_nstrips;
+  j.nbands = p->j_nbands;
+  j.lp_base = p->n_gen_wg + p->n_bc_wg;
+  j.adj = adj ? 1 : 0;
+  void *params[] = {&j};
+  const unsigned blocks = (unsigned)((p->j_waves + 3) / 4);
+  TDX_TRY_HIP(hipModuleLaunchKernel(adj ? p->jit.fn_adj : p->jit.fn_fwd, blocks, 1, 1, 256, 1, 1,
+                                    0, s, params, nullptr));
   return TDX_OK;
 }
 
@@ -876,6 +919,10 @@ int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t 
            : p->d.dp == 3 ? mat_tile_launch<T, 3, DIV>(p, a, adj, s)
                           : mat_tile_launch<T, 0, DIV>(p, a, adj, s);
   if (rc) return rc;
+  if (p->jit_on) {
+    rc = mat_jit_launch<T>(p, a, adj, s);
+    if (rc) return rc;
+  }
   if (p->n_bc_wg > 0) {
     hipLaunchKernelGGL((k_mat_bc<T, DIV>), dim3(p->n_bc_wg), dim3(MAT_THREADS), 0, s, a,
                        adj ? 1 : 0);
@@ -887,7 +934,7 @@ int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t 
     }
   }
   if (loss) {
-    const int n = p->n_tile_wg + p->n_bc_wg;
+    const int n = p->n_gen_wg + p->n_bc_wg + (p->jit_on ? p->j_waves : 0);
     const int g = std::min(256, (n + 4095) / 4096);
     T *mid = a.lp_mid;
     if (g > 1) {
@@ -904,9 +951,175 @@ int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t 
 
 template <typename T>
 int mat_dispatch(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t s) {
-  if (!p->d.exact_div) return mat_run<T, 0>(p, a, adj, loss, s);
-  if (sizeof(T) == 4 && p->div_checked) return mat_run<T, 2>(p, a, adj, loss, s);
+  if (p->div_mode == 0) return mat_run<T, 0>(p, a, adj, loss, s);
+  if (p->div_mode == 2) return mat_run<T, 2>(p, a, adj, loss, s);
   return mat_run<T, 1>(p, a, adj, loss, s);
+}
+
+}  // namespace
+
+namespace {
+
+// Division mode of a plan: 0 product with the rounded reciprocal, 1 IEEE
+// division, 2 (float) the corrected reciprocal product where it is exact for
+// the steps.
+int plan_div_mode(const tdx_mat_desc *d) {
+  if (d->exact_div == 0) return 0;
+  if (d->dtype == 0 && d->exact_div == 1 && quotient_step_exact((float)d->h[1]) &&
+      (d->n0 == 1 || quotient_step_exact((float)d->h[0])))
+    return 2;
+  return 1;
+}
+
+// The residual program of a float 2-D descriptor as a generator spec, with
+// the tables' ranges checked.  Returns false with the reason in *why.
+bool jit_spec_from_desc(const tdx_mat_desc *d, int div_mode, JitSpec *sp, std::string *why) {
+  if (d->dtype != 0 || d->n0 < 2 || d->n1 < 2 || d->dp < 2 || d->dp > MMAX_DP || d->n_eq < 1 ||
+      d->n_eq > MMAX_EQ) {
+    *why = "not a float 2-D grid";
+    return false;
+  }
+  const int64_t N = (int64_t)d->n0 * d->n1;
+  auto in_ints = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_ints; };
+  auto in_reals = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_reals; };
+  auto in_data = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_data; };
+  auto real_at = [&](int64_t k) { return (double)((const float *)d->reals)[k]; };
+  sp->n_eq = d->n_eq;
+  sp->dp = d->dp;
+  sp->div_mode = div_mode;
+  sp->N = N;
+  for (int i = 0; i < 8; ++i) {
+    sp->ab[i] = i < d->dp ? (float)d->alpha_back[i] : 0.0f;
+    sp->af[i] = i < d->dp ? (float)d->alpha_forw[i] : 0.0f;
+  }
+  for (int ax = 0; ax < 2; ++ax) {
+    sp->h2[ax] = (float)(2.0 * d->h[ax]);
+    sp->ih2[ax] = (float)(1.0 / (2.0 * d->h[ax]));
+    for (int i = 0; i < 8; ++i) {
+      // the transposed products ab/(2h), af/(2h) rounded once, as in the
+      // general kernel's coefficient table
+      sp->tb2[ax][i] = i < d->dp ? (float)((double)d->alpha_back[i] / (2.0 * d->h[ax])) : 0.0f;
+      sp->tf2[ax][i] = i < d->dp ? (float)((double)d->alpha_forw[i] / (2.0 * d->h[ax])) : 0.0f;
+    }
+  }
+  int cnt0 = 0, cnt1 = 0, n_fs = 0;
+  sp->eqs.clear();
+  if (!in_ints(d->eq_i, (int64_t)d->n_eq * TDX_MEQ_INTS)) {
+    *why = "eq table out of range";
+    return false;
+  }
+  for (int q = 0; q < d->n_eq; ++q) {
+    const int32_t *E = d->ints + d->eq_i + q * TDX_MEQ_INTS;
+    if (E[0] < 1 || !in_ints(E[1], (int64_t)E[0] * TDX_MTERM_INTS) || !in_reals(E[2], 1)) {
+      *why = "term table out of range";
+      return false;
+    }
+    JitEq eq;
+    eq.lam = real_at(E[2]);
+    for (int t = 0; t < E[0]; ++t) {
+      const int32_t *Tm = d->ints + E[1] + t * TDX_MTERM_INTS;
+      if (Tm[0] < 1 || Tm[0] > MMAX_TF || !in_ints(Tm[1], (int64_t)Tm[0] * TDX_MFACTOR_INTS) ||
+          !((Tm[2] == 0 && in_reals(Tm[3], 1)) || (Tm[2] == 1 && in_data(Tm[3], N)))) {
+        *why = "term record out of range";
+        return false;
+      }
+      JitTerm tm;
+      tm.coeff_kind = Tm[2];
+      tm.coeff = Tm[2] == 0 ? real_at(Tm[3]) : 0.0;
+      tm.data_off = Tm[2] == 0 ? 0 : Tm[3];
+      for (int f = 0; f < Tm[0]; ++f) {
+        const int32_t *F = d->ints + Tm[1] + f * TDX_MFACTOR_INTS;
+        if (F[0] < 0 || F[0] >= d->n_eq || F[4] < 0 || F[4] > 3 ||
+            (F[1] != 0 && !in_reals(F[3], 1)) || ++n_fs > MMAX_FS) {
+          *why = "factor record out of range";
+          return false;
+        }
+        JitFactor jf;
+        jf.var = F[0];
+        jf.nax = F[4];
+        int c0 = 0, c1 = 0;
+        for (int k = 0; k < 3; ++k) {
+          jf.ax[k] = k < F[4] ? F[5 + k] : 0;
+          if (k < F[4] && F[5 + k] != 0 && F[5 + k] != 1) {
+            *why = "axis out of range";
+            return false;
+          }
+          if (k < F[4]) (F[5 + k] == 0 ? c0 : c1)++;
+        }
+        cnt0 = std::max(cnt0, c0);
+        cnt1 = std::max(cnt1, c1);
+        jf.pow_kind = F[1] != 0 ? 1 : 0;
+        jf.pow_int = F[2];
+        jf.pow_real = F[1] != 0 ? real_at(F[3]) : 0.0;
+        tm.f.push_back(jf);
+      }
+      eq.terms.push_back(tm);
+    }
+    sp->eqs.push_back(eq);
+  }
+  sp->R0 = cnt0 * (d->dp - 1);
+  sp->R1 = cnt1 * (d->dp - 1);
+  return true;
+}
+
+// Sets up the specialized interior kernel of a float 2-D plan (see the
+// specialize field of tdx_mat_desc): the interior tiles of the plan's tiling
+// -- those whose halo lies inside the grid, so every stencil they apply or
+// transpose is central -- form a rectangle of output points that the
+// generated kernel evaluates; the general kernel keeps the other tiles.
+// Returns an error only when specialization was required (specialize = 1).
+int mat_specialize(tdx_mat_plan *p, const tdx_mat_desc *d) {
+  if (d->dtype != 0 || d->n0 == 1 || d->specialize < 0) return TDX_OK;
+  const bool required = d->specialize > 0;
+  const int T0 = p->T0, T1 = p->T1, R0 = p->R0, R1 = p->R1;
+  // tile t along an axis is interior when t*T >= 2R and t*T + T + 2R <= n
+  const int ta0 = (2 * R0 + T0 - 1) / T0, ta1 = (2 * R1 + T1 - 1) / T1;
+  const int tb0 = d->n0 - T0 - 2 * R0 >= 0 ? (d->n0 - T0 - 2 * R0) / T0 + 1 : 0;
+  const int tb1 = d->n1 - T1 - 2 * R1 >= 0 ? (d->n1 - T1 - 2 * R1) / T1 + 1 : 0;
+  if (tb0 <= ta0 || tb1 <= ta1) return TDX_OK;
+  const int64_t pts = (int64_t)(tb0 - ta0) * T0 * (tb1 - ta1) * T1;
+  if (!required && pts < (1LL << 20)) return TDX_OK;
+  JitSpec sp;
+  std::string why;
+  if (!jit_spec_from_desc(d, p->div_mode, &sp, &why)) {
+    set_error("%s", why.c_str());
+    return TDX_EINVAL;
+  }
+  const std::string src = mat_jit_source(sp, &why);
+  if (src.empty()) {
+    if (!required) return TDX_OK;
+    set_error("no specialized kernel for this program: %s", why.c_str());
+    return TDX_EINVAL;
+  }
+  const int rc = mat_jit_compile(src, &p->jit);
+  if (rc) return required ? rc : TDX_OK;
+  // geometry: a wavefront owns 64 - 4 R1 columns of a band of rows; bands
+  // of up to 256 rows, fewer while the grid would have under 8192
+  // wavefronts (32 per CU)
+  p->jr_lo = ta0 * T0;
+  p->jr_hi = tb0 * T0;
+  p->jc_lo = ta1 * T1;
+  p->jc_hi = tb1 * T1;
+  const int OW = 64 - 4 * R1;
+  p->j_nstrips = (p->jc_hi - p->jc_lo + OW - 1) / OW;
+  const int rows = p->jr_hi - p->jr_lo;
+  int hb = 256;
+  while (hb > 32 && (int64_t)p->j_nstrips * ((rows + hb - 1) / hb) < 8192) hb /= 2;
+  p->j_hb = hb;
+  p->j_nbands = (rows + hb - 1) / hb;
+  p->j_waves = p->j_nstrips * p->j_nbands;
+  // the general kernel's tiles: every tile outside the interior rectangle
+  std::vector<int32_t> list;
+  for (int t0 = 0; t0 < p->tiles0; ++t0)
+    for (int t1 = 0; t1 < p->tiles1; ++t1)
+      if (t0 < ta0 || t0 >= tb0 || t1 < ta1 || t1 >= tb1) list.push_back(t0 * p->tiles1 + t1);
+  if (!list.empty()) {
+    const int urc = p->gen_list.upload(list.data(), list.size() * 4);
+    if (urc) return urc;
+  }
+  p->n_gen_wg = (int)list.size();
+  p->jit_on = 1;
+  return TDX_OK;
 }
 
 }  // namespace
@@ -919,6 +1132,7 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   TDX_REQUIRE(d->n0 >= 1 && d->n1 >= 2, "bad grid shape");
   TDX_REQUIRE(d->dp >= 2 && d->dp <= MMAX_DP, "derivative_points must be 2..%d", MMAX_DP);
   TDX_REQUIRE(d->tile_rows >= 0 && d->tile_rows <= 1024, "tile_rows must be 0..1024");
+  TDX_REQUIRE(d->exact_div >= 0 && d->exact_div <= 2, "exact_div must be 0, 1 or 2");
   auto in_ints = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_ints; };
   auto in_reals = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_reals; };
   auto in_data = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_data; };
@@ -1102,16 +1316,15 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   p->T1 = p->E1 - 4 * p->R1;
   if (p->T0 < 1 || p->T1 < 1 || lds_for(p->E0, p->E1, true) > 160 * 1024)
     return fail("stencil reach too large for LDS");
-  p->div_checked = 0;
-  if (d->dtype == 0 && d->exact_div)
-    p->div_checked = quotient_step_exact((float)d->h[1]) &&
-                     (d->n0 == 1 || quotient_step_exact((float)d->h[0]));
+  p->div_mode = plan_div_mode(d);
   p->tile_lds = lds_for(p->E0, p->E1, true);
   p->tile_lds_fwd = lds_for(p->E0, p->E1, false);
   p->tiles0 = (d->n0 + p->T0 - 1) / p->T0;
   p->tiles1 = (d->n1 + p->T1 - 1) / p->T1;
   p->n_tile_wg = p->tiles0 * p->tiles1;
   p->n_bc_wg = (d->n_bc + MAT_THREADS - 1) / MAT_THREADS;
+  p->jit_on = 0;
+  p->n_gen_wg = p->n_tile_wg;
 
   // stencil coefficients and their transposed products, in T
   std::vector<double> coef(MCOEF_N, 0.0);
@@ -1145,6 +1358,11 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
     tdx_mat_plan_destroy(p);
     return rc;
   }
+  rc = mat_specialize(p, d);
+  if (rc) {
+    tdx_mat_plan_destroy(p);
+    return rc;
+  }
   p->d.ints = nullptr;
   p->d.reals = nullptr;
   p->d.data = nullptr;
@@ -1152,7 +1370,8 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   p->d.alpha_forw = nullptr;
   size_t off = 0;
   p->off_lp = off;
-  off = align_up(off + (size_t)(p->n_tile_wg + p->n_bc_wg) * 2 * ts, 256);
+  const int n_pairs = p->n_gen_wg + p->n_bc_wg + (p->jit_on ? p->j_waves : 0);
+  off = align_up(off + (size_t)n_pairs * 2 * ts, 256);
   p->off_seeds = off;
   off = align_up(off + (size_t)std::max(d->n_bc_seeds, 1) * ts, 256);
   p->off_one = off;
@@ -1177,6 +1396,7 @@ extern "C" int64_t tdx_quotient_check(float h, int mode) {
 extern "C" void tdx_mat_plan_destroy(tdx_mat_plan *p) {
   if (!p) return;
   p->ints.release();
+  p->gen_list.release();
   p->reals.release();
   p->data.release();
   p->fs_tab.release();
@@ -1187,6 +1407,40 @@ extern "C" void tdx_mat_plan_destroy(tdx_mat_plan *p) {
 }
 
 extern "C" size_t tdx_mat_workspace_bytes(const tdx_mat_plan *p) { return p ? p->ws_bytes : 0; }
+
+extern "C" int64_t tdx_mat_specialized_source(const tdx_mat_desc *d, char *buf, int64_t cap,
+                                              int compile) {
+  TDX_REQUIRE(d, "null argument");
+  TDX_REQUIRE(d->exact_div >= 0 && d->exact_div <= 2, "exact_div must be 0, 1 or 2");
+  TDX_REQUIRE(d->alpha_back && d->alpha_forw && d->ints && d->reals, "null table");
+  JitSpec sp;
+  std::string why;
+  if (!jit_spec_from_desc(d, plan_div_mode(d), &sp, &why)) {
+    set_error("%s", why.c_str());
+    return 0;
+  }
+  const std::string src = mat_jit_source(sp, &why);
+  if (src.empty()) {
+    set_error("%s", why.c_str());
+    return 0;
+  }
+  if (compile) {
+    std::vector<char> code;
+    const int rc = mat_jit_build(src, &code);
+    if (rc) return rc;
+  }
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>(src.size(), (size_t)(cap - 1));
+    std::memcpy(buf, src.data(), n);
+    buf[n] = '\0';
+  }
+  return (int64_t)src.size();
+}
+
+extern "C" int64_t tdx_mat_specialized_points(const tdx_mat_plan *p) {
+  if (!p || !p->jit_on) return 0;
+  return (int64_t)(p->jr_hi - p->jr_lo) * (p->jc_hi - p->jc_lo);
+}
 
 extern "C" int tdx_mat_forward(const tdx_mat_plan *p, const void *u, void *loss, void *op,
                                void *bval, void *ws, tdx_stream_t stream) {

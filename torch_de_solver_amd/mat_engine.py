@@ -174,13 +174,56 @@ class MatTables:
                      else np.zeros(1, dtype=np_dtype)).astype(np_dtype)
         self.ab = np.asarray(prob.alpha_back, dtype=np.float64)
         self.af = np.asarray(prob.alpha_forw, dtype=np.float64)
+        self.dtype_code = 0 if np_dtype == np.float32 else 1
+        self.n_eq = prob.n_eq
+        self.dp = prob.dp
+
+    def desc(self, exact_div=1, tile_rows: int = 0, specialize: int = 0) -> 'native.MatDesc':
+        """Plan descriptor over these tables (which must outlive it).
+        exact_div: 0 fast product, 1 correctly rounded, 2 IEEE division."""
+        d = native.MatDesc()
+        d.dtype = self.dtype_code
+        d.n_eq = self.n_eq
+        d.n0, d.n1 = self.n0, self.n1
+        d.dp = self.dp
+        d.h[0], d.h[1] = self.h[0], self.h[1]
+        d.alpha_back = self.ab.ctypes.data_as(native.c_dp)
+        d.alpha_forw = self.af.ctypes.data_as(native.c_dp)
+        d.eq_i = self.eq_i
+        d.n_bc, d.bc_i, d.n_bc_seeds = self.n_bc, self.bc_i, self.n_bc_seeds
+        d.n_touch = self.n_touch
+        d.bct_ptr_i, d.bct_pos_i, d.bct_seed_i, d.bct_w_i = (self.bct_ptr_i, self.bct_pos_i,
+                                                              self.bct_seed_i, self.bct_w_i)
+        d.n_ints, d.ints = self.ints.size, self.ints.ctypes.data_as(native.c_i32p)
+        d.n_reals, d.reals = self.reals.size, self.reals.ctypes.data
+        d.n_data, d.data = self.data.size, self.data.ctypes.data
+        d.exact_div = int(exact_div)
+        d.tile_rows = int(tile_rows)
+        d.specialize = int(specialize)
+        return d
+
+    def specialized_source(self, exact_div=1, compile: bool = False) -> str:
+        """Source of the interior kernel a plan over these tables generates
+        ('' when it has none); with compile, also compiled for gfx950."""
+        d = self.desc(exact_div)
+        lib = native.lib()
+        n = int(lib.tdx_mat_specialized_source(ctypes.byref(d), None, 0, 0))
+        if n < 0:
+            native.check(int(n))
+        if n == 0:
+            return ''
+        buf = ctypes.create_string_buffer(n + 1)
+        rc = int(lib.tdx_mat_specialized_source(ctypes.byref(d), buf, n + 1, 1 if compile else 0))
+        if rc < 0:
+            native.check(rc)
+        return buf.value.decode()
 
 
 class MatEngine:
     """Native 'mat' residual/loss engine for one grid tensor."""
 
     def __init__(self, prob: MatProblem, device: torch.device, exact_div: bool = True,
-                 tile_rows: int = 0):
+                 tile_rows: int = 0, specialize: int = 0):
         if prob.dtype not in (torch.float32, torch.float64):
             raise NotImplementedError(f"dtype {prob.dtype} is not supported")
         self.prob = prob
@@ -191,24 +234,7 @@ class MatEngine:
             raise native.NativeUnavailable("the native engine runs on a HIP device only")
         t = MatTables(prob, np_dtype)
         self.tables = t
-        d = native.MatDesc()
-        d.dtype = 0 if prob.dtype == torch.float32 else 1
-        d.n_eq = prob.n_eq
-        d.n0, d.n1 = t.n0, t.n1
-        d.dp = prob.dp
-        d.h[0], d.h[1] = t.h[0], t.h[1]
-        d.alpha_back = t.ab.ctypes.data_as(native.c_dp)
-        d.alpha_forw = t.af.ctypes.data_as(native.c_dp)
-        d.eq_i = t.eq_i
-        d.n_bc, d.bc_i, d.n_bc_seeds = t.n_bc, t.bc_i, t.n_bc_seeds
-        d.n_touch = t.n_touch
-        d.bct_ptr_i, d.bct_pos_i, d.bct_seed_i, d.bct_w_i = (t.bct_ptr_i, t.bct_pos_i,
-                                                              t.bct_seed_i, t.bct_w_i)
-        d.n_ints, d.ints = t.ints.size, t.ints.ctypes.data_as(native.c_i32p)
-        d.n_reals, d.reals = t.reals.size, t.reals.ctypes.data
-        d.n_data, d.data = t.data.size, t.data.ctypes.data
-        d.exact_div = 1 if exact_div else 0
-        d.tile_rows = int(tile_rows)
+        d = t.desc(exact_div, tile_rows, specialize)
         handle = ctypes.c_void_p()
         lib = native.lib()
         with torch.cuda.device(self.device):
@@ -217,6 +243,11 @@ class MatEngine:
         nbytes = int(lib.tdx_mat_workspace_bytes(handle))
         self.workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device)
         self.numel = prob.n_eq * t.N
+
+    @property
+    def specialized_points(self) -> int:
+        """Grid points evaluated by the kernel generated for this program."""
+        return int(native.lib().tdx_mat_specialized_points(self._plan))
 
     def __del__(self):
         try:

@@ -29,6 +29,7 @@ EXPORTS = [
     'tdx_mat_plan_create', 'tdx_mat_plan_destroy', 'tdx_mat_workspace_bytes',
     'tdx_mat_forward', 'tdx_mat_backward', 'tdx_mat_train_step',
     'tdx_adam', 'tdx_step_increment', 'tdx_quotient_check',
+    'tdx_mat_specialized_points', 'tdx_mat_specialized_source',
 ]
 
 
@@ -71,6 +72,7 @@ class MatDesc(ctypes.Structure):
         ('n_data', ctypes.c_int64), ('data', ctypes.c_void_p),
         ('exact_div', ctypes.c_int),
         ('tile_rows', ctypes.c_int),
+        ('specialize', ctypes.c_int),
     ]
 
 
@@ -97,6 +99,11 @@ def _declare(lib: ctypes.CDLL) -> None:
                                         ctypes.POINTER(ctypes.c_int)]
     lib.tdx_quotient_check.argtypes = [ctypes.c_float, ctypes.c_int]
     lib.tdx_quotient_check.restype = i64
+    lib.tdx_mat_specialized_points.argtypes = [vp]
+    lib.tdx_mat_specialized_points.restype = i64
+    lib.tdx_mat_specialized_source.argtypes = [ctypes.POINTER(MatDesc), ctypes.c_char_p, i64,
+                                               ctypes.c_int]
+    lib.tdx_mat_specialized_source.restype = i64
     lib.tdx_mat_plan_create.argtypes = [ctypes.POINTER(MatDesc), ctypes.POINTER(vp)]
     lib.tdx_mat_plan_destroy.argtypes = [vp]
     lib.tdx_mat_plan_destroy.restype = None
