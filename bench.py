@@ -363,8 +363,11 @@ def bench_mat(args):
     if not args.no_tune and tile_rows == 0:
         from torch_de_solver_amd.autotune import tune_mat
         tile_rows, tuning = tune_mat(prob, dev, u, exact_div=not args.fast_div)
-    eng = MatEngine(prob, dev, exact_div=not args.fast_div, tile_rows=tile_rows)
-    log(f"[bench] compiled {n}x{n} in {time.time() - t0:.1f}s (tuning: {tuning})")
+    eng = MatEngine(prob, dev, exact_div=not args.fast_div, tile_rows=tile_rows,
+                    specialize=args.specialize)
+    spec_pts = eng.specialized_points
+    log(f"[bench] compiled {n}x{n} in {time.time() - t0:.1f}s (tuning: {tuning}; "
+        f"specialized interior points: {spec_pts})")
     m, v = torch.zeros_like(u), torch.zeros_like(u)
     step = torch.zeros(1, dtype=torch.int64, device=dev)
     loss = torch.zeros(2, device=dev)
@@ -381,7 +384,9 @@ def bench_mat(args):
         g.replay()
     ms = time_cuda(g.replay, args.steps)
     N = n * n
-    # roofline of the fused residual+adjoint kernel: 8 B/point (read u, write grad)
+    # roofline of the residual+adjoint pass (every launch of tdx_mat_backward:
+    # the generated interior kernel, the general kernel's edge tiles and the
+    # boundary kernels): 8 B/point (read u, write grad)
     dl = torch.ones(1, device=dev)
     grad = torch.empty_like(u)
     bwd_ms = time_cuda(lambda: eng.backward(u, dl, grad), args.steps)
@@ -393,14 +398,20 @@ def bench_mat(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"wave u_tt=4u_xx 'mat' {n}x{n}, Adam lr 1e-3",
                    "collocation_points": N, "parallelism": "single GPU",
-                   "tile_rows": tile_rows, "autotune": tuning},
+                   "tile_rows": tile_rows, "autotune": tuning,
+                   "specialized_points": spec_pts},
         "colloc_pts_per_s": N * 1e3 / ms,
-        "roofline": {"bound": "hbm", "kernel": "k_mat_tile (residual + loss + adjoint)",
+        "roofline": {"bound": "hbm",
+                     "kernel": ("residual + loss + adjoint pass: tdx_mat_jit_adj (generated for "
+                                "the program, interior) + k_mat_tile (edge tiles) + boundary kernels")
+                     if spec_pts else "k_mat_tile (residual + loss + adjoint)",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
-                     "traffic": profiled_traffic(f"k_mat_tile_{args.workload}"),
-                     "traffic_source": f"profiles/k_mat_tile_{args.workload}_traffic.txt "
-                                       "(rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
+                     "traffic": profiled_traffic(f"mat_adjoint_{args.workload}") if spec_pts
+                     else None,
+                     "traffic_source": f"profiles/mat_adjoint_{args.workload}_traffic.txt "
+                                       "(rocprofv3 FETCH_SIZE/WRITE_SIZE summed over the pass's "
+                                       "kernels, copy-calibrated)",
                      "algorithmic_bytes_per_launch": 8.0 * N, "avg_launch_ms": bwd_ms},
     }
     if not args.no_plain and n <= 4096:
@@ -476,6 +487,9 @@ def main():
     ap.add_argument('--no-tune', action='store_true',
                     help="default tile shapes instead of the autotuner's choice")
     ap.add_argument('--tile-rows', type=int, default=0, help="'mat': output rows per tile")
+    ap.add_argument('--specialize', type=int, default=0,
+                    help="'mat': kernel generated for the program on the interior (0 auto, "
+                         "1 always, -1 never)")
     ap.add_argument('--chunk-tiles', type=int, default=0,
                     help="'NN': backward tiles per work item (0: autotuned)")
     ap.add_argument('--nn-tile-rows', type=int, default=0,

@@ -143,7 +143,10 @@ std::string mat_jit_source(const JitSpec &s, std::string *why) {
     F[sl.fwd].back = std::max(F[sl.fwd].back, LAGF - F[sl.fwd].lag);
     F[sl.out].back = std::max(F[sl.out].back, LAGG - F[sl.out].lag);
   }
-  int UNR = 1;
+  // the march is unrolled by UNR rows (a power of two at least every ring
+  // window, so ring slots are constants) and loads run one unrolled block
+  // ahead, so at least 8 rows are in flight per wavefront
+  int UNR = 8;
   for (Field &f : F) {
     f.W = pow2_at_least(f.back + 1);
     UNR = std::max(UNR, f.W);
@@ -235,24 +238,38 @@ std::string mat_jit_source(const JitSpec &s, std::string *why) {
          "  const int b_hi = min(b_lo + a.hb, a.r_hi);\n"
          "  int t0 = b_lo - " << 2 * R0 << ";\n"
          "  t0 -= ((t0 % " << UNR << ") + " << UNR << ") % " << UNR << ";\n"
-         "  const int t1 = b_hi - 1 + " << (adj ? LAGG : LAGF) << ";\n"
+         "  // last row step, rounded up to whole unrolled blocks (the extra\n"
+         "  // steps' rows lie outside the band and are not stored)\n"
+         "  const int tl = b_hi - 1 + " << (adj ? LAGG : LAGF) << ";\n"
+         "  const int t1 = t0 + ((tl - t0) / " << UNR << " + 1) * " << UNR << " - 1;\n"
          "  float lsum = 0.0f, lnsum = 0.0f;\n";
     if (adj) o << "  const float dl = *a.dloss;\n";
+    auto load = [&](const Field &f, const std::string &row) {
+      return "{ const int r = " + row + "; float x = 0.0f; if (r >= 0 && r < a.n0 && cin) x = a.u[(i64)" +
+             std::to_string(f.var) + " * a.N + (i64)r * a.n1 + c]; ";
+    };
     for (const Field &f : F) {
       if (!adj && (f.kind == 2 || f.kind == 3)) continue;
       o << "  float " << f.name << "[" << f.W << "];\n"
         << "  for (int k = 0; k < " << f.W << "; ++k) " << f.name << "[k] = 0.0f;\n";
+      if (f.kind == 0) {
+        // rows of the first unrolled block
+        o << "  float pf_" << f.name << "[" << UNR << "];\n";
+        for (int j = 0; j < UNR; ++j)
+          o << "  " << load(f, "t0 + " + std::to_string(j)) << "pf_" << f.name << "[" << j << "] = x; }\n";
+      }
     }
     o << "  for (int tb = t0; tb <= t1; tb += " << UNR << ") {\n";
     for (int j = 0; j < UNR; ++j) {
-      o << "  if (tb + " << j << " <= t1) {\n"
+      o << "  {\n"
         << "    const int t = tb + " << j << ";\n";
       // loads, then forward levels in creation order
       for (size_t fid = 0; fid < F.size(); ++fid) {
         const Field &f = F[fid];
         if (f.kind == 0) {
-          o << "    { float x = 0.0f; if (t >= 0 && t < a.n0 && cin) x = a.u[(i64)" << f.var
-            << " * a.N + (i64)t * a.n1 + c]; " << ring((int)fid, j, 0) << " = x; }\n";
+          // take row t from the prefetch registers, then fetch row t + UNR
+          o << "    " << ring((int)fid, j, 0) << " = pf_" << f.name << "[" << j << "];\n"
+            << "    " << load(f, "t + " + std::to_string(UNR)) << "pf_" << f.name << "[" << j << "] = x; }\n";
         } else if (f.kind == 1) {
           // central stencil of src for the row (t - lag): b = sum ab[i] src[r + i - (dp-1)],
           // f = sum af[i] src[r + i], (b + f) / (2h)
