@@ -24,7 +24,11 @@ instance.  Reported next to it, measured in the same run:
 
 Other workloads (--workload): burgers, kdv, legendre ('NN', one instance),
 wave (the 128 x 128 'mat' problem) and wave16k (the wave operator on a
-16384 x 16384 field, the HBM-roofline case of the stencil kernels).
+16384 x 16384 field, the HBM-roofline case of the stencil kernels).  The
+'mat' step reads one grid buffer and writes the other (as Model.train does),
+so the kernel generated for the grid interior applies Adam as it forms each
+gradient; its roofline is that whole step at 24 B per point, the residual +
+adjoint pass alone is reported beside it at 8 B per point.
 
 --calibrate runs two microbenchmarks instead: a stream copy of 1 GiB
 (out = -in, float32: 1 GiB read + 1 GiB written per launch; its kernel is
@@ -373,24 +377,42 @@ def bench_mat(args):
     loss = torch.zeros(2, device=dev)
     lr = 1e-3
 
-    def one():
-        eng.train_step(u, m, v, step, lr, 0.9, 0.999, 1e-8, loss)
+    # the training step reads one grid buffer and writes the other (as
+    # Model.train does): with the grid read-only the generated interior
+    # kernel applies Adam as it forms each gradient
+    bufs = [u, torch.empty_like(u)]
+    done = [0]
+
+    def one(k=None):
+        k = done[0] if k is None else k
+        eng.train_step_to(bufs[k % 2], bufs[(k + 1) % 2], m, v, step, lr, 0.9, 0.999, 1e-8,
+                          loss)
+        done[0] += 1
+    one()
     one()
     torch.cuda.synchronize()
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
-        one()
+    graphs = [torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()]
+    for par, g in enumerate(graphs):
+        with torch.cuda.graph(g):
+            eng.train_step_to(bufs[par], bufs[1 - par], m, v, step, lr, 0.9, 0.999, 1e-8, loss)
+
+    def replay():
+        graphs[done[0] % 2].replay()
+        done[0] += 1
     for _ in range(args.warmup):
-        g.replay()
-    ms = time_cuda(g.replay, args.steps)
+        replay()
+    ms = time_cuda(replay, args.steps)
     N = n * n
-    # roofline of the residual+adjoint pass (every launch of tdx_mat_backward:
-    # the generated interior kernel, the general kernel's edge tiles and the
-    # boundary kernels): 8 B/point (read u, write grad)
+    # roofline of the fused step (every launch of tdx_mat_train_step_to: the
+    # generated interior kernel with its Adam update, the general kernel's
+    # edge tiles, the boundary kernels and the edge points' Adam update):
+    # 24 B/point (read u, m, v; write the new u, m, v)
+    step_ms = time_cuda(one, args.steps)
+    achieved = 24.0 * N / (step_ms * 1e-3)
+    # the residual + adjoint pass alone (tdx_mat_backward): 8 B/point
     dl = torch.ones(1, device=dev)
     grad = torch.empty_like(u)
-    bwd_ms = time_cuda(lambda: eng.backward(u, dl, grad), args.steps)
-    achieved = 8.0 * N / (bwd_ms * 1e-3)
+    bwd_ms = time_cuda(lambda: eng.backward(bufs[done[0] % 2], dl, grad), args.steps)
     out = {
         "metric": "train steps/s & collocation pts/s, fused+adjoint vs plain GPU, at equal rel-L2",
         "value": 1e3 / ms, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
@@ -402,17 +424,21 @@ def bench_mat(args):
                    "specialized_points": spec_pts},
         "colloc_pts_per_s": N * 1e3 / ms,
         "roofline": {"bound": "hbm",
-                     "kernel": ("residual + loss + adjoint pass: tdx_mat_jit_adj (generated for "
-                                "the program, interior) + k_mat_tile (edge tiles) + boundary kernels")
-                     if spec_pts else "k_mat_tile (residual + loss + adjoint)",
+                     "kernel": ("fused train step: tdx_mat_jit_step (generated for the program: "
+                                "residual + loss + adjoint + Adam on the interior) + k_mat_tile "
+                                "(edge tiles) + boundary kernels + k_mat_frame_adam (edge points)")
+                     if spec_pts else "fused train step: k_mat_tile + k_mat_frame_adam",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
-                     "traffic": profiled_traffic(f"mat_adjoint_{args.workload}") if spec_pts
+                     "traffic": profiled_traffic(f"mat_step_{args.workload}") if spec_pts
                      else None,
-                     "traffic_source": f"profiles/mat_adjoint_{args.workload}_traffic.txt "
-                                       "(rocprofv3 FETCH_SIZE/WRITE_SIZE summed over the pass's "
+                     "traffic_source": f"profiles/mat_step_{args.workload}_traffic.txt "
+                                       "(rocprofv3 FETCH_SIZE/WRITE_SIZE summed over the step's "
                                        "kernels, copy-calibrated)",
-                     "algorithmic_bytes_per_launch": 8.0 * N, "avg_launch_ms": bwd_ms},
+                     "algorithmic_bytes_per_launch": 24.0 * N, "avg_launch_ms": step_ms},
+        "adjoint_pass": {"kernel": "tdx_mat_backward (residual + loss + adjoint, no update)",
+                         "ms": bwd_ms, "GB_per_s": 8.0 * N / (bwd_ms * 1e-3) / 1e9,
+                         "frac": 8.0 * N / (bwd_ms * 1e-3) / HBM_PEAK},
     }
     if not args.no_plain and n <= 4096:
         pl = PlainMat(prob, grid.to(dev))

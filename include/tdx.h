@@ -250,6 +250,18 @@ int tdx_mat_train_step(const tdx_mat_plan *plan, void *u, void *exp_avg,
                        void *exp_avg_sq, int64_t *step, double lr,
                        double beta1, double beta2, double eps, void *loss,
                        void *workspace, tdx_stream_t stream);
+/* The same step with the updated grid written to u_out (u_in is left
+ * unchanged; exp_avg, exp_avg_sq and step are updated in place): results
+ * identical to copying u_in to u_out and calling tdx_mat_train_step on it.
+ * With u_in read-only, the kernel generated for the grid interior applies
+ * the Adam update as it forms each gradient (24 B per point instead of 8 for
+ * the gradient plus 28 for the update); callers alternate two grid buffers.
+ * Replaces the same optimizer.step(closure) as tdx_mat_train_step. */
+int tdx_mat_train_step_to(const tdx_mat_plan *plan, const void *u_in,
+                          void *u_out, void *exp_avg, void *exp_avg_sq,
+                          int64_t *step, double lr, double beta1, double beta2,
+                          double eps, void *loss, void *workspace,
+                          tdx_stream_t stream);
 
 /* ===================================================================== */
 /* Optimizer                                                              */

@@ -127,3 +127,82 @@ def test_specialized_by_default_on_large_grids():
     sol = Solution(grid, pre, u0.to(DEV).requires_grad_(), 'mat', None, 1, 10, derivative_points=2)
     assert sol.engine.specialized_points >= (1 << 20)
     _compare(sol.problem, u0.reshape(-1), exact_div=1, specialize=0, steps=2)
+
+
+# ---------------------------------------------------------------------------
+# fused step into a second grid buffer (Adam inside the generated kernel)
+# ---------------------------------------------------------------------------
+
+def _compare_step_to(prob, u0, specialize, steps=3, dtype=torch.float32):
+    """tdx_mat_train_step_to alternating two buffers against the in-place
+    tdx_mat_train_step: identical grids, moments and step counts."""
+    eng = MatEngine(prob, DEV, exact_div=1, specialize=specialize)
+    hp = (1e-3, 0.9, 0.999, 1e-8)
+    u0 = u0.to(DEV, dtype).contiguous()
+    w = u0.clone()
+    m, v = torch.zeros_like(w), torch.zeros_like(w)
+    step = torch.zeros(1, dtype=torch.int64, device=DEV)
+    loss = torch.zeros(2, dtype=dtype, device=DEV)
+    losses = []
+    for _ in range(steps):
+        eng.train_step(w, m, v, step, *hp, loss)
+        losses.append(loss.cpu().numpy().copy())
+    bufs = [u0.clone(), torch.empty_like(u0)]
+    m2, v2 = torch.zeros_like(w), torch.zeros_like(w)
+    step2 = torch.zeros(1, dtype=torch.int64, device=DEV)
+    loss2 = torch.zeros(2, dtype=dtype, device=DEV)
+    for k in range(steps):
+        eng.train_step_to(bufs[k % 2], bufs[(k + 1) % 2], m2, v2, step2, *hp, loss2)
+        if k == 0:
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(bufs[0].cpu().numpy(), u0.cpu().numpy())
+        np.testing.assert_allclose(loss2.cpu().numpy(), losses[k], rtol=1e-6)
+    torch.cuda.synchronize()
+    out = bufs[steps % 2]
+    _print(f"  specialized points {eng.specialized_points}; u max diff "
+           f"{float((out - w).abs().max()):.3e}, m {float((m2 - m).abs().max()):.3e}")
+    np.testing.assert_array_equal(out.cpu().numpy(), w.cpu().numpy())
+    np.testing.assert_array_equal(m2.cpu().numpy(), m.cpu().numpy())
+    np.testing.assert_array_equal(v2.cpu().numpy(), v.cpu().numpy())
+    assert int(step2.item()) == int(step.item()) == steps
+    return eng
+
+
+@pytest.mark.parametrize('name', list(CASES))
+def test_fused_step_to_equals_in_place_step(name):
+    _print(f"[mat step_to] {name} specialized")
+    prob, u0 = _problem(name)
+    eng = _compare_step_to(prob, u0.reshape(-1), specialize=1)
+    assert eng.specialized_points > 0
+
+
+def test_fused_step_to_general_kernel_only():
+    _print("[mat step_to] wave_dp2 without the generated kernel")
+    prob, u0 = _problem('wave_dp2')
+    eng = _compare_step_to(prob, u0.reshape(-1), specialize=-1)
+    assert eng.specialized_points == 0
+
+
+def test_fused_step_to_double():
+    _print("[mat step_to] nonlinear float64")
+    d, eq, b, kw = problems.nonlinear_mat(api, n=40, dtype='float64')
+    grid = d.build('mat')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict)).set_strategy('mat')
+    u0 = problems.mat_init([1] + list(grid.shape[1:]), seed=8, dtype=torch.float64, std=0.05)
+    sol = Solution(grid, pre, u0.to(DEV).requires_grad_(), 'mat', None, 1, 10)
+    _compare_step_to(sol.problem, u0.reshape(-1), specialize=0, dtype=torch.float64)
+
+
+def test_fused_step_to_boundary_inside_interior():
+    """A Dirichlet line through the interior (t = 0.5): its adjoint adds to
+    points of the generated kernel's rectangle, so the step keeps the
+    gradient pass and updates every point afterwards."""
+    _print("[mat step_to] wave with an interior Dirichlet line")
+    d, eq, b, dp, n_eq = problems.spec_wave(api, 300, 262, 2)
+    b.dirichlet({'x': [0, 1], 't': 0.5}, value=0)
+    grid = d.build('mat')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict)).set_strategy('mat')
+    u0 = problems.mat_init([1] + list(grid.shape[1:]), seed=9, std=0.05)
+    sol = Solution(grid, pre, u0.to(DEV).requires_grad_(), 'mat', None, 1, 10, derivative_points=2)
+    assert sol.problem.bcs and any(len(bc.positions[0]) > 0 for bc in sol.problem.bcs)
+    _compare_step_to(sol.problem, u0.reshape(-1), specialize=1)

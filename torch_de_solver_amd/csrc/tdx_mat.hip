@@ -85,6 +85,12 @@ struct MatArgs {
   T *bc_seeds;
   const T *dloss;
   int n_tile_wg;
+  // fused train step (u_in -> u_out): moments and the step's Adam scalars
+  int fused;
+  T *u_out;
+  T *m;
+  T *v;
+  const T *adam;
 };
 
 // x / h by division mode DIV: 0 multiplies by the rounded reciprocal inv
@@ -416,7 +422,7 @@ __device__ __forceinline__ void mat_tile_body(const MatArgs<T> &a, T *smem, int 
       }
       if (ing && in_tile) {
         lsum += lam_n * (r * r);
-        lnsum += (r * r) / T(N);
+        lnsum += r * r;
         if (a.op) a.op[eq * N + gp] = r;
       }
       if (ADJ) {
@@ -521,9 +527,10 @@ __device__ __forceinline__ void mat_tile_body(const MatArgs<T> &a, T *smem, int 
       });
   }
 
-  // deterministic workgroup sum of the loss partials
+  // deterministic workgroup sum of the loss partials (the normalized sum
+  // divided by N once per thread)
   red[tid] = lsum;
-  red[MAT_THREADS + tid] = lnsum;
+  red[MAT_THREADS + tid] = lnsum / T(N);
   __syncthreads();
   for (int st = MAT_THREADS / 2; st > 0; st >>= 1) {
     if (tid < st) {
@@ -726,6 +733,65 @@ __global__ void k_mat_loss(const T *lp, int n, T *loss) {
   }
 }
 
+// The step's Adam scalars (t = *step + 1), computed once for every kernel of
+// the fused step: {1-b1, 1-b2, b2, lr/(1-b1^t), sqrt(1-b2^t), eps}.
+template <typename T>
+__global__ void k_mat_adam_prep(const int64_t *step, AdamHyper h, T *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const AdamScalars<T> sc = adam_scalars<T>(*step + 1, h);
+    out[0] = sc.w1;
+    out[1] = sc.w2;
+    out[2] = sc.b2;
+    out[3] = sc.step_size;
+    out[4] = sc.bc2s;
+    out[5] = sc.eps;
+  }
+}
+
+// Adam update u_out = step(u, grad, m, v) of every grid point outside the
+// rectangle [r_lo, r_hi) x [c_lo, c_hi) of each plane (the points the
+// generated kernel does not update; an empty rectangle covers the grid).
+template <typename T>
+__global__ __launch_bounds__(256) void k_mat_frame_adam(MatArgs<T> a, int r_lo, int r_hi,
+                                                        int c_lo, int c_hi) {
+  const int n0 = a.n0, n1 = a.n1;
+  const int64_t top = (int64_t)r_lo * n1, bot = (int64_t)(n0 - r_hi) * n1;
+  const int wl = c_lo, wm = c_lo + (n1 - c_hi);
+  const int64_t per = top + bot + (int64_t)(r_hi - r_lo) * wm;
+  const int64_t tot = per * a.n_eq;
+  AdamScalars<T> sc;
+  sc.w1 = a.adam[0];
+  sc.w2 = a.adam[1];
+  sc.b2 = a.adam[2];
+  sc.step_size = a.adam[3];
+  sc.bc2s = a.adam[4];
+  sc.eps = a.adam[5];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot; idx += stride) {
+    const int64_t e = idx / per, k = idx - e * per;
+    int64_t row, col;
+    if (k < top) {
+      row = k / n1;
+      col = k - row * n1;
+    } else if (k < top + bot) {
+      const int64_t k2 = k - top;
+      row = r_hi + k2 / n1;
+      col = k2 - (k2 / n1) * n1;
+    } else {
+      const int64_t k3 = k - top - bot;
+      const int64_t j = k3 % wm;
+      row = r_lo + k3 / wm;
+      col = j < wl ? j : c_hi + (j - wl);
+    }
+    const int64_t gi = e * a.N + row * n1 + col;
+    T p = a.u[gi], m = a.m[gi], v = a.v[gi];
+    adam_apply(p, a.grad[gi], m, v, sc);
+    a.u_out[gi] = p;
+    a.m[gi] = m;
+    a.v[gi] = v;
+  }
+}
+
 template <typename T>
 __global__ void k_mat_one(T *x) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *x = T(1);
@@ -749,9 +815,11 @@ struct tdx_mat_plan {
   int jit_on;
   JitKernel jit;
   int jr_lo, jr_hi, jc_lo, jc_hi, j_hb, j_nstrips, j_nbands, j_waves;
+  int jit_fuse;     // no boundary adjoint touches the rectangle: the fused
+                    // step updates it inside the generated kernel
   DevTable gen_list;
   int n_gen_wg;
-  size_t off_lp, off_mid, off_seeds, off_one, off_grad, ws_bytes;
+  size_t off_lp, off_mid, off_seeds, off_one, off_grad, off_adam, ws_bytes;
   double ab[MMAX_DP], af[MMAX_DP];
 };
 
@@ -858,6 +926,11 @@ MatArgs<T> mat_args(const tdx_mat_plan *p, const void *u, void *ws) {
   a.bc_seeds = (T *)(w + p->off_seeds);
   a.dloss = (const T *)(w + p->off_one);
   a.n_tile_wg = p->n_gen_wg;
+  a.fused = 0;
+  a.u_out = nullptr;
+  a.m = nullptr;
+  a.v = nullptr;
+  a.adam = nullptr;
   return a;
 }
 
@@ -906,10 +979,15 @@ int mat_jit_launch(const tdx_mat_plan *p, const MatArgs<T> &a, bool adj, hipStre
   j.nbands = p->j_nbands;
   j.lp_base = p->n_gen_wg + p->n_bc_wg;
   j.adj = adj ? 1 : 0;
+  const bool step = adj && a.fused && p->jit_fuse;
+  j.u_out = step ? (float *)a.u_out : nullptr;
+  j.m = step ? (float *)a.m : nullptr;
+  j.v = step ? (float *)a.v : nullptr;
+  j.adam = step ? (const float *)a.adam : nullptr;
   void *params[] = {&j};
   const unsigned blocks = (unsigned)((p->j_waves + 3) / 4);
-  TDX_TRY_HIP(hipModuleLaunchKernel(adj ? p->jit.fn_adj : p->jit.fn_fwd, blocks, 1, 1, 256, 1, 1,
-                                    0, s, params, nullptr));
+  hipFunction_t fn = step ? p->jit.fn_step : adj ? p->jit.fn_adj : p->jit.fn_fwd;
+  TDX_TRY_HIP(hipModuleLaunchKernel(fn, blocks, 1, 1, 256, 1, 1, 0, s, params, nullptr));
   return TDX_OK;
 }
 
@@ -930,6 +1008,21 @@ int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t 
     if (adj && p->d.n_touch > 0) {
       hipLaunchKernelGGL((k_mat_bc_adj<T>), dim3((p->d.n_touch + 255) / 256), dim3(256), 0, s,
                          a);
+      TDX_TRY_HIP(hipGetLastError());
+    }
+  }
+  if (adj && a.fused) {
+    // Adam on the points the generated kernel did not update
+    const bool in_jit = p->jit_on && p->jit_fuse;
+    const int r_lo = in_jit ? p->jr_lo : 0, r_hi = in_jit ? p->jr_hi : 0;
+    const int c_lo = in_jit ? p->jc_lo : 0, c_hi = in_jit ? p->jc_hi : 0;
+    const int64_t per = (int64_t)p->d.n0 * p->d.n1 - (int64_t)(r_hi - r_lo) * (c_hi - c_lo);
+    const int64_t tot = per * p->d.n_eq;
+    if (tot > 0) {
+      const unsigned blocks =
+          (unsigned)std::max<int64_t>(1, std::min<int64_t>((tot + 255) / 256, 16384));
+      hipLaunchKernelGGL((k_mat_frame_adam<T>), dim3(blocks), dim3(256), 0, s, a, r_lo, r_hi,
+                         c_lo, c_hi);
       TDX_TRY_HIP(hipGetLastError());
     }
   }
@@ -1118,6 +1211,14 @@ int mat_specialize(tdx_mat_plan *p, const tdx_mat_desc *d) {
     if (urc) return urc;
   }
   p->n_gen_wg = (int)list.size();
+  // the fused step finalizes the rectangle inside the generated kernel, so
+  // no boundary adjoint may add to a point of it afterwards
+  p->jit_fuse = 1;
+  for (int j = 0; j < d->n_touch; ++j) {
+    const int64_t pos = d->ints[d->bct_pos_i + j] % sp.N;
+    const int64_t r = pos / d->n1, c = pos - (pos / d->n1) * d->n1;
+    if (r >= p->jr_lo && r < p->jr_hi && c >= p->jc_lo && c < p->jc_hi) p->jit_fuse = 0;
+  }
   p->jit_on = 1;
   return TDX_OK;
 }
@@ -1324,6 +1425,7 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   p->n_tile_wg = p->tiles0 * p->tiles1;
   p->n_bc_wg = (d->n_bc + MAT_THREADS - 1) / MAT_THREADS;
   p->jit_on = 0;
+  p->jit_fuse = 0;
   p->n_gen_wg = p->n_tile_wg;
 
   // stencil coefficients and their transposed products, in T
@@ -1380,6 +1482,8 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   off = align_up(off + 2 * 256 * ts, 256);
   p->off_grad = off;
   off = align_up(off + (size_t)N * d->n_eq * ts, 256);
+  p->off_adam = off;
+  off = align_up(off + 8 * ts, 256);
   p->ws_bytes = off;
   *out = p;
   return TDX_OK;
@@ -1472,6 +1576,45 @@ extern "C" int tdx_mat_backward(const tdx_mat_plan *p, const void *u, const void
   a.grad = (double *)grad;
   a.dloss = (const double *)dloss;
   return mat_dispatch<double>(p, a, true, nullptr, s);
+}
+
+extern "C" int tdx_mat_train_step_to(const tdx_mat_plan *p, const void *u_in, void *u_out,
+                                     void *m, void *v, int64_t *step, double lr, double b1,
+                                     double b2, double eps, void *loss, void *ws,
+                                     tdx_stream_t stream) {
+  TDX_REQUIRE(p && u_in && u_out && m && v && step && ws, "null argument");
+  TDX_REQUIRE(u_in != u_out, "u_out must be a different buffer from u_in");
+  hipStream_t s = (hipStream_t)stream;
+  const AdamHyper h{lr, b1, b2, eps};
+  char *w = (char *)ws;
+  int rc;
+  if (p->d.dtype == 0) {
+    MatArgs<float> a = mat_args<float>(p, u_in, ws);
+    a.grad = (float *)(w + p->off_grad);
+    a.fused = 1;
+    a.u_out = (float *)u_out;
+    a.m = (float *)m;
+    a.v = (float *)v;
+    a.adam = (const float *)(w + p->off_adam);
+    hipLaunchKernelGGL((k_mat_one<float>), dim3(1), dim3(64), 0, s, (float *)(w + p->off_one));
+    hipLaunchKernelGGL((k_mat_adam_prep<float>), dim3(1), dim3(64), 0, s, (const int64_t *)step,
+                       h, (float *)(w + p->off_adam));
+    rc = mat_dispatch<float>(p, a, true, (float *)loss, s);
+  } else {
+    MatArgs<double> a = mat_args<double>(p, u_in, ws);
+    a.grad = (double *)(w + p->off_grad);
+    a.fused = 1;
+    a.u_out = (double *)u_out;
+    a.m = (double *)m;
+    a.v = (double *)v;
+    a.adam = (const double *)(w + p->off_adam);
+    hipLaunchKernelGGL((k_mat_one<double>), dim3(1), dim3(64), 0, s, (double *)(w + p->off_one));
+    hipLaunchKernelGGL((k_mat_adam_prep<double>), dim3(1), dim3(64), 0, s,
+                       (const int64_t *)step, h, (double *)(w + p->off_adam));
+    rc = mat_dispatch<double>(p, a, true, (double *)loss, s);
+  }
+  if (rc) return rc;
+  return tdx_step_increment(step, stream);
 }
 
 extern "C" int tdx_mat_train_step(const tdx_mat_plan *p, void *u, void *m, void *v,

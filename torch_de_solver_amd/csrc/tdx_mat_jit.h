@@ -57,12 +57,19 @@ struct JitArgs {
   int nstrips, nbands;
   int lp_base;                 // first loss-partial pair of this kernel
   int adj;
+  // fused Adam step: updated grid, moments, and the step's scalars
+  // {1-b1, 1-b2, b2, lr/(1-b1^t), sqrt(1-b2^t), eps}
+  float *u_out;
+  float *m;
+  float *v;
+  const float *adam;
 };
 
 struct JitKernel {
   hipModule_t module = nullptr;
   hipFunction_t fn_adj = nullptr;  // residual + loss + gradient
   hipFunction_t fn_fwd = nullptr;  // residual + loss
+  hipFunction_t fn_step = nullptr; // residual + loss + Adam update into u_out, m, v
 };
 
 // Generates the source for `spec` (returns "" when the program is outside

@@ -143,14 +143,42 @@ std::string mat_jit_source(const JitSpec &s, std::string *why) {
     F[sl.fwd].back = std::max(F[sl.fwd].back, LAGF - F[sl.fwd].lag);
     F[sl.out].back = std::max(F[sl.out].back, LAGG - F[sl.out].lag);
   }
-  // the march is unrolled by UNR rows (a power of two at least every ring
+  std::vector<int> load_of(s.n_eq, -1);
+  for (size_t k = 0; k < F.size(); ++k)
+    if (F[k].kind == 0) load_of[F[k].var] = (int)k;
+  // The march is unrolled by UNR rows (a power of two at least every ring
   // window, so ring slots are constants) and loads run one unrolled block
-  // ahead, so at least 8 rows are in flight per wavefront
+  // ahead, so at least 8 rows are in flight per wavefront.  The fused Adam
+  // step also keeps u in the load ring until its gradient row when that
+  // does not deepen the unrolling (else it reads u again).
+  std::vector<int> back0(F.size());
+  for (size_t k = 0; k < F.size(); ++k) back0[k] = F[k].back;
   int UNR = 8;
-  for (Field &f : F) {
-    f.W = pow2_at_least(f.back + 1);
-    UNR = std::max(UNR, f.W);
-  }
+  bool u_in_ring = false;
+  auto windows = [&](bool step) {
+    int unr = 8;
+    for (size_t k = 0; k < F.size(); ++k) {
+      F[k].back = back0[k];
+      F[k].W = pow2_at_least(F[k].back + 1);
+      unr = std::max(unr, F[k].W);
+    }
+    u_in_ring = false;
+    if (step) {
+      int unr2 = unr;
+      for (const Field &f : F)
+        if (f.kind == 0) unr2 = std::max(unr2, pow2_at_least(std::max(f.back, LAGG) + 1));
+      if (unr2 == unr) {
+        u_in_ring = true;
+        for (Field &f : F)
+          if (f.kind == 0) {
+            f.back = std::max(f.back, LAGG);
+            f.W = pow2_at_least(f.back + 1);
+          }
+      }
+    }
+    UNR = unr;
+  };
+  windows(false);
   if (UNR > 16) {
     *why = "stencil windows too deep";
     return "";
@@ -167,7 +195,8 @@ std::string mat_jit_source(const JitSpec &s, std::string *why) {
        "typedef long long i64;\n"
        "struct JitArgs { const float *u; float *grad; float *op; float *lp; const float *data;\n"
        "  const float *dloss; i64 N; int n0, n1; int r_lo, r_hi, c_lo, c_hi; int hb;\n"
-       "  int nstrips, nbands; int lp_base; int adj; };\n"
+       "  int nstrips, nbands; int lp_base; int adj;\n"
+       "  float *u_out; float *m; float *v; const float *adam; };\n"
        "__device__ __forceinline__ float shl1(float v) {  // lane l gets lane l+1\n"
        "  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false)); }\n"
        "__device__ __forceinline__ float shr1(float v) {  // lane l gets lane l-1\n"
@@ -219,9 +248,13 @@ std::string mat_jit_source(const JitSpec &s, std::string *why) {
     return "(" + fl((float)p) + " * " + powexpr(g, x) + ")";
   };
 
-  for (int adj = 1; adj >= 0; --adj) {
-    o << "extern \"C\" __global__ __launch_bounds__(256) void tdx_mat_jit_" << (adj ? "adj" : "fwd")
-      << "(JitArgs a) {\n"
+  // mode 2: residual + loss + adjoint + Adam update into u_out, m, v;
+  // 1: residual + loss + gradient; 0: residual + loss
+  for (int mode = 2; mode >= 0; --mode) {
+    const bool adj = mode >= 1, step = mode == 2;
+    windows(step);
+    o << "extern \"C\" __global__ __launch_bounds__(256) void tdx_mat_jit_"
+      << (step ? "step" : adj ? "adj" : "fwd") << "(JitArgs a) {\n"
          "  // blocks go to the 8 XCDs round-robin: block b takes logical block\n"
          "  // (b % 8) * per + b / 8, so each XCD walks one contiguous range of\n"
          "  // strips and the halo columns neighbouring strips share stay in its L2\n"
@@ -242,8 +275,29 @@ std::string mat_jit_source(const JitSpec &s, std::string *why) {
          "  // steps' rows lie outside the band and are not stored)\n"
          "  const int tl = b_hi - 1 + " << (adj ? LAGG : LAGF) << ";\n"
          "  const int t1 = t0 + ((tl - t0) / " << UNR << " + 1) * " << UNR << " - 1;\n"
-         "  float lsum = 0.0f, lnsum = 0.0f;\n";
+         "  // sum of r^2 per equation; lambda and 1/N are applied once at the end\n"
+         "  float esum[" << s.n_eq << "];\n"
+         "  for (int k = 0; k < " << s.n_eq << "; ++k) esum[k] = 0.0f;\n";
     if (adj) o << "  const float dl = *a.dloss;\n";
+    if (step)
+      o << "  // torch.optim.Adam scalars of this step: 1-b1, 1-b2, b2, lr/(1-b1^t),\n"
+           "  // sqrt(1-b2^t), eps\n"
+           "  const float w1 = a.adam[0], w2 = a.adam[1], b2 = a.adam[2], ss = a.adam[3],\n"
+           "              bc2s = a.adam[4], eps = a.adam[5];\n";
+    // m and v of the gradient rows, fetched one unrolled block ahead
+    auto mvload = [&](int v, const std::string &row, int slot) {
+      const std::string sl = std::to_string(slot), vs = std::to_string(v);
+      return "  { const int r = " + row + "; float xm = 0.0f, xv = 0.0f;\n"
+             "    if (own && r >= b_lo && r < b_hi) { const i64 gi = (i64)" + vs +
+             " * a.N + (i64)r * a.n1 + c; xm = a.m[gi]; xv = a.v[gi]; }\n"
+             "    pm" + vs + "[" + sl + "] = xm; pv" + vs + "[" + sl + "] = xv; }\n";
+    };
+    if (step)
+      for (int v = 0; v < s.n_eq; ++v) {
+        o << "  float pm" << v << "[" << UNR << "], pv" << v << "[" << UNR << "];\n";
+        for (int j = 0; j < UNR; ++j)
+          o << mvload(v, "t0 - " + std::to_string(LAGG) + " + " + std::to_string(j), j);
+      }
     auto load = [&](const Field &f, const std::string &row) {
       return "{ const int r = " + row + "; float x = 0.0f; if (r >= 0 && r < a.n0 && cin) x = a.u[(i64)" +
              std::to_string(f.var) + " * a.N + (i64)r * a.n1 + c]; ";
@@ -328,8 +382,7 @@ std::string mat_jit_source(const JitSpec &s, std::string *why) {
           o << "        { const float tv = " << cn << " * der" << ti << "; r = " << (ti == 0 ? "tv" : "r + tv") << "; }\n";
         }
         o << "        if (rin) {\n"
-          << "          lsum += " << fl((float)eq.lam) << " * (r * r);\n"
-          << "          lnsum += (r * r) / " << fl((float)s.N) << ";\n"
+          << "          esum[" << e << "] += r * r;\n"
           << "          if (a.op) a.op[(i64)" << e << " * a.N + gp] = r;\n"
           << "        }\n";
         if (adj) {
@@ -385,7 +438,7 @@ std::string mat_jit_source(const JitSpec &s, std::string *why) {
         // gradient for row t - LAGG: per variable, slots in order
         o << "    {\n"
           << "      const int gr = t - " << LAGG << ";\n"
-          << "      if (a.adj && own && gr >= b_lo && gr < b_hi) {\n";
+          << "      if (" << (step ? "" : "a.adj && ") << "own && gr >= b_lo && gr < b_hi) {\n";
         for (int v = 0; v < s.n_eq; ++v) {
           // the general kernel adds the slots into a zeroed accumulator in
           // slot order: ((0 + s0) + s1) + ...; a variable no factor reads
@@ -400,13 +453,37 @@ std::string mat_jit_source(const JitSpec &s, std::string *why) {
               first = false;
             }
           }
-          o << "        a.grad[(i64)" << v << " * a.N + (i64)gr * a.n1 + c] = " << acc << ";\n";
+          if (!step) {
+            o << "        a.grad[(i64)" << v << " * a.N + (i64)gr * a.n1 + c] = " << acc << ";\n";
+            continue;
+          }
+          // Adam, in the operation order of the library's update kernel
+          const std::string vs = std::to_string(v), js = std::to_string(j);
+          o << "        {\n"
+            << "          const i64 gi = (i64)" << vs << " * a.N + (i64)gr * a.n1 + c;\n"
+            << "          const float g = " << acc << ";\n"
+            << "          float p = " << (u_in_ring && load_of[v] >= 0 ? ring(load_of[v], j, LAGG) : "a.u[gi]") << ";\n"
+            << "          float mm = pm" << vs << "[" << js << "], vv = pv" << vs << "[" << js << "];\n"
+            << "          mm = mm + w1 * (g - mm);\n"
+            << "          vv = vv * b2 + w2 * g * g;\n"
+            << "          const float den = sqrtf(vv) / bc2s + eps;\n"
+            << "          p = p - ss * (mm / den);\n"
+            << "          a.u_out[gi] = p; a.m[gi] = mm; a.v[gi] = vv;\n"
+            << "        }\n";
         }
         o << "      }\n    }\n";
+        if (step)
+          for (int v = 0; v < s.n_eq; ++v)
+            o << mvload(v, "t + " + std::to_string(UNR) + " - " + std::to_string(LAGG), j);
       }
       o << "  }\n";
     }
     o << "  }\n"
+         "  float lsum = 0.0f, lnsum = 0.0f;\n";
+    for (size_t e = 0; e < s.eqs.size(); ++e)
+      o << "  lsum += " << fl((float)s.eqs[e].lam) << " * esum[" << e << "];\n"
+        << "  lnsum += esum[" << e << "];\n";
+    o << "  lnsum = lnsum / " << fl((float)s.N) << ";\n"
          "  for (int m = 32; m >= 1; m >>= 1) {\n"
          "    lsum += __shfl_xor(lsum, m);\n"
          "    lnsum += __shfl_xor(lnsum, m);\n"
@@ -470,7 +547,8 @@ int mat_jit_compile(const std::string &src, JitKernel *k) {
     return TDX_ELAUNCH;
   }
   if (hipModuleGetFunction(&out.fn_adj, out.module, "tdx_mat_jit_adj") != hipSuccess ||
-      hipModuleGetFunction(&out.fn_fwd, out.module, "tdx_mat_jit_fwd") != hipSuccess) {
+      hipModuleGetFunction(&out.fn_fwd, out.module, "tdx_mat_jit_fwd") != hipSuccess ||
+      hipModuleGetFunction(&out.fn_step, out.module, "tdx_mat_jit_step") != hipSuccess) {
     set_error("specialized 'mat' kernel: entry point missing");
     return TDX_ELAUNCH;
   }

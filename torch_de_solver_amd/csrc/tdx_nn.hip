@@ -1288,38 +1288,6 @@ __global__ __launch_bounds__(256) void k_loss(NNArgs<T> a, T *loss) {
   }
 }
 
-struct AdamHyper {
-  double lr, b1, b2, eps;
-};
-
-// torch.optim.Adam (single-tensor form): m = b1 m + (1-b1) g;
-// v = b2 v + (1-b2) g^2; p -= lr/(1-b1^t) * m / (sqrt(v)/sqrt(1-b2^t) + eps)
-// The per-step scalars (bias corrections in double, as torch computes them
-// in Python floats) depend only on the step t.
-template <typename T>
-struct AdamScalars {
-  T w1, w2, b2, step_size, bc2s, eps;
-};
-template <typename T>
-__device__ __forceinline__ AdamScalars<T> adam_scalars(int64_t t, const AdamHyper &h) {
-  AdamScalars<T> s;
-  s.b2 = T(h.b2);
-  s.w1 = T(1.0 - h.b1);
-  s.w2 = T(1.0 - h.b2);
-  const double bc1 = 1.0 - pow(h.b1, (double)t);
-  const double bc2 = 1.0 - pow(h.b2, (double)t);
-  s.step_size = T(h.lr / bc1);
-  s.bc2s = T(sqrt(bc2));
-  s.eps = T(h.eps);
-  return s;
-}
-template <typename T>
-__device__ __forceinline__ void adam_apply(T &p, T g, T &m, T &v, const AdamScalars<T> &s) {
-  m = m + s.w1 * (g - m);
-  v = v * s.b2 + s.w2 * g * g;
-  const T denom = sqrt(v) / s.bc2s + s.eps;
-  p = p - s.step_size * (m / denom);
-}
 template <typename T>
 __device__ __forceinline__ void adam_update(T &p, T g, T &m, T &v, int64_t t,
                                             const AdamHyper &h) {

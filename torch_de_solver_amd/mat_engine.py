@@ -281,6 +281,17 @@ class MatEngine:
             native.ptr(step), float(lr), float(beta1), float(beta2), float(eps),
             native.ptr(loss), native.ptr(self.workspace), native.stream_ptr()))
 
+    def train_step_to(self, u_in, u_out, exp_avg, exp_avg_sq, step, lr, beta1, beta2, eps,
+                      loss=None):
+        """One fused Adam step from u_in into u_out (u_in unchanged): the
+        result of copying u_in to u_out and calling train_step on it."""
+        self._check(u_in)
+        self._check(u_out)
+        native.check(native.lib().tdx_mat_train_step_to(
+            self._plan, native.ptr(u_in), native.ptr(u_out), native.ptr(exp_avg),
+            native.ptr(exp_avg_sq), native.ptr(step), float(lr), float(beta1), float(beta2),
+            float(eps), native.ptr(loss), native.ptr(self.workspace), native.stream_ptr()))
+
     def loss(self, u: torch.Tensor) -> torch.Tensor:
         return MatResidualLoss.apply(u, self)
 

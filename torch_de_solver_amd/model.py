@@ -120,29 +120,50 @@ class Model:
         n_steps = max(epochs - 1, 0)
         self.loss_history_device = torch.zeros(0)
 
-        def one_step():
-            eng.train_step(params, m, v, step, lr, b1, b2, eps, loss_buf)
+        # 'mat': the step reads one grid buffer and writes the other (the
+        # kernel generated for the interior then applies Adam as it forms
+        # each gradient); the buffers alternate and the result is copied
+        # back into the model's tensor at the end
+        pingpong = self.mode == 'mat' and hasattr(eng, 'train_step_to')
+        bufs = [params, torch.empty_like(params)] if pingpong else [params]
+        done = [0]
+
+        def one_step(k=None):
+            k = done[0] if k is None else k
+            if pingpong:
+                eng.train_step_to(bufs[k % 2], bufs[(k + 1) % 2], m, v, step, lr, b1, b2, eps,
+                                  loss_buf)
+            else:
+                eng.train_step(params, m, v, step, lr, b1, b2, eps, loss_buf)
 
         remaining = n_steps
         if remaining > 0:
             one_step()  # first call sets kernel attributes outside any capture
+            done[0] += 1
             remaining -= 1
         graph = None
         if self.use_graph and remaining > 4:
-            graph = torch.cuda.CUDAGraph()
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
                 one_step()
+                done[0] += 1
             torch.cuda.current_stream().wait_stream(side)
             remaining -= 1
-            with torch.cuda.graph(graph):
-                one_step()  # recorded, not run
+            graphs = [torch.cuda.CUDAGraph() for _ in bufs]
+            for par, g in enumerate(graphs):
+                with torch.cuda.graph(g):
+                    one_step(par)  # recorded, not run
             for _ in range(remaining):
-                graph.replay()
+                graphs[done[0] % len(graphs)].replay()
+                done[0] += 1
+            graph = graphs[0]
         else:
             for _ in range(remaining):
                 one_step()
+                done[0] += 1
+        if pingpong and done[0] % 2 == 1:
+            params.copy_(bufs[1])
         self.t = epochs
         self.steps_done = int(step.item())
         self.cur_loss = loss_buf[0:1].clone()

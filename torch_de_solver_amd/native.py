@@ -27,7 +27,7 @@ EXPORTS = [
     'tdx_nn_workspace_bytes', 'tdx_nn_forward', 'tdx_nn_backward',
     'tdx_nn_train_step', 'tdx_nn_profile', 'tdx_nn_profile_read',
     'tdx_mat_plan_create', 'tdx_mat_plan_destroy', 'tdx_mat_workspace_bytes',
-    'tdx_mat_forward', 'tdx_mat_backward', 'tdx_mat_train_step',
+    'tdx_mat_forward', 'tdx_mat_backward', 'tdx_mat_train_step', 'tdx_mat_train_step_to',
     'tdx_adam', 'tdx_step_increment', 'tdx_quotient_check',
     'tdx_mat_specialized_points', 'tdx_mat_specialized_source',
 ]
@@ -112,6 +112,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.tdx_mat_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     lib.tdx_mat_backward.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.tdx_mat_train_step.argtypes = [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]
+    lib.tdx_mat_train_step_to.argtypes = [vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]
     lib.tdx_adam.argtypes = [ctypes.c_int, i64, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp]
     lib.tdx_step_increment.argtypes = [vp, vp]
 
