@@ -368,7 +368,7 @@ def bench_mat(args):
         from torch_de_solver_amd.autotune import tune_mat
         tile_rows, tuning = tune_mat(prob, dev, u, exact_div=not args.fast_div)
     eng = MatEngine(prob, dev, exact_div=not args.fast_div, tile_rows=tile_rows,
-                    specialize=args.specialize)
+                    specialize=args.specialize, band_rows=args.band_rows)
     spec_pts = eng.specialized_points
     log(f"[bench] compiled {n}x{n} in {time.time() - t0:.1f}s (tuning: {tuning}; "
         f"specialized interior points: {spec_pts})")
@@ -421,7 +421,7 @@ def bench_mat(args):
         "config": {"workload": f"wave u_tt=4u_xx 'mat' {n}x{n}, Adam lr 1e-3",
                    "collocation_points": N, "parallelism": "single GPU",
                    "tile_rows": tile_rows, "autotune": tuning,
-                   "specialized_points": spec_pts},
+                   "specialized_points": spec_pts, "band_rows": args.band_rows},
         "colloc_pts_per_s": N * 1e3 / ms,
         "roofline": {"bound": "hbm",
                      "kernel": ("fused train step: tdx_mat_jit_step (generated for the program: "
@@ -513,6 +513,8 @@ def main():
     ap.add_argument('--no-tune', action='store_true',
                     help="default tile shapes instead of the autotuner's choice")
     ap.add_argument('--tile-rows', type=int, default=0, help="'mat': output rows per tile")
+    ap.add_argument('--band-rows', type=int, default=0,
+                    help="'mat': rows per wavefront of the generated kernel (0: plan's choice)")
     ap.add_argument('--specialize', type=int, default=0,
                     help="'mat': kernel generated for the program on the interior (0 auto, "
                          "1 always, -1 never)")

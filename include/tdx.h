@@ -214,6 +214,8 @@ typedef struct tdx_mat_desc {
                      0: when the interior holds at least 2^20 points;
                      1: whenever the grid has an interior (errors are
                      reported); -1: never */
+  int band_rows;  /* rows per wavefront of the specialized kernel; 0: chosen
+                     by the plan */
 } tdx_mat_desc;
 
 typedef struct tdx_mat_plan tdx_mat_plan;

@@ -814,7 +814,10 @@ struct tdx_mat_plan {
   // takes the n_gen_wg tiles of gen_list (every tile when jit_on is 0).
   int jit_on;
   JitKernel jit;
-  int jr_lo, jr_hi, jc_lo, jc_hi, j_hb, j_nstrips, j_nbands, j_waves;
+  int jr_lo, jr_hi, jc_lo, jc_hi, j_nstrips;
+  // per kernel variant (0 forward, 1 adjoint, 2 fused step): rows per band,
+  // bands, wavefronts
+  int j_hb[3], j_nbands[3], j_waves[3], j_waves_max;
   int jit_fuse;     // no boundary adjoint touches the rectangle: the fused
                     // step updates it inside the generated kernel
   DevTable gen_list;
@@ -974,18 +977,19 @@ int mat_jit_launch(const tdx_mat_plan *p, const MatArgs<T> &a, bool adj, hipStre
   j.r_hi = p->jr_hi;
   j.c_lo = p->jc_lo;
   j.c_hi = p->jc_hi;
-  j.hb = p->j_hb;
+  const bool step = adj && a.fused && p->jit_fuse;
+  const int var = step ? 2 : adj ? 1 : 0;
+  j.hb = p->j_hb[var];
   j.nstrips = p->j_nstrips;
-  j.nbands = p->j_nbands;
+  j.nbands = p->j_nbands[var];
   j.lp_base = p->n_gen_wg + p->n_bc_wg;
   j.adj = adj ? 1 : 0;
-  const bool step = adj && a.fused && p->jit_fuse;
   j.u_out = step ? (float *)a.u_out : nullptr;
   j.m = step ? (float *)a.m : nullptr;
   j.v = step ? (float *)a.v : nullptr;
   j.adam = step ? (const float *)a.adam : nullptr;
   void *params[] = {&j};
-  const unsigned blocks = (unsigned)((p->j_waves + 3) / 4);
+  const unsigned blocks = (unsigned)((p->j_waves[var] + 3) / 4);
   hipFunction_t fn = step ? p->jit.fn_step : adj ? p->jit.fn_adj : p->jit.fn_fwd;
   TDX_TRY_HIP(hipModuleLaunchKernel(fn, blocks, 1, 1, 256, 1, 1, 0, s, params, nullptr));
   return TDX_OK;
@@ -1027,7 +1031,8 @@ int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t 
     }
   }
   if (loss) {
-    const int n = p->n_gen_wg + p->n_bc_wg + (p->jit_on ? p->j_waves : 0);
+    const int var = adj && a.fused && p->jit_fuse ? 2 : adj ? 1 : 0;
+    const int n = p->n_gen_wg + p->n_bc_wg + (p->jit_on ? p->j_waves[var] : 0);
     const int g = std::min(256, (n + 4095) / 4096);
     T *mid = a.lp_mid;
     if (g > 1) {
@@ -1186,9 +1191,7 @@ int mat_specialize(tdx_mat_plan *p, const tdx_mat_desc *d) {
   }
   const int rc = mat_jit_compile(src, &p->jit);
   if (rc) return required ? rc : TDX_OK;
-  // geometry: a wavefront owns 64 - 4 R1 columns of a band of rows; bands
-  // of up to 256 rows, fewer while the grid would have under 8192
-  // wavefronts (32 per CU)
+  // geometry: a wavefront owns 64 - 4 R1 columns of a band of rows
   p->jr_lo = ta0 * T0;
   p->jr_hi = tb0 * T0;
   p->jc_lo = ta1 * T1;
@@ -1196,11 +1199,45 @@ int mat_specialize(tdx_mat_plan *p, const tdx_mat_desc *d) {
   const int OW = 64 - 4 * R1;
   p->j_nstrips = (p->jc_hi - p->jc_lo + OW - 1) / OW;
   const int rows = p->jr_hi - p->jr_lo;
-  int hb = 256;
-  while (hb > 32 && (int64_t)p->j_nstrips * ((rows + hb - 1) / hb) < 8192) hb /= 2;
-  p->j_hb = hb;
-  p->j_nbands = (rows + hb - 1) / hb;
-  p->j_waves = p->j_nstrips * p->j_nbands;
+  // Rows per band, per kernel variant: the wavefronts run in rounds of as
+  // many as are resident at once (the variant's occupancy), each round as
+  // long as a band plus its halo and pipeline rows, so the band height that
+  // minimizes rounds x (rows + 4 R0 + 8) is taken (on a 16384^2 grid the
+  // model orders the band heights 64..1024 as the measured times do).
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  const hipFunction_t fns[3] = {p->jit.fn_fwd, p->jit.fn_adj, p->jit.fn_step};
+  p->j_waves_max = 0;
+  for (int var = 0; var < 3; ++var) {
+    int hb = 0;
+    if (d->band_rows > 0) {
+      hb = std::min(d->band_rows, rows);
+    } else {
+      int per_cu = 0;
+      if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fns[var], 256, 0) !=
+              hipSuccess ||
+          per_cu <= 0)
+        per_cu = 2;
+      const int64_t resident = (int64_t)per_cu * 4 * cus;
+      int64_t best = -1;
+      for (int cand : {32, 48, 64, 96, 128, 192, 256, 384}) {
+        const int h = std::min(cand, rows);
+        const int64_t waves = (int64_t)p->j_nstrips * ((rows + h - 1) / h);
+        const int64_t cost = ((waves + resident - 1) / resident) * (h + 4 * R0 + 8);
+        if (best < 0 || cost < best) {  // ties: the shorter band (measured faster)
+          best = cost;
+          hb = h;
+        }
+      }
+    }
+    p->j_hb[var] = hb;
+    p->j_nbands[var] = (rows + hb - 1) / hb;
+    p->j_waves[var] = p->j_nstrips * p->j_nbands[var];
+    p->j_waves_max = std::max(p->j_waves_max, p->j_waves[var]);
+  }
   // the general kernel's tiles: every tile outside the interior rectangle
   std::vector<int32_t> list;
   for (int t0 = 0; t0 < p->tiles0; ++t0)
@@ -1234,6 +1271,7 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   TDX_REQUIRE(d->dp >= 2 && d->dp <= MMAX_DP, "derivative_points must be 2..%d", MMAX_DP);
   TDX_REQUIRE(d->tile_rows >= 0 && d->tile_rows <= 1024, "tile_rows must be 0..1024");
   TDX_REQUIRE(d->exact_div >= 0 && d->exact_div <= 2, "exact_div must be 0, 1 or 2");
+  TDX_REQUIRE(d->band_rows >= 0 && d->band_rows <= (1 << 20), "band_rows must be 0..2^20");
   auto in_ints = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_ints; };
   auto in_reals = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_reals; };
   auto in_data = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_data; };
@@ -1472,7 +1510,7 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   p->d.alpha_forw = nullptr;
   size_t off = 0;
   p->off_lp = off;
-  const int n_pairs = p->n_gen_wg + p->n_bc_wg + (p->jit_on ? p->j_waves : 0);
+  const int n_pairs = p->n_gen_wg + p->n_bc_wg + (p->jit_on ? p->j_waves_max : 0);
   off = align_up(off + (size_t)n_pairs * 2 * ts, 256);
   p->off_seeds = off;
   off = align_up(off + (size_t)std::max(d->n_bc_seeds, 1) * ts, 256);

@@ -178,7 +178,8 @@ class MatTables:
         self.n_eq = prob.n_eq
         self.dp = prob.dp
 
-    def desc(self, exact_div=1, tile_rows: int = 0, specialize: int = 0) -> 'native.MatDesc':
+    def desc(self, exact_div=1, tile_rows: int = 0, specialize: int = 0,
+             band_rows: int = 0) -> 'native.MatDesc':
         """Plan descriptor over these tables (which must outlive it).
         exact_div: 0 fast product, 1 correctly rounded, 2 IEEE division."""
         d = native.MatDesc()
@@ -200,6 +201,7 @@ class MatTables:
         d.exact_div = int(exact_div)
         d.tile_rows = int(tile_rows)
         d.specialize = int(specialize)
+        d.band_rows = int(band_rows)
         return d
 
     def specialized_source(self, exact_div=1, compile: bool = False) -> str:
@@ -223,7 +225,7 @@ class MatEngine:
     """Native 'mat' residual/loss engine for one grid tensor."""
 
     def __init__(self, prob: MatProblem, device: torch.device, exact_div: bool = True,
-                 tile_rows: int = 0, specialize: int = 0):
+                 tile_rows: int = 0, specialize: int = 0, band_rows: int = 0):
         if prob.dtype not in (torch.float32, torch.float64):
             raise NotImplementedError(f"dtype {prob.dtype} is not supported")
         self.prob = prob
@@ -234,7 +236,7 @@ class MatEngine:
             raise native.NativeUnavailable("the native engine runs on a HIP device only")
         t = MatTables(prob, np_dtype)
         self.tables = t
-        d = t.desc(exact_div, tile_rows, specialize)
+        d = t.desc(exact_div, tile_rows, specialize, band_rows)
         handle = ctypes.c_void_p()
         lib = native.lib()
         with torch.cuda.device(self.device):

@@ -73,6 +73,7 @@ class MatDesc(ctypes.Structure):
         ('exact_div', ctypes.c_int),
         ('tile_rows', ctypes.c_int),
         ('specialize', ctypes.c_int),
+        ('band_rows', ctypes.c_int),
     ]
 
 
