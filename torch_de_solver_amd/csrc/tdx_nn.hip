@@ -693,8 +693,16 @@ __device__ __forceinline__ void mma_layer_nt(const T *img, int lane, const T (&b
 // layer by layer; weight gradients accumulate in MFMA tiles (registers) over
 // the whole item, and the workgroup's wavefronts add theirs in a fixed order
 // at the end.
+// Wavefronts per SIMD the register allocation must allow: 4 where the
+// kernel fits 128 registers without spilling (two 32-wide hidden layers:
+// 118), else the compiler's choice.
+constexpr int backward_w_min_waves(int NB, int NH, int NT) {
+  return (NB == 2 && NH <= 2 && NT == 1) ? 4 : 1;
+}
+
 template <typename T, int NB, int NH, int NT>
-__global__ __launch_bounds__(256) void k_backward_w(NNArgs<T> a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    backward_w_min_waves(NB, NH, NT)))) void k_backward_w(NNArgs<T> a) {
   constexpr int HP = 16 * NB;
   constexpr int FB = WRS * HP;  // words of one feature-major buffer
   typedef typename V4<T>::type v4;
