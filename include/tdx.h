@@ -174,7 +174,11 @@ int64_t tdx_quotient_check(float h, int mode);
  *   bc entry [TDX_MBC_INTS]: n_sets, pos_i (ints: n_sets flat positions),
  *           n_terms (0: the plain value u[var]), term_i, var, target_r
  *           (data index of the padded target), lam_r (lambda/max_len),
- *           invden_r (1/max_len), seed_i (first seed slot of this entry)
+ *           invden_r (1/max_len), seed_i (first seed slot of this entry),
+ *           robin (0, or 1 + ints offset of {n_beta, alpha_r}: the entry's
+ *           value is alpha u[var] + sum_k beta_k (its operator), with the
+ *           betas at reals alpha_r + 1 .. alpha_r + n_beta and one extra
+ *           seed slot, first, for the alpha u[var] term; one point set)
  *   bc adjoint CSR over touched grid points: bct_ptr[n_touch+1],
  *           bct_pos[n_touch] (flat index into u incl. the n_eq plane),
  *           bct_seed[nnz], bct_w_r[nnz] (reals index of the weight)

@@ -318,42 +318,62 @@ class NNOracle:
         self.ops = [nn_prepare_operator(op, self.grid, self.central, 'central', h,
                                         inner_order, boundary_order) for op in operators]
         self.conds = build_conditions(conds, axes)
+        lut = {tuple(r): i for i, r in enumerate(self.grid.tolist())}
+
+        def by_type(bop, pts):
+            # the operator prepared per point type of the points
+            # (tedeous/input_preprocessing.py:371-408)
+            typ = [self.types[lut[tuple(p)]] for p in pts.tolist()]
+            prepared = []
+            for t in dict.fromkeys(typ):
+                sel = [i for i, tt in enumerate(typ) if tt == t]
+                prepared.append((sel, nn_prepare_operator(bop, self.grid, pts[sel], t,
+                                                          h, inner_order, boundary_order)))
+            return prepared
+
         for c in self.conds:
-            if c['bop'] is not None and c['type'] == 'operator':
-                # per point type of the condition's points (tedeous/input_preprocessing.py:390-408)
-                prepared = []
-                lut = {tuple(r): i for i, r in enumerate(self.grid.tolist())}
-                typ = [self.types[lut[tuple(p)]] for p in c['bnd'].tolist()]
-                for t in dict.fromkeys(typ):
-                    sel = [i for i, tt in enumerate(typ) if tt == t]
-                    prepared.append((sel, nn_prepare_operator(c['bop'], self.grid, c['bnd'][sel], t,
-                                                              h, inner_order, boundary_order)))
-                c['prepared'] = prepared
+            if c['bop'] is None:
+                continue
+            if c['type'] in ('operator', 'data'):
+                c['prepared'] = by_type(c['bop'], c['bnd'])
+            elif c['type'] == 'periodic':
+                # one prepared operator per point set (tedeous/input_preprocessing.py:401-403)
+                c['prepared'] = [by_type(c['bop'], b) for b in c['bnd']]
 
     def residual(self, net) -> torch.Tensor:
         cols = [apply_operator(lambda t: nn_term_value(net, t), op).reshape(-1, 1) for op in self.ops]
         return cols[0] if len(cols) == 1 else torch.cat(cols, 1)
 
+    @staticmethod
+    def _operator_values(net, prepared):
+        """Operator values at a condition's points, per point type, put back
+        in the points' order: each point keeps its own target (the
+        reference concatenates the types in the iteration order of a Python
+        set of strings, which changes from run to run)."""
+        parts = [(sel, apply_operator(lambda t: nn_term_value(net, t), op)) for sel, op in prepared]
+        v = torch.cat([p for _, p in parts]).reshape(-1, 1)
+        order = [i for sel, _ in parts for i in sel]
+        inv = torch.empty(len(order), dtype=torch.long)
+        inv[torch.tensor(order)] = torch.arange(len(order))
+        return v[inv]
+
     def boundary(self, net):
         bval, true = {}, {}
         for c in self.conds:
-            if c['type'] == 'dirichlet':
+            if c['type'] in ('dirichlet', 'data') and c['bop'] is None:
                 v = net(c['bnd'])[:, c['var']].reshape(-1, 1)
             elif c['type'] == 'periodic' and c['bop'] is None:
                 v = net(c['bnd'][0])[:, c['var']].reshape(-1, 1)
                 for b in c['bnd'][1:]:
                     v = v - net(b)[:, c['var']].reshape(-1, 1)
-            elif c['type'] == 'operator':
-                # each point keeps its own target (see the compiler's note)
-                v = torch.zeros(len(c['bnd']), 1, dtype=self.grid.dtype)
-                parts = []
-                for sel, op in c['prepared']:
-                    parts.append((sel, apply_operator(lambda t: nn_term_value(net, t), op)))
-                v = torch.cat([p for _, p in parts]).reshape(-1, 1)
-                order = [i for sel, _ in parts for i in sel]
-                inv = torch.empty(len(order), dtype=torch.long)
-                inv[torch.tensor(order)] = torch.arange(len(order))
-                v = v[inv]
+            elif c['type'] == 'periodic':
+                # operator at set 0 minus the operator at the other sets
+                # (tedeous/eval.py:346-350)
+                v = self._operator_values(net, c['prepared'][0])
+                for prep in c['prepared'][1:]:
+                    v = v - self._operator_values(net, prep)
+            elif c['type'] in ('operator', 'data'):
+                v = self._operator_values(net, c['prepared'])
             else:
                 raise NotImplementedError(c['type'])
             k = c['type']
@@ -488,7 +508,22 @@ class MatOracle:
     def boundary(self, model):
         bval, true = {}, {}
         for c in self.conds:
-            if c['bop'] is None:
+            if c['type'] == 'robin':
+                # alpha u[var] + sum_beta beta * (operator at the points), alpha
+                # and the betas being the terms' coefficients in order
+                # (tedeous/eval.py:357-388)
+                keys = list(c['bop'].keys())
+                alpha, *betas = [c['bop'][k]['coeff'] for k in keys]
+                pos = c['pos'][0]
+                value_term = alpha * torch.cat([model[c['var']][p] for p in pos]).reshape(-1, 1)
+                var = c['bop'][keys[0]]['var'][0]
+                f = self.field(model, c['bop'])
+                op_val = torch.cat([f[var][p] for p in pos]).reshape(-1, 1)
+                derivative_term = 0
+                for beta in betas:
+                    derivative_term += beta * op_val
+                vals = [value_term + derivative_term]
+            elif c['bop'] is None:
                 vals = [torch.cat([model[c['var']][p] for p in pos]).reshape(-1, 1) for pos in c['pos']]
             else:
                 var = c['bop'][list(c['bop'].keys())[0]]['var'][0]

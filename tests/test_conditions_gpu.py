@@ -1,0 +1,162 @@
+"""Condition types beyond Dirichlet/operator/periodic: the native engines
+against the CPU oracle.
+
+* 'NN' periodic conditions with an operator (the reference's
+  example_wave_1d_basic_periodic.py: du/dx at x = 0 minus du/dx at x = 1,
+  each side with the one-sided scheme of its own point type);
+* 'data' conditions, plain values and operator values, in both modes;
+* 'mat' Robin conditions: alpha u + sum_beta beta * (the whole operator),
+  tedeous/eval.py:357-388 (the reference evaluates Robin in 'mat' mode only:
+  in 'NN' mode its prepared operator is a list and the coefficient lookup
+  fails).
+
+float64: loss to 1e-10, condition values to 1e-12 of their scale, gradients
+to 1e-8.  float32 'mat': residual field and condition values bit-identical
+(correctly rounded division), loss to 1e-6, gradients to 1e-5.  float32
+'NN': the float32 bound of the other NN tests (stencil amplification 1/h^2
+of last-bit differences of the network output).
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import problems
+from conftest import gpu_available
+from oracle import tedeous_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():
+    pytest.skip("no HIP device", allow_module_level=True)
+
+from torch_de_solver_amd import data as api  # noqa: E402
+from torch_de_solver_amd.solution import Operator_bcond_preproc, Solution  # noqa: E402
+
+DEV = torch.device('cuda')
+
+
+def _print(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _nn_problem(kind, dtype):
+    d = api.Domain()
+    d.variable('x', [0, 1], 12, dtype=dtype)
+    d.variable('t', [0, 1], 8, dtype=dtype)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=torch.sin(np.pi * x))
+    if kind == 'periodic_operator':
+        b.periodic([{'x': 0, 't': [0, 1]}, {'x': 1, 't': [0, 1]}])
+        b.periodic([{'x': 0, 't': [0, 1]}, {'x': 1, 't': [0, 1]}],
+                   operator={'du/dx': {'coeff': 1, 'du/dx': [0], 'pow': 1, 'var': 0}})
+    else:
+        b.data({'x': [0, 1], 't': 0.5}, operator=None, value=0.3)
+        b.data({'x': [0, 1], 't': 1},
+               operator={'2 du/dt': {'coeff': 2.0, 'du/dt': [1], 'pow': 1, 'var': 0},
+                         'u^2': {'coeff': -1.0, 'u': [None], 'pow': 2, 'var': 0}},
+               value=0.1)
+    eq = api.Equation()
+    eq.add(problems.wave_eq())
+    return d, eq, b
+
+
+@pytest.mark.parametrize('kind', ['periodic_operator', 'data'])
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
+def test_nn_conditions_against_oracle(kind, dtype):
+    _print(f"[conditions nn] {kind} {dtype}")
+    tdt = torch.float64 if dtype == 'float64' else torch.float32
+    d, eq, b = _nn_problem(kind, dtype)
+    h = 0.01
+    net = problems.make_mlp([2, 24, 24, 1], seed=11, dtype=tdt)
+    problems.randomize_biases(net, seed=12)
+    ref = problems.make_mlp([2, 24, 24, 1], seed=11, dtype=tdt)
+    ref.load_state_dict(net.state_dict())
+    orc = O.NNOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst],
+                     h=h)
+    lo, no, op, bv, tv, keys, lens = orc.evaluate(ref, 1, 100)
+    lo.backward()
+    g_ref = torch.cat([p.grad.reshape(-1) for p in ref.parameters()]).numpy()
+
+    grid = d.build('NN')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict),
+                                 h=h).set_strategy('NN')
+    net = net.to(DEV)
+    sol = Solution(grid, pre, net, 'NN', None, 1, 100)
+    ln, nn_ = sol.evaluate()
+    ln.backward()
+    g = torch.cat([p.grad.reshape(-1) for p in net.parameters()]).cpu().numpy()
+    assert sol.bval_keys == keys
+    bvn = sol.bval.detach().cpu().numpy()
+    bvo = bv.detach().numpy()
+    scale = np.abs(bvo).max()
+    _print(f"  loss {float(ln):.10g} (oracle {float(lo):.10g}); bval max diff "
+           f"{np.abs(bvn - bvo).max():.3e} of {scale:.3e}")
+    if dtype == 'float64':
+        np.testing.assert_allclose(float(ln), float(lo), rtol=1e-10)
+        np.testing.assert_allclose(bvn, bvo, rtol=0, atol=1e-12 * scale)
+        np.testing.assert_allclose(g, g_ref, rtol=1e-8, atol=1e-10 * np.abs(g_ref).max())
+    else:
+        eps = np.finfo(np.float32).eps
+        np.testing.assert_allclose(bvn, bvo, rtol=0, atol=8 * eps * 2.0 / h)
+        assert abs(float(ln) - float(lo)) / float(lo) < 2e-3
+        assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 2e-2
+
+
+def _mat_problem(dtype):
+    d = api.Domain()
+    d.variable('x', [0, 1], 40, dtype=dtype)
+    d.variable('t', [0, 1], 30, dtype=dtype)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=torch.sin(np.pi * x))
+    b.robin({'x': 1, 't': [0, 1]}, value=0.2,
+            operator={'2u': {'coeff': 2.0, 'u': [None], 'pow': 1, 'var': 0},
+                      'du/dx': {'coeff': 0.5, 'du/dx': [0], 'pow': 1, 'var': 0},
+                      'du/dt': {'coeff': -0.25, 'du/dt': [1], 'pow': 1, 'var': 0}})
+    b.robin({'x': 0, 't': [0, 1]}, value=0.0,
+            operator={'u': {'coeff': 1.5, 'u': [None], 'pow': 1, 'var': 0}})
+    b.data({'x': 0.5, 't': [0, 1]}, operator=None, value=0.1)
+    b.data({'x': [0, 1], 't': 1},
+           operator={'du/dt': {'coeff': 3.0, 'du/dt': [1], 'pow': 1, 'var': 0}}, value=0.0)
+    eq = api.Equation()
+    eq.add(problems.wave_eq())
+    return d, eq, b
+
+
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
+def test_mat_robin_and_data_against_oracle(dtype):
+    _print(f"[conditions mat] robin + data {dtype}")
+    tdt = torch.float64 if dtype == 'float64' else torch.float32
+    d, eq, b = _mat_problem(dtype)
+    orc = O.MatOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst])
+    u0 = problems.mat_init([1, 41, 31], seed=13, dtype=tdt, std=0.3)
+    uo = u0.clone().requires_grad_()
+    lo, no, op, bv, tv, keys, lens = orc.evaluate(uo, 1, 10)
+    lo.backward()
+    grid = d.build('mat')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict)).set_strategy('mat')
+    un = u0.to(DEV).requires_grad_()
+    sol = Solution(grid, pre, un, 'mat', None, 1, 10)
+    ln, nn_ = sol.evaluate()
+    ln.backward()
+    assert sol.bval_keys == keys == ['dirichlet', 'robin', 'data']
+    bvn = sol.bval.detach().cpu().numpy()
+    bvo = bv.detach().numpy()
+    gn, go = un.grad.cpu().numpy(), uo.grad.numpy()
+    _print(f"  loss {float(ln):.10g} (oracle {float(lo):.10g}); bval max diff "
+           f"{np.abs(bvn - bvo).max():.3e}; grad max diff {np.abs(gn - go).max():.3e}")
+    if dtype == 'float64':
+        np.testing.assert_allclose(sol.op.cpu().numpy(), op.detach().numpy(), rtol=1e-12,
+                                   atol=1e-12 * float(op.abs().max()))
+        np.testing.assert_allclose(bvn, bvo, rtol=1e-12, atol=1e-13)
+        np.testing.assert_allclose(float(ln), float(lo), rtol=1e-10)
+        np.testing.assert_allclose(gn, go, rtol=1e-8, atol=1e-10 * np.abs(go).max())
+    else:
+        np.testing.assert_array_equal(sol.op.cpu().numpy(), op.detach().numpy())
+        np.testing.assert_array_equal(bvn, bvo)
+        np.testing.assert_allclose(float(ln), float(lo), rtol=1e-6)
+        np.testing.assert_allclose(gn, go, rtol=1e-5, atol=1e-6 * np.abs(go).max())

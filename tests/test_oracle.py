@@ -164,3 +164,83 @@ def test_padded_bc_mean_divides_by_max_len():
     assert m.shape == (4, 2) and lens == [4, 1]
     mse = torch.mean((m - tm) ** 2, 0)
     assert torch.allclose(mse, torch.tensor([30 / 4, 4 / 4]))
+
+
+# ---------------------------------------------------------------------------
+# condition types beyond the reference's golden vectors: pinned by the
+# identities that tie them to the operator and Dirichlet paths pinned above
+# ---------------------------------------------------------------------------
+
+def _wave_periodic_nn():
+    """The reference's example_wave_1d_basic_periodic.py setup, small: u
+    periodic in x, du/dx periodic in x (an operator on each point set)."""
+    d = api.Domain()
+    d.variable('x', [0, 1], 12, dtype='float64')
+    d.variable('t', [0, 1], 8, dtype='float64')
+    axes = dict(d.variable_dict)
+    bop = {'du/dx': {'coeff': 1, 'du/dx': [0], 'pow': 1, 'var': 0}}
+    sets = [{'x': 0, 't': [0, 1]}, {'x': 1, 't': [0, 1]}]
+    return axes, sets, bop
+
+
+def test_oracle_nn_periodic_operator_is_difference_of_operator_conditions():
+    axes, sets, bop = _wave_periodic_nn()
+    eq = [problems.wave_eq()]
+    per = O.NNOracle(axes, eq, [{'bnd': sets, 'bop': bop, 'bval': torch.tensor([0.]),
+                                 'var': 0, 'type': 'periodic'}], h=0.01)
+    ops = [O.NNOracle(axes, eq, [{'bnd': s, 'bop': bop, 'bval': 0.0, 'var': 0,
+                                  'type': 'operator'}], h=0.01) for s in sets]
+    net = problems.make_mlp([2, 16, 16, 1], seed=3, dtype=torch.float64)
+    problems.randomize_biases(net, seed=4)
+    with torch.no_grad():
+        vp = per.boundary(net)[0][:, 0]
+        v0 = ops[0].boundary(net)[0][:, 0]
+        v1 = ops[1].boundary(net)[0][:, 0]
+    # the two sets have points of different types ('f.' at x = 0, 'b.' at x = 1)
+    assert torch.all(v0 != 0)
+    torch.testing.assert_close(vp, v0 - v1, rtol=0, atol=0)
+    assert not torch.allclose(vp, v0 + v1)
+
+
+def test_oracle_nn_data_conditions_equal_dirichlet_and_operator():
+    axes, sets, bop = _wave_periodic_nn()
+    eq = [problems.wave_eq()]
+    pts = {'x': [0, 1], 't': 0.5}
+    net = problems.make_mlp([2, 16, 16, 1], seed=5, dtype=torch.float64)
+    problems.randomize_biases(net, seed=6)
+    for b in (None, bop):
+        data = O.NNOracle(axes, eq, [{'bnd': pts, 'bop': b, 'bval': 0.25, 'var': 0,
+                                      'type': 'data'}], h=0.01)
+        same = O.NNOracle(axes, eq, [{'bnd': pts, 'bop': b, 'bval': 0.25, 'var': 0,
+                                      'type': 'dirichlet' if b is None else 'operator'}], h=0.01)
+        with torch.no_grad():
+            bd, td, kd, _ = data.boundary(net)
+            bs, ts, ks, _ = same.boundary(net)
+        assert kd == ['data']
+        torch.testing.assert_close(bd, bs, rtol=0, atol=0)
+        torch.testing.assert_close(td, ts, rtol=0, atol=0)
+
+
+def test_oracle_mat_robin_is_alpha_u_plus_beta_operator():
+    d = api.Domain()
+    d.variable('x', [0, 1], 10, dtype='float64')
+    d.variable('t', [0, 1], 8, dtype='float64')
+    axes = dict(d.variable_dict)
+    eq = [problems.wave_eq()]
+    bop = {'u': {'coeff': 2.0, 'u': [None], 'pow': 1, 'var': 0},
+           'du/dt': {'coeff': 0.5, 'du/dt': [1], 'pow': 1, 'var': 0},
+           'du/dx': {'coeff': -3.0, 'du/dx': [0], 'pow': 1, 'var': 0}}
+    pts = {'x': 1, 't': [0, 1]}
+    rob = O.MatOracle(axes, eq, [{'bnd': pts, 'bop': bop, 'bval': 0.0, 'var': 0, 'type': 'robin'}])
+    opc = O.MatOracle(axes, eq, [{'bnd': pts, 'bop': bop, 'bval': 0.0, 'var': 0,
+                                  'type': 'operator'}])
+    dic = O.MatOracle(axes, eq, [{'bnd': pts, 'bop': None, 'bval': 0.0, 'var': 0,
+                                  'type': 'dirichlet'}])
+    u = problems.mat_init([1, 11, 9], seed=7, dtype=torch.float64, std=1.0)
+    vr = rob.boundary(u)[0][:, 0]
+    vo = opc.boundary(u)[0][:, 0]
+    vd = dic.boundary(u)[0][:, 0]
+    # alpha = 2 (first term), betas = 0.5, -3: the whole operator (all three
+    # terms) is scaled by each beta
+    torch.testing.assert_close(vr, 2.0 * vd + (0.5 * vo + -3.0 * vo), rtol=1e-15, atol=1e-15)
+    assert not torch.allclose(vr, vo)

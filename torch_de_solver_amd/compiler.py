@@ -230,8 +230,10 @@ def compile_nn(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[
             col_len[typ] = 0
         true = bc['bval'].detach().cpu().reshape(-1).numpy()
         col_vals[typ].append(true)
-        if typ == 'dirichlet' or (typ == 'periodic' and bc['bop'] is None):
-            sets = [bc['bnd']] if typ == 'dirichlet' else list(bc['bnd'])
+        if typ == 'dirichlet' or (typ in ('periodic', 'data') and bc['bop'] is None):
+            # 'data' without an operator: the network's values at the data
+            # points (tedeous/eval.py:402-403)
+            sets = list(bc['bnd']) if typ == 'periodic' else [bc['bnd']]
             pts = [s.detach().cpu().to(dtype).numpy() for s in sets]
             cnt = pts[0].shape[0]
             if any(p.shape[0] != cnt for p in pts):
@@ -249,7 +251,53 @@ def compile_nn(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[
             groups.append(g)
             pending.append((eq, typ, col_len[typ], np.arange(cnt)))
             col_len[typ] += cnt
-        elif typ == 'operator':
+        elif typ == 'periodic':
+            # u-operator at set 0 minus the same operator at sets 1, 2, ...
+            # (tedeous/eval.py:346-350); each set's points take the scheme of
+            # their own point type, so entries are grouped by the tuple of
+            # their sets' types, and a set's entry e pairs with set 0's e
+            sets = [s.detach().cpu().to(dtype) for s in bc['bnd']]
+            cnt = sets[0].shape[0]
+            if any(s.shape[0] != cnt for s in sets):
+                raise ValueError("periodic point sets must have equal sizes")
+            rows_k = [_match_grid_rows(grid_np, s.numpy()) for s in sets]
+            if any(np.any(r < 0) for r in rows_k):
+                raise ValueError("periodic-condition points must be grid points in 'NN' mode")
+            tuples = list(zip(*[types[r].tolist() for r in rows_k]))
+            terms = parse_operator(bc['bop'])
+            start = col_len[typ]
+            for tt in dict.fromkeys(tuples):
+                sel = np.array([e for e in range(cnt) if tuples[e] == tt], dtype=np.int64)
+                slot_base: List[int] = []
+                slot_offset: List[List[float]] = []
+                all_terms: List[NNTerm] = []
+                for k, pt in enumerate(tt):
+                    order = inner_order if pt == 'central' else boundary_order
+                    s_index: Dict = {}
+                    s_list: List = []
+                    tl = _nn_terms(terms, d, pt, order, h, grid_cpu, rows_k[k][sel],
+                                   sets[k][torch.from_numpy(sel)], s_index, s_list)
+                    first = len(slot_base)
+                    slot_base += [k] * len(s_list)
+                    slot_offset += [[s * h for s in sh] for sh in s_list]
+                    for t in tl:
+                        for f in t.factors:
+                            f.stencil = [(slot + first, w) for slot, w in f.stencil]
+                        if k > 0:
+                            t.coeff_scalar = None if t.coeff_scalar is None else -t.coeff_scalar
+                            t.coeff_array = None if t.coeff_array is None else -t.coeff_array
+                        all_terms.append(t)
+                eq = NNEquation(terms=all_terms, lam=0.0, denom=1.0, target=None)
+                g = NNGroup(kind=typ, n_entries=len(sel), slot_base=slot_base,
+                            slot_offset=slot_offset,
+                            bases=[sk[torch.from_numpy(sel)].numpy() for sk in sets],
+                            equations=[eq])
+                groups.append(g)
+                pending.append((eq, typ, start, sel))
+            col_len[typ] += cnt
+        elif typ in ('operator', 'data'):
+            # 'data' with an operator: the operator's values at the data
+            # points in their own column (tedeous/eval.py:390-406)
             bnd = bc['bnd'].detach().cpu().to(dtype)
             rows = _match_grid_rows(grid_np, bnd.numpy())
             if np.any(rows < 0):
@@ -332,6 +380,9 @@ class MatBC:
     terms: Optional[List[MatTerm]]             # None: plain value u[var]
     column: int = -1
     col_start: int = 0
+    # Robin: value alpha * u[var] + sum_beta beta * (operator), with alpha
+    # the first term's coefficient and the betas the later terms'
+    robin: Optional[Tuple[float, List[float]]] = None
 
 
 @dataclass
@@ -463,11 +514,32 @@ def compile_mat(grid: torch.Tensor, operators: List[dict], bconds: Optional[List
             col_len[typ] = 0
             col_true[typ] = []
         col_true[typ].append(bc['bval'].detach().cpu().reshape(-1).numpy())
-        if typ == 'dirichlet':
+        robin = None
+        if typ == 'dirichlet' or (typ == 'data' and bc['bop'] is None):
+            # 'data' without an operator: grid values at the data points
+            # (tedeous/eval.py:402-403)
             pos = [_mat_positions(g, bc['bnd'])]
             terms = None
             var = int(bc['var'])
-        elif typ == 'operator':
+        elif typ == 'robin':
+            # alpha * u[var] + sum over the later terms' coefficients beta of
+            # beta * (the whole operator at the points, read from its first
+            # term's variable) (tedeous/eval.py:357-388)
+            pos = [_mat_positions(g, bc['bnd'])]
+            parsed = parse_operator(bc['bop'])
+            coeffs = []
+            for t in parsed:
+                c = t.coeff
+                if isinstance(c, torch.Tensor) and c.numel() == 1:
+                    c = float(c.reshape(-1)[0])
+                if not isinstance(c, (int, float)):
+                    raise NotImplementedError("Robin conditions take scalar coefficients "
+                                              "in the native 'mat' engine")
+                coeffs.append(float(c))
+            terms = _mat_terms(bc['bop'], g, shape, n_eq)
+            robin = (coeffs[0], coeffs[1:])
+            var = int(bc['var'])
+        elif typ in ('operator', 'data'):
             pos = [_mat_positions(g, bc['bnd'])]
             terms = _mat_terms(bc['bop'], g, shape, n_eq)
             # value read from the bop's first term variable
@@ -483,7 +555,7 @@ def compile_mat(grid: torch.Tensor, operators: List[dict], bconds: Optional[List
             raise NotImplementedError(
                 f"boundary type {typ!r} is not supported by the native 'mat' engine")
         m = MatBC(typ=typ, var=var, positions=pos, terms=terms,
-                  column=keys.index(typ), col_start=col_len[typ])
+                  column=keys.index(typ), col_start=col_len[typ], robin=robin)
         col_len[typ] += len(pos[0])
         bcs.append(m)
     max_len = max(col_len.values()) if keys else 0

@@ -637,6 +637,15 @@ __global__ __launch_bounds__(MAT_THREADS) void k_mat_bc(MatArgs<T> a, int adj) {
           r = (t == 0) ? tv : r + tv;
         }
         v = r;
+        if (Bc[9] > 0) {
+          // Robin: alpha u[var] + (0 + beta_1 r) + beta_2 r ... in the
+          // reference's order (tedeous/eval.py:372-387)
+          const int32_t *Rb = a.ints + Bc[9] - 1;
+          const T *R = a.reals + Rb[1];
+          T dt = T(0);
+          for (int k = 0; k < Rb[0]; ++k) dt = dt + R[1 + k] * r;
+          v = R[0] * a.u[var * a.N + pos] + dt;
+        }
       }
       val = (s == 0) ? v : val - v;
     }
@@ -647,8 +656,18 @@ __global__ __launch_bounds__(MAT_THREADS) void k_mat_bc(MatArgs<T> a, int adj) {
     if (adj) {
       const T dr = T(2) * lam_d * diff * (*a.dloss);
       int sidx = seed_i;
+      T robin_w = T(1);
+      if (Bc[9] > 0) {
+        // d value / d u[var] = alpha (its own seed, first); the operator's
+        // seeds carry the sum of the betas
+        const int32_t *Rb = a.ints + Bc[9] - 1;
+        const T *R = a.reals + Rb[1];
+        a.bc_seeds[sidx++] = dr * R[0];
+        robin_w = T(0);
+        for (int k = 0; k < Rb[0]; ++k) robin_w = robin_w + R[1 + k];
+      }
       for (int s = 0; s < n_sets; ++s) {
-        const T sg = s == 0 ? dr : -dr;
+        const T sg = (s == 0 ? dr : -dr) * robin_w;
         if (n_terms == 0) {
           a.bc_seeds[sidx++] = sg;
           continue;
@@ -1390,6 +1409,13 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
           nf += Tm[0];
         }
         nseed = nf * B[0];
+      }
+      if (B[9] != 0) {
+        // Robin: one point set, an operator, {n_beta, alpha_r} in range
+        if (B[9] < 1 || !in_ints(B[9] - 1, 2) || B[0] != 1 || B[2] < 1) return fail("bc robin record");
+        const int32_t *Rb = d->ints + B[9] - 1;
+        if (Rb[0] < 0 || !in_reals(Rb[1], 1 + (int64_t)Rb[0])) return fail("bc robin coefficients");
+        nseed += 1;
       }
       if (B[8] < 0 || B[8] + nseed > d->n_bc_seeds) return fail("bc seed slots out of range");
     }

@@ -128,6 +128,17 @@ class MatTables:
                 tgt = prob.targets[bc.col_start + e, bc.column] if bc.col_start + e < max_len else 0.0
                 target_r = add_data([tgt])
                 seed_i = seed
+                robin_rec = 0
+                if bc.robin is not None:
+                    # ints {number of betas, reals index of alpha}, then the
+                    # betas follow alpha in the reals; first seed: alpha u[var]
+                    alpha, betas = bc.robin
+                    robin_rec = len(ints) + 1
+                    ints.extend([len(betas), add_real(np_dtype(alpha))])
+                    for b in betas:
+                        add_real(np_dtype(b))
+                    csr.setdefault(bc.var * N + sets[0], []).append((seed, 1.0))
+                    seed += 1
                 for s, pos in enumerate(sets):
                     g0, g1 = divmod(pos, self.n1)
                     if n_terms == 0:
@@ -144,7 +155,7 @@ class MatTables:
                                     csr.setdefault(f.var * N + j, []).append((seed, w))
                             seed += 1
                 entries.append([len(sets), pos_i, n_terms, term_i, bc.var, target_r, lam_r,
-                                inv_r, seed_i, 0])
+                                inv_r, seed_i, robin_rec])
                 self.bc_entries.append((bi, bc.column, bc.col_start + e))
         self.bc_i = len(ints)
         for rec in entries:
