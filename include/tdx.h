@@ -70,8 +70,10 @@ const char *tdx_last_error(void);
  *           slot_off_r, n_eq, eq_i, row_base
  *   slot_base_i -> n_slots ints: data offset of entry 0 of the slot's base
  *   eq      [TDX_EQ_INTS]: n_terms, term_i, target_off (-1: zero target),
- *           resid_off (-1: none), lam_r (lambda/denominator),
- *           invden_r (1/denominator)
+ *           resid_off (-1: none), lam_r (lambda/denominator: loss weight),
+ *           invden_r (1/denominator: normalized-loss weight), seed_r
+ *           (weight of the seeds 2 w r dr/du; lam_r except for the causal
+ *           loss's operator equations)
  *   term    [TDX_TERM_INTS]: n_factors, factor_i, coeff_kind
  *           (0: reals[coeff_idx], 1: data[coeff_idx + entry]), coeff_idx
  *   factor  [TDX_FACTOR_INTS]: var, pow_kind (0: integer pow_int,
@@ -80,7 +82,7 @@ const char *tdx_last_error(void);
  * Seed rows: row (group g, slot s, entry e) = row_base + s*n_entries + e.
  */
 #define TDX_GROUP_INTS 8
-#define TDX_EQ_INTS 6
+#define TDX_EQ_INTS 7
 #define TDX_TERM_INTS 4
 #define TDX_FACTOR_INTS 6
 #define TDX_STENCIL_INTS 2
@@ -102,6 +104,13 @@ typedef struct tdx_nn_desc {
                        32: two 16-row tiles per wavefront iteration of the
                        wavefront-private kernel (float), otherwise one) */
   int chunk_tiles;  /* tiles per backward work item (0: default) */
+  /* Causal loss (tedeous/losses.py:137-182), one instance: tol != 0 replaces
+   * the operator term of group 0's equations (loss and normalized weights 0,
+   * seed weight 1/N) by mean(w * res), res[p] = sum_eq r_eq[p]^2 over the
+   * N points of group 0 read as causal_nt slices of N / causal_nt points,
+   * w[i][k] = exp(-tol * sum_{j<i} res[j][k]) (no gradient through w). */
+  double causal_tol;
+  int causal_nt;
 } tdx_nn_desc;
 
 typedef struct tdx_nn_plan tdx_nn_plan;

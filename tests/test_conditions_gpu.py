@@ -8,7 +8,10 @@ against the CPU oracle.
 * 'mat' Robin conditions: alpha u + sum_beta beta * (the whole operator),
   tedeous/eval.py:357-388 (the reference evaluates Robin in 'mat' mode only:
   in 'NN' mode its prepared operator is a list and the coefficient lookup
-  fails).
+  fails);
+* the causal loss in 'NN' mode (tedeous/losses.py:137-182): several points
+  per slice (2-D) and one point per slice (1-D), and the fused train step
+  against torch.optim.Adam over the autograd path.
 
 float64: loss to 1e-10, condition values to 1e-12 of their scale, gradients
 to 1e-8.  float32 'mat': residual field and condition values bit-identical
@@ -160,3 +163,93 @@ def test_mat_robin_and_data_against_oracle(dtype):
         np.testing.assert_array_equal(bvn, bvo)
         np.testing.assert_allclose(float(ln), float(lo), rtol=1e-6)
         np.testing.assert_allclose(gn, go, rtol=1e-5, atol=1e-6 * np.abs(go).max())
+
+
+# ---------------------------------------------------------------------------
+# causal loss ('NN' mode, tedeous/losses.py:137-182)
+# ---------------------------------------------------------------------------
+
+def _causal_problem(name, dtype):
+    if name == 'wave':
+        d, eq, b = _nn_problem('data', dtype)
+        return d, eq, b, [2, 24, 24, 1], 0.5
+    d = api.Domain()
+    d.variable('x', [-1, 1], 30, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'x': -1}, value=-1)
+    b.dirichlet({'x': 1}, value=1)
+    eq = api.Equation()
+    eq.add({
+        '(1-x^2)u_xx': {'coeff': lambda g: 1 - g ** 2, 'u_xx': [0, 0], 'pow': 1, 'var': 0},
+        '-2x u_x': {'coeff': lambda g: -2 * g, 'u_x': [0], 'pow': 1, 'var': 0},
+        '12u': {'coeff': 12, 'u': [None], 'pow': 1, 'var': 0},
+    })
+    return d, eq, b, [1, 20, 20, 1], 2.0
+
+
+@pytest.mark.parametrize('name', ['wave', 'legendre'])
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
+def test_nn_causal_loss_against_oracle(name, dtype):
+    _print(f"[causal nn] {name} {dtype}")
+    tdt = torch.float64 if dtype == 'float64' else torch.float32
+    d, eq, b, layers, tol = _causal_problem(name, dtype)
+    h = 0.01
+    net = problems.make_mlp(layers, seed=21, dtype=tdt)
+    problems.randomize_biases(net, seed=22)
+    ref = problems.make_mlp(layers, seed=21, dtype=tdt)
+    ref.load_state_dict(net.state_dict())
+    orc = O.NNOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst],
+                     h=h)
+    lo, no, op, bv, tv, keys, lens = orc.evaluate(ref, 1, 100, tol=tol)
+    lo.backward()
+    g_ref = torch.cat([p.grad.reshape(-1) for p in ref.parameters()]).numpy()
+    grid = d.build('NN')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict),
+                                 h=h).set_strategy('NN')
+    net = net.to(DEV)
+    sol = Solution(grid, pre, net, 'NN', None, 1, 100, tol=tol)
+    _print(f"  slices {sol.problem.causal_nt} of {sol.problem.n_central} points")
+    ln, nn_ = sol.evaluate()
+    ln.backward()
+    g = torch.cat([p.grad.reshape(-1) for p in net.parameters()]).cpu().numpy()
+    _print(f"  loss {float(ln):.10g} (oracle {float(lo):.10g}), normalized {float(nn_):.10g} "
+           f"({float(no):.10g})")
+    if dtype == 'float64':
+        np.testing.assert_allclose(float(ln), float(lo), rtol=1e-10)
+        np.testing.assert_allclose(float(nn_), float(no), rtol=1e-10)
+        np.testing.assert_allclose(g, g_ref, rtol=1e-8, atol=1e-10 * np.abs(g_ref).max())
+    else:
+        assert abs(float(ln) - float(lo)) / float(lo) < 2e-3
+        assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 2e-2
+
+
+def test_nn_causal_fused_step_equals_autograd_adam():
+    """Three fused native Adam steps against torch.optim.Adam over the
+    autograd path of the same Solution (float64)."""
+    _print("[causal nn] fused step vs torch Adam")
+    d, eq, b, layers, tol = _causal_problem('wave', 'float64')
+    grid = d.build('NN')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict),
+                                 h=0.01).set_strategy('NN')
+    nets = []
+    for _ in range(2):
+        n = problems.make_mlp(layers, seed=23, dtype=torch.float64)
+        problems.randomize_biases(n, seed=24)
+        nets.append(n.to(DEV))
+    sol_a = Solution(grid, pre, nets[0], 'NN', None, 1, 100, tol=tol)
+    opt = torch.optim.Adam(nets[0].parameters(), lr=1e-3)
+    for _ in range(3):
+        opt.zero_grad()
+        sol_a.evaluate()[0].backward()
+        opt.step()
+    sol_b = Solution(grid, pre, nets[1], 'NN', None, 1, 100, tol=tol)
+    from torch_de_solver_amd.nn_engine import flatten_module_params
+    flat = flatten_module_params(nets[1])
+    m, v = torch.zeros_like(flat), torch.zeros_like(flat)
+    step = torch.zeros(1, dtype=torch.int64, device=DEV)
+    loss = torch.zeros(2, dtype=torch.float64, device=DEV)
+    for _ in range(3):
+        sol_b.engine.train_step(flat, m, v, step, 1e-3, 0.9, 0.999, 1e-8, loss)
+    torch.cuda.synchronize()
+    pa = torch.cat([p.detach().reshape(-1) for p in nets[0].parameters()]).cpu().numpy()
+    np.testing.assert_allclose(flat.cpu().numpy(), pa, rtol=1e-9, atol=1e-12)

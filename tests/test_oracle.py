@@ -244,3 +244,49 @@ def test_oracle_mat_robin_is_alpha_u_plus_beta_operator():
     # terms) is scaled by each beta
     torch.testing.assert_close(vr, 2.0 * vd + (0.5 * vo + -3.0 * vo), rtol=1e-15, atol=1e-15)
     assert not torch.allclose(vr, vo)
+
+
+def test_oracle_causal_loss_weights_by_loops():
+    """The causal loss (tedeous/losses.py:137-182) against explicit loops:
+    w[i][k] = exp(-tol * sum_{j<i} res[j][k]) over slices i of the points,
+    res = sum over equations of op^2, loss = mean(w res) + boundary part,
+    no gradient through w."""
+    g = torch.Generator().manual_seed(3)
+    n_t, K, n_eq = 5, 4, 2
+    op = torch.randn(n_t * K, n_eq, generator=g, dtype=torch.float64).requires_grad_()
+    bval = torch.randn(6, 2, generator=g, dtype=torch.float64)
+    true = torch.randn(6, 2, generator=g, dtype=torch.float64)
+    lam_b = torch.tensor([[10.0, 3.0]], dtype=torch.float64)
+    tol = 0.7
+    loss, norm = O.causal_loss(op, bval, true, lam_b, tol, n_t)
+    loss.backward()
+    o = op.detach().numpy()
+    res = (o ** 2).sum(1).reshape(n_t, K)
+    w = np.ones_like(res)
+    for i in range(n_t):
+        for k in range(K):
+            w[i, k] = np.exp(-tol * sum(res[j, k] for j in range(i)))
+    loss_oper = (w * res).mean()
+    bd = ((bval - true) ** 2).mean(0).numpy()
+    np.testing.assert_allclose(float(loss), loss_oper + (bd * [10.0, 3.0]).sum(), rtol=1e-14)
+    np.testing.assert_allclose(float(norm), loss_oper + bd.sum(), rtol=1e-14)
+    # gradient: 2 w op / N per equation column (w is a constant)
+    gw = 2 * np.repeat(w.reshape(-1, 1), n_eq, 1) * o / (n_t * K)
+    np.testing.assert_allclose(op.grad.numpy(), gw, rtol=1e-13)
+    # slices that do not divide the points: one slice per point
+    l2, _ = O.causal_loss(op.detach(), bval, true, lam_b, tol, 3)
+    r1 = (o ** 2).sum(1)
+    w1 = np.exp(-tol * np.concatenate([[0.0], np.cumsum(r1)[:-1]]))
+    np.testing.assert_allclose(float(l2), (w1 * r1).mean() + (bd * [10.0, 3.0]).sum(), rtol=1e-13)
+
+
+def test_oracle_causal_loss_small_tol_is_default_loss():
+    g = torch.Generator().manual_seed(4)
+    op = torch.randn(12, 1, generator=g, dtype=torch.float64)
+    bval = torch.randn(5, 1, generator=g, dtype=torch.float64)
+    true = torch.zeros(5, 1, dtype=torch.float64)
+    lam_b = torch.tensor([[100.0]], dtype=torch.float64)
+    lc, nc = O.causal_loss(op, bval, true, lam_b, 1e-14, 4)
+    ld, nd = O.default_loss(op, bval, true, torch.ones(1, 1, dtype=torch.float64), lam_b)
+    np.testing.assert_allclose(float(lc), float(ld), rtol=1e-12)
+    np.testing.assert_allclose(float(nc), float(nd), rtol=1e-12)

@@ -116,6 +116,10 @@ class NNProblem:
     bc_max_len: int
     dtype: torch.dtype
     h: float
+    # causal loss (tedeous/losses.py:137-182): tolerance and the number of
+    # slices the operator points are read in (0: default loss)
+    causal_tol: float = 0.0
+    causal_nt: int = 0
 
 
 def _coeff_for_entries(coeff, grid: torch.Tensor, entry_idx: np.ndarray,
@@ -184,7 +188,7 @@ def _match_grid_rows(grid_np: np.ndarray, pts: np.ndarray) -> np.ndarray:
 def compile_nn(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[dict]],
                n_out: int, h: float = 0.001, inner_order: str = '1',
                boundary_order: str = '2', lambda_operator=1,
-               lambda_bound=1) -> NNProblem:
+               lambda_bound=1, tol: float = 0.0) -> NNProblem:
     """Lower an 'NN' problem (tedeous/input_preprocessing.py:173-408,
     tedeous/eval.py:90-461, tedeous/losses.py:84-135)."""
     if inner_order == '2':
@@ -347,10 +351,21 @@ def compile_nn(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[
         eq.col_pos = pos
         eq.target = padded[typ][pos]
 
+    # causal slices: the number of distinct first coordinates among the
+    # 'central' points (tedeous/solution.py:51-53); when it does not divide
+    # the points, the reference's reshape fails and it takes one slice per
+    # point (tedeous/losses.py:162-166)
+    causal_nt = 0
+    if tol:
+        n_c = len(central_idx)
+        causal_nt = int(torch.unique(central_pts[:, 0]).numel())
+        if causal_nt == 0 or n_c % causal_nt != 0:
+            causal_nt = n_c
     return NNProblem(d_in=d, n_out=n_out, groups=groups,
                      n_central=len(central_idx), central_index=central_idx,
                      bc_keys=keys, bc_lengths=[col_len[k] for k in keys],
-                     bc_max_len=max_len, dtype=dtype, h=h)
+                     bc_max_len=max_len, dtype=dtype, h=h,
+                     causal_tol=float(tol), causal_nt=causal_nt)
 
 
 # ----------------------------------------------------------------------------

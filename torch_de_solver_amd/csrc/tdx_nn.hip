@@ -34,6 +34,7 @@
 #include "tdx_common.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace tdx {
@@ -393,7 +394,7 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
         const T lam_d = a.reals[E[4]];
         lsum += lam_d * sq;
         lnsum += a.reals[E[5]] * sq;
-        const T dr = T(2) * lam_d * diff;
+        const T dr = T(2) * a.reals[E[6]] * diff;
         // seeds: d r / d u(slot, var), through every factor of every term;
         // d(prod_j f_j^p_j)/d f_k = p_k f_k^(p_k-1) * prefix_k * suffix_k
         for (int t = 0; t < n_terms; ++t) {
@@ -1270,6 +1271,58 @@ __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
 // ---------------------------------------------------------------------------
 // reductions and the optimizer
 // ---------------------------------------------------------------------------
+// Causal loss (tedeous/losses.py:137-182).  Thread k walks column k of the
+// N = nt x K points of group 0 (point p = i K + k): res = sum_eq r_eq[p]^2,
+// w = exp(-tol * (sum of res over the earlier slices of the column)),
+// stores w and w * res.
+template <typename T>
+__global__ void k_causal_w(const T *resid, int n_eq, int N, int nt, int K, T tol, T *w, T *wr) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  T cum = T(0);
+  for (int i = 0; i < nt; ++i) {
+    const int p = i * K + k;
+    T res = T(0);
+    for (int q = 0; q < n_eq; ++q) {
+      const T r = resid[(int64_t)q * N + p];
+      res = (q == 0) ? r * r : res + r * r;
+    }
+    const T wv = exp(-tol * cum);
+    w[p] = wv;
+    wr[p] = wv * res;
+    cum = cum + res;
+  }
+}
+
+// Seeds of group 0's rows (slot-major: row s N + p) times w[p].
+template <typename T>
+__global__ void k_causal_seeds(T *seeds, const T *w, int N, int n_slots, int n_out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t tot = (int64_t)n_slots * N * n_out;
+  if (i >= tot) return;
+  const int p = (int)((i / n_out) % N);
+  seeds[i] *= w[p];
+}
+
+// mean(w * res) in a fixed order, added to both loss values.
+template <typename T>
+__global__ void k_causal_loss(const T *wr, int N, T *loss) {
+  __shared__ T red[256];
+  T s = T(0);
+  for (int p = threadIdx.x; p < N; p += 256) s += wr[p];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const T m = red[0] / T(N);
+    loss[0] = loss[0] + m;
+    loss[1] = loss[1] + m;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_loss(NNArgs<T> a, T *loss) {
   __shared__ T red[512];
@@ -1399,6 +1452,12 @@ struct tdx_nn_plan {
   DevTable groups, ints, reals, data, k1_items, k2_items, ik1p, ik1i, ik2p, ik2i;
   // workspace layout (byte offsets)
   size_t off_packed, off_seeds, off_lp, off_partial, off_one, ws_bytes;
+  // causal loss: tolerance, slices, points of group 0 (N = nt * K), its
+  // equations and slots; workspace for residuals, weights and w * res
+  int causal = 0;
+  double causal_tol = 0;
+  int c_nt = 0, c_K = 0, c_N = 0, c_neq = 0, c_slots = 0;
+  size_t off_resid = 0, off_cw = 0, off_cwr = 0;
   // optional launch timing: event pairs per kernel kind
   int prof_cap = 0;
   int prof_n[TDX_NN_KERNEL_KINDS] = {0};
@@ -1478,9 +1537,12 @@ int launch_pack(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
 }
 
 template <typename T, int NB>
-int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStream_t s) {
+int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStream_t s,
+                   void *ws) {
   int rc = launch_pack<T, NB>(p, a, s);
   if (rc) return rc;
+  // the causal loss reads group 0's residuals
+  if (p->causal && !resid) resid = (T *)((char *)ws + p->off_resid);
   a.resid = resid;
   {
     ProfScope ps(p, 0, s);
@@ -1493,6 +1555,20 @@ int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStre
   if (loss) {
     hipLaunchKernelGGL((k_loss<T>), dim3(p->n_inst), dim3(256), 0, s, a, loss);
     TDX_TRY_HIP(hipGetLastError());
+  }
+  if (p->causal) {
+    T *w = (T *)((char *)ws + p->off_cw), *wr = (T *)((char *)ws + p->off_cwr);
+    hipLaunchKernelGGL((k_causal_w<T>), dim3((p->c_K + 255) / 256), dim3(256), 0, s,
+                       (const T *)resid, p->c_neq, p->c_N, p->c_nt, p->c_K, T(p->causal_tol), w, wr);
+    TDX_TRY_HIP(hipGetLastError());
+    const int64_t tot = (int64_t)p->c_slots * p->c_N * p->n_out;
+    hipLaunchKernelGGL((k_causal_seeds<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       a.seeds, (const T *)w, p->c_N, p->c_slots, p->n_out);
+    TDX_TRY_HIP(hipGetLastError());
+    if (loss) {
+      hipLaunchKernelGGL((k_causal_loss<T>), dim3(1), dim3(256), 0, s, (const T *)wr, p->c_N, loss);
+      TDX_TRY_HIP(hipGetLastError());
+    }
   }
   return TDX_OK;
 }
@@ -1608,7 +1684,7 @@ int nn_dispatch(const tdx_nn_plan *p, int op, const void *params, void *params_m
   case NBV: {                                                                     \
     int rc = TDX_OK;                                                              \
     if (op == 0) {                                                                \
-      rc = launch_forward<T, NBV>(p, a, (T *)loss, (T *)resid, s);                \
+      rc = launch_forward<T, NBV>(p, a, (T *)loss, (T *)resid, s, ws);            \
     } else if (op == 1) {                                                         \
       a.dloss = (const T *)dloss;                                                 \
       rc = launch_pack<T, NBV>(p, a, s);                                          \
@@ -1616,7 +1692,7 @@ int nn_dispatch(const tdx_nn_plan *p, int op, const void *params, void *params_m
       if (!rc) rc = launch_reduce<T>(p, a, 0, (T *)dparams, nullptr, nullptr,     \
                                      nullptr, nullptr, h, s);                     \
     } else {                                                                      \
-      rc = launch_forward<T, NBV>(p, a, (T *)loss, nullptr, s);                   \
+      rc = launch_forward<T, NBV>(p, a, (T *)loss, nullptr, s, ws);               \
       if (!rc) {                                                                  \
         hipLaunchKernelGGL((k_set_one<T>), dim3(1), dim3(64), 0, s,               \
                            (T *)((char *)ws + p->off_one));                       \
@@ -1673,7 +1749,8 @@ int validate_nn(const tdx_nn_desc *d, int64_t *n_rows_out) {
       TDX_REQUIRE(E[0] >= 1 && in_ints(E[1], (int64_t)E[0] * TDX_TERM_INTS), "eq table");
       TDX_REQUIRE(E[2] < 0 || in_data(E[2], ne), "target out of range");
       TDX_REQUIRE(E[3] < 0 || (E[3] + (int64_t)ne <= d->n_resid), "residual slot out of range");
-      TDX_REQUIRE(in_reals(E[4], 1) && in_reals(E[5], 1), "eq weights out of range");
+      TDX_REQUIRE(in_reals(E[4], 1) && in_reals(E[5], 1) && in_reals(E[6], 1),
+                  "eq weights out of range");
       for (int t = 0; t < E[0]; ++t) {
         const int32_t *Tm = d->ints + E[1] + t * TDX_TERM_INTS;
         TDX_REQUIRE(Tm[0] >= 1 && Tm[0] <= MAXF, "a term needs 1..%d factors", MAXF);
@@ -1713,6 +1790,17 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   if (rc) return rc;
 
   TDX_REQUIRE(n_rows < (1LL << 31), "too many rows");
+  if (d->causal_tol != 0.0) {
+    const int32_t *G0 = d->groups;
+    TDX_REQUIRE(std::isfinite(d->causal_tol), "causal tol must be finite");
+    TDX_REQUIRE(d->n_inst == 1, "the causal loss serves one instance");
+    TDX_REQUIRE(d->causal_nt >= 1 && G0[1] % d->causal_nt == 0,
+                "causal slices must divide the %d points of group 0", G0[1]);
+    TDX_REQUIRE(G0[7] == 0, "group 0 must start at row 0");
+    for (int q = 0; q < G0[5]; ++q)
+      TDX_REQUIRE(d->ints[G0[6] + q * TDX_EQ_INTS + 3] == (int64_t)q * G0[1],
+                  "causal loss: group 0's residuals must lead the residual output");
+  }
   tdx_nn_plan *p = new tdx_nn_plan();
   p->dtype = d->dtype;
   p->d_in = d->d_in;
@@ -1842,6 +1930,22 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   off = align_up(off + (size_t)p->n_k2 * p->P * ts, 256);
   p->off_one = off;
   off = align_up(off + ts, 256);
+  if (d->causal_tol != 0.0) {
+    const int32_t *G0 = d->groups;
+    p->causal = 1;
+    p->causal_tol = d->causal_tol;
+    p->c_N = G0[1];
+    p->c_slots = G0[2];
+    p->c_neq = G0[5];
+    p->c_nt = d->causal_nt;
+    p->c_K = p->c_N / std::max(1, p->c_nt);
+    p->off_resid = off;
+    off = align_up(off + (size_t)std::max<int64_t>(d->n_resid, 1) * ts, 256);
+    p->off_cw = off;
+    off = align_up(off + (size_t)p->c_N * ts, 256);
+    p->off_cwr = off;
+    off = align_up(off + (size_t)p->c_N * ts, 256);
+  }
   p->ws_bytes = off;
   *out = p;
   return TDX_OK;

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libtdx.so')
 
 TDX_GROUP_INTS = 8
-TDX_EQ_INTS = 6
+TDX_EQ_INTS = 7
 TDX_TERM_INTS = 4
 TDX_FACTOR_INTS = 6
 TDX_STENCIL_INTS = 2
@@ -55,6 +55,7 @@ class NNDesc(ctypes.Structure):
         ('n_data', ctypes.c_int64), ('data', ctypes.c_void_p),
         ('n_resid', ctypes.c_int64),
         ('tile_rows', ctypes.c_int), ('chunk_tiles', ctypes.c_int),
+        ('causal_tol', ctypes.c_double), ('causal_nt', ctypes.c_int),
     ]
 
 

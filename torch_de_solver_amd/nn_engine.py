@@ -139,12 +139,21 @@ class NNTables:
                         ints[term_i + ti * native.TDX_TERM_INTS:
                              term_i + (ti + 1) * native.TDX_TERM_INTS] = [len(t.factors), factor_i, ck, ci]
                     target_off = add_data(eq.target) if eq.target is not None else -1
-                    lam_r = add_real(eq.lam / eq.denom)
-                    inv_r = add_real(1.0 / eq.denom)
+                    if prob.causal_tol and gi == 0:
+                        # causal loss: the operator term is mean(w res),
+                        # formed after the forward pass; its seeds carry
+                        # 1/N here and w there
+                        lam_r = add_real(0.0)
+                        inv_r = add_real(0.0)
+                        seed_r = add_real(1.0 / eq.denom)
+                    else:
+                        lam_r = add_real(eq.lam / eq.denom)
+                        inv_r = add_real(1.0 / eq.denom)
+                        seed_r = lam_r
                     self.resid.append(ResidSlot(inst, gi, qi, g.kind, n_resid, g.n_entries,
                                                 eq.column, eq.col_pos))
                     ints[eq_i + qi * native.TDX_EQ_INTS: eq_i + (qi + 1) * native.TDX_EQ_INTS] = [
-                        len(eq.terms), term_i, target_off, n_resid, lam_r, inv_r]
+                        len(eq.terms), term_i, target_off, n_resid, lam_r, inv_r, seed_r]
                     n_resid += g.n_entries
                 groups.append([inst, g.n_entries, len(g.slot_base), slot_base_i, slot_off_r,
                                len(g.equations), eq_i, row])
@@ -200,6 +209,11 @@ class NNEngine:
         desc.n_resid = t.n_resid
         desc.tile_rows = tile_rows
         desc.chunk_tiles = chunk_tiles
+        if problems[0].causal_tol:
+            if self.n_inst != 1:
+                raise NotImplementedError("the causal loss serves one instance per plan")
+            desc.causal_tol = float(problems[0].causal_tol)
+            desc.causal_nt = int(problems[0].causal_nt)
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             native.check(lib.tdx_nn_plan_create(ctypes.byref(desc), ctypes.byref(handle)))

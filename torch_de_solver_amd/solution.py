@@ -59,8 +59,8 @@ class Solution:
                  exact_div: bool = True):
         if weak_form:
             raise NotImplementedError("weak-form losses are not served by this engine")
-        if tol:
-            raise NotImplementedError("causal losses (tol != 0) are not served by this engine")
+        if tol and mode != 'NN':
+            raise NotImplementedError("causal losses (tol != 0) are served in 'NN' mode only")
         if batch_size is not None:
             raise NotImplementedError("mini-batching is not served by this engine")
         if mode not in ('NN', 'mat'):
@@ -79,7 +79,7 @@ class Solution:
             self.problem: NNProblem = compile_nn(
                 grid, ops, equal_cls.bconds, n_out=self.arch.n_out, h=equal_cls.h,
                 inner_order=equal_cls.inner_order, boundary_order=equal_cls.boundary_order,
-                lambda_operator=lambda_operator, lambda_bound=lambda_bound)
+                lambda_operator=lambda_operator, lambda_bound=lambda_bound, tol=tol)
             self.engine = NNEngine([self.problem], self.arch, device)
         else:
             self.problem: MatProblem = compile_mat(
