@@ -694,16 +694,16 @@ __device__ __forceinline__ void mma_layer_nt(const T *img, int lane, const T (&b
 // layer by layer; weight gradients accumulate in MFMA tiles (registers) over
 // the whole item, and the workgroup's wavefronts add theirs in a fixed order
 // at the end.
-// Wavefronts per SIMD the register allocation must allow: 4 where the
-// kernel fits 128 registers without spilling (two 32-wide hidden layers:
-// 118), else the compiler's choice.
-constexpr int backward_w_min_waves(int NB, int NH, int NT) {
-  return (NB == 2 && NH <= 2 && NT == 1) ? 4 : 1;
+// Wavefronts per SIMD the register allocation must allow: 3 for float
+// networks of two 32-wide hidden layers (fits without spilling), else the
+// compiler's choice.
+constexpr int backward_w_min_waves(int bytes, int NB, int NH, int NT) {
+  return (bytes == 4 && NB == 2 && NH <= 2 && NT == 1) ? 3 : 1;
 }
 
 template <typename T, int NB, int NH, int NT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
-    backward_w_min_waves(NB, NH, NT)))) void k_backward_w(NNArgs<T> a) {
+    backward_w_min_waves(sizeof(T), NB, NH, NT)))) void k_backward_w(NNArgs<T> a) {
   constexpr int HP = 16 * NB;
   constexpr int FB = WRS * HP;  // words of one feature-major buffer
   typedef typename V4<T>::type v4;
@@ -761,6 +761,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   for (int b = 0; b < NB; ++b) sacc[b] = v4{0, 0, 0, 0};
   T blacc = T(0);
   const int col_b1 = d_in, col_wl = d_in + 1;
+  // one output: its weight gradient per lane (feature 16 mb + crow(q, i) over
+  // the lane's rows) on the VALU, summed over the row lanes at the end,
+  // instead of a tile column of matrix products
+  const bool wl_valu = n_out == 1;
+  // up to two inputs: the first layer's weight and bias gradients likewise
+  const bool w1_valu = d_in <= 2;
+  T wlacc[NB][4], w1acc[NB][4][2], b1acc[NB][4];
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      wlacc[b][i] = T(0);
+      w1acc[b][i][0] = T(0);
+      w1acc[b][i][1] = T(0);
+      b1acc[b][i] = T(0);
+    }
 
   const int64_t r_begin = item.y, r_end = item.z;
   for (int64_t g0 = r_begin + 16 * NT * w; g0 < r_end; g0 += 16 * NT * NW) {
@@ -830,7 +846,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
         if (o < n_out) {
           const T so = valid[t] ? a.seeds[rc[t] * n_out + o] * dl_scale : T(0);
           if (o == q) blacc += so;
-          if (q == 0) sbuf[t][o * WRS + row] = so;
+          if (wl_valu) {
+#pragma unroll
+            for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+              for (int i = 0; i < 4; ++i) wlacc[mb][i] = fma(so, act[NH][t][mb][i], wlacc[mb][i]);
+          } else if (q == 0) {
+            sbuf[t][o * WRS + row] = so;
+          }
 #pragma unroll
           for (int mb = 0; mb < NB; ++mb) {
             const v4 wl = ld4(net.WL + o * HP + 16 * mb + 4 * q);
@@ -845,7 +868,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
         for (int i = 0; i < 4; ++i) {
           const T av = act[NH][t][mb][i];
           dl[t][mb][i] *= (T(1) - av * av);
-          abuf[t][(16 * mb + crow<T>(q, i)) * WRS + row] = av;
+          if (!wl_valu) abuf[t][(16 * mb + crow<T>(q, i)) * WRS + row] = av;
         }
     }
     wave_sync();
@@ -853,6 +876,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     // q takes row position 4q + kb in both operands
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
+      if (wl_valu) break;
       const int o = row - col_wl;
       const v4 sv = (o >= 0 && o < n_out)
                         ? *reinterpret_cast<const v4 *>(sbuf[t] + o * WRS + 4 * q)
@@ -919,6 +943,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
           }
     }
     // first layer: dW_1[f][c] = sum_rows delta_0[f] x_c, bias in column d_in
+    if (w1_valu) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const T dv = dl[t][mb][i];
+            b1acc[mb][i] += dv;
+            w1acc[mb][i][0] = fma(dv, x[t][0], w1acc[mb][i][0]);
+            if (MAXD > 1) w1acc[mb][i][1] = fma(dv, x[t][MAXD > 1 ? 1 : 0], w1acc[mb][i][1]);
+          }
+      continue;
+    }
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -962,10 +1000,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
               if (in < H) put(L.oWh(jj) + out * H + in, dwacc[jj][ob * NB + ib][i]);
             }
           const T sv = sacc[ob][i];
-          if (row < d_in) put(L.oW1() + out * d_in + row, sv);
-          else if (row == col_b1) put(L.ob1() + out, sv);
-          else if (row < col_wl + n_out) put(L.oWL() + (row - col_wl) * H + out, sv);
+          if (!w1_valu && row < d_in) put(L.oW1() + out * d_in + row, sv);
+          else if (!w1_valu && row == col_b1) put(L.ob1() + out, sv);
+          else if (!wl_valu && row < col_wl + n_out) put(L.oWL() + (row - col_wl) * H + out, sv);
         }
+      if (w1_valu) {
+#pragma unroll
+        for (int ob = 0; ob < NB; ++ob)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int out = 16 * ob + crow<T>(q, i);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+              const T v = rowsum16(w1acc[ob][i][c]);
+              if (row == 0 && out < H && c < d_in) put(L.oW1() + out * d_in + c, v);
+            }
+            const T vb = rowsum16(b1acc[ob][i]);
+            if (row == 0 && out < H) put(L.ob1() + out, vb);
+          }
+      }
+      if (wl_valu) {
+#pragma unroll
+        for (int ob = 0; ob < NB; ++ob)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const T v = rowsum16(wlacc[ob][i]);
+            const int out = 16 * ob + crow<T>(q, i);
+            if (row == 0 && out < H) put(L.oWL() + out, v);
+          }
+      }
 #pragma unroll
       for (int jj = 0; jj < NH; ++jj)
 #pragma unroll
