@@ -206,3 +206,37 @@ def test_fused_step_to_boundary_inside_interior():
     sol = Solution(grid, pre, u0.to(DEV).requires_grad_(), 'mat', None, 1, 10, derivative_points=2)
     assert sol.problem.bcs and any(len(bc.positions[0]) > 0 for bc in sol.problem.bcs)
     _compare_step_to(sol.problem, u0.reshape(-1), specialize=1)
+
+
+@pytest.mark.parametrize('epochs', [7, 8])
+def test_model_train_mat_with_generated_kernel(epochs):
+    """Model.train in 'mat' mode on a grid whose interior takes the
+    generated kernel by default: the fused steps alternate two grid buffers
+    (two captured graphs) and leave the result in the model's tensor, for an
+    even and an odd number of steps; identical to in-place steps of the
+    general kernel."""
+    import torch_de_solver_amd as tdx
+    _print(f"[mat model] wave 1301 x 1201, {epochs - 1} steps")
+    d, eq, b, dp, n_eq = problems.spec_wave(api, 1300, 1200, 2)
+    grid = d.build('mat')
+    u0 = problems.mat_init([1] + list(grid.shape[1:]), seed=31, std=0.05).to(DEV)
+    u_ref = u0.clone()
+    model = tdx.Model(u0.clone(), d, eq, b)
+    model.verbose = False
+    model.compile('mat', lambda_operator=1, lambda_bound=10)
+    assert model.solution_cls.engine.specialized_points >= (1 << 20)
+    model.train(tdx.Optimizer('Adam', {'lr': 1e-3}), epochs)
+    assert model.steps_done == epochs - 1
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict)).set_strategy('mat')
+    sol = Solution(grid, pre, u_ref.clone().requires_grad_(), 'mat', None, 1, 10)
+    gen = MatEngine(sol.problem, DEV, exact_div=1, specialize=-1)
+    w = u_ref.reshape(-1).contiguous()
+    m, v = torch.zeros_like(w), torch.zeros_like(w)
+    step = torch.zeros(1, dtype=torch.int64, device=DEV)
+    loss = torch.zeros(2, dtype=w.dtype, device=DEV)
+    for _ in range(epochs - 1):
+        gen.train_step(w, m, v, step, 1e-3, 0.9, 0.999, 1e-8, loss)
+    torch.cuda.synchronize()
+    out = model.net.detach().reshape(-1)
+    _print(f"  max diff {float((out - w).abs().max()):.3e}")
+    np.testing.assert_array_equal(out.cpu().numpy(), w.cpu().numpy())
