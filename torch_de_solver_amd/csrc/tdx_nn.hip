@@ -1085,7 +1085,7 @@ __device__ __forceinline__ void mma_layer_fm(const T *img, int lane, const T *sr
 }
 
 __host__ __device__ inline size_t block_lds_words(int HP, int NH) {
-  return (size_t)(NH + 2) * HP * CTRS + (size_t)(MAXD + MAXO) * CTRS + 16;
+  return (size_t)(NH + 1) * HP * CTRS + (size_t)(MAXD + MAXO) * CTRS + 16;
 }
 
 template <typename T, int NB, int NH>
@@ -1096,9 +1096,10 @@ __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
   typedef typename V4<T>::type v4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
-  T *actb = smem;                          // [NH+1][HP][CTRS] activations
-  T *dbuf = actb + (NH + 1) * HP * CTRS;   // [HP][CTRS] deltas of one layer
-  T *xbuf = dbuf + HP * CTRS;              // [MAXD][CTRS] coordinates
+  // [NH+1][HP][CTRS] activations; the delta of layer j replaces layer j's
+  // activations once their last readers are done (dW_{j+1} and delta_j)
+  T *actb = smem;
+  T *xbuf = actb + (NH + 1) * HP * CTRS;   // [MAXD][CTRS] coordinates
   T *sbuf = xbuf + MAXD * CTRS;            // [MAXO][CTRS] seeds
   T *red = sbuf + MAXO * CTRS;             // 4 wavefront partials of the output bias
   const int4 item = a.k2_items[blockIdx.x];
@@ -1222,11 +1223,13 @@ __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
         }
       }
     }
+    __syncthreads();  // the last activations' readers are done
 
 #pragma unroll
     for (int j = NH; j >= 1; --j) {
       const int jj = j - 1;
       const T *ain = layer_buf(j - 1);
+      T *dbuf = layer_buf(j);  // delta_j over act_j (read for the last time above)
       store_rows(dbuf, dl);
       __syncthreads();
       // owned blocks of dW_jj over the 64 rows; hidden bias from the delta
@@ -1268,7 +1271,8 @@ __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
       __syncthreads();
     }
     // first layer: columns c < d_in of the small-term tiles (B = coordinate
-    // c) and column d_in (B = 1)
+    // c) and column d_in (B = 1); delta_0 over act_0
+    T *dbuf = layer_buf(0);
     store_rows(dbuf, dl);
     __syncthreads();
 #pragma unroll
