@@ -364,8 +364,14 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
   }
   __syncthreads();
 
+  // residual, loss and seeds of the workgroup's 64 entries: wavefront 0,
+  // one entry per lane (the other wavefronts' SIMDs are free meanwhile)
   T lsum = T(0), lnsum = T(0);
-  if (q == 0) {
+  if (w == 0) {
+    const int e = item.y + lane;
+    const bool valid = e < n_entries;
+    T *u_lds = smem + lane * SO;
+    T *sd_lds = smem + K1_WAVES * 16 * SO + lane * SO;
     for (int k = 0; k < SO; ++k) sd_lds[k] = T(0);
     if (valid) {
       for (int eq = 0; eq < n_eq; ++eq) {
