@@ -224,7 +224,8 @@ typedef struct tdx_mat_desc {
   int tile_rows; /* output rows per tile workgroup (2-D grids); 0: chosen by the plan */
   int specialize; /* float 2-D grids: interior points by a kernel generated and
                      compiled (hipRTC) for this program at plan creation.
-                     0: when the interior holds at least 2^20 points;
+                     0: when the interior holds at least 2^20 points and the
+                     generated kernels need no scratch;
                      1: whenever the grid has an interior (errors are
                      reported); -1: never */
   int band_rows;  /* rows per wavefront of the specialized kernel; 0: chosen

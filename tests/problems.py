@@ -291,3 +291,36 @@ SPECIALIZED_CASES = {
     'mixed_dp3': (spec_mixed, 199, 240, 3, 1.0),
     'system_dp2': (spec_system, 230, 250, 2, 0.0),
 }
+
+
+def schrodinger_mat(api, nx=40, nt=40, dtype='float32'):
+    """The reference's example_schrodinger_matrix.py problem ('mat' mode,
+    derivative_points=5): i h_t + h_xx / 2 + |h|^2 h = 0 as two real
+    equations (u = Re h, v = Im h) on x in [-5, 5], t in [0, pi/2];
+    h(x, 0) = 2 / cosh(x); u, v, u_x and v_x periodic in x."""
+    d = api.Domain()
+    d.variable('x', [-5, 5], nx, dtype=dtype)
+    d.variable('t', [0, np.pi / 2], nt, dtype=dtype)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [-5, 5], 't': 0}, value=2 / torch.cosh(x), var=0)
+    b.dirichlet({'x': [-5, 5], 't': 0}, value=0, var=1)
+    per = [{'x': -5, 't': [0, np.pi / 2]}, {'x': 5, 't': [0, np.pi / 2]}]
+    b.periodic(per, var=0)
+    b.periodic(per, var=1)
+    b.periodic(per, operator={'du/dx': {'coeff': 1, 'du/dx': [0], 'pow': 1, 'var': 0}}, var=0)
+    b.periodic(per, operator={'dv/dx': {'coeff': 1, 'dv/dx': [0], 'pow': 1, 'var': 1}}, var=1)
+    eq = api.Equation()
+    eq.add({
+        'du/dt': {'coeff': 1, 'term': [1], 'pow': 1, 'var': 0},
+        '1/2*d2v/dx2': {'coeff': 1 / 2, 'term': [0, 0], 'pow': 1, 'var': 1},
+        'v * u**2': {'coeff': 1, 'term': [[None], [None]], 'pow': [1, 2], 'var': [1, 0]},
+        'v**3': {'coeff': 1, 'term': [None], 'pow': 3, 'var': 1},
+    })
+    eq.add({
+        'dv/dt': {'coeff': 1, 'term': [1], 'pow': 1, 'var': 1},
+        '-1/2*d2u/dx2': {'coeff': -1 / 2, 'term': [0, 0], 'pow': 1, 'var': 0},
+        '-u * v ** 2': {'coeff': -1, 'term': [[None], [None]], 'pow': [1, 2], 'var': [0, 1]},
+        '-u ** 3': {'coeff': -1, 'term': [None], 'pow': 3, 'var': 0},
+    })
+    return d, eq, b, dict(mode='mat', lambda_operator=1, lambda_bound=1, derivative_points=5)

@@ -253,3 +253,74 @@ def test_nn_causal_fused_step_equals_autograd_adam():
     torch.cuda.synchronize()
     pa = torch.cat([p.detach().reshape(-1) for p in nets[0].parameters()]).cpu().numpy()
     np.testing.assert_allclose(flat.cpu().numpy(), pa, rtol=1e-9, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# a reference example end to end: example_schrodinger_matrix.py ('mat')
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
+def test_schrodinger_matrix_example_against_oracle(dtype):
+    """Two coupled equations, 5-point stencils, plain and operator periodic
+    conditions, Dirichlet conditions per variable, on the example's grid."""
+    _print(f"[schrodinger mat] {dtype}")
+    tdt = torch.float64 if dtype == 'float64' else torch.float32
+    d, eq, b, kw = problems.schrodinger_mat(api, dtype=dtype)
+    orc = O.MatOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst],
+                      derivative_points=5)
+    u0 = problems.mat_init([2, 41, 41], seed=17, dtype=tdt, std=0.3)
+    uo = u0.clone().requires_grad_()
+    lo, no, op, bv, tv, keys, lens = orc.evaluate(uo, 1, 1)
+    lo.backward()
+    grid = d.build('mat')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict)).set_strategy('mat')
+    un = u0.to(DEV).requires_grad_()
+    sol = Solution(grid, pre, un, 'mat', None, 1, 1, derivative_points=5)
+    ln, nn_ = sol.evaluate()
+    ln.backward()
+    assert sol.bval_keys == keys == ['dirichlet', 'periodic']
+    gn, go = un.grad.cpu().numpy(), uo.grad.numpy()
+    opn = sol.op.cpu().numpy()
+    _print(f"  loss {float(ln.detach()):.10g} (oracle {float(lo.detach()):.10g}); grad max diff "
+           f"{np.abs(gn - go).max():.3e} of {np.abs(go).max():.3e}")
+    if dtype == 'float64':
+        np.testing.assert_allclose(opn, op.detach().numpy(), rtol=1e-12,
+                                   atol=1e-12 * float(op.abs().max()))
+        np.testing.assert_allclose(sol.bval.cpu().numpy(), bv.detach().numpy(), rtol=1e-12,
+                                   atol=1e-13)
+        np.testing.assert_allclose(float(ln.detach()), float(lo.detach()), rtol=1e-10)
+        np.testing.assert_allclose(gn, go, rtol=1e-8, atol=1e-10 * np.abs(go).max())
+    else:
+        np.testing.assert_array_equal(opn, op.detach().numpy())
+        np.testing.assert_array_equal(sol.bval.cpu().numpy(), bv.detach().numpy())
+        np.testing.assert_allclose(float(ln.detach()), float(lo.detach()), rtol=1e-6)
+        np.testing.assert_allclose(gn, go, rtol=1e-5, atol=1e-6 * np.abs(go).max())
+
+
+def test_schrodinger_matrix_generated_kernel():
+    """The same program on a 201 x 161 grid through the kernel generated for
+    it (dp = 5: 16-row rings) against the general kernel, bit for bit."""
+    from torch_de_solver_amd.mat_engine import MatEngine
+    _print("[schrodinger mat] generated kernel, 201 x 161")
+    d, eq, b, kw = problems.schrodinger_mat(api, nx=200, nt=160)
+    grid = d.build('mat')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict)).set_strategy('mat')
+    u0 = problems.mat_init([2, 201, 161], seed=18, std=0.3)
+    sol = Solution(grid, pre, u0.to(DEV).requires_grad_(), 'mat', None, 1, 1, derivative_points=5)
+    gen = MatEngine(sol.problem, DEV, specialize=-1)
+    spc = MatEngine(sol.problem, DEV, specialize=1)
+    assert spc.specialized_points > 0
+    u = u0.to(DEV).reshape(-1).contiguous()
+    one = torch.ones(1, device=DEV)
+    out = []
+    for eng in (gen, spc):
+        loss, op, bval = eng.fields(u)
+        g = torch.empty_like(u)
+        eng.backward(u, one, g)
+        torch.cuda.synchronize()
+        out.append((loss.cpu().numpy(), op.cpu().numpy(), bval.cpu().numpy(), g.cpu().numpy()))
+    _print(f"  specialized points {spc.specialized_points}")
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    np.testing.assert_array_equal(out[1][2], out[0][2])
+    np.testing.assert_array_equal(out[1][3], out[0][3])
+    np.testing.assert_allclose(out[1][0], out[0][0], rtol=1e-6)

@@ -68,7 +68,7 @@ struct MatArgs {
   int n_fs;
   int R0, R1;         // reach per axis
   int T0, T1;         // output tile
-  int E0, E1;         // tile with its halo of 2R per side; E1 is 64 * 2^k
+  int E0, E1;         // tile with its halo of 2R per side; E1 is 16, 32 or 64 * 2^k
   int tpr_log;        // log2 of the threads per tile row, min(E1, 256)
   int tiles1;         // tiles along axis 1
   const int32_t *tile_list;  // tiles of this launch; null: every tile
@@ -256,13 +256,18 @@ __device__ __forceinline__ void for_box(int E1, int tpr_log, int lo0, int hi0, i
   const int tpr = 1 << tpr_log;
   const int c0 = threadIdx.x & (tpr - 1);
   const int r0 = lo0 + (int)(threadIdx.x >> tpr_log), rstep = MAT_THREADS >> tpr_log;
-  if (E1 == tpr) {
+  if (E1 == tpr && tpr >= 64) {
     // one column per thread: the column test is made once, and since a row
-    // spans whole wavefronts (E1 >= 64) every lane of a wavefront has the
-    // same rows, which readfirstlane tells the compiler (a scalar loop)
+    // spans whole wavefronts every lane of a wavefront has the same rows,
+    // which readfirstlane tells the compiler (a scalar loop)
     const int r0u = __builtin_amdgcn_readfirstlane(r0);
     if (c0 >= lo1 && c0 < hi1)
       for (int i0 = r0u; i0 < hi0; i0 += rstep) fn(i0, c0);
+  } else if (E1 == tpr) {
+    // rows narrower than a wavefront: one column per thread, several rows
+    // per wavefront
+    if (c0 >= lo1 && c0 < hi1)
+      for (int i0 = r0; i0 < hi0; i0 += rstep) fn(i0, c0);
   } else {
     for (int i0 = r0; i0 < hi0; i0 += rstep)
       for (int i1 = c0; i1 < E1; i1 += tpr)
@@ -274,9 +279,9 @@ __device__ __forceinline__ void for_box(int E1, int tpr_log, int lo0, int hi0, i
 // tiled residual + loss + adjoint
 // ---------------------------------------------------------------------------
 // LDS: U[n_eq][S] | F[n_fs][S] (factor values, then seeds) | A[S] | B[S] |
-// G[n_eq][S] (adjoint) | 2 x 256 reduction words | coefficient table |
-// program reals | program ints, S = E0 * E1.  DP: number of derivative
-// points, 0 = any (read from dp).
+// 2 x 256 reduction words | coefficient table | program reals | program ints,
+// S = E0 * E1.  The adjoint G[n_eq][S] takes U's place: u is last read by the
+// forward chains.  DP: number of derivative points, 0 = any (read from dp).
 // INT: the tile with its halo lies inside the grid, so every point exists and
 // every stencil row the tile evaluates or transposes is an interior row.
 template <typename T, int DP, int DIV, bool ADJ, bool INT>
@@ -291,8 +296,8 @@ __device__ __forceinline__ void mat_tile_body(const MatArgs<T> &a, T *smem, int 
   T *F = U + a.n_eq * S;
   T *A = F + a.n_fs * S;
   T *B = A + S;
-  T *G = B + S;
-  T *red = G + (ADJ ? a.n_eq * S : 0);
+  T *G = U;
+  T *red = B + S;
   const int tid = threadIdx.x;
   auto in_grid = [&](int g0, int g1) {
     return INT || (g0 >= 0 && g0 < n0 && g1 >= 0 && g1 < n1);
@@ -341,7 +346,6 @@ __device__ __forceinline__ void mat_tile_body(const MatArgs<T> &a, T *smem, int 
       T val = T(0);
       if (in_grid(g0, g1)) val = a.u[v * N + (int64_t)g0 * n1 + g1];
       U[v * S + i0 * E1 + i1] = val;
-      if (ADJ) G[v * S + i0 * E1 + i1] = T(0);
     });
   __syncthreads();
 
@@ -392,6 +396,11 @@ __device__ __forceinline__ void mat_tile_body(const MatArgs<T> &a, T *smem, int 
     }
   }
   __syncthreads();
+  // the adjoint overwrites u (read by the forward chains only); the barrier
+  // before the transposed chains orders these stores before their sums
+  if (ADJ)
+    for (int v = 0; v < a.n_eq; ++v)
+      for_box(E1, tl, 0, E0, 0, E1, [&](int i0, int i1) { G[v * S + i0 * E1 + i1] = T(0); });
 
   // residual, loss and seeds on the tile plus R; each point is read and
   // written by one thread, so a term's seeds overwrite its factor values
@@ -1210,6 +1219,14 @@ int mat_specialize(tdx_mat_plan *p, const tdx_mat_desc *d) {
   }
   const int rc = mat_jit_compile(src, &p->jit);
   if (rc) return required ? rc : TDX_OK;
+  // a generated kernel that needs scratch (register spills) is not taken
+  // unless asked for: the general kernel serves such programs better
+  auto spills = [](hipFunction_t f) {
+    int bytes = 0;
+    return hipFuncGetAttribute(&bytes, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f) == hipSuccess &&
+           bytes > 0;
+  };
+  if (!required && (spills(p->jit.fn_adj) || spills(p->jit.fn_step))) return TDX_OK;
   // geometry: a wavefront owns 64 - 4 R1 columns of a band of rows
   p->jr_lo = ta0 * T0;
   p->jr_hi = tb0 * T0;
@@ -1449,41 +1466,50 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   // wide (one coalesced wavefront access per 64), at least 32 of them
   // output; 1-D grids use one row of 1024.  The number of rows is the
   // largest that keeps the tile within about 48 KB of LDS (several
-  // workgroups per CU), with at least 8 output rows.
-  const size_t words = d->n_eq + p->n_fs + 2 + d->n_eq;  // U, F, A, B, G per point
-  auto lds_for = [&](int E0, int E1, bool adj) {
-    return ((size_t)E0 * E1 * (words - (adj ? 0 : d->n_eq)) + 2 * MAT_THREADS + MCOEF_N +
-            prog_r.size()) * ts + prog_i.size() * 4;
+  // workgroups per CU), with at least 8 output rows.  Programs with long
+  // reach and many factors, whose smallest such tile exceeds the 160 KB of
+  // a CU, take rows of 32 or 16 values instead (at least 8 output columns).
+  const size_t words = d->n_eq + p->n_fs + 2;  // U (then G), F, A, B per point
+  const size_t fixed = (2 * MAT_THREADS + MCOEF_N + prog_r.size()) * ts + prog_i.size() * 4;
+  auto lds_for = [&](int E0, int E1) { return (size_t)E0 * E1 * words * ts + fixed; };
+  auto set_tpr = [&]() {
+    p->tpr_log = 0;
+    while ((1 << (p->tpr_log + 1)) <= std::min(p->E1, MAT_THREADS)) ++p->tpr_log;
+    return MAT_THREADS >> p->tpr_log;  // tile rows per sweep of the workgroup
   };
+  auto min_rows = [&](int rps) { return (4 * p->R0 + 8 + rps - 1) / rps * rps; };
   if (d->n0 == 1) {
     p->E0 = 1;
     p->E1 = 1024;
     if (p->E1 - 4 * p->R1 < 256) p->E1 = 2048;
+    set_tpr();
   } else {
     p->E1 = 64;
     while (p->E1 - 4 * p->R1 < 32) p->E1 *= 2;
-  }
-  p->tpr_log = 0;
-  while ((1 << (p->tpr_log + 1)) <= std::min(p->E1, MAT_THREADS)) ++p->tpr_log;
-  const int rps = MAT_THREADS >> p->tpr_log;
-  if (d->n0 > 1 && d->tile_rows > 0) {
-    // requested output rows, rounded up so the tile rows fill whole sweeps
-    p->E0 = (4 * p->R0 + d->tile_rows + rps - 1) / rps * rps;
-  } else if (d->n0 > 1) {
-    const size_t fixed = (2 * MAT_THREADS + MCOEF_N + prog_r.size()) * ts + prog_i.size() * 4;
-    int e0 = (int)((48 * 1024 - std::min<size_t>(fixed, 32 * 1024)) / ((size_t)p->E1 * words * ts));
-    e0 = std::min(e0, d->n0 + 4 * p->R0);
-    e0 = e0 / rps * rps;
-    if (e0 - 4 * p->R0 < 8) e0 = (4 * p->R0 + 8 + rps - 1) / rps * rps;
-    p->E0 = e0;
+    int rps = set_tpr();
+    if (d->tile_rows > 0) {
+      // requested output rows, rounded up so the tile rows fill whole sweeps
+      p->E0 = (4 * p->R0 + d->tile_rows + rps - 1) / rps * rps;
+    } else {
+      int e0 = (int)((48 * 1024 - std::min<size_t>(fixed, 32 * 1024)) / ((size_t)p->E1 * words * ts));
+      e0 = std::min(e0, d->n0 + 4 * p->R0);
+      e0 = e0 / rps * rps;
+      if (e0 - 4 * p->R0 < 8) e0 = min_rows(rps);
+      p->E0 = e0;
+      while (lds_for(p->E0, p->E1) > 160 * 1024 && p->E1 > 16 && p->E1 / 2 - 4 * p->R1 >= 8) {
+        p->E1 /= 2;
+        rps = set_tpr();
+        p->E0 = min_rows(rps);
+      }
+    }
   }
   p->T0 = p->E0 - 4 * p->R0;
   p->T1 = p->E1 - 4 * p->R1;
-  if (p->T0 < 1 || p->T1 < 1 || lds_for(p->E0, p->E1, true) > 160 * 1024)
+  if (p->T0 < 1 || p->T1 < 1 || lds_for(p->E0, p->E1) > 160 * 1024)
     return fail("stencil reach too large for LDS");
   p->div_mode = plan_div_mode(d);
-  p->tile_lds = lds_for(p->E0, p->E1, true);
-  p->tile_lds_fwd = lds_for(p->E0, p->E1, false);
+  p->tile_lds = lds_for(p->E0, p->E1);
+  p->tile_lds_fwd = p->tile_lds;
   p->tiles0 = (d->n0 + p->T0 - 1) / p->T0;
   p->tiles1 = (d->n1 + p->T1 - 1) / p->T1;
   p->n_tile_wg = p->tiles0 * p->tiles1;
