@@ -158,6 +158,22 @@ def test_unsupported_features_raise():
                                                  'bop': None, 'bval': bc[0]['bval'], 'var': 0}], 1)
 
 
+def test_mini_batching_limits():
+    """'mat' mini-batching and batching with a causal loss raise before any
+    device work; a batch larger than the grid turns batching off."""
+    import torch_de_solver_amd as tdx
+    d, eq, b, kw = problems.wave_mat(api, n=15)
+    u = problems.mat_init([1, 16, 16])
+    m = tdx.Model(u, d, eq, b, batch_size=1)
+    with pytest.raises(NotImplementedError):
+        m.compile('mat', lambda_operator=1, lambda_bound=100)
+    d, eq, b, kw = problems.burgers_nn(api)
+    net = problems.make_mlp([2, 8, 1])
+    m = tdx.Model(net, d, eq, b, batch_size=16)
+    with pytest.raises(NotImplementedError):
+        m.compile('NN', lambda_operator=1, lambda_bound=10, tol=0.1)
+
+
 def test_native_forward_requires_device_tensors():
     """The product path has no CPU fallback: without a HIP device the
     engines refuse to build."""

@@ -349,6 +349,35 @@ def test_model_api_nn_fused_and_torch_optimizer_agree():
     np.testing.assert_allclose(res[0].numpy(), res[1].numpy(), rtol=1e-9, atol=1e-12)
 
 
+def test_model_api_nn_mini_batching():
+    """batch_size in 'NN' mode: one full-operator Adam step per DataLoader
+    batch (the reference's 'NN' derivative ignores the batch's points), so
+    3 epochs of 3 batches reach the parameters of 9 unbatched steps, on the
+    fused path and through torch's Adam."""
+    def run(batch_size, epochs, fused):
+        d, eq, b, kw = problems.burgers_nn(api, dtype='float64')
+        net = problems.make_mlp([2, 12, 12, 12, 1], dtype=torch.float64)
+        model = tdx.Model(net, d, eq, b, batch_size=batch_size)
+        model.verbose = False
+        model.compile('NN', lambda_operator=1, lambda_bound=10)
+        n = len(d.build('NN'))
+        model.train(tdx.Optimizer('Adam', {'lr': 1e-3}), epochs,
+                    callbacks=None if fused else [object()])
+        return model, n, torch.cat([p.detach().reshape(-1) for p in net.parameters()]).cpu()
+
+    _, n, _ = run(None, 1, True)
+    bs = -(-n // 3)
+    print(f"[case] burgers_nn float64, N={n}, batch_size={bs}", flush=True)
+    mb, _, pb = run(bs, 4, True)
+    assert mb.n_batches == 3
+    _, _, pu = run(None, 10, True)
+    np.testing.assert_array_equal(pb.numpy(), pu.numpy())
+    _, _, pt = run(bs, 4, False)
+    np.testing.assert_allclose(pt.numpy(), pu.numpy(), rtol=1e-9, atol=1e-12)
+    mo, _, _ = run(n + 1, 2, True)  # larger than the grid: batching off
+    assert mo.batch_size is None and mo.n_batches == 1
+
+
 def test_model_api_mat_trains():
     d, eq, b, kw = problems.wave_mat(api, n=31)
     u = problems.mat_init([1, 32, 32]).to(DEV)

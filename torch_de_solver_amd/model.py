@@ -2,7 +2,15 @@
 
 ``compile`` lowers the problem once to native tables.  ``train`` keeps the
 reference's loop semantics: one initial evaluation, then ``epochs - 1``
-optimizer steps (tedeous/model.py:154-188), callbacks around each epoch.
+epochs (tedeous/model.py:154-188), callbacks around each epoch.
+
+Mini-batching ('NN' mode, ``batch_size``): the reference draws batches of
+the sorted grid with a DataLoader and takes one optimizer step per batch
+(tedeous/model.py:178-183), but its 'NN' derivative ignores the batch's
+points (tedeous/derivative.py:30-58 never reads ``grid_points``), so every
+step evaluates the full operator.  An epoch here is therefore
+``ceil(N / batch_size)`` full steps, and a ``batch_size`` above the number
+of grid points turns batching off (tedeous/model.py:107-109).
 
 When the optimizer is plain Adam with no scheduler and no callbacks, the
 whole step (forward, adjoint, Adam update) runs as one native call per step
@@ -46,6 +54,10 @@ class Model:
         self.removed_domains = removed_domains
         grid = self.domain.build(mode=mode, removed_domains=removed_domains)
         dtype = grid.dtype
+        if self.batch_size is not None and len(grid) < self.batch_size:
+            self.batch_size = None
+        # one optimizer step per DataLoader batch of the N sorted grid points
+        self.n_batches = 1 if self.batch_size is None else -(-len(grid) // int(self.batch_size))
         if isinstance(self.net, torch.nn.Module):
             self.net.to(dtype)
             if torch.cuda.is_available():
@@ -86,9 +98,10 @@ class Model:
         while self.t < epochs and self.stop_training is False:
             cbs.on_epoch_begin()
             self.optimizer.zero_grad()
-            self.optimizer.step(closure)
-            if optimizer.gamma is not None and self.t % optimizer.decay_every == 0:
-                optimizer.scheduler.step()
+            for _ in range(self.n_batches):
+                self.optimizer.step(closure)
+                if optimizer.gamma is not None and self.t % optimizer.decay_every == 0:
+                    optimizer.scheduler.step()
             cbs.on_epoch_end()
             self.t += 1
         cbs.on_train_end()
@@ -117,7 +130,7 @@ class Model:
         if self.verbose:
             print('[{}] initial (min) loss is {}'.format(datetime.datetime.now(),
                                                          self.min_loss.item()))
-        n_steps = max(epochs - 1, 0)
+        n_steps = max(epochs - 1, 0) * self.n_batches
         self.loss_history_device = torch.zeros(0)
 
         # 'mat': the step reads one grid buffer and writes the other (the

@@ -61,8 +61,13 @@ class Solution:
             raise NotImplementedError("weak-form losses are not served by this engine")
         if tol and mode != 'NN':
             raise NotImplementedError("causal losses (tol != 0) are served in 'NN' mode only")
-        if batch_size is not None:
-            raise NotImplementedError("mini-batching is not served by this engine")
+        if batch_size is not None and mode != 'NN':
+            raise NotImplementedError("mini-batching is served in 'NN' mode only")
+        if batch_size is not None and tol:
+            # the reference's causal loss re-derives n_t from the batch
+            # residual with the 'NN' point-type lookup and fails
+            # (tedeous/solution.py:54, tedeous/losses.py:160-161)
+            raise NotImplementedError("mini-batching with a causal loss is not served")
         if mode not in ('NN', 'mat'):
             raise NotImplementedError(f"mode {mode!r} is not served by this engine")
         self.grid = grid
