@@ -280,6 +280,94 @@ int tdx_mat_train_step_to(const tdx_mat_plan *plan, const void *u_in,
                           tdx_stream_t stream);
 
 /* ===================================================================== */
+/* 'autograd' mode                                                        */
+/* ===================================================================== */
+/*
+ * The same tanh MLP and parameter layout as 'NN' mode (one instance), with
+ * derivatives taken exactly instead of by finite differences.  The
+ * reference differentiates the network with nested
+ * torch.autograd.grad(create_graph=True) calls (Derivative_autograd,
+ * tedeous/derivative.py:61-132); here every point carries a jet: the value
+ * and each partial derivative (a multiset of input axes, order <= 4) the
+ * program reads, pushed through the layers in one forward pass (linear
+ * maps act on each jet component; tanh combines them by Faa di Bruno's
+ * formula over the set partitions of each multiset).  The adjoint runs the
+ * transposed maps back.
+ *
+ * Jets: jet 0 is the value.  ints[jet_i + 2j] = order of jet j,
+ * ints[jet_i + 2j + 1] = its axis when the order is 1, else -1.  The
+ * partitions of jet j are part records ints[part_i + TDX_AD_PART_INTS * k]
+ * for k in [ints[pptr_i + j], ints[pptr_i + j + 1]): (number of blocks m,
+ * jet of block 0, ..., block m-1; -1 pads), every block a jet of the list.
+ *
+ * Points: points[n_points][d_in].  Group g (TDX_AD_GROUP_INTS at
+ * ints[groups_i + TDX_AD_GROUP_INTS * g]): (n_entries, n_sets, set_i, n_eq,
+ * eq_i); entry e of set s is point ints[set_i + s] + e.  Every point
+ * belongs to exactly one (group, set, entry).
+ * Equation (TDX_AD_EQ_INTS): (n_terms, term_i, lam_r, denom_r, target_d,
+ * resid_off): value = in-order sum of terms at the entry; loss term
+ * reals[lam_r] / reals[denom_r] * (value - target)^2 with target
+ * data[target_d + e] (0 when target_d < 0); the value of entry e goes to
+ * resid[resid_off + e].
+ * Term (TDX_AD_TERM_INTS): (n_factors, factor_i, coeff_kind, coeff_ref):
+ * coefficient reals[coeff_ref] (kind 0) or data[coeff_ref + e] (kind 1)
+ * times the product of its factors.
+ * Factor (TDX_AD_FACTOR_INTS): (n_st, st_i, var, pow_kind, pow_ref):
+ * (sum over its stencil entries of w * jet component) ** pow, pow the
+ * integer pow_ref (kind 0) or reals[pow_ref] (kind 1).
+ * Stencil entry (TDX_AD_STENCIL_INTS): (set, jet, weight_r).
+ */
+#define TDX_AD_MAX_ORDER 4
+#define TDX_AD_PART_INTS 5
+#define TDX_AD_GROUP_INTS 5
+#define TDX_AD_EQ_INTS 6
+#define TDX_AD_TERM_INTS 4
+#define TDX_AD_FACTOR_INTS 5
+#define TDX_AD_STENCIL_INTS 3
+
+typedef struct tdx_ad_desc {
+  int dtype;
+  int d_in, n_out, width, n_layers;
+  int n_jets, jet_i, pptr_i, part_i;
+  int64_t n_points;
+  const void *points;
+  int n_groups, groups_i;
+  int64_t n_ints;
+  const int32_t *ints;
+  int64_t n_reals;
+  const double *reals;
+  int64_t n_data;
+  const void *data;
+  int64_t n_resid;
+} tdx_ad_desc;
+
+typedef struct tdx_ad_plan tdx_ad_plan;
+
+/* Upload the tables.  Replaces Solution.__init__ (tedeous/solution.py:23-88)
+ * with Equation_autograd (tedeous/input_preprocessing.py:411-505). */
+int tdx_ad_plan_create(const tdx_ad_desc *desc, tdx_ad_plan **out);
+void tdx_ad_plan_destroy(tdx_ad_plan *plan);
+int64_t tdx_ad_param_count(const tdx_ad_plan *plan);
+size_t tdx_ad_workspace_bytes(const tdx_ad_plan *plan);
+/* Forward: loss[0] (lambda-weighted), loss[1] (normalized); resid (may be
+ * NULL) receives every equation's value per entry.  Replaces
+ * Solution.evaluate() (tedeous/solution.py:129-168) in 'autograd' mode. */
+int tdx_ad_forward(const tdx_ad_plan *plan, const void *params, void *loss,
+                   void *resid, void *workspace, tdx_stream_t stream);
+/* dparams = dloss[0] * dloss/dparams after a tdx_ad_forward with the same
+ * params and workspace.  Replaces loss.backward()
+ * (tedeous/optimizers/closure.py:60). */
+int tdx_ad_backward(const tdx_ad_plan *plan, const void *params,
+                    const void *dloss, void *dparams, void *workspace,
+                    tdx_stream_t stream);
+/* Fused step: forward, adjoint and torch.optim.Adam update (see
+ * tdx_nn_train_step for step and loss). */
+int tdx_ad_train_step(const tdx_ad_plan *plan, void *params, void *exp_avg,
+                      void *exp_avg_sq, int64_t *step, double lr, double beta1,
+                      double beta2, double eps, void *loss, void *workspace,
+                      tdx_stream_t stream);
+
+/* ===================================================================== */
 /* Optimizer                                                              */
 /* ===================================================================== */
 /* torch.optim.Adam update of n values (weight_decay 0, amsgrad off,

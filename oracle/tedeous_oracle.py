@@ -5,14 +5,14 @@ used by tests/, by ``__graft_entry__.smoke()`` and by the CPU-baseline leg of
 bench.py, always as the checker (never as the thing measured or shipped).
 
 It restates, in torch CPU ops (the arithmetic library the reference itself
-computes with), the reference's 'NN' and 'mat' evaluation path:
+computes with), the reference's 'NN', 'mat' and 'autograd' evaluation path:
 
 * grids and conditions        tedeous/data.py:41-107, 241-323
 * point typing (qhull)        tedeous/points_type.py:21-125
 * stencil schemes             tedeous/finite_diffs.py:9-268
 * operator/bc preparation     tedeous/input_preprocessing.py:51-81, 173-408,
                               511-594
-* derivatives                 tedeous/derivative.py:18-58, 135-323
+* derivatives                 tedeous/derivative.py:18-58, 61-132, 135-323
 * residual / bcs              tedeous/eval.py:55-87, 143-193, 283-461
 * loss                        tedeous/losses.py:84-135
 * training loop               tedeous/model.py:154-188,
@@ -570,6 +570,105 @@ class MatOracle:
     def evaluate(self, model, lam_op=1, lam_bnd=1):
         op = self.residual(model)
         b, t, keys, lens = self.boundary(model)
+        loss, norm = default_loss(op, b, t, lam_row(op, lam_op).to(op.dtype),
+                                  lam_row(b, lam_bnd).to(op.dtype))
+        return loss, norm, op, b, t, keys, lens
+
+
+# ----------------------------------------------------------------------------
+# 'autograd' mode
+# ----------------------------------------------------------------------------
+
+def ad_derivative(net, points: torch.Tensor, var: int, axis: List[int]) -> torch.Tensor:
+    """Derivative_autograd._nn_autograd (tedeous/derivative.py:73-98): the
+    network's output ``var`` summed over the points, differentiated along
+    each axis in turn with ``create_graph=True``; the last gradient's
+    column is the mixed partial at every point."""
+    points = points.detach().clone().requires_grad_(True)
+    fi = net(points)[:, var].sum(0)
+    grads = None
+    for ax in axis:
+        grads, = torch.autograd.grad(fi, points, create_graph=True)
+        fi = grads[:, ax].sum()
+    return grads[:, axis[-1]].reshape(-1, 1)
+
+
+def ad_term_value(net, term: dict, pts: torch.Tensor) -> torch.Tensor:
+    """Derivative_autograd.take_derivative (tedeous/derivative.py:100-132)
+    with the coefficient checks of tedeous/input_preprocessing.py:433-458."""
+    key = list(term.keys())[1]
+    c = term['coeff']
+    if callable(c) and not isinstance(c, torch.Tensor):
+        c = c(pts).reshape(-1, 1)
+    elif isinstance(c, torch.Tensor):
+        c = c.reshape(-1, 1)
+    der = 1.
+    for j, derivative in enumerate(term[key]):
+        if derivative == [None]:
+            d = net(pts)[:, term['var'][j]].reshape(-1, 1)
+        else:
+            d = ad_derivative(net, pts, term['var'][j], derivative)
+        p = term['pow'][j]
+        der = der * d ** p if isinstance(p, (int, float)) else p(der * d)
+    return c * der
+
+
+class ADOracle:
+    """Reference-semantics evaluation of an 'autograd' problem on the CPU:
+    the operator on every grid point (tedeous/eval.py:117-123), conditions
+    at their points (tedeous/eval.py:283-461, mode 'autograd')."""
+
+    def __init__(self, axes: Dict[str, torch.Tensor], operators: List[dict], conds: List[dict]):
+        self.grid = grid_points(list(axes.values()))
+        self.ops = [unify(op) for op in operators]
+        self.conds = build_conditions(conds, axes)
+
+    def _op(self, net, op, pts):
+        return apply_operator(lambda t: ad_term_value(net, t, pts), op).reshape(-1, 1)
+
+    def residual(self, net) -> torch.Tensor:
+        cols = [self._op(net, op, self.grid) for op in self.ops]
+        return cols[0] if len(cols) == 1 else torch.cat(cols, 1)
+
+    def boundary(self, net):
+        bval, true = {}, {}
+        for c in self.conds:
+            typ, bop = c['type'], c['bop']
+            if typ in ('dirichlet', 'data') and bop is None:
+                v = net(c['bnd'])[:, c['var']].reshape(-1, 1)
+            elif typ == 'periodic' and bop is None:
+                v = net(c['bnd'][0])[:, c['var']].reshape(-1, 1)
+                for b in c['bnd'][1:]:
+                    v = v - net(b)[:, c['var']].reshape(-1, 1)
+            elif typ == 'periodic':
+                # tedeous/eval.py:351-354
+                v = self._op(net, bop, c['bnd'][0])
+                for b in c['bnd'][1:]:
+                    v = v - self._op(net, bop, b)
+            elif typ in ('operator', 'data'):
+                v = self._op(net, bop, c['bnd'])
+            elif typ == 'robin':
+                # tedeous/eval.py:357-388: alpha * u + beta * (whole operator)
+                alpha, *betas = [bop[k]['coeff'] for k in bop.keys()]
+                v = alpha * net(c['bnd'])[:, c['var']].reshape(-1, 1)
+                dt = 0
+                for beta in betas:
+                    dt += beta * self._op(net, bop, c['bnd'])
+                v = v + dt
+            else:
+                raise NotImplementedError(typ)
+            k = typ
+            if k in bval:
+                bval[k] = torch.cat((bval[k], v.reshape(-1)))
+                true[k] = torch.cat((true[k], c['bval'].reshape(-1)))
+            else:
+                bval[k] = v.reshape(-1)
+                true[k] = c['bval'].reshape(-1)
+        return pad_matrix(bval, true)
+
+    def evaluate(self, net, lam_op=1, lam_bnd=1):
+        op = self.residual(net)
+        b, t, keys, lens = self.boundary(net)
         loss, norm = default_loss(op, b, t, lam_row(op, lam_op).to(op.dtype),
                                   lam_row(b, lam_bnd).to(op.dtype))
         return loss, norm, op, b, t, keys, lens

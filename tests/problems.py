@@ -324,3 +324,130 @@ def schrodinger_mat(api, nx=40, nt=40, dtype='float32'):
         '-u ** 3': {'coeff': -1, 'term': [None], 'pow': 3, 'var': 0},
     })
     return d, eq, b, dict(mode='mat', lambda_operator=1, lambda_bound=1, derivative_points=5)
+
+
+# ---------------------------------------------------------------------------
+# 'autograd' mode (exact derivatives of the network)
+# ---------------------------------------------------------------------------
+
+def burgers_ad(api, nx=12, nt=8, dtype='float64', mu=MU):
+    d, eq, b, kw = burgers_nn(api, nx=nx, nt=nt, dtype=dtype, mu=mu)
+    return d, eq, b, dict(mode='autograd', lambda_operator=1, lambda_bound=10)
+
+
+def kdv_ad(api, nx=12, nt=6, dtype='float64', L=20.0):
+    """Third order in x; plain periodic values and periodic u_x (a periodic
+    condition with an operator)."""
+    d = api.Domain()
+    d.variable('x', [0, L], nx, dtype=dtype)
+    d.variable('t', [0, 1], nt, dtype=dtype)
+    b = api.Conditions()
+    per = [{'x': 0, 't': [0, 1]}, {'x': L, 't': [0, 1]}]
+    b.periodic(per)
+    b.periodic(per, operator={'du/dx': {'coeff': 1, 'du/dx': [0], 'pow': 1, 'var': 0}})
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, L], 't': 0}, value=0.5 / torch.cosh(0.5 * (x - L / 2)) ** 2)
+    eq = api.Equation()
+    eq.add(kdv_eq())
+    return d, eq, b, dict(mode='autograd', lambda_operator=1, lambda_bound=100)
+
+
+def wave_ad(api, n=10, dtype='float64'):
+    """Second order in both axes, an operator condition u_t = 0 at t = 0
+    and a Robin condition 2 u + 0.5 u_x at x = 1."""
+    d = api.Domain()
+    d.variable('x', [0, 1], n, dtype=dtype)
+    d.variable('t', [0, 1], n, dtype=dtype)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=torch.sin(np.pi * x))
+    b.dirichlet({'x': 0, 't': [0, 1]}, value=0)
+    b.operator({'x': [0, 1], 't': 0}, operator={'du/dt': {'coeff': 1, 'du/dt': [1], 'pow': 1}},
+               value=0)
+    b.robin({'x': 1, 't': [0, 1]}, value=0.25,
+            operator={'u': {'coeff': 2.0, 'u': [None], 'pow': 1, 'var': 0},
+                      'du/dx': {'coeff': 0.5, 'du/dx': [0], 'pow': 1, 'var': 0}})
+    eq = api.Equation()
+    eq.add(wave_eq())
+    return d, eq, b, dict(mode='autograd', lambda_operator=1, lambda_bound=100)
+
+
+def mixed_ad(api, n=9, dtype='float64'):
+    """Mixed partials in either order, powers, a product of three factors, a
+    tensor coefficient and 'data' conditions with and without an operator."""
+    d = api.Domain()
+    d.variable('x', [0, 1], n, dtype=dtype)
+    d.variable('t', [0, 2], n, dtype=dtype)
+    grid = d.build('autograd')
+    cx = (1 + grid[:, 0] * grid[:, 1]).reshape(-1, 1)
+    b = api.Conditions()
+    b.dirichlet({'x': 0, 't': [0, 2]}, value=lambda g: torch.sin(g[:, 1]))
+    pts = torch.tensor([[0.25, 0.5], [0.5, 1.0], [0.75, 1.5]], dtype=grid.dtype)
+    b.data(pts, operator=None, value=torch.tensor([0.1, 0.2, 0.3], dtype=grid.dtype), var=0)
+    b.data(pts, value=torch.tensor([0.0, -0.1, 0.1], dtype=grid.dtype),
+           operator={'d2u/dxdt': {'coeff': 1, 'd2u/dxdt': [0, 1], 'pow': 1, 'var': 0}})
+    eq = api.Equation()
+    eq.add({
+        'd2u/dxdt': {'coeff': cx, 'd2u/dxdt': [0, 1], 'pow': 1, 'var': 0},
+        'd2u/dtdx^2': {'coeff': 0.5, 'd2u/dtdx': [1, 0], 'pow': 2, 'var': 0},
+        'u*ux*ut': {'coeff': -1, 'u*ux*ut': [[None], [0], [1]], 'pow': [1, 1, 2],
+                    'var': [0, 0, 0]},
+        'd3u/dx2dt': {'coeff': lambda g: g[:, 0:1] - 0.5, 'd3u/dx2dt': [0, 1, 0], 'pow': 1,
+                      'var': 0},
+    })
+    return d, eq, b, dict(mode='autograd', lambda_operator=2, lambda_bound=[10, 3])
+
+
+def system_ad(api, n=30, dtype='float64'):
+    """Two outputs on one axis: u' = v, v' = -u, with u(0) = 1, v(0) = 0."""
+    d = api.Domain()
+    d.variable('t', [0, 3], n, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'t': 0}, value=1.0, var=0)
+    b.dirichlet({'t': 0}, value=0.0, var=1)
+    eq = api.Equation()
+    eq.add({'du/dt': {'coeff': 1, 'du/dt': [0], 'pow': 1, 'var': 0},
+            '-v': {'coeff': -1, 'v': [None], 'pow': 1, 'var': 1}})
+    eq.add({'dv/dt': {'coeff': 1, 'dv/dt': [0], 'pow': 1, 'var': 1},
+            'u': {'coeff': 1, 'u': [None], 'pow': 1, 'var': 0}})
+    return d, eq, b, dict(mode='autograd', lambda_operator=[1, 2], lambda_bound=10)
+
+
+def beam_ad(api, n=20, dtype='float64'):
+    """Fourth order: u'''' = 1 on [0, 1], u(0) = u'(0) = 0, u''(1) = u'''(1) = 0."""
+    d = api.Domain()
+    d.variable('x', [0, 1], n, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'x': 0}, value=0.0)
+    b.operator({'x': 0}, operator={'du/dx': {'coeff': 1, 'du/dx': [0], 'pow': 1}}, value=0.0)
+    b.operator({'x': 1}, operator={'d2u/dx2': {'coeff': 1, 'd2u/dx2': [0, 0], 'pow': 1}},
+               value=0.0)
+    b.operator({'x': 1}, operator={'d3u/dx3': {'coeff': 1, 'd3u/dx3': [0, 0, 0], 'pow': 1}},
+               value=0.0)
+    eq = api.Equation()
+    eq.add({'d4u/dx4': {'coeff': 1, 'd4u/dx4': [0, 0, 0, 0], 'pow': 1, 'var': 0},
+            '-1': {'coeff': -1, 'u': [None], 'pow': 0, 'var': 0}})
+    return d, eq, b, dict(mode='autograd', lambda_operator=1, lambda_bound=10)
+
+
+def legendre_ad(api, n=20, dtype='float64'):
+    d = api.Domain()
+    d.variable('x', [-1, 1], n, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'x': -1}, value=-1)
+    b.dirichlet({'x': 1}, value=1)
+    eq = api.Equation()
+    eq.add(legendre_eq(mode='autograd'))
+    return d, eq, b, dict(mode='autograd', lambda_operator=1, lambda_bound=10)
+
+
+# builder, network layers
+AD_CASES = {
+    'burgers_ad': (burgers_ad, [2, 16, 16, 16, 1]),
+    'kdv_ad': (kdv_ad, [2, 20, 20, 1]),
+    'wave_ad': (wave_ad, [2, 12, 12, 1]),
+    'mixed_ad': (mixed_ad, [2, 24, 24, 1]),
+    'system_ad': (system_ad, [1, 16, 16, 2]),
+    'beam_ad': (beam_ad, [1, 10, 10, 10, 1]),
+    'legendre_ad': (legendre_ad, [1, 32, 32, 1]),
+}

@@ -290,3 +290,110 @@ def test_oracle_causal_loss_small_tol_is_default_loss():
     ld, nd = O.default_loss(op, bval, true, torch.ones(1, 1, dtype=torch.float64), lam_b)
     np.testing.assert_allclose(float(lc), float(ld), rtol=1e-12)
     np.testing.assert_allclose(float(nc), float(nd), rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# 'autograd' mode: the nested-autograd restatement pinned by closed forms
+# ---------------------------------------------------------------------------
+
+class _Analytic(torch.nn.Module):
+    """u0 = sin(2x) e^-t + x^3 t^2, u1 = cos(x t): a stand-in network whose
+    partial derivatives are known in closed form."""
+
+    def forward(self, p):
+        x, t = p[:, 0], p[:, 1]
+        return torch.stack([torch.sin(2 * x) * torch.exp(-t) + x ** 3 * t ** 2,
+                            torch.cos(x * t)], 1)
+
+
+def _u0_derivs(x, t):
+    s, c, e = np.sin(2 * x), np.cos(2 * x), np.exp(-t)
+    return {
+        (): s * e + x ** 3 * t ** 2,
+        (0,): 2 * c * e + 3 * x ** 2 * t ** 2,
+        (1,): -s * e + 2 * x ** 3 * t,
+        (0, 0): -4 * s * e + 6 * x * t ** 2,
+        (0, 1): -2 * c * e + 6 * x ** 2 * t,
+        (1, 1): s * e + 2 * x ** 3,
+        (0, 0, 0): -8 * c * e + 6 * t ** 2,
+        (0, 0, 1): 4 * s * e + 12 * x * t,
+        (0, 0, 0, 0): 16 * s * e,
+    }
+
+
+def test_ad_oracle_derivatives_closed_form():
+    """Every mixed partial up to order 4, in any axis order, and a second
+    output: the nested autograd chain equals the closed form."""
+    rng = np.random.default_rng(3)
+    pts = torch.from_numpy(rng.uniform(-1, 1, size=(17, 2)))
+    x, t = pts[:, 0].numpy(), pts[:, 1].numpy()
+    want = _u0_derivs(x, t)
+    net = _Analytic()
+    for axes in ([0], [1], [0, 0], [0, 1], [1, 0], [1, 1], [0, 0, 0], [0, 1, 0], [1, 0, 0],
+                 [0, 0, 0, 0]):
+        got = O.ad_derivative(net, pts, 0, axes).detach().numpy().reshape(-1)
+        np.testing.assert_allclose(got, want[tuple(sorted(axes))], rtol=1e-12, atol=1e-12)
+    # u1 = cos(xt): d2/dxdt = -sin(xt) - xt cos(xt)
+    got = O.ad_derivative(net, pts, 1, [0, 1]).detach().numpy().reshape(-1)
+    np.testing.assert_allclose(got, -np.sin(x * t) - x * t * np.cos(x * t), rtol=1e-12, atol=1e-12)
+
+
+def test_ad_oracle_residual_and_conditions_closed_form():
+    """Residual, operator, Robin and data values and the padded loss of the
+    'autograd' oracle on two problems, with the analytic stand-in network."""
+    net = _Analytic()
+    # wave: u_tt - 4 u_xx; operator u_t at t = 0; Robin 2u + 0.5 u_x at x = 1
+    d, eq, b, kw = problems.wave_ad(api, n=6)
+    orc = O.ADOracle(_axes(d), eq.equation_lst, _conds(b))
+    loss, norm, op, bv, tv, keys, lens = orc.evaluate(net, kw['lambda_operator'], kw['lambda_bound'])
+    g = orc.grid.numpy()
+    D = _u0_derivs(g[:, 0], g[:, 1])
+    np.testing.assert_allclose(op.detach().numpy().reshape(-1), D[(1, 1)] - 4 * D[(0, 0)],
+                               rtol=1e-12, atol=1e-12)
+    assert keys == ['dirichlet', 'operator', 'robin'] and lens == [14, 7, 7]
+    xs = np.linspace(0, 1, 7)
+    np.testing.assert_allclose(bv[:7, 1].detach().numpy(), _u0_derivs(xs, 0 * xs)[(1,)],
+                               rtol=1e-12, atol=1e-12)
+    ts = np.linspace(0, 1, 7)
+    Dr = _u0_derivs(1 + 0 * ts, ts)
+    # the reference's Robin: alpha u + beta * (alpha u + beta u_x)
+    np.testing.assert_allclose(bv[:7, 2].detach().numpy(),
+                               2 * Dr[()] + 0.5 * (2 * Dr[()] + 0.5 * Dr[(0,)]),
+                               rtol=1e-12, atol=1e-12)
+    # padded mean over max_len = 14 for every column
+    r = D[(1, 1)] - 4 * D[(0, 0)]
+    bdiff = (bv - tv).detach().numpy()
+    want = np.mean(r ** 2) + 100 * np.sum(bdiff ** 2) / 14
+    np.testing.assert_allclose(float(loss), want, rtol=1e-12)
+    np.testing.assert_allclose(float(norm), np.mean(r ** 2) + np.sum(bdiff ** 2) / 14, rtol=1e-12)
+
+    # mixed partials, powers, a three-factor product, tensor and callable
+    # coefficients, 'data' with and without an operator
+    d, eq, b, kw = problems.mixed_ad(api, n=5)
+    orc = O.ADOracle(_axes(d), eq.equation_lst, _conds(b))
+    loss, norm, op, bv, tv, keys, lens = orc.evaluate(net, kw['lambda_operator'], kw['lambda_bound'])
+    g = orc.grid.numpy()
+    x, t = g[:, 0], g[:, 1]
+    D = _u0_derivs(x, t)
+    want_r = (1 + x * t) * D[(0, 1)] + 0.5 * D[(0, 1)] ** 2 - D[()] * D[(0,)] * D[(1,)] ** 2 \
+        + (x - 0.5) * D[(0, 0, 1)]
+    np.testing.assert_allclose(op.detach().numpy().reshape(-1), want_r, rtol=1e-11, atol=1e-11)
+    assert keys == ['dirichlet', 'data'] and lens == [6, 6]
+    px, pt = np.array([0.25, 0.5, 0.75]), np.array([0.5, 1.0, 1.5])
+    Dp = _u0_derivs(px, pt)
+    np.testing.assert_allclose(bv[:, 1].detach().numpy(), np.concatenate([Dp[()], Dp[(0, 1)]]),
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_ad_oracle_system_exact_solution():
+    """u' = v, v' = -u is solved exactly by (cos t, -sin t): residuals vanish
+    and the loss is the boundary misfit alone."""
+    class Osc(torch.nn.Module):
+        def forward(self, p):
+            return torch.cat([torch.cos(p), -torch.sin(p)], 1)
+
+    d, eq, b, kw = problems.system_ad(api, n=12)
+    orc = O.ADOracle(_axes(d), eq.equation_lst, _conds(b))
+    loss, norm, op, bv, tv, keys, lens = orc.evaluate(Osc(), kw['lambda_operator'], kw['lambda_bound'])
+    assert float(op.abs().max()) < 1e-14
+    assert float(loss) < 1e-28

@@ -27,6 +27,7 @@ same values, so the residual is unchanged.
 entries given by flat grid positions (tedeous/input_preprocessing.py:553-594).
 """
 
+import itertools
 from dataclasses import dataclass, field
 from typing import Any, Dict, List, Optional, Sequence, Tuple
 
@@ -585,3 +586,240 @@ def compile_mat(grid: torch.Tensor, operators: List[dict], bconds: Optional[List
                       bcs=bcs, bc_keys=keys, bc_lengths=[col_len[k] for k in keys],
                       bc_max_len=max_len, lam_bnd=lam_b, targets=targets,
                       dtype=g.dtype)
+
+
+# ----------------------------------------------------------------------------
+# 'autograd' mode
+# ----------------------------------------------------------------------------
+
+AD_MAX_ORDER = 4
+
+
+@dataclass
+class ADFactor:
+    var: int
+    pow_kind: int
+    pow_val: float
+    stencil: List[Tuple[int, int, float]]      # (point set, jet, weight)
+
+
+@dataclass
+class ADTerm:
+    coeff_scalar: Optional[float]
+    coeff_array: Optional[np.ndarray]          # per entry, or None
+    factors: List[ADFactor]
+
+
+@dataclass
+class ADGroup:
+    kind: str                                  # 'operator' or a bc type
+    sets: List[np.ndarray]                     # point coordinates per set [n, d]
+    equations: List[NNEquation]                # terms are ADTerm
+
+
+@dataclass
+class ADProblem:
+    d_in: int
+    n_out: int
+    jets: List[Tuple[int, ...]]                # jet 0 = () (the value)
+    groups: List[ADGroup]
+    n_grid: int
+    bc_keys: List[str]
+    bc_lengths: List[int]
+    bc_max_len: int
+    dtype: torch.dtype
+
+    def partitions(self) -> List[List[List[int]]]:
+        """Per jet, its set partitions as lists of block jets (Faa di Bruno:
+        the k-th derivative of tanh(z) along the axes of a multiset is the
+        sum over the set partitions of its positions of
+        tanh^(#blocks)(z) * prod over blocks of the block's derivative)."""
+        index = {j: i for i, j in enumerate(self.jets)}
+        out = []
+        for jet in self.jets:
+            parts = []
+            for blocks in _set_partitions(list(range(len(jet)))):
+                parts.append([index[tuple(sorted(jet[q] for q in b))] for b in blocks])
+            out.append(parts if jet else [])
+        return out
+
+
+def _set_partitions(items: List[int]):
+    """All set partitions of ``items`` (each a list of blocks)."""
+    if not items:
+        yield []
+        return
+    first, rest = items[0], items[1:]
+    for part in _set_partitions(rest):
+        # put ``first`` in its own block, or into one of the blocks
+        yield [[first]] + part
+        for k in range(len(part)):
+            yield part[:k] + [[first] + part[k]] + part[k + 1:]
+
+
+def _jet_of(f: Factor) -> Tuple[int, ...]:
+    return () if f.axes is None else tuple(sorted(int(a) for a in f.axes))
+
+
+def _ad_coeff(coeff, pts: torch.Tensor):
+    """Coefficient at the entries of a point set: a scalar, or one value per
+    entry (a tensor of that length, or a callable evaluated at the points
+    once, here; the reference calls it every step on the same points,
+    tedeous/derivative.py:116-118)."""
+    if isinstance(coeff, torch.nn.Parameter):
+        raise NotImplementedError("trainable coefficients are not supported")
+    if isinstance(coeff, (int, float)):
+        return float(coeff), None
+    n = pts.shape[0]
+    if isinstance(coeff, torch.Tensor):
+        flat = coeff.detach().cpu().reshape(-1)
+        if flat.numel() == 1:
+            return float(flat[0]), None
+        if flat.numel() != n:
+            raise ValueError(f"a tensor coefficient must have one value per point ({n})")
+        return None, flat.numpy().astype(np.float64)
+    if callable(coeff):
+        vals = coeff(pts)
+        if isinstance(vals, torch.Tensor):
+            vals = vals.detach().cpu().numpy()
+        vals = np.asarray(vals, dtype=np.float64).reshape(-1)
+        if vals.size == 1:
+            vals = np.full(n, float(vals[0]))
+        return None, vals
+    raise NameError('"coeff" should be: torch.Tensor or callable or int or float!')
+
+
+def _ad_terms(terms: List[Term], pts: torch.Tensor, set_k: int, sign: float = 1.0,
+              scale: float = 1.0) -> List[ADTerm]:
+    out = []
+    for t in terms:
+        cs, ca = _ad_coeff(t.coeff, pts)
+        k = sign * scale
+        cs = None if cs is None else (cs if k == 1.0 else k * cs)
+        ca = None if ca is None else (ca if k == 1.0 else k * ca)
+        factors = []
+        for f in t.factors:
+            if f.axes is not None and len(f.axes) > AD_MAX_ORDER:
+                raise NotImplementedError(f"derivative order above {AD_MAX_ORDER} in 'autograd' mode")
+            kind, pv = _pow_code(f.pow)
+            factors.append(ADFactor(var=f.var, pow_kind=kind, pow_val=pv,
+                                    stencil=[(set_k, _jet_of(f), 1.0)]))
+        if len(factors) > 4:
+            raise NotImplementedError("more than 4 factors in a term")
+        out.append(ADTerm(coeff_scalar=cs, coeff_array=ca, factors=factors))
+    return out
+
+
+def compile_ad(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[dict]],
+               n_out: int, lambda_operator=1, lambda_bound=1) -> ADProblem:
+    """Lower an 'autograd' problem (tedeous/input_preprocessing.py:411-505,
+    tedeous/eval.py:90-461 with mode 'autograd', tedeous/losses.py:84-135).
+
+    The operator is evaluated on every grid point (the reference's sorted
+    grid is the grid itself in this mode, tedeous/eval.py:121-122); boundary
+    operators on the condition's points (tedeous/eval.py:318-319)."""
+    grid_cpu = grid.detach().cpu()
+    dtype = grid_cpu.dtype
+    n, d = grid_cpu.shape
+    if isinstance(operators, dict):
+        operators = [operators]
+    n_eq = len(operators)
+    lam_op = _lambda_list(lambda_operator, n_eq, 'lambda_operator')
+
+    eqs = []
+    for i, op in enumerate(operators):
+        eqs.append(NNEquation(terms=_ad_terms(parse_operator(op), grid_cpu, 0), lam=lam_op[i],
+                              denom=float(n), target=None))
+    groups = [ADGroup(kind='operator', sets=[grid_cpu.numpy()], equations=eqs)]
+
+    keys: List[str] = []
+    col_vals: Dict[str, List[np.ndarray]] = {}
+    col_len: Dict[str, int] = {}
+    pending = []
+    for bc in (bconds or []):
+        typ = bc['type']
+        if typ not in keys:
+            keys.append(typ)
+            col_vals[typ] = []
+            col_len[typ] = 0
+        col_vals[typ].append(bc['bval'].detach().cpu().reshape(-1).numpy())
+        var = int(bc['var']) if bc.get('var') is not None else 0
+        bop = bc['bop']
+        if typ == 'periodic':
+            sets = [s.detach().cpu().to(dtype) for s in bc['bnd']]
+        else:
+            sets = [bc['bnd'].detach().cpu().to(dtype)]
+        cnt = sets[0].shape[0]
+        if any(s.shape[0] != cnt for s in sets):
+            raise ValueError("periodic point sets must have equal sizes")
+        if typ in ('dirichlet', 'periodic', 'data') and bop is None:
+            # u(set0) - u(set1) - ... (tedeous/eval.py:283-303, 342-345)
+            st = [(k, (), 1.0 if k == 0 else -1.0) for k in range(len(sets))]
+            terms = [ADTerm(1.0, None, [ADFactor(var=var, pow_kind=0, pow_val=1.0, stencil=st)])]
+        elif typ in ('operator', 'data'):
+            terms = _ad_terms(parse_operator(bop), sets[0], 0)
+        elif typ == 'periodic':
+            # operator at set 0 minus the operator at sets 1, 2, ...
+            # (tedeous/eval.py:351-354)
+            parsed = parse_operator(bop)
+            terms = []
+            for k, s in enumerate(sets):
+                terms += _ad_terms(parsed, s, k, sign=1.0 if k == 0 else -1.0)
+        elif typ == 'robin':
+            # alpha * u[var] + sum over beta of beta * (the whole operator)
+            # (tedeous/eval.py:357-388)
+            parsed = parse_operator(bop)
+            coeffs = []
+            for t in parsed:
+                c = t.coeff
+                if isinstance(c, torch.Tensor) and c.numel() == 1:
+                    c = float(c.reshape(-1)[0])
+                if not isinstance(c, (int, float)):
+                    raise NotImplementedError("Robin conditions take scalar coefficients")
+                coeffs.append(float(c))
+            terms = [ADTerm(coeffs[0], None,
+                            [ADFactor(var=var, pow_kind=0, pow_val=1.0, stencil=[(0, (), 1.0)])])]
+            for beta in coeffs[1:]:
+                terms += _ad_terms(parsed, sets[0], 0, scale=beta)
+        else:
+            raise NotImplementedError(f"boundary type {typ!r} is not supported")
+        eq = NNEquation(terms=terms, lam=0.0, denom=1.0, target=None)
+        groups.append(ADGroup(kind=typ, sets=[s.numpy() for s in sets], equations=[eq]))
+        pending.append((eq, typ, col_len[typ], np.arange(cnt)))
+        col_len[typ] += cnt
+
+    max_len = max(col_len.values()) if keys else 0
+    lam_b = _lambda_list(lambda_bound, len(keys), 'lambda_bound') if keys else []
+    padded = {}
+    for k in keys:
+        t = np.concatenate(col_vals[k]) if col_vals[k] else np.zeros(0)
+        col = np.zeros(max_len, dtype=np.float64)
+        m = min(len(t), max_len)
+        col[:m] = t[:m]
+        padded[k] = col
+    for eq, typ, start, sel in pending:
+        ci = keys.index(typ)
+        eq.column = ci
+        eq.lam = lam_b[ci]
+        eq.denom = float(max_len)
+        pos = start + np.asarray(sel, dtype=np.int64)
+        eq.col_pos = pos
+        eq.target = padded[typ][pos]
+
+    # jets: every derivative read, closed under sub-multisets, by order
+    jets = {()}
+    for g in groups:
+        for eq in g.equations:
+            for t in eq.terms:
+                for f in t.factors:
+                    for _, jet, _ in f.stencil:
+                        for r in range(len(jet) + 1):
+                            for sub in itertools.combinations(jet, r):
+                                jets.add(tuple(sorted(sub)))
+    for jet in jets:
+        if any(a < 0 or a >= d for a in jet):
+            raise ValueError(f"derivative axis out of range in {jet}")
+    jet_list = sorted(jets, key=lambda j: (len(j), j))
+    return ADProblem(d_in=d, n_out=n_out, jets=jet_list, groups=groups, n_grid=n,
+                     bc_keys=keys, bc_lengths=[col_len[k] for k in keys],
+                     bc_max_len=max_len, dtype=dtype)

@@ -1,0 +1,892 @@
+// 'autograd' mode: exact derivatives of the tanh MLP by jets.
+//
+// The reference differentiates the network with nested
+// torch.autograd.grad(create_graph=True) calls, one chain per derivative
+// term (Derivative_autograd._nn_autograd, tedeous/derivative.py:73-98), and
+// evaluates the operator on every grid point (tedeous/eval.py:117-123).
+// Here each point carries a jet -- the value and every partial derivative
+// the program reads -- through the layers in one pass:
+//   linear layer:  z_a = W h_a (+ b for the value), per jet component a;
+//   tanh layer:    h_a = sum over the set partitions P of a of
+//                  tanh^(|P|)(z_0) * prod_{B in P} z_B    (Faa di Bruno).
+// tanh^(n) is a polynomial in t = tanh(z_0): P_0 = t, P_{n+1} =
+// (1 - t^2) P_n'(t).  The adjoint of that map, the transposed linear maps
+// and a deterministic chunked reduction of the weight gradients make the
+// backward pass; loss and seeds come from the residual program at the
+// network's output jets.
+//
+// Layout in the workspace (T = float or double, J jets, P points):
+//   IN0[P][J][d_in]       first layer's input jets (x, unit vectors, zeros)
+//   Z_l, A_l[P][J][H]     pre-activation and activation jets, hidden l
+//   U[P][J][n_out]        output jets
+//   SEED, DU[P][J][n_out] dloss/dU (unscaled) and its dloss-scaled copy
+//   DZ_l[P][J][H]         dloss/dZ_l
+//   partials of the weight gradients, loss partials, gradient, one.
+
+#include <algorithm>
+#include <cmath>
+
+#include "tdx_common.h"
+
+using namespace tdx;
+
+namespace {
+
+constexpr int AD_THREADS = 256;
+constexpr int AD_MAXF = 4;        // factors per term
+constexpr int AD_WG_EPT = 16;     // weight-gradient elements per thread
+constexpr int AD_WG_CHUNK = 2048; // rows (point, jet) per reduction chunk
+
+template <typename T>
+struct TanhDerivs {
+  T s0, s1, s2, s3, s4, s5;  // tanh^(n)(z), n = 0 .. 5
+};
+
+// P_1 = 1 - t^2, P_2 = -2 t P_1, P_3 = P_1 (6 t^2 - 2),
+// P_4 = P_1 t (16 - 24 t^2), P_5 = P_1 (16 - 120 t^2 + 120 t^4)
+template <typename T>
+__device__ __forceinline__ TanhDerivs<T> tanh_derivs(T z) {
+  TanhDerivs<T> d;
+  const T t = dtanh(z);
+  const T t2 = t * t;
+  const T u = T(1) - t2;
+  d.s0 = t;
+  d.s1 = u;
+  d.s2 = T(-2) * t * u;
+  d.s3 = u * (T(6) * t2 - T(2));
+  d.s4 = u * t * (T(16) - T(24) * t2);
+  d.s5 = u * (T(16) + t2 * (T(-120) + T(120) * t2));
+  return d;
+}
+
+// tanh^(n) for a runtime n in 1 .. 5
+template <typename T>
+__device__ __forceinline__ T deriv_n(const T s1, const T s2, const T s3, const T s4, const T s5,
+                                    int n) {
+  // value selects (a select of field addresses becomes a private-memory load)
+  const T a = n >= 4 ? s4 : s2, b = n >= 4 ? s5 : s3;
+  const T c = (n & 1) ? (n == 1 ? s1 : b) : a;
+  return c;
+}
+
+struct Tables {
+  const int32_t *ints;
+  int jet_i, pptr_i, part_i;
+};
+
+// ---------------------------------------------------------------------------
+// forward: one layer
+// ---------------------------------------------------------------------------
+template <typename T>
+struct FwdArgs {
+  int layer, din, dout, act, J;
+  int64_t P;
+  int ppb, tpp_log;
+  const T *x;      // layer 0: points [P][din]
+  const T *in;     // layer > 0: input jets [P][J][din]
+  const T *W, *b;  // [dout][din], [dout]
+  T *in0;          // layer 0: input jets written here
+  T *zout;         // pre-activation jets (act) [P][J][dout]
+  T *out;          // activation jets (act) or output jets [P][J][dout]
+  Tables tb;
+};
+
+template <typename T>
+__global__ __launch_bounds__(AD_THREADS) void k_ad_fwd(FwdArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *Wt = reinterpret_cast<T *>(smem_raw);        // [din][dout]
+  T *IN = Wt + (size_t)a.din * a.dout;             // [ppb][J][din]
+  T *Z = IN + (size_t)a.ppb * a.J * a.din;         // [ppb][J][dout]
+  const int tid = threadIdx.x;
+  const int J = a.J, din = a.din, dout = a.dout;
+  const int64_t p0 = (int64_t)blockIdx.x * a.ppb;
+  const int npts = (int)std::min<int64_t>(a.ppb, a.P - p0);
+  for (int k = tid; k < din * dout; k += AD_THREADS) {
+    const int i = k / din, j = k - (k / din) * din;
+    Wt[j * dout + i] = a.W[k];
+  }
+  const int per_pt = J * din;
+  for (int k = tid; k < npts * per_pt; k += AD_THREADS) {
+    T v;
+    if (a.layer == 0) {
+      const int pp = k / per_pt, r = k - pp * per_pt;
+      const int al = r / din, j = r - al * din;
+      const int ord = a.tb.ints[a.tb.jet_i + 2 * al];
+      const int ax = a.tb.ints[a.tb.jet_i + 2 * al + 1];
+      v = al == 0 ? a.x[(p0 + pp) * din + j] : (ord == 1 && ax == j ? T(1) : T(0));
+      a.in0[p0 * per_pt + k] = v;
+    } else {
+      v = a.in[p0 * per_pt + k];
+    }
+    IN[k] = v;
+  }
+  __syncthreads();
+  const int pp = tid >> a.tpp_log, i = tid & ((1 << a.tpp_log) - 1);
+  if (pp >= npts || i >= dout) return;
+  const int64_t p = p0 + pp;
+  const T *inp = IN + (size_t)pp * per_pt;
+  T *zl = Z + (size_t)pp * J * dout;
+  for (int al = 0; al < J; ++al) {
+    T acc = T(0);
+    for (int j = 0; j < din; ++j) acc = acc + Wt[j * dout + i] * inp[al * din + j];
+    if (al == 0) acc = acc + a.b[i];
+    const size_t o = ((size_t)p * J + al) * dout + i;
+    if (a.act) {
+      zl[al * dout + i] = acc;
+      a.zout[o] = acc;
+    } else {
+      a.out[o] = acc;
+    }
+  }
+  if (!a.act) return;
+  const TanhDerivs<T> s = tanh_derivs(zl[i]);
+  a.out[((size_t)p * J) * dout + i] = s.s0;
+  for (int al = 1; al < J; ++al) {
+    const int k0 = a.tb.ints[a.tb.pptr_i + al], k1 = a.tb.ints[a.tb.pptr_i + al + 1];
+    T h = T(0);
+    for (int k = k0; k < k1; ++k) {
+      const int32_t *pr = a.tb.ints + a.tb.part_i + TDX_AD_PART_INTS * k;
+      const int m = pr[0];
+      T prod = deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m);
+#pragma unroll
+      for (int q = 0; q < TDX_AD_MAX_ORDER; ++q)
+        if (q < m) prod = prod * zl[pr[1 + q] * dout + i];
+      h = (k == k0) ? prod : h + prod;
+    }
+    a.out[((size_t)p * J + al) * dout + i] = h;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// residual program, loss and seeds
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ResArgs {
+  int J, n_out, n_groups;
+  int64_t n_entries;
+  const int64_t *gstart;  // entry prefix per group [n_groups + 1]
+  const int32_t *ints;
+  const T *reals, *data;
+  int groups_i;
+  const T *U;
+  T *seed, *resid, *lp;
+};
+
+template <typename T>
+__device__ __forceinline__ T fpow(int kind, int ref, const T *reals, T x) {
+  return kind == 0 ? ipow(x, ref) : dpow(x, reals[ref]);
+}
+template <typename T>
+__device__ __forceinline__ T fdpow(int kind, int ref, const T *reals, T x) {
+  if (kind == 0) return ref == 0 ? T(0) : T(ref) * ipow(x, ref - 1);
+  const T p = reals[ref];
+  return p * dpow(x, p - T(1));
+}
+
+template <typename T>
+__global__ __launch_bounds__(AD_THREADS) void k_ad_resid(ResArgs<T> a) {
+  __shared__ T red[2 * AD_THREADS];
+  const int tid = threadIdx.x;
+  const int64_t ge = (int64_t)blockIdx.x * AD_THREADS + tid;
+  T lsum = T(0), lnsum = T(0);
+  if (ge < a.n_entries) {
+    int g = 0;
+    while (g + 1 < a.n_groups && a.gstart[g + 1] <= ge) ++g;
+    const int e = (int)(ge - a.gstart[g]);
+    const int32_t *G = a.ints + a.groups_i + TDX_AD_GROUP_INTS * g;
+    const int n_sets = G[1], set_i = G[2], n_eq = G[3], eq_i = G[4];
+    const int row = a.J * a.n_out;
+    for (int s = 0; s < n_sets; ++s) {
+      T *sd = a.seed + (size_t)(a.ints[set_i + s] + e) * row;
+      for (int k = 0; k < row; ++k) sd[k] = T(0);
+    }
+    auto factor_value = [&](const int32_t *F) {
+      const int n_st = F[0], st_i = F[1], var = F[2];
+      T fv = T(0);
+      for (int k = 0; k < n_st; ++k) {
+        const int32_t *S = a.ints + st_i + TDX_AD_STENCIL_INTS * k;
+        const int64_t pt = a.ints[set_i + S[0]] + e;
+        const T v = a.U[((size_t)pt * a.J + S[1]) * a.n_out + var];
+        const T w = a.reals[S[2]];
+        fv = (k == 0) ? w * v : fv + w * v;
+      }
+      return fv;
+    };
+    for (int q = 0; q < n_eq; ++q) {
+      const int32_t *E = a.ints + eq_i + TDX_AD_EQ_INTS * q;
+      const int n_terms = E[0], term_i = E[1];
+      const T lam = a.reals[E[2]], den = a.reals[E[3]];
+      T r = T(0);
+      for (int t = 0; t < n_terms; ++t) {
+        const int32_t *Tr = a.ints + term_i + TDX_AD_TERM_INTS * t;
+        const T c = Tr[2] == 0 ? a.reals[Tr[3]] : a.data[Tr[3] + e];
+        T prod = T(1);
+        for (int f = 0; f < Tr[0]; ++f) {
+          const int32_t *F = a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f;
+          prod = prod * fpow(F[3], F[4], a.reals, factor_value(F));
+        }
+        const T tv = c * prod;
+        r = (t == 0) ? tv : r + tv;
+      }
+      if (a.resid) a.resid[E[5] + e] = r;
+      const T diff = E[4] >= 0 ? r - a.data[E[4] + e] : r;
+      const T d2 = diff * diff;
+      lsum += lam * d2 / den;
+      lnsum += d2 / den;
+      const T gq = T(2) * lam * diff / den;
+      for (int t = 0; t < n_terms; ++t) {
+        const int32_t *Tr = a.ints + term_i + TDX_AD_TERM_INTS * t;
+        const int nf = Tr[0];
+        const T c = Tr[2] == 0 ? a.reals[Tr[3]] : a.data[Tr[3] + e];
+        T fv[AD_MAXF], pw[AD_MAXF];
+        for (int f = 0; f < AD_MAXF; ++f) {
+          if (f < nf) {
+            const int32_t *F = a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f;
+            fv[f] = factor_value(F);
+            pw[f] = fpow(F[3], F[4], a.reals, fv[f]);
+          }
+        }
+        for (int f = 0; f < nf; ++f) {
+          const int32_t *F = a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f;
+          T dv = gq * c * fdpow(F[3], F[4], a.reals, fv[f]);
+          for (int f2 = 0; f2 < nf; ++f2)
+            if (f2 != f) dv = dv * pw[f2];
+          const int n_st = F[0], st_i = F[1], var = F[2];
+          for (int k = 0; k < n_st; ++k) {
+            const int32_t *S = a.ints + st_i + TDX_AD_STENCIL_INTS * k;
+            const int64_t pt = a.ints[set_i + S[0]] + e;
+            a.seed[((size_t)pt * a.J + S[1]) * a.n_out + var] += dv * a.reals[S[2]];
+          }
+        }
+      }
+    }
+  }
+  red[tid] = lsum;
+  red[AD_THREADS + tid] = lnsum;
+  __syncthreads();
+  for (int st = AD_THREADS / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      red[tid] += red[tid + st];
+      red[AD_THREADS + tid] += red[AD_THREADS + tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    a.lp[2 * blockIdx.x] = red[0];
+    a.lp[2 * blockIdx.x + 1] = red[AD_THREADS];
+  }
+}
+
+// fixed-order sum of the loss partials of n blocks
+template <typename T>
+__global__ __launch_bounds__(AD_THREADS) void k_ad_loss(const T *lp, int n, T *loss) {
+  __shared__ T red[2 * AD_THREADS];
+  const int tid = threadIdx.x;
+  T s0 = T(0), s1 = T(0);
+  for (int k = tid; k < n; k += AD_THREADS) {
+    s0 += lp[2 * k];
+    s1 += lp[2 * k + 1];
+  }
+  red[tid] = s0;
+  red[AD_THREADS + tid] = s1;
+  __syncthreads();
+  for (int st = AD_THREADS / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      red[tid] += red[tid + st];
+      red[AD_THREADS + tid] += red[AD_THREADS + tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    loss[0] = red[0];
+    loss[1] = red[AD_THREADS];
+  }
+}
+
+template <typename T>
+__global__ void k_ad_scale(const T *seed, const T *dloss, T *du, int64_t n) {
+  const T dl = *dloss;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    du[i] = seed[i] * dl;
+}
+
+template <typename T>
+__global__ void k_ad_one(T *x) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *x = T(1);
+}
+
+// ---------------------------------------------------------------------------
+// backward: one hidden layer
+// ---------------------------------------------------------------------------
+template <typename T>
+struct BwdArgs {
+  int H, dn, J;
+  int64_t P;
+  int ppb, tpp_log;
+  const T *Wn;   // next layer's weights [dn][H]
+  const T *DN;   // dloss/d(next pre-activation jets) [P][J][dn]
+  const T *Z;    // this layer's pre-activation jets [P][J][H]
+  T *DZ;         // out: dloss/dZ [P][J][H]
+  Tables tb;
+};
+
+template <typename T>
+__global__ __launch_bounds__(AD_THREADS) void k_ad_bwd(BwdArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int J = a.J, H = a.H, dn = a.dn;
+  T *Wn = reinterpret_cast<T *>(smem_raw);          // [dn][H]
+  T *DN = Wn + (size_t)dn * H;                       // [ppb][J][dn]
+  T *ZS = DN + (size_t)a.ppb * J * dn;               // [ppb][J][H]
+  T *DZ = ZS + (size_t)a.ppb * J * H;                // [ppb][J][H]
+  const int tid = threadIdx.x;
+  const int64_t p0 = (int64_t)blockIdx.x * a.ppb;
+  const int npts = (int)std::min<int64_t>(a.ppb, a.P - p0);
+  for (int k = tid; k < dn * H; k += AD_THREADS) Wn[k] = a.Wn[k];
+  for (int k = tid; k < npts * J * dn; k += AD_THREADS) DN[k] = a.DN[p0 * J * dn + k];
+  for (int k = tid; k < npts * J * H; k += AD_THREADS) {
+    ZS[k] = a.Z[p0 * J * H + k];
+    DZ[k] = T(0);
+  }
+  __syncthreads();
+  const int pp = tid >> a.tpp_log, j = tid & ((1 << a.tpp_log) - 1);
+  if (pp >= npts || j >= H) return;
+  const T *dnp = DN + (size_t)pp * J * dn;
+  const T *zl = ZS + (size_t)pp * J * H;
+  T *dz = DZ + (size_t)pp * J * H;
+  const TanhDerivs<T> s = tanh_derivs(zl[j]);
+  T dz0 = T(0);
+  for (int al = 0; al < J; ++al) {
+    // dloss/d(activation jet al) at unit j
+    T da = T(0);
+    for (int i = 0; i < dn; ++i) da = da + Wn[i * H + j] * dnp[al * dn + i];
+    if (al == 0) {
+      dz0 = dz0 + da * s.s1;
+      continue;
+    }
+    const int k0 = a.tb.ints[a.tb.pptr_i + al], k1 = a.tb.ints[a.tb.pptr_i + al + 1];
+    for (int k = k0; k < k1; ++k) {
+      const int32_t *pr = a.tb.ints + a.tb.part_i + TDX_AD_PART_INTS * k;
+      const int m = pr[0];
+      T zb[TDX_AD_MAX_ORDER];
+      T all = T(1);
+#pragma unroll
+      for (int q = 0; q < TDX_AD_MAX_ORDER; ++q) {
+        zb[q] = q < m ? zl[pr[1 + q] * H + j] : T(1);
+        all = all * zb[q];
+      }
+      dz0 = dz0 + da * deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m + 1) * all;
+      const T sm = deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m);
+#pragma unroll
+      for (int q = 0; q < TDX_AD_MAX_ORDER; ++q) {
+        if (q < m) {
+          T others = T(1);
+#pragma unroll
+          for (int q2 = 0; q2 < TDX_AD_MAX_ORDER; ++q2)
+            if (q2 != q && q2 < m) others = others * zb[q2];
+          dz[pr[1 + q] * H + j] += da * sm * others;
+        }
+      }
+    }
+  }
+  dz[j] += dz0;
+  for (int al = 0; al < J; ++al) a.DZ[((size_t)(p0 + pp) * J + al) * H + j] = dz[al * H + j];
+}
+
+// ---------------------------------------------------------------------------
+// weight gradients: dW[i][j] = sum_rows D[r][i] S[r][j], db[i] = sum over
+// value rows of D[r][i]; rows r = (point, jet), chunked, fixed order
+// ---------------------------------------------------------------------------
+template <typename T>
+struct WgArgs {
+  int din, dout, J;
+  int64_t K;      // rows
+  const T *D;     // [K][dout]
+  const T *S;     // [K][din]
+  T *part;        // [n_chunks][dout * (din + 1)]
+};
+
+template <typename T>
+__global__ __launch_bounds__(AD_THREADS) void k_ad_wgrad(WgArgs<T> a) {
+  constexpr int AD_WG_RT = sizeof(T) == 4 ? 16 : 8;  // rows per LDS tile (33 KB)
+  __shared__ T Ds[AD_WG_RT * 257];
+  __shared__ T Ss[AD_WG_RT * 258];
+  const int tid = threadIdx.x;
+  const int din = a.din, dout = a.dout, ne = dout * (din + 1);
+  const int64_t r0 = (int64_t)blockIdx.x * AD_WG_CHUNK;
+  const int64_t r1 = std::min<int64_t>(a.K, r0 + AD_WG_CHUNK);
+  const int e0 = blockIdx.y * AD_THREADS * AD_WG_EPT;
+  T acc[AD_WG_EPT];
+  int ei[AD_WG_EPT], ej[AD_WG_EPT];
+#pragma unroll
+  for (int k = 0; k < AD_WG_EPT; ++k) {
+    acc[k] = T(0);
+    const int e = e0 + k * AD_THREADS + tid;
+    ei[k] = e < ne ? e / (din + 1) : -1;
+    ej[k] = e < ne ? e - (e / (din + 1)) * (din + 1) : 0;
+  }
+  for (int64_t rt = r0; rt < r1; rt += AD_WG_RT) {
+    const int nr = (int)std::min<int64_t>(AD_WG_RT, r1 - rt);
+    __syncthreads();
+    for (int k = tid; k < nr * dout; k += AD_THREADS) {
+      const int r = k / dout, i = k - (k / dout) * dout;
+      Ds[r * 257 + i] = a.D[(rt + r) * dout + i];
+    }
+    for (int k = tid; k < nr * (din + 1); k += AD_THREADS) {
+      const int r = k / (din + 1), j = k - (k / (din + 1)) * (din + 1);
+      Ss[r * 258 + j] = j < din ? a.S[(rt + r) * din + j] : (((rt + r) % a.J) == 0 ? T(1) : T(0));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < AD_WG_EPT; ++k) {
+      if (ei[k] < 0) continue;
+      T s = acc[k];
+      for (int r = 0; r < nr; ++r) s = s + Ds[r * 257 + ei[k]] * Ss[r * 258 + ej[k]];
+      acc[k] = s;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < AD_WG_EPT; ++k) {
+    const int e = e0 + k * AD_THREADS + tid;
+    if (e < ne) a.part[(size_t)blockIdx.x * ne + e] = acc[k];
+  }
+}
+
+template <typename T>
+__global__ void k_ad_wreduce(const T *part, int n_chunks, int din, int dout, T *gw, T *gb) {
+  const int ne = dout * (din + 1);
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  T s = T(0);
+  for (int c = 0; c < n_chunks; ++c) s += part[(size_t)c * ne + e];
+  const int i = e / (din + 1), j = e - (e / (din + 1)) * (din + 1);
+  if (j < din)
+    gw[i * din + j] = s;
+  else
+    gb[i] = s;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+struct tdx_ad_plan {
+  int dtype, d_in, n_out, H, L, J;
+  int64_t P, n_entries, n_resid, n_params;
+  int n_groups, groups_i, jet_i, pptr_i, part_i;
+  DevTable ints, reals, data, points, gstart;
+  std::vector<int64_t> off_w, off_b;
+  std::vector<int> din, dout;
+  // launch shapes per layer
+  std::vector<int> f_ppb, f_tpp, b_ppb, b_tpp;
+  std::vector<size_t> f_lds, b_lds;
+  int n_res_blocks;
+  int64_t n_chunks;
+  // workspace offsets (bytes)
+  size_t off_in0, off_u, off_seed, off_du, off_lp, off_part, off_grad, off_one, total;
+  std::vector<size_t> off_z, off_a, off_dz;
+};
+
+namespace {
+
+size_t ad_fwd_lds(int din, int dout, int J, int ppb, bool act, size_t ts) {
+  return ((size_t)din * dout + (size_t)ppb * J * din + (act ? (size_t)ppb * J * dout : 0)) * ts;
+}
+size_t ad_bwd_lds(int H, int dn, int J, int ppb, size_t ts) {
+  return ((size_t)dn * H + (size_t)ppb * J * (dn + 2 * H)) * ts;
+}
+int pow2_log(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+// largest points-per-block that fits the LDS budget (64 KB when possible)
+template <typename F>
+int pick_ppb(int tpp_log, F lds) {
+  int ppb = AD_THREADS >> tpp_log;
+  while (ppb > 1 && lds(ppb) > 64 * 1024) --ppb;
+  if (lds(ppb) > 160 * 1024) return 0;
+  return ppb;
+}
+
+int fail(const char *msg) {
+  set_error("autograd plan: %s", msg);
+  return TDX_EINVAL;
+}
+
+}  // namespace
+
+extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
+  TDX_REQUIRE(d && out, "null argument");
+  *out = nullptr;
+  if (d->dtype != 0 && d->dtype != 1) return fail("dtype must be 0 or 1");
+  if (d->d_in < 1 || d->n_out < 1 || d->width < 1 || d->n_layers < 2) return fail("network shape");
+  if (d->width > 256 || d->n_out > 256 || d->d_in > 256) return fail("widths above 256");
+  if (d->n_jets < 1 || d->n_points < 1 || d->n_groups < 1) return fail("empty problem");
+  if (!d->ints || !d->reals || !d->points || d->n_ints < 1) return fail("null table");
+  const int64_t NI = d->n_ints;
+  const int32_t *I = d->ints;
+  auto in_ints = [&](int64_t i, int64_t n) { return i >= 0 && n >= 0 && i + n <= NI; };
+  auto in_reals = [&](int64_t i) { return i >= 0 && i < d->n_reals; };
+  auto in_data = [&](int64_t i, int64_t n) { return i >= 0 && n >= 0 && i + n <= d->n_data; };
+  const int J = d->n_jets;
+  // jets and partitions
+  if (!in_ints(d->jet_i, 2 * (int64_t)J) || !in_ints(d->pptr_i, J + 1)) return fail("jet table");
+  if (I[d->jet_i] != 0) return fail("jet 0 must be the value");
+  for (int j = 0; j < J; ++j) {
+    const int ord = I[d->jet_i + 2 * j], ax = I[d->jet_i + 2 * j + 1];
+    if (ord < 0 || ord > TDX_AD_MAX_ORDER) return fail("jet order above 4");
+    if (ord == 1 && (ax < 0 || ax >= d->d_in)) return fail("jet axis");
+    if (j > 0 && ord == 0) return fail("only jet 0 has order 0");
+    const int k0 = I[d->pptr_i + j], k1 = I[d->pptr_i + j + 1];
+    if (k0 < 0 || k1 < k0) return fail("partition pointers");
+    if (j == 0 && k1 != k0) return fail("the value has no partitions");
+    if (j > 0 && k1 == k0) return fail("a derivative jet needs partitions");
+    if (!in_ints(d->part_i + (int64_t)TDX_AD_PART_INTS * k0, (int64_t)TDX_AD_PART_INTS * (k1 - k0)))
+      return fail("partition records");
+    for (int k = k0; k < k1; ++k) {
+      const int32_t *pr = I + d->part_i + TDX_AD_PART_INTS * k;
+      if (pr[0] < 1 || pr[0] > ord) return fail("partition block count");
+      for (int q = 0; q < pr[0]; ++q) {
+        const int b = pr[1 + q];
+        if (b < 1 || b >= J || I[d->jet_i + 2 * b] >= ord + (pr[0] == 1 ? 1 : 0))
+          return fail("partition block jet");
+        if (pr[0] == 1 && b != j) return fail("one-block partition must be the jet itself");
+      }
+    }
+  }
+  // groups
+  if (!in_ints(d->groups_i, (int64_t)TDX_AD_GROUP_INTS * d->n_groups)) return fail("group table");
+  std::vector<int64_t> gstart(d->n_groups + 1, 0);
+  std::vector<char> owned(d->n_points, 0);
+  int64_t n_resid_used = 0;
+  for (int g = 0; g < d->n_groups; ++g) {
+    const int32_t *G = I + d->groups_i + TDX_AD_GROUP_INTS * g;
+    const int ne = G[0], ns = G[1], set_i = G[2], neq = G[3], eq_i = G[4];
+    if (ne < 0 || ns < 1 || neq < 0) return fail("group record");
+    if (!in_ints(set_i, ns) || !in_ints(eq_i, (int64_t)TDX_AD_EQ_INTS * neq)) return fail("group ranges");
+    for (int s = 0; s < ns; ++s) {
+      const int64_t off = I[set_i + s];
+      if (off < 0 || off + ne > d->n_points) return fail("set points out of range");
+      for (int e = 0; e < ne; ++e) {
+        if (owned[off + e]) return fail("a point belongs to two entries");
+        owned[off + e] = 1;
+      }
+    }
+    for (int q = 0; q < neq; ++q) {
+      const int32_t *E = I + eq_i + TDX_AD_EQ_INTS * q;
+      const int nt = E[0], ti = E[1];
+      if (nt < 1 || !in_ints(ti, (int64_t)TDX_AD_TERM_INTS * nt)) return fail("equation terms");
+      if (!in_reals(E[2]) || !in_reals(E[3])) return fail("equation weights");
+      if (d->reals[E[3]] == 0.0) return fail("zero loss denominator");
+      if (E[4] >= 0 && !in_data(E[4], ne)) return fail("equation target");
+      if (E[5] < 0 || E[5] + (int64_t)ne > d->n_resid) return fail("residual offset");
+      n_resid_used = std::max<int64_t>(n_resid_used, E[5] + (int64_t)ne);
+      for (int t = 0; t < nt; ++t) {
+        const int32_t *Tr = I + ti + TDX_AD_TERM_INTS * t;
+        const int nf = Tr[0], fi = Tr[1];
+        if (nf < 1 || nf > AD_MAXF || !in_ints(fi, (int64_t)TDX_AD_FACTOR_INTS * nf))
+          return fail("term factors (1 to 4 per term)");
+        if (Tr[2] == 0 ? !in_reals(Tr[3]) : (Tr[2] != 1 || !in_data(Tr[3], ne)))
+          return fail("term coefficient");
+        for (int f = 0; f < nf; ++f) {
+          const int32_t *F = I + fi + TDX_AD_FACTOR_INTS * f;
+          const int nst = F[0], si = F[1], var = F[2];
+          if (nst < 1 || !in_ints(si, (int64_t)TDX_AD_STENCIL_INTS * nst)) return fail("factor stencil");
+          if (var < 0 || var >= d->n_out) return fail("factor variable");
+          if (F[3] == 1 ? !in_reals(F[4]) : F[3] != 0) return fail("factor power");
+          for (int k = 0; k < nst; ++k) {
+            const int32_t *S = I + si + TDX_AD_STENCIL_INTS * k;
+            if (S[0] < 0 || S[0] >= ns || S[1] < 0 || S[1] >= J || !in_reals(S[2]))
+              return fail("stencil entry");
+          }
+        }
+      }
+    }
+    gstart[g + 1] = gstart[g] + ne;
+  }
+  if (gstart[d->n_groups] < 1) return fail("no entries");
+  if (gstart[d->n_groups] > (int64_t)1 << 31) return fail("more than 2^31 entries");
+
+  const size_t ts = d->dtype == 0 ? 4 : 8;
+  auto *p = new tdx_ad_plan();
+  p->dtype = d->dtype;
+  p->d_in = d->d_in;
+  p->n_out = d->n_out;
+  p->H = d->width;
+  p->L = d->n_layers;
+  p->J = J;
+  p->P = d->n_points;
+  p->n_entries = gstart[d->n_groups];
+  p->n_resid = d->n_resid;
+  p->n_groups = d->n_groups;
+  p->groups_i = d->groups_i;
+  p->jet_i = d->jet_i;
+  p->pptr_i = d->pptr_i;
+  p->part_i = d->part_i;
+  int64_t off = 0;
+  for (int l = 0; l < p->L; ++l) {
+    const int di = l == 0 ? p->d_in : p->H, dd = l == p->L - 1 ? p->n_out : p->H;
+    p->din.push_back(di);
+    p->dout.push_back(dd);
+    p->off_w.push_back(off);
+    off += (int64_t)di * dd;
+    p->off_b.push_back(off);
+    off += dd;
+  }
+  p->n_params = off;
+  for (int l = 0; l < p->L; ++l) {
+    const bool act = l < p->L - 1;
+    const int tl = pow2_log(p->dout[l]);
+    const int ppb = pick_ppb(tl, [&](int n) { return ad_fwd_lds(p->din[l], p->dout[l], J, n, act, ts); });
+    if (ppb < 1) {
+      delete p;
+      return fail("layer too wide for LDS");
+    }
+    p->f_tpp.push_back(tl);
+    p->f_ppb.push_back(ppb);
+    p->f_lds.push_back(ad_fwd_lds(p->din[l], p->dout[l], J, ppb, act, ts));
+    if (act) {
+      const int dn = p->dout[l + 1];
+      const int tb = pow2_log(p->H);
+      const int pb = pick_ppb(tb, [&](int n) { return ad_bwd_lds(p->H, dn, J, n, ts); });
+      if (pb < 1) {
+        delete p;
+        return fail("layer too wide for LDS");
+      }
+      p->b_tpp.push_back(tb);
+      p->b_ppb.push_back(pb);
+      p->b_lds.push_back(ad_bwd_lds(p->H, dn, J, pb, ts));
+    }
+  }
+  p->n_res_blocks = (int)((p->n_entries + AD_THREADS - 1) / AD_THREADS);
+  const int64_t K = p->P * J;
+  p->n_chunks = (K + AD_WG_CHUNK - 1) / AD_WG_CHUNK;
+
+  // workspace
+  size_t w = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = w;
+    w = align_up(w + bytes, 256);
+    return o;
+  };
+  const size_t jet_row = (size_t)p->P * J * ts;
+  p->off_in0 = take(jet_row * p->d_in);
+  for (int l = 0; l < p->L - 1; ++l) {
+    p->off_z.push_back(take(jet_row * p->H));
+    p->off_a.push_back(take(jet_row * p->H));
+    p->off_dz.push_back(take(jet_row * p->H));
+  }
+  p->off_u = take(jet_row * p->n_out);
+  p->off_seed = take(jet_row * p->n_out);
+  p->off_du = take(jet_row * p->n_out);
+  p->off_lp = take((size_t)2 * p->n_res_blocks * ts);
+  int max_ne = 0;
+  for (int l = 0; l < p->L; ++l) max_ne = std::max(max_ne, p->dout[l] * (p->din[l] + 1));
+  p->off_part = take((size_t)p->n_chunks * max_ne * ts);
+  p->off_grad = take((size_t)p->n_params * ts);
+  p->off_one = take(ts);
+  p->total = w;
+
+  // tables: reals and data in the working precision
+  int rc = p->ints.upload(d->ints, (size_t)d->n_ints * 4);
+  if (rc == TDX_OK) {
+    if (d->dtype == 0) {
+      std::vector<float> r(d->reals, d->reals + d->n_reals);
+      rc = p->reals.upload(r.data(), r.size() * 4);
+    } else {
+      rc = p->reals.upload(d->reals, (size_t)d->n_reals * 8);
+    }
+  }
+  if (rc == TDX_OK && d->n_data > 0) rc = p->data.upload(d->data, (size_t)d->n_data * ts);
+  if (rc == TDX_OK) rc = p->points.upload(d->points, (size_t)d->n_points * d->d_in * ts);
+  if (rc == TDX_OK) rc = p->gstart.upload(gstart.data(), gstart.size() * 8);
+  if (rc != TDX_OK) {
+    tdx_ad_plan_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return TDX_OK;
+}
+
+extern "C" void tdx_ad_plan_destroy(tdx_ad_plan *p) {
+  if (!p) return;
+  p->ints.release();
+  p->reals.release();
+  p->data.release();
+  p->points.release();
+  p->gstart.release();
+  delete p;
+}
+
+extern "C" int64_t tdx_ad_param_count(const tdx_ad_plan *p) { return p ? p->n_params : -1; }
+extern "C" size_t tdx_ad_workspace_bytes(const tdx_ad_plan *p) { return p ? p->total : 0; }
+
+namespace {
+
+template <typename T>
+T *wsp(void *ws, size_t off) {
+  return reinterpret_cast<T *>(static_cast<unsigned char *>(ws) + off);
+}
+
+template <typename T>
+Tables tables_of(const tdx_ad_plan *p) {
+  Tables t;
+  t.ints = (const int32_t *)p->ints.ptr;
+  t.jet_i = p->jet_i;
+  t.pptr_i = p->pptr_i;
+  t.part_i = p->part_i;
+  return t;
+}
+
+template <typename T>
+int ad_forward(const tdx_ad_plan *p, const T *params, T *loss, T *resid, void *ws, hipStream_t s) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void *)k_ad_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    (void)hipFuncSetAttribute((const void *)k_ad_bwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    attr = true;
+  }
+  const int J = p->J;
+  for (int l = 0; l < p->L; ++l) {
+    FwdArgs<T> a;
+    a.layer = l;
+    a.din = p->din[l];
+    a.dout = p->dout[l];
+    a.act = l < p->L - 1;
+    a.J = J;
+    a.P = p->P;
+    a.ppb = p->f_ppb[l];
+    a.tpp_log = p->f_tpp[l];
+    a.x = (const T *)p->points.ptr;
+    a.in = l == 0 ? nullptr : wsp<T>(ws, p->off_a[l - 1]);
+    a.W = params + p->off_w[l];
+    a.b = params + p->off_b[l];
+    a.in0 = wsp<T>(ws, p->off_in0);
+    a.zout = a.act ? wsp<T>(ws, p->off_z[l]) : nullptr;
+    a.out = a.act ? wsp<T>(ws, p->off_a[l]) : wsp<T>(ws, p->off_u);
+    a.tb = tables_of<T>(p);
+    const unsigned blocks = (unsigned)((p->P + a.ppb - 1) / a.ppb);
+    hipLaunchKernelGGL((k_ad_fwd<T>), dim3(blocks), dim3(AD_THREADS), p->f_lds[l], s, a);
+    TDX_TRY_HIP(hipGetLastError());
+  }
+  ResArgs<T> r;
+  r.J = J;
+  r.n_out = p->n_out;
+  r.n_groups = p->n_groups;
+  r.n_entries = p->n_entries;
+  r.gstart = (const int64_t *)p->gstart.ptr;
+  r.ints = (const int32_t *)p->ints.ptr;
+  r.reals = (const T *)p->reals.ptr;
+  r.data = (const T *)p->data.ptr;
+  r.groups_i = p->groups_i;
+  r.U = wsp<T>(ws, p->off_u);
+  r.seed = wsp<T>(ws, p->off_seed);
+  r.resid = resid;
+  r.lp = wsp<T>(ws, p->off_lp);
+  hipLaunchKernelGGL((k_ad_resid<T>), dim3(p->n_res_blocks), dim3(AD_THREADS), 0, s, r);
+  TDX_TRY_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_ad_loss<T>), dim3(1), dim3(AD_THREADS), 0, s, (const T *)r.lp,
+                     p->n_res_blocks, loss);
+  TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}
+
+template <typename T>
+int ad_backward(const tdx_ad_plan *p, const T *params, const T *dloss, T *grad, void *ws,
+                hipStream_t s) {
+  const int J = p->J;
+  const int64_t nu = p->P * J * p->n_out;
+  T *du = wsp<T>(ws, p->off_du);
+  hipLaunchKernelGGL((k_ad_scale<T>), dim3((unsigned)std::min<int64_t>(4096, (nu + 255) / 256)),
+                     dim3(256), 0, s, (const T *)wsp<T>(ws, p->off_seed), dloss, du, nu);
+  TDX_TRY_HIP(hipGetLastError());
+  for (int l = p->L - 2; l >= 0; --l) {
+    BwdArgs<T> b;
+    b.H = p->H;
+    b.dn = p->dout[l + 1];
+    b.J = J;
+    b.P = p->P;
+    b.ppb = p->b_ppb[l];
+    b.tpp_log = p->b_tpp[l];
+    b.Wn = params + p->off_w[l + 1];
+    b.DN = l == p->L - 2 ? du : wsp<T>(ws, p->off_dz[l + 1]);
+    b.Z = wsp<T>(ws, p->off_z[l]);
+    b.DZ = wsp<T>(ws, p->off_dz[l]);
+    b.tb = tables_of<T>(p);
+    const unsigned blocks = (unsigned)((p->P + b.ppb - 1) / b.ppb);
+    hipLaunchKernelGGL((k_ad_bwd<T>), dim3(blocks), dim3(AD_THREADS), p->b_lds[l], s, b);
+    TDX_TRY_HIP(hipGetLastError());
+  }
+  T *part = wsp<T>(ws, p->off_part);
+  for (int l = 0; l < p->L; ++l) {
+    WgArgs<T> w;
+    w.din = p->din[l];
+    w.dout = p->dout[l];
+    w.J = J;
+    w.K = p->P * J;
+    w.D = l == p->L - 1 ? du : wsp<T>(ws, p->off_dz[l]);
+    w.S = l == 0 ? wsp<T>(ws, p->off_in0) : wsp<T>(ws, p->off_a[l - 1]);
+    w.part = part;
+    const int ne = w.dout * (w.din + 1);
+    const unsigned ytiles = (unsigned)((ne + AD_THREADS * AD_WG_EPT - 1) / (AD_THREADS * AD_WG_EPT));
+    hipLaunchKernelGGL((k_ad_wgrad<T>), dim3((unsigned)p->n_chunks, ytiles), dim3(AD_THREADS), 0,
+                       s, w);
+    TDX_TRY_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_ad_wreduce<T>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s,
+                       (const T *)part, (int)p->n_chunks, w.din, w.dout, grad + p->off_w[l],
+                       grad + p->off_b[l]);
+    TDX_TRY_HIP(hipGetLastError());
+  }
+  return TDX_OK;
+}
+
+}  // namespace
+
+extern "C" int tdx_ad_forward(const tdx_ad_plan *p, const void *params, void *loss, void *resid,
+                              void *ws, tdx_stream_t stream) {
+  TDX_REQUIRE(p && params && loss && ws, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == 0)
+    return ad_forward<float>(p, (const float *)params, (float *)loss, (float *)resid, ws, s);
+  return ad_forward<double>(p, (const double *)params, (double *)loss, (double *)resid, ws, s);
+}
+
+extern "C" int tdx_ad_backward(const tdx_ad_plan *p, const void *params, const void *dloss,
+                               void *dparams, void *ws, tdx_stream_t stream) {
+  TDX_REQUIRE(p && params && dloss && dparams && ws, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == 0)
+    return ad_backward<float>(p, (const float *)params, (const float *)dloss, (float *)dparams, ws, s);
+  return ad_backward<double>(p, (const double *)params, (const double *)dloss, (double *)dparams,
+                             ws, s);
+}
+
+extern "C" int tdx_ad_train_step(const tdx_ad_plan *p, void *params, void *m, void *v,
+                                 int64_t *step, double lr, double b1, double b2, double eps,
+                                 void *loss, void *ws, tdx_stream_t stream) {
+  TDX_REQUIRE(p && params && m && v && step && loss && ws, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  void *grad = static_cast<unsigned char *>(ws) + p->off_grad;
+  void *one = static_cast<unsigned char *>(ws) + p->off_one;
+  if (p->dtype == 0) {
+    hipLaunchKernelGGL((k_ad_one<float>), dim3(1), dim3(64), 0, s, (float *)one);
+    rc = ad_forward<float>(p, (const float *)params, (float *)loss, nullptr, ws, s);
+    if (rc == TDX_OK)
+      rc = ad_backward<float>(p, (const float *)params, (const float *)one, (float *)grad, ws, s);
+  } else {
+    hipLaunchKernelGGL((k_ad_one<double>), dim3(1), dim3(64), 0, s, (double *)one);
+    rc = ad_forward<double>(p, (const double *)params, (double *)loss, nullptr, ws, s);
+    if (rc == TDX_OK)
+      rc = ad_backward<double>(p, (const double *)params, (const double *)one, (double *)grad, ws, s);
+  }
+  if (rc != TDX_OK) return rc;
+  rc = tdx_adam(p->dtype, p->n_params, params, grad, m, v, step, lr, b1, b2, eps, stream);
+  if (rc != TDX_OK) return rc;
+  return tdx_step_increment(step, stream);
+}

@@ -107,7 +107,7 @@ class Model:
         cbs.on_train_end()
 
     def _flat_state(self) -> torch.Tensor:
-        if self.mode == 'NN':
+        if self.mode in ('NN', 'autograd'):
             flat = module_flat_view(self.net)
             if flat is None:
                 flat = flatten_module_params(self.net)

@@ -30,6 +30,8 @@ EXPORTS = [
     'tdx_mat_forward', 'tdx_mat_backward', 'tdx_mat_train_step', 'tdx_mat_train_step_to',
     'tdx_adam', 'tdx_step_increment', 'tdx_quotient_check',
     'tdx_mat_specialized_points', 'tdx_mat_specialized_source',
+    'tdx_ad_plan_create', 'tdx_ad_plan_destroy', 'tdx_ad_param_count',
+    'tdx_ad_workspace_bytes', 'tdx_ad_forward', 'tdx_ad_backward', 'tdx_ad_train_step',
 ]
 
 
@@ -78,6 +80,21 @@ class MatDesc(ctypes.Structure):
     ]
 
 
+class ADDesc(ctypes.Structure):
+    _fields_ = [
+        ('dtype', ctypes.c_int), ('d_in', ctypes.c_int), ('n_out', ctypes.c_int),
+        ('width', ctypes.c_int), ('n_layers', ctypes.c_int),
+        ('n_jets', ctypes.c_int), ('jet_i', ctypes.c_int), ('pptr_i', ctypes.c_int),
+        ('part_i', ctypes.c_int),
+        ('n_points', ctypes.c_int64), ('points', ctypes.c_void_p),
+        ('n_groups', ctypes.c_int), ('groups_i', ctypes.c_int),
+        ('n_ints', ctypes.c_int64), ('ints', c_i32p),
+        ('n_reals', ctypes.c_int64), ('reals', c_dp),
+        ('n_data', ctypes.c_int64), ('data', ctypes.c_void_p),
+        ('n_resid', ctypes.c_int64),
+    ]
+
+
 _lib: Optional[ctypes.CDLL] = None
 _load_error: Optional[str] = None
 
@@ -115,6 +132,16 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.tdx_mat_backward.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.tdx_mat_train_step.argtypes = [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]
     lib.tdx_mat_train_step_to.argtypes = [vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]
+    lib.tdx_ad_plan_create.argtypes = [ctypes.POINTER(ADDesc), ctypes.POINTER(vp)]
+    lib.tdx_ad_plan_destroy.argtypes = [vp]
+    lib.tdx_ad_plan_destroy.restype = None
+    lib.tdx_ad_param_count.argtypes = [vp]
+    lib.tdx_ad_param_count.restype = i64
+    lib.tdx_ad_workspace_bytes.argtypes = [vp]
+    lib.tdx_ad_workspace_bytes.restype = sz
+    lib.tdx_ad_forward.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.tdx_ad_backward.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.tdx_ad_train_step.argtypes = [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]
     lib.tdx_adam.argtypes = [ctypes.c_int, i64, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp]
     lib.tdx_step_increment.argtypes = [vp, vp]
 

@@ -1,6 +1,7 @@
 """``Solution``: the per-step objective, served by the native engines.
 
-Drop-in for tedeous/solution.py:19-168.  ``evaluate()`` returns
+Drop-in for tedeous/solution.py:19-168 in the 'NN', 'mat' and 'autograd'
+modes.  ``evaluate()`` returns
 ``(loss, loss_normalized)`` with shape ``[1]`` each, like the reference
 (tedeous/losses.py:119-135), and differentiates through the native adjoint.
 The reference's side outputs ``op``, ``bval``, ``true_bval``, ``bval_keys``
@@ -12,7 +13,8 @@ from typing import List, Optional, Tuple, Union
 
 import torch
 
-from .compiler import MatProblem, NNProblem, compile_mat, compile_nn
+from .ad_engine import ADEngine
+from .compiler import ADProblem, MatProblem, NNProblem, compile_ad, compile_mat, compile_nn
 from .device import check_device
 from .mat_engine import MatEngine
 from .nn_engine import MLPArch, NNEngine, mlp_arch
@@ -36,9 +38,9 @@ class Operator_bcond_preproc:
         self.mode = None
 
     def set_strategy(self, strategy: str) -> 'Operator_bcond_preproc':
-        if strategy not in ('NN', 'mat'):
+        if strategy not in ('NN', 'mat', 'autograd'):
             raise NotImplementedError(
-                f"mode {strategy!r} is not served by this engine (only 'NN' and 'mat')")
+                f"mode {strategy!r} is not served by this engine")
         self.mode = strategy
         return self
 
@@ -68,7 +70,7 @@ class Solution:
             # residual with the 'NN' point-type lookup and fails
             # (tedeous/solution.py:54, tedeous/losses.py:160-161)
             raise NotImplementedError("mini-batching with a causal loss is not served")
-        if mode not in ('NN', 'mat'):
+        if mode not in ('NN', 'mat', 'autograd'):
             raise NotImplementedError(f"mode {mode!r} is not served by this engine")
         self.grid = grid
         self.mode = mode
@@ -86,6 +88,12 @@ class Solution:
                 inner_order=equal_cls.inner_order, boundary_order=equal_cls.boundary_order,
                 lambda_operator=lambda_operator, lambda_bound=lambda_bound, tol=tol)
             self.engine = NNEngine([self.problem], self.arch, device)
+        elif mode == 'autograd':
+            self.arch = mlp_arch(model)
+            self.problem: ADProblem = compile_ad(
+                grid, ops, equal_cls.bconds, n_out=self.arch.n_out,
+                lambda_operator=lambda_operator, lambda_bound=lambda_bound)
+            self.engine = ADEngine(self.problem, self.arch, device)
         else:
             self.problem: MatProblem = compile_mat(
                 grid, ops, equal_cls.bconds, n_eq=len(ops), derivative_points=derivative_points,
@@ -101,7 +109,7 @@ class Solution:
 
     def evaluate(self, save_graph: bool = True) -> Tuple[torch.Tensor, torch.Tensor]:
         """``(loss, loss_normalized)``, each of shape ``[1]``."""
-        if self.mode == 'NN':
+        if self.mode in ('NN', 'autograd'):
             pair = self.engine.loss(self._flat_params())[0]
         else:
             u = self.model
@@ -119,7 +127,28 @@ class Solution:
     def _side(self):
         if self.mode == 'NN':
             return self._side_nn()
+        if self.mode == 'autograd':
+            return self._side_ad()
         return self._side_mat()
+
+    def _side_ad(self):
+        eng: ADEngine = self.engine
+        with torch.no_grad():
+            resid = eng.residuals(self._flat_params().detach())
+        prob = self.problem
+        cols = [resid[off:off + cnt] for (_, _, off, cnt, col, _) in eng.tables.resid if col < 0]
+        op = torch.stack(cols, 1) if len(cols) > 1 else cols[0].reshape(-1, 1)
+        ncol = len(prob.bc_keys)
+        bval = torch.zeros(prob.bc_max_len, max(ncol, 1), dtype=resid.dtype, device=resid.device)
+        true = torch.zeros_like(bval)
+        for (gi, qi, off, cnt, col, col_pos) in eng.tables.resid:
+            if col < 0:
+                continue
+            pos = torch.as_tensor(col_pos, device=resid.device)
+            bval[pos, col] = resid[off:off + cnt]
+            t = prob.groups[gi].equations[qi].target
+            true[pos, col] = torch.as_tensor(t, dtype=resid.dtype, device=resid.device)
+        return op, bval[:, :ncol], true[:, :ncol], list(prob.bc_keys), list(prob.bc_lengths)
 
     def _side_nn(self):
         eng: NNEngine = self.engine
