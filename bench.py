@@ -23,8 +23,12 @@ instance.  Reported next to it, measured in the same run:
   KdV and Legendre), for a spread of instances.
 
 Other workloads (--workload): burgers, kdv, legendre ('NN', one instance),
-wave (the 128 x 128 'mat' problem) and wave16k (the wave operator on a
-16384 x 16384 field, the HBM-roofline case of the stencil kernels).  The
+wave (the 128 x 128 'mat' problem), wave16k (the wave operator on a
+16384 x 16384 field, the HBM-roofline case of the stencil kernels) and
+poisson_ad (the 'autograd' mode: a 2-D Poisson problem on 201 x 201 points
+with the 2-100x5-1 network of the reference's Poisson examples; roofline of
+the whole fused step against FP32 matrix peak from the algorithmic FLOPs of
+its linear jet maps).  The
 'mat' step reads one grid buffer and writes the other (as Model.train does),
 so the kernel generated for the grid interior applies Adam as it forms each
 gradient; its roofline is that whole step at 24 B per point, the residual +
@@ -475,6 +479,122 @@ def bench_mat(args):
     return out
 
 
+# ---------------------------------------------------------------------------
+# 'autograd' workload
+# ---------------------------------------------------------------------------
+
+def ad_flops_per_step(eng) -> float:
+    """Algorithmic FLOPs of one fused 'autograd' step: per point and jet
+    component, every layer's linear map forward (2 din dout), its transpose
+    backward for layers above the first (2 din dout) and its weight-gradient
+    product (2 (din + 1) dout)."""
+    a = eng.arch
+    dims = [(a.d_in if l == 0 else a.width, a.n_out if l == a.n_layers - 1 else a.width)
+            for l in range(a.n_layers)]
+    rows = eng.tables.n_points * eng.tables.n_jets
+    fwd = sum(2 * di * do for di, do in dims)
+    bwd = sum(2 * di * do for di, do in dims[1:])
+    wg = sum(2 * (di + 1) * do for di, do in dims)
+    return float(rows) * (fwd + bwd + wg)
+
+
+def bench_ad(args):
+    import copy
+    from torch_de_solver_amd import workloads as W
+    from torch_de_solver_amd.ad_engine import ADEngine
+    from torch_de_solver_amd.compiler import compile_ad
+    from torch_de_solver_amd.nn_engine import flatten_module_params, mlp_arch
+    from torch_de_solver_amd.plain import PlainAD
+    dom, eq, conds, net0, kw = W.poisson_ad(args.grid_res)
+    grid = dom.build('autograd')
+    prob = compile_ad(grid, eq.equation_lst, conds.build(dom.variable_dict), 1,
+                      lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
+    dev = torch.device('cuda')
+    arch = mlp_arch(net0)
+    eng = ADEngine(prob, arch, dev)
+    net = copy.deepcopy(net0).to(dev)
+    params = flatten_module_params(net)
+    m, v = torch.zeros_like(params), torch.zeros_like(params)
+    step = torch.zeros(1, dtype=torch.int64, device=dev)
+    loss = torch.zeros(2, device=dev)
+    lr = 1e-3
+
+    def one():
+        eng.train_step(params, m, v, step, lr, 0.9, 0.999, 1e-8, loss)
+    one()
+    one()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        one()
+    for _ in range(args.warmup):
+        g.replay()
+    ms = time_cuda(g.replay, args.steps)
+    # passes of the step, timed on their own (the stream they run on)
+    dl = torch.ones(1, device=dev)
+    grad = torch.empty_like(params)
+    fwd_ms = time_cuda(lambda: eng.forward(params, loss), args.steps)
+    bwd_ms = time_cuda(lambda: eng.backward(params, dl, grad), args.steps)
+    step_ms = time_cuda(one, args.steps)
+    flops = ad_flops_per_step(eng)
+    N = prob.n_grid
+    out = {
+        "metric": "train steps/s & collocation pts/s, fused+adjoint vs plain GPU, at equal rel-L2",
+        "value": 1e3 / ms, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"Poisson 'autograd' {args.grid_res + 1}x{args.grid_res + 1}, "
+                               "MLP 2-100x5-1, Adam lr 1e-3",
+                   "collocation_points": N, "points_with_conditions": eng.tables.n_points,
+                   "jets": [list(j) for j in prob.jets], "parallelism": "single GPU"},
+        "colloc_pts_per_s": N * 1e3 / ms,
+        "roofline": {"bound": "mfma",
+                     "kernel": "fused train step: k_ad_fwd, k_ad_resid, k_ad_bwd, k_ad_wgrad, "
+                               "k_ad_wreduce, k_adam (linear jet maps on FP32 FMA)",
+                     "achieved": flops / (step_ms * 1e-3) / 1e12,
+                     "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": flops / (step_ms * 1e-3) / FP32_MFMA_PEAK, "traffic": None,
+                     "algorithmic_flops_per_launch": flops, "avg_launch_ms": step_ms},
+        "forward_pass_ms": fwd_ms, "backward_pass_ms": bwd_ms,
+    }
+    if not args.no_plain:
+        pnet = copy.deepcopy(net0).to(dev)
+        pl = PlainAD(prob, pnet, dev)
+        opt = torch.optim.Adam(pnet.parameters(), lr=lr)
+
+        def pstep():
+            opt.zero_grad()
+            l = pl.loss()
+            l.backward()
+            opt.step()
+        pstep()
+        pms = time_cuda(pstep, max(args.plain_steps, 3))
+        out["plain_gpu"] = {"value": 1e3 / pms, "unit": "steps/s", "ms_per_step": pms,
+                            "what": "the reference's nested torch.autograd.grad chains in "
+                                    "stock torch ops on the same GPU"}
+        out["speedup_vs_plain_gpu"] = pms / ms
+    if not args.no_cpu:
+        from oracle import tedeous_oracle as O
+        torch.set_num_threads(cpu_cores())
+        orc = O.ADOracle(dict(dom.variable_dict), eq.equation_lst,
+                         [dict(c) for c in conds.conditions_lst])
+        cnet = copy.deepcopy(net0)
+        opt = torch.optim.Adam(cnet.parameters(), lr=lr)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 20) or k < 1:
+            opt.zero_grad()
+            l = orc.evaluate(cnet, kw['lambda_operator'], kw['lambda_bound'])[0]
+            l.backward()
+            opt.step()
+            k += 1
+        cms = (time.perf_counter() - t0) * 1e3 / k
+        out["cpu_baseline"] = {"value": 1e3 / cms, "unit": "steps/s", "cores": cpu_cores(),
+                               "kind": "port",
+                               "sample": f"{k} full Adam steps of the CPU oracle (nested autograd)"}
+    return out
+
+
 def calibrate(args):
     dev = torch.device('cuda')
     n = 1 << 28                      # 2^28 float32 = 1 GiB
@@ -500,7 +620,10 @@ def main():
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='sweep',
-                    choices=['sweep', 'burgers', 'kdv', 'legendre', 'wave', 'wave16k'])
+                    choices=['sweep', 'burgers', 'kdv', 'legendre', 'wave', 'wave16k',
+                             'poisson_ad'])
+    ap.add_argument('--grid-res', type=int, default=200,
+                    help="'poisson_ad': intervals per axis (points per axis minus one)")
     ap.add_argument('--plain-steps', type=int, default=2)
     ap.add_argument('--plain-instances', type=int, default=256)
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
@@ -536,6 +659,8 @@ def main():
         out = calibrate(args)
     elif args.workload in ('wave', 'wave16k'):
         out = bench_mat(args)
+    elif args.workload == 'poisson_ad':
+        out = bench_ad(args)
     else:
         out = bench_nn(args)
     print(json.dumps(out), flush=True)

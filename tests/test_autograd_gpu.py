@@ -102,11 +102,13 @@ def test_ad_float32_against_float64_oracle(name):
     assert gn < 1e-3, gn
 
 
-def test_ad_wide_network_against_oracle():
-    """Width 100 (the reference examples' width; several points per
-    workgroup do not divide it) and four hidden layers."""
-    _print("[ad f64] burgers_ad width 100")
-    layers = [2, 100, 100, 100, 100, 1]
+@pytest.mark.parametrize('width', [100, 160])
+def test_ad_wide_network_against_oracle(width):
+    """Width 100 (the reference examples' width; its 16-wide MFMA tiles are
+    partly padding) and 160 (inputs wider than 128: 64 k-steps of weight
+    fragments held per lane), four hidden layers."""
+    _print(f"[ad f64] burgers_ad width {width}")
+    layers = [2, width, width, width, width, 1]
     ref = _oracle('burgers_ad', _net(layers, torch.float64))
     net_dev = _net(layers, torch.float64).to(DEV)
     sol, _ = _engine('burgers_ad', net_dev, 'float64')

@@ -262,3 +262,77 @@ def test_specialized_kernel_not_for_1d_or_double():
     prob = compile_mat(grid, ops, pre.bconds, n_eq=1, derivative_points=2,
                        lambda_operator=1, lambda_bound=10)
     assert MatTables(prob, np.float32).specialized_source(1) == ''
+
+
+def _ad_desc(name='wave_ad'):
+    from torch_de_solver_amd.ad_engine import ADTables
+    from torch_de_solver_amd.compiler import compile_ad
+    builder, layers = problems.AD_CASES[name]
+    d, eq, b, kw = builder(api)
+    prob = compile_ad(d.build('autograd'), eq.equation_lst, b.build(d.variable_dict),
+                      n_out=layers[-1], lambda_operator=kw['lambda_operator'],
+                      lambda_bound=kw['lambda_bound'])
+    t = ADTables(prob, np.float64)
+    desc = native.ADDesc()
+    desc.dtype, desc.d_in, desc.n_out, desc.width, desc.n_layers = 1, layers[0], layers[-1], \
+        layers[1], len(layers) - 1
+    desc.n_jets, desc.jet_i, desc.pptr_i, desc.part_i = t.n_jets, t.jet_i, t.pptr_i, t.part_i
+    desc.n_points, desc.points = t.n_points, t.points.ctypes.data
+    desc.n_groups, desc.groups_i = len(prob.groups), t.groups_i
+    desc.n_ints, desc.ints = t.ints.size, t.ints.ctypes.data_as(native.c_i32p)
+    desc.n_reals, desc.reals = t.reals.size, t.reals.ctypes.data_as(native.c_dp)
+    desc.n_data, desc.data = t.data.size, t.data.ctypes.data
+    desc.n_resid = t.n_resid
+    return prob, t, desc
+
+
+def test_ad_jets_and_partitions():
+    """Jets are closed under sub-multisets and sorted by order; each jet's
+    partitions are its set partitions (Bell numbers 1, 2, 5, 15) with blocks
+    that multiply back to the jet."""
+    import itertools
+    for name in problems.AD_CASES:
+        prob, t, _ = _ad_desc(name)
+        jets = prob.jets
+        assert jets[0] == ()
+        assert [len(j) for j in jets] == sorted(len(j) for j in jets)
+        for j in jets:
+            for r in range(len(j)):
+                for sub in itertools.combinations(j, r):
+                    assert tuple(sorted(sub)) in jets
+        for j, parts in zip(jets, prob.partitions()):
+            assert len(parts) == [1, 1, 2, 5, 15][len(j)] if j else parts == []
+            for blocks in parts:
+                merged = tuple(sorted(a for b in blocks for a in jets[b]))
+                assert merged == j
+    prob, _, _ = _ad_desc('beam_ad')
+    assert prob.jets[-1] == (0, 0, 0, 0)
+
+
+def test_ad_plan_validation_rejects_bad_tables():
+    """The 'autograd' plan checks every table index before any device work."""
+    lib = native.lib()
+    prob, t, desc = _ad_desc()
+    h = ctypes.c_void_p()
+    keep = t.ints.copy()
+    # a stencil entry pointing past the jet list
+    bad = keep.copy()
+    g0 = bad[t.groups_i: t.groups_i + 5]
+    eq0 = bad[g0[4]: g0[4] + 6]
+    term0 = bad[eq0[1]: eq0[1] + 4]
+    fac0 = bad[term0[1]: term0[1] + 5]
+    bad[fac0[1] + 1] = t.n_jets
+    desc.ints = bad.ctypes.data_as(native.c_i32p)
+    assert lib.tdx_ad_plan_create(ctypes.byref(desc), ctypes.byref(h)) == -1 and not h.value
+    assert b'stencil entry' in lib.tdx_last_error()
+    # a point claimed by two entries
+    bad = keep.copy()
+    g1 = t.groups_i + 5
+    bad[bad[g1 + 2]] = 0
+    desc.ints = bad.ctypes.data_as(native.c_i32p)
+    assert lib.tdx_ad_plan_create(ctypes.byref(desc), ctypes.byref(h)) == -1
+    assert b'two entries' in lib.tdx_last_error()
+    desc.ints = keep.ctypes.data_as(native.c_i32p)
+    desc.width = 300
+    assert lib.tdx_ad_plan_create(ctypes.byref(desc), ctypes.byref(h)) == -1
+    assert lib.tdx_ad_plan_create(None, ctypes.byref(h)) == -1

@@ -34,8 +34,7 @@ namespace {
 
 constexpr int AD_THREADS = 256;
 constexpr int AD_MAXF = 4;        // factors per term
-constexpr int AD_WG_EPT = 16;     // weight-gradient elements per thread
-constexpr int AD_WG_CHUNK = 2048; // rows (point, jet) per reduction chunk
+constexpr int AD_WG_CHUNK = 512;  // rows (point, jet) per reduction chunk
 
 template <typename T>
 struct TanhDerivs {
@@ -72,89 +71,175 @@ __device__ __forceinline__ T deriv_n(const T s1, const T s2, const T s3, const T
 struct Tables {
   const int32_t *ints;
   int jet_i, pptr_i, part_i;
+  int J, n_parts;  // LDS copy: jet (2J) | pptr (J + 1) | parts (5 n_parts) ints
 };
+
+// Copies the jet and partition tables into LDS (the epilogues read them for
+// every point and unit); returns a view with the same offsets into the copy.
+__device__ __forceinline__ Tables tables_to_lds(const Tables &t, int32_t *lds) {
+  const int nj = 2 * t.J, np = t.J + 1, nr = TDX_AD_PART_INTS * t.n_parts;
+  for (int k = threadIdx.x; k < nj + np + nr; k += AD_THREADS)
+    lds[k] = k < nj ? t.ints[t.jet_i + k]
+                    : (k < nj + np ? t.ints[t.pptr_i + k - nj] : t.ints[t.part_i + k - nj - np]);
+  Tables l = t;
+  l.ints = lds;
+  l.jet_i = 0;
+  l.pptr_i = nj;
+  l.part_i = nj + np;
+  return l;
+}
+__host__ __device__ inline size_t tables_lds_bytes(int J, int n_parts) {
+  return (size_t)(2 * J + J + 1 + TDX_AD_PART_INTS * n_parts) * 4;
+}
+
+// Copies rows [0, rows) of a row-major [*][ncols] array (row r at src +
+// r * ncols) into LDS with leading dimension ld, zeros for r >= valid.
+// Eight independent loads per thread are issued before their stores, so a
+// workgroup waits for HBM once per batch instead of once per element.
+template <typename T>
+__device__ __forceinline__ void stage_rows(T *dst, int ld, const T *src, int rows, int valid,
+                                           int ncols) {
+  constexpr int NB = 8;
+  const int total = rows * ncols;
+  for (int k0 = threadIdx.x; k0 < total; k0 += AD_THREADS * NB) {
+    T v[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int k = k0 + u * AD_THREADS;
+      v[u] = (k < total && k < valid * ncols) ? src[k] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int k = k0 + u * AD_THREADS;
+      if (k < total) {
+        const int r = k / ncols;
+        dst[r * ld + (k - r * ncols)] = v[u];
+      }
+    }
+  }
+}
 
 // ---------------------------------------------------------------------------
 // forward: one layer
 // ---------------------------------------------------------------------------
+// A workgroup takes PB points, i.e. R = PB * J rows (point, jet) of the
+// layer's input jets, staged in LDS.  The linear map of every row is one
+// GEMM, Z[R][dout] = IN[R][din] * W^T, on 16x16x4 MFMA tiles (A operand
+// from LDS, B operand from the transposed weights W^T[din][dout] in HBM/L2,
+// 64-byte rows per k).  The epilogue adds the bias to the value row and
+// forms the activation jets per (point, unit) by Faa di Bruno's formula.
 template <typename T>
 struct FwdArgs {
   int layer, din, dout, act, J;
   int64_t P;
-  int ppb, tpp_log;
+  int pb, rpad;    // points per workgroup, rows rounded up to 16
   const T *x;      // layer 0: points [P][din]
   const T *in;     // layer > 0: input jets [P][J][din]
-  const T *W, *b;  // [dout][din], [dout]
+  const T *WT, *b; // [din][dout], [dout]
   T *in0;          // layer 0: input jets written here
   T *zout;         // pre-activation jets (act) [P][J][dout]
   T *out;          // activation jets (act) or output jets [P][J][dout]
   Tables tb;
 };
 
-template <typename T>
+template <typename T, int KM>
 __global__ __launch_bounds__(AD_THREADS) void k_ad_fwd(FwdArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T *Wt = reinterpret_cast<T *>(smem_raw);        // [din][dout]
-  T *IN = Wt + (size_t)a.din * a.dout;             // [ppb][J][din]
-  T *Z = IN + (size_t)a.ppb * a.J * a.din;         // [ppb][J][dout]
-  const int tid = threadIdx.x;
   const int J = a.J, din = a.din, dout = a.dout;
-  const int64_t p0 = (int64_t)blockIdx.x * a.ppb;
-  const int npts = (int)std::min<int64_t>(a.ppb, a.P - p0);
-  for (int k = tid; k < din * dout; k += AD_THREADS) {
-    const int i = k / din, j = k - (k / din) * din;
-    Wt[j * dout + i] = a.W[k];
-  }
-  const int per_pt = J * din;
-  for (int k = tid; k < npts * per_pt; k += AD_THREADS) {
-    T v;
-    if (a.layer == 0) {
-      const int pp = k / per_pt, r = k - pp * per_pt;
-      const int al = r / din, j = r - al * din;
-      const int ord = a.tb.ints[a.tb.jet_i + 2 * al];
-      const int ax = a.tb.ints[a.tb.jet_i + 2 * al + 1];
-      v = al == 0 ? a.x[(p0 + pp) * din + j] : (ord == 1 && ax == j ? T(1) : T(0));
-      a.in0[p0 * per_pt + k] = v;
-    } else {
-      v = a.in[p0 * per_pt + k];
+  const int ldi = din + 1, ldz = dout + 1;
+  T *IN = reinterpret_cast<T *>(smem_raw);       // [rpad][ldi]
+  T *Z = IN + (size_t)a.rpad * ldi;              // [rpad][ldz]
+  const Tables tb = tables_to_lds(a.tb, reinterpret_cast<int32_t *>(Z + (size_t)a.rpad * ldz));
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t p0 = (int64_t)blockIdx.x * a.pb;
+  const int npts = (int)std::min<int64_t>(a.pb, a.P - p0);
+  const int R = npts * J;
+  __syncthreads();
+  if (a.layer == 0) {
+    for (int k = tid; k < a.rpad * din; k += AD_THREADS) {
+      const int r = k / din, j = k - (k / din) * din;
+      T v = T(0);
+      if (r < R) {
+        const int pp = r / J, al = r - (r / J) * J;
+        const int ord = tb.ints[tb.jet_i + 2 * al];
+        const int ax = tb.ints[tb.jet_i + 2 * al + 1];
+        v = al == 0 ? a.x[(p0 + pp) * din + j] : (ord == 1 && ax == j ? T(1) : T(0));
+        a.in0[p0 * J * din + k] = v;
+      }
+      IN[r * ldi + j] = v;
     }
-    IN[k] = v;
+  } else {
+    stage_rows(IN, ldi, a.in + p0 * J * din, a.rpad, R, din);
   }
   __syncthreads();
-  const int pp = tid >> a.tpp_log, i = tid & ((1 << a.tpp_log) - 1);
-  if (pp >= npts || i >= dout) return;
-  const int64_t p = p0 + pp;
-  const T *inp = IN + (size_t)pp * per_pt;
-  T *zl = Z + (size_t)pp * J * dout;
-  for (int al = 0; al < J; ++al) {
-    T acc = T(0);
-    for (int j = 0; j < din; ++j) acc = acc + Wt[j * dout + i] * inp[al * din + j];
-    if (al == 0) acc = acc + a.b[i];
-    const size_t o = ((size_t)p * J + al) * dout + i;
-    if (a.act) {
-      zl[al * dout + i] = acc;
-      a.zout[o] = acc;
-    } else {
-      a.out[o] = acc;
-    }
-  }
-  if (!a.act) return;
-  const TanhDerivs<T> s = tanh_derivs(zl[i]);
-  a.out[((size_t)p * J) * dout + i] = s.s0;
-  for (int al = 1; al < J; ++al) {
-    const int k0 = a.tb.ints[a.tb.pptr_i + al], k1 = a.tb.ints[a.tb.pptr_i + al + 1];
-    T h = T(0);
-    for (int k = k0; k < k1; ++k) {
-      const int32_t *pr = a.tb.ints + a.tb.part_i + TDX_AD_PART_INTS * k;
-      const int m = pr[0];
-      T prod = deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m);
+  // GEMM: each wavefront takes column tiles of 16 outputs, holds their B
+  // fragments (W^T, all k) in registers and sweeps the row tiles
+  const int ntc = (dout + 15) >> 4;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int c = wave; c < ntc; c += AD_THREADS / 64) {
+    const int n = c * 16 + lr;
+    T bf[KM];
 #pragma unroll
-      for (int q = 0; q < TDX_AD_MAX_ORDER; ++q)
-        if (q < m) prod = prod * zl[pr[1 + q] * dout + i];
-      h = (k == k0) ? prod : h + prod;
+    for (int u = 0; u < KM; ++u) {
+      const int k = 4 * u + lk;
+      bf[u] = (4 * u < din && k < din && n < dout) ? a.WT[(size_t)k * dout + n] : T(0);
     }
-    a.out[((size_t)p * J + al) * dout + i] = h;
+    for (int r0 = 0; r0 < a.rpad; r0 += 16) {
+      typename V4<T>::type acc = {T(0), T(0), T(0), T(0)};
+      const T *row = IN + (r0 + lr) * ldi + lk;
+#pragma unroll
+      for (int u = 0; u < KM; ++u) {
+        if (4 * u < din) {
+          const T av = 4 * u + lk < din ? row[4 * u] : T(0);
+          acc = mma16(av, bf[u], acc);
+        }
+      }
+      if (n < dout) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Z[(r0 + crow<T>(lk, i)) * ldz + n] = acc[i];
+      }
+    }
   }
+  __syncthreads();
+  // a uniform loop: every lane runs the same number of rounds
+  for (int base = 0; base < npts * dout; base += AD_THREADS) {
+    const int item = base + tid;
+    if (item >= npts * dout) continue;
+    const int pp = item / dout, i = item - (item / dout) * dout;
+    const int64_t p = p0 + pp;
+    T *zl = Z + (size_t)pp * J * ldz;
+    zl[i] = zl[i] + a.b[i];
+    if (!a.act) {
+      for (int al = 0; al < J; ++al) a.out[((size_t)p * J + al) * dout + i] = zl[al * ldz + i];
+      continue;
+    }
+    for (int al = 0; al < J; ++al) a.zout[((size_t)p * J + al) * dout + i] = zl[al * ldz + i];
+    const TanhDerivs<T> s = tanh_derivs(zl[i]);
+    a.out[((size_t)p * J) * dout + i] = s.s0;
+    for (int al = 1; al < J; ++al) {
+      const int k0 = tb.ints[tb.pptr_i + al], k1 = tb.ints[tb.pptr_i + al + 1];
+      T h = T(0);
+      for (int k = k0; k < k1; ++k) {
+        const int32_t *pr = tb.ints + tb.part_i + TDX_AD_PART_INTS * k;
+        const int m = pr[0];
+        T prod = deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m);
+#pragma unroll
+        for (int q = 0; q < TDX_AD_MAX_ORDER; ++q)
+          if (q < m) prod = prod * zl[pr[1 + q] * ldz + i];
+        h = (k == k0) ? prod : h + prod;
+      }
+      a.out[((size_t)p * J + al) * dout + i] = h;
+    }
+  }
+}
+
+// W^T of one layer: WT[k][n] = W[n][k]
+template <typename T>
+__global__ void k_ad_transpose(const T *W, int dout, int din, T *WT) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= dout * din) return;
+  const int n = e / din, k = e - (e / din) * din;
+  WT[(size_t)k * dout + n] = W[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -319,11 +404,15 @@ __global__ void k_ad_one(T *x) {
 // ---------------------------------------------------------------------------
 // backward: one hidden layer
 // ---------------------------------------------------------------------------
+// dloss/d(activation jets) of PB points: dA[R][H] = DN[R][dn] * Wn[dn][H]
+// on MFMA tiles (A operand: the next layer's pre-activation adjoints staged
+// in LDS; B operand: Wn rows from HBM/L2), then the adjoint of the tanh
+// jet map per (point, unit) into DZ.
 template <typename T>
 struct BwdArgs {
   int H, dn, J;
   int64_t P;
-  int ppb, tpp_log;
+  int pb, rpad;
   const T *Wn;   // next layer's weights [dn][H]
   const T *DN;   // dloss/d(next pre-activation jets) [P][J][dn]
   const T *Z;    // this layer's pre-activation jets [P][J][H]
@@ -331,71 +420,101 @@ struct BwdArgs {
   Tables tb;
 };
 
-template <typename T>
+template <typename T, int KM>
 __global__ __launch_bounds__(AD_THREADS) void k_ad_bwd(BwdArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int J = a.J, H = a.H, dn = a.dn;
-  T *Wn = reinterpret_cast<T *>(smem_raw);          // [dn][H]
-  T *DN = Wn + (size_t)dn * H;                       // [ppb][J][dn]
-  T *ZS = DN + (size_t)a.ppb * J * dn;               // [ppb][J][H]
-  T *DZ = ZS + (size_t)a.ppb * J * H;                // [ppb][J][H]
-  const int tid = threadIdx.x;
-  const int64_t p0 = (int64_t)blockIdx.x * a.ppb;
-  const int npts = (int)std::min<int64_t>(a.ppb, a.P - p0);
-  for (int k = tid; k < dn * H; k += AD_THREADS) Wn[k] = a.Wn[k];
-  for (int k = tid; k < npts * J * dn; k += AD_THREADS) DN[k] = a.DN[p0 * J * dn + k];
-  for (int k = tid; k < npts * J * H; k += AD_THREADS) {
-    ZS[k] = a.Z[p0 * J * H + k];
-    DZ[k] = T(0);
+  const int ldd = dn + 1, ldh = H + 1;
+  T *DN = reinterpret_cast<T *>(smem_raw);        // [rpad][ldd]
+  T *DA = DN + (size_t)a.rpad * ldd;              // [rpad][ldh]
+  T *DZ = DA + (size_t)a.rpad * ldh;              // [rpad][ldh]
+  T *ZS = DZ + (size_t)a.rpad * ldh;              // [rpad][ldh]
+  const Tables tb = tables_to_lds(a.tb, reinterpret_cast<int32_t *>(ZS + (size_t)a.rpad * ldh));
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t p0 = (int64_t)blockIdx.x * a.pb;
+  const int npts = (int)std::min<int64_t>(a.pb, a.P - p0);
+  const int R = npts * J;
+  stage_rows(DN, ldd, a.DN + p0 * J * dn, a.rpad, R, dn);
+  stage_rows(ZS, ldh, a.Z + p0 * J * H, R, R, H);
+  for (int k = tid; k < R * H; k += AD_THREADS) {
+    const int r = k / H, j = k - (k / H) * H;
+    DZ[r * ldh + j] = T(0);
   }
   __syncthreads();
-  const int pp = tid >> a.tpp_log, j = tid & ((1 << a.tpp_log) - 1);
-  if (pp >= npts || j >= H) return;
-  const T *dnp = DN + (size_t)pp * J * dn;
-  const T *zl = ZS + (size_t)pp * J * H;
-  T *dz = DZ + (size_t)pp * J * H;
-  const TanhDerivs<T> s = tanh_derivs(zl[j]);
-  T dz0 = T(0);
-  for (int al = 0; al < J; ++al) {
-    // dloss/d(activation jet al) at unit j
-    T da = T(0);
-    for (int i = 0; i < dn; ++i) da = da + Wn[i * H + j] * dnp[al * dn + i];
-    if (al == 0) {
-      dz0 = dz0 + da * s.s1;
-      continue;
+  const int ntc = (H + 15) >> 4;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int c = wave; c < ntc; c += AD_THREADS / 64) {
+    const int n = c * 16 + lr;
+    T bf[KM];
+#pragma unroll
+    for (int u = 0; u < KM; ++u) {
+      const int k = 4 * u + lk;
+      bf[u] = (4 * u < dn && k < dn && n < H) ? a.Wn[(size_t)k * H + n] : T(0);
     }
-    const int k0 = a.tb.ints[a.tb.pptr_i + al], k1 = a.tb.ints[a.tb.pptr_i + al + 1];
-    for (int k = k0; k < k1; ++k) {
-      const int32_t *pr = a.tb.ints + a.tb.part_i + TDX_AD_PART_INTS * k;
-      const int m = pr[0];
-      T zb[TDX_AD_MAX_ORDER];
-      T all = T(1);
+    for (int r0 = 0; r0 < a.rpad; r0 += 16) {
+      typename V4<T>::type acc = {T(0), T(0), T(0), T(0)};
+      const T *row = DN + (r0 + lr) * ldd + lk;
 #pragma unroll
-      for (int q = 0; q < TDX_AD_MAX_ORDER; ++q) {
-        zb[q] = q < m ? zl[pr[1 + q] * H + j] : T(1);
-        all = all * zb[q];
+      for (int u = 0; u < KM; ++u) {
+        if (4 * u < dn) {
+          const T av = 4 * u + lk < dn ? row[4 * u] : T(0);
+          acc = mma16(av, bf[u], acc);
+        }
       }
-      dz0 = dz0 + da * deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m + 1) * all;
-      const T sm = deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m);
+      if (n < H) {
 #pragma unroll
-      for (int q = 0; q < TDX_AD_MAX_ORDER; ++q) {
-        if (q < m) {
-          T others = T(1);
+        for (int i = 0; i < 4; ++i) DA[(r0 + crow<T>(lk, i)) * ldh + n] = acc[i];
+      }
+    }
+  }
+  __syncthreads();
+  for (int base = 0; base < npts * H; base += AD_THREADS) {
+    const int item = base + tid;
+    if (item >= npts * H) continue;
+    const int pp = item / H, j = item - (item / H) * H;
+    const int64_t p = p0 + pp;
+    const T *zg = ZS + (size_t)pp * J * ldh + j;  // z of jet al at zg[al * ldh]
+    const T *da = DA + (size_t)pp * J * ldh + j;
+    T *dz = DZ + (size_t)pp * J * ldh + j;
+    const TanhDerivs<T> s = tanh_derivs(zg[0]);
+    T dz0 = da[0] * s.s1;
+    for (int al = 1; al < J; ++al) {
+      const T d = da[al * ldh];
+      const int k0 = tb.ints[tb.pptr_i + al], k1 = tb.ints[tb.pptr_i + al + 1];
+      for (int k = k0; k < k1; ++k) {
+        const int32_t *pr = tb.ints + tb.part_i + TDX_AD_PART_INTS * k;
+        const int m = pr[0];
+        T zb[TDX_AD_MAX_ORDER];
+        T all = T(1);
 #pragma unroll
-          for (int q2 = 0; q2 < TDX_AD_MAX_ORDER; ++q2)
-            if (q2 != q && q2 < m) others = others * zb[q2];
-          dz[pr[1 + q] * H + j] += da * sm * others;
+        for (int q = 0; q < TDX_AD_MAX_ORDER; ++q) {
+          zb[q] = q < m ? zg[pr[1 + q] * ldh] : T(1);
+          all = all * zb[q];
+        }
+        dz0 = dz0 + d * deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m + 1) * all;
+        const T sm = deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m);
+#pragma unroll
+        for (int q = 0; q < TDX_AD_MAX_ORDER; ++q) {
+          if (q < m) {
+            T others = T(1);
+#pragma unroll
+            for (int q2 = 0; q2 < TDX_AD_MAX_ORDER; ++q2)
+              if (q2 != q && q2 < m) others = others * zb[q2];
+            dz[pr[1 + q] * ldh] += d * sm * others;
+          }
         }
       }
     }
+    dz[0] += dz0;
+    for (int al = 0; al < J; ++al) a.DZ[((size_t)p * J + al) * H + j] = dz[al * ldh];
   }
-  dz[j] += dz0;
-  for (int al = 0; al < J; ++al) a.DZ[((size_t)(p0 + pp) * J + al) * H + j] = dz[al * H + j];
 }
 
 // ---------------------------------------------------------------------------
 // weight gradients: dW[i][j] = sum_rows D[r][i] S[r][j], db[i] = sum over
-// value rows of D[r][i]; rows r = (point, jet), chunked, fixed order
+// value rows of D[r][i]; rows r = (point, jet).  A workgroup reduces a
+// chunk of rows for up to 32 output tiles of 16 x 16 on MFMA (8 per
+// wavefront); the chunks' partials are summed in a fixed order after.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct WgArgs {
@@ -406,64 +525,82 @@ struct WgArgs {
   T *part;        // [n_chunks][dout * (din + 1)]
 };
 
+constexpr int AD_WG_TPW = 8;   // output tiles per wavefront
+
 template <typename T>
 __global__ __launch_bounds__(AD_THREADS) void k_ad_wgrad(WgArgs<T> a) {
-  constexpr int AD_WG_RT = sizeof(T) == 4 ? 16 : 8;  // rows per LDS tile (33 KB)
-  __shared__ T Ds[AD_WG_RT * 257];
-  __shared__ T Ss[AD_WG_RT * 258];
-  const int tid = threadIdx.x;
-  const int din = a.din, dout = a.dout, ne = dout * (din + 1);
+  constexpr int RT = sizeof(T) == 4 ? 16 : 8;   // rows per LDS stage (33 KB)
+  __shared__ T Ds[RT * 257];
+  __shared__ T Ss[RT * 258];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int din = a.din, dout = a.dout, d1 = din + 1, ne = dout * d1;
+  const int ntn = (d1 + 15) >> 4, nt = ((dout + 15) >> 4) * ntn;
   const int64_t r0 = (int64_t)blockIdx.x * AD_WG_CHUNK;
   const int64_t r1 = std::min<int64_t>(a.K, r0 + AD_WG_CHUNK);
-  const int e0 = blockIdx.y * AD_THREADS * AD_WG_EPT;
-  T acc[AD_WG_EPT];
-  int ei[AD_WG_EPT], ej[AD_WG_EPT];
+  const int t0 = blockIdx.y * (AD_THREADS / 64) * AD_WG_TPW;
+  const int lr = lane & 15, lk = lane >> 4;
+  typename V4<T>::type acc[AD_WG_TPW];
 #pragma unroll
-  for (int k = 0; k < AD_WG_EPT; ++k) {
-    acc[k] = T(0);
-    const int e = e0 + k * AD_THREADS + tid;
-    ei[k] = e < ne ? e / (din + 1) : -1;
-    ej[k] = e < ne ? e - (e / (din + 1)) * (din + 1) : 0;
-  }
-  for (int64_t rt = r0; rt < r1; rt += AD_WG_RT) {
-    const int nr = (int)std::min<int64_t>(AD_WG_RT, r1 - rt);
+  for (int u = 0; u < AD_WG_TPW; ++u) acc[u] = {T(0), T(0), T(0), T(0)};
+  for (int64_t rt = r0; rt < r1; rt += RT) {
+    const int nr = (int)std::min<int64_t>(RT, r1 - rt);
     __syncthreads();
-    for (int k = tid; k < nr * dout; k += AD_THREADS) {
-      const int r = k / dout, i = k - (k / dout) * dout;
-      Ds[r * 257 + i] = a.D[(rt + r) * dout + i];
-    }
-    for (int k = tid; k < nr * (din + 1); k += AD_THREADS) {
-      const int r = k / (din + 1), j = k - (k / (din + 1)) * (din + 1);
-      Ss[r * 258 + j] = j < din ? a.S[(rt + r) * din + j] : (((rt + r) % a.J) == 0 ? T(1) : T(0));
-    }
+    stage_rows(Ds, 257, a.D + rt * dout, RT, nr, dout);
+    stage_rows(Ss, 258, a.S + rt * din, RT, nr, din);
+    // the bias column: 1 on value rows
+    if (tid < RT) Ss[tid * 258 + din] = tid < nr && ((rt + tid) % a.J) == 0 ? T(1) : T(0);
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < AD_WG_EPT; ++k) {
-      if (ei[k] < 0) continue;
-      T s = acc[k];
-      for (int r = 0; r < nr; ++r) s = s + Ds[r * 257 + ei[k]] * Ss[r * 258 + ej[k]];
-      acc[k] = s;
+    for (int u = 0; u < AD_WG_TPW; ++u) {
+      const int t = t0 + wave + (AD_THREADS / 64) * u;
+      if (t >= nt) continue;
+      const int i = (t / ntn) * 16 + lr, j = (t - (t / ntn) * ntn) * 16 + lr;
+#pragma unroll
+      for (int k0 = 0; k0 < RT; k0 += 4) {
+        const int r = k0 + lk;
+        const T av = i < dout ? Ds[r * 257 + i] : T(0);
+        const T bv = j < d1 ? Ss[r * 258 + j] : T(0);
+        acc[u] = mma16(av, bv, acc[u]);
+      }
     }
   }
 #pragma unroll
-  for (int k = 0; k < AD_WG_EPT; ++k) {
-    const int e = e0 + k * AD_THREADS + tid;
-    if (e < ne) a.part[(size_t)blockIdx.x * ne + e] = acc[k];
+  for (int u = 0; u < AD_WG_TPW; ++u) {
+    const int t = t0 + wave + (AD_THREADS / 64) * u;
+    if (t >= nt) continue;
+    const int j = (t - (t / ntn) * ntn) * 16 + lr;
+    if (j >= d1) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = (t / ntn) * 16 + crow<T>(lk, q);
+      if (i < dout) a.part[(size_t)blockIdx.x * ne + i * d1 + j] = acc[u][q];
+    }
   }
 }
 
+// Sum of the chunks' partials: 64 elements per workgroup, each wavefront
+// sums a quarter of the chunks in order, then the quarters are added in order.
 template <typename T>
-__global__ void k_ad_wreduce(const T *part, int n_chunks, int din, int dout, T *gw, T *gb) {
+__global__ __launch_bounds__(AD_THREADS) void k_ad_wreduce(const T *part, int n_chunks, int din,
+                                                           int dout, T *gw, T *gb) {
+  __shared__ T red[AD_THREADS];
   const int ne = dout * (din + 1);
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + lane;
+  const int q = (n_chunks + 3) / 4;
+  const int c0 = wave * q, c1 = min(n_chunks, c0 + q);
   T s = T(0);
-  for (int c = 0; c < n_chunks; ++c) s += part[(size_t)c * ne + e];
+  if (e < ne)
+    for (int c = c0; c < c1; ++c) s += part[(size_t)c * ne + e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (wave != 0 || e >= ne) return;
+  const T tot = ((red[lane] + red[64 + lane]) + red[128 + lane]) + red[192 + lane];
   const int i = e / (din + 1), j = e - (e / (din + 1)) * (din + 1);
   if (j < din)
-    gw[i * din + j] = s;
+    gw[i * din + j] = tot;
   else
-    gb[i] = s;
+    gb[i] = tot;
 }
 
 }  // namespace
@@ -474,41 +611,38 @@ __global__ void k_ad_wreduce(const T *part, int n_chunks, int din, int dout, T *
 struct tdx_ad_plan {
   int dtype, d_in, n_out, H, L, J;
   int64_t P, n_entries, n_resid, n_params;
-  int n_groups, groups_i, jet_i, pptr_i, part_i;
+  int n_groups, groups_i, jet_i, pptr_i, part_i, n_parts;
   DevTable ints, reals, data, points, gstart;
   std::vector<int64_t> off_w, off_b;
   std::vector<int> din, dout;
   // launch shapes per layer
-  std::vector<int> f_ppb, f_tpp, b_ppb, b_tpp;
+  std::vector<int> f_pb, f_rpad, b_pb, b_rpad;  // points per workgroup, rows rounded to 16
   std::vector<size_t> f_lds, b_lds;
   int n_res_blocks;
   int64_t n_chunks;
   // workspace offsets (bytes)
   size_t off_in0, off_u, off_seed, off_du, off_lp, off_part, off_grad, off_one, total;
-  std::vector<size_t> off_z, off_a, off_dz;
+  std::vector<size_t> off_z, off_a, off_dz, off_wt;
 };
 
 namespace {
 
-size_t ad_fwd_lds(int din, int dout, int J, int ppb, bool act, size_t ts) {
-  return ((size_t)din * dout + (size_t)ppb * J * din + (act ? (size_t)ppb * J * dout : 0)) * ts;
+int rows16(int pb, int J) { return (pb * J + 15) / 16 * 16; }
+size_t ad_fwd_lds(int din, int dout, int J, int pb, size_t ts) {
+  return (size_t)rows16(pb, J) * ((din + 1) + (dout + 1)) * ts;
 }
-size_t ad_bwd_lds(int H, int dn, int J, int ppb, size_t ts) {
-  return ((size_t)dn * H + (size_t)ppb * J * (dn + 2 * H)) * ts;
-}
-int pow2_log(int n) {
-  int l = 0;
-  while ((1 << l) < n) ++l;
-  return l;
+size_t ad_bwd_lds(int H, int dn, int J, int pb, size_t ts) {
+  return (size_t)rows16(pb, J) * ((dn + 1) + 3 * (H + 1)) * ts;
 }
 
-// largest points-per-block that fits the LDS budget (64 KB when possible)
+// points per workgroup: at most 64, within 80 KB of LDS (two workgroups per
+// CU) when one point fits, else within 160 KB; 0 if one point does not fit
 template <typename F>
-int pick_ppb(int tpp_log, F lds) {
-  int ppb = AD_THREADS >> tpp_log;
-  while (ppb > 1 && lds(ppb) > 64 * 1024) --ppb;
-  if (lds(ppb) > 160 * 1024) return 0;
-  return ppb;
+int pick_pb(F lds) {
+  int pb = 64;
+  while (pb > 1 && lds(pb) > 80 * 1024) --pb;
+  if (lds(pb) > 160 * 1024) return 0;
+  return pb;
 }
 
 int fail(const char *msg) {
@@ -626,6 +760,8 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
   p->jet_i = d->jet_i;
   p->pptr_i = d->pptr_i;
   p->part_i = d->part_i;
+  p->n_parts = I[d->pptr_i + J];
+  const size_t tab = tables_lds_bytes(J, p->n_parts);
   int64_t off = 0;
   for (int l = 0; l < p->L; ++l) {
     const int di = l == 0 ? p->d_in : p->H, dd = l == p->L - 1 ? p->n_out : p->H;
@@ -639,26 +775,24 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
   p->n_params = off;
   for (int l = 0; l < p->L; ++l) {
     const bool act = l < p->L - 1;
-    const int tl = pow2_log(p->dout[l]);
-    const int ppb = pick_ppb(tl, [&](int n) { return ad_fwd_lds(p->din[l], p->dout[l], J, n, act, ts); });
-    if (ppb < 1) {
+    const int fpb = pick_pb([&](int n) { return ad_fwd_lds(p->din[l], p->dout[l], J, n, ts) + tab; });
+    if (fpb < 1) {
       delete p;
       return fail("layer too wide for LDS");
     }
-    p->f_tpp.push_back(tl);
-    p->f_ppb.push_back(ppb);
-    p->f_lds.push_back(ad_fwd_lds(p->din[l], p->dout[l], J, ppb, act, ts));
+    p->f_pb.push_back(fpb);
+    p->f_rpad.push_back(rows16(fpb, J));
+    p->f_lds.push_back(ad_fwd_lds(p->din[l], p->dout[l], J, fpb, ts) + tab);
     if (act) {
       const int dn = p->dout[l + 1];
-      const int tb = pow2_log(p->H);
-      const int pb = pick_ppb(tb, [&](int n) { return ad_bwd_lds(p->H, dn, J, n, ts); });
-      if (pb < 1) {
+      const int bpb = pick_pb([&](int n) { return ad_bwd_lds(p->H, dn, J, n, ts) + tab; });
+      if (bpb < 1) {
         delete p;
         return fail("layer too wide for LDS");
       }
-      p->b_tpp.push_back(tb);
-      p->b_ppb.push_back(pb);
-      p->b_lds.push_back(ad_bwd_lds(p->H, dn, J, pb, ts));
+      p->b_pb.push_back(bpb);
+      p->b_rpad.push_back(rows16(bpb, J));
+      p->b_lds.push_back(ad_bwd_lds(p->H, dn, J, bpb, ts) + tab);
     }
   }
   p->n_res_blocks = (int)((p->n_entries + AD_THREADS - 1) / AD_THREADS);
@@ -687,6 +821,7 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
   for (int l = 0; l < p->L; ++l) max_ne = std::max(max_ne, p->dout[l] * (p->din[l] + 1));
   p->off_part = take((size_t)p->n_chunks * max_ne * ts);
   p->off_grad = take((size_t)p->n_params * ts);
+  for (int l = 0; l < p->L; ++l) p->off_wt.push_back(take((size_t)p->din[l] * p->dout[l] * ts));
   p->off_one = take(ts);
   p->total = w;
 
@@ -738,6 +873,8 @@ Tables tables_of(const tdx_ad_plan *p) {
   t.jet_i = p->jet_i;
   t.pptr_i = p->pptr_i;
   t.part_i = p->part_i;
+  t.J = p->J;
+  t.n_parts = p->n_parts;
   return t;
 }
 
@@ -745,13 +882,20 @@ template <typename T>
 int ad_forward(const tdx_ad_plan *p, const T *params, T *loss, T *resid, void *ws, hipStream_t s) {
   static bool attr = false;
   if (!attr) {
-    (void)hipFuncSetAttribute((const void *)k_ad_fwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    (void)hipFuncSetAttribute((const void *)k_ad_bwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
+    const void *fns[] = {(const void *)k_ad_fwd<T, 4>, (const void *)k_ad_fwd<T, 32>,
+                         (const void *)k_ad_fwd<T, 64>, (const void *)k_ad_bwd<T, 4>,
+                         (const void *)k_ad_bwd<T, 32>, (const void *)k_ad_bwd<T, 64>};
+    for (const void *f : fns)
+      (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr = true;
   }
   const int J = p->J;
+  for (int l = 0; l < p->L; ++l) {
+    const int ne = p->din[l] * p->dout[l];
+    hipLaunchKernelGGL((k_ad_transpose<T>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s,
+                       params + p->off_w[l], p->dout[l], p->din[l], wsp<T>(ws, p->off_wt[l]));
+    TDX_TRY_HIP(hipGetLastError());
+  }
   for (int l = 0; l < p->L; ++l) {
     FwdArgs<T> a;
     a.layer = l;
@@ -760,18 +904,24 @@ int ad_forward(const tdx_ad_plan *p, const T *params, T *loss, T *resid, void *w
     a.act = l < p->L - 1;
     a.J = J;
     a.P = p->P;
-    a.ppb = p->f_ppb[l];
-    a.tpp_log = p->f_tpp[l];
+    a.pb = p->f_pb[l];
+    a.rpad = p->f_rpad[l];
     a.x = (const T *)p->points.ptr;
     a.in = l == 0 ? nullptr : wsp<T>(ws, p->off_a[l - 1]);
-    a.W = params + p->off_w[l];
+    a.WT = wsp<T>(ws, p->off_wt[l]);
     a.b = params + p->off_b[l];
     a.in0 = wsp<T>(ws, p->off_in0);
     a.zout = a.act ? wsp<T>(ws, p->off_z[l]) : nullptr;
     a.out = a.act ? wsp<T>(ws, p->off_a[l]) : wsp<T>(ws, p->off_u);
     a.tb = tables_of<T>(p);
-    const unsigned blocks = (unsigned)((p->P + a.ppb - 1) / a.ppb);
-    hipLaunchKernelGGL((k_ad_fwd<T>), dim3(blocks), dim3(AD_THREADS), p->f_lds[l], s, a);
+    const unsigned blocks = (unsigned)((p->P + a.pb - 1) / a.pb);
+    // B fragments held in registers: k-steps of 4 over the input width
+    if (a.din <= 16)
+      hipLaunchKernelGGL((k_ad_fwd<T, 4>), dim3(blocks), dim3(AD_THREADS), p->f_lds[l], s, a);
+    else if (a.din <= 128)
+      hipLaunchKernelGGL((k_ad_fwd<T, 32>), dim3(blocks), dim3(AD_THREADS), p->f_lds[l], s, a);
+    else
+      hipLaunchKernelGGL((k_ad_fwd<T, 64>), dim3(blocks), dim3(AD_THREADS), p->f_lds[l], s, a);
     TDX_TRY_HIP(hipGetLastError());
   }
   ResArgs<T> r;
@@ -811,15 +961,20 @@ int ad_backward(const tdx_ad_plan *p, const T *params, const T *dloss, T *grad, 
     b.dn = p->dout[l + 1];
     b.J = J;
     b.P = p->P;
-    b.ppb = p->b_ppb[l];
-    b.tpp_log = p->b_tpp[l];
+    b.pb = p->b_pb[l];
+    b.rpad = p->b_rpad[l];
     b.Wn = params + p->off_w[l + 1];
     b.DN = l == p->L - 2 ? du : wsp<T>(ws, p->off_dz[l + 1]);
     b.Z = wsp<T>(ws, p->off_z[l]);
     b.DZ = wsp<T>(ws, p->off_dz[l]);
     b.tb = tables_of<T>(p);
-    const unsigned blocks = (unsigned)((p->P + b.ppb - 1) / b.ppb);
-    hipLaunchKernelGGL((k_ad_bwd<T>), dim3(blocks), dim3(AD_THREADS), p->b_lds[l], s, b);
+    const unsigned blocks = (unsigned)((p->P + b.pb - 1) / b.pb);
+    if (b.dn <= 16)
+      hipLaunchKernelGGL((k_ad_bwd<T, 4>), dim3(blocks), dim3(AD_THREADS), p->b_lds[l], s, b);
+    else if (b.dn <= 128)
+      hipLaunchKernelGGL((k_ad_bwd<T, 32>), dim3(blocks), dim3(AD_THREADS), p->b_lds[l], s, b);
+    else
+      hipLaunchKernelGGL((k_ad_bwd<T, 64>), dim3(blocks), dim3(AD_THREADS), p->b_lds[l], s, b);
     TDX_TRY_HIP(hipGetLastError());
   }
   T *part = wsp<T>(ws, p->off_part);
@@ -833,11 +988,13 @@ int ad_backward(const tdx_ad_plan *p, const T *params, const T *dloss, T *grad, 
     w.S = l == 0 ? wsp<T>(ws, p->off_in0) : wsp<T>(ws, p->off_a[l - 1]);
     w.part = part;
     const int ne = w.dout * (w.din + 1);
-    const unsigned ytiles = (unsigned)((ne + AD_THREADS * AD_WG_EPT - 1) / (AD_THREADS * AD_WG_EPT));
+    const int nt = ((w.dout + 15) / 16) * ((w.din + 1 + 15) / 16);
+    const int per_block = (AD_THREADS / 64) * AD_WG_TPW;
+    const unsigned ytiles = (unsigned)((nt + per_block - 1) / per_block);
     hipLaunchKernelGGL((k_ad_wgrad<T>), dim3((unsigned)p->n_chunks, ytiles), dim3(AD_THREADS), 0,
                        s, w);
     TDX_TRY_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_ad_wreduce<T>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL((k_ad_wreduce<T>), dim3((unsigned)((ne + 63) / 64)), dim3(AD_THREADS), 0, s,
                        (const T *)part, (int)p->n_chunks, w.din, w.dout, grad + p->off_w[l],
                        grad + p->off_b[l]);
     TDX_TRY_HIP(hipGetLastError());

@@ -10,8 +10,10 @@ ops (tedeous/eval.py:55-87, 143-193, 283-461), the loss is
 (tedeous/losses.py:84-135) and torch autograd differentiates it.  'mat' mode
 differentiates with flattened ``torch.roll`` stencils and recomputes the
 step with ``torch.unique`` on every application (tedeous/derivative.py:
-199-323).  It runs on whatever device its tensors live on and never calls
-the native library.
+199-323).  'autograd' mode differentiates the network by nested
+``torch.autograd.grad(create_graph=True)`` calls, one chain per derivative
+factor (tedeous/derivative.py:73-132).  It runs on whatever device its
+tensors live on and never calls the native library.
 """
 
 from typing import List, Optional
@@ -19,7 +21,7 @@ from typing import List, Optional
 import numpy as np
 import torch
 
-from .compiler import MatProblem, NNProblem
+from .compiler import ADProblem, MatProblem, NNProblem
 
 
 class PlainNN:
@@ -177,4 +179,77 @@ class PlainMat:
             true = torch.as_tensor(prob.targets, dtype=u.dtype, device=u.device)
             lam_b = torch.tensor([prob.lam_bnd], dtype=u.dtype, device=u.device)
             loss = loss + torch.mean((bval - true) ** 2, 0) @ lam_b.T
+        return loss
+
+
+class PlainAD:
+    """'autograd' mode as the reference computes it: every derivative factor
+    of every term is its own chain of ``torch.autograd.grad`` calls on the
+    whole point set (tedeous/derivative.py:73-132)."""
+
+    def __init__(self, prob: ADProblem, net: torch.nn.Module, device):
+        self.net = net
+        dev = torch.device(device)
+        dt = prob.dtype
+        self.dtype, self.device = dt, dev
+        self.eqs, self.bcs = [], []
+        self.keys, self.max_len = prob.bc_keys, prob.bc_max_len
+        for g in prob.groups:
+            sets = [torch.as_tensor(s, dtype=dt, device=dev) for s in g.sets]
+            for eq in g.equations:
+                terms = []
+                for t in eq.terms:
+                    c = (float(t.coeff_scalar) if t.coeff_array is None else
+                         torch.as_tensor(t.coeff_array, dtype=dt, device=dev).reshape(-1, 1))
+                    facs = [(f.var, int(f.pow_val) if f.pow_kind == 0 else f.pow_val,
+                             [(k, list(jet), w) for k, jet, w in f.stencil]) for f in t.factors]
+                    terms.append((c, facs))
+                if eq.column < 0:
+                    self.eqs.append((sets, terms, eq.lam))
+                else:
+                    self.bcs.append((sets, terms, eq.column,
+                                     torch.as_tensor(eq.col_pos, device=dev),
+                                     torch.as_tensor(eq.target, dtype=dt, device=dev), eq.lam))
+
+    def _derivative(self, pts, var, axes):
+        if not axes:
+            return self.net(pts)[:, var].reshape(-1, 1)
+        pts = pts.clone().requires_grad_(True)
+        fi = self.net(pts)[:, var].sum(0)
+        for ax in axes:
+            grads, = torch.autograd.grad(fi, pts, create_graph=True)
+            fi = grads[:, ax].sum()
+        return grads[:, axes[-1]].reshape(-1, 1)
+
+    def _value(self, sets, terms):
+        total = None
+        for c, facs in terms:
+            der = 1.
+            for var, p, entries in facs:
+                s = 0.
+                for k, axes, w in entries:
+                    s = s + self._derivative(sets[k], var, axes) * w
+                der = der * s ** p
+            v = c * der
+            total = v if total is None else total + v
+        return total
+
+    def loss(self):
+        ops = [self._value(sets, terms).reshape(-1, 1) for sets, terms, _ in self.eqs]
+        op = ops[0] if len(ops) == 1 else torch.cat(ops, 1)
+        lam_op = torch.tensor([[lam for _, _, lam in self.eqs]], dtype=self.dtype,
+                              device=self.device)
+        loss = torch.mean(op ** 2, 0) @ lam_op.T
+        if self.bcs:
+            ncol = len(self.keys)
+            bval = torch.zeros(self.max_len, ncol, dtype=self.dtype, device=self.device)
+            true = torch.zeros_like(bval)
+            lam_b = [0.0] * ncol
+            for sets, terms, col, pos, tgt, lam in self.bcs:
+                v = self._value(sets, terms).reshape(-1)
+                bval = bval.index_put((pos, torch.full_like(pos, col)), v)
+                true = true.index_put((pos, torch.full_like(pos, col)), tgt)
+                lam_b[col] = lam
+            lb = torch.tensor([lam_b], dtype=self.dtype, device=self.device)
+            loss = loss + torch.mean((bval - true) ** 2, 0) @ lb.T
         return loss

@@ -14,6 +14,8 @@ with seeded initialisation.  Sizes follow the problem statements:
 * KdV: u_t + 6 u u_x + u_xxx = 0, x in [0, 20) periodic (512 points),
   t in [0, 1] (128 points), u(x,0) = 0.5 sech^2(0.5(x-10)), MLP 2-64x4-1,
   h = 1e-2.
+* Poisson ('autograd'): -u_xx - u_yy = 2 pi^2 sin(pi x) sin(pi y) on
+  [0, 1]^2, u = 0 on the sides, 201 x 201 points, MLP 2-100x5-1.
 * sweep: 256 Burgers solves, nu = 10^U(-3,-1), nx in [64, 512],
   nt in [32, 128], MLP 2-32-32-32-1 each, seeded per instance.
 """
@@ -80,6 +82,30 @@ def wave(n: int = 128, seed: int = 0):
     g = torch.Generator().manual_seed(seed)
     u0 = 0.01 * torch.randn((1, n, n), generator=g)
     return d, eq, b, u0, dict(mode='mat', lambda_operator=1, lambda_bound=100)
+
+
+def poisson_ad(grid_res: int = 200, seed: int = 0):
+    """'autograd' mode: -u_xx - u_yy = 2 pi^2 sin(pi x) sin(pi y) on [0, 1]^2
+    with u = 0 on the sides (u = sin(pi x) sin(pi y)), (grid_res + 1)^2
+    points, MLP 2-100x5-1 (the network of the reference's 2-D Poisson
+    autograd examples)."""
+    d = Domain()
+    d.variable('x', [0, 1], grid_res)
+    d.variable('y', [0, 1], grid_res)
+    b = Conditions()
+    b.dirichlet({'x': 0, 'y': [0, 1]}, value=0)
+    b.dirichlet({'x': 1, 'y': [0, 1]}, value=0)
+    b.dirichlet({'x': [0, 1], 'y': 0}, value=0)
+    b.dirichlet({'x': [0, 1], 'y': 1}, value=0)
+    eq = Equation()
+    eq.add({
+        '-u_xx': {'coeff': -1., 'u_xx': [0, 0], 'pow': 1, 'var': 0},
+        '-u_yy': {'coeff': -1., 'u_yy': [1, 1], 'pow': 1, 'var': 0},
+        '-f': {'coeff': lambda g: -2 * np.pi ** 2 * torch.sin(np.pi * g[:, 0]) *
+               torch.sin(np.pi * g[:, 1]), 'u': [None], 'pow': 0, 'var': 0},
+    })
+    net = tanh_mlp([2, 100, 100, 100, 100, 100, 1], seed)
+    return d, eq, b, net, dict(mode='autograd', lambda_operator=1, lambda_bound=100)
 
 
 def burgers_equation(nu: float) -> dict:
