@@ -93,28 +93,34 @@ __host__ __device__ inline size_t tables_lds_bytes(int J, int n_parts) {
 }
 
 // Copies rows [0, rows) of a row-major [*][ncols] array (row r at src +
-// r * ncols) into LDS with leading dimension ld, zeros for r >= valid.
-// Eight independent loads per thread are issued before their stores, so a
-// workgroup waits for HBM once per batch instead of once per element.
+// r * ncols, ncols <= 256) into LDS with leading dimension ld, zeros for
+// rows r >= valid.  A row's columns go over a power-of-two span of threads
+// (no division), several rows a sweep; eight independent loads per thread
+// are issued before their stores, so a workgroup waits for HBM once per
+// batch instead of once per element.
+__device__ __forceinline__ int col_span_log(int ncols) {
+  int cl = 0;
+  while ((1 << cl) < ncols) ++cl;
+  return cl;
+}
+
 template <typename T>
 __device__ __forceinline__ void stage_rows(T *dst, int ld, const T *src, int rows, int valid,
                                            int ncols) {
   constexpr int NB = 8;
-  const int total = rows * ncols;
-  for (int k0 = threadIdx.x; k0 < total; k0 += AD_THREADS * NB) {
+  const int cl = col_span_log(ncols);
+  const int c = threadIdx.x & ((1 << cl) - 1), rs = AD_THREADS >> cl;
+  for (int rb = threadIdx.x >> cl; rb < rows; rb += rs * NB) {
     T v[NB];
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      const int k = k0 + u * AD_THREADS;
-      v[u] = (k < total && k < valid * ncols) ? src[k] : T(0);
+      const int r = rb + u * rs;
+      v[u] = (r < valid && c < ncols) ? src[(size_t)r * ncols + c] : T(0);
     }
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      const int k = k0 + u * AD_THREADS;
-      if (k < total) {
-        const int r = k / ncols;
-        dst[r * ld + (k - r * ncols)] = v[u];
-      }
+      const int r = rb + u * rs;
+      if (r < rows && c < ncols) dst[r * ld + c] = v[u];
     }
   }
 }
@@ -436,9 +442,11 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_bwd(BwdArgs<T> a) {
   const int R = npts * J;
   stage_rows(DN, ldd, a.DN + p0 * J * dn, a.rpad, R, dn);
   stage_rows(ZS, ldh, a.Z + p0 * J * H, R, R, H);
-  for (int k = tid; k < R * H; k += AD_THREADS) {
-    const int r = k / H, j = k - (k / H) * H;
-    DZ[r * ldh + j] = T(0);
+  {
+    const int cl = col_span_log(H);
+    const int c = tid & ((1 << cl) - 1);
+    if (c < H)
+      for (int r = tid >> cl; r < R; r += AD_THREADS >> cl) DZ[r * ldh + c] = T(0);
   }
   __syncthreads();
   const int ntc = (H + 15) >> 4;
@@ -635,12 +643,20 @@ size_t ad_bwd_lds(int H, int dn, int J, int pb, size_t ts) {
   return (size_t)rows16(pb, J) * ((dn + 1) + 3 * (H + 1)) * ts;
 }
 
-// points per workgroup: at most 64, within 80 KB of LDS (two workgroups per
-// CU) when one point fits, else within 160 KB; 0 if one point does not fit
+// LDS budgets of the layer kernels' workgroups
+#ifndef TDX_AD_LDS_FWD
+#define TDX_AD_LDS_FWD (40 * 1024)
+#endif
+#ifndef TDX_AD_LDS_BWD
+#define TDX_AD_LDS_BWD (40 * 1024)
+#endif
+
+// points per workgroup: at most 64, within the budget when one point fits,
+// else within 160 KB; 0 if one point does not fit
 template <typename F>
-int pick_pb(F lds) {
+int pick_pb(F lds, size_t budget) {
   int pb = 64;
-  while (pb > 1 && lds(pb) > 80 * 1024) --pb;
+  while (pb > 1 && lds(pb) > budget) --pb;
   if (lds(pb) > 160 * 1024) return 0;
   return pb;
 }
@@ -775,7 +791,8 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
   p->n_params = off;
   for (int l = 0; l < p->L; ++l) {
     const bool act = l < p->L - 1;
-    const int fpb = pick_pb([&](int n) { return ad_fwd_lds(p->din[l], p->dout[l], J, n, ts) + tab; });
+    const int fpb = pick_pb([&](int n) { return ad_fwd_lds(p->din[l], p->dout[l], J, n, ts) + tab; },
+                            TDX_AD_LDS_FWD);
     if (fpb < 1) {
       delete p;
       return fail("layer too wide for LDS");
@@ -785,7 +802,8 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
     p->f_lds.push_back(ad_fwd_lds(p->din[l], p->dout[l], J, fpb, ts) + tab);
     if (act) {
       const int dn = p->dout[l + 1];
-      const int bpb = pick_pb([&](int n) { return ad_bwd_lds(p->H, dn, J, n, ts) + tab; });
+      const int bpb = pick_pb([&](int n) { return ad_bwd_lds(p->H, dn, J, n, ts) + tab; },
+                              TDX_AD_LDS_BWD);
       if (bpb < 1) {
         delete p;
         return fail("layer too wide for LDS");
