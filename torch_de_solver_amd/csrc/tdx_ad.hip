@@ -72,24 +72,33 @@ struct Tables {
   const int32_t *ints;
   int jet_i, pptr_i, part_i;
   int J, n_parts;  // LDS copy: jet (2J) | pptr (J + 1) | parts (5 n_parts) ints
+  // where each jet occurs as a block: tptr (J + 1) | entries (alpha, part, block)
+  const int32_t *occ;
+  int n_occ, tptr_i, tent_i;
 };
 
 // Copies the jet and partition tables into LDS (the epilogues read them for
 // every point and unit); returns a view with the same offsets into the copy.
 __device__ __forceinline__ Tables tables_to_lds(const Tables &t, int32_t *lds) {
   const int nj = 2 * t.J, np = t.J + 1, nr = TDX_AD_PART_INTS * t.n_parts;
-  for (int k = threadIdx.x; k < nj + np + nr; k += AD_THREADS)
+  const int no = t.J + 1 + 3 * t.n_occ;
+  for (int k = threadIdx.x; k < nj + np + nr + no; k += AD_THREADS)
     lds[k] = k < nj ? t.ints[t.jet_i + k]
-                    : (k < nj + np ? t.ints[t.pptr_i + k - nj] : t.ints[t.part_i + k - nj - np]);
+                    : (k < nj + np ? t.ints[t.pptr_i + k - nj]
+                                   : (k < nj + np + nr ? t.ints[t.part_i + k - nj - np]
+                                                       : t.occ[k - nj - np - nr]));
   Tables l = t;
   l.ints = lds;
   l.jet_i = 0;
   l.pptr_i = nj;
   l.part_i = nj + np;
+  l.occ = lds + nj + np + nr;
+  l.tptr_i = 0;
+  l.tent_i = t.J + 1;
   return l;
 }
-__host__ __device__ inline size_t tables_lds_bytes(int J, int n_parts) {
-  return (size_t)(2 * J + J + 1 + TDX_AD_PART_INTS * n_parts) * 4;
+__host__ __device__ inline size_t tables_lds_bytes(int J, int n_parts, int n_occ) {
+  return (size_t)(2 * J + J + 1 + TDX_AD_PART_INTS * n_parts + J + 1 + 3 * n_occ) * 4;
 }
 
 // Copies rows [0, rows) of a row-major [*][ncols] array (row r at src +
@@ -433,8 +442,7 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_bwd(BwdArgs<T> a) {
   const int ldd = dn + 1, ldh = H + 1;
   T *DN = reinterpret_cast<T *>(smem_raw);        // [rpad][ldd]
   T *DA = DN + (size_t)a.rpad * ldd;              // [rpad][ldh]
-  T *DZ = DA + (size_t)a.rpad * ldh;              // [rpad][ldh]
-  T *ZS = DZ + (size_t)a.rpad * ldh;              // [rpad][ldh]
+  T *ZS = DA + (size_t)a.rpad * ldh;              // [rpad][ldh]
   const Tables tb = tables_to_lds(a.tb, reinterpret_cast<int32_t *>(ZS + (size_t)a.rpad * ldh));
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t p0 = (int64_t)blockIdx.x * a.pb;
@@ -442,12 +450,6 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_bwd(BwdArgs<T> a) {
   const int R = npts * J;
   stage_rows(DN, ldd, a.DN + p0 * J * dn, a.rpad, R, dn);
   stage_rows(ZS, ldh, a.Z + p0 * J * H, R, R, H);
-  {
-    const int cl = col_span_log(H);
-    const int c = tid & ((1 << cl) - 1);
-    if (c < H)
-      for (int r = tid >> cl; r < R; r += AD_THREADS >> cl) DZ[r * ldh + c] = T(0);
-  }
   __syncthreads();
   const int ntc = (H + 15) >> 4;
   const int lr = lane & 15, lk = lane >> 4;
@@ -483,8 +485,9 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_bwd(BwdArgs<T> a) {
     const int64_t p = p0 + pp;
     const T *zg = ZS + (size_t)pp * J * ldh + j;  // z of jet al at zg[al * ldh]
     const T *da = DA + (size_t)pp * J * ldh + j;
-    T *dz = DZ + (size_t)pp * J * ldh + j;
+    T *dzo = a.DZ + (size_t)p * J * H + j;
     const TanhDerivs<T> s = tanh_derivs(zg[0]);
+    // the value's adjoint: every jet's map depends on z_0 through tanh^(m)
     T dz0 = da[0] * s.s1;
     for (int al = 1; al < J; ++al) {
       const T d = da[al * ldh];
@@ -492,29 +495,32 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_bwd(BwdArgs<T> a) {
       for (int k = k0; k < k1; ++k) {
         const int32_t *pr = tb.ints + tb.part_i + TDX_AD_PART_INTS * k;
         const int m = pr[0];
-        T zb[TDX_AD_MAX_ORDER];
-        T all = T(1);
+        T all = deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m + 1);
 #pragma unroll
-        for (int q = 0; q < TDX_AD_MAX_ORDER; ++q) {
-          zb[q] = q < m ? zg[pr[1 + q] * ldh] : T(1);
-          all = all * zb[q];
-        }
-        dz0 = dz0 + d * deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m + 1) * all;
-        const T sm = deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m);
-#pragma unroll
-        for (int q = 0; q < TDX_AD_MAX_ORDER; ++q) {
-          if (q < m) {
-            T others = T(1);
-#pragma unroll
-            for (int q2 = 0; q2 < TDX_AD_MAX_ORDER; ++q2)
-              if (q2 != q && q2 < m) others = others * zb[q2];
-            dz[pr[1 + q] * ldh] += d * sm * others;
-          }
-        }
+        for (int q = 0; q < TDX_AD_MAX_ORDER; ++q)
+          if (q < m) all = all * zg[pr[1 + q] * ldh];
+        dz0 = dz0 + d * all;
       }
     }
-    dz[0] += dz0;
-    for (int al = 0; al < J; ++al) a.DZ[((size_t)p * J + al) * H + j] = dz[al * ldh];
+    dzo[0] = dz0;
+    // jet b's adjoint: over the blocks where b occurs, tanh^(m) times the
+    // other blocks of that partition
+    for (int b = 1; b < J; ++b) {
+      T acc = T(0);
+      const int t0 = tb.occ[tb.tptr_i + b], t1 = tb.occ[tb.tptr_i + b + 1];
+      for (int t = t0; t < t1; ++t) {
+        const int32_t *oc = tb.occ + tb.tent_i + 3 * t;
+        const int al = oc[0], q = oc[2];
+        const int32_t *pr = tb.ints + tb.part_i + TDX_AD_PART_INTS * oc[1];
+        const int m = pr[0];
+        T v = da[al * ldh] * deriv_n(s.s1, s.s2, s.s3, s.s4, s.s5, m);
+#pragma unroll
+        for (int q2 = 0; q2 < TDX_AD_MAX_ORDER; ++q2)
+          if (q2 < m && q2 != q) v = v * zg[pr[1 + q2] * ldh];
+        acc = acc + v;
+      }
+      dzo[(size_t)b * H] = acc;
+    }
   }
 }
 
@@ -619,8 +625,8 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_wreduce(const T *part, int n_
 struct tdx_ad_plan {
   int dtype, d_in, n_out, H, L, J;
   int64_t P, n_entries, n_resid, n_params;
-  int n_groups, groups_i, jet_i, pptr_i, part_i, n_parts;
-  DevTable ints, reals, data, points, gstart;
+  int n_groups, groups_i, jet_i, pptr_i, part_i, n_parts, n_occ;
+  DevTable ints, reals, data, points, gstart, occ;
   std::vector<int64_t> off_w, off_b;
   std::vector<int> din, dout;
   // launch shapes per layer
@@ -640,7 +646,7 @@ size_t ad_fwd_lds(int din, int dout, int J, int pb, size_t ts) {
   return (size_t)rows16(pb, J) * ((din + 1) + (dout + 1)) * ts;
 }
 size_t ad_bwd_lds(int H, int dn, int J, int pb, size_t ts) {
-  return (size_t)rows16(pb, J) * ((dn + 1) + 3 * (H + 1)) * ts;
+  return (size_t)rows16(pb, J) * ((dn + 1) + 2 * (H + 1)) * ts;
 }
 
 // LDS budgets of the layer kernels' workgroups
@@ -777,7 +783,18 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
   p->pptr_i = d->pptr_i;
   p->part_i = d->part_i;
   p->n_parts = I[d->pptr_i + J];
-  const size_t tab = tables_lds_bytes(J, p->n_parts);
+  // where each jet occurs as a block of a partition: (alpha, part, block)
+  std::vector<std::vector<int32_t>> occ_of(J);
+  for (int al = 1; al < J; ++al)
+    for (int k = I[d->pptr_i + al]; k < I[d->pptr_i + al + 1]; ++k) {
+      const int32_t *pr = I + d->part_i + TDX_AD_PART_INTS * k;
+      for (int q = 0; q < pr[0]; ++q) occ_of[pr[1 + q]].insert(occ_of[pr[1 + q]].end(), {al, k, q});
+    }
+  std::vector<int32_t> occ(J + 1, 0);
+  for (int b = 0; b < J; ++b) occ[b + 1] = occ[b] + (int)occ_of[b].size() / 3;
+  for (int b = 0; b < J; ++b) occ.insert(occ.end(), occ_of[b].begin(), occ_of[b].end());
+  p->n_occ = occ[J];
+  const size_t tab = tables_lds_bytes(J, p->n_parts, p->n_occ);
   int64_t off = 0;
   for (int l = 0; l < p->L; ++l) {
     const int di = l == 0 ? p->d_in : p->H, dd = l == p->L - 1 ? p->n_out : p->H;
@@ -856,6 +873,7 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
   if (rc == TDX_OK && d->n_data > 0) rc = p->data.upload(d->data, (size_t)d->n_data * ts);
   if (rc == TDX_OK) rc = p->points.upload(d->points, (size_t)d->n_points * d->d_in * ts);
   if (rc == TDX_OK) rc = p->gstart.upload(gstart.data(), gstart.size() * 8);
+  if (rc == TDX_OK) rc = p->occ.upload(occ.data(), occ.size() * 4);
   if (rc != TDX_OK) {
     tdx_ad_plan_destroy(p);
     return rc;
@@ -871,6 +889,7 @@ extern "C" void tdx_ad_plan_destroy(tdx_ad_plan *p) {
   p->data.release();
   p->points.release();
   p->gstart.release();
+  p->occ.release();
   delete p;
 }
 
@@ -893,6 +912,10 @@ Tables tables_of(const tdx_ad_plan *p) {
   t.part_i = p->part_i;
   t.J = p->J;
   t.n_parts = p->n_parts;
+  t.occ = (const int32_t *)p->occ.ptr;
+  t.n_occ = p->n_occ;
+  t.tptr_i = 0;
+  t.tent_i = p->J + 1;
   return t;
 }
 
