@@ -1,13 +1,14 @@
 /*
  * tdx.h -- C ABI of the MI355X residual/loss engine for TEDEouS-style solvers.
  *
- * The library replaces, for the 'NN' and 'mat' solution modes, the per-step
- * objective of the reference:
+ * The library replaces, for the 'NN', 'mat' and 'autograd' solution modes,
+ * the per-step objective of the reference:
  *
  *   Solution.evaluate()             tedeous/solution.py:129-168
  *     Operator.operator_compute()   tedeous/eval.py:167-193, 223-232
  *       Derivative_NN.take_derivative   tedeous/derivative.py:30-58
  *       Derivative_mat.take_derivative  tedeous/derivative.py:293-323
+ *       Derivative_autograd             tedeous/derivative.py:61-132
  *     Bounds.apply_bcs()            tedeous/eval.py:283-461 (+ dict_to_matrix 55-87)
  *     Losses._default_loss()        tedeous/losses.py:84-135
  *   loss.backward()                 tedeous/optimizers/closure.py:60
@@ -111,6 +112,19 @@ typedef struct tdx_nn_desc {
    * w[i][k] = exp(-tol * sum_{j<i} res[j][k]) (no gradient through w). */
   double causal_tol;
   int causal_nt;
+  /* Weak form (tedeous/eval.py:13-52, 195-221, tedeous/losses.py:184-228),
+   * one instance: weak_levels 1 or 2 replaces the operator term of group
+   * 0's equations (loss and normalized weights 0, seed weight lambda) by
+   * sum_eq lambda_eq W_eq, with on one level W = sum_i c_i r_i^2 and on two
+   * levels (one equation) W = sum_g x_g (sum_{i in run g} c_i r_i^2)^2,
+   * c_i = data[weak_d + i] (N values: trapezoid weight times the squared
+   * basis product), x_g = data[weak_d + N + g] (weak_runs values),
+   * lambda_eq = data[weak_d + N + weak_runs + eq]; run g holds points
+   * ints[weak_i + g] .. ints[weak_i + g + 1] - 1 of group 0. */
+  int weak_levels;
+  int weak_runs;
+  int64_t weak_d;
+  int weak_i;
 } tdx_nn_desc;
 
 typedef struct tdx_nn_plan tdx_nn_plan;

@@ -385,9 +385,16 @@ class NNOracle:
                 true[k] = c['bval'].reshape(-1)
         return pad_matrix(bval, true)
 
-    def evaluate(self, net, lam_op=1, lam_bnd=1, tol=0):
+    def evaluate(self, net, lam_op=1, lam_bnd=1, tol=0, weak_form=None):
         op = self.residual(net)
         b, t, keys, lens = self.boundary(net)
+        if weak_form:
+            # the residual on the 'central' points, weighted and integrated
+            # (tedeous/eval.py:195-221, tedeous/losses.py:184-228)
+            ow = weak_operator(op, self.central, weak_form)
+            loss, norm = weak_loss(ow, b, t, lam_row(op, lam_op).to(op.dtype),
+                                   lam_row(b, lam_bnd).to(op.dtype))
+            return loss, norm, op, b, t, keys, lens
         if tol:
             # slices: distinct first coordinates of the 'central' points
             # (tedeous/solution.py:51-53)
@@ -397,6 +404,58 @@ class NNOracle:
             loss, norm = default_loss(op, b, t, lam_row(op, lam_op).to(op.dtype),
                                       lam_row(b, lam_bnd).to(op.dtype))
         return loss, norm, op, b, t, keys, lens
+
+
+def integration(func: torch.Tensor, grid: torch.Tensor, power: int = 2):
+    """Trapezoid over the last column of a sorted grid (tedeous/eval.py:13-52):
+    per run of equal second-to-last coordinates on a 2-D grid (pairs across
+    runs are skipped), over all points on a 1-D grid; the integrand is
+    func ** power and the steps are Python floats."""
+    column = -1 if grid.shape[-1] == 1 else -2
+    marker = grid[0][column]
+    index = [0]
+    result = []
+    u = 0.
+    for i in range(1, len(grid)):
+        if grid[i][column] == marker or column == -1:
+            u += (grid[i][-1] - grid[i - 1][-1]).item() * (func[i] ** power + func[i - 1] ** power) / 2
+        else:
+            result.append(u)
+            marker = grid[i][column]
+            index.append(i)
+            u = 0.
+    if column == -1:
+        return u, 0.
+    result.append(u)
+    return result, grid[index, :-1]
+
+
+def weak_operator(op: torch.Tensor, grid: torch.Tensor, weak_form) -> torch.Tensor:
+    """Operator._weak_pde_compute (tedeous/eval.py:195-221): each equation's
+    residual times every basis function, integrated over every axis."""
+    sols = []
+    for i in range(op.shape[-1]):
+        sol = op[:, i]
+        for func in weak_form:
+            sol = sol * func(grid).reshape(-1)
+        g = torch.clone(grid)
+        for _ in range(grid.shape[-1]):
+            sol, g = integration(sol, g)
+            if isinstance(sol, list):
+                sol = torch.stack([v if isinstance(v, torch.Tensor) else torch.tensor(v) for v in sol])
+        sols.append(sol.reshape(-1, 1))
+    return sols[0] if len(sols) == 1 else torch.cat(sols).reshape(1, -1)
+
+
+def weak_loss(op_w, bval, true, lam_op, lam_bnd):
+    """Losses._weak_loss (tedeous/losses.py:184-228): op @ lambda + padded
+    boundary mean @ lambda; the normalized loss with lambdas 1."""
+    op_w = op_w.reshape(1, -1)
+    mse_b = torch.mean((bval - true) ** 2, 0)
+    loss = op_w @ lam_op.T + mse_b @ lam_bnd.T
+    with torch.no_grad():
+        norm = op_w @ torch.ones_like(lam_op).T + mse_b @ torch.ones_like(lam_bnd).T
+    return loss, norm
 
 
 def causal_loss(op, bval, true, lam_bnd, tol, n_t):

@@ -451,3 +451,41 @@ AD_CASES = {
     'beam_ad': (beam_ad, [1, 10, 10, 10, 1]),
     'legendre_ad': (legendre_ad, [1, 32, 32, 1]),
 }
+
+
+# ---------------------------------------------------------------------------
+# 'NN' weak form (tedeous/eval.py:195-221)
+# ---------------------------------------------------------------------------
+
+def lotka_weak_nn(api, n=40, dtype='float64'):
+    """Lotka-Volterra on t in [0, 1] in weak form with one basis function
+    (two equations, 1-D: one trapezoid level)."""
+    d = api.Domain()
+    d.variable('t', [0, 1], n, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'t': 0}, value=1.0, var=0)
+    b.dirichlet({'t': 0}, value=0.5, var=1)
+    eq = api.Equation()
+    eq.add({'du/dt': {'coeff': 1, 'du/dt': [0], 'pow': 1, 'var': 0},
+            '-a*u': {'coeff': -0.8, 'u': [None], 'pow': 1, 'var': 0},
+            '+b*u*v': {'coeff': 0.5, 'u*v': [[None], [None]], 'pow': [1, 1], 'var': [0, 1]}})
+    eq.add({'dv/dt': {'coeff': 1, 'dv/dt': [0], 'pow': 1, 'var': 1},
+            '+c*v': {'coeff': 0.6, 'v': [None], 'pow': 1, 'var': 1},
+            '-d*u*v': {'coeff': -0.3, 'u*v': [[None], [None]], 'pow': [1, 1], 'var': [0, 1]}})
+    weak = [lambda g: torch.sin(np.pi * g[:, 0]) + 1.5]
+    return d, eq, b, dict(mode='NN', lambda_operator=[1, 2], lambda_bound=10, h=1e-3), weak
+
+
+def burgers_weak_nn(api, dtype='float64'):
+    """Burgers in weak form with two basis functions (2-D: two trapezoid
+    levels)."""
+    d, eq, b, kw = burgers_nn(api, dtype=dtype)
+    weak = [lambda g: torch.cos(0.5 * np.pi * g[:, 0]) + 0.2,
+            lambda g: 1 + g[:, 1]]
+    return d, eq, b, kw, weak
+
+
+WEAK_CASES = {
+    'lotka_weak_nn': (lotka_weak_nn, [1, 16, 16, 2]),
+    'burgers_weak_nn': (burgers_weak_nn, [2, 12, 12, 12, 1]),
+}

@@ -336,3 +336,25 @@ def test_ad_plan_validation_rejects_bad_tables():
     desc.width = 300
     assert lib.tdx_ad_plan_create(ctypes.byref(desc), ctypes.byref(h)) == -1
     assert lib.tdx_ad_plan_create(None, ctypes.byref(h)) == -1
+
+
+@pytest.mark.parametrize('name', ['lotka_weak_nn', 'burgers_weak_nn'])
+def test_weak_tables_reproduce_oracle_integration(name):
+    """The compiled weak-form weights give the oracle's weak residual
+    (tedeous/eval.py:13-52, 195-221) for random residuals on the problem's
+    'central' points."""
+    from torch_de_solver_amd.compiler import weak_tables
+    builder, _ = problems.WEAK_CASES[name]
+    d, eq, b, kw, weak = builder(api)
+    grid = d.build('NN')
+    types = point_types(grid.numpy())
+    central = grid[torch.from_numpy(np.nonzero(types == 'central')[0])]
+    lv, ctf2, cx, gstart = weak_tables(central, weak)
+    rng = np.random.default_rng(7)
+    op = torch.from_numpy(rng.standard_normal((len(central), 1)))
+    want = float(O.weak_operator(op, central, weak))
+    r2 = op[:, 0].numpy() ** 2
+    runs = np.array([np.sum(ctf2[a:b_] * r2[a:b_]) for a, b_ in zip(gstart[:-1], gstart[1:])])
+    got = float(runs[0]) if lv == 1 else float(np.sum(cx * runs ** 2))
+    np.testing.assert_allclose(got, want, rtol=1e-12)
+    assert lv == (1 if central.shape[1] == 1 else 2)

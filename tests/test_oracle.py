@@ -397,3 +397,31 @@ def test_ad_oracle_system_exact_solution():
     loss, norm, op, bv, tv, keys, lens = orc.evaluate(Osc(), kw['lambda_operator'], kw['lambda_bound'])
     assert float(op.abs().max()) < 1e-14
     assert float(loss) < 1e-28
+
+
+# ---------------------------------------------------------------------------
+# weak form: the restated trapezoid loop pinned by numpy's trapezoid
+# ---------------------------------------------------------------------------
+
+def test_weak_integration_matches_numpy_trapezoid():
+    """One level on a 1-D grid; two levels on a 2-D grid with the integrand
+    squared at each level and no pairs across runs of the first coordinate
+    (tedeous/eval.py:13-52)."""
+    rng = np.random.default_rng(5)
+    x = np.sort(rng.uniform(0, 1, 9))
+    f = rng.standard_normal(9)
+    u, _ = O.integration(torch.from_numpy(f), torch.from_numpy(x).reshape(-1, 1))
+    np.testing.assert_allclose(float(u), np.trapezoid(f ** 2, x), rtol=1e-13)
+    xs, ts = np.array([0.1, 0.3, 0.35, 0.8]), np.array([0.0, 0.2, 0.5, 0.55, 1.0])
+    grid = torch.cartesian_prod(torch.from_numpy(xs), torch.from_numpy(ts))
+    s = rng.standard_normal(len(grid))
+    op = torch.from_numpy(s).reshape(-1, 1)
+    w = O.weak_operator(op, grid, [])
+    R = np.array([np.trapezoid(s.reshape(4, 5)[k] ** 2, ts) for k in range(4)])
+    np.testing.assert_allclose(float(w), np.trapezoid(R ** 2, xs), rtol=1e-13)
+    # the basis functions multiply the residual before squaring
+    fb = lambda g: 1 + g[:, 0] * g[:, 1]  # noqa: E731
+    w2 = O.weak_operator(op, grid, [fb])
+    sb = s * (1 + grid[:, 0] * grid[:, 1]).numpy()
+    R2 = np.array([np.trapezoid(sb.reshape(4, 5)[k] ** 2, ts) for k in range(4)])
+    np.testing.assert_allclose(float(w2), np.trapezoid(R2 ** 2, xs), rtol=1e-13)

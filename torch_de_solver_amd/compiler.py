@@ -121,6 +121,16 @@ class NNProblem:
     # slices the operator points are read in (0: default loss)
     causal_tol: float = 0.0
     causal_nt: int = 0
+    # weak form (tedeous/eval.py:13-52, 195-221): 0 = none, 1 = one
+    # trapezoid level (1-D grids), 2 = two levels (2-D grids); per 'central'
+    # point its inner trapezoid weight times the squared basis product, per
+    # run of equal first coordinates its outer trapezoid weight, and each
+    # point's run (W = sum_g cx_g (sum_{i in g} ctf2_i r_i^2)^2, or
+    # sum_i ctf2_i r_i^2 on one level)
+    weak_levels: int = 0
+    weak_ctf2: Optional[np.ndarray] = None
+    weak_cx: Optional[np.ndarray] = None
+    weak_gstart: Optional[np.ndarray] = None
 
 
 def _coeff_for_entries(coeff, grid: torch.Tensor, entry_idx: np.ndarray,
@@ -186,10 +196,47 @@ def _match_grid_rows(grid_np: np.ndarray, pts: np.ndarray) -> np.ndarray:
     return np.array([lut.get(tuple(p), -1) for p in pts.tolist()], dtype=np.int64)
 
 
+def _trapezoid_weights(steps: np.ndarray) -> np.ndarray:
+    """Weights c_i with sum_i c_i f_i = sum_{i>=1} s_i (f_i + f_{i-1}) / 2,
+    s_i = x_i - x_{i-1} (len(steps) + 1 points)."""
+    c = np.zeros(len(steps) + 1, dtype=np.float64)
+    c[1:] += steps / 2
+    c[:-1] += steps / 2
+    return c
+
+
+def weak_tables(pts: torch.Tensor, weak_form) -> Tuple[int, np.ndarray, np.ndarray, np.ndarray]:
+    """(levels, ctf2, cx, gstart) of the weak form on the sorted points
+    ``pts`` (tedeous/eval.py:13-52, 195-221).  Steps are taken in the grid's
+    precision and then as Python floats, as the reference's ``.item()``."""
+    n, d = pts.shape
+    if d > 2:
+        raise NotImplementedError("the weak form is served on 1-D and 2-D grids")
+    f = torch.ones(n, dtype=torch.float64)
+    for func in weak_form:
+        f = f * torch.as_tensor(func(pts)).reshape(-1).to(torch.float64)
+    f2 = (f * f).numpy()
+    last = pts[:, -1]
+    if d == 1:
+        steps = (last[1:] - last[:-1]).to(torch.float64).numpy()
+        return 1, _trapezoid_weights(steps) * f2, np.ones(1), np.array([0, n], dtype=np.int64)
+    first = pts[:, 0]
+    starts = [0] + [i for i in range(1, n) if first[i] != first[i - 1]]
+    gstart = np.array(starts + [n], dtype=np.int64)
+    ct = np.zeros(n, dtype=np.float64)
+    for a, b in zip(gstart[:-1], gstart[1:]):
+        steps = (last[a + 1:b] - last[a:b - 1]).to(torch.float64).numpy()
+        ct[a:b] = _trapezoid_weights(steps)
+    gx = first[torch.from_numpy(gstart[:-1])]
+    steps = (gx[1:] - gx[:-1]).to(torch.float64).numpy()
+    cx = _trapezoid_weights(steps)
+    return 2, ct * f2, cx, gstart
+
+
 def compile_nn(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[dict]],
                n_out: int, h: float = 0.001, inner_order: str = '1',
                boundary_order: str = '2', lambda_operator=1,
-               lambda_bound=1, tol: float = 0.0) -> NNProblem:
+               lambda_bound=1, tol: float = 0.0, weak_form=None) -> NNProblem:
     """Lower an 'NN' problem (tedeous/input_preprocessing.py:173-408,
     tedeous/eval.py:90-461, tedeous/losses.py:84-135)."""
     if inner_order == '2':
@@ -362,11 +409,19 @@ def compile_nn(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[
         causal_nt = int(torch.unique(central_pts[:, 0]).numel())
         if causal_nt == 0 or n_c % causal_nt != 0:
             causal_nt = n_c
-    return NNProblem(d_in=d, n_out=n_out, groups=groups,
+    prob = NNProblem(d_in=d, n_out=n_out, groups=groups,
                      n_central=len(central_idx), central_index=central_idx,
                      bc_keys=keys, bc_lengths=[col_len[k] for k in keys],
                      bc_max_len=max_len, dtype=dtype, h=h,
                      causal_tol=float(tol), causal_nt=causal_nt)
+    if weak_form:
+        if tol:
+            raise NotImplementedError("a weak form with a causal loss is not served")
+        lv, ctf2, cx, gstart = weak_tables(central_pts, weak_form)
+        if lv == 2 and n_eq > 1:
+            raise NotImplementedError("the weak form of a system is served on 1-D grids")
+        prob.weak_levels, prob.weak_ctf2, prob.weak_cx, prob.weak_gstart = lv, ctf2, cx, gstart
+    return prob
 
 
 # ----------------------------------------------------------------------------

@@ -1008,7 +1008,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
           const T sv = sacc[ob][i];
           if (!w1_valu && row < d_in) put(L.oW1() + out * d_in + row, sv);
           else if (!w1_valu && row == col_b1) put(L.ob1() + out, sv);
-          else if (!wl_valu && row < col_wl + n_out) put(L.oWL() + (row - col_wl) * H + out, sv);
+          else if (!wl_valu && row >= col_wl && row < col_wl + n_out)
+            put(L.oWL() + (row - col_wl) * H + out, sv);
         }
       if (w1_valu) {
 #pragma unroll
@@ -1377,6 +1378,62 @@ __global__ void k_causal_seeds(T *seeds, const T *w, int N, int n_slots, int n_o
   seeds[i] *= w[p];
 }
 
+// Weak form (tedeous/eval.py:13-52, 195-221).  R[q][g] = sum over run g of
+// c_i r_q[i]^2, one thread per (run, equation), points in order.
+template <typename T>
+__global__ void k_weak_runs(const T *resid, int n_eq, int N, int G, const int32_t *gstart,
+                            const T *c, T *R) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G * n_eq) return;
+  const int q = t / G, g = t - (t / G) * G;
+  T s = T(0);
+  for (int i = gstart[g]; i < gstart[g + 1]; ++i) {
+    const T r = resid[(int64_t)q * N + i];
+    s += c[i] * (r * r);
+  }
+  R[t] = s;
+}
+
+// Seed multipliers of group 0's points: dW/dr_i = 2 m_i r_i with m_i = c_i
+// on one level and 2 x_g R_g c_i on two (one equation); one workgroup per run.
+template <typename T>
+__global__ void k_weak_mult(int levels, const int32_t *gstart, const T *c, const T *x,
+                            const T *R, T *m) {
+  const int g = blockIdx.x;
+  const T f = levels == 2 ? T(2) * x[g] * R[g] : T(1);
+  for (int i = gstart[g] + threadIdx.x; i < gstart[g + 1]; i += blockDim.x) m[i] = f * c[i];
+}
+
+// sum_q lambda_q W_q added to the loss, sum_q W_q to the normalized loss;
+// W_q = R[q][0] on one level, sum_g x_g R[q][g]^2 on two (fixed order).
+template <typename T>
+__global__ void k_weak_loss(int levels, int n_eq, int G, const T *x, const T *lam, const T *R,
+                            T *loss) {
+  __shared__ T red[256];
+  T tot = T(0), totn = T(0);
+  for (int q = 0; q < n_eq; ++q) {
+    T s = T(0);
+    for (int g = threadIdx.x; g < G; g += 256) {
+      const T v = R[q * G + g];
+      s += levels == 2 ? x[g] * (v * v) : v;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+    const T W = red[0];
+    __syncthreads();
+    tot = tot + lam[q] * W;
+    totn = totn + W;
+  }
+  if (threadIdx.x == 0) {
+    loss[0] = loss[0] + tot;
+    loss[1] = loss[1] + totn;
+  }
+}
+
 // mean(w * res) in a fixed order, added to both loss values.
 template <typename T>
 __global__ void k_causal_loss(const T *wr, int N, T *loss) {
@@ -1531,6 +1588,12 @@ struct tdx_nn_plan {
   double causal_tol = 0;
   int c_nt = 0, c_K = 0, c_N = 0, c_neq = 0, c_slots = 0;
   size_t off_resid = 0, off_cw = 0, off_cwr = 0;
+  // weak form: levels (0 none), runs, group 0's points and equations, table
+  // offsets (data: c_i, x_g, lambda_eq; ints: run starts); workspace for
+  // the runs' sums and the points' seed multipliers
+  int weak = 0, w_G = 0, w_N = 0, w_neq = 0;
+  int64_t w_d = 0, w_i = 0;
+  size_t off_wR = 0, off_wm = 0;
   // optional launch timing: event pairs per kernel kind
   int prof_cap = 0;
   int prof_n[TDX_NN_KERNEL_KINDS] = {0};
@@ -1614,8 +1677,8 @@ int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStre
                    void *ws) {
   int rc = launch_pack<T, NB>(p, a, s);
   if (rc) return rc;
-  // the causal loss reads group 0's residuals
-  if (p->causal && !resid) resid = (T *)((char *)ws + p->off_resid);
+  // the causal loss and the weak form read group 0's residuals
+  if ((p->causal || p->weak) && !resid) resid = (T *)((char *)ws + p->off_resid);
   a.resid = resid;
   {
     ProfScope ps(p, 0, s);
@@ -1640,6 +1703,27 @@ int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStre
     TDX_TRY_HIP(hipGetLastError());
     if (loss) {
       hipLaunchKernelGGL((k_causal_loss<T>), dim3(1), dim3(256), 0, s, (const T *)wr, p->c_N, loss);
+      TDX_TRY_HIP(hipGetLastError());
+    }
+  }
+  if (p->weak) {
+    const T *c = (const T *)p->data.ptr + p->w_d;
+    const T *x = c + p->w_N, *lam = x + p->w_G;
+    const int32_t *gs = (const int32_t *)p->ints.ptr + p->w_i;
+    T *R = (T *)((char *)ws + p->off_wR), *m = (T *)((char *)ws + p->off_wm);
+    hipLaunchKernelGGL((k_weak_runs<T>), dim3((p->w_G * p->w_neq + 255) / 256), dim3(256), 0, s,
+                       (const T *)resid, p->w_neq, p->w_N, p->w_G, gs, c, R);
+    TDX_TRY_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_weak_mult<T>), dim3(p->w_G), dim3(256), 0, s, p->weak, gs, c, x,
+                       (const T *)R, m);
+    TDX_TRY_HIP(hipGetLastError());
+    const int64_t tot = (int64_t)p->c_slots * p->w_N * p->n_out;
+    hipLaunchKernelGGL((k_causal_seeds<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       a.seeds, (const T *)m, p->w_N, p->c_slots, p->n_out);
+    TDX_TRY_HIP(hipGetLastError());
+    if (loss) {
+      hipLaunchKernelGGL((k_weak_loss<T>), dim3(1), dim3(256), 0, s, p->weak, p->w_neq, p->w_G, x,
+                         lam, (const T *)R, loss);
       TDX_TRY_HIP(hipGetLastError());
     }
   }
@@ -1874,6 +1958,24 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
       TDX_REQUIRE(d->ints[G0[6] + q * TDX_EQ_INTS + 3] == (int64_t)q * G0[1],
                   "causal loss: group 0's residuals must lead the residual output");
   }
+  if (d->weak_levels != 0) {
+    const int32_t *G0 = d->groups;
+    const int N = G0[1], G = d->weak_runs, neq = G0[5];
+    TDX_REQUIRE(d->weak_levels == 1 || d->weak_levels == 2, "weak form: 1 or 2 levels");
+    TDX_REQUIRE(d->n_inst == 1, "the weak form serves one instance");
+    TDX_REQUIRE(d->causal_tol == 0.0, "a weak form with a causal loss");
+    TDX_REQUIRE(G >= 1 && (d->weak_levels == 2 || G == 1), "weak form: runs");
+    TDX_REQUIRE(d->weak_levels == 1 || neq == 1, "weak form on two levels: one equation");
+    TDX_REQUIRE(d->weak_d >= 0 && d->weak_d + N + G + neq <= d->n_data, "weak form: data range");
+    TDX_REQUIRE(d->weak_i >= 0 && d->weak_i + G + 1 <= d->n_ints, "weak form: run table range");
+    const int32_t *gs = d->ints + d->weak_i;
+    TDX_REQUIRE(gs[0] == 0 && gs[G] == N, "weak form: runs must cover group 0");
+    for (int g = 0; g < G; ++g) TDX_REQUIRE(gs[g + 1] > gs[g], "weak form: empty run");
+    TDX_REQUIRE(G0[7] == 0, "group 0 must start at row 0");
+    for (int q = 0; q < neq; ++q)
+      TDX_REQUIRE(d->ints[G0[6] + q * TDX_EQ_INTS + 3] == (int64_t)q * N,
+                  "weak form: group 0's residuals must lead the residual output");
+  }
   tdx_nn_plan *p = new tdx_nn_plan();
   p->dtype = d->dtype;
   p->d_in = d->d_in;
@@ -2018,6 +2120,22 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
     off = align_up(off + (size_t)p->c_N * ts, 256);
     p->off_cwr = off;
     off = align_up(off + (size_t)p->c_N * ts, 256);
+  }
+  if (d->weak_levels != 0) {
+    const int32_t *G0 = d->groups;
+    p->weak = d->weak_levels;
+    p->w_N = G0[1];
+    p->w_G = d->weak_runs;
+    p->w_neq = G0[5];
+    p->c_slots = G0[2];
+    p->w_d = d->weak_d;
+    p->w_i = d->weak_i;
+    p->off_resid = off;
+    off = align_up(off + (size_t)std::max<int64_t>(d->n_resid, 1) * ts, 256);
+    p->off_wR = off;
+    off = align_up(off + (size_t)p->w_G * p->w_neq * ts, 256);
+    p->off_wm = off;
+    off = align_up(off + (size_t)p->w_N * ts, 256);
   }
   p->ws_bytes = off;
   *out = p;

@@ -58,6 +58,8 @@ class NNDesc(ctypes.Structure):
         ('n_resid', ctypes.c_int64),
         ('tile_rows', ctypes.c_int), ('chunk_tiles', ctypes.c_int),
         ('causal_tol', ctypes.c_double), ('causal_nt', ctypes.c_int),
+        ('weak_levels', ctypes.c_int), ('weak_runs', ctypes.c_int),
+        ('weak_d', ctypes.c_int64), ('weak_i', ctypes.c_int),
     ]
 
 

@@ -139,7 +139,14 @@ class NNTables:
                         ints[term_i + ti * native.TDX_TERM_INTS:
                              term_i + (ti + 1) * native.TDX_TERM_INTS] = [len(t.factors), factor_i, ck, ci]
                     target_off = add_data(eq.target) if eq.target is not None else -1
-                    if prob.causal_tol and gi == 0:
+                    if prob.weak_levels and gi == 0:
+                        # weak form: the operator term is sum_eq lambda W_eq,
+                        # formed after the forward pass; its seeds carry
+                        # lambda here and the trapezoid factors there
+                        lam_r = add_real(0.0)
+                        inv_r = add_real(0.0)
+                        seed_r = add_real(eq.lam)
+                    elif prob.causal_tol and gi == 0:
                         # causal loss: the operator term is mean(w res),
                         # formed after the forward pass; its seeds carry
                         # 1/N here and w there
@@ -158,6 +165,17 @@ class NNTables:
                 groups.append([inst, g.n_entries, len(g.slot_base), slot_base_i, slot_off_r,
                                len(g.equations), eq_i, row])
                 row += len(g.slot_base) * g.n_entries
+        # weak-form tables of the first instance (one instance per plan)
+        self.weak_levels = self.weak_runs = self.weak_i = 0
+        self.weak_d = 0
+        prob = problems[0]
+        if prob.weak_levels:
+            lams = [eq.lam for eq in prob.groups[0].equations]
+            self.weak_levels = prob.weak_levels
+            self.weak_runs = len(prob.weak_cx)
+            self.weak_d = add_data(np.concatenate([prob.weak_ctf2, prob.weak_cx, lams]))
+            self.weak_i = len(ints)
+            ints.extend(int(v) for v in prob.weak_gstart)
         self.groups = np.asarray(groups, dtype=np.int32).reshape(-1)
         self.n_groups = len(groups)
         self.ints = np.asarray(ints if ints else [0], dtype=np.int32)
@@ -214,6 +232,11 @@ class NNEngine:
                 raise NotImplementedError("the causal loss serves one instance per plan")
             desc.causal_tol = float(problems[0].causal_tol)
             desc.causal_nt = int(problems[0].causal_nt)
+        if t.weak_levels:
+            if self.n_inst != 1:
+                raise NotImplementedError("the weak form serves one instance per plan")
+            desc.weak_levels, desc.weak_runs = t.weak_levels, t.weak_runs
+            desc.weak_d, desc.weak_i = t.weak_d, t.weak_i
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             native.check(lib.tdx_nn_plan_create(ctypes.byref(desc), ctypes.byref(handle)))

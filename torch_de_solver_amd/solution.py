@@ -59,8 +59,8 @@ class Solution:
                  weak_form=None, lambda_operator=1, lambda_bound=1, tol: float = 0,
                  derivative_points: int = 2, batch_size: int = None,
                  exact_div: bool = True):
-        if weak_form:
-            raise NotImplementedError("weak-form losses are not served by this engine")
+        if weak_form and mode != 'NN':
+            raise NotImplementedError("weak-form losses are served in 'NN' mode only")
         if tol and mode != 'NN':
             raise NotImplementedError("causal losses (tol != 0) are served in 'NN' mode only")
         if batch_size is not None and mode != 'NN':
@@ -86,7 +86,8 @@ class Solution:
             self.problem: NNProblem = compile_nn(
                 grid, ops, equal_cls.bconds, n_out=self.arch.n_out, h=equal_cls.h,
                 inner_order=equal_cls.inner_order, boundary_order=equal_cls.boundary_order,
-                lambda_operator=lambda_operator, lambda_bound=lambda_bound, tol=tol)
+                lambda_operator=lambda_operator, lambda_bound=lambda_bound, tol=tol,
+                weak_form=weak_form)
             self.engine = NNEngine([self.problem], self.arch, device)
         elif mode == 'autograd':
             self.arch = mlp_arch(model)
