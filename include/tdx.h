@@ -318,11 +318,13 @@ int tdx_mat_train_step_to(const tdx_mat_plan *plan, const void *u_in,
  * ints[groups_i + TDX_AD_GROUP_INTS * g]): (n_entries, n_sets, set_i, n_eq,
  * eq_i); entry e of set s is point ints[set_i + s] + e.  Every point
  * belongs to exactly one (group, set, entry).
- * Equation (TDX_AD_EQ_INTS): (n_terms, term_i, lam_r, denom_r, target_d,
- * resid_off): value = in-order sum of terms at the entry; loss term
- * reals[lam_r] / reals[denom_r] * (value - target)^2 with target
- * data[target_d + e] (0 when target_d < 0); the value of entry e goes to
- * resid[resid_off + e].
+ * Equation (TDX_AD_EQ_INTS): (n_terms, term_i, lw_r, nw_r, target_d,
+ * resid_off, sw_r): value = in-order sum of terms at the entry, diff =
+ * value - target with target data[target_d + e] (0 when target_d < 0);
+ * reals[lw_r] * diff^2 adds to the loss, reals[nw_r] * diff^2 to the
+ * normalized loss, and the seed of the value is 2 reals[sw_r] diff (lambda
+ * / denominator, 1 / denominator, lambda / denominator for the default
+ * loss); the value of entry e goes to resid[resid_off + e].
  * Term (TDX_AD_TERM_INTS): (n_factors, factor_i, coeff_kind, coeff_ref):
  * coefficient reals[coeff_ref] (kind 0) or data[coeff_ref + e] (kind 1)
  * times the product of its factors.
@@ -334,7 +336,7 @@ int tdx_mat_train_step_to(const tdx_mat_plan *plan, const void *u_in,
 #define TDX_AD_MAX_ORDER 4
 #define TDX_AD_PART_INTS 5
 #define TDX_AD_GROUP_INTS 5
-#define TDX_AD_EQ_INTS 6
+#define TDX_AD_EQ_INTS 7
 #define TDX_AD_TERM_INTS 4
 #define TDX_AD_FACTOR_INTS 5
 #define TDX_AD_STENCIL_INTS 3
@@ -353,6 +355,13 @@ typedef struct tdx_ad_desc {
   int64_t n_data;
   const void *data;
   int64_t n_resid;
+  /* Causal loss (tedeous/losses.py:137-182): tol != 0 adds mean(w * res)
+   * over the N points of group 0 (its equations carry zero loss weights and
+   * seed weight 1/N), res[p] = sum_eq r_eq[p]^2 read as causal_nt slices of
+   * N / causal_nt points, w[i][k] = exp(-tol * sum_{j<i} res[j][k]); the
+   * seeds of group 0's points are scaled by w (no gradient through w). */
+  double causal_tol;
+  int causal_nt;
 } tdx_ad_desc;
 
 typedef struct tdx_ad_plan tdx_ad_plan;

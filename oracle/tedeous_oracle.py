@@ -725,9 +725,15 @@ class ADOracle:
                 true[k] = c['bval'].reshape(-1)
         return pad_matrix(bval, true)
 
-    def evaluate(self, net, lam_op=1, lam_bnd=1):
+    def evaluate(self, net, lam_op=1, lam_bnd=1, tol=0):
         op = self.residual(net)
         b, t, keys, lens = self.boundary(net)
+        if tol:
+            # slices: distinct first coordinates of the grid
+            # (tedeous/solution.py:55-57)
+            n_t = len(self.grid[:, 0].unique())
+            loss, norm = causal_loss(op, b, t, lam_row(b, lam_bnd).to(op.dtype), tol, n_t)
+            return loss, norm, op, b, t, keys, lens
         loss, norm = default_loss(op, b, t, lam_row(op, lam_op).to(op.dtype),
                                   lam_row(b, lam_bnd).to(op.dtype))
         return loss, norm, op, b, t, keys, lens

@@ -160,3 +160,44 @@ def test_ad_model_trains_legendre():
     err = np.linalg.norm(u - p3) / np.linalg.norm(p3)
     _print(f"rel L2 to P3: {err:.3e}")
     assert err < 0.05, err
+
+
+@pytest.mark.parametrize('name', ['burgers_ad', 'system_ad'])
+def test_ad_causal_against_oracle(name):
+    """The causal loss (tedeous/losses.py:137-182) in 'autograd' mode: the
+    residual read as slices of the first coordinate, weights
+    exp(-tol * earlier slices) without gradient; float64 to 1e-9 / 1e-8."""
+    _print(f"[ad causal f64] {name}")
+    builder, layers = problems.AD_CASES[name]
+    d, eq, b, kw = builder(api)
+    net = _net(layers, torch.float64)
+    orc = O.ADOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst])
+    lo, no, *_ = orc.evaluate(net, kw['lambda_operator'], kw['lambda_bound'], tol=0.5)
+    lo.sum().backward()
+    go = torch.cat([p.grad.reshape(-1) for p in net.parameters()]).numpy()
+    net_dev = _net(layers, torch.float64).to(DEV)
+    grid = d.build('autograd')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst,
+                                 b.build(d.variable_dict)).set_strategy('autograd')
+    sol = Solution(grid, pre, net_dev, 'autograd', None, kw['lambda_operator'],
+                   kw['lambda_bound'], 0.5)
+    ln, nn = sol.evaluate()
+    ln.backward()
+    gn = torch.cat([p.grad.reshape(-1) for p in net_dev.parameters()]).cpu().numpy()
+    np.testing.assert_allclose(float(ln), float(lo.sum()), rtol=1e-9)
+    np.testing.assert_allclose(float(nn), float(no.sum()), rtol=1e-9)
+    np.testing.assert_allclose(gn, go, rtol=1e-8, atol=1e-10 * np.abs(go).max())
+
+
+def test_ad_causal_model_fused_and_torch_optimizer_agree():
+    _print("[ad causal model] burgers_ad")
+    res = []
+    for fused in (True, False):
+        d, eq, b, kw = problems.burgers_ad(api)
+        net = _net([2, 16, 16, 16, 1], torch.float64)
+        model = tdx.Model(net, d, eq, b)
+        model.verbose = False
+        model.compile('autograd', lambda_operator=1, lambda_bound=10, tol=0.5)
+        model.train(tdx.Optimizer('Adam', {'lr': 1e-3}), 21, callbacks=None if fused else [object()])
+        res.append(torch.cat([p.detach().reshape(-1) for p in net.parameters()]).cpu())
+    np.testing.assert_allclose(res[0].numpy(), res[1].numpy(), rtol=1e-9, atol=1e-12)

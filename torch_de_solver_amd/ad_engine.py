@@ -91,7 +91,7 @@ class ADTables:
             set_i = len(ints)
             ints += set_offsets[gi]
             eq_i = len(ints)
-            ints += [0] * (6 * len(g.equations))
+            ints += [0] * (7 * len(g.equations))
             for qi, eq in enumerate(g.equations):
                 _, _, roff, _, _, _ = next(r_iter)
                 term_i = len(ints)
@@ -112,8 +112,14 @@ class ADTables:
                         ck, cr = 0, add_real(t.coeff_scalar)
                     ints[term_i + 4 * ti: term_i + 4 * ti + 4] = [len(t.factors), fac_i, ck, cr]
                 tgt = add_data(eq.target) if eq.target is not None else -1
-                ints[eq_i + 6 * qi: eq_i + 6 * qi + 6] = [len(eq.terms), term_i, add_real(eq.lam),
-                                                          add_real(eq.denom), tgt, roff]
+                if prob.causal_tol and gi == 0:
+                    # causal loss: the operator term is mean(w res), formed
+                    # after the forward pass; the seeds carry 1/N here and w there
+                    lw, nw, sw = 0.0, 0.0, 1.0 / eq.denom
+                else:
+                    lw, nw, sw = eq.lam / eq.denom, 1.0 / eq.denom, eq.lam / eq.denom
+                ints[eq_i + 7 * qi: eq_i + 7 * qi + 7] = [len(eq.terms), term_i, add_real(lw),
+                                                          add_real(nw), tgt, roff, add_real(sw)]
             ints[self.groups_i + 5 * gi: self.groups_i + 5 * gi + 5] = [
                 ne, len(g.sets), set_i, len(g.equations), eq_i]
         self.n_jets = J
@@ -160,6 +166,8 @@ class ADEngine:
         desc.n_data = t.data.size
         desc.data = t.data.ctypes.data
         desc.n_resid = t.n_resid
+        desc.causal_tol = float(prob.causal_tol)
+        desc.causal_nt = int(prob.causal_nt)
         handle = ctypes.c_void_p()
         with torch.cuda.device(device):
             native.check(lib.tdx_ad_plan_create(ctypes.byref(desc), ctypes.byref(handle)))

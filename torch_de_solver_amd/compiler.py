@@ -683,6 +683,10 @@ class ADProblem:
     bc_lengths: List[int]
     bc_max_len: int
     dtype: torch.dtype
+    # causal loss (tedeous/losses.py:137-182): tolerance and the number of
+    # slices the grid points are read in (0: default loss)
+    causal_tol: float = 0.0
+    causal_nt: int = 0
 
     def partitions(self) -> List[List[List[int]]]:
         """Per jet, its set partitions as lists of block jets (Faa di Bruno:
@@ -766,7 +770,7 @@ def _ad_terms(terms: List[Term], pts: torch.Tensor, set_k: int, sign: float = 1.
 
 
 def compile_ad(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[dict]],
-               n_out: int, lambda_operator=1, lambda_bound=1) -> ADProblem:
+               n_out: int, lambda_operator=1, lambda_bound=1, tol: float = 0.0) -> ADProblem:
     """Lower an 'autograd' problem (tedeous/input_preprocessing.py:411-505,
     tedeous/eval.py:90-461 with mode 'autograd', tedeous/losses.py:84-135).
 
@@ -875,6 +879,16 @@ def compile_ad(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[
         if any(a < 0 or a >= d for a in jet):
             raise ValueError(f"derivative axis out of range in {jet}")
     jet_list = sorted(jets, key=lambda j: (len(j), j))
+    # causal slices: the distinct first coordinates of the grid
+    # (tedeous/solution.py:55-57); one slice per point when they do not
+    # divide the points (the reference's reshape fails and it takes one,
+    # tedeous/losses.py:162-166)
+    causal_nt = 0
+    if tol:
+        causal_nt = int(torch.unique(grid_cpu[:, 0]).numel())
+        if causal_nt == 0 or n % causal_nt != 0:
+            causal_nt = n
     return ADProblem(d_in=d, n_out=n_out, jets=jet_list, groups=groups, n_grid=n,
                      bc_keys=keys, bc_lengths=[col_len[k] for k in keys],
-                     bc_max_len=max_len, dtype=dtype)
+                     bc_max_len=max_len, dtype=dtype, causal_tol=float(tol),
+                     causal_nt=causal_nt)

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_resid(ResArgs<T> a) {
     for (int q = 0; q < n_eq; ++q) {
       const int32_t *E = a.ints + eq_i + TDX_AD_EQ_INTS * q;
       const int n_terms = E[0], term_i = E[1];
-      const T lam = a.reals[E[2]], den = a.reals[E[3]];
+      const T lw = a.reals[E[2]], nw = a.reals[E[3]], sw = a.reals[E[6]];
       T r = T(0);
       for (int t = 0; t < n_terms; ++t) {
         const int32_t *Tr = a.ints + term_i + TDX_AD_TERM_INTS * t;
@@ -331,9 +331,9 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_resid(ResArgs<T> a) {
       if (a.resid) a.resid[E[5] + e] = r;
       const T diff = E[4] >= 0 ? r - a.data[E[4] + e] : r;
       const T d2 = diff * diff;
-      lsum += lam * d2 / den;
-      lnsum += d2 / den;
-      const T gq = T(2) * lam * diff / den;
+      lsum += lw * d2;
+      lnsum += nw * d2;
+      const T gq = T(2) * sw * diff;
       for (int t = 0; t < n_terms; ++t) {
         const int32_t *Tr = a.ints + term_i + TDX_AD_TERM_INTS * t;
         const int nf = Tr[0];
@@ -414,6 +414,56 @@ __global__ void k_ad_scale(const T *seed, const T *dloss, T *du, int64_t n) {
 template <typename T>
 __global__ void k_ad_one(T *x) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *x = T(1);
+}
+
+// Causal loss (tedeous/losses.py:137-182).  Thread k walks column k of the
+// N = nt x K points of group 0 (point p = i K + k): res = sum_eq r_eq[p]^2,
+// w = exp(-tol * (sum of res over the earlier slices of the column)).
+template <typename T>
+__global__ void k_ad_causal_w(const T *resid, int n_eq, int N, int nt, int K, T tol, T *w,
+                              T *wr) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  T cum = T(0);
+  for (int i = 0; i < nt; ++i) {
+    const int p = i * K + k;
+    T res = T(0);
+    for (int q = 0; q < n_eq; ++q) {
+      const T r = resid[(int64_t)q * N + p];
+      res = (q == 0) ? r * r : res + r * r;
+    }
+    const T wv = exp(-tol * cum);
+    w[p] = wv;
+    wr[p] = wv * res;
+    cum = cum + res;
+  }
+}
+
+// The seeds of group 0's points (points 0 .. N-1, row of J * n_out) times w.
+template <typename T>
+__global__ void k_ad_causal_seeds(T *seed, const T *w, int N, int row) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)N * row) return;
+  seed[i] *= w[i / row];
+}
+
+// mean(w * res) in a fixed order, added to both loss values.
+template <typename T>
+__global__ __launch_bounds__(AD_THREADS) void k_ad_causal_loss(const T *wr, int N, T *loss) {
+  __shared__ T red[AD_THREADS];
+  T s = T(0);
+  for (int p = threadIdx.x; p < N; p += AD_THREADS) s += wr[p];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = AD_THREADS / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const T m = red[0] / T(N);
+    loss[0] = loss[0] + m;
+    loss[1] = loss[1] + m;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -626,6 +676,11 @@ struct tdx_ad_plan {
   int dtype, d_in, n_out, H, L, J;
   int64_t P, n_entries, n_resid, n_params;
   int n_groups, groups_i, jet_i, pptr_i, part_i, n_parts, n_occ;
+  // causal loss: tolerance, slices and points of group 0 (N = nt K), its
+  // equations; workspace for residuals, weights and w * res
+  int causal = 0, c_nt = 0, c_K = 0, c_N = 0, c_neq = 0;
+  double causal_tol = 0;
+  size_t off_resid = 0, off_cw = 0, off_cwr = 0;
   DevTable ints, reals, data, points, gstart, occ;
   std::vector<int64_t> off_w, off_b;
   std::vector<int> din, dout;
@@ -735,8 +790,7 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
       const int32_t *E = I + eq_i + TDX_AD_EQ_INTS * q;
       const int nt = E[0], ti = E[1];
       if (nt < 1 || !in_ints(ti, (int64_t)TDX_AD_TERM_INTS * nt)) return fail("equation terms");
-      if (!in_reals(E[2]) || !in_reals(E[3])) return fail("equation weights");
-      if (d->reals[E[3]] == 0.0) return fail("zero loss denominator");
+      if (!in_reals(E[2]) || !in_reals(E[3]) || !in_reals(E[6])) return fail("equation weights");
       if (E[4] >= 0 && !in_data(E[4], ne)) return fail("equation target");
       if (E[5] < 0 || E[5] + (int64_t)ne > d->n_resid) return fail("residual offset");
       n_resid_used = std::max<int64_t>(n_resid_used, E[5] + (int64_t)ne);
@@ -764,6 +818,15 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
     gstart[g + 1] = gstart[g] + ne;
   }
   if (gstart[d->n_groups] < 1) return fail("no entries");
+  if (d->causal_tol != 0.0) {
+    const int32_t *G0 = I + d->groups_i;
+    if (!std::isfinite(d->causal_tol)) return fail("causal tol must be finite");
+    if (G0[1] != 1 || I[G0[2]] != 0) return fail("causal loss: group 0 must hold points 0 .. N-1");
+    if (d->causal_nt < 1 || G0[0] % d->causal_nt != 0) return fail("causal slices must divide group 0");
+    for (int q = 0; q < G0[3]; ++q)
+      if (I[G0[4] + TDX_AD_EQ_INTS * q + 5] != q * G0[0])
+        return fail("causal loss: group 0's residuals must lead the residual output");
+  }
   if (gstart[d->n_groups] > (int64_t)1 << 31) return fail("more than 2^31 entries");
 
   const size_t ts = d->dtype == 0 ? 4 : 8;
@@ -858,6 +921,18 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
   p->off_grad = take((size_t)p->n_params * ts);
   for (int l = 0; l < p->L; ++l) p->off_wt.push_back(take((size_t)p->din[l] * p->dout[l] * ts));
   p->off_one = take(ts);
+  if (d->causal_tol != 0.0) {
+    const int32_t *G0 = I + d->groups_i;
+    p->causal = 1;
+    p->causal_tol = d->causal_tol;
+    p->c_N = G0[0];
+    p->c_neq = G0[3];
+    p->c_nt = d->causal_nt;
+    p->c_K = p->c_N / p->c_nt;
+    p->off_resid = take((size_t)std::max<int64_t>(d->n_resid, 1) * ts);
+    p->off_cw = take((size_t)p->c_N * ts);
+    p->off_cwr = take((size_t)p->c_N * ts);
+  }
   p->total = w;
 
   // tables: reals and data in the working precision
@@ -977,13 +1052,28 @@ int ad_forward(const tdx_ad_plan *p, const T *params, T *loss, T *resid, void *w
   r.groups_i = p->groups_i;
   r.U = wsp<T>(ws, p->off_u);
   r.seed = wsp<T>(ws, p->off_seed);
-  r.resid = resid;
+  // the causal loss reads group 0's residuals
+  r.resid = (p->causal && !resid) ? wsp<T>(ws, p->off_resid) : resid;
   r.lp = wsp<T>(ws, p->off_lp);
   hipLaunchKernelGGL((k_ad_resid<T>), dim3(p->n_res_blocks), dim3(AD_THREADS), 0, s, r);
   TDX_TRY_HIP(hipGetLastError());
   hipLaunchKernelGGL((k_ad_loss<T>), dim3(1), dim3(AD_THREADS), 0, s, (const T *)r.lp,
                      p->n_res_blocks, loss);
   TDX_TRY_HIP(hipGetLastError());
+  if (p->causal) {
+    T *w = wsp<T>(ws, p->off_cw), *wr = wsp<T>(ws, p->off_cwr);
+    hipLaunchKernelGGL((k_ad_causal_w<T>), dim3((p->c_K + 255) / 256), dim3(256), 0, s,
+                       (const T *)r.resid, p->c_neq, p->c_N, p->c_nt, p->c_K, T(p->causal_tol), w,
+                       wr);
+    TDX_TRY_HIP(hipGetLastError());
+    const int row = J * p->n_out;
+    hipLaunchKernelGGL((k_ad_causal_seeds<T>), dim3((unsigned)(((int64_t)p->c_N * row + 255) / 256)),
+                       dim3(256), 0, s, r.seed, (const T *)w, p->c_N, row);
+    TDX_TRY_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_ad_causal_loss<T>), dim3(1), dim3(AD_THREADS), 0, s, (const T *)wr,
+                       p->c_N, loss);
+    TDX_TRY_HIP(hipGetLastError());
+  }
   return TDX_OK;
 }
 

@@ -94,6 +94,7 @@ class ADDesc(ctypes.Structure):
         ('n_reals', ctypes.c_int64), ('reals', c_dp),
         ('n_data', ctypes.c_int64), ('data', ctypes.c_void_p),
         ('n_resid', ctypes.c_int64),
+        ('causal_tol', ctypes.c_double), ('causal_nt', ctypes.c_int),
     ]
 
 

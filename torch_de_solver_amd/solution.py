@@ -61,8 +61,9 @@ class Solution:
                  exact_div: bool = True):
         if weak_form and mode != 'NN':
             raise NotImplementedError("weak-form losses are served in 'NN' mode only")
-        if tol and mode != 'NN':
-            raise NotImplementedError("causal losses (tol != 0) are served in 'NN' mode only")
+        if tol and mode not in ('NN', 'autograd'):
+            raise NotImplementedError("causal losses (tol != 0) are served in 'NN' and "
+                                      "'autograd' modes")
         if batch_size is not None and mode != 'NN':
             raise NotImplementedError("mini-batching is served in 'NN' mode only")
         if batch_size is not None and tol:
@@ -93,7 +94,7 @@ class Solution:
             self.arch = mlp_arch(model)
             self.problem: ADProblem = compile_ad(
                 grid, ops, equal_cls.bconds, n_out=self.arch.n_out,
-                lambda_operator=lambda_operator, lambda_bound=lambda_bound)
+                lambda_operator=lambda_operator, lambda_bound=lambda_bound, tol=tol)
             self.engine = ADEngine(self.problem, self.arch, device)
         else:
             self.problem: MatProblem = compile_mat(
