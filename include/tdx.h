@@ -362,6 +362,19 @@ typedef struct tdx_ad_desc {
    * seeds of group 0's points are scaled by w (no gradient through w). */
   double causal_tol;
   int causal_nt;
+  /* Weak form (tedeous/eval.py:13-52, 195-221, tedeous/losses.py:184-228):
+   * weak_levels 1 (1-D) or 2 (2-D) adds sum_eq lambda_eq W_eq over the N
+   * points of group 0 (its equations carry zero loss weights and seed
+   * weight lambda_eq), W_eq = sum_i c_i r_i^2 on one level and
+   * sum_g x_g (sum_{i in run g} c_i r_i^2)^2 on two; c_i = data[weak_d + i],
+   * x_g = data[weak_d + N + g], lambda_eq = data[weak_d + N + weak_runs + eq],
+   * run g = points ints[weak_i + g] .. ints[weak_i + g + 1] - 1.  The
+   * residual program runs a second time with each equation's seeds scaled
+   * by dW/dr / (2 r). */
+  int weak_levels;
+  int weak_runs;
+  int64_t weak_d;
+  int weak_i;
 } tdx_ad_desc;
 
 typedef struct tdx_ad_plan tdx_ad_plan;

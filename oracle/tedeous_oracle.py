@@ -725,9 +725,16 @@ class ADOracle:
                 true[k] = c['bval'].reshape(-1)
         return pad_matrix(bval, true)
 
-    def evaluate(self, net, lam_op=1, lam_bnd=1, tol=0):
+    def evaluate(self, net, lam_op=1, lam_bnd=1, tol=0, weak_form=None):
         op = self.residual(net)
         b, t, keys, lens = self.boundary(net)
+        if weak_form:
+            # the residual on the grid, weighted and integrated
+            # (tedeous/eval.py:195-221, tedeous/losses.py:184-228)
+            ow = weak_operator(op, self.grid, weak_form)
+            loss, norm = weak_loss(ow, b, t, lam_row(op, lam_op).to(op.dtype),
+                                   lam_row(b, lam_bnd).to(op.dtype))
+            return loss, norm, op, b, t, keys, lens
         if tol:
             # slices: distinct first coordinates of the grid
             # (tedeous/solution.py:55-57)

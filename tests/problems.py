@@ -489,3 +489,23 @@ WEAK_CASES = {
     'lotka_weak_nn': (lotka_weak_nn, [1, 16, 16, 2]),
     'burgers_weak_nn': (burgers_weak_nn, [2, 12, 12, 12, 1]),
 }
+
+
+def coupled_weak_ad(api, n=10, dtype='float64'):
+    """Two coupled equations on a 2-D grid in weak form ('autograd'):
+    u_t - v_x = 0, v_t - u_x + u v = 0."""
+    d = api.Domain()
+    d.variable('x', [0, 1], n, dtype=dtype)
+    d.variable('t', [0, 1], n, dtype=dtype)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=torch.sin(np.pi * x), var=0)
+    b.dirichlet({'x': [0, 1], 't': 0}, value=0.0, var=1)
+    eq = api.Equation()
+    eq.add({'du/dt': {'coeff': 1, 'du/dt': [1], 'pow': 1, 'var': 0},
+            '-dv/dx': {'coeff': -1, 'dv/dx': [0], 'pow': 1, 'var': 1}})
+    eq.add({'dv/dt': {'coeff': 1, 'dv/dt': [1], 'pow': 1, 'var': 1},
+            '-du/dx': {'coeff': -1, 'du/dx': [0], 'pow': 1, 'var': 0},
+            'u*v': {'coeff': 1, 'u*v': [[None], [None]], 'pow': [1, 1], 'var': [0, 1]}})
+    weak = [lambda g: torch.sin(np.pi * g[:, 0]) + 0.5, lambda g: 1 + g[:, 1] ** 2]
+    return d, eq, b, dict(mode='autograd', lambda_operator=[1, 3], lambda_bound=10), weak

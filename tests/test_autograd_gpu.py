@@ -201,3 +201,34 @@ def test_ad_causal_model_fused_and_torch_optimizer_agree():
         model.train(tdx.Optimizer('Adam', {'lr': 1e-3}), 21, callbacks=None if fused else [object()])
         res.append(torch.cat([p.detach().reshape(-1) for p in net.parameters()]).cpu())
     np.testing.assert_allclose(res[0].numpy(), res[1].numpy(), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize('case', ['coupled', 'legendre'])
+def test_ad_weak_form_against_oracle(case):
+    """The weak form in 'autograd' mode on the grid points: a 2-D system of
+    two equations (each equation's seeds rescaled by its own run sums) and a
+    1-D equation; float64 to 1e-9 / 1e-8."""
+    _print(f"[ad weak f64] {case}")
+    if case == 'coupled':
+        d, eq, b, kw, weak = problems.coupled_weak_ad(api)
+        layers = [2, 16, 16, 2]
+    else:
+        d, eq, b, kw = problems.legendre_ad(api)
+        weak = [lambda g: torch.cos(0.5 * np.pi * g[:, 0]) + 0.1]
+        layers = [1, 32, 32, 1]
+    net = _net(layers, torch.float64)
+    orc = O.ADOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst])
+    lo, no, *_ = orc.evaluate(net, kw['lambda_operator'], kw['lambda_bound'], weak_form=weak)
+    lo.sum().backward()
+    go = torch.cat([p.grad.reshape(-1) for p in net.parameters()]).numpy()
+    net_dev = _net(layers, torch.float64).to(DEV)
+    grid = d.build('autograd')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst,
+                                 b.build(d.variable_dict)).set_strategy('autograd')
+    sol = Solution(grid, pre, net_dev, 'autograd', weak, kw['lambda_operator'], kw['lambda_bound'])
+    ln, nn = sol.evaluate()
+    ln.backward()
+    gn = torch.cat([p.grad.reshape(-1) for p in net_dev.parameters()]).cpu().numpy()
+    np.testing.assert_allclose(float(ln), float(lo.sum()), rtol=1e-9)
+    np.testing.assert_allclose(float(nn), float(no.sum()), rtol=1e-9)
+    np.testing.assert_allclose(gn, go, rtol=1e-8, atol=1e-10 * np.abs(go).max())

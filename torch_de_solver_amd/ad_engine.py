@@ -112,7 +112,11 @@ class ADTables:
                         ck, cr = 0, add_real(t.coeff_scalar)
                     ints[term_i + 4 * ti: term_i + 4 * ti + 4] = [len(t.factors), fac_i, ck, cr]
                 tgt = add_data(eq.target) if eq.target is not None else -1
-                if prob.causal_tol and gi == 0:
+                if prob.weak_levels and gi == 0:
+                    # weak form: sum_eq lambda W_eq, formed after the forward
+                    # pass; the seeds carry lambda here and dW/dr / (2 r) there
+                    lw, nw, sw = 0.0, 0.0, eq.lam
+                elif prob.causal_tol and gi == 0:
                     # causal loss: the operator term is mean(w res), formed
                     # after the forward pass; the seeds carry 1/N here and w there
                     lw, nw, sw = 0.0, 0.0, 1.0 / eq.denom
@@ -122,6 +126,12 @@ class ADTables:
                                                           add_real(nw), tgt, roff, add_real(sw)]
             ints[self.groups_i + 5 * gi: self.groups_i + 5 * gi + 5] = [
                 ne, len(g.sets), set_i, len(g.equations), eq_i]
+        self.weak_d = self.weak_i = 0
+        if prob.weak_levels:
+            lams = [eq.lam for eq in prob.groups[0].equations]
+            self.weak_d = add_data(np.concatenate([prob.weak_ctf2, prob.weak_cx, lams]))
+            self.weak_i = len(ints)
+            ints += [int(v) for v in prob.weak_gstart]
         self.n_jets = J
         self.ints = np.asarray(ints, dtype=np.int32)
         self.reals = np.asarray(reals, dtype=np.float64)
@@ -168,6 +178,9 @@ class ADEngine:
         desc.n_resid = t.n_resid
         desc.causal_tol = float(prob.causal_tol)
         desc.causal_nt = int(prob.causal_nt)
+        if prob.weak_levels:
+            desc.weak_levels, desc.weak_runs = prob.weak_levels, len(prob.weak_cx)
+            desc.weak_d, desc.weak_i = t.weak_d, t.weak_i
         handle = ctypes.c_void_p()
         with torch.cuda.device(device):
             native.check(lib.tdx_ad_plan_create(ctypes.byref(desc), ctypes.byref(handle)))

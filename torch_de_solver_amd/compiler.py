@@ -687,6 +687,11 @@ class ADProblem:
     # slices the grid points are read in (0: default loss)
     causal_tol: float = 0.0
     causal_nt: int = 0
+    # weak form on the grid points (as NNProblem's, any number of equations)
+    weak_levels: int = 0
+    weak_ctf2: Optional[np.ndarray] = None
+    weak_cx: Optional[np.ndarray] = None
+    weak_gstart: Optional[np.ndarray] = None
 
     def partitions(self) -> List[List[List[int]]]:
         """Per jet, its set partitions as lists of block jets (Faa di Bruno:
@@ -770,7 +775,8 @@ def _ad_terms(terms: List[Term], pts: torch.Tensor, set_k: int, sign: float = 1.
 
 
 def compile_ad(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[dict]],
-               n_out: int, lambda_operator=1, lambda_bound=1, tol: float = 0.0) -> ADProblem:
+               n_out: int, lambda_operator=1, lambda_bound=1, tol: float = 0.0,
+               weak_form=None) -> ADProblem:
     """Lower an 'autograd' problem (tedeous/input_preprocessing.py:411-505,
     tedeous/eval.py:90-461 with mode 'autograd', tedeous/losses.py:84-135).
 
@@ -888,7 +894,14 @@ def compile_ad(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[
         causal_nt = int(torch.unique(grid_cpu[:, 0]).numel())
         if causal_nt == 0 or n % causal_nt != 0:
             causal_nt = n
-    return ADProblem(d_in=d, n_out=n_out, jets=jet_list, groups=groups, n_grid=n,
+    prob = ADProblem(d_in=d, n_out=n_out, jets=jet_list, groups=groups, n_grid=n,
                      bc_keys=keys, bc_lengths=[col_len[k] for k in keys],
                      bc_max_len=max_len, dtype=dtype, causal_tol=float(tol),
                      causal_nt=causal_nt)
+    if weak_form:
+        if tol:
+            raise NotImplementedError("a weak form with a causal loss is not served")
+        # the operator points of this mode are the grid (tedeous/eval.py:206-207)
+        lv, ctf2, cx, gstart = weak_tables(grid_cpu, weak_form)
+        prob.weak_levels, prob.weak_ctf2, prob.weak_cx, prob.weak_gstart = lv, ctf2, cx, gstart
+    return prob

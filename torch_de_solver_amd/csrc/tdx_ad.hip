@@ -270,6 +270,9 @@ struct ResArgs {
   int groups_i;
   const T *U;
   T *seed, *resid, *lp;
+  // seed multipliers of group 0's equations, [n_eq][N] (null: none)
+  const T *mult;
+  int mult_n;
 };
 
 template <typename T>
@@ -333,7 +336,7 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_resid(ResArgs<T> a) {
       const T d2 = diff * diff;
       lsum += lw * d2;
       lnsum += nw * d2;
-      const T gq = T(2) * sw * diff;
+      const T gq = T(2) * sw * diff * ((a.mult && g == 0) ? a.mult[q * a.mult_n + e] : T(1));
       for (int t = 0; t < n_terms; ++t) {
         const int32_t *Tr = a.ints + term_i + TDX_AD_TERM_INTS * t;
         const int nf = Tr[0];
@@ -445,6 +448,64 @@ __global__ void k_ad_causal_seeds(T *seed, const T *w, int N, int row) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= (int64_t)N * row) return;
   seed[i] *= w[i / row];
+}
+
+// Weak form (tedeous/eval.py:13-52, 195-221).  R[q][g] = sum over run g
+// of c_i r_q[i]^2, one thread per (run, equation), points in order.
+template <typename T>
+__global__ void k_ad_weak_runs(const T *resid, int n_eq, int N, int G, const int32_t *gstart,
+                               const T *c, T *R) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G * n_eq) return;
+  const int q = t / G, g = t - (t / G) * G;
+  T s = T(0);
+  for (int i = gstart[g]; i < gstart[g + 1]; ++i) {
+    const T r = resid[(int64_t)q * N + i];
+    s += c[i] * (r * r);
+  }
+  R[t] = s;
+}
+
+// Seed multipliers M[q][i] = c_i (one level) or 2 x_g R[q][g] c_i (two),
+// one workgroup per (run, equation).
+template <typename T>
+__global__ void k_ad_weak_mult(int levels, int N, int G, const int32_t *gstart, const T *c,
+                               const T *x, const T *R, T *M) {
+  const int g = blockIdx.x % G, q = blockIdx.x / G;
+  const T f = levels == 2 ? T(2) * x[g] * R[q * G + g] : T(1);
+  for (int i = gstart[g] + threadIdx.x; i < gstart[g + 1]; i += blockDim.x)
+    M[(int64_t)q * N + i] = f * c[i];
+}
+
+// sum_q lambda_q W_q to the loss, sum_q W_q to the normalized loss, with
+// W_q = sum_g R[q][g] (one level: one run) or sum_g x_g R[q][g]^2.
+template <typename T>
+__global__ __launch_bounds__(AD_THREADS) void k_ad_weak_loss(int levels, int n_eq, int G,
+                                                             const T *x, const T *lam,
+                                                             const T *R, T *loss) {
+  __shared__ T red[AD_THREADS];
+  T tot = T(0), totn = T(0);
+  for (int q = 0; q < n_eq; ++q) {
+    T s = T(0);
+    for (int g = threadIdx.x; g < G; g += AD_THREADS) {
+      const T v = R[q * G + g];
+      s += levels == 2 ? x[g] * (v * v) : v;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = AD_THREADS / 2; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+    const T W = red[0];
+    __syncthreads();
+    tot = tot + lam[q] * W;
+    totn = totn + W;
+  }
+  if (threadIdx.x == 0) {
+    loss[0] = loss[0] + tot;
+    loss[1] = loss[1] + totn;
+  }
 }
 
 // mean(w * res) in a fixed order, added to both loss values.
@@ -681,6 +742,11 @@ struct tdx_ad_plan {
   int causal = 0, c_nt = 0, c_K = 0, c_N = 0, c_neq = 0;
   double causal_tol = 0;
   size_t off_resid = 0, off_cw = 0, off_cwr = 0;
+  // weak form: levels (0 none), runs, group 0's points and equations, table
+  // offsets; workspace for the runs' sums and the seed multipliers
+  int weak = 0, w_G = 0, w_N = 0, w_neq = 0;
+  int64_t w_d = 0, w_i = 0;
+  size_t off_wR = 0, off_wM = 0;
   DevTable ints, reals, data, points, gstart, occ;
   std::vector<int64_t> off_w, off_b;
   std::vector<int> din, dout;
@@ -818,6 +884,23 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
     gstart[g + 1] = gstart[g] + ne;
   }
   if (gstart[d->n_groups] < 1) return fail("no entries");
+  if (d->weak_levels != 0) {
+    const int32_t *G0 = I + d->groups_i;
+    const int N = G0[0], G = d->weak_runs, neq = G0[3];
+    if (d->weak_levels != 1 && d->weak_levels != 2) return fail("weak form: 1 or 2 levels");
+    if (d->causal_tol != 0.0) return fail("a weak form with a causal loss");
+    if (G < 1 || (d->weak_levels == 1 && G != 1)) return fail("weak form: runs");
+    if (G0[1] != 1 || I[G0[2]] != 0) return fail("weak form: group 0 must hold points 0 .. N-1");
+    if (!in_data(d->weak_d, (int64_t)N + G + neq)) return fail("weak form: data range");
+    if (!in_ints(d->weak_i, G + 1)) return fail("weak form: run table range");
+    const int32_t *gs = I + d->weak_i;
+    if (gs[0] != 0 || gs[G] != N) return fail("weak form: runs must cover group 0");
+    for (int g = 0; g < G; ++g)
+      if (gs[g + 1] <= gs[g]) return fail("weak form: empty run");
+    for (int q = 0; q < neq; ++q)
+      if (I[G0[4] + TDX_AD_EQ_INTS * q + 5] != q * N)
+        return fail("weak form: group 0's residuals must lead the residual output");
+  }
   if (d->causal_tol != 0.0) {
     const int32_t *G0 = I + d->groups_i;
     if (!std::isfinite(d->causal_tol)) return fail("causal tol must be finite");
@@ -932,6 +1015,18 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
     p->off_resid = take((size_t)std::max<int64_t>(d->n_resid, 1) * ts);
     p->off_cw = take((size_t)p->c_N * ts);
     p->off_cwr = take((size_t)p->c_N * ts);
+  }
+  if (d->weak_levels != 0) {
+    const int32_t *G0 = I + d->groups_i;
+    p->weak = d->weak_levels;
+    p->w_N = G0[0];
+    p->w_neq = G0[3];
+    p->w_G = d->weak_runs;
+    p->w_d = d->weak_d;
+    p->w_i = d->weak_i;
+    if (!p->off_resid) p->off_resid = take((size_t)std::max<int64_t>(d->n_resid, 1) * ts);
+    p->off_wR = take((size_t)p->w_G * p->w_neq * ts);
+    p->off_wM = take((size_t)p->w_N * p->w_neq * ts);
   }
   p->total = w;
 
@@ -1052,14 +1147,41 @@ int ad_forward(const tdx_ad_plan *p, const T *params, T *loss, T *resid, void *w
   r.groups_i = p->groups_i;
   r.U = wsp<T>(ws, p->off_u);
   r.seed = wsp<T>(ws, p->off_seed);
-  // the causal loss reads group 0's residuals
-  r.resid = (p->causal && !resid) ? wsp<T>(ws, p->off_resid) : resid;
+  // the causal loss and the weak form read group 0's residuals
+  r.resid = ((p->causal || p->weak) && !resid) ? wsp<T>(ws, p->off_resid) : resid;
   r.lp = wsp<T>(ws, p->off_lp);
+  r.mult = nullptr;
+  r.mult_n = 0;
   hipLaunchKernelGGL((k_ad_resid<T>), dim3(p->n_res_blocks), dim3(AD_THREADS), 0, s, r);
   TDX_TRY_HIP(hipGetLastError());
+  if (p->weak) {
+    // runs' sums from the residuals, each equation's seed multipliers, then
+    // the residual program again with them (group 0 carries no other loss)
+    const T *c = (const T *)p->data.ptr + p->w_d;
+    const T *x = c + p->w_N, *lam = x + p->w_G;
+    const int32_t *gs = (const int32_t *)p->ints.ptr + p->w_i;
+    T *R = wsp<T>(ws, p->off_wR), *M = wsp<T>(ws, p->off_wM);
+    hipLaunchKernelGGL((k_ad_weak_runs<T>), dim3((p->w_G * p->w_neq + 255) / 256), dim3(256), 0,
+                       s, (const T *)r.resid, p->w_neq, p->w_N, p->w_G, gs, c, R);
+    TDX_TRY_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_ad_weak_mult<T>), dim3(p->w_G * p->w_neq), dim3(256), 0, s, p->weak,
+                       p->w_N, p->w_G, gs, c, x, (const T *)R, M);
+    TDX_TRY_HIP(hipGetLastError());
+    ResArgs<T> r2 = r;
+    r2.mult = M;
+    r2.mult_n = p->w_N;
+    hipLaunchKernelGGL((k_ad_resid<T>), dim3(p->n_res_blocks), dim3(AD_THREADS), 0, s, r2);
+    TDX_TRY_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL((k_ad_loss<T>), dim3(1), dim3(AD_THREADS), 0, s, (const T *)r.lp,
                      p->n_res_blocks, loss);
   TDX_TRY_HIP(hipGetLastError());
+  if (p->weak) {
+    const T *x = (const T *)p->data.ptr + p->w_d + p->w_N, *lam = x + p->w_G;
+    hipLaunchKernelGGL((k_ad_weak_loss<T>), dim3(1), dim3(AD_THREADS), 0, s, p->weak, p->w_neq,
+                       p->w_G, x, lam, (const T *)wsp<T>(ws, p->off_wR), loss);
+    TDX_TRY_HIP(hipGetLastError());
+  }
   if (p->causal) {
     T *w = wsp<T>(ws, p->off_cw), *wr = wsp<T>(ws, p->off_cwr);
     hipLaunchKernelGGL((k_ad_causal_w<T>), dim3((p->c_K + 255) / 256), dim3(256), 0, s,

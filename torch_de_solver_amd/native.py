@@ -95,6 +95,8 @@ class ADDesc(ctypes.Structure):
         ('n_data', ctypes.c_int64), ('data', ctypes.c_void_p),
         ('n_resid', ctypes.c_int64),
         ('causal_tol', ctypes.c_double), ('causal_nt', ctypes.c_int),
+        ('weak_levels', ctypes.c_int), ('weak_runs', ctypes.c_int),
+        ('weak_d', ctypes.c_int64), ('weak_i', ctypes.c_int),
     ]
 
 

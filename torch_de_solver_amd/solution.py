@@ -59,8 +59,8 @@ class Solution:
                  weak_form=None, lambda_operator=1, lambda_bound=1, tol: float = 0,
                  derivative_points: int = 2, batch_size: int = None,
                  exact_div: bool = True):
-        if weak_form and mode != 'NN':
-            raise NotImplementedError("weak-form losses are served in 'NN' mode only")
+        if weak_form and mode not in ('NN', 'autograd'):
+            raise NotImplementedError("weak-form losses are served in 'NN' and 'autograd' modes")
         if tol and mode not in ('NN', 'autograd'):
             raise NotImplementedError("causal losses (tol != 0) are served in 'NN' and "
                                       "'autograd' modes")
@@ -94,7 +94,8 @@ class Solution:
             self.arch = mlp_arch(model)
             self.problem: ADProblem = compile_ad(
                 grid, ops, equal_cls.bconds, n_out=self.arch.n_out,
-                lambda_operator=lambda_operator, lambda_bound=lambda_bound, tol=tol)
+                lambda_operator=lambda_operator, lambda_bound=lambda_bound, tol=tol,
+                weak_form=weak_form)
             self.engine = ADEngine(self.problem, self.arch, device)
         else:
             self.problem: MatProblem = compile_mat(
