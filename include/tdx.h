@@ -116,7 +116,8 @@ typedef struct tdx_nn_desc {
    * one instance: weak_levels 1 or 2 replaces the operator term of group
    * 0's equations (loss and normalized weights 0, seed weight lambda) by
    * sum_eq lambda_eq W_eq, with on one level W = sum_i c_i r_i^2 and on two
-   * levels (one equation) W = sum_g x_g (sum_{i in run g} c_i r_i^2)^2,
+   * levels W = sum_g x_g (sum_{i in run g} c_i r_i^2)^2 (several equations
+   * on two levels: the forward pass runs again with per-equation seeds),
    * c_i = data[weak_d + i] (N values: trapezoid weight times the squared
    * basis product), x_g = data[weak_d + N + g] (weak_runs values),
    * lambda_eq = data[weak_d + N + weak_runs + eq]; run g holds points

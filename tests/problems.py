@@ -509,3 +509,13 @@ def coupled_weak_ad(api, n=10, dtype='float64'):
             'u*v': {'coeff': 1, 'u*v': [[None], [None]], 'pow': [1, 1], 'var': [0, 1]}})
     weak = [lambda g: torch.sin(np.pi * g[:, 0]) + 0.5, lambda g: 1 + g[:, 1] ** 2]
     return d, eq, b, dict(mode='autograd', lambda_operator=[1, 3], lambda_bound=10), weak
+
+
+def coupled_weak_nn(api, dtype='float64'):
+    """The coupled two-equation system in weak form on a 2-D grid, 'NN' mode
+    (each equation's seeds carry its own run sums: a second forward pass)."""
+    d, eq, b, kw, weak = coupled_weak_ad(api, dtype=dtype)
+    return d, eq, b, dict(mode='NN', lambda_operator=[1, 3], lambda_bound=10, h=1e-3), weak
+
+
+WEAK_CASES['coupled_weak_nn'] = (coupled_weak_nn, [2, 16, 16, 2])

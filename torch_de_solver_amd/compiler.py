@@ -418,8 +418,6 @@ def compile_nn(grid: torch.Tensor, operators: List[dict], bconds: Optional[List[
         if tol:
             raise NotImplementedError("a weak form with a causal loss is not served")
         lv, ctf2, cx, gstart = weak_tables(central_pts, weak_form)
-        if lv == 2 and n_eq > 1:
-            raise NotImplementedError("the weak form of a system is served on 1-D grids")
         prob.weak_levels, prob.weak_ctf2, prob.weak_cx, prob.weak_gstart = lv, ctf2, cx, gstart
     return prob
 

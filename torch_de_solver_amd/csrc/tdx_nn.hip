@@ -69,6 +69,9 @@ struct NNArgs {
   const T *dloss;
   int acc_lds;                // backward accumulates in LDS
   int tile_rows;
+  // seed multipliers of group 0's equations, [n_eq][N] (null: none)
+  const T *seed_mult;
+  int seed_mult_n;
 };
 
 template <typename T>
@@ -400,7 +403,8 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
         const T lam_d = a.reals[E[4]];
         lsum += lam_d * sq;
         lnsum += a.reals[E[5]] * sq;
-        const T dr = T(2) * a.reals[E[6]] * diff;
+        T dr = T(2) * a.reals[E[6]] * diff;
+        if (a.seed_mult && item.x == 0) dr *= a.seed_mult[(int64_t)eq * a.seed_mult_n + e];
         // seeds: d r / d u(slot, var), through every factor of every term;
         // d(prod_j f_j^p_j)/d f_k = p_k f_k^(p_k-1) * prefix_k * suffix_k
         for (int t = 0; t < n_terms; ++t) {
@@ -1404,6 +1408,17 @@ __global__ void k_weak_mult(int levels, const int32_t *gstart, const T *c, const
   for (int i = gstart[g] + threadIdx.x; i < gstart[g + 1]; i += blockDim.x) m[i] = f * c[i];
 }
 
+// Per-equation multipliers M[q][i] = 2 x_g R[q][g] c_i (two levels, several
+// equations); one workgroup per (run, equation).
+template <typename T>
+__global__ void k_weak_mult_eq(int N, int G, const int32_t *gstart, const T *c, const T *x,
+                               const T *R, T *M) {
+  const int g = blockIdx.x % G, q = blockIdx.x / G;
+  const T f = T(2) * x[g] * R[q * G + g];
+  for (int i = gstart[g] + threadIdx.x; i < gstart[g + 1]; i += blockDim.x)
+    M[(int64_t)q * N + i] = f * c[i];
+}
+
 // sum_q lambda_q W_q added to the loss, sum_q W_q to the normalized loss;
 // W_q = R[q][0] on one level, sum_g x_g R[q][g]^2 on two (fixed order).
 template <typename T>
@@ -1660,6 +1675,8 @@ NNArgs<T> make_args(const tdx_nn_plan *p, const void *params, void *ws) {
   a.dloss = (const T *)(w + p->off_one);
   a.acc_lds = p->acc_lds;
   a.tile_rows = 16 * p->k2_waves;
+  a.seed_mult = nullptr;
+  a.seed_mult_n = 0;
   return a;
 }
 
@@ -1714,13 +1731,29 @@ int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStre
     hipLaunchKernelGGL((k_weak_runs<T>), dim3((p->w_G * p->w_neq + 255) / 256), dim3(256), 0, s,
                        (const T *)resid, p->w_neq, p->w_N, p->w_G, gs, c, R);
     TDX_TRY_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_weak_mult<T>), dim3(p->w_G), dim3(256), 0, s, p->weak, gs, c, x,
-                       (const T *)R, m);
-    TDX_TRY_HIP(hipGetLastError());
-    const int64_t tot = (int64_t)p->c_slots * p->w_N * p->n_out;
-    hipLaunchKernelGGL((k_causal_seeds<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
-                       a.seeds, (const T *)m, p->w_N, p->c_slots, p->n_out);
-    TDX_TRY_HIP(hipGetLastError());
+    if (p->weak == 2 && p->w_neq > 1) {
+      // one multiplier per equation: the forward pass again, each
+      // equation's seeds of group 0 scaled by its own 2 x_g R_g c_i
+      hipLaunchKernelGGL((k_weak_mult_eq<T>), dim3(p->w_G * p->w_neq), dim3(256), 0, s, p->w_N,
+                         p->w_G, gs, c, x, (const T *)R, m);
+      TDX_TRY_HIP(hipGetLastError());
+      NNArgs<T> a2 = a;
+      a2.seed_mult = m;
+      a2.seed_mult_n = p->w_N;
+      if (p->k1_stage)
+        hipLaunchKernelGGL((k_forward<T, NB, true>), dim3(p->n_k1), dim3(256), p->k1_lds, s, a2);
+      else
+        hipLaunchKernelGGL((k_forward<T, NB, false>), dim3(p->n_k1), dim3(256), p->k1_lds, s, a2);
+      TDX_TRY_HIP(hipGetLastError());
+    } else {
+      hipLaunchKernelGGL((k_weak_mult<T>), dim3(p->w_G), dim3(256), 0, s, p->weak, gs, c, x,
+                         (const T *)R, m);
+      TDX_TRY_HIP(hipGetLastError());
+      const int64_t tot = (int64_t)p->c_slots * p->w_N * p->n_out;
+      hipLaunchKernelGGL((k_causal_seeds<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                         s, a.seeds, (const T *)m, p->w_N, p->c_slots, p->n_out);
+      TDX_TRY_HIP(hipGetLastError());
+    }
     if (loss) {
       hipLaunchKernelGGL((k_weak_loss<T>), dim3(1), dim3(256), 0, s, p->weak, p->w_neq, p->w_G, x,
                          lam, (const T *)R, loss);
@@ -1965,7 +1998,6 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
     TDX_REQUIRE(d->n_inst == 1, "the weak form serves one instance");
     TDX_REQUIRE(d->causal_tol == 0.0, "a weak form with a causal loss");
     TDX_REQUIRE(G >= 1 && (d->weak_levels == 2 || G == 1), "weak form: runs");
-    TDX_REQUIRE(d->weak_levels == 1 || neq == 1, "weak form on two levels: one equation");
     TDX_REQUIRE(d->weak_d >= 0 && d->weak_d + N + G + neq <= d->n_data, "weak form: data range");
     TDX_REQUIRE(d->weak_i >= 0 && d->weak_i + G + 1 <= d->n_ints, "weak form: run table range");
     const int32_t *gs = d->ints + d->weak_i;
@@ -2135,7 +2167,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
     p->off_wR = off;
     off = align_up(off + (size_t)p->w_G * p->w_neq * ts, 256);
     p->off_wm = off;
-    off = align_up(off + (size_t)p->w_N * ts, 256);
+    off = align_up(off + (size_t)p->w_N * p->w_neq * ts, 256);
   }
   p->ws_bytes = off;
   *out = p;
