@@ -519,3 +519,28 @@ def coupled_weak_nn(api, dtype='float64'):
 
 
 WEAK_CASES['coupled_weak_nn'] = (coupled_weak_nn, [2, 16, 16, 2])
+
+
+def heat3_ad(api, n=5, dtype='float64'):
+    """Three inputs: u_t = 0.3 (u_xx + u_yy) + 0.1 u_xyt on the unit cube,
+    u = sin(pi x) sin(pi y) at t = 0, an operator condition u_x + u_y at
+    x = 0 ('autograd')."""
+    d = api.Domain()
+    d.variable('x', [0, 1], n, dtype=dtype)
+    d.variable('y', [0, 1], n, dtype=dtype)
+    d.variable('t', [0, 1], n, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'x': [0, 1], 'y': [0, 1], 't': 0},
+                value=lambda g: torch.sin(np.pi * g[:, 0]) * torch.sin(np.pi * g[:, 1]))
+    b.operator({'x': 0, 'y': [0, 1], 't': [0, 1]},
+               operator={'u_x': {'coeff': 1, 'u_x': [0], 'pow': 1},
+                         'u_y': {'coeff': 1, 'u_y': [1], 'pow': 1}}, value=0.0)
+    eq = api.Equation()
+    eq.add({'u_t': {'coeff': 1, 'u_t': [2], 'pow': 1},
+            '-0.3u_xx': {'coeff': -0.3, 'u_xx': [0, 0], 'pow': 1},
+            '-0.3u_yy': {'coeff': -0.3, 'u_yy': [1, 1], 'pow': 1},
+            '-0.1u_xyt': {'coeff': -0.1, 'u_xyt': [0, 1, 2], 'pow': 1}})
+    return d, eq, b, dict(mode='autograd', lambda_operator=1, lambda_bound=10)
+
+
+AD_CASES['heat3_ad'] = (heat3_ad, [3, 20, 20, 1])
