@@ -550,7 +550,7 @@ def bench_ad(args):
         "colloc_pts_per_s": N * 1e3 / ms,
         "roofline": {"bound": "mfma",
                      "kernel": "fused train step: k_ad_fwd, k_ad_resid, k_ad_bwd, k_ad_wgrad, "
-                               "k_ad_wreduce, k_adam (linear jet maps on FP32 FMA)",
+                               "k_ad_wreduce, k_adam (linear jet maps on FP32 MFMA)",
                      "achieved": flops / (step_ms * 1e-3) / 1e12,
                      "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": flops / (step_ms * 1e-3) / FP32_MFMA_PEAK, "traffic": None,
