@@ -714,9 +714,19 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_wreduce(const T *part, int n_
   const int e = blockIdx.x * 64 + lane;
   const int q = (n_chunks + 3) / 4;
   const int c0 = wave * q, c1 = min(n_chunks, c0 + q);
-  T s = T(0);
-  if (e < ne)
-    for (int c = c0; c < c1; ++c) s += part[(size_t)c * ne + e];
+  // eight independent sums (loads in flight), combined in a fixed order
+  T acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) acc[u] = T(0);
+  if (e < ne) {
+    int c = c0;
+    for (; c + 8 <= c1; c += 8) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += part[(size_t)(c + u) * ne + e];
+    }
+    for (; c < c1; ++c) acc[0] += part[(size_t)c * ne + e];
+  }
+  const T s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
   red[threadIdx.x] = s;
   __syncthreads();
   if (wave != 0 || e >= ne) return;
