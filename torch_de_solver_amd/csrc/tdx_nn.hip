@@ -1509,9 +1509,18 @@ __global__ void k_reduce(NNArgs<T> a, int mode, T *dparams, T *params, T *m,
   if (idx >= tot) return;
   const int inst = (int)(idx / a.P);
   const int p = (int)(idx - (int64_t)inst * a.P);
-  T g = T(0);
-  for (int k = a.inst_k2_ptr[inst]; k < a.inst_k2_ptr[inst + 1]; ++k)
-    g += a.partial[(int64_t)a.inst_k2_idx[k] * a.P + p];
+  // eight independent sums (loads in flight), combined in a fixed order
+  T acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) acc[u] = T(0);
+  int k = a.inst_k2_ptr[inst];
+  const int k1 = a.inst_k2_ptr[inst + 1];
+  for (; k + 8 <= k1; k += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += a.partial[(int64_t)a.inst_k2_idx[k + u] * a.P + p];
+  }
+  for (; k < k1; ++k) acc[0] += a.partial[(int64_t)a.inst_k2_idx[k] * a.P + p];
+  const T g = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
   if (mode == 0) {
     dparams[idx] = g;
   } else {
