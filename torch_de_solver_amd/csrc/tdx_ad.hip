@@ -34,7 +34,10 @@ namespace {
 
 constexpr int AD_THREADS = 256;
 constexpr int AD_MAXF = 4;        // factors per term
-constexpr int AD_WG_CHUNK = 512;  // rows (point, jet) per reduction chunk
+#ifndef TDX_AD_WG_CHUNK
+#define TDX_AD_WG_CHUNK 256
+#endif
+constexpr int AD_WG_CHUNK = TDX_AD_WG_CHUNK;  // rows (point, jet) per reduction chunk
 
 template <typename T>
 struct TanhDerivs {
