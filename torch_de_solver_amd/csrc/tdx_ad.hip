@@ -1171,7 +1171,7 @@ int ad_forward(const tdx_ad_plan *p, const T *params, T *loss, T *resid, void *w
     // runs' sums from the residuals, each equation's seed multipliers, then
     // the residual program again with them (group 0 carries no other loss)
     const T *c = (const T *)p->data.ptr + p->w_d;
-    const T *x = c + p->w_N, *lam = x + p->w_G;
+    const T *x = c + p->w_N;
     const int32_t *gs = (const int32_t *)p->ints.ptr + p->w_i;
     T *R = wsp<T>(ws, p->off_wR), *M = wsp<T>(ws, p->off_wM);
     hipLaunchKernelGGL((k_ad_weak_runs<T>), dim3((p->w_G * p->w_neq + 255) / 256), dim3(256), 0,

@@ -461,7 +461,7 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
   red[256 + threadIdx.x] = lnsum;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) {
+    if ((int)threadIdx.x < st) {
       red[threadIdx.x] += red[threadIdx.x + st];
       red[256 + threadIdx.x] += red[256 + threadIdx.x + st];
     }
