@@ -1587,7 +1587,6 @@ __global__ void k_set_one(T *x) {
 // ===========================================================================
 using namespace tdx;
 
-static inline int wave_ls_host(int HP) { return 20 * HP / 16; }
 struct tdx_nn_plan;
 static size_t wave_private_lds(const tdx_nn_plan *p, const tdx_nn_desc *d);
 
