@@ -160,7 +160,7 @@ struct FwdArgs {
   Tables tb;
 };
 
-template <typename T, int KM>
+template <typename T, int KM, bool SPLIT>
 __global__ __launch_bounds__(AD_THREADS) void k_ad_fwd(FwdArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int J = a.J, din = a.din, dout = a.dout;
@@ -192,9 +192,13 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_fwd(FwdArgs<T> a) {
   __syncthreads();
   // GEMM: each wavefront takes column tiles of 16 outputs, holds their B
   // fragments (W^T, all k) in registers and sweeps the row tiles
+  // SPLIT (fewer column tiles than wavefronts): (column tile, row-tile
+  // group) pairs over the wavefronts, so every wavefront has products
   const int ntc = (dout + 15) >> 4;
+  const int nrg = SPLIT ? max(1, min(a.rpad >> 4, (AD_THREADS / 64) / ntc)) : 1;
   const int lr = lane & 15, lk = lane >> 4;
-  for (int c = wave; c < ntc; c += AD_THREADS / 64) {
+  for (int pc = wave; pc < ntc * nrg; pc += AD_THREADS / 64) {
+    const int c = SPLIT ? pc / nrg : pc, rg = SPLIT ? pc - (pc / nrg) * nrg : 0;
     const int n = c * 16 + lr;
     T bf[KM];
 #pragma unroll
@@ -202,7 +206,7 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_fwd(FwdArgs<T> a) {
       const int k = 4 * u + lk;
       bf[u] = (4 * u < din && k < din && n < dout) ? a.WT[(size_t)k * dout + n] : T(0);
     }
-    for (int r0 = 0; r0 < a.rpad; r0 += 16) {
+    for (int r0 = 16 * rg; r0 < a.rpad; r0 += 16 * nrg) {
       typename V4<T>::type acc = {T(0), T(0), T(0), T(0)};
       const T *row = IN + (r0 + lr) * ldi + lk;
 #pragma unroll
@@ -549,7 +553,7 @@ struct BwdArgs {
   Tables tb;
 };
 
-template <typename T, int KM>
+template <typename T, int KM, bool SPLIT>
 __global__ __launch_bounds__(AD_THREADS) void k_ad_bwd(BwdArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int J = a.J, H = a.H, dn = a.dn;
@@ -566,8 +570,10 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_bwd(BwdArgs<T> a) {
   stage_rows(ZS, ldh, a.Z + p0 * J * H, R, R, H);
   __syncthreads();
   const int ntc = (H + 15) >> 4;
+  const int nrg = SPLIT ? max(1, min(a.rpad >> 4, (AD_THREADS / 64) / ntc)) : 1;
   const int lr = lane & 15, lk = lane >> 4;
-  for (int c = wave; c < ntc; c += AD_THREADS / 64) {
+  for (int pc = wave; pc < ntc * nrg; pc += AD_THREADS / 64) {
+    const int c = SPLIT ? pc / nrg : pc, rg = SPLIT ? pc - (pc / nrg) * nrg : 0;
     const int n = c * 16 + lr;
     T bf[KM];
 #pragma unroll
@@ -575,7 +581,7 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_bwd(BwdArgs<T> a) {
       const int k = 4 * u + lk;
       bf[u] = (4 * u < dn && k < dn && n < H) ? a.Wn[(size_t)k * H + n] : T(0);
     }
-    for (int r0 = 0; r0 < a.rpad; r0 += 16) {
+    for (int r0 = 16 * rg; r0 < a.rpad; r0 += 16 * nrg) {
       typename V4<T>::type acc = {T(0), T(0), T(0), T(0)};
       const T *row = DN + (r0 + lr) * ldd + lk;
 #pragma unroll
@@ -1106,9 +1112,13 @@ template <typename T>
 int ad_forward(const tdx_ad_plan *p, const T *params, T *loss, T *resid, void *ws, hipStream_t s) {
   static bool attr = false;
   if (!attr) {
-    const void *fns[] = {(const void *)k_ad_fwd<T, 4>, (const void *)k_ad_fwd<T, 32>,
-                         (const void *)k_ad_fwd<T, 64>, (const void *)k_ad_bwd<T, 4>,
-                         (const void *)k_ad_bwd<T, 32>, (const void *)k_ad_bwd<T, 64>};
+    const void *fns[] = {
+        (const void *)k_ad_fwd<T, 4, false>,  (const void *)k_ad_fwd<T, 32, false>,
+        (const void *)k_ad_fwd<T, 64, false>, (const void *)k_ad_bwd<T, 4, false>,
+        (const void *)k_ad_bwd<T, 32, false>, (const void *)k_ad_bwd<T, 64, false>,
+        (const void *)k_ad_fwd<T, 4, true>,   (const void *)k_ad_fwd<T, 32, true>,
+        (const void *)k_ad_fwd<T, 64, true>,  (const void *)k_ad_bwd<T, 4, true>,
+        (const void *)k_ad_bwd<T, 32, true>,  (const void *)k_ad_bwd<T, 64, true>};
     for (const void *f : fns)
       (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr = true;
@@ -1139,13 +1149,26 @@ int ad_forward(const tdx_ad_plan *p, const T *params, T *loss, T *resid, void *w
     a.out = a.act ? wsp<T>(ws, p->off_a[l]) : wsp<T>(ws, p->off_u);
     a.tb = tables_of<T>(p);
     const unsigned blocks = (unsigned)((p->P + a.pb - 1) / a.pb);
-    // B fragments held in registers: k-steps of 4 over the input width
-    if (a.din <= 16)
-      hipLaunchKernelGGL((k_ad_fwd<T, 4>), dim3(blocks), dim3(AD_THREADS), p->f_lds[l], s, a);
-    else if (a.din <= 128)
-      hipLaunchKernelGGL((k_ad_fwd<T, 32>), dim3(blocks), dim3(AD_THREADS), p->f_lds[l], s, a);
-    else
-      hipLaunchKernelGGL((k_ad_fwd<T, 64>), dim3(blocks), dim3(AD_THREADS), p->f_lds[l], s, a);
+    // B fragments held in registers: k-steps of 4 over the input width;
+    // layers of fewer than four column tiles share the row tiles out
+    const bool split = a.dout <= 48;
+    const size_t lds = p->f_lds[l];
+    if (a.din <= 16) {
+      if (split)
+        hipLaunchKernelGGL((k_ad_fwd<T, 4, true>), dim3(blocks), dim3(AD_THREADS), lds, s, a);
+      else
+        hipLaunchKernelGGL((k_ad_fwd<T, 4, false>), dim3(blocks), dim3(AD_THREADS), lds, s, a);
+    } else if (a.din <= 128) {
+      if (split)
+        hipLaunchKernelGGL((k_ad_fwd<T, 32, true>), dim3(blocks), dim3(AD_THREADS), lds, s, a);
+      else
+        hipLaunchKernelGGL((k_ad_fwd<T, 32, false>), dim3(blocks), dim3(AD_THREADS), lds, s, a);
+    } else {
+      if (split)
+        hipLaunchKernelGGL((k_ad_fwd<T, 64, true>), dim3(blocks), dim3(AD_THREADS), lds, s, a);
+      else
+        hipLaunchKernelGGL((k_ad_fwd<T, 64, false>), dim3(blocks), dim3(AD_THREADS), lds, s, a);
+    }
     TDX_TRY_HIP(hipGetLastError());
   }
   ResArgs<T> r;
@@ -1235,12 +1258,24 @@ int ad_backward(const tdx_ad_plan *p, const T *params, const T *dloss, T *grad, 
     b.DZ = wsp<T>(ws, p->off_dz[l]);
     b.tb = tables_of<T>(p);
     const unsigned blocks = (unsigned)((p->P + b.pb - 1) / b.pb);
-    if (b.dn <= 16)
-      hipLaunchKernelGGL((k_ad_bwd<T, 4>), dim3(blocks), dim3(AD_THREADS), p->b_lds[l], s, b);
-    else if (b.dn <= 128)
-      hipLaunchKernelGGL((k_ad_bwd<T, 32>), dim3(blocks), dim3(AD_THREADS), p->b_lds[l], s, b);
-    else
-      hipLaunchKernelGGL((k_ad_bwd<T, 64>), dim3(blocks), dim3(AD_THREADS), p->b_lds[l], s, b);
+    const bool split = b.H <= 48;
+    const size_t lds = p->b_lds[l];
+    if (b.dn <= 16) {
+      if (split)
+        hipLaunchKernelGGL((k_ad_bwd<T, 4, true>), dim3(blocks), dim3(AD_THREADS), lds, s, b);
+      else
+        hipLaunchKernelGGL((k_ad_bwd<T, 4, false>), dim3(blocks), dim3(AD_THREADS), lds, s, b);
+    } else if (b.dn <= 128) {
+      if (split)
+        hipLaunchKernelGGL((k_ad_bwd<T, 32, true>), dim3(blocks), dim3(AD_THREADS), lds, s, b);
+      else
+        hipLaunchKernelGGL((k_ad_bwd<T, 32, false>), dim3(blocks), dim3(AD_THREADS), lds, s, b);
+    } else {
+      if (split)
+        hipLaunchKernelGGL((k_ad_bwd<T, 64, true>), dim3(blocks), dim3(AD_THREADS), lds, s, b);
+      else
+        hipLaunchKernelGGL((k_ad_bwd<T, 64, false>), dim3(blocks), dim3(AD_THREADS), lds, s, b);
+    }
     TDX_TRY_HIP(hipGetLastError());
   }
   T *part = wsp<T>(ws, p->off_part);
