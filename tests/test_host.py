@@ -358,3 +358,25 @@ def test_weak_tables_reproduce_oracle_integration(name):
     got = float(runs[0]) if lv == 1 else float(np.sum(cx * runs ** 2))
     np.testing.assert_allclose(got, want, rtol=1e-12)
     assert lv == (1 if central.shape[1] == 1 else 2)
+
+
+def test_poisson_autograd_flop_model():
+    """The algorithmic FLOPs behind the 'autograd' roofline: per (point, jet)
+    row, forward 2 din dout per layer, the transposed maps above the first
+    layer, and 2 (din + 1) dout for the weight gradients."""
+    import types
+    import bench
+    from torch_de_solver_amd import workloads as W
+    from torch_de_solver_amd.ad_engine import ADTables
+    from torch_de_solver_amd.compiler import compile_ad
+    from torch_de_solver_amd.nn_engine import mlp_arch
+    dom, eq, conds, net, kw = W.poisson_ad(200)
+    prob = compile_ad(dom.build('autograd'), eq.equation_lst, conds.build(dom.variable_dict), 1)
+    eng = types.SimpleNamespace(arch=mlp_arch(net), tables=ADTables(prob, np.float32))
+    rows = (201 * 201 + 4 * 201) * 5            # grid + boundary points, jets (), x, y, xx, yy
+    dims = [(2, 100)] + [(100, 100)] * 4 + [(100, 1)]
+    fwd = sum(2 * a * b for a, b in dims)
+    bwd = sum(2 * a * b for a, b in dims[1:])
+    wg = sum(2 * (a + 1) * b for a, b in dims)
+    assert eng.tables.n_points * eng.tables.n_jets == rows
+    assert bench.ad_flops_per_step(eng) == rows * (fwd + bwd + wg) == 49940872050
