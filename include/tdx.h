@@ -152,6 +152,34 @@ int tdx_nn_forward(const tdx_nn_plan *plan, const void *params, void *loss,
 int tdx_nn_backward(const tdx_nn_plan *plan, const void *params,
                     const void *dloss, void *dparams, void *workspace,
                     tdx_stream_t stream);
+/* Seed rows of the plan: one per (group, slot, entry), all instances. */
+int64_t tdx_nn_row_count(const tdx_nn_plan *plan);
+/* tdx_nn_forward that also writes the network output of every seed row to
+ * values[(row_base + slot*n_entries + entry)*n_out + o] (the value the
+ * residual program reads: the reference's net(x + shift*h)[:, o] of
+ * tedeous/derivative.py:48-52 and the boundary values of tedeous/eval.py:
+ * 283-345).  loss may be NULL.  Splits the parity of the 'NN' path at the
+ * network boundary: the network alone is compared through these values. */
+int tdx_nn_network_values(const tdx_nn_plan *plan, const void *params, void *values,
+                          void *loss, void *workspace, tdx_stream_t stream);
+/* The residual program, boundary values, losses and seeds on given network
+ * values ([n_rows][n_out], as written by tdx_nn_network_values) instead of
+ * the network: the reference's Operator/Bounds/Losses (tedeous/eval.py:
+ * 143-193, 283-461, tedeous/losses.py:84-135) applied to fixed outputs.
+ * Writes loss (may be NULL), resid (may be NULL) and the seeds dL/du of
+ * every row to `seeds` ([n_rows][n_out]) or, when NULL, to the workspace
+ * for a following tdx_nn_backward.  Needs no parameters. */
+int tdx_nn_forward_values(const tdx_nn_plan *plan, const void *values, void *loss,
+                          void *resid, void *seeds, void *workspace, tdx_stream_t stream);
+/* Jacobian of the residuals with respect to the parameters, for the natural
+ * gradient's Gram matrix: jt[p * n_resid + r] = d resid[r] / d params[p]
+ * (P x n_resid, the transpose of the reference's J; r indexes the residual
+ * output of tdx_nn_forward, operator and boundary entries alike).  Replaces
+ * the per-residual torch.autograd.grad loop of NGD.gram_factory
+ * (tedeous/optimizers/ngd.py:57-78).  One instance per plan; needs no
+ * workspace. */
+int tdx_nn_jacobian(const tdx_nn_plan *plan, const void *params, void *jt,
+                    tdx_stream_t stream);
 /* One fused training step: forward, adjoint and torch.optim.Adam update
  * (no weight decay, amsgrad off) of all instances.  step is a device int64
  * holding the number of completed steps; it is incremented by the call.

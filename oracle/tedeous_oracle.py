@@ -769,3 +769,82 @@ def train_adam(params: List[torch.Tensor], loss_fn: Callable[[], torch.Tensor], 
         opt.zero_grad()
         opt.step(closure)
     return history
+
+
+# ----------------------------------------------------------------------------
+# natural gradient (NGD)
+# ----------------------------------------------------------------------------
+
+def ngd_gram(residuals: torch.Tensor, params: List[torch.Tensor]) -> torch.Tensor:
+    """``NGD.gram_factory`` (tedeous/optimizers/ngd.py:57-78): one
+    ``torch.autograd.grad`` per residual entry gives a row of J; the Gram
+    matrix is ``1/len(residuals) * J.T @ J``."""
+    rows = []
+    for r in residuals:
+        g = torch.autograd.grad(r, params, retain_graph=True, allow_unused=True)
+        g = [torch.zeros_like(p) if x is None else x for p, x in zip(params, g)]
+        rows.append(torch.cat([x.reshape(-1) for x in g]).reshape(1, -1))
+    J = torch.cat(rows)
+    return 1.0 / len(residuals) * J.T @ J
+
+
+def ngd_lstsq(A: torch.Tensor, B: torch.Tensor, tol: Optional[float] = None) -> torch.Tensor:
+    """``NGD.torch_cuda_lstsq`` (tedeous/optimizers/ngd.py:104-126): the
+    pseudo-inverse solution by SVD, singular values above ``tol`` (machine
+    epsilon of the dtype, absolute) inverted."""
+    tol = torch.finfo(A.dtype).eps if tol is None else tol
+    U, S, Vh = torch.linalg.svd(A, full_matrices=False)
+    Spinv = torch.zeros_like(S)
+    Spinv[S > tol] = 1 / S[S > tol]
+    UhB = U.adjoint() @ B
+    if Spinv.ndim != UhB.ndim:
+        Spinv = Spinv.unsqueeze(-1)
+    return Vh.adjoint() @ (Spinv * UhB)
+
+
+def ngd_steps(grid_steps_number: int) -> torch.Tensor:
+    """Candidate step sizes ``0.5 ** linspace(0, n, n + 1)`` (float32,
+    tedeous/optimizers/ngd.py:24-25)."""
+    return 0.5 ** torch.linspace(0, grid_steps_number, grid_steps_number + 1)
+
+
+def ngd_line_search(params: List[torch.Tensor], f_nat_grad: torch.Tensor,
+                    loss_fn: Callable[[], torch.Tensor], steps: torch.Tensor):
+    """``NGD.grid_line_search_update`` (tedeous/optimizers/ngd.py:29-55): the
+    loss at ``params - step * f`` for every candidate step, the first
+    minimum taken; returns (losses, chosen index)."""
+    from torch.nn.utils import parameters_to_vector, vector_to_parameters
+    base = parameters_to_vector(params).detach().clone()
+    losses = []
+    with torch.no_grad():
+        for st in steps:
+            vector_to_parameters(base - st * f_nat_grad, params)
+            losses.append(loss_fn().reshape(1))
+        losses = torch.cat(losses)
+        k = int(torch.argmin(losses))
+        vector_to_parameters(base - steps[k] * f_nat_grad, params)
+    return losses, k
+
+
+def ngd_step(net, orc, lam_op, lam_bnd, grid_steps_number: int = 30):
+    """One ``NGD.step`` with the closure of tedeous/optimizers/closure.py:
+    98-117 on an oracle problem: loss and gradient, the Gram matrices of the
+    operator residuals (flattened [N, n_eq]) and of the padded boundary
+    residuals ``bval - true_bval`` (flattened [max_len, n_types]), the
+    Marquardt term ``min(loss, 0) * I`` (zero for a non-negative loss), the
+    SVD least-squares natural gradient and the grid line search.  Returns a
+    dict of the intermediate values."""
+    params = list(net.parameters())
+    loss, _, op, b, t, _, _ = orc.evaluate(net, lam_op, lam_bnd)
+    grads = torch.autograd.grad(loss, params, retain_graph=True, allow_unused=True)
+    f_grads = torch.cat([g.reshape(-1) for g in grads])
+    G_int = ngd_gram(op.reshape(-1), params)
+    G_b = ngd_gram((b - t).reshape(-1), params)
+    G = G_int + G_b
+    damp = torch.minimum(loss.detach().reshape(()), torch.zeros((), dtype=loss.dtype))
+    G = damp * torch.eye(len(G), dtype=G.dtype) + G
+    f = ngd_lstsq(G.detach(), f_grads.detach())
+    steps = ngd_steps(grid_steps_number)
+    losses, k = ngd_line_search(params, f, lambda: orc.evaluate(net, lam_op, lam_bnd)[0], steps)
+    return dict(loss=float(loss), grad=f_grads.detach(), G_int=G_int.detach(), G_b=G_b.detach(),
+                G=G.detach(), f=f, losses=losses, k=k, step=float(steps[k]))

@@ -102,6 +102,26 @@ def test_ad_float32_against_float64_oracle(name):
     assert gn < 1e-3, gn
 
 
+@pytest.mark.parametrize('name', ['burgers_ad', 'kdv_ad', 'mixed_ad', 'beam_ad'])
+def test_ad_float32_against_float32_oracle(name):
+    """float32 engine against the float32 oracle (nested torch autograd on
+    the CPU, the reference's algorithm at the reference's precision): loss
+    to 1e-5 relative and gradient to 1e-5 in norm."""
+    _print(f"[ad f32 vs f32] {name}")
+    _, layers = problems.AD_CASES[name]
+    ref = _oracle(name, _net(layers, torch.float64).to(torch.float32), dtype='float32')
+    net32 = _net(layers, torch.float64).to(torch.float32).to(DEV)
+    sol, _ = _engine(name, net32, 'float32')
+    loss, _ = sol.evaluate()
+    loss.backward()
+    grad = torch.cat([p.grad.reshape(-1) for p in net32.parameters()]).cpu().numpy()
+    rel = abs(float(loss) - ref[0]) / ref[0]
+    gn = np.linalg.norm(grad - ref[7]) / np.linalg.norm(ref[7])
+    _print(f"  loss rel {rel:.3e}, gradient rel {gn:.3e}")
+    assert rel < 1e-5, (float(loss), ref[0])
+    assert gn < 1e-5, gn
+
+
 @pytest.mark.parametrize('width', [100, 160])
 def test_ad_wide_network_against_oracle(width):
     """Width 100 (the reference examples' width; its 16-wide MFMA tiles are

@@ -380,3 +380,26 @@ def test_poisson_autograd_flop_model():
     wg = sum(2 * (a + 1) * b for a, b in dims)
     assert eng.tables.n_points * eng.tables.n_jets == rows
     assert bench.ad_flops_per_step(eng) == rows * (fwd + bwd + wg) == 49940872050
+
+
+def test_lambda_values_compare_by_value():
+    """A lambda given as a number, a list or the reference's [1, n] tensor is
+    the same weight; a changed value is a change (the plan is rebuilt)."""
+    from torch_de_solver_amd.solution import _lambda_key
+    assert _lambda_key(10) == _lambda_key(10.0) == _lambda_key([10]) == \
+        _lambda_key(torch.tensor([[10.0]], dtype=torch.float64))
+    assert _lambda_key([1, 2]) == _lambda_key(torch.tensor([[1.0, 2.0]]))
+    assert _lambda_key([1, 2]) != _lambda_key([1, 3])
+    assert _lambda_key(100) != _lambda_key(10)
+
+
+def test_no_conditions_raise_like_the_reference():
+    """Without boundary conditions the reference fails building its boundary
+    matrix (max() of an empty sequence, tedeous/eval.py:75) in every mode."""
+    from torch_de_solver_amd.solution import Operator_bcond_preproc, Solution
+    d, eq, b, kw = problems.burgers_nn(api)
+    grid = d.build('NN')
+    for mode in ('NN', 'mat', 'autograd'):
+        pre = Operator_bcond_preproc(grid, eq.equation_lst, []).set_strategy(mode)
+        with pytest.raises(ValueError):
+            Solution(grid, pre, problems.make_mlp([2, 8, 1]), mode, None, 1, 10)

@@ -425,3 +425,93 @@ def test_weak_integration_matches_numpy_trapezoid():
     sb = s * (1 + grid[:, 0] * grid[:, 1]).numpy()
     R2 = np.array([np.trapezoid(sb.reshape(4, 5)[k] ** 2, ts) for k in range(4)])
     np.testing.assert_allclose(float(w2), np.trapezoid(R2 ** 2, xs), rtol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# natural gradient: the restated NGD pieces pinned by closed forms
+# ---------------------------------------------------------------------------
+
+def test_ngd_gram_closed_forms():
+    """Per-residual autograd rows: a linear model r = A th - b has J = A, so
+    G = A^T A / N; r = sin(A th) has J = diag(cos(A th)) A."""
+    rng = np.random.default_rng(11)
+    A = torch.from_numpy(rng.standard_normal((13, 5)))
+    b = torch.from_numpy(rng.standard_normal(13))
+    th = torch.from_numpy(rng.standard_normal(5)).requires_grad_()
+    G = O.ngd_gram(A @ th - b, [th])
+    np.testing.assert_allclose(G.detach().numpy(), (A.T @ A).numpy() / 13, rtol=1e-14, atol=1e-15)
+    G2 = O.ngd_gram(torch.sin(A @ th), [th])
+    J = torch.cos(A @ th).detach().reshape(-1, 1) * A
+    np.testing.assert_allclose(G2.detach().numpy(), (J.T @ J).numpy() / 13, rtol=1e-13,
+                               atol=1e-15)
+    # a parameter that does not enter gives zero rows and columns
+    z = torch.zeros(2, dtype=torch.float64, requires_grad=True)
+    G3 = O.ngd_gram(A @ th, [th, z])
+    assert G3.shape == (7, 7) and float(G3[5:].abs().max()) == 0.0
+
+
+def test_ngd_lstsq_is_the_pseudo_inverse():
+    """SVD solution against numpy's pseudo-inverse: full rank (the solve),
+    and exactly rank-deficient (singular values below the absolute machine
+    epsilon dropped)."""
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((6, 6)) + 6 * np.eye(6)
+    B = rng.standard_normal(6)
+    x = O.ngd_lstsq(torch.from_numpy(A), torch.from_numpy(B)).numpy()
+    np.testing.assert_allclose(x, np.linalg.solve(A, B), rtol=1e-12)
+    # exactly rank 4 (a permuted diagonal: its zero singular values are
+    # exact, so none is inverted)
+    Pm = np.eye(6)[[2, 0, 5, 1, 4, 3]]
+    S = np.array([3.0, 2.0, 1.0, 0.5, 0.0, 0.0])
+    Ar = Pm @ np.diag(S) @ Pm.T
+    xr = O.ngd_lstsq(torch.from_numpy(Ar), torch.from_numpy(B)).numpy()
+    np.testing.assert_allclose(xr, np.linalg.pinv(Ar, rcond=1e-10) @ B, rtol=1e-10, atol=1e-12)
+    # a right-hand side with several columns
+    Bm = rng.standard_normal((6, 3))
+    xm = O.ngd_lstsq(torch.from_numpy(A), torch.from_numpy(Bm)).numpy()
+    np.testing.assert_allclose(xm, np.linalg.solve(A, Bm), rtol=1e-12)
+
+
+def test_ngd_line_search_takes_the_first_minimum():
+    steps = O.ngd_steps(20)
+    assert steps.dtype == torch.float32 and len(steps) == 21
+    np.testing.assert_array_equal(steps.numpy(), 0.5 ** np.arange(21, dtype=np.float32))
+    th = torch.tensor([1.0, -2.0], dtype=torch.float64, requires_grad=True)
+    f = torch.tensor([0.75, -1.5], dtype=torch.float64)
+    target = torch.tensor([0.625, -1.25], dtype=torch.float64)  # th - 0.5 f
+
+    def loss():
+        return ((th - target) ** 2).sum()
+
+    losses, k = O.ngd_line_search([th], f, loss, steps)
+    assert k == 1 and float(losses[1]) == 0.0
+    np.testing.assert_array_equal(th.detach().numpy(), target.numpy())
+    # ties: the first candidate
+    th2 = torch.tensor([3.0], dtype=torch.float64, requires_grad=True)
+    losses, k = O.ngd_line_search([th2], torch.tensor([1.0], dtype=torch.float64),
+                                  lambda: torch.zeros((), dtype=torch.float64), steps)
+    assert k == 0
+
+
+def test_ngd_step_closed_form_on_a_linear_model():
+    """Legendre's operator on the linear network u = w x + b: every central
+    difference of a linear function is exact, so the residual is
+    r = 10 x w + 12 b (J row [10 x, 12]) and the Dirichlet values are
+    u(-1) = b - w, u(1) = w + b (J rows [-1, 1], [1, 1], one 'dirichlet'
+    column of length 2): G_int = sum [10x, 12]^T [10x, 12] / N and
+    G_b = I."""
+    d, eq, b, kw = problems.legendre_nn(api, n=20, dtype='float64')
+    orc = O.NNOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst])
+    net = torch.nn.Linear(1, 1).double()
+    with torch.no_grad():
+        net.weight.fill_(0.3)
+        net.bias.fill_(-0.2)
+    out = O.ngd_step(net, orc, 1, 10, grid_steps_number=20)
+    x = d.variable_dict['x'].double().numpy()
+    rows = np.stack([10 * x, 12 * np.ones_like(x)], 1)
+    np.testing.assert_allclose(out['G_int'].numpy(), rows.T @ rows / len(x), rtol=1e-8,
+                               atol=1e-12)
+    np.testing.assert_allclose(out['G_b'].numpy(), np.eye(2), rtol=1e-12, atol=1e-12)
+    # the natural gradient solves G f = grad
+    np.testing.assert_allclose(out['G'].numpy() @ out['f'].numpy(), out['grad'].numpy(),
+                               rtol=1e-8, atol=1e-10)

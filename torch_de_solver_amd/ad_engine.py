@@ -188,6 +188,9 @@ class ADEngine:
         self.n_params = int(lib.tdx_ad_param_count(handle))
         nbytes = int(lib.tdx_ad_workspace_bytes(handle))
         self.workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+        # forward passes so far: the adjoint reads the jets and seeds of the
+        # latest one, so a backward checks that it belongs to its forward
+        self.fwd_gen = 0
 
     def __del__(self):
         h = getattr(self, 'handle', None)
@@ -210,6 +213,7 @@ class ADEngine:
         native.check(native.lib().tdx_ad_forward(
             self.handle, native.ptr(params), native.ptr(loss), native.ptr(resid),
             native.ptr(self.workspace), native.stream_ptr()))
+        self.fwd_gen += 1
 
     def backward(self, params: torch.Tensor, dloss: torch.Tensor, dparams: torch.Tensor) -> None:
         self._check(params)
@@ -223,6 +227,7 @@ class ADEngine:
             self.handle, native.ptr(params), native.ptr(exp_avg), native.ptr(exp_avg_sq),
             native.ptr(step), float(lr), float(beta1), float(beta2), float(eps),
             native.ptr(loss), native.ptr(self.workspace), native.stream_ptr()))
+        self.fwd_gen += 1
 
     def loss(self, params: torch.Tensor) -> torch.Tensor:
         """``[1, 2]``: (lambda-weighted loss, normalized loss), differentiable
@@ -248,6 +253,7 @@ class ADResidualLoss(torch.autograd.Function):
         loss = torch.empty(2, dtype=engine.dtype, device=engine.device)
         engine.forward(p, loss)
         ctx.engine = engine
+        ctx.gen = engine.fwd_gen
         ctx.save_for_backward(p)
         return loss.view(1, 2)
 
@@ -255,6 +261,10 @@ class ADResidualLoss(torch.autograd.Function):
     def backward(ctx, grad_out: torch.Tensor):
         (p,) = ctx.saved_tensors
         eng: ADEngine = ctx.engine
+        if eng.fwd_gen != ctx.gen:
+            # another forward pass ran since this node's: restore its jets
+            # and seeds before the adjoint
+            eng.forward(p, torch.empty(2, dtype=eng.dtype, device=eng.device))
         # the normalized loss carries no gradient (tedeous/losses.py:124-126)
         dloss = grad_out.reshape(2)[:1].contiguous().to(eng.dtype)
         dparams = torch.empty_like(p)

@@ -1110,19 +1110,12 @@ Tables tables_of(const tdx_ad_plan *p) {
 
 template <typename T>
 int ad_forward(const tdx_ad_plan *p, const T *params, T *loss, T *resid, void *ws, hipStream_t s) {
-  static bool attr = false;
-  if (!attr) {
-    const void *fns[] = {
-        (const void *)k_ad_fwd<T, 4, false>,  (const void *)k_ad_fwd<T, 32, false>,
-        (const void *)k_ad_fwd<T, 64, false>, (const void *)k_ad_bwd<T, 4, false>,
-        (const void *)k_ad_bwd<T, 32, false>, (const void *)k_ad_bwd<T, 64, false>,
-        (const void *)k_ad_fwd<T, 4, true>,   (const void *)k_ad_fwd<T, 32, true>,
-        (const void *)k_ad_fwd<T, 64, true>,  (const void *)k_ad_bwd<T, 4, true>,
-        (const void *)k_ad_bwd<T, 32, true>,  (const void *)k_ad_bwd<T, 64, true>};
-    for (const void *f : fns)
-      (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr = true;
-  }
+  raise_lds_limit({(const void *)k_ad_fwd<T, 4, false>,  (const void *)k_ad_fwd<T, 32, false>,
+                   (const void *)k_ad_fwd<T, 64, false>, (const void *)k_ad_bwd<T, 4, false>,
+                   (const void *)k_ad_bwd<T, 32, false>, (const void *)k_ad_bwd<T, 64, false>,
+                   (const void *)k_ad_fwd<T, 4, true>,   (const void *)k_ad_fwd<T, 32, true>,
+                   (const void *)k_ad_fwd<T, 64, true>,  (const void *)k_ad_bwd<T, 4, true>,
+                   (const void *)k_ad_bwd<T, 32, true>,  (const void *)k_ad_bwd<T, 64, true>});
   const int J = p->J;
   for (int l = 0; l < p->L; ++l) {
     const int ne = p->din[l] * p->dout[l];

@@ -5,7 +5,10 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
+#include <set>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "tdx.h"
@@ -13,6 +16,7 @@
 namespace tdx {
 
 void set_error(const char *fmt, ...);
+void clear_error();
 
 #define TDX_TRY_HIP(expr)                                                    \
   do {                                                                       \
@@ -55,6 +59,19 @@ struct DevTable {
 };
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Raises the dynamic LDS limit of each given kernel to 160 KB, once per
+// (kernel, device) and process; safe to call from several threads.
+inline void raise_lds_limit(std::initializer_list<const void *> kernels) {
+  static std::mutex mu;
+  static std::set<std::pair<const void *, int>> done;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  for (const void *k : kernels)
+    if (done.insert({k, dev}).second)
+      (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
 
 // ---- device helpers -------------------------------------------------------
 

@@ -14,6 +14,8 @@ void set_error(const char *fmt, ...) {
   g_last_error = buf;
 }
 
+void clear_error() { g_last_error.clear(); }
+
 }  // namespace tdx
 
 extern "C" int tdx_version(void) { return 1; }

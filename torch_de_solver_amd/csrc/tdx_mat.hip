@@ -967,14 +967,8 @@ MatArgs<T> mat_args(const tdx_mat_plan *p, const void *u, void *ws) {
 
 template <typename T, int DP, int DIV>
 int mat_tile_launch(const tdx_mat_plan *p, const MatArgs<T> &a, bool adj, hipStream_t s) {
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void *)k_mat_tile<T, DP, DIV, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void *)k_mat_tile<T, DP, DIV, false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr = true;
-  }
+  raise_lds_limit({(const void *)k_mat_tile<T, DP, DIV, true>,
+                   (const void *)k_mat_tile<T, DP, DIV, false>});
   if (p->n_gen_wg == 0) return TDX_OK;
   if (adj)
     hipLaunchKernelGGL((k_mat_tile<T, DP, DIV, true>), dim3(p->n_gen_wg), dim3(MAT_THREADS),
@@ -1218,7 +1212,11 @@ int mat_specialize(tdx_mat_plan *p, const tdx_mat_desc *d) {
     return TDX_EINVAL;
   }
   const int rc = mat_jit_compile(src, &p->jit);
-  if (rc) return required ? rc : TDX_OK;
+  if (rc) {
+    if (required) return rc;
+    clear_error();  // the general kernel serves the plan
+    return TDX_OK;
+  }
   // a generated kernel that needs scratch (register spills) is not taken
   // unless asked for: the general kernel serves such programs better
   auto spills = [](hipFunction_t f) {

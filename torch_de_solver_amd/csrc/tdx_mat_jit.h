@@ -76,9 +76,10 @@ struct JitKernel {
 // what the generator serves; reason in *why).
 std::string mat_jit_source(const JitSpec &spec, std::string *why);
 
-// Compiles `src` to a gfx950 code object (hipRTC; needs no device);
+// Compiles `src` to a code object for `arch` (hipRTC; needs no device);
 // returns 0, or a negative code with the compiler log in the error message.
-int mat_jit_build(const std::string &src, std::vector<char> *code);
+int mat_jit_build(const std::string &src, std::vector<char> *code,
+                  const std::string &arch = "gfx950");
 
 // Compiles and loads `src` on the current device once per process (cached by
 // device and source); returns 0 and fills *k, or a negative code with the

@@ -497,14 +497,15 @@ std::string mat_jit_source(const JitSpec &s, std::string *why) {
   return o.str();
 }
 
-int mat_jit_build(const std::string &src, std::vector<char> *code) {
+int mat_jit_build(const std::string &src, std::vector<char> *code, const std::string &arch) {
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "tdx_mat_jit.hip", 0, nullptr, nullptr) !=
       HIPRTC_SUCCESS) {
     set_error("hiprtcCreateProgram failed");
     return TDX_ELAUNCH;
   }
-  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-Xclang",
+  const std::string arch_opt = "--offload-arch=" + arch;
+  const char *opts[] = {arch_opt.c_str(), "-O3", "-ffp-contract=off", "-Xclang",
                         "-target-feature", "-Xclang", "-packed-fp32-ops"};
   const hiprtcResult rc = hiprtcCompileProgram(prog, 7, opts);
   if (rc != HIPRTC_SUCCESS) {
@@ -538,8 +539,17 @@ int mat_jit_compile(const std::string &src, JitKernel *k) {
     *k = it->second;
     return TDX_OK;
   }
+  // code for the current device's architecture (the name before any
+  // feature suffix, e.g. "gfx950" of "gfx950:sramecc+:xnack-")
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    set_error("hipGetDeviceProperties failed");
+    return TDX_ELAUNCH;
+  }
+  std::string arch(prop.gcnArchName);
+  arch = arch.substr(0, arch.find(':'));
   std::vector<char> code;
-  const int rc = mat_jit_build(src, &code);
+  const int rc = mat_jit_build(src, &code, arch);
   if (rc) return rc;
   JitKernel out;
   if (hipModuleLoadData(&out.module, code.data()) != hipSuccess) {
@@ -549,6 +559,7 @@ int mat_jit_compile(const std::string &src, JitKernel *k) {
   if (hipModuleGetFunction(&out.fn_adj, out.module, "tdx_mat_jit_adj") != hipSuccess ||
       hipModuleGetFunction(&out.fn_fwd, out.module, "tdx_mat_jit_fwd") != hipSuccess ||
       hipModuleGetFunction(&out.fn_step, out.module, "tdx_mat_jit_step") != hipSuccess) {
+    (void)hipModuleUnload(out.module);
     set_error("specialized 'mat' kernel: entry point missing");
     return TDX_ELAUNCH;
   }

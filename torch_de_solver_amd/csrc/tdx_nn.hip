@@ -72,6 +72,11 @@ struct NNArgs {
   // seed multipliers of group 0's equations, [n_eq][N] (null: none)
   const T *seed_mult;
   int seed_mult_n;
+  // network values per seed row [n_rows][n_out]: u_out receives them
+  // (null: not kept); with u_in the forward kernel reads them instead of
+  // evaluating the network (null: evaluate)
+  T *u_out;
+  const T *u_in;
 };
 
 template <typename T>
@@ -344,6 +349,13 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
   T *sd_lds = smem + K1_WAVES * 16 * SO + (w * 16 + row) * SO;
   T *red = smem + 2 * K1_WAVES * 16 * SO;
 
+  if (a.u_in) {
+    // network values given: the residual program runs on them
+    if (q == 0)
+      for (int s = 0; s < n_slots; ++s)
+        for (int o = 0; o < a.n_out; ++o)
+          u_lds[s * a.n_out + o] = a.u_in[(row_base + (int64_t)s * n_entries + ec) * a.n_out + o];
+  } else {
   Net<T, NB> net;
   if constexpr (STAGE)
     net = stage_net<T, NB>(a, inst, red + 512, false);
@@ -363,18 +375,17 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
     for (int o = 0; o < a.n_out; ++o) {
       const T u = layer_out<T, NB>(net, o, q, act);
       if (q == 0) u_lds[s * a.n_out + o] = u;
+      if (a.u_out && q == 0 && valid)
+        a.u_out[(row_base + (int64_t)s * n_entries + e) * a.n_out + o] = u;
     }
+  }
   }
   __syncthreads();
 
-  // residual, loss and seeds of the workgroup's 64 entries: wavefront 0,
-  // one entry per lane (the other wavefronts' SIMDs are free meanwhile)
+  // residual, loss and seeds: every wavefront its own 16 entries, one per
+  // lane of quarter 0, so the four SIMDs share the interpretation
   T lsum = T(0), lnsum = T(0);
-  if (w == 0) {
-    const int e = item.y + lane;
-    const bool valid = e < n_entries;
-    T *u_lds = smem + lane * SO;
-    T *sd_lds = smem + K1_WAVES * 16 * SO + lane * SO;
+  if (q == 0) {
     for (int k = 0; k < SO; ++k) sd_lds[k] = T(0);
     if (valid) {
       for (int eq = 0; eq < n_eq; ++eq) {
@@ -1580,6 +1591,176 @@ __global__ void k_set_one(T *x) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *x = T(1);
 }
 
+// ---------------------------------------------------------------------------
+// k_jac: per-residual Jacobian (natural-gradient Gram assembly)
+// ---------------------------------------------------------------------------
+// The reference forms J row by row with one torch.autograd.grad per residual
+// (tedeous/optimizers/ngd.py:57-78).  Here one thread owns one (group,
+// entry): it evaluates the network at every slot of its entry, forms, per
+// equation, the seeds dr/du(slot, var) of the residual program (the
+// k_forward seeds with dr = 1), and for every slot with a non-zero seed runs
+// the network forward again and back with those seeds, adding
+// seed * du/dparams into its own column r of jt[P][n_resid] (coalesced
+// across the threads of a workgroup; no two threads share a column).
+// Activations, deltas, values and seeds live in LDS, thread-fastest.
+template <typename T>
+__global__ void k_jac_zero(T *jt, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    jt[i] = T(0);
+}
+
+template <typename T>
+__global__ void k_jac(NNArgs<T> a, T *jt, int64_t n_resid) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  const int B = blockDim.x, t = threadIdx.x;
+  const int2 item = a.k1_items[blockIdx.x];
+  const int32_t *G = a.groups + item.x * TDX_GROUP_INTS;
+  const int n_entries = G[1], n_slots = G[2];
+  const int slot_base_i = G[3], slot_off_r = G[4], n_eq = G[5], eq_i = G[6];
+  const int H = a.H, NH = a.NH, d = a.d_in, no = a.n_out;
+  const int SO = n_slots * no;
+  const Layout<T> L{d, no, H, NH};
+  const T *W = a.params;
+  // per-thread LDS arrays, element k of thread t at [k * B + t]
+  T *act = smem;                         // (NH + 1) x H
+  T *dA = act + (int64_t)(NH + 1) * H * B;   // H
+  T *dB = dA + (int64_t)H * B;               // H
+  T *uv = dB + (int64_t)H * B;               // SO network values
+  T *sd = uv + (int64_t)SO * B;              // SO seeds
+#define ACT(l, i) act[((l) * H + (i)) * B + t]
+  for (int e = item.y + t; e < item.y + 16 * K1_WAVES && e < n_entries; e += B) {
+    // inputs and network values of every slot
+    auto input = [&](int s, T (&x)[MAXD]) {
+      const int64_t base = a.ints[slot_base_i + s];
+#pragma unroll
+      for (int k = 0; k < MAXD; ++k)
+        x[k] = k < d ? a.data[base + (int64_t)e * d + k] + a.reals[slot_off_r + s * d + k] : T(0);
+    };
+    auto forward = [&](const T (&x)[MAXD]) {
+      for (int i = 0; i < H; ++i) {
+        T z = W[L.ob1() + i];
+#pragma unroll
+        for (int k = 0; k < MAXD; ++k)
+          if (k < d) z = fma(W[L.oW1() + i * d + k], x[k], z);
+        ACT(0, i) = dtanh(z);
+      }
+      for (int j = 0; j < NH; ++j)
+        for (int i = 0; i < H; ++i) {
+          T z = W[L.obh(j) + i];
+          for (int k = 0; k < H; ++k) z = fma(W[L.oWh(j) + i * H + k], ACT(j, k), z);
+          ACT(j + 1, i) = dtanh(z);
+        }
+    };
+    for (int s = 0; s < n_slots; ++s) {
+      T x[MAXD];
+      input(s, x);
+      forward(x);
+      for (int o = 0; o < no; ++o) {
+        T u = W[L.obL() + o];
+        for (int k = 0; k < H; ++k) u = fma(W[L.oWL() + o * H + k], ACT(NH, k), u);
+        uv[(s * no + o) * B + t] = u;
+      }
+    }
+    for (int eq = 0; eq < n_eq; ++eq) {
+      const int32_t *E = a.ints + eq_i + eq * TDX_EQ_INTS;
+      const int64_t r = (int64_t)E[3] + e;
+      if (E[3] < 0) continue;
+      for (int k = 0; k < SO; ++k) sd[k * B + t] = T(0);
+      // dr/du(slot, var) through every factor of every term
+      for (int tt = 0; tt < E[0]; ++tt) {
+        const int32_t *Tm = a.ints + E[1] + tt * TDX_TERM_INTS;
+        const int nf = Tm[0];
+        T gsv[MAXF], pw[MAXF], pre[MAXF + 1], suf[MAXF + 1];
+#pragma unroll
+        for (int f = 0; f < MAXF; ++f) {
+          gsv[f] = T(0);
+          pw[f] = T(1);
+          if (f < nf) {
+            const int32_t *F = a.ints + Tm[1] + f * TDX_FACTOR_INTS;
+            T gs = T(0);
+            for (int k = 0; k < F[3]; ++k) {
+              const int32_t *S = a.ints + F[4] + k * TDX_STENCIL_INTS;
+              gs = gs + uv[(S[0] * no + F[0]) * B + t] * a.reals[S[1]];
+            }
+            gsv[f] = gs;
+            pw[f] = fpow(a, F, gs);
+          }
+        }
+        pre[0] = T(1);
+#pragma unroll
+        for (int f = 0; f < MAXF; ++f) pre[f + 1] = pre[f] * pw[f];
+        suf[MAXF] = T(1);
+#pragma unroll
+        for (int f = MAXF - 1; f >= 0; --f) suf[f] = suf[f + 1] * pw[f];
+        const T c = coeff_of(a, Tm, e);
+#pragma unroll
+        for (int f = 0; f < MAXF; ++f) {
+          if (f < nf) {
+            const int32_t *F = a.ints + Tm[1] + f * TDX_FACTOR_INTS;
+            const T g = c * dfpow(a, F, gsv[f]) * pre[f] * suf[f + 1];
+            for (int k = 0; k < F[3]; ++k) {
+              const int32_t *S = a.ints + F[4] + k * TDX_STENCIL_INTS;
+              sd[(S[0] * no + F[0]) * B + t] += g * a.reals[S[1]];
+            }
+          }
+        }
+      }
+      // seed * du/dparams of every slot with a non-zero seed
+      T *col = jt + r;
+      auto add = [&](int p, T v) { col[(int64_t)p * n_resid] += v; };
+      for (int s = 0; s < n_slots; ++s) {
+        bool any = false;
+        for (int o = 0; o < no; ++o) any |= sd[(s * no + o) * B + t] != T(0);
+        if (!any) continue;
+        T x[MAXD];
+        input(s, x);
+        forward(x);
+        // output layer
+        for (int o = 0; o < no; ++o) {
+          const T so = sd[(s * no + o) * B + t];
+          for (int k = 0; k < H; ++k) add(L.oWL() + o * H + k, so * ACT(NH, k));
+          add(L.obL() + o, so);
+        }
+        T *dc = dA, *dn = dB;
+        for (int k = 0; k < H; ++k) {
+          T g = T(0);
+          for (int o = 0; o < no; ++o) g = fma(W[L.oWL() + o * H + k], sd[(s * no + o) * B + t], g);
+          const T av = ACT(NH, k);
+          dc[k * B + t] = g * (T(1) - av * av);
+        }
+        // hidden layers, last to first: layer j maps ACT(j) to ACT(j + 1)
+        for (int j = NH - 1; j >= 0; --j) {
+          for (int i = 0; i < H; ++i) {
+            const T di = dc[i * B + t];
+            for (int k = 0; k < H; ++k) add(L.oWh(j) + i * H + k, di * ACT(j, k));
+            add(L.obh(j) + i, di);
+          }
+          for (int k = 0; k < H; ++k) {
+            T g = T(0);
+            for (int i = 0; i < H; ++i) g = fma(W[L.oWh(j) + i * H + k], dc[i * B + t], g);
+            const T av = ACT(j, k);
+            dn[k * B + t] = g * (T(1) - av * av);
+          }
+          T *tmp = dc;
+          dc = dn;
+          dn = tmp;
+        }
+        // first layer
+        for (int i = 0; i < H; ++i) {
+          const T di = dc[i * B + t];
+#pragma unroll
+          for (int k = 0; k < MAXD; ++k)
+            if (k < d) add(L.oW1() + i * d + k, di * x[k]);
+          add(L.ob1() + i, di);
+        }
+      }
+    }
+  }
+#undef ACT
+}
+
 }  // namespace tdx
 
 // ===========================================================================
@@ -1593,6 +1774,8 @@ static size_t wave_private_lds(const tdx_nn_plan *p, const tdx_nn_desc *d);
 struct tdx_nn_plan {
   int dtype, d_in, n_out, H, n_layers, NH, NB, HP, P, n_inst;
   int n_groups;
+  int max_slots;      // largest slot count of a group
+  int64_t n_resid;    // length of the residual output
   int64_t n_rows;
   int n_k1, n_k2;
   int k2_waves;
@@ -1675,16 +1858,18 @@ NNArgs<T> make_args(const tdx_nn_plan *p, const void *params, void *ws) {
   a.n_inst = p->n_inst;
   char *w = (char *)ws;
   a.params = (const T *)params;
-  a.packed = (T *)(w + p->off_packed);
-  a.seeds = (T *)(w + p->off_seeds);
-  a.lp = (T *)(w + p->off_lp);
-  a.partial = (T *)(w + p->off_partial);
+  a.packed = w ? (T *)(w + p->off_packed) : nullptr;
+  a.seeds = w ? (T *)(w + p->off_seeds) : nullptr;
+  a.lp = w ? (T *)(w + p->off_lp) : nullptr;
+  a.partial = w ? (T *)(w + p->off_partial) : nullptr;
   a.resid = nullptr;
-  a.dloss = (const T *)(w + p->off_one);
+  a.dloss = w ? (const T *)(w + p->off_one) : nullptr;
   a.acc_lds = p->acc_lds;
   a.tile_rows = 16 * p->k2_waves;
   a.seed_mult = nullptr;
   a.seed_mult_n = 0;
+  a.u_out = nullptr;
+  a.u_in = nullptr;
   return a;
 }
 
@@ -1700,8 +1885,11 @@ int launch_pack(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
 template <typename T, int NB>
 int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStream_t s,
                    void *ws) {
-  int rc = launch_pack<T, NB>(p, a, s);
-  if (rc) return rc;
+  // given network values need no network image
+  if (!a.u_in) {
+    int rc = launch_pack<T, NB>(p, a, s);
+    if (rc) return rc;
+  }
   // the causal loss and the weak form read group 0's residuals
   if ((p->causal || p->weak) && !resid) resid = (T *)((char *)ws + p->off_resid);
   a.resid = resid;
@@ -1773,12 +1961,7 @@ int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStre
 
 template <typename T, int NB, int NH, int NT>
 int launch_backward_wt(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)k_backward_w<T, NB, NH, NT>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_set = true;
-  }
+  raise_lds_limit({(const void *)k_backward_w<T, NB, NH, NT>});
   ProfScope ps(p, 1, s);
   hipLaunchKernelGGL((k_backward_w<T, NB, NH, NT>), dim3(p->n_k2), dim3(64 * p->k2_waves),
                      p->k2_lds, s, a);
@@ -1794,12 +1977,7 @@ __host__ __device__ constexpr int block_acc_regs(int NB, int NH) {
 template <typename T, int NB, int NH>
 int launch_backward_c(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
   if constexpr (block_acc_regs(NB, NH) <= 192) {
-    static bool attr_set = false;
-    if (!attr_set) {
-      (void)hipFuncSetAttribute((const void *)k_backward_c<T, NB, NH>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      attr_set = true;
-    }
+    raise_lds_limit({(const void *)k_backward_c<T, NB, NH>});
     ProfScope ps(p, 1, s);
     hipLaunchKernelGGL((k_backward_c<T, NB, NH>), dim3(p->n_k2), dim3(256), p->k2_lds, s, a);
     TDX_TRY_HIP(hipGetLastError());
@@ -1847,12 +2025,7 @@ int launch_backward(const tdx_nn_plan *p, NNArgs<T> a, hipStream_t s) {
     set_error("no block-owning backward for this shape");
     return TDX_EUNSUPPORTED;
   }
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)k_backward<T, NB>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_set = true;
-  }
+  raise_lds_limit({(const void *)k_backward<T, NB>});
   {
     ProfScope ps(p, 1, s);
     hipLaunchKernelGGL((k_backward<T, NB>), dim3(p->n_k2), dim3(64 * p->k2_waves), p->k2_lds,
@@ -1873,15 +2046,24 @@ int launch_reduce(const tdx_nn_plan *p, const NNArgs<T> &a, int mode, T *dparams
   return TDX_OK;
 }
 
+// op 0: forward; 1: backward; 2: fused Adam step; 3: forward keeping the
+// network values (values = u_out); 4: forward from given network values
+// (values = u_in, seeds to `seeds` when given)
 template <typename T>
 int nn_dispatch(const tdx_nn_plan *p, int op, const void *params, void *params_mut, void *loss,
                 void *resid, const void *dloss, void *dparams, void *m, void *v,
-                int64_t *step, AdamHyper h, void *ws, hipStream_t s) {
+                int64_t *step, AdamHyper h, void *ws, hipStream_t s, void *values = nullptr,
+                void *seeds = nullptr) {
   NNArgs<T> a = make_args<T>(p, params ? params : params_mut, ws);
+  if (op == 3) a.u_out = (T *)values;
+  if (op == 4) {
+    a.u_in = (const T *)values;
+    if (seeds) a.seeds = (T *)seeds;
+  }
 #define TDX_NB_CASE(NBV)                                                          \
   case NBV: {                                                                     \
     int rc = TDX_OK;                                                              \
-    if (op == 0) {                                                                \
+    if (op == 0 || op == 3 || op == 4) {                                          \
       rc = launch_forward<T, NBV>(p, a, (T *)loss, (T *)resid, s, ws);            \
     } else if (op == 1) {                                                         \
       a.dloss = (const T *)dloss;                                                 \
@@ -2028,6 +2210,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   p->n_inst = d->n_inst;
   p->n_groups = d->n_groups;
   p->n_rows = n_rows;
+  p->n_resid = d->n_resid;
   const int H = p->H;
   p->P = H * d->d_in + H + p->NH * (H * H + H) + d->n_out * H + d->n_out;
   const size_t ts = d->dtype == 0 ? 4 : 8;
@@ -2045,6 +2228,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   int max_slots = 1;
   for (int g = 0; g < d->n_groups; ++g)
     max_slots = std::max(max_slots, (int)d->groups[(int64_t)g * TDX_GROUP_INTS + 2]);
+  p->max_slots = max_slots;
   p->k1_lds = (2 * K1_WAVES * 16 * max_slots * d->n_out + 512) * ts;
   {
     // stage the forward operand images in LDS when they are small
@@ -2238,6 +2422,63 @@ extern "C" int tdx_nn_forward(const tdx_nn_plan *p, const void *params, void *lo
                               nullptr, nullptr, h, ws, s);
   return nn_dispatch<double>(p, 0, params, nullptr, loss, resid, nullptr, nullptr, nullptr,
                              nullptr, nullptr, h, ws, s);
+}
+
+extern "C" int64_t tdx_nn_row_count(const tdx_nn_plan *p) { return p ? p->n_rows : -1; }
+
+extern "C" int tdx_nn_network_values(const tdx_nn_plan *p, const void *params, void *values,
+                                     void *loss, void *ws, tdx_stream_t stream) {
+  TDX_REQUIRE(p && params && values && ws, "null argument");
+  AdamHyper h{0, 0, 0, 0};
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == 0)
+    return nn_dispatch<float>(p, 3, params, nullptr, loss, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, nullptr, h, ws, s, values);
+  return nn_dispatch<double>(p, 3, params, nullptr, loss, nullptr, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, h, ws, s, values);
+}
+
+extern "C" int tdx_nn_forward_values(const tdx_nn_plan *p, const void *values, void *loss,
+                                     void *resid, void *seeds, void *ws, tdx_stream_t stream) {
+  TDX_REQUIRE(p && values && ws, "null argument");
+  AdamHyper h{0, 0, 0, 0};
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == 0)
+    return nn_dispatch<float>(p, 4, nullptr, nullptr, loss, resid, nullptr, nullptr, nullptr,
+                              nullptr, nullptr, h, ws, s, const_cast<void *>(values), seeds);
+  return nn_dispatch<double>(p, 4, nullptr, nullptr, loss, resid, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, h, ws, s, const_cast<void *>(values), seeds);
+}
+
+// LDS words per thread of k_jac.
+static size_t jac_words(const tdx_nn_plan *p, int max_slots) {
+  return (size_t)(p->NH + 3) * p->H + 2 * (size_t)max_slots * p->n_out;
+}
+
+extern "C" int tdx_nn_jacobian(const tdx_nn_plan *p, const void *params, void *jt,
+                               tdx_stream_t stream) {
+  TDX_REQUIRE(p && params && jt, "null argument");
+  TDX_REQUIRE(p->n_inst == 1, "the Jacobian serves one instance per plan");
+  const size_t ts = p->dtype == 0 ? 4 : 8;
+  const size_t per_thread = jac_words(p, p->max_slots) * ts;
+  int B = 64;
+  while (B > 1 && per_thread * B > 160 * 1024) B >>= 1;
+  TDX_REQUIRE(per_thread * B <= 160 * 1024, "network too wide for the Jacobian kernel");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = (int64_t)p->P * p->n_resid;
+  const unsigned zb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
+  auto run = [&](auto tag) -> int {
+    using T = decltype(tag);
+    NNArgs<T> a = make_args<T>(p, params, nullptr);
+    hipLaunchKernelGGL((k_jac_zero<T>), dim3(zb), dim3(256), 0, s, (T *)jt, n);
+    TDX_TRY_HIP(hipGetLastError());
+    raise_lds_limit({(const void *)k_jac<T>});
+    hipLaunchKernelGGL((k_jac<T>), dim3(p->n_k1), dim3(B), per_thread * B, s, a, (T *)jt,
+                       (int64_t)p->n_resid);
+    TDX_TRY_HIP(hipGetLastError());
+    return TDX_OK;
+  };
+  return p->dtype == 0 ? run(0.0f) : run(0.0);
 }
 
 extern "C" int tdx_nn_backward(const tdx_nn_plan *p, const void *params, const void *dloss,

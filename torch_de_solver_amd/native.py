@@ -25,6 +25,7 @@ EXPORTS = [
     'tdx_version', 'tdx_last_error',
     'tdx_nn_plan_create', 'tdx_nn_plan_destroy', 'tdx_nn_param_count',
     'tdx_nn_workspace_bytes', 'tdx_nn_forward', 'tdx_nn_backward',
+    'tdx_nn_row_count', 'tdx_nn_network_values', 'tdx_nn_forward_values', 'tdx_nn_jacobian',
     'tdx_nn_train_step', 'tdx_nn_profile', 'tdx_nn_profile_read',
     'tdx_mat_plan_create', 'tdx_mat_plan_destroy', 'tdx_mat_workspace_bytes',
     'tdx_mat_forward', 'tdx_mat_backward', 'tdx_mat_train_step', 'tdx_mat_train_step_to',
@@ -116,6 +117,11 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.tdx_nn_workspace_bytes.argtypes = [vp]
     lib.tdx_nn_workspace_bytes.restype = sz
     lib.tdx_nn_forward.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.tdx_nn_row_count.argtypes = [vp]
+    lib.tdx_nn_row_count.restype = i64
+    lib.tdx_nn_network_values.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.tdx_nn_forward_values.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.tdx_nn_jacobian.argtypes = [vp, vp, vp, vp]
     lib.tdx_nn_backward.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.tdx_nn_train_step.argtypes = [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]
     lib.tdx_nn_profile.argtypes = [vp, ctypes.c_int]

@@ -1,0 +1,124 @@
+"""Natural-gradient descent (NGD) over the native 'NN' engine.
+
+The reference's optimizer (tedeous/optimizers/ngd.py:8-178, with the closure
+of tedeous/optimizers/closure.py:98-117) takes one step as:
+
+1. loss and gradient of the lambda-weighted loss;
+2. the Gram matrices ``1/N J^T J`` of the operator residuals (flattened
+   ``[N, n_eq]``) and of the padded boundary residuals ``bval - true_bval``
+   (flattened ``[max_len, n_types]``), J built with one
+   ``torch.autograd.grad`` per residual entry (ngd.py:57-78);
+3. the Marquardt term ``min(loss, 0) * I`` (zero for a non-negative loss);
+4. the natural gradient ``f`` as the SVD least-squares solution of
+   ``G f = grad`` (singular values above the dtype's machine epsilon
+   inverted, ngd.py:104-126);
+5. a grid line search: the loss at ``params - 0.5**k * f`` for
+   ``k = 0..grid_steps_number``, the first minimum taken (ngd.py:29-55).
+
+Here J comes from one kernel (``tdx_nn_jacobian``: every residual's
+seed-weighted network adjoint, one thread per entry), the Gram products are
+two library GEMMs on the device, and the line search evaluates all candidate
+parameter vectors in one launch of a grouped plan that holds the problem
+once per candidate.  The SVD is torch's (the reference calls the same
+routine).  The step stays on the device: no host synchronisation.
+"""
+
+from typing import Optional
+
+import torch
+
+from .nn_engine import NNEngine
+
+
+class NGD(torch.optim.Optimizer):
+    """Drop-in for ``tedeous.optimizers.ngd.NGD`` on 'NN'-mode solutions."""
+
+    def __init__(self, params, grid_steps_number: int = 30):
+        defaults = {'grid_steps_number': grid_steps_number}
+        super().__init__(params, defaults)
+        self.params = self.param_groups[0]['params']
+        self.grid_steps_number = grid_steps_number
+        self.grid_steps = torch.linspace(0, grid_steps_number, grid_steps_number + 1)
+        self.steps = 0.5 ** self.grid_steps
+        self._cache = None      # (engine, grouped line-search engine, index sets)
+        self.last: dict = {}    # intermediate values of the last step
+
+    # -- pieces ----------------------------------------------------------------
+    @staticmethod
+    def lstsq(A: torch.Tensor, B: torch.Tensor, tol: Optional[float] = None) -> torch.Tensor:
+        """Least-squares solution by SVD with singular values above ``tol``
+        (default: machine epsilon of the dtype) inverted."""
+        tol = torch.finfo(A.dtype).eps if tol is None else tol
+        U, S, Vh = torch.linalg.svd(A, full_matrices=False)
+        Spinv = torch.where(S > tol, 1 / S, torch.zeros_like(S))
+        UhB = U.adjoint() @ B
+        if Spinv.ndim != UhB.ndim:
+            Spinv = Spinv.unsqueeze(-1)
+        return Vh.adjoint() @ (Spinv * UhB)
+
+    def _prepare(self, sol):
+        eng: NNEngine = sol.engine
+        if self._cache is not None and self._cache[0] is eng:
+            return self._cache
+        if eng.n_inst != 1:
+            raise NotImplementedError("NGD serves one instance per solution")
+        op_idx, b_idx = [], []
+        for r in eng.tables.resid:
+            (op_idx if r.column < 0 else b_idx).extend(range(r.offset, r.offset + r.count))
+        dev = eng.device
+        op_idx = torch.tensor(op_idx, dtype=torch.long, device=dev)
+        b_idx = torch.tensor(b_idx, dtype=torch.long, device=dev)
+        prob = sol.problem
+        n_b = prob.bc_max_len * len(prob.bc_keys)
+        search = NNEngine([prob] * len(self.steps), sol.arch, dev)
+        self._cache = (eng, search, op_idx, b_idx, max(n_b, 1))
+        return self._cache
+
+    def gram(self, jt: torch.Tensor, op_idx: torch.Tensor, b_idx: torch.Tensor,
+             n_b: int) -> torch.Tensor:
+        """``1/N_op J_op^T J_op + 1/N_b J_b^T J_b`` from the transposed J."""
+        j_op = jt.index_select(1, op_idx)
+        g = (1.0 / op_idx.numel() * j_op) @ j_op.T
+        if b_idx.numel():
+            j_b = jt.index_select(1, b_idx)
+            g = g + (1.0 / n_b * j_b) @ j_b.T
+        return g
+
+    # -- one step --------------------------------------------------------------
+    def step(self, closure=None):
+        if closure is None:
+            raise ValueError("NGD needs the closure of Closure.get_closure('NGD')")
+        int_res, bval, true_bval, loss, loss_function = closure()
+        sol = getattr(loss_function, '__self__', None)
+        if sol is None or getattr(sol, 'mode', None) != 'NN':
+            raise NotImplementedError("NGD is served for 'NN'-mode solutions")
+        eng, search, op_idx, b_idx, n_b = self._prepare(sol)
+        params = self.params
+        flat = torch.cat([p.detach().reshape(-1) for p in params]).contiguous()
+        grads = [p.grad for p in params]
+        if any(g is None for g in grads):
+            grads = torch.autograd.grad(loss, params, retain_graph=True, allow_unused=True)
+            grads = [torch.zeros_like(p) if g is None else g for p, g in zip(params, grads)]
+        f_grads = torch.cat([g.reshape(-1) for g in grads]).to(flat.dtype)
+        jt = eng.jacobian(flat)
+        G = self.gram(jt, op_idx, b_idx, n_b)
+        damp = torch.minimum(loss.detach().reshape(()).to(G.dtype), G.new_zeros(()))
+        G = damp * torch.eye(len(G), dtype=G.dtype, device=G.device) + G
+        f = self.lstsq(G, f_grads)
+        # every candidate's loss in one launch
+        K = len(self.steps)
+        steps = self.steps.to(device=flat.device, dtype=flat.dtype)
+        cand = (flat.unsqueeze(0) - steps.view(K, 1) * f.unsqueeze(0)).contiguous()
+        losses = torch.empty(2 * K, dtype=flat.dtype, device=flat.device)
+        search.forward(cand.view(-1), losses)
+        losses = losses.view(K, 2)[:, 0]
+        k = torch.argmin(losses)
+        new = cand.index_select(0, k.view(1)).view(-1)
+        with torch.no_grad():
+            off = 0
+            for p in params:
+                n = p.numel()
+                p.copy_(new[off:off + n].view_as(p))
+                off += n
+        self.last = dict(G=G, f=f, losses=losses, k=k, grad=f_grads, jt=jt)
+        return loss

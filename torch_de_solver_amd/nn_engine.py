@@ -247,6 +247,10 @@ class NNEngine:
         self.workspace = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device)
         self.loss_buf = torch.zeros(2 * self.n_inst, dtype=dtype, device=self.device)
         self.resid_buf = torch.zeros(max(t.n_resid, 1), dtype=dtype, device=self.device)
+        self.n_rows = int(lib.tdx_nn_row_count(handle))
+        # forward passes so far: the adjoint reads the seeds of the latest
+        # one, so a backward checks that it belongs to its forward
+        self.fwd_gen = 0
 
     def __del__(self):
         try:
@@ -264,6 +268,52 @@ class NNEngine:
             self._plan, native.ptr(params), native.ptr(loss),
             native.ptr(resid) if resid is not None else None,
             native.ptr(self.workspace), native.stream_ptr()))
+        self.fwd_gen += 1
+
+    def network_values(self, params: torch.Tensor) -> torch.Tensor:
+        """Network output of every (group, slot, entry) row ``[n_rows, n_out]``
+        (row ``row_base + slot * n_entries + entry``), from a forward pass."""
+        self._check_params(params)
+        out = torch.empty(self.n_rows, self.arch.n_out, dtype=self.dtype, device=self.device)
+        native.check(native.lib().tdx_nn_network_values(
+            self._plan, native.ptr(params), native.ptr(out), None,
+            native.ptr(self.workspace), native.stream_ptr()))
+        self.fwd_gen += 1
+        return out
+
+    def forward_values(self, values: torch.Tensor, loss: torch.Tensor,
+                       resid: Optional[torch.Tensor] = None,
+                       seeds: Optional[torch.Tensor] = None) -> None:
+        """The residual program on given network values ``[n_rows, n_out]``:
+        losses, residuals and the seeds dL/du of every row (to ``seeds`` when
+        given, else to the workspace for a following backward)."""
+        if values.device.type != 'cuda' or values.dtype != self.dtype or \
+                not values.is_contiguous() or values.numel() != self.n_rows * self.arch.n_out:
+            raise ValueError(f"values must be a contiguous [{self.n_rows}, {self.arch.n_out}] "
+                             "device tensor of the problem dtype")
+        if seeds is not None and (seeds.dtype != self.dtype or not seeds.is_contiguous() or
+                                  seeds.numel() != values.numel()):
+            raise ValueError("seeds must match values")
+        native.check(native.lib().tdx_nn_forward_values(
+            self._plan, native.ptr(values), native.ptr(loss),
+            native.ptr(resid) if resid is not None else None,
+            native.ptr(seeds) if seeds is not None else None,
+            native.ptr(self.workspace), native.stream_ptr()))
+        self.fwd_gen += 1
+
+    def jacobian(self, params: torch.Tensor) -> torch.Tensor:
+        """``[P, n_resid]``: d resid[r] / d params[p] of every residual entry
+        (the transposed J of the natural gradient), one instance."""
+        if self.n_inst != 1:
+            raise NotImplementedError("the Jacobian serves one instance per plan")
+        self._check_params(params)
+        n = self.tables.n_resid
+        if self.n_params * n * params.element_size() > (16 << 30):
+            raise ValueError(f"the Jacobian would take {self.n_params} x {n} values")
+        jt = torch.empty(self.n_params, n, dtype=self.dtype, device=self.device)
+        native.check(native.lib().tdx_nn_jacobian(
+            self._plan, native.ptr(params), native.ptr(jt), native.stream_ptr()))
+        return jt
 
     def backward(self, params: torch.Tensor, dloss: torch.Tensor,
                  dparams: torch.Tensor) -> None:
@@ -284,6 +334,7 @@ class NNEngine:
             native.ptr(step), float(lr), float(beta1), float(beta2), float(eps),
             native.ptr(loss) if loss is not None else None,
             native.ptr(self.workspace), native.stream_ptr()))
+        self.fwd_gen += 1
 
     def profile(self, capacity: int) -> None:
         """Time up to ``capacity`` launches per kernel kind with HIP events
@@ -327,6 +378,7 @@ class NNResidualLoss(torch.autograd.Function):
         loss = torch.empty(2 * engine.n_inst, dtype=engine.dtype, device=engine.device)
         engine.forward(p, loss)
         ctx.engine = engine
+        ctx.gen = engine.fwd_gen
         ctx.save_for_backward(p)
         return loss.view(engine.n_inst, 2)
 
@@ -334,6 +386,10 @@ class NNResidualLoss(torch.autograd.Function):
     def backward(ctx, grad_out: torch.Tensor):
         (p,) = ctx.saved_tensors
         eng: NNEngine = ctx.engine
+        if eng.fwd_gen != ctx.gen:
+            # another forward pass ran since this node's: restore its seeds
+            # before the adjoint
+            eng.forward(p, torch.empty(2 * eng.n_inst, dtype=eng.dtype, device=eng.device))
         g = grad_out.reshape(eng.n_inst, 2)
         # the normalized loss carries no gradient (computed under no_grad in
         # tedeous/losses.py:124-126); instances share one adjoint call, which

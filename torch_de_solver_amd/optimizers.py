@@ -1,8 +1,9 @@
 """``Optimizer`` and ``Closure`` (tedeous/optimizers/optimizer.py:11-73,
 tedeous/optimizers/closure.py:5-140), for the optimizers this engine serves.
 
-The swarm, natural-gradient and Nystrom-Newton optimizers of the reference
-(PSO, CSO, NGD, NNCG) are not provided.
+Natural-gradient descent ('NGD') runs on the native engine (``ngd.NGD``);
+the swarm and Nystrom-Newton optimizers of the reference (PSO, CSO, NNCG)
+are not provided.
 """
 
 from typing import Union
@@ -33,9 +34,15 @@ class Optimizer:
     def optimizer_choice(self, mode: str, model):
         """torch optimizer over the network parameters ('NN') or the grid
         tensor ('mat') (tedeous/optimizers/optimizer.py:25-73)."""
-        if self.optimizer not in _TORCH:
+        if self.optimizer == 'NGD':
+            if mode != 'NN':
+                raise NotImplementedError("NGD is served for 'NN'-mode solutions")
+            from .ngd import NGD
+            cls = NGD
+        elif self.optimizer not in _TORCH:
             raise NotImplementedError(f"optimizer {self.optimizer!r} is not provided")
-        cls = _TORCH[self.optimizer]
+        else:
+            cls = _TORCH[self.optimizer]
         if mode == 'mat':
             opt = cls([model.requires_grad_()], **self.params)
         else:
@@ -83,7 +90,21 @@ class Closure:
         self.model.cur_loss = loss_normalized if self.normalized_loss_stop else loss
         return loss
 
+    def _closure_ngd(self):
+        """(tedeous/optimizers/closure.py:98-117): loss and gradient, then the
+        operator residuals and the boundary values, and the evaluate function
+        the line search calls."""
+        self.optimizer.zero_grad()
+        sol = self.model.solution_cls
+        loss, loss_normalized = sol.evaluate()
+        loss.backward(retain_graph=True)
+        self.model.cur_loss = loss_normalized if self.normalized_loss_stop else loss
+        op, bval, true_bval, _, _ = sol._side()
+        return op, bval, true_bval, loss, sol.evaluate
+
     def get_closure(self, _type: str):
-        if _type in ('PSO', 'CSO', 'NGD', 'NNCG'):
+        if _type in ('PSO', 'CSO', 'NNCG'):
             raise NotImplementedError(f"optimizer {_type} is not provided")
+        if _type == 'NGD':
+            return self._closure_ngd
         return self._closure

@@ -51,6 +51,15 @@ def _operators(op) -> List[dict]:
     return list(op)
 
 
+def _lambda_key(lam) -> Tuple:
+    """Value identity of a lambda argument (number, list or tensor)."""
+    if isinstance(lam, torch.Tensor):
+        return tuple(float(x) for x in lam.detach().cpu().reshape(-1).tolist())
+    if isinstance(lam, (int, float)):
+        return (float(lam),)
+    return tuple(float(x) for x in lam)
+
+
 class Solution:
     """Objective of one problem (tedeous/solution.py:19-168)."""
 
@@ -73,37 +82,65 @@ class Solution:
             raise NotImplementedError("mini-batching with a causal loss is not served")
         if mode not in ('NN', 'mat', 'autograd'):
             raise NotImplementedError(f"mode {mode!r} is not served by this engine")
+        if not equal_cls.bconds:
+            # the reference builds its boundary matrix with max() over the
+            # condition columns and fails without conditions in every mode
+            # (tedeous/eval.py:75); its np.inf return for 'mat'/'autograd'
+            # (tedeous/losses.py:252-255) is never reached
+            raise ValueError("a problem without boundary conditions has no boundary "
+                             "matrix (max() arg is an empty sequence)")
         self.grid = grid
         self.mode = mode
         self.model = model
         self.lambda_operator = lambda_operator
         self.lambda_bound = lambda_bound
         self.derivative_points = derivative_points
-        ops = _operators(equal_cls.operator)
+        self._equal_cls = equal_cls
+        self._weak_form = weak_form
+        self._tol = tol
+        self._exact_div = exact_div
+        self._build()
+        self.loss = None
+        self.loss_normalized = None
+
+    def _build(self) -> None:
+        """Lower the problem with the current lambdas and create its plan."""
+        eq = self._equal_cls
+        ops = _operators(eq.operator)
         device = torch.device('cuda', torch.cuda.current_device()) if torch.cuda.is_available() \
             else torch.device('cpu')
-        if mode == 'NN':
-            self.arch: MLPArch = mlp_arch(model)
+        lam_op, lam_bnd = self.lambda_operator, self.lambda_bound
+        if self.mode == 'NN':
+            self.arch: MLPArch = mlp_arch(self.model)
             self.problem: NNProblem = compile_nn(
-                grid, ops, equal_cls.bconds, n_out=self.arch.n_out, h=equal_cls.h,
-                inner_order=equal_cls.inner_order, boundary_order=equal_cls.boundary_order,
-                lambda_operator=lambda_operator, lambda_bound=lambda_bound, tol=tol,
-                weak_form=weak_form)
+                self.grid, ops, eq.bconds, n_out=self.arch.n_out, h=eq.h,
+                inner_order=eq.inner_order, boundary_order=eq.boundary_order,
+                lambda_operator=lam_op, lambda_bound=lam_bnd, tol=self._tol,
+                weak_form=self._weak_form)
             self.engine = NNEngine([self.problem], self.arch, device)
-        elif mode == 'autograd':
-            self.arch = mlp_arch(model)
+        elif self.mode == 'autograd':
+            self.arch = mlp_arch(self.model)
             self.problem: ADProblem = compile_ad(
-                grid, ops, equal_cls.bconds, n_out=self.arch.n_out,
-                lambda_operator=lambda_operator, lambda_bound=lambda_bound, tol=tol,
-                weak_form=weak_form)
+                self.grid, ops, eq.bconds, n_out=self.arch.n_out,
+                lambda_operator=lam_op, lambda_bound=lam_bnd, tol=self._tol,
+                weak_form=self._weak_form)
             self.engine = ADEngine(self.problem, self.arch, device)
         else:
             self.problem: MatProblem = compile_mat(
-                grid, ops, equal_cls.bconds, n_eq=len(ops), derivative_points=derivative_points,
-                lambda_operator=lambda_operator, lambda_bound=lambda_bound)
-            self.engine = MatEngine(self.problem, device, exact_div=exact_div)
-        self.loss = None
-        self.loss_normalized = None
+                self.grid, ops, eq.bconds, n_eq=len(ops),
+                derivative_points=self.derivative_points,
+                lambda_operator=lam_op, lambda_bound=lam_bnd)
+            self.engine = MatEngine(self.problem, device, exact_div=self._exact_div)
+        self._lam_key = (_lambda_key(lam_op), _lambda_key(lam_bnd))
+
+    def _sync_lambdas(self) -> None:
+        """The reference reads ``lambda_operator``/``lambda_bound`` on every
+        evaluate (tedeous/solution.py:149-150), and callbacks may replace
+        them between steps (tedeous/callbacks/adaptive_lambda.py:114): a
+        change rebuilds the plan's tables with the new weights."""
+        key = (_lambda_key(self.lambda_operator), _lambda_key(self.lambda_bound))
+        if key != self._lam_key:
+            self._build()
 
     # -- per step ------------------------------------------------------------
     def _flat_params(self) -> torch.Tensor:
@@ -112,6 +149,7 @@ class Solution:
 
     def evaluate(self, save_graph: bool = True) -> Tuple[torch.Tensor, torch.Tensor]:
         """``(loss, loss_normalized)``, each of shape ``[1]``."""
+        self._sync_lambdas()
         if self.mode in ('NN', 'autograd'):
             pair = self.engine.loss(self._flat_params())[0]
         else:

@@ -117,25 +117,26 @@ def burgers_equation(nu: float) -> dict:
 
 
 def burgers(nx: int = 256, nt: int = 100, nu: float = 0.01 / math.pi,
-            layers=(2, 100, 100, 100, 100, 1), seed: int = 0, h: float = 1e-3):
+            layers=(2, 100, 100, 100, 100, 1), seed: int = 0, h: float = 1e-3,
+            dtype: str = 'float32'):
     d = Domain()
-    d.variable('x', [-1, 1], nx - 1)
-    d.variable('t', [0, 1], nt - 1)
+    d.variable('x', [-1, 1], nx - 1, dtype=dtype)
+    d.variable('t', [0, 1], nt - 1, dtype=dtype)
     b = Conditions()
     b.dirichlet({'x': [-1, 1], 't': 0}, value=lambda g: -torch.sin(np.pi * g[:, 0]))
     b.dirichlet({'x': -1, 't': [0, 1]}, value=0)
     b.dirichlet({'x': 1, 't': [0, 1]}, value=0)
     eq = Equation()
     eq.add(burgers_equation(nu))
-    net = tanh_mlp(list(layers), seed)
+    net = tanh_mlp(list(layers), seed, dtype=torch.float64 if dtype == 'float64' else torch.float32)
     return d, eq, b, net, dict(mode='NN', lambda_operator=1, lambda_bound=10, h=h)
 
 
-def kdv(nx: int = 512, nt: int = 128, L: float = 20.0, seed: int = 0):
+def kdv(nx: int = 512, nt: int = 128, L: float = 20.0, seed: int = 0, dtype: str = 'float32'):
     d = Domain()
     x = L * torch.arange(nx, dtype=torch.float32) / nx
-    d.variable('x', x, None)
-    d.variable('t', [0, 1], nt - 1)
+    d.variable('x', x, None, dtype=dtype)
+    d.variable('t', [0, 1], nt - 1, dtype=dtype)
     b = Conditions()
     b.periodic([{'x': 0, 't': [0, 1]}, {'x': L, 't': [0, 1]}])
     b.dirichlet({'x': [0, L], 't': 0}, value=0.5 / torch.cosh(0.5 * (x - L / 2)) ** 2)
@@ -145,7 +146,8 @@ def kdv(nx: int = 512, nt: int = 128, L: float = 20.0, seed: int = 0):
         '6u*u_x': {'coeff': 6, 'u*u_x': [[None], [0]], 'pow': [1, 1], 'var': [0, 0]},
         'u_xxx': {'coeff': 1, 'u_xxx': [0, 0, 0], 'pow': 1, 'var': 0},
     })
-    net = tanh_mlp([2, 64, 64, 64, 64, 1], seed)
+    net = tanh_mlp([2, 64, 64, 64, 64, 1], seed,
+                   dtype=torch.float64 if dtype == 'float64' else torch.float32)
     return d, eq, b, net, dict(mode='NN', lambda_operator=1, lambda_bound=100, h=0.01)
 
 
@@ -170,8 +172,9 @@ def sweep_instances(n: int = 256, seed: int = 0) -> List[SweepInstance]:
     return out
 
 
-def sweep_problem(inst: SweepInstance):
-    return burgers(nx=inst.nx, nt=inst.nt, nu=inst.nu, layers=(2, 32, 32, 32, 1), seed=inst.seed)
+def sweep_problem(inst: SweepInstance, dtype: str = 'float32'):
+    return burgers(nx=inst.nx, nt=inst.nt, nu=inst.nu, layers=(2, 32, 32, 32, 1), seed=inst.seed,
+                   dtype=dtype)
 
 
 def collocation_points(nx: int, nt: int) -> int:
