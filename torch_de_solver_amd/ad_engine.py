@@ -138,6 +138,38 @@ class ADTables:
         self.data = np.concatenate(data).astype(np_dtype) if data else np.zeros(1, dtype=np_dtype)
 
 
+    def desc(self, prob: ADProblem, arch: MLPArch) -> 'native.ADDesc':
+        """The plan descriptor of include/tdx.h over these tables (which must
+        outlive the plan-create call)."""
+        desc = native.ADDesc()
+        desc.dtype = 0 if prob.dtype == torch.float32 else 1
+        desc.d_in = arch.d_in
+        desc.n_out = arch.n_out
+        desc.width = arch.width
+        desc.n_layers = arch.n_layers
+        desc.n_jets = self.n_jets
+        desc.jet_i = self.jet_i
+        desc.pptr_i = self.pptr_i
+        desc.part_i = self.part_i
+        desc.n_points = self.n_points
+        desc.points = self.points.ctypes.data
+        desc.n_groups = len(prob.groups)
+        desc.groups_i = self.groups_i
+        desc.n_ints = self.ints.size
+        desc.ints = self.ints.ctypes.data_as(native.c_i32p)
+        desc.n_reals = self.reals.size
+        desc.reals = self.reals.ctypes.data_as(native.c_dp)
+        desc.n_data = self.data.size
+        desc.data = self.data.ctypes.data
+        desc.n_resid = self.n_resid
+        desc.causal_tol = float(prob.causal_tol)
+        desc.causal_nt = int(prob.causal_nt)
+        if prob.weak_levels:
+            desc.weak_levels, desc.weak_runs = prob.weak_levels, len(prob.weak_cx)
+            desc.weak_d, desc.weak_i = self.weak_d, self.weak_i
+        return desc
+
+
 class ADEngine:
     """One native 'autograd' plan (one network instance)."""
 
@@ -155,32 +187,7 @@ class ADEngine:
         self.tables = t = ADTables(prob, np_dtype)
         lib = native.lib()
         self._keep = (t.ints, t.reals, t.data, t.points)
-        desc = native.ADDesc()
-        desc.dtype = 0 if prob.dtype == torch.float32 else 1
-        desc.d_in = arch.d_in
-        desc.n_out = arch.n_out
-        desc.width = arch.width
-        desc.n_layers = arch.n_layers
-        desc.n_jets = t.n_jets
-        desc.jet_i = t.jet_i
-        desc.pptr_i = t.pptr_i
-        desc.part_i = t.part_i
-        desc.n_points = t.n_points
-        desc.points = t.points.ctypes.data
-        desc.n_groups = len(prob.groups)
-        desc.groups_i = t.groups_i
-        desc.n_ints = t.ints.size
-        desc.ints = t.ints.ctypes.data_as(native.c_i32p)
-        desc.n_reals = t.reals.size
-        desc.reals = t.reals.ctypes.data_as(native.c_dp)
-        desc.n_data = t.data.size
-        desc.data = t.data.ctypes.data
-        desc.n_resid = t.n_resid
-        desc.causal_tol = float(prob.causal_tol)
-        desc.causal_nt = int(prob.causal_nt)
-        if prob.weak_levels:
-            desc.weak_levels, desc.weak_runs = prob.weak_levels, len(prob.weak_cx)
-            desc.weak_d, desc.weak_i = t.weak_d, t.weak_i
+        desc = t.desc(prob, arch)
         handle = ctypes.c_void_p()
         with torch.cuda.device(device):
             native.check(lib.tdx_ad_plan_create(ctypes.byref(desc), ctypes.byref(handle)))

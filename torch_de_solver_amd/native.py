@@ -9,7 +9,9 @@ import os
 from typing import Optional
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, 'libtdx.so')
+# TDX_LIB_PATH selects another build of the library (e.g. an instrumented
+# host build); by default the in-tree libtdx.so
+LIB_PATH = os.environ.get('TDX_LIB_PATH') or os.path.join(HERE, 'libtdx.so')
 
 TDX_GROUP_INTS = 8
 TDX_EQ_INTS = 7

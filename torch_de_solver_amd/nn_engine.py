@@ -184,6 +184,38 @@ class NNTables:
                      else np.zeros(1, dtype=np_dtype)).astype(np_dtype)
         self.n_resid = n_resid
         self.n_rows = row
+        self.n_inst = len(problems)
+        self.causal = (float(prob.causal_tol), int(prob.causal_nt)) if prob.causal_tol else None
+
+    def desc(self, arch: MLPArch, tile_rows: int = 0, chunk_tiles: int = 0) -> 'native.NNDesc':
+        """The plan descriptor of include/tdx.h over these tables (which must
+        outlive the plan-create call)."""
+        desc = native.NNDesc()
+        desc.dtype = 0 if self.np_dtype == np.float32 else 1
+        desc.d_in, desc.n_out = arch.d_in, arch.n_out
+        desc.width, desc.n_layers = arch.width, arch.n_layers
+        desc.n_inst = self.n_inst
+        desc.n_groups = self.n_groups
+        desc.groups = self.groups.ctypes.data_as(native.c_i32p)
+        desc.n_ints = self.ints.size
+        desc.ints = self.ints.ctypes.data_as(native.c_i32p)
+        desc.n_reals = self.reals.size
+        desc.reals = self.reals.ctypes.data
+        desc.n_data = self.data.size
+        desc.data = self.data.ctypes.data
+        desc.n_resid = self.n_resid
+        desc.tile_rows = tile_rows
+        desc.chunk_tiles = chunk_tiles
+        if self.causal:
+            if self.n_inst != 1:
+                raise NotImplementedError("the causal loss serves one instance per plan")
+            desc.causal_tol, desc.causal_nt = self.causal
+        if self.weak_levels:
+            if self.n_inst != 1:
+                raise NotImplementedError("the weak form serves one instance per plan")
+            desc.weak_levels, desc.weak_runs = self.weak_levels, self.weak_runs
+            desc.weak_d, desc.weak_i = self.weak_d, self.weak_i
+        return desc
 
 
 # ----------------------------------------------------------------------------
@@ -211,32 +243,7 @@ class NNEngine:
         self.tables = NNTables(problems, arch, self.np_dtype)
         t = self.tables
         lib = native.lib()
-        desc = native.NNDesc()
-        desc.dtype = 0 if dtype == torch.float32 else 1
-        desc.d_in, desc.n_out = arch.d_in, arch.n_out
-        desc.width, desc.n_layers = arch.width, arch.n_layers
-        desc.n_inst = self.n_inst
-        desc.n_groups = t.n_groups
-        desc.groups = t.groups.ctypes.data_as(native.c_i32p)
-        desc.n_ints = t.ints.size
-        desc.ints = t.ints.ctypes.data_as(native.c_i32p)
-        desc.n_reals = t.reals.size
-        desc.reals = t.reals.ctypes.data
-        desc.n_data = t.data.size
-        desc.data = t.data.ctypes.data
-        desc.n_resid = t.n_resid
-        desc.tile_rows = tile_rows
-        desc.chunk_tiles = chunk_tiles
-        if problems[0].causal_tol:
-            if self.n_inst != 1:
-                raise NotImplementedError("the causal loss serves one instance per plan")
-            desc.causal_tol = float(problems[0].causal_tol)
-            desc.causal_nt = int(problems[0].causal_nt)
-        if t.weak_levels:
-            if self.n_inst != 1:
-                raise NotImplementedError("the weak form serves one instance per plan")
-            desc.weak_levels, desc.weak_runs = t.weak_levels, t.weak_runs
-            desc.weak_d, desc.weak_i = t.weak_d, t.weak_i
+        desc = t.desc(arch, tile_rows, chunk_tiles)
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             native.check(lib.tdx_nn_plan_create(ctypes.byref(desc), ctypes.byref(handle)))
