@@ -263,7 +263,7 @@ def bench_nn(args):
     nets0 = [copy.deepcopy(p[3]) for p in problems]
     t0 = time.time()
     solver = SweepSolver(problems, autotune=not args.no_tune, tile_rows=args.nn_tile_rows,
-                         chunk_tiles=args.chunk_tiles)
+                         chunk_tiles=args.chunk_tiles, persistent=args.persistent)
     log(f"[bench] compiled {solver.n_inst} instance(s) in {time.time() - t0:.1f}s"
         f" (tuning: {solver.tuning})")
     eng = solver.engine
@@ -641,6 +641,9 @@ def main():
     ap.add_argument('--specialize', type=int, default=0,
                     help="'mat': kernel generated for the program on the interior (0 auto, "
                          "1 always, -1 never)")
+    ap.add_argument('--persistent', type=int, default=0,
+                    help="'NN': 1 runs the forward kernel as a persistent grid (three "
+                         "launches per fused step instead of four)")
     ap.add_argument('--chunk-tiles', type=int, default=0,
                     help="'NN': backward tiles per work item (0: autotuned)")
     ap.add_argument('--nn-tile-rows', type=int, default=0,

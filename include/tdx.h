@@ -126,6 +126,18 @@ typedef struct tdx_nn_desc {
   int weak_runs;
   int64_t weak_d;
   int weak_i;
+  /* Persistent forward (1: on where the plan allows it; 0 or -1: one
+   * workgroup per work item): when the network's forward image fits in LDS
+   * and the plan has no causal or weak-form loss, the forward kernel runs
+   * as a grid of CUs x resident workgroups, each walking a contiguous range
+   * of equal cost of the (instance, entry tile) work items and gathering
+   * its instance's network into LDS from the parameters when the instance
+   * changes; the workgroup holding an instance's first work item also
+   * writes the packed image the backward kernel reads, so a fused step is
+   * three launches (forward, backward, reduction with the Adam update and
+   * the loss pairs) instead of four (packing first).  Results are the same
+   * bits either way. */
+  int persistent;
 } tdx_nn_desc;
 
 typedef struct tdx_nn_plan tdx_nn_plan;

@@ -626,11 +626,18 @@ class MatOracle:
                 true[k] = c['bval'].reshape(-1)
         return pad_matrix(bval, true)
 
-    def evaluate(self, model, lam_op=1, lam_bnd=1):
+    def evaluate(self, model, lam_op=1, lam_bnd=1, tol=0):
         op = self.residual(model)
         b, t, keys, lens = self.boundary(model)
-        loss, norm = default_loss(op, b, t, lam_row(op, lam_op).to(op.dtype),
-                                  lam_row(b, lam_bnd).to(op.dtype))
+        if tol:
+            # the operator matrix stacks each equation's field over n_eq
+            # broadcast copies ([n_eq N, n_eq]) and the slices are its
+            # columns: n_t = op.shape[1] (tedeous/solution.py:62-64,
+            # tedeous/losses.py:160-161)
+            loss, norm = causal_loss(op, b, t, lam_row(b, lam_bnd).to(op.dtype), tol, op.shape[1])
+        else:
+            loss, norm = default_loss(op, b, t, lam_row(op, lam_op).to(op.dtype),
+                                      lam_row(b, lam_bnd).to(op.dtype))
         return loss, norm, op, b, t, keys, lens
 
 

@@ -165,6 +165,35 @@ def test_mat_robin_and_data_against_oracle(dtype):
         np.testing.assert_allclose(gn, go, rtol=1e-5, atol=1e-6 * np.abs(go).max())
 
 
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
+def test_mat_causal_loss_against_oracle(dtype):
+    """'mat' causal loss with one equation: the oracle's restatement (one
+    slice: the operator matrix has one column) against the engine, with a
+    lambda_operator the causal loss ignores."""
+    _print(f"[conditions mat] causal {dtype}")
+    tdt = torch.float64 if dtype == 'float64' else torch.float32
+    d, eq, b = _mat_problem(dtype)
+    orc = O.MatOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst])
+    u0 = problems.mat_init([1, 41, 31], seed=5, dtype=tdt, std=0.3)
+    uo = u0.clone().requires_grad_()
+    lo, no, op, bv, tv, keys, lens = orc.evaluate(uo, 7, 10, tol=0.5)
+    lo.backward()
+    grid = d.build('mat')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict)).set_strategy('mat')
+    un = u0.to(DEV).requires_grad_()
+    sol = Solution(grid, pre, un, 'mat', None, 7, 10, tol=0.5)
+    ln, nn_ = sol.evaluate()
+    ln.backward()
+    gn, go = un.grad.cpu().numpy(), uo.grad.numpy()
+    rt = 1e-10 if dtype == 'float64' else 1e-6
+    np.testing.assert_allclose(float(ln), float(lo), rtol=rt)
+    np.testing.assert_allclose(float(nn_), float(no), rtol=rt)
+    if dtype == 'float32':
+        np.testing.assert_array_equal(sol.op.cpu().numpy(), op.detach().numpy())
+    np.testing.assert_allclose(gn, go, rtol=1e-8 if dtype == 'float64' else 1e-5,
+                               atol=(1e-10 if dtype == 'float64' else 1e-6) * np.abs(go).max())
+
+
 # ---------------------------------------------------------------------------
 # causal loss ('NN' mode, tedeous/losses.py:137-182)
 # ---------------------------------------------------------------------------

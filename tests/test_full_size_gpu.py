@@ -9,16 +9,19 @@ Workloads (torch_de_solver_amd/workloads.py):
   MLP 2-32x3-1 each) in one grouped plan.
 
 Tolerances:
-* float64 kernels: loss and normalized loss 1e-9 relative, residual field
-  1e-8 relative (atol 1e-9 of its largest value), gradient 1e-8 relative
-  (atol 1e-10 of its largest entry): the same arithmetic in another
-  summation order.
+* float64 kernels: loss and normalized loss 1e-9 relative, gradient 1e-8
+  relative (atol 1e-10 of its largest entry): the same arithmetic in
+  another summation order; residual field 1e-8 relative with atol
+  2^k/(2h)^k * 64 eps * max|u| (the stencil of order k divides the network
+  values' last-place differences by (2h)^k).
 * float32, split at the network boundary (tdx_nn_network_values,
   tdx_nn_forward_values; see test_float32_layer_by_layer):
-  - network values: every (slot, entry) row against torch's CPU network on
-    the same float32 inputs, max |du| <= 1e-6 max |u| (the MFMA and the CPU
-    GEMM sum the hidden products in different orders; each output carries
-    a few units of the last place);
+  - network values: every (slot, entry) row against the float64 network on
+    the same float32 inputs, within four times the error torch's float32 CPU
+    network makes there (or 1e-6 of max |u|): the MFMA and the CPU GEMM sum
+    the hidden products in different orders, and five 100-wide layers carry
+    a few units of the last place each (measured: kernel and torch both
+    near 1e-6 of max |u| on the Burgers network);
   - residual program: fed the oracle's own network values, the residual
     field and the boundary values are bit-identical to the oracle's, the
     loss agrees to 1e-6 (reduction order) and the seeds dL/du, summed per
@@ -100,11 +103,16 @@ def test_full_size_float64_against_oracle(name):
     orc, prob, net, ckw = _build(fn, 'float64', **kw)
     _print(f"[full f64] {name}: {prob.n_central} central points, net {mlp_arch(net)}")
     lo, no, op_o, g_o = _oracle_eval(orc, net, ckw)
-    ln, nn, op_n, g_n, _, _ = _engine_eval(prob, net)
+    ln, nn, op_n, g_n, eng, flat = _engine_eval(prob, net)
     np.testing.assert_allclose(ln, lo, rtol=1e-9)
     np.testing.assert_allclose(nn, no, rtol=1e-9)
-    np.testing.assert_allclose(op_n, op_o.reshape(-1), rtol=1e-8,
-                               atol=1e-9 * np.abs(op_o).max())
+    # a residual entry sums stencil weights of total magnitude 2^k / (2h)^k
+    # (k: highest derivative order) times network values that differ by a
+    # few units of the last place (summation order): atol from that bound
+    h, k = {'burgers_c3': (1e-3, 2), 'kdv_c4': (1e-2, 3)}[name]
+    u_max = float(eng.network_values(flat).abs().max())
+    atol = 2 ** k / (2 * h) ** k * 64 * np.finfo(np.float64).eps * u_max
+    np.testing.assert_allclose(op_n, op_o.reshape(-1), rtol=1e-8, atol=atol)
     np.testing.assert_allclose(g_n, g_o, rtol=1e-8, atol=1e-10 * np.abs(g_o).max())
 
 
@@ -157,8 +165,9 @@ def _oracle_on_values(orc, ckw, xb, vals):
 @pytest.mark.parametrize('name', list(CASES))
 def test_float32_layer_by_layer(name):
     """float32 'NN' parity split at the network boundary:
-    (1) network values of every (slot, entry) row against torch's CPU
-        network, max |du| <= 1e-6 max |u|;
+    (1) network values of every (slot, entry) row against the float64
+        network on the same float32 inputs, within four times the error of
+        torch's float32 CPU network (or 1e-6 of max |u|);
     (2) the residual program fed the oracle's own values: residual field
         and boundary matrix bit-identical, loss to 1e-6 (reduction order),
         seeds to 1e-5 of the largest;
@@ -178,13 +187,18 @@ def test_float32_layer_by_layer(name):
     x = _row_inputs(prob)
     assert x.shape[0] == eng.n_rows
     xb = np.ascontiguousarray(x).view(np.uint8).reshape(x.shape[0], -1)
-    # (1) network values
+    # (1) network values against the float64 network on the same inputs,
+    # next to torch's own float32 CPU network
     u_gpu = eng.network_values(flat).cpu().numpy()
     with torch.no_grad():
         u_cpu = net(torch.from_numpy(x)).numpy()
-    u_err = np.abs(u_gpu - u_cpu).max() / np.abs(u_cpu).max()
-    _print(f"  (1) network rows: max|du|/max|u| = {u_err:.3e}")
-    assert u_err <= 1e-6, u_err
+        u64 = copy.deepcopy(net).double()(torch.from_numpy(x).double()).numpy()
+    scale = np.abs(u64).max()
+    u_err = np.abs(u_gpu - u64).max() / scale
+    u_err_cpu = np.abs(u_cpu - u64).max() / scale
+    _print(f"  (1) network rows, max|u - u64|/max|u64|: kernel {u_err:.3e}, torch float32 "
+           f"{u_err_cpu:.3e}; kernel vs torch {np.abs(u_gpu - u_cpu).max() / scale:.3e}")
+    assert u_err <= max(4 * u_err_cpu, 1e-6), (u_err, u_err_cpu)
     # (2) residual program on the oracle's values (one value per distinct
     # input point: the first row that has it)
     lk, lo_c, op_c, bv_c, sg_c = _oracle_on_values(orc, ckw, xb, u_cpu)
@@ -294,7 +308,9 @@ def test_sweep_all_instances_float64_against_oracle():
 
 def test_sweep_grouped_step_bitwise_deterministic():
     """Two runs of the fused float32 grouped step (forward, adjoint, Adam of
-    all 256 instances) from the same state give the same bits."""
+    all 256 instances) from the same state give the same bits, and so does
+    the plan with the persistent forward (three launches per step instead
+    of four)."""
     insts = W.sweep_instances(256)
     probs, nets = [], []
     for inst in insts:
@@ -304,10 +320,11 @@ def test_sweep_grouped_step_bitwise_deterministic():
                                 lambda_bound=ckw['lambda_bound']))
         nets.append(net)
     eng = NNEngine(probs, mlp_arch(nets[0]), DEV)
+    persist = NNEngine(probs, mlp_arch(nets[0]), DEV, persistent=1)
     flat0 = torch.cat([torch.cat([p.detach().reshape(-1) for p in n.parameters()])
                        for n in nets]).to(DEV)
     outs = []
-    for _ in range(2):
+    for eng in (eng, eng, persist):
         p = flat0.clone()
         m, v = torch.zeros_like(p), torch.zeros_like(p)
         step = torch.zeros(1, dtype=torch.int64, device=DEV)
@@ -315,5 +332,6 @@ def test_sweep_grouped_step_bitwise_deterministic():
         for _ in range(3):
             eng.train_step(p, m, v, step, 1e-3, 0.9, 0.999, 1e-8, loss)
         outs.append((p.cpu(), loss.cpu()))
-    assert torch.equal(outs[0][0], outs[1][0])
-    assert torch.equal(outs[0][1], outs[1][1])
+    for k in (1, 2):
+        assert torch.equal(outs[0][0], outs[k][0])
+        assert torch.equal(outs[0][1], outs[k][1])

@@ -515,3 +515,32 @@ def test_ngd_step_closed_form_on_a_linear_model():
     # the natural gradient solves G f = grad
     np.testing.assert_allclose(out['G'].numpy() @ out['f'].numpy(), out['grad'].numpy(),
                                rtol=1e-8, atol=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# 'mat' causal loss: the slices are the operator matrix's columns
+# ---------------------------------------------------------------------------
+
+def test_mat_causal_loss_restatement():
+    """With one equation the 'mat' operator matrix has one column, so the
+    causal loss has one slice, w = 1, and equals the default loss with
+    lambda_operator = 1 whatever lambda_operator was; with two equations
+    (each field broadcast over two copies, [2N, 2]) the operator term is
+    (1/(2N)) sum_i R_i (1 + exp(-tol R_i)), R_i = r_0[i]^2 + r_1[i]^2."""
+    d, eq, b, kw = problems.wave_mat(api, dtype='float64')
+    orc = O.MatOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst])
+    u = problems.mat_init([1, 12, 12], dtype=torch.float64, std=0.3)
+    lc, nc = orc.evaluate(u, 7, 100, tol=0.5)[:2]
+    ld, nd = orc.evaluate(u, 1, 100)[:2]
+    assert float(lc) == float(ld) and float(nc) == float(nd)
+    # two equations: explicit loops over copies and points
+    rng = np.random.default_rng(2)
+    N = 9
+    r = torch.from_numpy(rng.standard_normal((2, N)))
+    op = torch.stack([r[0].repeat(2), r[1].repeat(2)], 1)      # [2N, 2]
+    zb = torch.zeros(3, 1, dtype=torch.float64)
+    tol = 0.3
+    loss, _ = O.causal_loss(op, zb, zb, torch.ones(1, 1, dtype=torch.float64), tol, op.shape[1])
+    R = (r ** 2).sum(0).numpy()
+    ref = sum(R[i] * (1 + np.exp(-tol * R[i])) for i in range(N)) / (2 * N)
+    np.testing.assert_allclose(float(loss), ref, rtol=1e-14)

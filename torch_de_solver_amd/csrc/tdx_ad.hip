@@ -747,6 +747,243 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_wreduce(const T *part, int n_
     gb[i] = tot;
 }
 
+// ---------------------------------------------------------------------------
+// per-residual Jacobian (natural-gradient Gram assembly)
+// ---------------------------------------------------------------------------
+// The reference forms J with one torch.autograd.grad per residual
+// (tedeous/optimizers/ngd.py:57-78).  One thread owns one (group, entry):
+// per equation it forms the seeds dr/dU(set, jet, var) of the residual
+// program with dr = 1 (the k_ad_resid seeds), and for every set with a
+// non-zero seed pushes that point's jets through the network and the jet
+// maps' adjoints back, adding seed * dU/dparams into its own column r of
+// jt[P][n_resid].  Per-thread state lives in LDS, thread-fastest:
+// pre-activation and activation jets of every hidden layer, two adjoint
+// buffers, the output jets of every set and the seeds.
+template <typename T>
+struct JacArgs {
+  int d_in, n_out, H, L, J, n_groups, groups_i, max_sets;
+  const int32_t *ints;
+  const T *reals, *data, *points, *params;
+  const int64_t *gstart;
+  Tables tb;
+  const int64_t *off_w, *off_b;   // parameter offsets per layer (device)
+  T *jt;
+  int64_t n_resid;
+};
+
+template <typename T>
+__global__ void k_ad_jac(JacArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  const int B = blockDim.x, t = threadIdx.x;
+  const int J = a.J, H = a.H, L = a.L, NL = L - 1, no = a.n_out, din = a.d_in;
+  const int JH = J * H;
+  // LDS per thread (element k at [k * B + t]): Z[NL][J][H] | A[NL][J][H] |
+  // D0[J][H] | D1[J][H] | U[max_sets][J][no] | S[max_sets][J][no] | X[J][din]
+  T *Zs = smem;
+  T *As = Zs + (size_t)NL * JH * B;
+  T *D0 = As + (size_t)NL * JH * B;
+  T *D1 = D0 + (size_t)JH * B;
+  T *Us = D1 + (size_t)JH * B;
+  T *Ss = Us + (size_t)a.max_sets * J * no * B;
+  T *Xs = Ss + (size_t)a.max_sets * J * no * B;
+#define LV(base, k) base[(size_t)(k) * B + t]
+  const int64_t ge = (int64_t)blockIdx.x * B + t;
+  const int64_t total = a.gstart[a.n_groups];
+  if (ge >= total) return;
+  int g = 0;
+  while (g + 1 < a.n_groups && a.gstart[g + 1] <= ge) ++g;
+  const int e = (int)(ge - a.gstart[g]);
+  const int32_t *G = a.ints + a.groups_i + TDX_AD_GROUP_INTS * g;
+  const int n_sets = G[1], set_i = G[2], n_eq = G[3], eq_i = G[4];
+  const Tables &tb = a.tb;
+  const T *W = a.params;
+
+  // jets of point pt through the network: Z, A of every hidden layer, and
+  // (with u_out) the output jets
+  auto forward = [&](int64_t pt, T *u_out) {
+    for (int al = 0; al < J; ++al) {
+      const int ord = tb.ints[tb.jet_i + 2 * al], ax = tb.ints[tb.jet_i + 2 * al + 1];
+      for (int j = 0; j < din; ++j)
+        LV(Xs, al * din + j) = al == 0 ? a.points[pt * din + j] : (ord == 1 && ax == j ? T(1) : T(0));
+    }
+    for (int l = 0; l < NL; ++l) {
+      const int dl = l == 0 ? din : H;
+      const T *Wl = W + a.off_w[l], *bl = W + a.off_b[l];
+      T *Z = Zs + (size_t)l * JH * B, *A = As + (size_t)l * JH * B;
+      const T *In = l == 0 ? Xs : As + (size_t)(l - 1) * JH * B;
+      for (int i = 0; i < H; ++i) {
+        for (int al = 0; al < J; ++al) {
+          T z = al == 0 ? bl[i] : T(0);
+          for (int j = 0; j < dl; ++j) z = fma(Wl[i * dl + j], LV(In, al * dl + j), z);
+          LV(Z, al * H + i) = z;
+        }
+        const TanhDerivs<T> sd = tanh_derivs(LV(Z, i));
+        LV(A, i) = sd.s0;
+        for (int al = 1; al < J; ++al) {
+          const int k0 = tb.ints[tb.pptr_i + al], k1 = tb.ints[tb.pptr_i + al + 1];
+          T h = T(0);
+          for (int k = k0; k < k1; ++k) {
+            const int32_t *pr = tb.ints + tb.part_i + TDX_AD_PART_INTS * k;
+            const int m = pr[0];
+            T prod = deriv_n(sd.s1, sd.s2, sd.s3, sd.s4, sd.s5, m);
+            for (int q = 0; q < m; ++q) prod = prod * LV(Z, pr[1 + q] * H + i);
+            h = (k == k0) ? prod : h + prod;
+          }
+          LV(A, al * H + i) = h;
+        }
+      }
+    }
+    if (u_out) {
+      const T *Wl = W + a.off_w[L - 1], *bl = W + a.off_b[L - 1];
+      const T *In = As + (size_t)(NL - 1) * JH * B;
+      for (int o = 0; o < no; ++o)
+        for (int al = 0; al < J; ++al) {
+          T u = al == 0 ? bl[o] : T(0);
+          for (int j = 0; j < H; ++j) u = fma(Wl[o * H + j], LV(In, al * H + j), u);
+          LV(u_out, al * no + o) = u;
+        }
+    }
+  };
+
+  for (int s = 0; s < n_sets; ++s) forward(a.ints[set_i + s] + e, Us + (size_t)s * J * no * B);
+
+  for (int q = 0; q < n_eq; ++q) {
+    const int32_t *E = a.ints + eq_i + TDX_AD_EQ_INTS * q;
+    const int n_terms = E[0], term_i = E[1];
+    if (E[5] < 0) continue;
+    T *col = a.jt + E[5] + e;
+    for (int k = 0; k < n_sets * J * no; ++k) LV(Ss, k) = T(0);
+    auto factor_value = [&](const int32_t *F) {
+      T fv = T(0);
+      for (int k = 0; k < F[0]; ++k) {
+        const int32_t *S = a.ints + F[1] + TDX_AD_STENCIL_INTS * k;
+        const T v = LV(Us, (S[0] * J + S[1]) * no + F[2]);
+        fv = (k == 0) ? a.reals[S[2]] * v : fv + a.reals[S[2]] * v;
+      }
+      return fv;
+    };
+    // dr/dU through every factor of every term
+    for (int tt = 0; tt < n_terms; ++tt) {
+      const int32_t *Tr = a.ints + term_i + TDX_AD_TERM_INTS * tt;
+      const int nf = Tr[0];
+      const T c = Tr[2] == 0 ? a.reals[Tr[3]] : a.data[Tr[3] + e];
+      T fv[AD_MAXF], pw[AD_MAXF];
+      for (int f = 0; f < AD_MAXF; ++f) {
+        fv[f] = T(0);
+        pw[f] = T(1);
+        if (f < nf) {
+          const int32_t *F = a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f;
+          fv[f] = factor_value(F);
+          pw[f] = fpow(F[3], F[4], a.reals, fv[f]);
+        }
+      }
+      for (int f = 0; f < nf; ++f) {
+        const int32_t *F = a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f;
+        T dv = c * fdpow(F[3], F[4], a.reals, fv[f]);
+        for (int f2 = 0; f2 < nf; ++f2)
+          if (f2 != f) dv = dv * pw[f2];
+        for (int k = 0; k < F[0]; ++k) {
+          const int32_t *S = a.ints + F[1] + TDX_AD_STENCIL_INTS * k;
+          LV(Ss, (S[0] * J + S[1]) * no + F[2]) += dv * a.reals[S[2]];
+        }
+      }
+    }
+    auto add = [&](int64_t pidx, T v) { col[pidx * a.n_resid] += v; };
+    for (int s = 0; s < n_sets; ++s) {
+      const T *Sd = Ss + (size_t)s * J * no * B;
+      bool any = false;
+      for (int k = 0; k < J * no; ++k) any |= LV(Sd, k) != T(0);
+      if (!any) continue;
+      forward(a.ints[set_i + s] + e, nullptr);
+      // output layer: dW[o][j] += sum_al dU[al][o] A[al][j]; dA = W^T dU
+      {
+        const T *Wl = W + a.off_w[L - 1];
+        const T *Al = As + (size_t)(NL - 1) * JH * B;
+        for (int o = 0; o < no; ++o) {
+          for (int j = 0; j < H; ++j) {
+            T g = T(0);
+            for (int al = 0; al < J; ++al) g = fma(LV(Sd, al * no + o), LV(Al, al * H + j), g);
+            add(a.off_w[L - 1] + o * H + j, g);
+          }
+          add(a.off_b[L - 1] + o, LV(Sd, o));
+        }
+        for (int al = 0; al < J; ++al)
+          for (int j = 0; j < H; ++j) {
+            T g = T(0);
+            for (int o = 0; o < no; ++o) g = fma(Wl[o * H + j], LV(Sd, al * no + o), g);
+            LV(D0, al * H + j) = g;
+          }
+      }
+      T *dA = D0, *dZ = D1;
+      for (int l = NL - 1; l >= 0; --l) {
+        const T *Z = Zs + (size_t)l * JH * B;
+        // adjoint of the tanh jet map (unit i): dZ from dA
+        for (int i = 0; i < H; ++i) {
+          const TanhDerivs<T> sd = tanh_derivs(LV(Z, i));
+          T dz0 = LV(dA, i) * sd.s1;
+          for (int al = 1; al < J; ++al) {
+            const T dd = LV(dA, al * H + i);
+            const int k0 = tb.ints[tb.pptr_i + al], k1 = tb.ints[tb.pptr_i + al + 1];
+            for (int k = k0; k < k1; ++k) {
+              const int32_t *pr = tb.ints + tb.part_i + TDX_AD_PART_INTS * k;
+              const int m = pr[0];
+              T all = deriv_n(sd.s1, sd.s2, sd.s3, sd.s4, sd.s5, m + 1);
+              for (int q2 = 0; q2 < m; ++q2) all = all * LV(Z, pr[1 + q2] * H + i);
+              dz0 = dz0 + dd * all;
+            }
+          }
+          LV(dZ, i) = dz0;
+          for (int b = 1; b < J; ++b) {
+            T acc = T(0);
+            const int t0 = tb.occ[tb.tptr_i + b], t1 = tb.occ[tb.tptr_i + b + 1];
+            for (int tt = t0; tt < t1; ++tt) {
+              const int32_t *oc = tb.occ + tb.tent_i + 3 * tt;
+              const int al = oc[0], qq = oc[2];
+              const int32_t *pr = tb.ints + tb.part_i + TDX_AD_PART_INTS * oc[1];
+              const int m = pr[0];
+              T v = LV(dA, al * H + i) * deriv_n(sd.s1, sd.s2, sd.s3, sd.s4, sd.s5, m);
+              for (int q2 = 0; q2 < m; ++q2)
+                if (q2 != qq) v = v * LV(Z, pr[1 + q2] * H + i);
+              acc = acc + v;
+            }
+            LV(dZ, b * H + i) = acc;
+          }
+        }
+        // linear layer l: dW[i][j] += sum_al dZ[al][i] In[al][j]; db += dZ[0]
+        const int dl = l == 0 ? din : H;
+        const T *In = l == 0 ? Xs : As + (size_t)(l - 1) * JH * B;
+        const T *Wl = W + a.off_w[l];
+        for (int i = 0; i < H; ++i) {
+          for (int j = 0; j < dl; ++j) {
+            T g = T(0);
+            for (int al = 0; al < J; ++al) g = fma(LV(dZ, al * H + i), LV(In, al * dl + j), g);
+            add(a.off_w[l] + i * dl + j, g);
+          }
+          add(a.off_b[l] + i, LV(dZ, i));
+        }
+        if (l > 0) {
+          // dA of the layer below: W^T dZ
+          for (int al = 0; al < J; ++al)
+            for (int j = 0; j < H; ++j) {
+              T g = T(0);
+              for (int i = 0; i < H; ++i) g = fma(Wl[i * H + j], LV(dZ, al * H + i), g);
+              LV(dA, al * H + j) = g;
+            }
+        }
+      }
+    }
+  }
+#undef LV
+}
+
+template <typename T>
+__global__ void k_ad_jac_zero(T *jt, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    jt[i] = T(0);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -768,6 +1005,8 @@ struct tdx_ad_plan {
   size_t off_wR = 0, off_wM = 0;
   DevTable ints, reals, data, points, gstart, occ;
   std::vector<int64_t> off_w, off_b;
+  DevTable offs;     // off_w | off_b on the device (the Jacobian kernel)
+  int max_sets = 0;  // most point sets of a group
   std::vector<int> din, dout;
   // launch shapes per layer
   std::vector<int> f_pb, f_rpad, b_pb, b_rpad;  // points per workgroup, rows rounded to 16
@@ -857,6 +1096,7 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
   if (!in_ints(d->groups_i, (int64_t)TDX_AD_GROUP_INTS * d->n_groups)) return fail("group table");
   std::vector<int64_t> gstart(d->n_groups + 1, 0);
   std::vector<char> owned(d->n_points, 0);
+  int max_sets = 0;
   int64_t n_resid_used = 0;
   for (int g = 0; g < d->n_groups; ++g) {
     const int32_t *G = I + d->groups_i + TDX_AD_GROUP_INTS * g;
@@ -901,6 +1141,7 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
       }
     }
     gstart[g + 1] = gstart[g] + ne;
+    max_sets = std::max(max_sets, ns);
   }
   if (gstart[d->n_groups] < 1) return fail("no entries");
   if (d->weak_levels != 0) {
@@ -942,6 +1183,7 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
   p->P = d->n_points;
   p->n_entries = gstart[d->n_groups];
   p->n_resid = d->n_resid;
+  p->max_sets = max_sets;
   p->n_groups = d->n_groups;
   p->groups_i = d->groups_i;
   p->jet_i = d->jet_i;
@@ -1063,6 +1305,11 @@ extern "C" int tdx_ad_plan_create(const tdx_ad_desc *d, tdx_ad_plan **out) {
   if (rc == TDX_OK) rc = p->points.upload(d->points, (size_t)d->n_points * d->d_in * ts);
   if (rc == TDX_OK) rc = p->gstart.upload(gstart.data(), gstart.size() * 8);
   if (rc == TDX_OK) rc = p->occ.upload(occ.data(), occ.size() * 4);
+  if (rc == TDX_OK) {
+    std::vector<int64_t> offs(p->off_w);
+    offs.insert(offs.end(), p->off_b.begin(), p->off_b.end());
+    rc = p->offs.upload(offs.data(), offs.size() * 8);
+  }
   if (rc != TDX_OK) {
     tdx_ad_plan_destroy(p);
     return rc;
@@ -1079,6 +1326,7 @@ extern "C" void tdx_ad_plan_destroy(tdx_ad_plan *p) {
   p->points.release();
   p->gstart.release();
   p->occ.release();
+  p->offs.release();
   delete p;
 }
 
@@ -1315,6 +1563,58 @@ extern "C" int tdx_ad_backward(const tdx_ad_plan *p, const void *params, const v
     return ad_backward<float>(p, (const float *)params, (const float *)dloss, (float *)dparams, ws, s);
   return ad_backward<double>(p, (const double *)params, (const double *)dloss, (double *)dparams,
                              ws, s);
+}
+
+namespace {
+
+template <typename T>
+int ad_jacobian(const tdx_ad_plan *p, const T *params, T *jt, hipStream_t s) {
+  const int NL = p->L - 1, J = p->J, H = p->H, no = p->n_out;
+  const size_t words = (size_t)(2 * NL + 2) * J * H + (size_t)2 * p->max_sets * J * no +
+                       (size_t)J * p->d_in;
+  int B = 64;
+  while (B > 1 && words * sizeof(T) * B > (160 * 1024)) B >>= 1;
+  TDX_REQUIRE(words * sizeof(T) * B <= (160 * 1024), "network too large for the Jacobian kernel");
+  raise_lds_limit({(const void *)k_ad_jac<T>});
+  const int64_t n = p->n_params * p->n_resid;
+  hipLaunchKernelGGL((k_ad_jac_zero<T>), dim3((unsigned)std::min<int64_t>((n + 255) / 256, 65536)),
+                     dim3(256), 0, s, jt, n);
+  TDX_TRY_HIP(hipGetLastError());
+  JacArgs<T> a;
+  a.d_in = p->d_in;
+  a.n_out = no;
+  a.H = H;
+  a.L = p->L;
+  a.J = J;
+  a.n_groups = p->n_groups;
+  a.groups_i = p->groups_i;
+  a.max_sets = p->max_sets;
+  a.ints = (const int32_t *)p->ints.ptr;
+  a.reals = (const T *)p->reals.ptr;
+  a.data = (const T *)p->data.ptr;
+  a.points = (const T *)p->points.ptr;
+  a.params = params;
+  a.gstart = (const int64_t *)p->gstart.ptr;
+  a.tb = tables_of<T>(p);
+  a.off_w = (const int64_t *)p->offs.ptr;
+  a.off_b = a.off_w + p->L;
+  a.jt = jt;
+  a.n_resid = p->n_resid;
+  const unsigned blocks = (unsigned)((p->n_entries + B - 1) / B);
+  hipLaunchKernelGGL((k_ad_jac<T>), dim3(blocks), dim3(B), words * sizeof(T) * B, s, a);
+  TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}
+
+}  // namespace
+
+extern "C" int tdx_ad_jacobian(const tdx_ad_plan *p, const void *params, void *jt,
+                               tdx_stream_t stream) {
+  TDX_REQUIRE(p && params && jt, "null argument");
+  TDX_REQUIRE(!p->weak, "the Jacobian of a weak-form plan is not served");
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == 0) return ad_jacobian<float>(p, (const float *)params, (float *)jt, s);
+  return ad_jacobian<double>(p, (const double *)params, (double *)jt, s);
 }
 
 extern "C" int tdx_ad_train_step(const tdx_ad_plan *p, void *params, void *m, void *v,

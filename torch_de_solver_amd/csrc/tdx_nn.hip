@@ -77,6 +77,13 @@ struct NNArgs {
   // evaluating the network (null: evaluate)
   T *u_out;
   const T *u_in;
+  // persistent launches: work item counts, the largest slot count times
+  // n_out (LDS stride of the forward kernel), and the fused step's unit
+  // seed scale and Adam step counter (null: not touched)
+  int n_k1, n_k2, so_max;
+  const int32_t *k1_bounds;   // persistent forward: item range of each workgroup
+  T *one_out;
+  int64_t *step_inc;
 };
 
 template <typename T>
@@ -125,20 +132,16 @@ __device__ __forceinline__ int feature_at(int p) {
   return (p & ~15) + crow<T>((p & 15) >> 2, p & 3);
 }
 
+// Word `off` of an instance's packed image, from its parameters W (torch
+// layout).
 template <typename T, int NB>
-__global__ void k_pack(NNArgs<T> a) {
+__device__ __forceinline__ T image_value(const NNArgs<T> &a, const T *W, int off) {
   constexpr int HP = 16 * NB;
   const int NH = a.NH, H = a.H, d = a.d_in, no = a.n_out;
   const int img = HP * HP;
   const int SW = small_words(HP, NH, d, no);
-  const int IMG = image_words(HP, NH, d, no);
-  const int64_t total = (int64_t)a.n_inst * IMG;
   const Layout<T> L{d, no, H, NH};
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int inst = (int)(t / IMG);
-    int off = (int)(t - (int64_t)inst * IMG);
-    const T *W = a.params + (int64_t)inst * a.P;
+  {
     T v = T(0);
     if (off < NH * img || off >= NH * img + SW) {
       const int which = off < NH * img ? 0 : 1;
@@ -174,7 +177,27 @@ __global__ void k_pack(NNArgs<T> a) {
         if (o < no) v = W[L.obL() + o];
       }
     }
-    a.packed[t] = v;
+    return v;
+  }
+}
+
+// Packs every instance's image; in a fused step it also sets the adjoint's
+// unit seed scale and advances the Adam step counter (a.one_out,
+// a.step_inc), which saves two launches.
+template <typename T, int NB>
+__global__ void k_pack(NNArgs<T> a) {
+  constexpr int HP = 16 * NB;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (a.one_out) *a.one_out = T(1);
+    if (a.step_inc) *a.step_inc += 1;
+  }
+  const int IMG = image_words(HP, a.NH, a.d_in, a.n_out);
+  const int64_t total = (int64_t)a.n_inst * IMG;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int inst = (int)(t / IMG);
+    const int off = (int)(t - (int64_t)inst * IMG);
+    a.packed[t] = image_value<T, NB>(a, a.params + (int64_t)inst * a.P, off);
   }
 }
 
@@ -224,6 +247,18 @@ __device__ Net<T, NB> stage_net(const NNArgs<T> &a, int inst, T *dst, bool with_
   for (int k = threadIdx.x; k < n4; k += blockDim.x) d4[k] = s4[k];
   __syncthreads();
   return net_at<T, NB>(a, dst);
+}
+
+// The same LDS copy as stage_net, gathered from the instance's parameters
+// (no packed image in global memory needed).  Ends with a workgroup
+// barrier.
+template <typename T, int NB>
+__device__ void gather_image(const NNArgs<T> &a, int inst, T *dst, bool with_wb) {
+  constexpr int HP = 16 * NB;
+  const T *W = a.params + (int64_t)inst * a.P;
+  const int n = stage_words(HP, a.NH, a.d_in, a.n_out, with_wb);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) dst[k] = image_value<T, NB>(a, W, k);
+  __syncthreads();
 }
 
 // Four consecutive values of a small array at accumulator position p.
@@ -330,13 +365,22 @@ __device__ __forceinline__ T dfpow(const NNArgs<T> &a, const int32_t *F, T x) {
 // ---------------------------------------------------------------------------
 // k_forward
 // ---------------------------------------------------------------------------
+// LDS regions of the forward kernel: network values and seeds of the 64
+// entries (stride so_max words per entry), then 512 words of reduction
+// space, then the staged network image.
+template <typename T>
+__device__ __forceinline__ T *forward_stage_area(const NNArgs<T> &a, T *smem) {
+  return smem + 2 * K1_WAVES * 16 * a.so_max + 512;
+}
+
+// One forward work item (64 entries of one group): network at every slot,
+// residual program, loss partials (lp[2 it], lp[2 it + 1]) and seeds.
 template <typename T, int NB, bool STAGE>
-__global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T *smem = reinterpret_cast<T *>(smem_raw);
-  const int2 item = a.k1_items[blockIdx.x];
+__device__ __forceinline__ void forward_item(const NNArgs<T> &a, int it, const Net<T, NB> &net,
+                                             T *smem) {
+  const int2 item = a.k1_items[it];
   const int32_t *G = a.groups + item.x * TDX_GROUP_INTS;
-  const int inst = G[0], n_entries = G[1], n_slots = G[2];
+  const int n_entries = G[1], n_slots = G[2];
   const int slot_base_i = G[3], slot_off_r = G[4], n_eq = G[5], eq_i = G[6];
   const int64_t row_base = G[7];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -346,8 +390,8 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
   const int ec = valid ? e : n_entries - 1;
   const int SO = n_slots * a.n_out;
   T *u_lds = smem + (w * 16 + row) * SO;
-  T *sd_lds = smem + K1_WAVES * 16 * SO + (w * 16 + row) * SO;
-  T *red = smem + 2 * K1_WAVES * 16 * SO;
+  T *sd_lds = smem + K1_WAVES * 16 * a.so_max + (w * 16 + row) * SO;
+  T *red = smem + 2 * K1_WAVES * 16 * a.so_max;
 
   if (a.u_in) {
     // network values given: the residual program runs on them
@@ -356,11 +400,6 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
         for (int o = 0; o < a.n_out; ++o)
           u_lds[s * a.n_out + o] = a.u_in[(row_base + (int64_t)s * n_entries + ec) * a.n_out + o];
   } else {
-  Net<T, NB> net;
-  if constexpr (STAGE)
-    net = stage_net<T, NB>(a, inst, red + 512, false);
-  else
-    net = make_net<T, NB>(a, inst);
   T act[NB][4];
   for (int s = 0; s < n_slots; ++s) {
     T x[MAXD];
@@ -382,10 +421,14 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
   }
   __syncthreads();
 
-  // residual, loss and seeds: every wavefront its own 16 entries, one per
-  // lane of quarter 0, so the four SIMDs share the interpretation
+  // residual, loss and seeds of the workgroup's 64 entries: wavefront 0,
+  // one entry per lane (the other wavefronts' SIMDs are free meanwhile)
   T lsum = T(0), lnsum = T(0);
-  if (q == 0) {
+  if (w == 0) {
+    const int e = item.y + lane;
+    const bool valid = e < n_entries;
+    T *u_lds = smem + lane * SO;
+    T *sd_lds = smem + K1_WAVES * 16 * a.so_max + lane * SO;
     for (int k = 0; k < SO; ++k) sd_lds[k] = T(0);
     if (valid) {
       for (int eq = 0; eq < n_eq; ++eq) {
@@ -479,8 +522,60 @@ __global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    a.lp[2 * blockIdx.x] = red[0];
-    a.lp[2 * blockIdx.x + 1] = red[256];
+    a.lp[2 * it] = red[0];
+    a.lp[2 * it + 1] = red[256];
+  }
+}
+
+// k_forward: one work item per workgroup, or (PERSIST) a persistent grid
+// whose workgroup g takes the contiguous items [k1_bounds[g],
+// k1_bounds[g + 1]) (ranges of equal cost, slots x entries, set at plan
+// creation),
+// gathering its instance's network into LDS from the parameters when the
+// instance changes (no per-item staging).  The workgroup that holds an
+// instance's first item also writes that instance's packed image for the
+// backward kernel, so no separate packing launch is needed.  The persistent
+// launch also sets the adjoint's unit seed scale and advances the Adam step
+// counter of a fused step (a.one_out, a.step_inc).
+template <typename T, int NB, bool STAGE, bool PERSIST>
+__global__ __launch_bounds__(256) void k_forward(NNArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  if constexpr (PERSIST) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (a.one_out) *a.one_out = T(1);
+      if (a.step_inc) *a.step_inc += 1;
+    }
+    const int lo = a.k1_bounds[blockIdx.x], hi = a.k1_bounds[blockIdx.x + 1];
+    int cur = -1;
+    T *stage = forward_stage_area(a, smem);
+    const Net<T, NB> net = net_at<T, NB>(a, stage);
+    for (int it = lo; it < hi; ++it) {
+      const int inst = a.groups[a.k1_items[it].x * TDX_GROUP_INTS];
+      __syncthreads();
+      if (!a.u_in && inst != cur) {
+        gather_image<T, NB>(a, inst, stage, false);
+        cur = inst;
+        if (it == a.inst_k1_idx[a.inst_k1_ptr[inst]]) {
+          constexpr int HP = 16 * NB;
+          const int IMG = image_words(HP, a.NH, a.d_in, a.n_out);
+          const T *W = a.params + (int64_t)inst * a.P;
+          T *dst = a.packed + (int64_t)inst * IMG;
+          for (int k = threadIdx.x; k < IMG; k += blockDim.x) dst[k] = image_value<T, NB>(a, W, k);
+        }
+      }
+      forward_item<T, NB, STAGE>(a, it, net, smem);
+    }
+  } else {
+    Net<T, NB> net;
+    if (!a.u_in) {
+      const int inst = a.groups[a.k1_items[blockIdx.x].x * TDX_GROUP_INTS];
+      if constexpr (STAGE)
+        net = stage_net<T, NB>(a, inst, forward_stage_area(a, smem), false);
+      else
+        net = make_net<T, NB>(a, inst);
+    }
+    forward_item<T, NB, STAGE>(a, blockIdx.x, net, smem);
   }
 }
 
@@ -722,15 +817,23 @@ constexpr int backward_w_min_waves(int bytes, int NB, int NH, int NT) {
   return (bytes == 4 && NB == 2 && NH <= 2 && NT == 1) ? 3 : 1;
 }
 
+// Words of the wavefront buffers (then the final sum) of k_backward_w; the
+// staged network image follows them.
+template <int NB, int NT>
+__device__ __forceinline__ int backward_w_buffer_words(int NW, int P) {
+  constexpr int FB = WRS * 16 * NB;
+  return max(NW * NT * (2 * FB + 2 * 16 * WRS), P);
+}
+
+// One backward work item (a range of rows of one group) of the
+// wavefront-private backward: its gradient partial goes to partial[it].
 template <typename T, int NB, int NH, int NT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
-    backward_w_min_waves(sizeof(T), NB, NH, NT)))) void k_backward_w(NNArgs<T> a) {
+__device__ __forceinline__ void backward_w_item(const NNArgs<T> &a, int it,
+                                                const Net<T, NB> &net, T *smem) {
   constexpr int HP = 16 * NB;
   constexpr int FB = WRS * HP;  // words of one feature-major buffer
   typedef typename V4<T>::type v4;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T *smem = reinterpret_cast<T *>(smem_raw);
-  const int4 item = a.k2_items[blockIdx.x];
+  const int4 item = a.k2_items[it];
   const int32_t *G = a.groups + item.x * TDX_GROUP_INTS;
   const int n_entries = G[1];
   const int slot_base_i = G[3], slot_off_r = G[4];
@@ -750,10 +853,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     sbuf[t] = abuf[t] + FB;          // seeds, one column per output
     xbuf[t] = sbuf[t] + 16 * WRS;    // coordinates, one column per input
   }
-  const int TRW = max(NW * NT * WB, P);  // buffers, then the final sum
-  T *stage = smem + TRW;
-
-  const Net<T, NB> net = stage_net<T, NB>(a, item.w, stage, true);
   const Layout<T> &L = net.L;
   const T dl_scale = *a.dloss;
 
@@ -1066,8 +1165,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     }
     __syncthreads();
   }
-  T *out = a.partial + (int64_t)blockIdx.x * P;
+  T *out = a.partial + (int64_t)it * P;
   for (int k = threadIdx.x; k < P; k += blockDim.x) out[k] = cp[k];
+}
+
+// k_backward_w: one work item per workgroup.
+template <typename T, int NB, int NH, int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    backward_w_min_waves(sizeof(T), NB, NH, NT)))) void k_backward_w(NNArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  T *stage = smem + backward_w_buffer_words<NB, NT>(blockDim.x >> 6, a.P);
+  const Net<T, NB> net = stage_net<T, NB>(a, a.k2_items[blockIdx.x].w, stage, true);
+  backward_w_item<T, NB, NH, NT>(a, blockIdx.x, net, smem);
 }
 
 // Block-owning backward for wide float networks (HP = 16 NB up to 128).
@@ -1479,10 +1589,11 @@ __global__ void k_causal_loss(const T *wr, int N, T *loss) {
   }
 }
 
+// Loss pair of instance i: fixed-order sum of its forward items' partials
+// (a workgroup of 256 threads).
 template <typename T>
-__global__ __launch_bounds__(256) void k_loss(NNArgs<T> a, T *loss) {
+__device__ __forceinline__ void loss_of_instance(const NNArgs<T> &a, int i, T *loss) {
   __shared__ T red[512];
-  const int i = blockIdx.x;
   T s0 = T(0), s1 = T(0);
   for (int k = a.inst_k1_ptr[i] + threadIdx.x; k < a.inst_k1_ptr[i + 1]; k += blockDim.x) {
     const int wg = a.inst_k1_idx[k];
@@ -1506,17 +1617,30 @@ __global__ __launch_bounds__(256) void k_loss(NNArgs<T> a, T *loss) {
 }
 
 template <typename T>
+__global__ __launch_bounds__(256) void k_loss(NNArgs<T> a, T *loss) {
+  loss_of_instance(a, blockIdx.x, loss);
+}
+
+template <typename T>
 __device__ __forceinline__ void adam_update(T &p, T g, T &m, T &v, int64_t t,
                                             const AdamHyper &h) {
   adam_apply(p, g, m, v, adam_scalars<T>(t, h));
 }
 
 // dparams (mode 0) or the Adam update (mode 1) from the per-item partials
+// Gradient sums and the Adam update.  The first loss_blocks workgroups
+// instead form the loss pairs of the instances (a fused step's k_loss);
+// with step_done the step counter already holds this step's number.
 template <typename T>
-__global__ void k_reduce(NNArgs<T> a, int mode, T *dparams, T *params, T *m,
-                         T *v, const int64_t *step, AdamHyper h) {
+__global__ __launch_bounds__(256) void k_reduce(NNArgs<T> a, int mode, T *dparams, T *params,
+                                                T *m, T *v, const int64_t *step, AdamHyper h,
+                                                T *loss, int loss_blocks, int step_done) {
+  if ((int)blockIdx.x < loss_blocks) {
+    loss_of_instance(a, blockIdx.x, loss);
+    return;
+  }
   const int64_t tot = (int64_t)a.n_inst * a.P;
-  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t idx = (blockIdx.x - loss_blocks) * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= tot) return;
   const int inst = (int)(idx / a.P);
   const int p = (int)(idx - (int64_t)inst * a.P);
@@ -1536,7 +1660,7 @@ __global__ void k_reduce(NNArgs<T> a, int mode, T *dparams, T *params, T *m,
     dparams[idx] = g;
   } else {
     T pv = params[idx], mv = m[idx], vv = v[idx];
-    adam_update(pv, g, mv, vv, *step + 1, h);
+    adam_update(pv, g, mv, vv, step_done ? *step : *step + 1, h);
     params[idx] = pv;
     m[idx] = mv;
     v[idx] = vv;
@@ -1776,6 +1900,12 @@ struct tdx_nn_plan {
   int n_groups;
   int max_slots;      // largest slot count of a group
   int64_t n_resid;    // length of the residual output
+  // persistent forward: a grid of CUs x resident workgroups (sized at the
+  // first launch) that gathers the networks into LDS from the parameters
+  // and writes the packed images; a fused step is then three launches
+  int persist = 0;
+  int g1 = 0;           // workgroups of the persistent forward
+  DevTable k1_bounds;   // their item ranges (g1 + 1 ints)
   int64_t n_rows;
   int n_k1, n_k2;
   int k2_waves;
@@ -1870,7 +2000,42 @@ NNArgs<T> make_args(const tdx_nn_plan *p, const void *params, void *ws) {
   a.seed_mult_n = 0;
   a.u_out = nullptr;
   a.u_in = nullptr;
+  a.n_k1 = p->n_k1;
+  a.n_k2 = p->n_k2;
+  a.so_max = p->max_slots * p->n_out;
+  a.k1_bounds = (const int32_t *)p->k1_bounds.ptr;
+  a.one_out = nullptr;
+  a.step_inc = nullptr;
   return a;
+}
+
+// Resident workgroups of the persistent forward kernel on the current
+// device (CUs x workgroups per CU at the plan's LDS size), 0 if unknown.
+template <typename T, int NB>
+int forward_grid_size(size_t lds) {
+  int dev = 0, cus = 0, per_cu = 0;
+  const void *k = (const void *)k_forward<T, NB, true, true>;
+  raise_lds_limit({k});
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, lds) != hipSuccess)
+    return 0;
+  return cus > 0 && per_cu > 0 ? cus * per_cu : 0;
+}
+
+template <typename T>
+int forward_grid_size_nb(int NB, size_t lds) {
+  switch (NB) {
+    case 1: return forward_grid_size<T, 1>(lds);
+    case 2: return forward_grid_size<T, 2>(lds);
+    case 3: return forward_grid_size<T, 3>(lds);
+    case 4: return forward_grid_size<T, 4>(lds);
+    case 5: return forward_grid_size<T, 5>(lds);
+    case 6: return forward_grid_size<T, 6>(lds);
+    case 7: return forward_grid_size<T, 7>(lds);
+    case 8: return forward_grid_size<T, 8>(lds);
+    default: return 0;
+  }
 }
 
 template <typename T, int NB>
@@ -1883,10 +2048,30 @@ int launch_pack(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
 }
 
 template <typename T, int NB>
+int launch_forward_kernel(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
+  ProfScope ps(p, 0, s);
+  if (p->persist) {
+    raise_lds_limit({(const void *)k_forward<T, NB, true, true>});
+    hipLaunchKernelGGL((k_forward<T, NB, true, true>), dim3(p->g1), dim3(256), p->k1_lds, s, a);
+  } else if (p->k1_stage) {
+    hipLaunchKernelGGL((k_forward<T, NB, true, false>), dim3(p->n_k1), dim3(256), p->k1_lds, s,
+                       a);
+  } else {
+    hipLaunchKernelGGL((k_forward<T, NB, false, false>), dim3(p->n_k1), dim3(256), p->k1_lds, s,
+                       a);
+  }
+  TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}
+
+// Forward pass: network image (the persistent forward writes it itself;
+// given network values need none), forward kernel, loss pairs (unless
+// fused_loss: the reduction kernel forms them), causal and weak-form
+// passes.
+template <typename T, int NB>
 int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStream_t s,
-                   void *ws) {
-  // given network values need no network image
-  if (!a.u_in) {
+                   void *ws, bool fused_loss = false) {
+  if (!a.u_in && !p->persist) {
     int rc = launch_pack<T, NB>(p, a, s);
     if (rc) return rc;
   }
@@ -1894,14 +2079,10 @@ int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStre
   if ((p->causal || p->weak) && !resid) resid = (T *)((char *)ws + p->off_resid);
   a.resid = resid;
   {
-    ProfScope ps(p, 0, s);
-    if (p->k1_stage)
-      hipLaunchKernelGGL((k_forward<T, NB, true>), dim3(p->n_k1), dim3(256), p->k1_lds, s, a);
-    else
-      hipLaunchKernelGGL((k_forward<T, NB, false>), dim3(p->n_k1), dim3(256), p->k1_lds, s, a);
+    int rc = launch_forward_kernel<T, NB>(p, a, s);
+    if (rc) return rc;
   }
-  TDX_TRY_HIP(hipGetLastError());
-  if (loss) {
+  if (loss && !fused_loss) {
     hipLaunchKernelGGL((k_loss<T>), dim3(p->n_inst), dim3(256), 0, s, a, loss);
     TDX_TRY_HIP(hipGetLastError());
   }
@@ -1936,11 +2117,10 @@ int launch_forward(const tdx_nn_plan *p, NNArgs<T> a, T *loss, T *resid, hipStre
       NNArgs<T> a2 = a;
       a2.seed_mult = m;
       a2.seed_mult_n = p->w_N;
-      if (p->k1_stage)
-        hipLaunchKernelGGL((k_forward<T, NB, true>), dim3(p->n_k1), dim3(256), p->k1_lds, s, a2);
-      else
-        hipLaunchKernelGGL((k_forward<T, NB, false>), dim3(p->n_k1), dim3(256), p->k1_lds, s, a2);
-      TDX_TRY_HIP(hipGetLastError());
+      a2.one_out = nullptr;
+      a2.step_inc = nullptr;
+      int rc = launch_forward_kernel<T, NB>(p, a2, s);
+      if (rc) return rc;
     } else {
       hipLaunchKernelGGL((k_weak_mult<T>), dim3(p->w_G), dim3(256), 0, s, p->weak, gs, c, x,
                          (const T *)R, m);
@@ -2037,11 +2217,13 @@ int launch_backward(const tdx_nn_plan *p, NNArgs<T> a, hipStream_t s) {
 
 template <typename T>
 int launch_reduce(const tdx_nn_plan *p, const NNArgs<T> &a, int mode, T *dparams, T *params,
-                  T *m, T *v, const int64_t *step, AdamHyper h, hipStream_t s) {
+                  T *m, T *v, const int64_t *step, AdamHyper h, hipStream_t s,
+                  T *loss = nullptr, int step_done = 0) {
   const int64_t tot = (int64_t)p->n_inst * p->P;
+  const int loss_blocks = loss ? p->n_inst : 0;
   ProfScope ps(p, 2, s);
-  hipLaunchKernelGGL((k_reduce<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, a,
-                     mode, dparams, params, m, v, step, h);
+  hipLaunchKernelGGL((k_reduce<T>), dim3((unsigned)((tot + 255) / 256) + loss_blocks), dim3(256),
+                     0, s, a, mode, dparams, params, m, v, step, h, loss, loss_blocks, step_done);
   TDX_TRY_HIP(hipGetLastError());
   return TDX_OK;
 }
@@ -2071,6 +2253,33 @@ int nn_dispatch(const tdx_nn_plan *p, int op, const void *params, void *params_m
       if (!rc) rc = launch_backward<T, NBV>(p, a, s);                             \
       if (!rc) rc = launch_reduce<T>(p, a, 0, (T *)dparams, nullptr, nullptr,     \
                                      nullptr, nullptr, h, s);                     \
+    } else if (p->persist) {                                                      \
+      /* three launches: the forward also sets the unit seed scale and */        \
+      /* advances the step; the reduction also forms the loss pairs */           \
+      a.one_out = (T *)((char *)ws + p->off_one);                                 \
+      a.step_inc = step;                                                          \
+      rc = launch_forward<T, NBV>(p, a, (T *)loss, nullptr, s, ws, true);         \
+      a.one_out = nullptr;                                                        \
+      a.step_inc = nullptr;                                                       \
+      if (!rc) rc = launch_backward<T, NBV>(p, a, s);                             \
+      if (!rc) rc = launch_reduce<T>(p, a, 1, nullptr, (T *)params_mut, (T *)m,  \
+                                     (T *)v, step, h, s, (T *)loss, 1);           \
+    } else if (!p->causal && !p->weak) {                                          \
+      /* four launches: the packing kernel also sets the unit seed scale */      \
+      /* and advances the step; the reduction also forms the loss pairs */       \
+      a.one_out = (T *)((char *)ws + p->off_one);                                 \
+      a.step_inc = step;                                                          \
+      rc = launch_pack<T, NBV>(p, a, s);                                          \
+      a.one_out = nullptr;                                                        \
+      a.step_inc = nullptr;                                                       \
+      if (!rc) {                                                                  \
+        NNArgs<T> af = a;                                                         \
+        af.u_in = nullptr;                                                        \
+        rc = launch_forward_kernel<T, NBV>(p, af, s);                             \
+      }                                                                           \
+      if (!rc) rc = launch_backward<T, NBV>(p, a, s);                             \
+      if (!rc) rc = launch_reduce<T>(p, a, 1, nullptr, (T *)params_mut, (T *)m,  \
+                                     (T *)v, step, h, s, (T *)loss, 1);           \
     } else {                                                                      \
       rc = launch_forward<T, NBV>(p, a, (T *)loss, nullptr, s, ws);               \
       if (!rc) {                                                                  \
@@ -2278,6 +2487,8 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
     p->acc_lds = 1;
     p->k2_lds = block_lds_words(p->HP, p->NH) * ts;
   }
+  p->persist = (d->persistent > 0 && p->k1_stage && d->causal_tol == 0.0 &&
+                d->weak_levels == 0) ? 1 : 0;
   const int TR = 16 * p->k2_waves;
   const int chunk_tiles = d->chunk_tiles > 0 ? d->chunk_tiles
                         : (p->k2_wave_private ? 256 : (p->k2_block ? 8 : (p->acc_lds ? 16 : 4)));
@@ -2314,6 +2525,34 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   if (!r2) r2 = p->ik1i.upload(ik1i.data(), ik1i.size() * 4);
   if (!r2) r2 = p->ik2p.upload(ik2p.data(), ik2p.size() * 4);
   if (!r2) r2 = p->ik2i.upload(ik2i.data(), ik2i.size() * 4);
+  if (!r2 && p->persist) {
+    // persistent forward: as many workgroups as the device keeps resident,
+    // each taking a contiguous range of work items of about equal cost
+    // (slots + 1 network evaluations per entry: the +1 stands for the
+    // residual program and the reductions)
+    const int g = p->dtype == 0 ? forward_grid_size_nb<float>(p->NB, p->k1_lds)
+                                : forward_grid_size_nb<double>(p->NB, p->k1_lds);
+    if (g <= 0) {
+      p->persist = 0;
+    } else {
+      p->g1 = std::min(g, p->n_k1);
+      std::vector<double> prefix(k1.size() + 1, 0.0);
+      for (size_t i = 0; i < k1.size(); ++i) {
+        const int32_t *G = d->groups + (int64_t)k1[i].x * TDX_GROUP_INTS;
+        const int ent = std::min(16 * K1_WAVES, G[1] - k1[i].y);
+        prefix[i + 1] = prefix[i] + (double)(G[2] + 1) * ent;
+      }
+      std::vector<int32_t> bounds(p->g1 + 1, 0);
+      size_t i = 0;
+      for (int w = 1; w < p->g1; ++w) {
+        const double target = prefix.back() * w / p->g1;
+        while (i < k1.size() && prefix[i] < target) ++i;
+        bounds[w] = (int32_t)i;
+      }
+      bounds[p->g1] = (int32_t)k1.size();
+      r2 = p->k1_bounds.upload(bounds.data(), bounds.size() * 4);
+    }
+  }
   if (r2) {
     tdx_nn_plan_destroy(p);
     return r2;
@@ -2404,7 +2643,7 @@ extern "C" void tdx_nn_plan_destroy(tdx_nn_plan *p) {
   if (!p) return;
   (void)tdx_nn_profile(p, 0);
   DevTable *t[] = {&p->groups, &p->ints, &p->reals, &p->data, &p->k1_items,
-                   &p->k2_items, &p->ik1p, &p->ik1i, &p->ik2p, &p->ik2i};
+                   &p->k2_items, &p->ik1p, &p->ik1i, &p->ik2p, &p->ik2i, &p->k1_bounds};
   for (DevTable *x : t) x->release();
   delete p;
 }

@@ -63,6 +63,7 @@ class NNDesc(ctypes.Structure):
         ('causal_tol', ctypes.c_double), ('causal_nt', ctypes.c_int),
         ('weak_levels', ctypes.c_int), ('weak_runs', ctypes.c_int),
         ('weak_d', ctypes.c_int64), ('weak_i', ctypes.c_int),
+        ('persistent', ctypes.c_int),
     ]
 
 

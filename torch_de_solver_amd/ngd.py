@@ -9,7 +9,7 @@ of tedeous/optimizers/closure.py:98-117) takes one step as:
    (flattened ``[max_len, n_types]``), J built with one
    ``torch.autograd.grad`` per residual entry (ngd.py:57-78);
 3. the Marquardt term ``min(loss, 0) * I`` (zero for a non-negative loss);
-4. the natural gradient ``f`` as the SVD least-squares solution of
+4. the natural gradient ``f`` as the pseudo-inverse solution of
    ``G f = grad`` (singular values above the dtype's machine epsilon
    inverted, ngd.py:104-126);
 5. a grid line search: the loss at ``params - 0.5**k * f`` for
@@ -17,10 +17,11 @@ of tedeous/optimizers/closure.py:98-117) takes one step as:
 
 Here J comes from one kernel (``tdx_nn_jacobian``: every residual's
 seed-weighted network adjoint, one thread per entry), the Gram products are
-two library GEMMs on the device, and the line search evaluates all candidate
-parameter vectors in one launch of a grouped plan that holds the problem
-once per candidate.  The SVD is torch's (the reference calls the same
-routine).  The step stays on the device: no host synchronisation.
+two library GEMMs on the device, the pseudo-inverse comes from the device's
+symmetric eigensolver (G is symmetric; see ``lstsq``), and the line search
+evaluates all candidate parameter vectors in one launch of a grouped plan
+that holds the problem once per candidate.  The step stays on the device:
+no host synchronisation.
 """
 
 from typing import Optional
@@ -46,15 +47,22 @@ class NGD(torch.optim.Optimizer):
     # -- pieces ----------------------------------------------------------------
     @staticmethod
     def lstsq(A: torch.Tensor, B: torch.Tensor, tol: Optional[float] = None) -> torch.Tensor:
-        """Least-squares solution by SVD with singular values above ``tol``
-        (default: machine epsilon of the dtype) inverted."""
+        """The reference's pseudo-inverse solution (singular values above
+        ``tol``, default the dtype's machine epsilon, inverted;
+        tedeous/optimizers/ngd.py:104-126) for the symmetric Gram matrix A.
+        For a symmetric A the singular values are the eigenvalues' magnitudes
+        and the pseudo-inverse is sum over |lambda| > tol of v v^T / lambda,
+        so the device's symmetric eigensolver gives the same solution; it
+        is used because the device SVD loses accuracy on ill-conditioned
+        Gram matrices (measured: 0.37 relative error against LAPACK at
+        condition 1e9, against 8e-8 for the eigensolver)."""
         tol = torch.finfo(A.dtype).eps if tol is None else tol
-        U, S, Vh = torch.linalg.svd(A, full_matrices=False)
-        Spinv = torch.where(S > tol, 1 / S, torch.zeros_like(S))
-        UhB = U.adjoint() @ B
-        if Spinv.ndim != UhB.ndim:
-            Spinv = Spinv.unsqueeze(-1)
-        return Vh.adjoint() @ (Spinv * UhB)
+        w, V = torch.linalg.eigh(A)
+        winv = torch.where(w.abs() > tol, 1 / w, torch.zeros_like(w))
+        VhB = V.adjoint() @ B
+        if winv.ndim != VhB.ndim:
+            winv = winv.unsqueeze(-1)
+        return V @ (winv * VhB)
 
     def _prepare(self, sol):
         eng: NNEngine = sol.engine

@@ -187,7 +187,8 @@ class NNTables:
         self.n_inst = len(problems)
         self.causal = (float(prob.causal_tol), int(prob.causal_nt)) if prob.causal_tol else None
 
-    def desc(self, arch: MLPArch, tile_rows: int = 0, chunk_tiles: int = 0) -> 'native.NNDesc':
+    def desc(self, arch: MLPArch, tile_rows: int = 0, chunk_tiles: int = 0,
+             persistent: int = 0) -> 'native.NNDesc':
         """The plan descriptor of include/tdx.h over these tables (which must
         outlive the plan-create call)."""
         desc = native.NNDesc()
@@ -206,6 +207,7 @@ class NNTables:
         desc.n_resid = self.n_resid
         desc.tile_rows = tile_rows
         desc.chunk_tiles = chunk_tiles
+        desc.persistent = persistent
         if self.causal:
             if self.n_inst != 1:
                 raise NotImplementedError("the causal loss serves one instance per plan")
@@ -226,7 +228,8 @@ class NNEngine:
     """Native 'NN' residual/loss engine for ``len(problems)`` instances."""
 
     def __init__(self, problems: Sequence[NNProblem], arch: MLPArch,
-                 device: torch.device, tile_rows: int = 0, chunk_tiles: int = 0):
+                 device: torch.device, tile_rows: int = 0, chunk_tiles: int = 0,
+                 persistent: int = 0):
         if not problems:
             raise ValueError("no problems")
         dtype = problems[0].dtype
@@ -243,7 +246,7 @@ class NNEngine:
         self.tables = NNTables(problems, arch, self.np_dtype)
         t = self.tables
         lib = native.lib()
-        desc = t.desc(arch, tile_rows, chunk_tiles)
+        desc = t.desc(arch, tile_rows, chunk_tiles, persistent)
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             native.check(lib.tdx_nn_plan_create(ctypes.byref(desc), ctypes.byref(handle)))

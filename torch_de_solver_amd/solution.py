@@ -69,10 +69,13 @@ class Solution:
                  derivative_points: int = 2, batch_size: int = None,
                  exact_div: bool = True):
         if weak_form and mode not in ('NN', 'autograd'):
-            raise NotImplementedError("weak-form losses are served in 'NN' and 'autograd' modes")
-        if tol and mode not in ('NN', 'autograd'):
-            raise NotImplementedError("causal losses (tol != 0) are served in 'NN' and "
-                                      "'autograd' modes")
+            # the reference's weak residual takes its points from the 'NN' or
+            # 'autograd' branch only and fails in 'mat' mode
+            # (tedeous/eval.py:204-208: grid_central is never bound)
+            raise NotImplementedError("weak-form losses exist in 'NN' and 'autograd' modes "
+                                      "(the reference has no 'mat' weak form)")
+        if tol and mode == 'mat' and len(_operators(equal_cls.operator)) > 1:
+            raise NotImplementedError("the 'mat' causal loss is served for one equation")
         if batch_size is not None and mode != 'NN':
             raise NotImplementedError("mini-batching is served in 'NN' mode only")
         if batch_size is not None and tol:
@@ -126,12 +129,19 @@ class Solution:
                 weak_form=self._weak_form)
             self.engine = ADEngine(self.problem, self.arch, device)
         else:
+            if self._tol:
+                # the reference's 'mat' causal loss slices the operator matrix
+                # by its columns (n_t = op.shape[1], tedeous/solution.py:62-64,
+                # tedeous/losses.py:160-161): one equation is one slice, so
+                # w = 1 and the operator term is mean(op^2) with no
+                # lambda_operator (tedeous/losses.py:162-170)
+                lam_op = 1
             self.problem: MatProblem = compile_mat(
                 self.grid, ops, eq.bconds, n_eq=len(ops),
                 derivative_points=self.derivative_points,
                 lambda_operator=lam_op, lambda_bound=lam_bnd)
             self.engine = MatEngine(self.problem, device, exact_div=self._exact_div)
-        self._lam_key = (_lambda_key(lam_op), _lambda_key(lam_bnd))
+        self._lam_key = (_lambda_key(self.lambda_operator), _lambda_key(self.lambda_bound))
 
     def _sync_lambdas(self) -> None:
         """The reference reads ``lambda_operator``/``lambda_bound`` on every

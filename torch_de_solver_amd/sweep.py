@@ -18,11 +18,13 @@ from .nn_engine import MLPArch, NNEngine, mlp_arch
 
 class SweepSolver:
     def __init__(self, problems: Sequence[Tuple], device=None, tile_rows: int = 0,
-                 chunk_tiles: int = 0, autotune: bool = False):
+                 chunk_tiles: int = 0, autotune: bool = False, persistent: int = 0):
         """``problems``: tuples ``(domain, equation, conditions, net, compile_kwargs)``
         as returned by the builders of ``workloads``; every net must have the
         same architecture.  ``autotune``: pick ``chunk_tiles`` by timing
-        candidates (``autotune.tune_nn``); the choice is in ``self.tuning``."""
+        candidates (``autotune.tune_nn``); the choice is in ``self.tuning``.
+        ``persistent=1``: the forward kernel as a persistent grid
+        (``tdx_nn_desc.persistent``)."""
         self.device = torch.device(device or 'cuda')
         self.nets = [p[3] for p in problems]
         self.arch: MLPArch = mlp_arch(self.nets[0])
@@ -56,7 +58,8 @@ class SweepSolver:
             from .autotune import tune_nn
             chunk_tiles, self.tuning = tune_nn(compiled, self.arch, self.device, self.params,
                                                tile_rows=tile_rows)
-        self.engine = NNEngine(compiled, self.arch, self.device, tile_rows, chunk_tiles)
+        self.engine = NNEngine(compiled, self.arch, self.device, tile_rows, chunk_tiles,
+                               persistent)
         self.exp_avg = torch.zeros_like(self.params)
         self.exp_avg_sq = torch.zeros_like(self.params)
         self.step_count = torch.zeros(1, dtype=torch.int64, device=self.device)
