@@ -201,6 +201,21 @@ int tdx_nn_train_step(const tdx_nn_plan *plan, void *params, void *exp_avg,
                       double beta1, double beta2, double eps, void *loss,
                       void *workspace, tdx_stream_t stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  tdx_nn_train_step with the workspace taken from a
+ * file-scope device array of tdx_noalloc_scratch_bytes() bytes (64 MiB) per
+ * GPU instead of a caller buffer; plans whose tdx_nn_workspace_bytes exceeds
+ * it are refused with TDX_EINVAL before anything is launched.  Allocates
+ * nothing (the array's device address is looked up once per device).  All
+ * calls on one GPU share that GPU's array: calls that overlap on one GPU
+ * (concurrent streams or threads) race; calls in stream order do not. */
+size_t tdx_noalloc_scratch_bytes(void);
+int tdx_nn_train_step_noalloc(const tdx_nn_plan *plan, void *params, void *exp_avg,
+                              void *exp_avg_sq, int64_t *step, double lr,
+                              double beta1, double beta2, double eps, void *loss,
+                              tdx_stream_t stream);
+
 /* Launch timing of the kernels of a plan, for measurement: with capacity
  * > 0 every following launch of the forward (kind 0), backward (kind 1) and
  * reduce/update (kind 2) kernels is bracketed by a pair of HIP events on its

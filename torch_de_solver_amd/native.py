@@ -29,6 +29,7 @@ EXPORTS = [
     'tdx_nn_workspace_bytes', 'tdx_nn_forward', 'tdx_nn_backward',
     'tdx_nn_row_count', 'tdx_nn_network_values', 'tdx_nn_forward_values', 'tdx_nn_jacobian',
     'tdx_nn_train_step', 'tdx_nn_profile', 'tdx_nn_profile_read',
+    'tdx_noalloc_scratch_bytes', 'tdx_nn_train_step_noalloc',
     'tdx_mat_plan_create', 'tdx_mat_plan_destroy', 'tdx_mat_workspace_bytes',
     'tdx_mat_forward', 'tdx_mat_backward', 'tdx_mat_train_step', 'tdx_mat_train_step_to',
     'tdx_adam', 'tdx_step_increment', 'tdx_quotient_check',
@@ -127,6 +128,9 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.tdx_nn_jacobian.argtypes = [vp, vp, vp, vp]
     lib.tdx_nn_backward.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.tdx_nn_train_step.argtypes = [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]
+    lib.tdx_noalloc_scratch_bytes.restype = sz
+    lib.tdx_noalloc_scratch_bytes.argtypes = []
+    lib.tdx_nn_train_step_noalloc.argtypes = [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp]
     lib.tdx_nn_profile.argtypes = [vp, ctypes.c_int]
     lib.tdx_nn_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int)]
