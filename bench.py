@@ -28,7 +28,10 @@ wave (the 128 x 128 'mat' problem), wave16k (the wave operator on a
 poisson_ad (the 'autograd' mode: a 2-D Poisson problem on 201 x 201 points
 with the 2-100x5-1 network of the reference's Poisson examples; roofline of
 the whole fused step against FP32 matrix peak from the algorithmic FLOPs of
-its linear jet maps).  The
+its linear jet maps) and ngd_lotka (natural-gradient descent: NGD steps/s
+on the reference's Lotka-Volterra NGD example in 'autograd' mode, 501
+points, 1-32-32-2 network; the reference's NGD algorithm in stock torch on
+the same GPU and the CPU oracle's NGD step beside it).  The
 'mat' step reads one grid buffer and writes the other (as Model.train does),
 so the kernel generated for the grid interior applies Adam as it forms each
 gradient; its roofline is that whole step at 24 B per point, the residual +
@@ -595,6 +598,85 @@ def bench_ad(args):
     return out
 
 
+# ---------------------------------------------------------------------------
+# natural-gradient workload
+# ---------------------------------------------------------------------------
+
+def bench_ngd(args):
+    """NGD steps (tedeous/optimizers/ngd.py) of the Lotka-Volterra 'autograd'
+    system through Model/Optimizer('NGD'): the Jacobian kernel, the Gram
+    GEMMs, the eigensolver's pseudo-inverse and the candidate forwards."""
+    import copy
+    import torch_de_solver_amd as tdx
+    from torch_de_solver_amd import workloads as W
+    from torch_de_solver_amd.compiler import compile_ad
+    from torch_de_solver_amd.optimizers import Closure
+    from torch_de_solver_amd.plain import PlainAD, PlainNGD
+    dom, eq, conds, net0, kw = W.lotka_ad(args.ngd_grid_res)
+    dev = torch.device('cuda')
+    gsn = 20
+    net = copy.deepcopy(net0).to(dev)
+    model = tdx.Model(net, dom, eq, conds)
+    model.verbose = False
+    model.compile('autograd', lambda_operator=kw['lambda_operator'],
+                  lambda_bound=kw['lambda_bound'])
+    opt = tdx.Optimizer('NGD', {'grid_steps_number': gsn})
+    model.optimizer = opt.optimizer_choice('autograd', model.solution_cls.model)
+    closure = Closure(False, model).get_closure('NGD')
+
+    def one():
+        model.optimizer.step(closure)
+    one()
+    one()
+    for _ in range(args.warmup):
+        one()
+    ms = time_cuda(one, args.steps)
+    loss_after = float(model.solution_cls.evaluate()[0])
+    eng = model.solution_cls.engine
+    flat = torch.cat([p.detach().reshape(-1) for p in model.solution_cls.model.parameters()])
+    jac_ms = time_cuda(lambda: eng.jacobian(flat), args.steps)
+    n_res, P = eng.tables.n_resid, eng.n_params
+    out = {
+        "metric": "NGD steps/s", "value": 1e3 / ms, "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"Lotka-Volterra 'autograd' {args.ngd_grid_res + 1} points, "
+                               "MLP 1-32-32-2, NGD grid_steps_number 20",
+                   "parameters": P, "residual_entries": n_res, "parallelism": "single GPU"},
+        "jacobian_ms": jac_ms, "loss_after": loss_after,
+    }
+    if not args.no_plain:
+        prob = compile_ad(dom.build('autograd'), eq.equation_lst,
+                          conds.build(dom.variable_dict), 2,
+                          lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
+        pnet = copy.deepcopy(net0).to(dev)
+        pl = PlainNGD(PlainAD(prob, pnet, dev), gsn)
+        pl.step()
+        pms = time_cuda(pl.step, max(args.plain_steps, 1))
+        out["plain_gpu"] = {"value": 1e3 / pms, "unit": "steps/s", "ms_per_step": pms,
+                            "what": "the reference's NGD step in stock torch ops on the same "
+                                    "GPU (one autograd.grad per residual entry, SVD "
+                                    "pseudo-inverse, a loss per candidate)"}
+        out["speedup_vs_plain_gpu"] = pms / ms
+    if not args.no_cpu:
+        from oracle import tedeous_oracle as O
+        torch.set_num_threads(cpu_cores())
+        orc = O.ADOracle(dict(dom.variable_dict), eq.equation_lst,
+                         [dict(c) for c in conds.conditions_lst])
+        cnet = copy.deepcopy(net0)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 20) or k < 1:
+            O.ngd_step(cnet, orc, kw['lambda_operator'], kw['lambda_bound'], gsn)
+            k += 1
+        cms = (time.perf_counter() - t0) * 1e3 / k
+        out["cpu_baseline"] = {"value": 1e3 / cms, "unit": "steps/s", "cores": cpu_cores(),
+                               "kind": "port",
+                               "sample": f"{k} NGD steps of the CPU oracle"}
+    return out
+
+
 def calibrate(args):
     dev = torch.device('cuda')
     n = 1 << 28                      # 2^28 float32 = 1 GiB
@@ -621,9 +703,11 @@ def main():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='sweep',
                     choices=['sweep', 'burgers', 'kdv', 'legendre', 'wave', 'wave16k',
-                             'poisson_ad'])
+                             'poisson_ad', 'ngd_lotka'])
     ap.add_argument('--grid-res', type=int, default=200,
                     help="'poisson_ad': intervals per axis (points per axis minus one)")
+    ap.add_argument('--ngd-grid-res', type=int, default=500,
+                    help="'ngd_lotka': intervals in t (points minus one)")
     ap.add_argument('--plain-steps', type=int, default=2)
     ap.add_argument('--plain-instances', type=int, default=256)
     ap.add_argument('--cpu-seconds', type=float, default=20.0)
@@ -664,6 +748,8 @@ def main():
         out = bench_mat(args)
     elif args.workload == 'poisson_ad':
         out = bench_ad(args)
+    elif args.workload == 'ngd_lotka':
+        out = bench_ngd(args)
     else:
         out = bench_nn(args)
     print(json.dumps(out), flush=True)

@@ -452,6 +452,15 @@ int tdx_ad_forward(const tdx_ad_plan *plan, const void *params, void *loss,
 int tdx_ad_backward(const tdx_ad_plan *plan, const void *params,
                     const void *dloss, void *dparams, void *workspace,
                     tdx_stream_t stream);
+/* Jacobian of the residuals with respect to the parameters,
+ * jt[p * n_resid + r] = d resid[r] / d params[p] (P x n_resid; r indexes the
+ * residual output of tdx_ad_forward), for the natural gradient's Gram
+ * matrix.  Replaces the per-residual torch.autograd.grad loop of
+ * NGD.gram_factory (tedeous/optimizers/ngd.py:57-78) over the 'autograd'
+ * operator (tedeous/derivative.py:61-132).  Needs no workspace; weak-form
+ * plans are refused (their residuals are integrals over runs). */
+int tdx_ad_jacobian(const tdx_ad_plan *plan, const void *params, void *jt,
+                    tdx_stream_t stream);
 /* Fused step: forward, adjoint and torch.optim.Adam update (see
  * tdx_nn_train_step for step and loss). */
 int tdx_ad_train_step(const tdx_ad_plan *plan, void *params, void *exp_avg,

@@ -823,13 +823,13 @@ def ngd_line_search(params: List[torch.Tensor], f_nat_grad: torch.Tensor,
     from torch.nn.utils import parameters_to_vector, vector_to_parameters
     base = parameters_to_vector(params).detach().clone()
     losses = []
-    with torch.no_grad():
-        for st in steps:
-            vector_to_parameters(base - st * f_nat_grad, params)
-            losses.append(loss_fn().reshape(1))
-        losses = torch.cat(losses)
-        k = int(torch.argmin(losses))
-        vector_to_parameters(base - steps[k] * f_nat_grad, params)
+    # autograd stays on: the 'autograd' operator differentiates the network
+    for st in steps:
+        vector_to_parameters(base - st * f_nat_grad, params)
+        losses.append(loss_fn().detach().reshape(1))
+    losses = torch.cat(losses)
+    k = int(torch.argmin(losses))
+    vector_to_parameters(base - steps[k] * f_nat_grad, params)
     return losses, k
 
 
@@ -853,5 +853,5 @@ def ngd_step(net, orc, lam_op, lam_bnd, grid_steps_number: int = 30):
     f = ngd_lstsq(G.detach(), f_grads.detach())
     steps = ngd_steps(grid_steps_number)
     losses, k = ngd_line_search(params, f, lambda: orc.evaluate(net, lam_op, lam_bnd)[0], steps)
-    return dict(loss=float(loss), grad=f_grads.detach(), G_int=G_int.detach(), G_b=G_b.detach(),
+    return dict(loss=float(loss.detach()), grad=f_grads.detach(), G_int=G_int.detach(), G_b=G_b.detach(),
                 G=G.detach(), f=f, losses=losses, k=k, step=float(steps[k]))

@@ -413,6 +413,27 @@ def system_ad(api, n=30, dtype='float64'):
     return d, eq, b, dict(mode='autograd', lambda_operator=[1, 2], lambda_bound=10)
 
 
+def lotka_ad(api, n=40, dtype='float64'):
+    """Lotka-Volterra in the reference's NGD-example notation
+    (examples/examples_lotka_volterra/example_Lotka_Volterra_paper_NGD.py:
+    175-224): one-factor terms written with 'term': [0] and 'var': [k],
+    two-factor terms with 'var': [0, 1]."""
+    d = api.Domain()
+    d.variable('t', [0, 1], n, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'t': 0}, value=4.0, var=0)
+    b.dirichlet({'t': 0}, value=2.0, var=1)
+    eq = api.Equation()
+    eq.add({'dx/dt': {'coeff': 1, 'term': [0], 'pow': 1, 'var': [0]},
+            '-x*alpha': {'coeff': -2., 'term': [None], 'pow': 1, 'var': [0]},
+            '+beta*x*y': {'coeff': 2., 'term': [[None], [None]], 'pow': [1, 1], 'var': [0, 1]}})
+    eq.add({'dy/dt': {'coeff': 1, 'term': [0], 'pow': 1, 'var': [1]},
+            '+y*delta': {'coeff': 2., 'term': [None], 'pow': 1, 'var': [1]},
+            '-gamma*x*y': {'coeff': -2., 'term': [[None], [None]], 'pow': [1, 1],
+                           'var': [0, 1]}})
+    return d, eq, b, dict(mode='autograd', lambda_operator=1, lambda_bound=100)
+
+
 def beam_ad(api, n=20, dtype='float64'):
     """Fourth order: u'''' = 1 on [0, 1], u(0) = u'(0) = 0, u''(1) = u'''(1) = 0."""
     d = api.Domain()
@@ -450,6 +471,7 @@ AD_CASES = {
     'system_ad': (system_ad, [1, 16, 16, 2]),
     'beam_ad': (beam_ad, [1, 10, 10, 10, 1]),
     'legendre_ad': (legendre_ad, [1, 32, 32, 1]),
+    'lotka_ad': (lotka_ad, [1, 12, 12, 2]),
 }
 
 

@@ -403,3 +403,20 @@ def test_no_conditions_raise_like_the_reference():
         pre = Operator_bcond_preproc(grid, eq.equation_lst, []).set_strategy(mode)
         with pytest.raises(ValueError):
             Solution(grid, pre, problems.make_mlp([2, 8, 1]), mode, None, 1, 10)
+
+
+def test_parse_operator_one_factor_variable_list():
+    """A one-factor term written with 'var': [k] (the reference's
+    Lotka-Volterra NGD example) addresses output k, as the reference's list
+    index net(x)[:, [k]] does; a two-factor term keeps one index per
+    factor."""
+    from torch_de_solver_amd.equations import parse_operator
+    d, eq, b, kw = problems.lotka_ad(api)
+    t0 = parse_operator(eq.equation_lst[0])
+    t1 = parse_operator(eq.equation_lst[1])
+    assert [f.var for f in t0[0].factors] == [0] and t0[0].factors[0].axes == [0]
+    assert [f.var for f in t1[0].factors] == [1] and t1[0].factors[0].axes == [0]
+    assert [f.var for f in t1[1].factors] == [1] and t1[1].factors[0].axes is None
+    assert [f.var for f in t0[2].factors] == [0, 1]
+    with pytest.raises(ValueError):
+        parse_operator({'bad': {'coeff': 1, 'term': [0], 'pow': 1, 'var': [0, 1]}})

@@ -87,25 +87,32 @@ def _oracle_jacobian(orc, net, kw):
     return rows(op), rows(b - t), b.shape
 
 
-def test_ngd_jacobian_and_gram_against_oracle():
-    d, eq, b, kw, orc = _problem()
-    net = _net()
-    J_op, J_b, bshape = _oracle_jacobian(orc, net, kw)
-    sol = _solution(d, eq, b, kw, _net().to(DEV))
+def _spans(sol):
+    """(offset, count, column, col_pos) of every residual equation."""
+    if sol.mode == 'NN':
+        return [(r.offset, r.count, r.column, r.col_pos) for r in sol.engine.tables.resid]
+    return [(off, cnt, col, pos) for (_, _, off, cnt, col, pos) in sol.engine.tables.resid]
+
+
+def _check_jacobian_and_gram(sol, orc, kw, net_cpu, tag):
+    J_op, J_b, bshape = _oracle_jacobian(orc, net_cpu, kw)
     eng = sol.engine
     flat = torch.cat([p.detach().reshape(-1) for p in sol.model.parameters()])
     jt = eng.jacobian(flat).cpu()
-    _print(f"[ngd] J: {jt.shape[0]} parameters x {jt.shape[1]} residuals")
+    _print(f"[ngd] {tag} J: {jt.shape[0]} parameters x {jt.shape[1]} residuals")
     # the engine's residual entries in the oracle's flattened order
     n_types = bshape[1]
     op_cols, b_cols = [], []
-    for r in eng.tables.resid:
-        cols = list(range(r.offset, r.offset + r.count))
-        if r.column < 0:
-            op_cols += cols
+    for off, cnt, col, pos in _spans(sol):
+        cols = list(range(off, off + cnt))
+        if col < 0:
+            op_cols.append(cols)
         else:
-            b_cols += [(int(p) * n_types + r.column, c) for p, c in zip(r.col_pos, cols)]
-    J_op_n = jt[:, op_cols].T
+            b_cols += [(int(p) * n_types + col, c) for p, c in zip(pos, cols)]
+    # operator residuals flattened [N, n_eq]: entry (i, q) at i * n_eq + q
+    n_eq = len(op_cols)
+    op_order = [op_cols[q][i] for i in range(len(op_cols[0])) for q in range(n_eq)]
+    J_op_n = jt[:, op_order].T
     np.testing.assert_allclose(J_op_n.numpy(), J_op.numpy(), rtol=1e-10,
                                atol=1e-12 * float(J_op.abs().max()))
     J_b_n = torch.zeros_like(J_b)
@@ -120,6 +127,46 @@ def test_ngd_jacobian_and_gram_against_oracle():
     G_o = 1.0 / len(J_op) * J_op.T @ J_op + 1.0 / len(J_b) * J_b.T @ J_b
     np.testing.assert_allclose(G.numpy(), G_o.numpy(), rtol=1e-10,
                                atol=1e-12 * float(G_o.abs().max()))
+
+
+def test_ngd_jacobian_and_gram_against_oracle():
+    d, eq, b, kw, orc = _problem()
+    net = _net()
+    sol = _solution(d, eq, b, kw, _net().to(DEV))
+    _check_jacobian_and_gram(sol, orc, kw, net, 'NN burgers')
+
+
+def _ad_problem(name):
+    builder, _ = problems.AD_CASES[name]
+    d, eq, b, kw = builder(api, dtype='float64')
+    orc = O.ADOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst])
+    return d, eq, b, kw, orc
+
+
+def _ad_solution(d, eq, b, kw, net):
+    grid = d.build('autograd')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst,
+                                 b.build(d.variable_dict)).set_strategy('autograd')
+    return Solution(grid, pre, net, 'autograd', None, kw['lambda_operator'], kw['lambda_bound'])
+
+
+# 'autograd' Jacobians: second (Burgers), third (KdV), fourth (beam) order
+# derivatives, two outputs (system), mixed partials (mixed)
+AD_JAC_CASES = {
+    'burgers_ad': [2, 6, 6, 1],
+    'kdv_ad': [2, 5, 5, 1],
+    'beam_ad': [1, 5, 5, 1],
+    'system_ad': [1, 6, 6, 2],
+    'mixed_ad': [2, 5, 5, 1],
+}
+
+
+@pytest.mark.parametrize('name', list(AD_JAC_CASES))
+def test_ngd_ad_jacobian_and_gram_against_oracle(name):
+    d, eq, b, kw, orc = _ad_problem(name)
+    layers = AD_JAC_CASES[name]
+    sol = _ad_solution(d, eq, b, kw, _net(layers).to(DEV))
+    _check_jacobian_and_gram(sol, orc, kw, _net(layers), f"autograd {name}")
 
 
 def test_ngd_lstsq_on_the_device():
@@ -138,19 +185,28 @@ def test_ngd_lstsq_on_the_device():
         np.testing.assert_allclose(xm, np.linalg.solve(A, Bm), rtol=1e-10, atol=1e-13)
 
 
-def _ngd_model(net, d, eq, b, kw):
+def _ngd_model(net, d, eq, b, kw, mode='NN'):
     model = tdx.Model(net, d, eq, b)
     model.verbose = False
-    model.compile('NN', lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
+    model.compile(mode, lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
     return model
 
 
 def test_ngd_step_against_oracle():
-    d, eq, b, kw, orc = _problem()
+    _check_step(*_problem(), 'NN')
+
+
+def test_ngd_ad_step_against_oracle():
+    """One NGD step of 'autograd' Burgers (2-4-1 network: condition number
+    of G near 3e8, interior step k = 14) against the oracle."""
+    _check_step(*_ad_problem('burgers_ad'), 'autograd')
+
+
+def _check_step(d, eq, b, kw, orc, mode):
     net_o = _step_net()
     ref = O.ngd_step(net_o, orc, kw['lambda_operator'], kw['lambda_bound'], grid_steps_number=20)
     net = _step_net().to(DEV)
-    model = _ngd_model(net, d, eq, b, kw)
+    model = _ngd_model(net, d, eq, b, kw, mode)
     model.train(tdx.Optimizer('NGD', {'grid_steps_number': 20}), 2)   # one NGD step
     last = model.optimizer.last
     np.testing.assert_allclose(last['G'].cpu().numpy(), ref['G'].numpy(), rtol=1e-10,
@@ -160,7 +216,7 @@ def test_ngd_step_against_oracle():
     cond = np.linalg.cond(ref['G'].numpy())
     f_n, f_o = last['f'].cpu().numpy(), ref['f'].numpy()
     ferr = np.linalg.norm(f_n - f_o) / np.linalg.norm(f_o)
-    _print(f"[ngd] cond(G) = {cond:.3e}; natural gradient rel {ferr:.3e}; "
+    _print(f"[ngd] {mode}: cond(G) = {cond:.3e}; natural gradient rel {ferr:.3e}; "
            f"step {ref['step']} (k = {ref['k']}); loss {ref['loss']:.6f} -> "
            f"{float(ref['losses'][ref['k']]):.6f}")
     assert cond < 1e10, cond
@@ -214,9 +270,11 @@ def test_ngd_line_search_of_a_given_direction():
         assert int(np.argmin(losses)) == k_o
 
 
-def test_ngd_training_matches_oracle():
-    """Five NGD steps through Model.train against five oracle steps."""
-    d, eq, b, kw, orc = _problem()
+@pytest.mark.parametrize('mode', ['NN', 'autograd'])
+def test_ngd_training_matches_oracle(mode):
+    """Five NGD steps through Model.train against five oracle steps (Burgers
+    in either mode)."""
+    d, eq, b, kw, orc = _problem() if mode == 'NN' else _ad_problem('burgers_ad')
     net_o = _step_net()
     hist_o = []
     for _ in range(5):
@@ -224,7 +282,7 @@ def test_ngd_training_matches_oracle():
                                  grid_steps_number=20)['loss'])
     final_o = float(orc.evaluate(net_o, kw['lambda_operator'], kw['lambda_bound'])[0])
     net = _step_net().to(DEV)
-    model = _ngd_model(net, d, eq, b, kw)
+    model = _ngd_model(net, d, eq, b, kw, mode)
     hist = []
 
     class Record:
@@ -245,7 +303,7 @@ def test_ngd_training_matches_oracle():
 
     model.train(tdx.Optimizer('NGD', {'grid_steps_number': 20}), 6, callbacks=[Record()])
     final = float(model.solution_cls.evaluate()[0])
-    _print(f"[ngd] losses {hist_o} -> {final_o:.6f}; native {hist} -> {final:.6f}")
+    _print(f"[ngd] {mode} losses {hist_o} -> {final_o:.6f}; native {hist} -> {final:.6f}")
     np.testing.assert_allclose(hist, hist_o, rtol=1e-6)
     np.testing.assert_allclose(final, final_o, rtol=1e-6)
     assert final < hist_o[0]

@@ -236,6 +236,18 @@ class ADEngine:
             native.ptr(loss), native.ptr(self.workspace), native.stream_ptr()))
         self.fwd_gen += 1
 
+    def jacobian(self, params: torch.Tensor) -> torch.Tensor:
+        """``[P, n_resid]``: d resid[r] / d params[p] of every residual entry
+        (the transposed J of the natural gradient)."""
+        self._check(params)
+        n = self.tables.n_resid
+        if self.n_params * n * params.element_size() > (16 << 30):
+            raise ValueError(f"the Jacobian would take {self.n_params} x {n} values")
+        jt = torch.empty(self.n_params, n, dtype=self.dtype, device=self.device)
+        native.check(native.lib().tdx_ad_jacobian(
+            self.handle, native.ptr(params), native.ptr(jt), native.stream_ptr()))
+        return jt
+
     def loss(self, params: torch.Tensor) -> torch.Tensor:
         """``[1, 2]``: (lambda-weighted loss, normalized loss), differentiable
         in ``params`` through the native adjoint."""

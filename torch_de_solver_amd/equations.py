@@ -102,6 +102,14 @@ def parse_operator(operator: dict) -> List[Term]:
                 ax = [int(a) for a in axes if a is not None]
                 if not ax:
                     ax = None
+            if isinstance(v, (list, tuple)):
+                # a one-factor term written with 'var': [k] is wrapped once
+                # more by the unification; the reference then indexes the
+                # output with the list [k], i.e. column k
+                # (tedeous/derivative.py:122-126)
+                if len(v) != 1:
+                    raise ValueError(f"term variable {v!r}: one index per factor")
+                v = v[0]
             factors.append(Factor(var=int(v), axes=ax, pow=p))
         terms.append(Term(coeff=term['coeff'], factors=factors))
     return terms

@@ -15,13 +15,16 @@ of tedeous/optimizers/closure.py:98-117) takes one step as:
 5. a grid line search: the loss at ``params - 0.5**k * f`` for
    ``k = 0..grid_steps_number``, the first minimum taken (ngd.py:29-55).
 
-Here J comes from one kernel (``tdx_nn_jacobian``: every residual's
-seed-weighted network adjoint, one thread per entry), the Gram products are
-two library GEMMs on the device, the pseudo-inverse comes from the device's
-symmetric eigensolver (G is symmetric; see ``lstsq``), and the line search
-evaluates all candidate parameter vectors in one launch of a grouped plan
-that holds the problem once per candidate.  The step stays on the device:
-no host synchronisation.
+Here J comes from one kernel (``tdx_nn_jacobian`` in 'NN' mode: every
+residual's seed-weighted network adjoint, one thread per entry;
+``tdx_ad_jacobian`` in 'autograd' mode: the same over the network's jets),
+the Gram products are two library GEMMs on the device, the pseudo-inverse
+comes from the device's symmetric eigensolver (G is symmetric; see
+``lstsq``), and the line search evaluates all candidate parameter vectors
+in one launch of a grouped plan that holds the problem once per candidate
+('NN'), or one forward per candidate ('autograd').  Every value stays on
+the device; the eigensolver's convergence check is the step's one wait for
+the device.
 """
 
 from typing import Optional
@@ -32,7 +35,8 @@ from .nn_engine import NNEngine
 
 
 class NGD(torch.optim.Optimizer):
-    """Drop-in for ``tedeous.optimizers.ngd.NGD`` on 'NN'-mode solutions."""
+    """Drop-in for ``tedeous.optimizers.ngd.NGD`` on 'NN' and 'autograd'
+    solutions."""
 
     def __init__(self, params, grid_steps_number: int = 30):
         defaults = {'grid_steps_number': grid_steps_number}
@@ -65,22 +69,37 @@ class NGD(torch.optim.Optimizer):
         return V @ (winv * VhB)
 
     def _prepare(self, sol):
-        eng: NNEngine = sol.engine
+        """Engine, line-search engine (None: the engine itself, once per
+        candidate), operator and boundary residual indices, boundary
+        padding length."""
+        eng = sol.engine
         if self._cache is not None and self._cache[0] is eng:
             return self._cache
         if eng.n_inst != 1:
             raise NotImplementedError("NGD serves one instance per solution")
         op_idx, b_idx = [], []
-        for r in eng.tables.resid:
-            (op_idx if r.column < 0 else b_idx).extend(range(r.offset, r.offset + r.count))
+        if sol.mode == 'NN':
+            spans = [(r.offset, r.count, r.column) for r in eng.tables.resid]
+        else:
+            spans = [(off, cnt, col) for (_, _, off, cnt, col, _) in eng.tables.resid]
+        for off, cnt, col in spans:
+            (op_idx if col < 0 else b_idx).extend(range(off, off + cnt))
         dev = eng.device
         op_idx = torch.tensor(op_idx, dtype=torch.long, device=dev)
         b_idx = torch.tensor(b_idx, dtype=torch.long, device=dev)
         prob = sol.problem
         n_b = prob.bc_max_len * len(prob.bc_keys)
-        search = NNEngine([prob] * len(self.steps), sol.arch, dev)
+        search = NNEngine([prob] * len(self.steps), sol.arch, dev) if sol.mode == 'NN' else None
         self._cache = (eng, search, op_idx, b_idx, max(n_b, 1))
         return self._cache
+
+    def _line_search(self, eng, search, cand: torch.Tensor, losses: torch.Tensor) -> None:
+        """Loss pairs of every candidate parameter vector (rows of cand)."""
+        if search is not None:
+            search.forward(cand.view(-1), losses)
+            return
+        for k in range(cand.shape[0]):
+            eng.forward(cand[k], losses[2 * k:2 * k + 2])
 
     def gram(self, jt: torch.Tensor, op_idx: torch.Tensor, b_idx: torch.Tensor,
              n_b: int) -> torch.Tensor:
@@ -98,8 +117,8 @@ class NGD(torch.optim.Optimizer):
             raise ValueError("NGD needs the closure of Closure.get_closure('NGD')")
         int_res, bval, true_bval, loss, loss_function = closure()
         sol = getattr(loss_function, '__self__', None)
-        if sol is None or getattr(sol, 'mode', None) != 'NN':
-            raise NotImplementedError("NGD is served for 'NN'-mode solutions")
+        if sol is None or getattr(sol, 'mode', None) not in ('NN', 'autograd'):
+            raise NotImplementedError("NGD is served for 'NN' and 'autograd' solutions")
         eng, search, op_idx, b_idx, n_b = self._prepare(sol)
         params = self.params
         flat = torch.cat([p.detach().reshape(-1) for p in params]).contiguous()
@@ -118,7 +137,7 @@ class NGD(torch.optim.Optimizer):
         steps = self.steps.to(device=flat.device, dtype=flat.dtype)
         cand = (flat.unsqueeze(0) - steps.view(K, 1) * f.unsqueeze(0)).contiguous()
         losses = torch.empty(2 * K, dtype=flat.dtype, device=flat.device)
-        search.forward(cand.view(-1), losses)
+        self._line_search(eng, search, cand, losses)
         losses = losses.view(K, 2)[:, 0]
         k = torch.argmin(losses)
         new = cand.index_select(0, k.view(1)).view(-1)

@@ -35,8 +35,8 @@ class Optimizer:
         """torch optimizer over the network parameters ('NN') or the grid
         tensor ('mat') (tedeous/optimizers/optimizer.py:25-73)."""
         if self.optimizer == 'NGD':
-            if mode != 'NN':
-                raise NotImplementedError("NGD is served for 'NN'-mode solutions")
+            if mode not in ('NN', 'autograd'):
+                raise NotImplementedError("NGD is served for 'NN' and 'autograd' solutions")
             from .ngd import NGD
             cls = NGD
         elif self.optimizer not in _TORCH:

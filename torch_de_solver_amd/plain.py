@@ -234,22 +234,77 @@ class PlainAD:
             total = v if total is None else total + v
         return total
 
-    def loss(self):
+    def residuals(self):
+        """Operator residuals ``[N, n_eq]``, padded boundary values and
+        targets ``[max_len, n_types]`` and the boundary lambdas."""
         ops = [self._value(sets, terms).reshape(-1, 1) for sets, terms, _ in self.eqs]
         op = ops[0] if len(ops) == 1 else torch.cat(ops, 1)
+        ncol = len(self.keys)
+        bval = torch.zeros(self.max_len, max(ncol, 1), dtype=self.dtype, device=self.device)
+        true = torch.zeros_like(bval)
+        lam_b = [0.0] * max(ncol, 1)
+        for sets, terms, col, pos, tgt, lam in self.bcs:
+            v = self._value(sets, terms).reshape(-1)
+            bval = bval.index_put((pos, torch.full_like(pos, col)), v)
+            true = true.index_put((pos, torch.full_like(pos, col)), tgt)
+            lam_b[col] = lam
+        return op, bval[:, :ncol], true[:, :ncol], lam_b[:ncol]
+
+    def loss(self):
+        op, bval, true, lam_b = self.residuals()
         lam_op = torch.tensor([[lam for _, _, lam in self.eqs]], dtype=self.dtype,
                               device=self.device)
         loss = torch.mean(op ** 2, 0) @ lam_op.T
         if self.bcs:
-            ncol = len(self.keys)
-            bval = torch.zeros(self.max_len, ncol, dtype=self.dtype, device=self.device)
-            true = torch.zeros_like(bval)
-            lam_b = [0.0] * ncol
-            for sets, terms, col, pos, tgt, lam in self.bcs:
-                v = self._value(sets, terms).reshape(-1)
-                bval = bval.index_put((pos, torch.full_like(pos, col)), v)
-                true = true.index_put((pos, torch.full_like(pos, col)), tgt)
-                lam_b[col] = lam
             lb = torch.tensor([lam_b], dtype=self.dtype, device=self.device)
             loss = loss + torch.mean((bval - true) ** 2, 0) @ lb.T
+        return loss
+
+
+class PlainNGD:
+    """Natural-gradient step as the reference computes it
+    (tedeous/optimizers/ngd.py:29-55, 57-78, 104-126, 144-209) in stock
+    torch ops on the device: one ``torch.autograd.grad`` per residual entry
+    for the Gram matrices, the SVD pseudo-inverse, a loss evaluation per
+    candidate step."""
+
+    def __init__(self, plain: PlainAD, grid_steps_number: int = 20):
+        self.plain = plain
+        self.params = list(plain.net.parameters())
+        self.steps = 0.5 ** torch.linspace(0, grid_steps_number, grid_steps_number + 1)
+
+    def _gram(self, res):
+        rows = []
+        for r in res.reshape(-1):
+            g = torch.autograd.grad(r, self.params, retain_graph=True, allow_unused=True)
+            g = [torch.zeros_like(p) if x is None else x for p, x in zip(self.params, g)]
+            rows.append(torch.cat([x.reshape(-1) for x in g]))
+        J = torch.stack(rows)
+        return 1.0 / len(rows) * J.T @ J
+
+    @staticmethod
+    def _lstsq(A, B):
+        tol = torch.finfo(A.dtype).eps
+        U, S, Vh = torch.linalg.svd(A, full_matrices=False)
+        Spinv = torch.where(S > tol, 1 / S, torch.zeros_like(S))
+        return Vh.adjoint() @ (Spinv * (U.adjoint() @ B))
+
+    def step(self):
+        from torch.nn.utils import parameters_to_vector, vector_to_parameters
+        op, bval, true, lam_b = self.plain.residuals()
+        loss = self.plain.loss()
+        grads = torch.autograd.grad(loss, self.params, retain_graph=True, allow_unused=True)
+        f_grads = torch.cat([(torch.zeros_like(p) if g is None else g).reshape(-1)
+                             for p, g in zip(self.params, grads)])
+        G = self._gram(op) + self._gram(bval - true)
+        G = torch.minimum(loss.detach().reshape(()), torch.zeros_like(G[0, 0])) * \
+            torch.eye(len(G), dtype=G.dtype, device=G.device) + G
+        f = self._lstsq(G, f_grads)
+        base = parameters_to_vector(self.params).detach().clone()
+        losses = []
+        for st in self.steps.tolist():
+            vector_to_parameters(base - st * f, self.params)
+            losses.append(self.plain.loss().detach().reshape(1))
+        k = torch.argmin(torch.cat(losses))
+        vector_to_parameters(base - self.steps.to(base.device, base.dtype)[k] * f, self.params)
         return loss

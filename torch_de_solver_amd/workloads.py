@@ -18,6 +18,9 @@ with seeded initialisation.  Sizes follow the problem statements:
   [0, 1]^2, u = 0 on the sides, 201 x 201 points, MLP 2-100x5-1.
 * sweep: 256 Burgers solves, nu = 10^U(-3,-1), nx in [64, 512],
   nt in [32, 128], MLP 2-32-32-32-1 each, seeded per instance.
+* Lotka-Volterra ('autograd', the natural-gradient workload): x' = x(a - b y),
+  y' = y(-d + g x), a = b = d = g = 20, x(0) = 4, y(0) = 2, t in [0, 1]
+  (501 points), MLP 1-32-32-2, lambda_bound 100.
 """
 
 import math
@@ -105,6 +108,33 @@ def poisson_ad(grid_res: int = 200, seed: int = 0):
                torch.sin(np.pi * g[:, 1]), 'u': [None], 'pow': 0, 'var': 0},
     })
     net = tanh_mlp([2, 100, 100, 100, 100, 100, 1], seed)
+    return d, eq, b, net, dict(mode='autograd', lambda_operator=1, lambda_bound=100)
+
+
+def lotka_ad(grid_res: int = 500, seed: int = 0, dtype: str = 'float32'):
+    """'autograd' Lotka-Volterra system as trained by natural-gradient
+    descent in the reference's examples (examples/examples_lotka_volterra/
+    example_Lotka_Volterra_paper_NGD.py:30-40, 159-228: the equations,
+    constants, conditions and lambda_bound = 100) with the plain 1-32-32-2
+    tanh network that example keeps as its alternative (lines 131-137)."""
+    al = be = de = ga = 20.
+    d = Domain()
+    d.variable('t', [0, 1], grid_res, dtype=dtype)
+    b = Conditions()
+    b.dirichlet({'t': 0}, value=4., var=0)
+    b.dirichlet({'t': 0}, value=2., var=1)
+    eq = Equation()
+    eq.add({
+        'dx/dt': {'coeff': 1, 'term': [0], 'pow': 1, 'var': [0]},
+        '-x*alpha': {'coeff': -al, 'term': [None], 'pow': 1, 'var': [0]},
+        '+beta*x*y': {'coeff': be, 'term': [[None], [None]], 'pow': [1, 1], 'var': [0, 1]},
+    })
+    eq.add({
+        'dy/dt': {'coeff': 1, 'term': [0], 'pow': 1, 'var': [1]},
+        '+y*delta': {'coeff': de, 'term': [None], 'pow': 1, 'var': [1]},
+        '-gamma*x*y': {'coeff': -ga, 'term': [[None], [None]], 'pow': [1, 1], 'var': [0, 1]},
+    })
+    net = tanh_mlp([1, 32, 32, 2], seed, torch.float32 if dtype == 'float32' else torch.float64)
     return d, eq, b, net, dict(mode='autograd', lambda_operator=1, lambda_bound=100)
 
 
