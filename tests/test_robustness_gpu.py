@@ -146,3 +146,27 @@ def test_backward_after_an_unrelated_forward():
     l2.backward()
     g2 = torch.cat([p.grad.reshape(-1) for p in params])
     np.testing.assert_array_equal(g.cpu().numpy(), g2.cpu().numpy())
+
+
+def test_model_change_rebinds_the_solution():
+    """Solution._model_change (tedeous/solution.py:109-127): after the swap,
+    evaluate() is the new network's loss (the plan kept for the same
+    architecture, lowered anew for another), equal to a solution built
+    with that network from the start."""
+    sol, net = _nn()
+    l0 = float(sol.evaluate()[0])
+    for layers in ((2, 12, 12, 12, 1), (2, 8, 8, 1)):
+        _print(f"[model change] to {layers}")
+        other = problems.randomize_biases(problems.make_mlp(list(layers), seed=5,
+                                                            dtype=torch.float64)).to(DEV)
+        plan = sol.engine
+        sol._model_change(other)
+        assert (sol.engine is plan) == (layers == (2, 12, 12, 12, 1))
+        l1 = float(sol.evaluate()[0])
+        ref, _ = _nn(layers=layers)
+        ref._model_change(other)
+        fresh, _ = _nn(layers=layers)
+        fresh.model = other
+        assert l1 == float(ref.evaluate()[0])
+        assert abs(l1 - float(fresh.evaluate()[0])) == 0.0
+        assert l1 != l0

@@ -15,34 +15,9 @@ import torch
 
 from .ad_engine import ADEngine
 from .compiler import ADProblem, MatProblem, NNProblem, compile_ad, compile_mat, compile_nn
-from .device import check_device
+from .input_preprocessing import Operator_bcond_preproc, _Prepared
 from .mat_engine import MatEngine
 from .nn_engine import MLPArch, NNEngine, mlp_arch
-
-
-class Operator_bcond_preproc:
-    """Holds the compile inputs (tedeous/input_preprocessing.py:597-644).
-
-    ``set_strategy(mode)`` returns the holder itself tagged with the mode;
-    the lowering to kernel tables happens in ``Solution``.
-    """
-
-    def __init__(self, grid: torch.Tensor, operator, bconds, h: float = 0.001,
-                 inner_order: str = '1', boundary_order: str = '2'):
-        self.grid = check_device(grid)
-        self.operator = operator
-        self.bconds = bconds
-        self.h = h
-        self.inner_order = inner_order
-        self.boundary_order = boundary_order
-        self.mode = None
-
-    def set_strategy(self, strategy: str) -> 'Operator_bcond_preproc':
-        if strategy not in ('NN', 'mat', 'autograd'):
-            raise NotImplementedError(
-                f"mode {strategy!r} is not served by this engine")
-        self.mode = strategy
-        return self
 
 
 def _operators(op) -> List[dict]:
@@ -63,7 +38,7 @@ def _lambda_key(lam) -> Tuple:
 class Solution:
     """Objective of one problem (tedeous/solution.py:19-168)."""
 
-    def __init__(self, grid: torch.Tensor, equal_cls: Operator_bcond_preproc,
+    def __init__(self, grid: torch.Tensor, equal_cls: _Prepared,
                  model: Union[torch.nn.Module, torch.Tensor], mode: str,
                  weak_form=None, lambda_operator=1, lambda_bound=1, tol: float = 0,
                  derivative_points: int = 2, batch_size: int = None,
@@ -150,6 +125,20 @@ class Solution:
         change rebuilds the plan's tables with the new weights."""
         key = (_lambda_key(self.lambda_operator), _lambda_key(self.lambda_bound))
         if key != self._lam_key:
+            self._build()
+
+    def _model_change(self, new_model: Union[torch.nn.Module, torch.Tensor]) -> None:
+        """Rebind the solution to another model (tedeous/solution.py:109-127,
+        used by the reference's cache and retraining paths).  The plan's
+        tables do not depend on the weights: the same plan serves a model
+        of the same architecture (or a grid of the same shape); another one
+        is lowered anew."""
+        if self.mode in ('NN', 'autograd'):
+            same = mlp_arch(new_model) == self.arch
+        else:
+            same = tuple(new_model.shape) == tuple(self.model.shape)
+        self.model = new_model
+        if not same:
             self._build()
 
     # -- per step ------------------------------------------------------------
