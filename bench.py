@@ -51,6 +51,7 @@ import os
 import sys
 import time
 
+import numpy as np
 import torch
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -149,7 +150,8 @@ def plain_nn_step_ms(problems, nets0, n_steps, max_instances, lr):
             loss.backward()
             opt.step()
         step()
-        total_ms += time_cuda(step, n_steps)
+        # median of three timed repeats after the warm-up step
+        total_ms += float(np.median([time_cuda(step, n_steps) for _ in range(3)]))
         pts_done += comp[i].n_central
     scale = pts_all / max(pts_done, 1)
     return total_ms * scale, len(sel), scale
@@ -174,12 +176,16 @@ def cpu_nn_step_ms(problems, nets0, budget_s, lr):
                          [dict(c) for c in conds.conditions_lst], h=kw.get('h', 1e-3))
         netc = nets0[i].to('cpu')
         opt = torch.optim.Adam(netc.parameters(), lr=lr)
-        t0 = time.perf_counter()
-        opt.zero_grad()
-        loss = orc.evaluate(netc, kw['lambda_operator'], kw['lambda_bound'])[0]
-        loss.backward()
-        opt.step()
-        ms_sum += (time.perf_counter() - t0) * 1e3
+
+        def step():
+            t0 = time.perf_counter()
+            opt.zero_grad()
+            loss = orc.evaluate(netc, kw['lambda_operator'], kw['lambda_bound'])[0]
+            loss.backward()
+            opt.step()
+            return (time.perf_counter() - t0) * 1e3
+        step()  # first-call overhead stays out of the sample
+        ms_sum += float(np.median([step() for _ in range(3)]))
         pts_done += len(orc.central)
         n_inst += 1
     return ms_sum * pts_all / pts_done, n_inst, pts_done, pts_all
@@ -336,15 +342,18 @@ def bench_nn(args):
                                                   args.plain_instances, lr)
         out["plain_gpu"] = {"value": 1e3 / plain_ms, "unit": "steps/s", "ms_per_step": plain_ms,
                             "instances_timed": n_sel, "scaled_by_points": scale,
-                            "steps_per_instance": args.plain_steps}
+                            "steps_per_instance": args.plain_steps,
+                            "repeats": "median of 3 timed repeats per instance after a warm-up step"}
         out["speedup_vs_plain_gpu"] = plain_ms / ms
         log(f"[bench] plain GPU: {plain_ms:.1f} ms/step ({n_sel} instances timed)")
     if not args.no_cpu:
         cms, n_inst, pts, pts_all = cpu_nn_step_ms(problems, nets0, args.cpu_seconds, lr)
         out["cpu_baseline"] = {"value": 1e3 / cms, "unit": "steps/s", "cores": cpu_cores(),
                                "kind": "port",
-                               "sample": f"{n_inst} instance(s) x 1 Adam step of the CPU oracle "
-                                         f"({pts} of {pts_all} collocation points), scaled by points"}
+                               "sample": f"{n_inst} instance(s): per instance one warm-up Adam "
+                                         f"step of the CPU oracle, then the median of 3 timed "
+                                         f"steps ({pts} of {pts_all} collocation points), "
+                                         f"scaled by points"}
         log(f"[bench] cpu oracle: {cms:.0f} ms/step (scaled)")
     if args.acc_steps > 0:
         out["accuracy"] = accuracy_nn(args, lr)

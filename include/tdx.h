@@ -239,17 +239,19 @@ int64_t tdx_quotient_check(float h, int mode);
 /* 'mat' mode                                                             */
 /* ===================================================================== */
 /*
- * The unknown is a grid tensor u[n_eq][n_0][n_1] (1-D grids use n_0 = 1;
- * the single axis is axis 1).  D_a is the reference's per-axis stencil
+ * The unknown is a grid tensor u[n_eq][n_0][n_1][n_2] (1-D grids use
+ * n_0 = n_2 = 1 and the single axis is axis 1; 2-D grids n_2 = 1; 3-D grids
+ * have axes 0, 1, 2).  D_a is the reference's per-axis stencil
  * (tedeous/derivative.py:199-291): interior (sum_b alpha_back[b] u[i+b-dp+1]
  * + sum_f alpha_forw[f] u[i+f]) / (2 h_a), first dp-1 points forw/h_a,
- * last dp-1 points back/h_a.
+ * last dp-1 points back/h_a.  A factor applies up to TDX_MAT_MAX_CHAIN
+ * of them.
  * Tables:
  *   eq      [TDX_MEQ_INTS]: n_terms, term_i, lam_r (lambda/N)
  *   term    [TDX_MTERM_INTS]: n_factors, factor_i, coeff_kind
  *           (0: reals[coeff_idx], 1: data[coeff_idx + flat point]), coeff_idx
  *   factor  [TDX_MFACTOR_INTS]: var, pow_kind, pow_int, pow_r, n_axes,
- *           axis0, axis1, axis2 (in application order)
+ *           axis[TDX_MAT_MAX_CHAIN] (in application order)
  *   bc entry [TDX_MBC_INTS]: n_sets, pos_i (ints: n_sets flat positions),
  *           n_terms (0: the plain value u[var]), term_i, var, target_r
  *           (data index of the padded target), lam_r (lambda/max_len),
@@ -261,18 +263,24 @@ int64_t tdx_quotient_check(float h, int mode);
  *   bc adjoint CSR over touched grid points: bct_ptr[n_touch+1],
  *           bct_pos[n_touch] (flat index into u incl. the n_eq plane),
  *           bct_seed[nnz], bct_w_r[nnz] (reals index of the weight)
+ * Plans of 1-D and 2-D grids whose factors apply at most 3 stencils run
+ * tiled kernels.  Other plans (3-D grids, longer chains, reaches beyond an
+ * LDS tile) run every factor's chain as full-grid passes; their boundary
+ * entries read the chain fields at their positions and scatter their seeds
+ * back, so the bc adjoint CSR is not read (n_touch may be 0).
  */
 #define TDX_MEQ_INTS 3
 #define TDX_MTERM_INTS 4
-#define TDX_MFACTOR_INTS 8
+#define TDX_MAT_MAX_CHAIN 8
+#define TDX_MFACTOR_INTS (5 + TDX_MAT_MAX_CHAIN)
 #define TDX_MBC_INTS 10
 
 typedef struct tdx_mat_desc {
   int dtype;
   int n_eq;
-  int n0, n1; /* grid shape; 1-D grids: n0 = 1 */
-  int dp;     /* derivative_points */
-  double h[2];
+  int n0, n1, n2; /* grid shape; 1-D grids: n0 = n2 = 1, 2-D grids: n2 = 1 */
+  int dp;         /* derivative_points */
+  double h[3];    /* step per axis (h[0] unused for 1-D, h[2] below 3-D) */
   const double *alpha_back; /* dp values */
   const double *alpha_forw; /* dp values */
   int eq_i;                 /* ints offset of the n_eq equation records */

@@ -194,6 +194,80 @@ MAT_CASES = {
 }
 
 
+def heat3_mat(api, n=(8, 7, 6), dtype='float32', dp=2):
+    """3-D 'mat' grid (x, y, t): u_t - 0.3 (u_xx + u_yy) - 0.1 u_xyt + 0.5 u u_x
+    = 0, u = sin(pi x) sin(pi y) at t = 0, u = 0 at x = 1, a periodic pair in
+    y, an operator condition u_x + u_y = 0 at x = 0."""
+    d = api.Domain()
+    d.variable('x', [0, 1], n[0] - 1, dtype=dtype)
+    d.variable('y', [0, 1], n[1] - 1, dtype=dtype)
+    d.variable('t', [0, 1], n[2] - 1, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'x': [0, 1], 'y': [0, 1], 't': 0},
+                value=lambda g: torch.sin(np.pi * g[:, 0]) * torch.sin(np.pi * g[:, 1]))
+    b.dirichlet({'x': 1, 'y': [0, 1], 't': [0, 1]}, value=0.0)
+    b.periodic([{'x': [0, 1], 'y': 0, 't': [0, 1]}, {'x': [0, 1], 'y': 1, 't': [0, 1]}])
+    b.operator({'x': 0, 'y': [0, 1], 't': [0, 1]},
+               operator={'u_x': {'coeff': 1, 'u_x': [0], 'pow': 1},
+                         'u_y': {'coeff': 1, 'u_y': [1], 'pow': 1}}, value=0.0)
+    eq = api.Equation()
+    eq.add({'u_t': {'coeff': 1, 'u_t': [2], 'pow': 1},
+            '-0.3u_xx': {'coeff': -0.3, 'u_xx': [0, 0], 'pow': 1},
+            '-0.3u_yy': {'coeff': -0.3, 'u_yy': [1, 1], 'pow': 1},
+            '-0.1u_xyt': {'coeff': -0.1, 'u_xyt': [0, 1, 2], 'pow': 1},
+            '0.5uu_x': {'coeff': 0.5, 'uux': [[None], [0]], 'pow': [1, 1], 'var': [0, 0]}})
+    return d, eq, b, dict(mode='mat', lambda_operator=1, lambda_bound=10, derivative_points=dp)
+
+
+def beam_mat(api, n=24, dtype='float32', dp=2):
+    """Fourth order on one axis ('mat'): u_xxxx = 1 with u = u_x = 0 at x = 0
+    and u_xx = u_xxx = 0 at x = 1."""
+    d = api.Domain()
+    d.variable('x', [0, 1], n - 1, dtype=dtype)
+    b = api.Conditions()
+    b.dirichlet({'x': 0}, value=0.0)
+    b.operator({'x': 0}, operator={'du/dx': {'coeff': 1, 'du/dx': [0], 'pow': 1}}, value=0.0)
+    b.operator({'x': 1}, operator={'d2u/dx2': {'coeff': 1, 'd2u/dx2': [0, 0], 'pow': 1}},
+               value=0.0)
+    b.operator({'x': 1}, operator={'d3u/dx3': {'coeff': 1, 'd3u/dx3': [0, 0, 0], 'pow': 1}},
+               value=0.0)
+    eq = api.Equation()
+    eq.add({'d4u/dx4': {'coeff': 1, 'd4u/dx4': [0, 0, 0, 0], 'pow': 1},
+            '-1': {'coeff': -1, 'u': [None], 'pow': 0}})
+    return d, eq, b, dict(mode='mat', lambda_operator=1, lambda_bound=10, derivative_points=dp)
+
+
+def plate_mat(api, n=(16, 12), dtype='float32', dp=2):
+    """Fourth order in 2-D ('mat'): u_tt + 0.01 u_xxxx = 0 on [0, 1]^2 with
+    u = sin(pi x) and u_t = 0 at t = 0, u = 0 and u_xx = 0 at x = 0 and 1."""
+    d = api.Domain()
+    d.variable('x', [0, 1], n[0] - 1, dtype=dtype)
+    d.variable('t', [0, 1], n[1] - 1, dtype=dtype)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=torch.sin(np.pi * x))
+    b.operator({'x': [0, 1], 't': 0}, operator={'u_t': {'coeff': 1, 'u_t': [1], 'pow': 1}},
+               value=0.0)
+    for xb in (0, 1):
+        b.dirichlet({'x': xb, 't': [0, 1]}, value=0.0)
+        b.operator({'x': xb, 't': [0, 1]},
+                   operator={'u_xx': {'coeff': 1, 'u_xx': [0, 0], 'pow': 1}}, value=0.0)
+    eq = api.Equation()
+    eq.add({'u_tt': {'coeff': 1, 'u_tt': [1, 1], 'pow': 1},
+            '0.01u_xxxx': {'coeff': 0.01, 'u_xxxx': [0, 0, 0, 0], 'pow': 1}})
+    return d, eq, b, dict(mode='mat', lambda_operator=1, lambda_bound=10, derivative_points=dp)
+
+
+# 'mat' problems served by the full-grid passes: 3-D grids, chains of 4
+MAT_ND_CASES = {
+    'heat3_mat': heat3_mat,
+    'heat3_mat_dp3': lambda api, dtype='float32': heat3_mat(api, n=(9, 8, 7), dtype=dtype, dp=3),
+    'beam_mat': beam_mat,
+    'beam_mat_dp3': lambda api, dtype='float32': beam_mat(api, dtype=dtype, dp=3),
+    'plate_mat': plate_mat,
+}
+
+
 def mat_init(shape, seed=0, dtype=torch.float32, std=0.01):
     g = torch.Generator().manual_seed(seed)
     return (std * torch.randn(shape, generator=g, dtype=torch.float64)).to(dtype)

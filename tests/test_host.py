@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     for s in syms:
         assert hasattr(lib, s), f"missing export {s}"
     assert set(syms) == set(native.EXPORTS)
-    assert lib.tdx_version() == 2
+    assert lib.tdx_version() == 3
 
 
 def test_plan_validation_rejects_bad_tables():

@@ -157,6 +157,53 @@ def test_mat_derivative_equals_per_axis_stencil():
         np.testing.assert_allclose(du, np.moveaxis(ref, -1, axis), rtol=1e-12)
 
 
+@pytest.mark.parametrize('dp', [2, 3])
+def test_mat_derivative_3d_equals_per_axis_stencil(dp):
+    """On a 3-D field (the reference's pos == 2 branch) the derivative along
+    each axis is the clean per-axis stencil with one-sided ends, against a
+    numpy restatement with the Vandermonde coefficients."""
+    D = O.MatDerivative(dp)
+    back, forw, ab, af = O.mat_alphas(dp)
+    u = torch.randn(6, 7, 8, dtype=torch.float64)
+    h = 0.1
+    for axis in (0, 1, 2):
+        du = D.d(u, torch.tensor(h, dtype=torch.float64), axis).numpy()
+        v = np.moveaxis(u.numpy(), axis, -1)
+        n = v.shape[-1]
+        ref = np.empty_like(v)
+        for i in range(n):
+            b = sum(a * v[..., i + o] for a, o in zip(ab, back)) if i + back[0] >= 0 else None
+            f = sum(a * v[..., i + o] for a, o in zip(af, forw)) if i + forw[-1] < n else None
+            if i < dp - 1:
+                ref[..., i] = f / h
+            elif i >= n - dp + 1:
+                ref[..., i] = b / h
+            else:
+                ref[..., i] = (b + f) / (2 * h)
+        np.testing.assert_allclose(du, np.moveaxis(ref, -1, axis), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize('dp', [4, 5])
+def test_mat_fourth_order_chain_is_exact_on_cubics(dp):
+    """Four successive applications (a factor written [0, 0, 0, 0]) on a
+    cubic in x, every stencil exact for degree < dp: the third application
+    is 6 everywhere and the fourth 0; three axes, the cubic along axis 1."""
+    d = O.MatDerivative(dp)
+    n = 3 * dp
+    x = torch.linspace(0, 1, n, dtype=torch.float64)
+    grid = torch.stack(torch.meshgrid(torch.linspace(0, 1, 5, dtype=torch.float64), x,
+                                      torch.linspace(0, 1, 4, dtype=torch.float64),
+                                      indexing='ij'))
+    u = (grid[1] ** 3 - 2 * grid[1] ** 2 + grid[0] * grid[2]).unsqueeze(0)
+    t3 = d.term(u, {'coeff': 1, 'd3': [[1, 1, 1]], 'pow': [1], 'var': [0]}, grid)
+    t4 = d.term(u, {'coeff': 1, 'd4': [[1, 1, 1, 1]], 'pow': [1], 'var': [0]}, grid)
+    # the step is the float32 difference of the axis values (as the
+    # reference takes it), 1e-8 relative off 1/(n-1): a relative error of
+    # about 3e-8 per power of 1/h
+    np.testing.assert_allclose(t3.numpy(), 6.0, rtol=1e-6)
+    np.testing.assert_allclose(t4.numpy(), 0.0, atol=1e-7)
+
+
 def test_padded_bc_mean_divides_by_max_len():
     b = {'dirichlet': torch.tensor([1., 2., 3., 4.]), 'operator': torch.tensor([2.])}
     t = {'dirichlet': torch.zeros(4), 'operator': torch.zeros(1)}

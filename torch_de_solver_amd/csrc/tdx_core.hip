@@ -18,5 +18,5 @@ void clear_error() { g_last_error.clear(); }
 
 }  // namespace tdx
 
-extern "C" int tdx_version(void) { return 2; }
+extern "C" int tdx_version(void) { return 3; }
 extern "C" const char *tdx_last_error(void) { return tdx::g_last_error.c_str(); }

@@ -39,12 +39,12 @@ constexpr int MMAX_EQ = 4;
 constexpr int MMAX_FS = 16;   // factor slots over all terms of all equations
 constexpr int MMAX_TF = 4;    // factors of one term
 constexpr int MAT_THREADS = 256;
-constexpr int MFS_INTS = 5;   // factor-slot record: var, n_axes, axis[3]
+constexpr int MFS_INTS = 2 + TDX_MAT_MAX_CHAIN;  // factor-slot record: var, n_axes, axes
 // Stencil coefficient table (T values): ab[MMAX_DP] af[MMAX_DP], then per
-// axis 4*MMAX_DP products for the transposed stencils:
+// axis (three) 4*MMAX_DP products for the transposed stencils:
 //   ab/(2h) | af/(2h) | ab/h | af/h
 constexpr int MCOEF_AX = 2 * MMAX_DP;
-constexpr int MCOEF_N = 2 * MMAX_DP + 2 * 4 * MMAX_DP;
+constexpr int MCOEF_N = 2 * MMAX_DP + 3 * 4 * MMAX_DP;
 
 template <typename T>
 struct MatArgs {
@@ -85,6 +85,10 @@ struct MatArgs {
   T *bc_seeds;
   const T *dloss;
   int n_tile_wg;
+  // full-grid plans: chain fields [n_fs][N], and per bc entry the first
+  // field of its operator's factors (-1: plain value)
+  T *fields;
+  const int32_t *bc_slot;
   // fused train step (u_in -> u_out): moments and the step's Adam scalars
   int fused;
   T *u_out;
@@ -611,7 +615,11 @@ __device__ T point_chain(const MatArgs<T> &a, const T *plane, int g0, int g1, co
   return qdiv<T, DIV>(b + f, h2, ih2);
 }
 
-template <typename T, int DIV>
+// Boundary entries: value, loss partials and (adj) seeds.  ND (full-grid
+// plans): the factor values are the chain fields at the entry's positions
+// (factor k of the entry's operator in field bc_slot[e] + k); otherwise the
+// chains are evaluated at the point (2-D, at most 3 applications).
+template <typename T, int DIV, bool ND>
 __global__ __launch_bounds__(MAT_THREADS) void k_mat_bc(MatArgs<T> a, int adj) {
   __shared__ T red[2 * MAT_THREADS];
   const int tid = threadIdx.x;
@@ -623,23 +631,31 @@ __global__ __launch_bounds__(MAT_THREADS) void k_mat_bc(MatArgs<T> a, int adj) {
     const T target = a.data[Bc[5]];
     const T lam_d = a.reals[Bc[6]], invden = a.reals[Bc[7]];
     const int seed_i = Bc[8];
+    const int slot0 = ND ? a.bc_slot[e] : 0;
+    // value of factor record Fc (the k-th factor of the operator) at pos
+    auto fval = [&](const int32_t *Fc, int k, int pos) -> T {
+      if constexpr (ND) {
+        return a.fields[(int64_t)(slot0 + k) * a.N + pos];
+      } else {
+        const int g0 = pos / a.n1, g1 = pos - (pos / a.n1) * a.n1;
+        return point_chain<T, DIV>(a, a.u + Fc[0] * a.N, g0, g1, Fc + 5, Fc[4]);
+      }
+    };
     T val = T(0);
     for (int s = 0; s < n_sets; ++s) {
       const int pos = a.ints[pos_i + s];
-      const int g0 = pos / a.n1, g1 = pos - (pos / a.n1) * a.n1;
       T v;
       if (n_terms == 0) {
         v = a.u[var * a.N + pos];
       } else {
         T r = T(0);
+        int k = 0;
         for (int t = 0; t < n_terms; ++t) {
           const int32_t *Tm = a.ints + term_i + t * TDX_MTERM_INTS;
           T der = T(1);
-          for (int f = 0; f < Tm[0]; ++f) {
+          for (int f = 0; f < Tm[0]; ++f, ++k) {
             const int32_t *Fc = a.ints + Tm[1] + f * TDX_MFACTOR_INTS;
-            const int ax[3] = {Fc[5], Fc[6], Fc[7]};
-            const T fv = point_chain<T, DIV>(a, a.u + Fc[0] * a.N, g0, g1, ax, Fc[4]);
-            der = der * mfpow(a, Fc, fv);
+            der = der * mfpow(a, Fc, fval(Fc, k, pos));
           }
           const T c = Tm[2] == 0 ? a.reals[Tm[3]] : a.data[Tm[3] + pos];
           const T tv = c * der;
@@ -652,7 +668,7 @@ __global__ __launch_bounds__(MAT_THREADS) void k_mat_bc(MatArgs<T> a, int adj) {
           const int32_t *Rb = a.ints + Bc[9] - 1;
           const T *R = a.reals + Rb[1];
           T dt = T(0);
-          for (int k = 0; k < Rb[0]; ++k) dt = dt + R[1 + k] * r;
+          for (int q = 0; q < Rb[0]; ++q) dt = dt + R[1 + q] * r;
           v = R[0] * a.u[var * a.N + pos] + dt;
         }
       }
@@ -673,7 +689,7 @@ __global__ __launch_bounds__(MAT_THREADS) void k_mat_bc(MatArgs<T> a, int adj) {
         const T *R = a.reals + Rb[1];
         a.bc_seeds[sidx++] = dr * R[0];
         robin_w = T(0);
-        for (int k = 0; k < Rb[0]; ++k) robin_w = robin_w + R[1 + k];
+        for (int q = 0; q < Rb[0]; ++q) robin_w = robin_w + R[1 + q];
       }
       for (int s = 0; s < n_sets; ++s) {
         const T sg = (s == 0 ? dr : -dr) * robin_w;
@@ -682,24 +698,22 @@ __global__ __launch_bounds__(MAT_THREADS) void k_mat_bc(MatArgs<T> a, int adj) {
           continue;
         }
         const int pos = a.ints[pos_i + s];
-        const int g0 = pos / a.n1, g1 = pos - (pos / a.n1) * a.n1;
+        int k0 = 0;
         for (int t = 0; t < n_terms; ++t) {
           const int32_t *Tm = a.ints + term_i + t * TDX_MTERM_INTS;
           const int nf = Tm[0];
           const T c = Tm[2] == 0 ? a.reals[Tm[3]] : a.data[Tm[3] + pos];
           for (int f = 0; f < nf; ++f) {
             const int32_t *Fc = a.ints + Tm[1] + f * TDX_MFACTOR_INTS;
-            const int ax[3] = {Fc[5], Fc[6], Fc[7]};
-            T d = c * sg *
-                  mdfpow(a, Fc, point_chain<T, DIV>(a, a.u + Fc[0] * a.N, g0, g1, ax, Fc[4]));
+            T d = c * sg * mdfpow(a, Fc, fval(Fc, k0 + f, pos));
             for (int f2 = 0; f2 < nf; ++f2) {
               if (f2 == f) continue;
               const int32_t *F2 = a.ints + Tm[1] + f2 * TDX_MFACTOR_INTS;
-              const int ax2[3] = {F2[5], F2[6], F2[7]};
-              d *= mfpow(a, F2, point_chain<T, DIV>(a, a.u + F2[0] * a.N, g0, g1, ax2, F2[4]));
+              d *= mfpow(a, F2, fval(F2, k0 + f2, pos));
             }
             a.bc_seeds[sidx++] = d;
           }
+          k0 += nf;
         }
       }
     }
@@ -820,6 +834,149 @@ __global__ __launch_bounds__(256) void k_mat_frame_adam(MatArgs<T> a, int r_lo, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// full-grid plans: 3-D grids, chains of more than 3 applications, reaches
+// beyond an LDS tile.  Every factor's chain is a sequence of passes over the
+// whole plane (one thread per point, coalesced along the last axis), the
+// residual program runs per point on the chain fields, and the adjoint runs
+// the transposed passes back, accumulating into the gradient.  Bounded by
+// HBM: each pass reads and writes the plane once.
+// ---------------------------------------------------------------------------
+template <typename T, int DIV>
+__global__ __launch_bounds__(256) void k_mat_nd_d(const T *src, T *dst, int N, int stride,
+                                                  int n, int dp, const T *coef, T h, T h2, T ih,
+                                                  T ih2) {
+  for (int p = blockIdx.x * 256 + threadIdx.x; p < N; p += gridDim.x * 256) {
+    const int g = (p / stride) % n;
+    dst[p] = dstencil<T, 0, DIV>(src, p, stride, g, n, dp, coef, coef + MMAX_DP, h, h2, ih, ih2);
+  }
+}
+
+// dst = D^T src along one axis (acc: dst += D^T src); c is the axis' block of
+// the coefficient table (ab/(2h), af/(2h), ab/h, af/h)
+template <typename T>
+__global__ __launch_bounds__(256) void k_mat_nd_dt(const T *src, T *dst, int N, int stride,
+                                                   int n, int dp, const T *c, int acc) {
+  for (int p = blockIdx.x * 256 + threadIdx.x; p < N; p += gridDim.x * 256) {
+    const int g = (p / stride) % n;
+    const T v = dtstencil<T, 0>(src, p, stride, g, n, dp, c, MMAX_DP);
+    dst[p] = acc ? dst[p] + v : v;
+  }
+}
+
+// dst = src (acc: dst += src)
+template <typename T>
+__global__ __launch_bounds__(256) void k_mat_nd_copy(const T *src, T *dst, int64_t n, int acc) {
+  for (int64_t p = blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256)
+    dst[p] = acc ? dst[p] + src[p] : src[p];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_mat_nd_zero(T *dst, int64_t n) {
+  for (int64_t p = blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256)
+    dst[p] = T(0);
+}
+
+// Residual program per point on the chain fields F (the n_fs factor slots of
+// the equations, [n_fs][N]): residual field, loss partials per workgroup
+// (fixed order) and, with adj, each factor's seed written over its value
+// (each slot is read by its own term only).
+template <typename T>
+__global__ __launch_bounds__(MAT_THREADS) void k_mat_nd_resid(MatArgs<T> a, int adj) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *PR = reinterpret_cast<T *>(smem_raw);
+  int32_t *PI = reinterpret_cast<int32_t *>(PR + a.n_preal);
+  __shared__ T red[2 * MAT_THREADS];
+  const int tid = threadIdx.x;
+  for (int k = tid; k < a.n_preal; k += MAT_THREADS) PR[k] = a.preal[k];
+  for (int k = tid; k < a.n_pint; k += MAT_THREADS) PI[k] = a.pint[k];
+  __syncthreads();
+  T *F = a.fields;
+  const int64_t N = a.N;
+  T lsum = T(0), lnsum = T(0);
+  const T dl = adj ? *a.dloss : T(0);
+  for (int64_t p = blockIdx.x * (int64_t)MAT_THREADS + tid; p < N;
+       p += (int64_t)gridDim.x * MAT_THREADS) {
+    for (int eq = 0; eq < a.n_eq; ++eq) {
+      const int n_terms = __builtin_amdgcn_readfirstlane(PI[2 * eq]);
+      const int term0 = __builtin_amdgcn_readfirstlane(PI[2 * eq + 1]);
+      const T lam_n = PR[eq];
+      T r = T(0);
+      for (int t = 0; t < n_terms; ++t) {
+        const int32_t *Tr = PI + a.pterm0 + 4 * (term0 + t);
+        const int nf = __builtin_amdgcn_readfirstlane(Tr[0]);
+        const int fs = __builtin_amdgcn_readfirstlane(Tr[3]);
+        T der = T(1);
+        for (int f = 0; f < nf; ++f)
+          der = der * ppow(PI + a.pfac0 + 3 * (fs + f), PR, F[(fs + f) * N + p]);
+        const T tv = term_coeff(Tr, PR, a.data, p) * der;
+        r = (t == 0) ? tv : r + tv;
+      }
+      lsum += lam_n * (r * r);
+      lnsum += r * r;
+      if (a.op) a.op[eq * N + p] = r;
+      if (adj) {
+        const T g = T(2) * lam_n * r * dl;
+        for (int t = 0; t < n_terms; ++t) {
+          const int32_t *Tr = PI + a.pterm0 + 4 * (term0 + t);
+          const int nf = __builtin_amdgcn_readfirstlane(Tr[0]);
+          const int fs = __builtin_amdgcn_readfirstlane(Tr[3]);
+          const T c = term_coeff(Tr, PR, a.data, p);
+          T fv[MMAX_TF], pw[MMAX_TF];
+#pragma unroll
+          for (int f = 0; f < MMAX_TF; ++f) {
+            fv[f] = f < nf ? F[(fs + f) * N + p] : T(0);
+            pw[f] = f < nf ? ppow(PI + a.pfac0 + 3 * (fs + f), PR, fv[f]) : T(1);
+          }
+#pragma unroll
+          for (int f = 0; f < MMAX_TF; ++f) {
+            if (f < nf) {
+              T d = c * g * pdpow(PI + a.pfac0 + 3 * (fs + f), PR, fv[f]);
+#pragma unroll
+              for (int f2 = 0; f2 < MMAX_TF; ++f2)
+                if (f2 != f && f2 < nf) d *= pw[f2];
+              F[(fs + f) * N + p] = d;
+            }
+          }
+        }
+      }
+    }
+  }
+  red[tid] = lsum;
+  red[MAT_THREADS + tid] = lnsum / T(N);
+  __syncthreads();
+  for (int st = MAT_THREADS / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      red[tid] += red[tid + st];
+      red[MAT_THREADS + tid] += red[MAT_THREADS + tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    a.lp[2 * blockIdx.x] = red[0];
+    a.lp[2 * blockIdx.x + 1] = red[MAT_THREADS];
+  }
+}
+
+// Boundary seeds back onto the fields: row j of the CSR (key, seed list)
+// adds its seeds, in order, to fields[key] (key < n_field_words) or to
+// grad[key - n_field_words].  Keys are distinct: no two threads write one
+// word.
+template <typename T>
+__global__ void k_mat_nd_scatter(const int32_t *ptr, const int64_t *key, const int32_t *sidx,
+                                 int rows, const T *seeds, T *fields, int64_t n_field_words,
+                                 T *grad) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= rows) return;
+  T v = T(0);
+  for (int k = ptr[j]; k < ptr[j + 1]; ++k) v += seeds[sidx[k]];
+  const int64_t kk = key[j];
+  if (kk < n_field_words)
+    fields[kk] += v;
+  else
+    grad[kk - n_field_words] += v;
+}
+
 template <typename T>
 __global__ void k_mat_one(T *x) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *x = T(1);
@@ -852,6 +1009,16 @@ struct tdx_mat_plan {
   int n_gen_wg;
   size_t off_lp, off_mid, off_seeds, off_one, off_grad, off_adam, ws_bytes;
   double ab[MMAX_DP], af[MMAX_DP];
+  // full-grid plans (nd = 1): chain fields per factor slot -- the equations'
+  // n_fs slots first, then the boundary operators' -- as {var, axes...};
+  // per bc entry its first slot; the boundary seed scatter CSR (rows keyed
+  // by field word, then by gradient word)
+  int nd = 0;
+  int64_t N = 0;
+  int n_slots = 0, n_nd_wg = 0, sc_rows_f = 0, sc_rows = 0;
+  std::vector<std::vector<int>> chains;
+  DevTable bc_slot_tab, sc_ptr, sc_key, sc_idx;
+  size_t off_fields = 0, off_tmp = 0;
 };
 
 
@@ -909,7 +1076,7 @@ MatArgs<T> mat_args(const tdx_mat_plan *p, const void *u, void *ws) {
   a.n0 = d.n0;
   a.n1 = d.n1;
   a.dp = d.dp;
-  a.N = (int64_t)d.n0 * d.n1;
+  a.N = p->N;
   a.h0 = T(d.h[0]);
   a.h1 = T(d.h[1]);
   a.h20 = T(2.0 * d.h[0]);
@@ -957,6 +1124,8 @@ MatArgs<T> mat_args(const tdx_mat_plan *p, const void *u, void *ws) {
   a.bc_seeds = (T *)(w + p->off_seeds);
   a.dloss = (const T *)(w + p->off_one);
   a.n_tile_wg = p->n_gen_wg;
+  a.fields = p->nd ? (T *)(w + p->off_fields) : nullptr;
+  a.bc_slot = (const int32_t *)p->bc_slot_tab.ptr;
   a.fused = 0;
   a.u_out = nullptr;
   a.m = nullptr;
@@ -1028,7 +1197,7 @@ int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t 
     if (rc) return rc;
   }
   if (p->n_bc_wg > 0) {
-    hipLaunchKernelGGL((k_mat_bc<T, DIV>), dim3(p->n_bc_wg), dim3(MAT_THREADS), 0, s, a,
+    hipLaunchKernelGGL((k_mat_bc<T, DIV, false>), dim3(p->n_bc_wg), dim3(MAT_THREADS), 0, s, a,
                        adj ? 1 : 0);
     TDX_TRY_HIP(hipGetLastError());
     if (adj && p->d.n_touch > 0) {
@@ -1069,8 +1238,135 @@ int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t 
   return TDX_OK;
 }
 
+// Axis geometry of a full-grid plan: stride, extent, step.
+struct NdAxis {
+  int stride, n;
+  double h;
+};
+inline NdAxis nd_axis(const tdx_mat_plan *p, int ax) {
+  const tdx_mat_desc &d = p->d;
+  if (ax == 0) return {d.n1 * d.n2, d.n0, d.h[0]};
+  if (ax == 1) return {d.n2, d.n1, d.h[1]};
+  return {1, d.n2, d.h[2]};
+}
+
+inline unsigned nd_blocks(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192));
+}
+
+// A full-grid plan's forward (and adjoint): chain passes for every factor
+// slot, the residual program, the boundary entries; with adj the scatter of
+// the boundary seeds, the transposed passes into the gradient and the
+// fused Adam update.
+template <typename T, int DIV>
+int mat_run_nd(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t s) {
+  const int N = (int)p->N;
+  const int dp = p->d.dp;
+  T *F = a.fields;
+  T *tmp[2] = {(T *)((char *)F + (p->off_tmp - p->off_fields)),
+               (T *)((char *)F + (p->off_tmp - p->off_fields)) + N};
+  const unsigned nb = nd_blocks(N);
+  for (int sl = 0; sl < p->n_slots; ++sl) {
+    const std::vector<int> &c = p->chains[sl];
+    const T *src = a.u + (int64_t)c[0] * N;
+    T *out = F + (int64_t)sl * N;
+    const int nax = (int)c.size() - 1;
+    if (nax == 0)
+      hipLaunchKernelGGL((k_mat_nd_copy<T>), dim3(nb), dim3(256), 0, s, src, out, (int64_t)N, 0);
+    for (int j = 0; j < nax; ++j) {
+      const NdAxis ax = nd_axis(p, c[1 + j]);
+      T *dst = j == nax - 1 ? out : tmp[j & 1];
+      hipLaunchKernelGGL((k_mat_nd_d<T, DIV>), dim3(nb), dim3(256), 0, s, src, dst, N, ax.stride,
+                         ax.n, dp, a.coef, T(ax.h), T(2.0 * ax.h), T(1.0 / ax.h),
+                         T(1.0 / (2.0 * ax.h)));
+      src = dst;
+    }
+  }
+  TDX_TRY_HIP(hipGetLastError());
+  const size_t lds = (size_t)p->n_preal * sizeof(T) + (size_t)p->n_pint * 4;
+  hipLaunchKernelGGL((k_mat_nd_resid<T>), dim3(p->n_nd_wg), dim3(MAT_THREADS), lds, s, a,
+                     adj ? 1 : 0);
+  TDX_TRY_HIP(hipGetLastError());
+  if (p->n_bc_wg > 0) {
+    hipLaunchKernelGGL((k_mat_bc<T, DIV, true>), dim3(p->n_bc_wg), dim3(MAT_THREADS), 0, s, a,
+                       adj ? 1 : 0);
+    TDX_TRY_HIP(hipGetLastError());
+  }
+  if (adj) {
+    const int64_t eqw = (int64_t)p->n_fs * N, allw = (int64_t)p->n_slots * N;
+    // the boundary slots' fields now take the boundary seeds
+    if (allw > eqw) {
+      hipLaunchKernelGGL((k_mat_nd_zero<T>), dim3(nd_blocks(allw - eqw)), dim3(256), 0, s,
+                         F + eqw, allw - eqw);
+      if (p->sc_rows_f > 0)
+        hipLaunchKernelGGL((k_mat_nd_scatter<T>), dim3((p->sc_rows_f + 255) / 256), dim3(256),
+                           0, s, (const int32_t *)p->sc_ptr.ptr, (const int64_t *)p->sc_key.ptr,
+                           (const int32_t *)p->sc_idx.ptr, p->sc_rows_f,
+                           (const T *)a.bc_seeds, F, allw, a.grad);
+    }
+    const int64_t gw = (int64_t)p->d.n_eq * N;
+    hipLaunchKernelGGL((k_mat_nd_zero<T>), dim3(nd_blocks(gw)), dim3(256), 0, s, a.grad, gw);
+    for (int sl = 0; sl < p->n_slots; ++sl) {
+      const std::vector<int> &c = p->chains[sl];
+      T *g = a.grad + (int64_t)c[0] * N;
+      const T *src = F + (int64_t)sl * N;
+      const int nax = (int)c.size() - 1;
+      if (nax == 0)
+        hipLaunchKernelGGL((k_mat_nd_copy<T>), dim3(nb), dim3(256), 0, s, src, g, (int64_t)N, 1);
+      for (int j = nax - 1, k = 0; j >= 0; --j, ++k) {
+        const int axi = c[1 + j];
+        const NdAxis ax = nd_axis(p, axi);
+        T *dst = j == 0 ? g : tmp[k & 1];
+        hipLaunchKernelGGL((k_mat_nd_dt<T>), dim3(nb), dim3(256), 0, s, src, dst, N, ax.stride,
+                           ax.n, dp, a.coef + MCOEF_AX + axi * 4 * MMAX_DP, j == 0 ? 1 : 0);
+        src = dst;
+      }
+    }
+    TDX_TRY_HIP(hipGetLastError());
+    if (p->sc_rows > p->sc_rows_f) {
+      const int rows = p->sc_rows - p->sc_rows_f;
+      hipLaunchKernelGGL((k_mat_nd_scatter<T>), dim3((rows + 255) / 256), dim3(256), 0, s,
+                         (const int32_t *)p->sc_ptr.ptr + p->sc_rows_f,
+                         (const int64_t *)p->sc_key.ptr + p->sc_rows_f,
+                         (const int32_t *)p->sc_idx.ptr, rows, (const T *)a.bc_seeds, F,
+                         (int64_t)p->n_slots * N, a.grad);
+      TDX_TRY_HIP(hipGetLastError());
+    }
+    if (a.fused) {
+      // Adam over every point: the frame kernel with an empty rectangle on
+      // a single row of N points
+      MatArgs<T> b = a;
+      b.n0 = 1;
+      b.n1 = N;
+      const int64_t tot = gw;
+      hipLaunchKernelGGL((k_mat_frame_adam<T>), dim3(nd_blocks(tot)), dim3(256), 0, s, b, 0, 0,
+                         0, 0);
+      TDX_TRY_HIP(hipGetLastError());
+    }
+  }
+  if (loss) {
+    const int n = p->n_nd_wg + p->n_bc_wg;
+    const int g = std::min(256, (n + 4095) / 4096);
+    T *mid = a.lp_mid;
+    if (g > 1) {
+      hipLaunchKernelGGL((k_mat_loss<T>), dim3(g), dim3(256), 0, s, a.lp, n, mid);
+      TDX_TRY_HIP(hipGetLastError());
+      hipLaunchKernelGGL((k_mat_loss<T>), dim3(1), dim3(256), 0, s, (const T *)mid, g, loss);
+    } else {
+      hipLaunchKernelGGL((k_mat_loss<T>), dim3(1), dim3(256), 0, s, a.lp, n, loss);
+    }
+    TDX_TRY_HIP(hipGetLastError());
+  }
+  return TDX_OK;
+}
+
 template <typename T>
 int mat_dispatch(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t s) {
+  if (p->nd) {
+    if (p->div_mode == 0) return mat_run_nd<T, 0>(p, a, adj, loss, s);
+    if (p->div_mode == 2) return mat_run_nd<T, 2>(p, a, adj, loss, s);
+    return mat_run_nd<T, 1>(p, a, adj, loss, s);
+  }
   if (p->div_mode == 0) return mat_run<T, 0>(p, a, adj, loss, s);
   if (p->div_mode == 2) return mat_run<T, 2>(p, a, adj, loss, s);
   return mat_run<T, 1>(p, a, adj, loss, s);
@@ -1086,7 +1382,8 @@ namespace {
 int plan_div_mode(const tdx_mat_desc *d) {
   if (d->exact_div == 0) return 0;
   if (d->dtype == 0 && d->exact_div == 1 && quotient_step_exact((float)d->h[1]) &&
-      (d->n0 == 1 || quotient_step_exact((float)d->h[0])))
+      (d->n0 == 1 || quotient_step_exact((float)d->h[0])) &&
+      (d->n2 == 1 || quotient_step_exact((float)d->h[2])))
     return 2;
   return 1;
 }
@@ -1094,7 +1391,8 @@ int plan_div_mode(const tdx_mat_desc *d) {
 // The residual program of a float 2-D descriptor as a generator spec, with
 // the tables' ranges checked.  Returns false with the reason in *why.
 bool jit_spec_from_desc(const tdx_mat_desc *d, int div_mode, JitSpec *sp, std::string *why) {
-  if (d->dtype != 0 || d->n0 < 2 || d->n1 < 2 || d->dp < 2 || d->dp > MMAX_DP || d->n_eq < 1 ||
+  if (d->dtype != 0 || d->n0 < 2 || d->n1 < 2 || d->n2 != 1 || d->dp < 2 || d->dp > MMAX_DP ||
+      d->n_eq < 1 ||
       d->n_eq > MMAX_EQ) {
     *why = "not a float 2-D grid";
     return false;
@@ -1189,7 +1487,7 @@ bool jit_spec_from_desc(const tdx_mat_desc *d, int div_mode, JitSpec *sp, std::s
 // generated kernel evaluates; the general kernel keeps the other tiles.
 // Returns an error only when specialization was required (specialize = 1).
 int mat_specialize(tdx_mat_plan *p, const tdx_mat_desc *d) {
-  if (d->dtype != 0 || d->n0 == 1 || d->specialize < 0) return TDX_OK;
+  if (d->dtype != 0 || d->n0 == 1 || d->n2 != 1 || d->specialize < 0) return TDX_OK;
   const bool required = d->specialize > 0;
   const int T0 = p->T0, T1 = p->T1, R0 = p->R0, R1 = p->R1;
   // tile t along an axis is interior when t*T >= 2R and t*T + T + 2R <= n
@@ -1294,6 +1592,78 @@ int mat_specialize(tdx_mat_plan *p, const tdx_mat_desc *d) {
   return TDX_OK;
 }
 
+// Full-grid plan set-up: factor slots of the boundary operators (one set
+// per operator term table, shared by the entries that use it), each bc
+// entry's first slot, the scatter CSR of the boundary seeds (keys: field
+// word slot * N + pos, then n_slots * N + gradient word; the seed order of
+// k_mat_bc), the residual grid.
+int nd_setup(tdx_mat_plan *p, const tdx_mat_desc *d) {
+  const int64_t N = p->N;
+  std::map<int, int> first_slot;
+  std::vector<int32_t> bc_slot(std::max(d->n_bc, 1), -1);
+  for (int e = 0; e < d->n_bc; ++e) {
+    const int32_t *B = d->ints + d->bc_i + e * TDX_MBC_INTS;
+    if (B[2] == 0) continue;
+    auto it = first_slot.find(B[3]);
+    if (it == first_slot.end()) {
+      const int base = (int)p->chains.size();
+      for (int t = 0; t < B[2]; ++t) {
+        const int32_t *Tm = d->ints + B[3] + t * TDX_MTERM_INTS;
+        for (int f = 0; f < Tm[0]; ++f) {
+          const int32_t *F = d->ints + Tm[1] + f * TDX_MFACTOR_INTS;
+          std::vector<int> ch{F[0]};
+          for (int k = 0; k < F[4]; ++k) ch.push_back(F[5 + k]);
+          p->chains.push_back(ch);
+        }
+      }
+      it = first_slot.emplace(B[3], base).first;
+    }
+    bc_slot[e] = it->second;
+  }
+  p->n_slots = (int)p->chains.size();
+  const int64_t allw = (int64_t)p->n_slots * N;
+  std::map<int64_t, std::vector<int32_t>> rows;
+  for (int e = 0; e < d->n_bc; ++e) {
+    const int32_t *B = d->ints + d->bc_i + e * TDX_MBC_INTS;
+    int sidx = B[8];
+    const int var = B[4];
+    if (B[9] > 0) rows[allw + var * N + d->ints[B[1]]].push_back(sidx++);
+    for (int s = 0; s < B[0]; ++s) {
+      const int pos = d->ints[B[1] + s];
+      if (B[2] == 0) {
+        rows[allw + var * N + pos].push_back(sidx++);
+        continue;
+      }
+      int k = 0;
+      for (int t = 0; t < B[2]; ++t) {
+        const int32_t *Tm = d->ints + B[3] + t * TDX_MTERM_INTS;
+        for (int f = 0; f < Tm[0]; ++f, ++k) rows[(int64_t)(bc_slot[e] + k) * N + pos].push_back(sidx++);
+      }
+    }
+  }
+  std::vector<int32_t> ptr{0}, idx;
+  std::vector<int64_t> key;
+  p->sc_rows_f = 0;
+  for (const auto &kv : rows) {
+    key.push_back(kv.first);
+    idx.insert(idx.end(), kv.second.begin(), kv.second.end());
+    ptr.push_back((int32_t)idx.size());
+    if (kv.first < allw) ++p->sc_rows_f;
+  }
+  p->sc_rows = (int)key.size();
+  if (idx.empty()) idx.push_back(0);
+  if (key.empty()) key.push_back(0);
+  int rc = p->bc_slot_tab.upload(bc_slot.data(), bc_slot.size() * 4);
+  if (!rc) rc = p->sc_ptr.upload(ptr.data(), ptr.size() * 4);
+  if (!rc) rc = p->sc_key.upload(key.data(), key.size() * 8);
+  if (!rc) rc = p->sc_idx.upload(idx.data(), idx.size() * 4);
+  if (rc) return rc;
+  p->n_nd_wg = (int)std::max<int64_t>(1, std::min<int64_t>((N + MAT_THREADS - 1) / MAT_THREADS, 2048));
+  p->n_gen_wg = p->n_nd_wg;
+  p->n_tile_wg = 0;
+  return TDX_OK;
+}
+
 }  // namespace
 
 extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
@@ -1301,7 +1671,8 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   *out = nullptr;
   TDX_REQUIRE(d->dtype == 0 || d->dtype == 1, "dtype must be 0 or 1");
   TDX_REQUIRE(d->n_eq >= 1 && d->n_eq <= MMAX_EQ, "n_eq must be 1..%d", MMAX_EQ);
-  TDX_REQUIRE(d->n0 >= 1 && d->n1 >= 2, "bad grid shape");
+  TDX_REQUIRE(d->n0 >= 1 && d->n1 >= 2 && d->n2 >= 1, "bad grid shape");
+  TDX_REQUIRE(d->n2 == 1 || d->n0 >= 2, "a 3-D grid needs axis 0 of length >= 2");
   TDX_REQUIRE(d->dp >= 2 && d->dp <= MMAX_DP, "derivative_points must be 2..%d", MMAX_DP);
   TDX_REQUIRE(d->tile_rows >= 0 && d->tile_rows <= 1024, "tile_rows must be 0..1024");
   TDX_REQUIRE(d->exact_div >= 0 && d->exact_div <= 2, "exact_div must be 0, 1 or 2");
@@ -1309,16 +1680,19 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   auto in_ints = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_ints; };
   auto in_reals = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_reals; };
   auto in_data = [&](int64_t off, int64_t n) { return off >= 0 && n >= 0 && off + n <= d->n_data; };
-  const int64_t N = (int64_t)d->n0 * d->n1;
+  const int64_t N = (int64_t)d->n0 * d->n1 * d->n2;
   TDX_REQUIRE(N * d->n_eq < (1LL << 31), "grid too large");
+  const int n_axes = d->n2 > 1 ? 3 : 2;
+  const int ext[3] = {d->n0, d->n1, d->n2};
 
   tdx_mat_plan *p = new tdx_mat_plan();
   p->d = *d;
+  p->N = N;
   for (int i = 0; i < d->dp; ++i) {
     p->ab[i] = d->alpha_back[i];
     p->af[i] = d->alpha_forw[i];
   }
-  int cnt0 = 0, cnt1 = 0;
+  int cnt0 = 0, cnt1 = 0, maxc = 0;
   p->n_fs = 0;
   std::vector<int32_t> fs_rec;
   // flattened residual program (see MatArgs)
@@ -1365,18 +1739,20 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
         const int32_t *F = d->ints + Tm[1] + f * TDX_MFACTOR_INTS;
         if (F[0] < 0 || F[0] >= d->n_eq) return fail("factor variable out of range");
         if (F[1] != 0 && !in_reals(F[3], 1)) return fail("power out of range");
-        if (F[4] < 0 || F[4] > 3) return fail("derivative order must be 0..3");
+        if (F[4] < 0 || F[4] > TDX_MAT_MAX_CHAIN)
+          return fail("derivative order must be 0..8 per factor");
         if (p->n_fs >= MMAX_FS) return fail("too many factors");
         int c0 = 0, c1 = 0;
         for (int k = 0; k < F[4]; ++k) {
-          if (F[5 + k] != 0 && F[5 + k] != 1) return fail("axis out of range");
-          if (F[5 + k] == 0) ++c0; else ++c1;
+          const int ax = F[5 + k];
+          if (ax < 0 || ax >= n_axes) return fail("axis out of range");
+          if (ext[ax] == 1) return fail("derivative along a length-1 axis");
+          if (ext[ax] < 2 * d->dp - 1) return fail("axis too short for the stencil");
+          if (ax == 0) ++c0; else ++c1;
         }
-        if (d->n0 == 1 && c0 > 0) return fail("derivative along a length-1 axis");
-        if (c0 > 0 && d->n0 < 2 * d->dp - 1) return fail("axis 0 too short for the stencil");
-        if (c1 > 0 && d->n1 < 2 * d->dp - 1) return fail("axis 1 too short for the stencil");
         cnt0 = std::max(cnt0, c0);
         cnt1 = std::max(cnt1, c1);
+        maxc = std::max(maxc, (int)F[4]);
         prog_f.push_back(F[1]);
         prog_f.push_back(F[2]);
         if (F[1] != 0) {
@@ -1387,7 +1763,10 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
         }
         fs_rec.push_back(F[0]);
         fs_rec.push_back(F[4]);
-        for (int k = 0; k < 3; ++k) fs_rec.push_back(k < F[4] ? F[5 + k] : 0);
+        for (int k = 0; k < TDX_MAT_MAX_CHAIN; ++k) fs_rec.push_back(k < F[4] ? F[5 + k] : 0);
+        std::vector<int> ch{F[0]};
+        for (int k = 0; k < F[4]; ++k) ch.push_back(F[5 + k]);
+        p->chains.push_back(ch);
         ++p->n_fs;
       }
     }
@@ -1413,13 +1792,15 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
             return fail("bc coefficient out of range");
           for (int f = 0; f < Tm[0]; ++f) {
             const int32_t *F = d->ints + Tm[1] + f * TDX_MFACTOR_INTS;
-            if (F[0] < 0 || F[0] >= d->n_eq || F[4] < 0 || F[4] > 3) return fail("bc factor");
+            if (F[0] < 0 || F[0] >= d->n_eq || F[4] < 0 || F[4] > TDX_MAT_MAX_CHAIN)
+              return fail("bc factor");
             if (F[1] != 0 && !in_reals(F[3], 1)) return fail("bc power");
             for (int k = 0; k < F[4]; ++k) {
-              if (F[5 + k] != 0 && F[5 + k] != 1) return fail("bc axis");
-              if (F[5 + k] == 0 && d->n0 < 2 * d->dp - 1) return fail("bc axis 0 too short");
-              if (F[5 + k] == 1 && d->n1 < 2 * d->dp - 1) return fail("bc axis 1 too short");
+              const int ax = F[5 + k];
+              if (ax < 0 || ax >= n_axes || ext[ax] == 1) return fail("bc axis");
+              if (ext[ax] < 2 * d->dp - 1) return fail("bc axis too short for the stencil");
             }
+            maxc = std::max(maxc, (int)F[4]);
           }
           nf += Tm[0];
         }
@@ -1476,7 +1857,10 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
     return MAT_THREADS >> p->tpr_log;  // tile rows per sweep of the workgroup
   };
   auto min_rows = [&](int rps) { return (4 * p->R0 + 8 + rps - 1) / rps * rps; };
-  if (d->n0 == 1) {
+  p->nd = (d->n2 > 1 || maxc > 3) ? 1 : 0;
+  if (p->nd) {
+    p->E0 = p->E1 = 1;
+  } else if (d->n0 == 1) {
     p->E0 = 1;
     p->E1 = 1024;
     if (p->E1 - 4 * p->R1 < 256) p->E1 = 2048;
@@ -1503,8 +1887,12 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   }
   p->T0 = p->E0 - 4 * p->R0;
   p->T1 = p->E1 - 4 * p->R1;
-  if (p->T0 < 1 || p->T1 < 1 || lds_for(p->E0, p->E1) > 160 * 1024)
-    return fail("stencil reach too large for LDS");
+  // a reach that no LDS tile holds: the full-grid passes serve the plan
+  if (!p->nd && (p->T0 < 1 || p->T1 < 1 || lds_for(p->E0, p->E1) > 160 * 1024)) p->nd = 1;
+  if (p->nd) {
+    p->E0 = p->E1 = p->T0 = p->T1 = 1;
+    p->R0 = p->R1 = 0;
+  }
   p->div_mode = plan_div_mode(d);
   p->tile_lds = lds_for(p->E0, p->E1);
   p->tile_lds_fwd = p->tile_lds;
@@ -1521,12 +1909,13 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   for (int i = 0; i < d->dp; ++i) {
     coef[i] = p->ab[i];
     coef[MMAX_DP + i] = p->af[i];
-    for (int ax = 0; ax < 2; ++ax) {
+    for (int ax = 0; ax < 3; ++ax) {
       double *c = coef.data() + MCOEF_AX + ax * 4 * MMAX_DP;
-      c[i] = p->ab[i] / (2.0 * d->h[ax]);
-      c[MMAX_DP + i] = p->af[i] / (2.0 * d->h[ax]);
-      c[2 * MMAX_DP + i] = p->ab[i] / d->h[ax];
-      c[3 * MMAX_DP + i] = p->af[i] / d->h[ax];
+      const double h = d->h[ax] > 0 ? d->h[ax] : 1.0;  // unused axes: any finite value
+      c[i] = p->ab[i] / (2.0 * h);
+      c[MMAX_DP + i] = p->af[i] / (2.0 * h);
+      c[2 * MMAX_DP + i] = p->ab[i] / h;
+      c[3 * MMAX_DP + i] = p->af[i] / h;
     }
   }
   std::vector<float> coef_f(coef.begin(), coef.end());
@@ -1548,7 +1937,7 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
     tdx_mat_plan_destroy(p);
     return rc;
   }
-  rc = mat_specialize(p, d);
+  rc = p->nd ? nd_setup(p, d) : mat_specialize(p, d);
   if (rc) {
     tdx_mat_plan_destroy(p);
     return rc;
@@ -1572,6 +1961,12 @@ extern "C" int tdx_mat_plan_create(const tdx_mat_desc *d, tdx_mat_plan **out) {
   off = align_up(off + (size_t)N * d->n_eq * ts, 256);
   p->off_adam = off;
   off = align_up(off + 8 * ts, 256);
+  if (p->nd) {
+    p->off_fields = off;
+    off = align_up(off + (size_t)p->n_slots * N * ts, 256);
+    p->off_tmp = off;
+    off = align_up(off + 2 * (size_t)N * ts, 256);
+  }
   p->ws_bytes = off;
   *out = p;
   return TDX_OK;
@@ -1587,6 +1982,10 @@ extern "C" int64_t tdx_quotient_check(float h, int mode) {
 
 extern "C" void tdx_mat_plan_destroy(tdx_mat_plan *p) {
   if (!p) return;
+  p->bc_slot_tab.release();
+  p->sc_ptr.release();
+  p->sc_key.release();
+  p->sc_idx.release();
   p->ints.release();
   p->gen_list.release();
   p->reals.release();
@@ -1710,7 +2109,7 @@ extern "C" int tdx_mat_train_step(const tdx_mat_plan *p, void *u, void *m, void 
                                   void *loss, void *ws, tdx_stream_t stream) {
   TDX_REQUIRE(p && u && m && v && step && ws, "null argument");
   hipStream_t s = (hipStream_t)stream;
-  const int64_t n = (int64_t)p->d.n_eq * p->d.n0 * p->d.n1;
+  const int64_t n = (int64_t)p->d.n_eq * p->N;
   int rc;
   if (p->d.dtype == 0) {
     MatArgs<float> a = mat_args<float>(p, u, ws);

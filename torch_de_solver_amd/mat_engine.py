@@ -54,17 +54,27 @@ class MatTables:
         data_parts: List[np.ndarray] = []
         self.n_data = 0
         shape = prob.shape
+        self.n2 = 1
         if len(shape) == 1:
             self.n0, self.n1 = 1, shape[0]
             axis_map = {0: 1}
-            self.h = [1.0, prob.h[0]]
+            self.h = [1.0, prob.h[0], 0.0]
         elif len(shape) == 2:
             self.n0, self.n1 = shape
             axis_map = {0: 0, 1: 1}
-            self.h = [prob.h[0], prob.h[1]]
+            self.h = [prob.h[0], prob.h[1], 0.0]
+        elif len(shape) == 3:
+            self.n0, self.n1, self.n2 = shape
+            axis_map = {0: 0, 1: 1, 2: 2}
+            self.h = [prob.h[0], prob.h[1], prob.h[2]]
         else:
-            raise NotImplementedError("the native 'mat' engine handles 1-D and 2-D grids")
-        N = self.n0 * self.n1
+            raise NotImplementedError("the native 'mat' engine handles grids of 1 to 3 axes")
+        N = self.n0 * self.n1 * self.n2
+        chains = [len(f.axes) for terms in prob.equations for t in terms for f in t.factors]
+        chains += [len(f.axes) for bc in prob.bcs for t in (bc.terms or []) for f in t.factors]
+        # plans of 3-D grids or longer chains run full-grid passes, which take
+        # the boundary seeds without the transposed-stencil table
+        self.full_grid = len(shape) == 3 or max(chains, default=0) > 3
         self.N = N
 
         def add_data(arr) -> int:
@@ -85,11 +95,12 @@ class MatTables:
                 factor_i = len(ints)
                 for f in t.factors:
                     ax = [axis_map[a] for a in f.axes]
-                    if len(ax) > 3:
-                        raise NotImplementedError("derivative order above 3 in 'mat' mode")
+                    if len(ax) > native.TDX_MAT_MAX_CHAIN:
+                        raise NotImplementedError(
+                            f"derivative order above {native.TDX_MAT_MAX_CHAIN} in 'mat' mode")
                     pow_r = add_real(f.pow_val)
                     ints.extend([f.var, f.pow_kind, int(f.pow_val) if f.pow_kind == 0 else 0,
-                                 pow_r, len(ax)] + ax + [0] * (3 - len(ax)))
+                                 pow_r, len(ax)] + ax + [0] * (native.TDX_MAT_MAX_CHAIN - len(ax)))
                 if t.coeff_field is not None:
                     ck, ci = 1, add_data(np.asarray(t.coeff_field).reshape(-1))
                 else:
@@ -137,22 +148,25 @@ class MatTables:
                     ints.extend([len(betas), add_real(np_dtype(alpha))])
                     for b in betas:
                         add_real(np_dtype(b))
-                    csr.setdefault(bc.var * N + sets[0], []).append((seed, 1.0))
+                    if not self.full_grid:
+                        csr.setdefault(bc.var * N + sets[0], []).append((seed, 1.0))
                     seed += 1
                 for s, pos in enumerate(sets):
                     g0, g1 = divmod(pos, self.n1)
                     if n_terms == 0:
-                        csr.setdefault(bc.var * N + pos, []).append((seed, 1.0))
+                        if not self.full_grid:
+                            csr.setdefault(bc.var * N + pos, []).append((seed, 1.0))
                         seed += 1
                         continue
                     for t in bc.terms:
                         for f in t.factors:
-                            ax = [axis_map[a] for a in f.axes]
-                            fn = _chain_functional(g0, g1, ax, (self.n0, self.n1), prob.dp,
-                                                   self.h, ab, af)
-                            for j, w in fn.items():
-                                if w != 0.0:
-                                    csr.setdefault(f.var * N + j, []).append((seed, w))
+                            if not self.full_grid:
+                                ax = [axis_map[a] for a in f.axes]
+                                fn = _chain_functional(g0, g1, ax, (self.n0, self.n1), prob.dp,
+                                                       self.h, ab, af)
+                                for j, w in fn.items():
+                                    if w != 0.0:
+                                        csr.setdefault(f.var * N + j, []).append((seed, w))
                             seed += 1
                 entries.append([len(sets), pos_i, n_terms, term_i, bc.var, target_r, lam_r,
                                 inv_r, seed_i, robin_rec])
@@ -196,9 +210,9 @@ class MatTables:
         d = native.MatDesc()
         d.dtype = self.dtype_code
         d.n_eq = self.n_eq
-        d.n0, d.n1 = self.n0, self.n1
+        d.n0, d.n1, d.n2 = self.n0, self.n1, self.n2
         d.dp = self.dp
-        d.h[0], d.h[1] = self.h[0], self.h[1]
+        d.h[0], d.h[1], d.h[2] = self.h[0], self.h[1], self.h[2]
         d.alpha_back = self.ab.ctypes.data_as(native.c_dp)
         d.alpha_forw = self.af.ctypes.data_as(native.c_dp)
         d.eq_i = self.eq_i

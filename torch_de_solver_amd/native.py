@@ -20,7 +20,8 @@ TDX_FACTOR_INTS = 6
 TDX_STENCIL_INTS = 2
 TDX_MEQ_INTS = 3
 TDX_MTERM_INTS = 4
-TDX_MFACTOR_INTS = 8
+TDX_MAT_MAX_CHAIN = 8
+TDX_MFACTOR_INTS = 5 + TDX_MAT_MAX_CHAIN
 TDX_MBC_INTS = 10
 
 EXPORTS = [
@@ -72,7 +73,8 @@ class NNDesc(ctypes.Structure):
 class MatDesc(ctypes.Structure):
     _fields_ = [
         ('dtype', ctypes.c_int), ('n_eq', ctypes.c_int), ('n0', ctypes.c_int),
-        ('n1', ctypes.c_int), ('dp', ctypes.c_int), ('h', ctypes.c_double * 2),
+        ('n1', ctypes.c_int), ('n2', ctypes.c_int), ('dp', ctypes.c_int),
+        ('h', ctypes.c_double * 3),
         ('alpha_back', c_dp), ('alpha_forw', c_dp),
         ('eq_i', ctypes.c_int), ('n_bc', ctypes.c_int), ('bc_i', ctypes.c_int),
         ('n_bc_seeds', ctypes.c_int), ('n_touch', ctypes.c_int),

@@ -74,19 +74,40 @@ class Optimizer:
 
 
 class Closure:
-    """zero_grad -> evaluate -> backward (tedeous/optimizers/closure.py:49-64)."""
+    """zero_grad -> evaluate -> backward (tedeous/optimizers/closure.py:49-64).
+
+    ``mixed_precision=True`` keeps the reference's control flow on a GPU
+    (tedeous/optimizers/closure.py:17-64): the evaluation under
+    ``torch.autocast`` and a ``GradScaler`` that scales the loss, steps the
+    optimizer inside the closure and updates its scale -- so, as in the
+    reference, the optimizer's own ``step(closure)`` then applies a second
+    update with the same gradient.  The kernels stay in FP32: autocast
+    would lower the reference's network products to FP16, and nothing here
+    is an autocast-eligible torch op.
+    """
 
     def __init__(self, mixed_precision: bool, model):
-        if mixed_precision:
-            raise NotImplementedError("mixed precision is not served: stencils stay in FP32")
         self.model = model
         self.optimizer = model.optimizer
         self.normalized_loss_stop = model.normalized_loss_stop
+        self.mixed_precision = bool(mixed_precision)
+        self.scaler = None
+        if self.mixed_precision:
+            if self.optimizer.__class__.__name__ == "LBFGS":
+                raise NotImplementedError("AMP and the LBFGS optimizer are not compatible.")
+            self.scaler = torch.amp.GradScaler('cuda', enabled=True)
 
     def _closure(self):
         self.optimizer.zero_grad()
-        loss, loss_normalized = self.model.solution_cls.evaluate()
-        loss.backward()
+        with torch.autocast(device_type='cuda', dtype=torch.float16,
+                            enabled=self.mixed_precision):
+            loss, loss_normalized = self.model.solution_cls.evaluate()
+        if self.scaler is not None:
+            self.scaler.scale(loss).backward()
+            self.scaler.step(self.optimizer)
+            self.scaler.update()
+        else:
+            loss.backward()
         self.model.cur_loss = loss_normalized if self.normalized_loss_stop else loss
         return loss
 
