@@ -1202,17 +1202,28 @@ __device__ __forceinline__ void mma_layer_fm(const T *img, int lane, const T *sr
   typedef typename V4<T>::type v4;
   const int q = lane >> 4;
   const v4 *wp = reinterpret_cast<const v4 *>(img) + lane;
-#pragma unroll 1
-  for (int mb = 0; mb < NB; ++mb) {
+  // two input blocks per round, the second block's weights loaded with the
+  // first's: their latency is paid once per pair
+  auto block = [&](int mb, const v4 (&w)[NB]) {
     T b[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) b[i] = src[(16 * mb + crow<T>(q, i)) * CTRS + trow];
 #pragma unroll
-    for (int ob = 0; ob < NB; ++ob) {
-      const v4 w = wp[(ob * NB + mb) * 64];
+    for (int ob = 0; ob < NB; ++ob)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) acc[ob] = mma16(w[i], b[i], acc[ob]);
+      for (int i = 0; i < 4; ++i) acc[ob] = mma16(w[ob][i], b[i], acc[ob]);
+  };
+#pragma unroll 1
+  for (int mb = 0; mb < NB; mb += 2) {
+    v4 wa[NB], wb[NB];
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) wa[ob] = wp[(ob * NB + mb) * 64];
+    if (mb + 1 < NB) {
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob) wb[ob] = wp[(ob * NB + mb + 1) * 64];
     }
+    block(mb, wa);
+    if (mb + 1 < NB) block(mb + 1, wb);
   }
 }
 
