@@ -296,13 +296,30 @@ def bench_nn(args):
     prof = eng.profile_read()
     eng.profile(0)
     F = mlp_flops_per_row(arch)
+    # the fused step (k_fused: forward and adjoint in one kernel) launches no
+    # separate forward kernel; its kernel is profiled under 'backward'
+    fused = prof['forward'][1] == 0
     bwd_ms = prof['backward'][0] / max(prof['backward'][1], 1)
-    fwd_ms = prof['forward'][0] / max(prof['forward'][1], 1)
+    fwd_ms = prof['forward'][0] / max(prof['forward'][1], 1) if not fused else None
     upd_ms = prof['update'][0] / max(prof['update'][1], 1)
-    bwd_flops = 2.0 * F * n_rows
+    # algorithmic FLOPs: F per row forward, 2F per row adjoint (a recompute
+    # of the forward inside the adjoint is not counted)
+    bwd_flops = (3.0 if fused else 2.0) * F * n_rows
     fwd_flops = 1.0 * F * n_rows
     achieved = bwd_flops / (bwd_ms * 1e-3)
-    log(f"[bench] kernels: forward {fwd_ms:.3f} ms, backward {bwd_ms:.3f} ms, update {upd_ms:.3f} ms")
+    if fused:
+        log(f"[bench] kernels: fused forward+adjoint {bwd_ms:.3f} ms, update {upd_ms:.3f} ms")
+    else:
+        log(f"[bench] kernels: forward {fwd_ms:.3f} ms, backward {bwd_ms:.3f} ms, "
+            f"update {upd_ms:.3f} ms")
+    kern = {'fused-generated': 'tdx_fused_jit', 'fused': 'k_fused'}.get(eng.step_kernel,
+                                                                         'k_backward_w')
+    kname = {'tdx_fused_jit': "tdx_fused_jit (forward + generated residual programs + adjoint "
+                              "+ weight gradients, compiled at plan creation)",
+             'k_fused': "k_fused (forward + interpreted residual programs + adjoint + weight "
+                        "gradients)",
+             'k_backward_w': "k_backward (recompute + adjoint + weight gradients)"}[kern]
+    tname = f"{kern}_{args.workload}"
     out = {
         "metric": "train steps/s & collocation pts/s, fused+adjoint vs plain GPU, at equal rel-L2",
         "value": 1e3 / ms,
@@ -325,17 +342,17 @@ def bench_nn(args):
                    "autotune": solver.tuning},
         "colloc_pts_per_s": colloc * 1e3 / ms,
         "loss_sum_after": loss_now,
-        "roofline": {"bound": "mfma", "kernel": "k_backward (recompute + adjoint + weight gradients)",
+        "roofline": {"bound": "mfma", "kernel": kname,
                      "achieved": achieved / 1e12, "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP32_MFMA_PEAK,
-                     "traffic": profiled_traffic(f"k_backward_w_{args.workload}")
-                     if args.workload == 'sweep' else None,
-                     "traffic_source": "profiles/k_backward_w_sweep_traffic.txt (rocprofv3 "
+                     "traffic": profiled_traffic(tname) if args.workload == 'sweep' else None,
+                     "traffic_source": f"profiles/{tname}_traffic.txt (rocprofv3 "
                                        "FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                      "algorithmic_flops_per_launch": bwd_flops,
                      "avg_launch_ms": bwd_ms},
-        "kernel_ms": {"forward": fwd_ms, "backward": bwd_ms, "reduce_adam": upd_ms,
-                      "forward_tflops": fwd_flops / (fwd_ms * 1e-3) / 1e12},
+        "kernel_ms": ({"fused": bwd_ms, "reduce_adam": upd_ms} if fused else
+                      {"forward": fwd_ms, "backward": bwd_ms, "reduce_adam": upd_ms,
+                       "forward_tflops": fwd_flops / (fwd_ms * 1e-3) / 1e12}),
     }
     if not args.no_plain:
         plain_ms, n_sel, scale = plain_nn_step_ms(problems, nets0, args.plain_steps,
@@ -735,10 +752,13 @@ def main():
                     help="'mat': kernel generated for the program on the interior (0 auto, "
                          "1 always, -1 never)")
     ap.add_argument('--persistent', type=int, default=0,
-                    help="'NN': 1 runs the forward kernel as a persistent grid (three "
-                         "launches per fused step instead of four)")
+                    help="'NN': 0 fuses forward and adjoint into one kernel where the "
+                         "network allows it (three launches per step); -1 keeps separate "
+                         "forward and adjoint kernels (four launches); 1 runs the separate "
+                         "forward as a persistent grid (three launches)")
     ap.add_argument('--chunk-tiles', type=int, default=0,
-                    help="'NN': backward tiles per work item (0: autotuned)")
+                    help="'NN': work per backward item: 64-row tiles (separate kernels) or "
+                         "64-entry blocks (fused kernel) (0: autotuned)")
     ap.add_argument('--nn-tile-rows', type=int, default=0,
                     help="'NN': backward tile rows (32: two tiles per wavefront iteration)")
     ap.add_argument('--calibrate', action='store_true',

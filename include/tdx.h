@@ -104,7 +104,9 @@ typedef struct tdx_nn_desc {
   int tile_rows;    /* backward tile rows (0: default; -1: workgroup-cooperative kernel;
                        32: two 16-row tiles per wavefront iteration of the
                        wavefront-private kernel (float), otherwise one) */
-  int chunk_tiles;  /* tiles per backward work item (0: default) */
+  int chunk_tiles;  /* work per gradient work item (0: default): 64-row tiles of the
+                       separate adjoint kernel, or 64-entry blocks of the fused
+                       training-step kernel (default 4) */
   /* Causal loss (tedeous/losses.py:137-182), one instance: tol != 0 replaces
    * the operator term of group 0's equations (loss and normalized weights 0,
    * seed weight 1/N) by mean(w * res), res[p] = sum_eq r_eq[p]^2 over the
@@ -126,8 +128,19 @@ typedef struct tdx_nn_desc {
   int weak_runs;
   int64_t weak_d;
   int weak_i;
-  /* Persistent forward (1: on where the plan allows it; 0 or -1: one
-   * workgroup per work item): when the network's forward image fits in LDS
+  /* Training-step kernels.  0: the fused step where the plan allows it --
+   * float networks of at most 32 hidden features, at most two H x H layers,
+   * one output, at most two inputs, at most 8 slots per group, default
+   * tile_rows, no causal or weak-form loss, every group's residual program
+   * within 512 record and 256 real words: one kernel evaluates the network,
+   * the residual program and the adjoint of 16-entry rounds with every
+   * activation kept in registers (no forward recompute, no seeds through
+   * HBM), so a step is three launches (packing, fused kernel, reduction with
+   * the Adam update and the loss pairs); the loss pairs are the same bits
+   * as with separate kernels, the gradient the same per-row terms summed in
+   * another order.  Otherwise, and with -1, separate forward and adjoint
+   * kernels, one workgroup per work item.  1: separate kernels with the
+   * persistent forward: when the network's forward image fits in LDS
    * and the plan has no causal or weak-form loss, the forward kernel runs
    * as a grid of CUs x resident workgroups, each walking a contiguous range
    * of equal cost of the (instance, entry tile) work items and gathering
@@ -135,8 +148,8 @@ typedef struct tdx_nn_desc {
    * changes; the workgroup holding an instance's first work item also
    * writes the packed image the backward kernel reads, so a fused step is
    * three launches (forward, backward, reduction with the Adam update and
-   * the loss pairs) instead of four (packing first).  Results are the same
-   * bits either way. */
+   * the loss pairs) instead of four (packing first), with the same result
+   * bits as -1. */
   int persistent;
 } tdx_nn_desc;
 
@@ -224,6 +237,22 @@ int tdx_nn_train_step_noalloc(const tdx_nn_plan *plan, void *params, void *exp_a
  * resets the counts).  capacity 0 turns timing off.  Not for use under
  * graph capture.  No counterpart in the reference (measurement only). */
 #define TDX_NN_KERNEL_KINDS 3
+/* Source of the fused training-step kernel that a plan of `desc` compiles
+ * at creation with its generated residual programs (float networks the
+ * fused step serves; see tdx_nn_desc.persistent), copied into buf (cap
+ * bytes, NUL-terminated; buf may be NULL); with compile != 0 also compiled
+ * by hipRTC for gfx950 (no device needed).  Returns the source length, 0
+ * when the plan has no fused step (reason in tdx_last_error) or a negative
+ * error code.  Plans generate the programs when they hold at least 2^20
+ * network rows, or always / never with the environment variable
+ * TDX_NN_JIT=1 / 0, and fall back to the interpreted fused kernel if
+ * compiling fails.  Diagnostic; no counterpart in the reference. */
+int64_t tdx_nn_fused_source(const tdx_nn_desc *desc, char *buf, int64_t cap, int compile);
+/* Training-step kernels of a plan: 0 separate forward and adjoint kernels,
+ * 1 the fused kernel interpreting the residual programs, 2 the fused kernel
+ * compiled with the plan's generated residual programs; -1 for NULL.
+ * Diagnostic; no counterpart in the reference. */
+int tdx_nn_step_kernel(const tdx_nn_plan *plan);
 int tdx_nn_profile(tdx_nn_plan *plan, int capacity);
 int tdx_nn_profile_read(tdx_nn_plan *plan, double *ms, int *count);
 

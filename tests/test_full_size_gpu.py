@@ -308,9 +308,12 @@ def test_sweep_all_instances_float64_against_oracle():
 
 def test_sweep_grouped_step_bitwise_deterministic():
     """Two runs of the fused float32 grouped step (forward, adjoint, Adam of
-    all 256 instances) from the same state give the same bits, and so does
-    the plan with the persistent forward (three launches per step instead
-    of four)."""
+    all 256 instances in three launches: packing, k_fused, reduction) from
+    the same state give the same bits; so do the two plans with separate
+    forward and adjoint kernels (four launches, and three with the
+    persistent forward), and the loss pairs of all three plans agree bit
+    for bit (the fused kernel sums the same per-row gradient terms in
+    another order, which tests/test_fused_gpu.py bounds)."""
     insts = W.sweep_instances(256)
     probs, nets = [], []
     for inst in insts:
@@ -320,18 +323,22 @@ def test_sweep_grouped_step_bitwise_deterministic():
                                 lambda_bound=ckw['lambda_bound']))
         nets.append(net)
     eng = NNEngine(probs, mlp_arch(nets[0]), DEV)
+    sep = NNEngine(probs, mlp_arch(nets[0]), DEV, persistent=-1)
     persist = NNEngine(probs, mlp_arch(nets[0]), DEV, persistent=1)
     flat0 = torch.cat([torch.cat([p.detach().reshape(-1) for p in n.parameters()])
                        for n in nets]).to(DEV)
     outs = []
-    for eng in (eng, eng, persist):
+    for e in (eng, eng, sep, persist):
         p = flat0.clone()
         m, v = torch.zeros_like(p), torch.zeros_like(p)
         step = torch.zeros(1, dtype=torch.int64, device=DEV)
         loss = torch.zeros(2 * len(probs), dtype=torch.float32, device=DEV)
-        for _ in range(3):
-            eng.train_step(p, m, v, step, 1e-3, 0.9, 0.999, 1e-8, loss)
-        outs.append((p.cpu(), loss.cpu()))
-    for k in (1, 2):
-        assert torch.equal(outs[0][0], outs[k][0])
-        assert torch.equal(outs[0][1], outs[k][1])
+        first = None
+        for k in range(3):
+            e.train_step(p, m, v, step, 1e-3, 0.9, 0.999, 1e-8, loss)
+            if k == 0:
+                first = loss.clone()
+        outs.append((p.cpu(), loss.cpu(), first.cpu()))
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    assert torch.equal(outs[2][0], outs[3][0]) and torch.equal(outs[2][1], outs[3][1])
+    assert torch.equal(outs[0][2], outs[2][2])

@@ -264,6 +264,53 @@ def test_specialized_kernel_not_for_1d_or_double():
     assert MatTables(prob, np.float32).specialized_source(1) == ''
 
 
+def _sweep_tables(idx, dtype=np.float32):
+    from torch_de_solver_amd import workloads as W
+    from torch_de_solver_amd.compiler import compile_nn
+    from torch_de_solver_amd.nn_engine import NNTables, mlp_arch
+    insts = W.sweep_instances(256)
+    probs, nets = [], []
+    for i in idx:
+        d, eq, b, net, ckw = W.sweep_problem(insts[i])
+        probs.append(compile_nn(d.build('NN'), eq.equation_lst, b.build(d.variable_dict), 1,
+                                h=ckw['h'], lambda_operator=ckw['lambda_operator'],
+                                lambda_bound=ckw['lambda_bound']))
+        nets.append(net)
+    return NNTables(probs, mlp_arch(nets[0]), dtype), mlp_arch(nets[0])
+
+
+def test_fused_step_kernel_generates_and_compiles():
+    """The fused 'NN' kernel with residual programs generated from a grouped
+    sweep plan's tables compiles for gfx950 (hipRTC, no device).  Instances
+    that differ only in their coefficient (nu) and grid share generated code:
+    one program for the Burgers operator (7 slots, every slot seeded), one for
+    the Dirichlet conditions (1 slot, a target array)."""
+    t, arch = _sweep_tables([0, 100, 255])
+    src = t.fused_source(arch, compile=True)
+    assert src
+    gen = src[src.index('struct GenResid'):]
+    cases = [c for c in gen.split('    case ')[1:]]
+    assert len(cases) == 2
+    op = next(c for c in cases if 'sdv[6]' in c)
+    assert all(f'sdv[{k}]' in op for k in range(7)) and 'sdv[7]' not in op
+    bc = next(c for c in cases if c is not op)
+    assert 'sdv[0]' in bc and 'sdv[1]' not in bc and 'data[(int64_t)I[' in bc
+    # fixed positions of the group's own tables, no loop over records
+    assert 'for (' not in op.replace('for (int s = 0; s < FUSE_MAX_SLOTS', '')
+
+
+def test_fused_step_kernel_not_for_double_or_wide_networks():
+    t, arch = _sweep_tables([0], np.float64)
+    assert t.fused_source(arch) == ''
+    assert 'fused step' in native.lib().tdx_last_error().decode()
+    from torch_de_solver_amd.compiler import compile_nn
+    from torch_de_solver_amd.nn_engine import NNTables, mlp_arch
+    d, eq, b, kw = problems.burgers_nn(api)
+    net = problems.make_mlp([2, 40, 40, 1])
+    prob = compile_nn(d.build('NN'), eq.equation_lst, b.build(d.variable_dict), 1, h=0.001)
+    assert NNTables([prob], mlp_arch(net), np.float32).fused_source(mlp_arch(net)) == ''
+
+
 def _ad_desc(name='wave_ad'):
     from torch_de_solver_amd.ad_engine import ADTables
     from torch_de_solver_amd.compiler import compile_ad

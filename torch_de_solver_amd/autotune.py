@@ -3,9 +3,10 @@
 The reference has no tunable launch shapes (it runs stock torch ops).  Here
 two knobs decide how work maps onto the GPU:
 
-* 'NN' plans: ``chunk_tiles``, the number of 64-row tiles one backward
-  workgroup walks through (more tiles: fewer gradient partials to reduce;
-  fewer tiles: a finer, better balanced grid over uneven instances);
+* 'NN' plans: ``chunk_tiles``, the work of one gradient workgroup — 64-row
+  tiles for the separate adjoint kernel, 64-entry blocks for the fused
+  forward-and-adjoint kernel (more: fewer gradient partials to reduce;
+  fewer: a finer, better balanced grid over uneven instances);
 * 'mat' plans: ``tile_rows``, the output rows of one stencil tile (more rows:
   less halo recomputed per point; fewer rows: less LDS per workgroup, so more
   workgroups in flight per CU).
@@ -92,25 +93,28 @@ def nn_key(problems, arch, dtype) -> str:
 
 def tune_nn(problems, arch, device, params: torch.Tensor,
             candidates: Sequence[int] = NN_CHUNK_CANDIDATES, reps: int = 3,
-            use_cache: bool = True, tile_rows: int = 0) -> Tuple[int, dict]:
+            use_cache: bool = True, tile_rows: int = 0,
+            persistent: int = 0) -> Tuple[int, dict]:
     """Pick ``chunk_tiles`` for an 'NN' plan over ``problems`` (compiled) by
-    timing forward + backward with ``params``.  Returns (chunk_tiles, report)."""
+    timing the training step (``tdx_nn_train_step`` at learning rate 0, on a
+    copy of ``params``: the fused kernel where the plan has it, else the
+    separate forward and adjoint kernels).  Returns (chunk_tiles, report)."""
     from .nn_engine import NNEngine
-    key = nn_key(problems, arch, params.dtype) + f"|tr{tile_rows}"
+    key = nn_key(problems, arch, params.dtype) + f"|tr{tile_rows}|ps{persistent}|step"
     if use_cache:
         hit = _load().get(key)
         if hit and hit.get('chunk_tiles') in candidates:
             return int(hit['chunk_tiles']), dict(hit, cached=True)
     loss = torch.empty(2 * len(problems), dtype=params.dtype, device=params.device)
-    grad = torch.empty_like(params)
-    one = torch.ones(1, dtype=params.dtype, device=params.device)
+    work = params.detach().clone()
+    m, v = torch.zeros_like(work), torch.zeros_like(work)
+    step = torch.zeros(1, dtype=torch.int64, device=params.device)
     times = {}
     for ct in candidates:
-        eng = NNEngine(problems, arch, device, tile_rows, ct)
+        eng = NNEngine(problems, arch, device, tile_rows, ct, persistent)
 
         def run():
-            eng.forward(params, loss)
-            eng.backward(params, one, grad)
+            eng.train_step(work, m, v, step, 0.0, 0.9, 0.999, 1e-8, loss)
         times[ct] = _time_ms(run, reps)
         del eng
     best = min(times, key=times.get)

@@ -32,99 +32,16 @@
 // features held in register [mb][i]); the weights are packed to match.
 
 #include "tdx_common.h"
+#include "tdx_nn_dev.h"
+#include "tdx_nn_jit.h"
 
 #include <algorithm>
+#include <cstdlib>
+#include <map>
 #include <cmath>
 #include <cstring>
 
 namespace tdx {
-
-constexpr int MAXD = 4;   // network inputs
-constexpr int MAXO = 4;   // network outputs
-constexpr int MAXF = 4;   // factors per term
-constexpr int K1_WAVES = 4;
-
-template <typename T>
-struct NNArgs {
-  // architecture
-  int d_in, n_out, H, NH, P;  // NH = hidden H x H layers
-  // tables
-  const int32_t *groups;
-  const int32_t *ints;
-  const T *reals;
-  const T *data;
-  // work lists
-  const int2 *k1_items;       // (group, entry start) per forward workgroup
-  const int4 *k2_items;       // (group, row start, row end, inst) per backward workgroup
-  const int32_t *inst_k1_ptr, *inst_k1_idx;
-  const int32_t *inst_k2_ptr, *inst_k2_idx;
-  int n_inst;
-  // buffers
-  const T *params;            // n_inst * P
-  T *packed;                  // n_inst packed network images
-  T *seeds;                   // n_rows * n_out
-  T *lp;                      // 2 per forward workgroup
-  T *partial;                 // P per backward workgroup
-  T *resid;
-  const T *dloss;
-  int acc_lds;                // backward accumulates in LDS
-  int tile_rows;
-  // seed multipliers of group 0's equations, [n_eq][N] (null: none)
-  const T *seed_mult;
-  int seed_mult_n;
-  // network values per seed row [n_rows][n_out]: u_out receives them
-  // (null: not kept); with u_in the forward kernel reads them instead of
-  // evaluating the network (null: evaluate)
-  T *u_out;
-  const T *u_in;
-  // persistent launches: work item counts, the largest slot count times
-  // n_out (LDS stride of the forward kernel), and the fused step's unit
-  // seed scale and Adam step counter (null: not touched)
-  int n_k1, n_k2, so_max;
-  const int32_t *k1_bounds;   // persistent forward: item range of each workgroup
-  T *one_out;
-  int64_t *step_inc;
-};
-
-template <typename T>
-struct Layout {
-  int d_in, n_out, H, NH;
-  __device__ __host__ int oW1() const { return 0; }
-  __device__ __host__ int ob1() const { return H * d_in; }
-  __device__ __host__ int oWh(int j) const { return H * d_in + H + j * (H * H + H); }
-  __device__ __host__ int obh(int j) const { return oWh(j) + H * H; }
-  __device__ __host__ int oWL() const { return H * d_in + H + NH * (H * H + H); }
-  __device__ __host__ int obL() const { return oWL() + n_out * H; }
-  __device__ __host__ int count() const { return obL() + n_out; }
-};
-
-// ---------------------------------------------------------------------------
-// packed network image
-// ---------------------------------------------------------------------------
-// One image per instance, in the workspace, rebuilt from the parameters by
-// k_pack at the start of every forward pass (HP = 16 NB):
-//   [Wf: NH x HP*HP][small: SW][Wb: NH x HP*HP]
-// Wf and Wb hold each hidden layer's W_j as MFMA operand images, laid out so
-// that lane l reads the four values of k-steps i = 0..3 of block (xb, mb)
-// with one 128-bit load:
-//   Wf[j][ob][mb][lane][i] = W_j[16*ob + lane%16][16*mb + crow(lane/16, i)]
-//   Wb[j][ib][mb][lane][i] = W_j[16*mb + crow(lane/16, i)][16*ib + lane%16]
-// The small arrays are indexed by accumulator position p = 16 mb + 4 q + i,
-// which holds hidden feature f = 16 mb + crow(q, i), zero for f >= H:
-//   b1[HP] | W1[d_in][HP] | bh[NH][HP] | WL[n_out][HP] | bL[n_out, padded to 4]
-// so a lane's four features of one block are one 128-bit load, and padding
-// features evaluate to tanh(0) = 0 without a guard.  The forward kernel
-// stages the prefix without Wb in LDS, the backward kernel all of it.
-__host__ __device__ inline int small_words(int HP, int NH, int d_in, int n_out) {
-  return HP * (1 + d_in + NH + n_out) + ((n_out + 3) & ~3);
-}
-__host__ __device__ inline int image_words(int HP, int NH, int d_in, int n_out) {
-  return 2 * NH * HP * HP + small_words(HP, NH, d_in, n_out);
-}
-// Words of the LDS copy (in T units).
-__host__ __device__ inline int stage_words(int HP, int NH, int d_in, int n_out, bool with_wb) {
-  return NH * HP * HP * (with_wb ? 2 : 1) + small_words(HP, NH, d_in, n_out);
-}
 
 // Hidden feature held at accumulator position p of a block.
 template <typename T>
@@ -201,54 +118,6 @@ __global__ void k_pack(NNArgs<T> a) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// network view: the packed image in global memory or its copy in LDS
-// ---------------------------------------------------------------------------
-template <typename T, int NB>
-struct Net {
-  const T *Wf, *Wb;          // layer j image at Wf + j*HP*HP
-  const T *b1, *W1, *bh, *WL, *bL;  // small arrays, row stride HP
-  Layout<T> L;
-};
-
-template <typename T, int NB>
-__device__ __forceinline__ Net<T, NB> net_at(const NNArgs<T> &a, const T *base) {
-  constexpr int HP = 16 * NB;
-  Net<T, NB> n;
-  n.L = Layout<T>{a.d_in, a.n_out, a.H, a.NH};
-  n.Wf = base;
-  const T *sm = base + a.NH * HP * HP;
-  n.b1 = sm;
-  n.W1 = sm + HP;
-  n.bh = n.W1 + a.d_in * HP;
-  n.WL = n.bh + a.NH * HP;
-  n.bL = n.WL + a.n_out * HP;
-  n.Wb = sm + small_words(HP, a.NH, a.d_in, a.n_out);
-  return n;
-}
-
-template <typename T, int NB>
-__device__ __forceinline__ Net<T, NB> make_net(const NNArgs<T> &a, int inst) {
-  constexpr int HP = 16 * NB;
-  return net_at<T, NB>(a, a.packed + (int64_t)inst * image_words(HP, a.NH, a.d_in, a.n_out));
-}
-
-// Cooperative copy of one instance's image (without Wb unless with_wb) into
-// LDS at `dst`; returns the network view on the copy.  Ends with a
-// workgroup barrier.
-template <typename T, int NB>
-__device__ Net<T, NB> stage_net(const NNArgs<T> &a, int inst, T *dst, bool with_wb) {
-  constexpr int HP = 16 * NB;
-  typedef typename V4<T>::type v4;
-  const T *src = a.packed + (int64_t)inst * image_words(HP, a.NH, a.d_in, a.n_out);
-  const int n4 = stage_words(HP, a.NH, a.d_in, a.n_out, with_wb) / 4;
-  const v4 *s4 = reinterpret_cast<const v4 *>(src);
-  v4 *d4 = reinterpret_cast<v4 *>(dst);
-  for (int k = threadIdx.x; k < n4; k += blockDim.x) d4[k] = s4[k];
-  __syncthreads();
-  return net_at<T, NB>(a, dst);
-}
-
 // The same LDS copy as stage_net, gathered from the instance's parameters
 // (no packed image in global memory needed).  Ends with a workgroup
 // barrier.
@@ -259,86 +128,6 @@ __device__ void gather_image(const NNArgs<T> &a, int inst, T *dst, bool with_wb)
   const int n = stage_words(HP, a.NH, a.d_in, a.n_out, with_wb);
   for (int k = threadIdx.x; k < n; k += blockDim.x) dst[k] = image_value<T, NB>(a, W, k);
   __syncthreads();
-}
-
-// Four consecutive values of a small array at accumulator position p.
-template <typename T>
-__device__ __forceinline__ typename V4<T>::type ld4(const T *p) {
-  return *reinterpret_cast<const typename V4<T>::type *>(p);
-}
-
-// Layer 1 (VALU): act[mb][i] = tanh(b1[f] + sum_a x[a] W1[f][a]).
-template <typename T, int NB>
-__device__ __forceinline__ void layer_in(const Net<T, NB> &n, const T *x, int q,
-                                         T (&act)[NB][4]) {
-  constexpr int HP = 16 * NB;
-  typedef typename V4<T>::type v4;
-  const int d = n.L.d_in;
-#pragma unroll
-  for (int mb = 0; mb < NB; ++mb) {
-    v4 z = ld4(n.b1 + 16 * mb + 4 * q);
-#pragma unroll
-    for (int k = 0; k < MAXD; ++k)
-      if (k < d) {
-        const v4 w = ld4(n.W1 + k * HP + 16 * mb + 4 * q);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) z[i] = fma(x[k], w[i], z[i]);
-      }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) act[mb][i] = dtanh(z[i]);
-  }
-}
-
-// acc[ob] += W_j (rows of block ob) x act, W given as an operand image.
-template <typename T, int NB>
-__device__ __forceinline__ void mma_layer(const T *img, int lane, const T (&b)[NB][4],
-                                          typename V4<T>::type (&acc)[NB]) {
-  typedef typename V4<T>::type v4;
-  const v4 *wp = reinterpret_cast<const v4 *>(img) + lane;
-#pragma unroll
-  for (int mb = 0; mb < NB; ++mb)
-#pragma unroll
-    for (int ob = 0; ob < NB; ++ob) {
-      const v4 w = wp[(ob * NB + mb) * 64];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc[ob] = mma16(w[i], b[mb][i], acc[ob]);
-    }
-}
-
-// Hidden layer j (MFMA, the accumulator starts at the bias):
-// act <- tanh(b_j + W_j act).
-template <typename T, int NB>
-__device__ __forceinline__ void layer_hidden(const Net<T, NB> &n, int j, int lane,
-                                             T (&act)[NB][4]) {
-  constexpr int HP = 16 * NB;
-  typedef typename V4<T>::type v4;
-  const int q = lane >> 4;
-  v4 acc[NB];
-#pragma unroll
-  for (int ob = 0; ob < NB; ++ob) acc[ob] = ld4(n.bh + j * HP + 16 * ob + 4 * q);
-  mma_layer<T, NB>(n.Wf + j * HP * HP, lane, act, acc);
-#pragma unroll
-  for (int ob = 0; ob < NB; ++ob)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) act[ob][i] = dtanh(acc[ob][i]);
-}
-
-// Output layer (VALU + cross-quarter shuffle): u[o] for the lane's row.
-template <typename T, int NB>
-__device__ __forceinline__ T layer_out(const Net<T, NB> &n, int o, int q,
-                                       const T (&act)[NB][4]) {
-  constexpr int HP = 16 * NB;
-  typedef typename V4<T>::type v4;
-  T s = T(0);
-#pragma unroll
-  for (int mb = 0; mb < NB; ++mb) {
-    const v4 w = ld4(n.WL + o * HP + 16 * mb + 4 * q);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) s = fma(act[mb][i], w[i], s);
-  }
-  s += __shfl_xor(s, 16);
-  s += __shfl_xor(s, 32);
-  return s + n.bL[o];
 }
 
 template <typename T>
@@ -430,77 +219,10 @@ __device__ __forceinline__ void forward_item(const NNArgs<T> &a, int it, const N
     T *u_lds = smem + lane * SO;
     T *sd_lds = smem + K1_WAVES * 16 * a.so_max + lane * SO;
     for (int k = 0; k < SO; ++k) sd_lds[k] = T(0);
-    if (valid) {
-      for (int eq = 0; eq < n_eq; ++eq) {
-        const int32_t *E = a.ints + eq_i + eq * TDX_EQ_INTS;
-        const int n_terms = E[0], term_i = E[1];
-        T r = T(0);
-        for (int t = 0; t < n_terms; ++t) {
-          const int32_t *Tm = a.ints + term_i + t * TDX_TERM_INTS;
-          T der = T(1);
-          for (int f = 0; f < Tm[0]; ++f) {
-            const int32_t *F = a.ints + Tm[1] + f * TDX_FACTOR_INTS;
-            T gs = T(0);
-            for (int k = 0; k < F[3]; ++k) {
-              const int32_t *S = a.ints + F[4] + k * TDX_STENCIL_INTS;
-              gs = gs + u_lds[S[0] * a.n_out + F[0]] * a.reals[S[1]];
-            }
-            der = der * fpow(a, F, gs);
-          }
-          const T tv = coeff_of(a, Tm, e) * der;
-          r = (t == 0) ? tv : r + tv;
-        }
-        const T target = E[2] >= 0 ? a.data[E[2] + e] : T(0);
-        if (E[3] >= 0 && a.resid) a.resid[E[3] + e] = r;
-        const T diff = r - target;
-        const T sq = diff * diff;
-        const T lam_d = a.reals[E[4]];
-        lsum += lam_d * sq;
-        lnsum += a.reals[E[5]] * sq;
-        T dr = T(2) * a.reals[E[6]] * diff;
-        if (a.seed_mult && item.x == 0) dr *= a.seed_mult[(int64_t)eq * a.seed_mult_n + e];
-        // seeds: d r / d u(slot, var), through every factor of every term;
-        // d(prod_j f_j^p_j)/d f_k = p_k f_k^(p_k-1) * prefix_k * suffix_k
-        for (int t = 0; t < n_terms; ++t) {
-          const int32_t *Tm = a.ints + term_i + t * TDX_TERM_INTS;
-          const int nf = Tm[0];
-          T gsv[MAXF], pw[MAXF], pre[MAXF + 1], suf[MAXF + 1];
-#pragma unroll
-          for (int f = 0; f < MAXF; ++f) {
-            gsv[f] = T(0);
-            pw[f] = T(1);
-            if (f < nf) {
-              const int32_t *F = a.ints + Tm[1] + f * TDX_FACTOR_INTS;
-              T gs = T(0);
-              for (int k = 0; k < F[3]; ++k) {
-                const int32_t *S = a.ints + F[4] + k * TDX_STENCIL_INTS;
-                gs = gs + u_lds[S[0] * a.n_out + F[0]] * a.reals[S[1]];
-              }
-              gsv[f] = gs;
-              pw[f] = fpow(a, F, gs);
-            }
-          }
-          pre[0] = T(1);
-#pragma unroll
-          for (int f = 0; f < MAXF; ++f) pre[f + 1] = pre[f] * pw[f];
-          suf[MAXF] = T(1);
-#pragma unroll
-          for (int f = MAXF - 1; f >= 0; --f) suf[f] = suf[f + 1] * pw[f];
-          const T cdr = coeff_of(a, Tm, e) * dr;
-#pragma unroll
-          for (int f = 0; f < MAXF; ++f) {
-            if (f < nf) {
-              const int32_t *F = a.ints + Tm[1] + f * TDX_FACTOR_INTS;
-              const T g = cdr * dfpow(a, F, gsv[f]) * pre[f] * suf[f + 1];
-              for (int k = 0; k < F[3]; ++k) {
-                const int32_t *S = a.ints + F[4] + k * TDX_STENCIL_INTS;
-                sd_lds[S[0] * a.n_out + F[0]] += g * a.reals[S[1]];
-              }
-            }
-          }
-        }
-      }
-    }
+    if (valid)
+      entry_program<T>(PtrProg<T>{a.ints, a.reals, a.data, u_lds, sd_lds}, eq_i, n_eq, e,
+                       a.n_out, lsum, lnsum, a.resid,
+                       (a.seed_mult && item.x == 0) ? a.seed_mult : nullptr, a.seed_mult_n);
   }
   __syncthreads();
   // store seeds: row (slot s, entry e) = row_base + s*n_entries + e
@@ -769,20 +491,6 @@ __global__ __launch_bounds__(256) void k_backward(NNArgs<T> a) {
 // MFMAs with K = the wavefront's 16 rows, after a transpose through a
 // wavefront-private LDS buffer.  All gradient accumulators are registers;
 // at the end the wavefronts are summed in a fixed order (deterministic).
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Wavefront-private transpose buffers are feature-major, [feature][row],
-// with a row stride of 20 words: the per-lane writes (16 rows x 4 quarters)
-// and the 128-bit operand reads (4 consecutive row positions per lane) both
-// touch 64 distinct banks.
-constexpr int WRS = 20;
-template <int HP>
-constexpr int wave_ls() { return WRS * HP / 16; }
 
 // acc[t][ob] += W (rows of block ob) x b[t] for NT independent tiles; each
 // weight operand is loaded once and used by every tile.
@@ -1178,6 +886,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   T *stage = smem + backward_w_buffer_words<NB, NT>(blockDim.x >> 6, a.P);
   const Net<T, NB> net = stage_net<T, NB>(a, a.k2_items[blockIdx.x].w, stage, true);
   backward_w_item<T, NB, NH, NT>(a, blockIdx.x, net, smem);
+}
+
+// k_fused: the fused training step with the interpreted residual program.
+template <int NB, int NH>
+__global__ __launch_bounds__(64 * FUSE_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void
+k_fused(NNArgs<float> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  fused_body<NB, NH, InterpResid>(a, reinterpret_cast<float *>(smem_raw));
 }
 
 // Block-owning backward for wide float networks (HP = 16 NB up to 128).
@@ -1924,6 +1640,15 @@ struct tdx_nn_plan {
   int k2_wave_private;
   int k2_nt;          // tiles per wavefront iteration of the wave-private backward
   int k2_block;       // block-owning backward (wide float networks)
+  // fused training step (k_fused): its work items, their per-instance
+  // lists, the groups' residual programs and the kernel's LDS bytes
+  int fused = 0;
+  int n_k3 = 0;
+  size_t k3_lds = 0;
+  DevTable k3_items, ik3p, ik3i, prog_idx, prog_ints, prog_reals, prog_kind;
+  // the fused kernel compiled with the plan's generated residual programs
+  // (null: the interpreted kernel)
+  hipFunction_t fused_fn = nullptr;
   int acc_lds;
   size_t k1_lds, k2_lds;
   DevTable groups, ints, reals, data, k1_items, k2_items, ik1p, ik1i, ik2p, ik2i;
@@ -2017,6 +1742,11 @@ NNArgs<T> make_args(const tdx_nn_plan *p, const void *params, void *ws) {
   a.k1_bounds = (const int32_t *)p->k1_bounds.ptr;
   a.one_out = nullptr;
   a.step_inc = nullptr;
+  a.k3_items = (const int4 *)p->k3_items.ptr;
+  a.prog_idx = (const int4 *)p->prog_idx.ptr;
+  a.prog_ints = (const int32_t *)p->prog_ints.ptr;
+  a.prog_reals = (const T *)p->prog_reals.ptr;
+  a.prog_kind = (const int32_t *)p->prog_kind.ptr;
   return a;
 }
 
@@ -2226,6 +1956,37 @@ int launch_backward(const tdx_nn_plan *p, NNArgs<T> a, hipStream_t s) {
   return TDX_OK;
 }
 
+template <int NB, int NH>
+int launch_fused_shape(const tdx_nn_plan *p, const NNArgs<float> &a, hipStream_t s) {
+  ProfScope ps(p, 1, s);
+  if (p->fused_fn) {
+    NNArgs<float> arg = a;
+    void *args[] = {&arg};
+    TDX_TRY_HIP(hipModuleLaunchKernel(p->fused_fn, p->n_k3, 1, 1, 64 * FUSE_WAVES, 1, 1,
+                                      (unsigned)p->k3_lds, s, args, nullptr));
+    return TDX_OK;
+  }
+  raise_lds_limit({(const void *)k_fused<NB, NH>});
+  hipLaunchKernelGGL((k_fused<NB, NH>), dim3(p->n_k3), dim3(64 * FUSE_WAVES), p->k3_lds, s, a);
+  TDX_TRY_HIP(hipGetLastError());
+  return TDX_OK;
+}
+
+// The fused step's kernel (the profile counts it as the backward kind).
+template <typename T, int NB>
+int launch_fused(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
+  if constexpr (sizeof(T) == 4 && NB <= 2) {
+    switch (p->NH) {
+      case 0: return launch_fused_shape<NB, 0>(p, a, s);
+      case 1: return launch_fused_shape<NB, 1>(p, a, s);
+      case 2: return launch_fused_shape<NB, 2>(p, a, s);
+      default: break;
+    }
+  }
+  set_error("no fused step for this shape");
+  return TDX_EUNSUPPORTED;
+}
+
 template <typename T>
 int launch_reduce(const tdx_nn_plan *p, const NNArgs<T> &a, int mode, T *dparams, T *params,
                   T *m, T *v, const int64_t *step, AdamHyper h, hipStream_t s,
@@ -2274,6 +2035,21 @@ int nn_dispatch(const tdx_nn_plan *p, int op, const void *params, void *params_m
       a.step_inc = nullptr;                                                       \
       if (!rc) rc = launch_backward<T, NBV>(p, a, s);                             \
       if (!rc) rc = launch_reduce<T>(p, a, 1, nullptr, (T *)params_mut, (T *)m,  \
+                                     (T *)v, step, h, s, (T *)loss, 1);           \
+    } else if (p->fused) {                                                        \
+      /* three launches: packing (unit seed scale, step), the fused */           \
+      /* forward and adjoint, the reduction over the fused items' */             \
+      /* partials with the Adam update and the loss pairs */                     \
+      a.one_out = (T *)((char *)ws + p->off_one);                                 \
+      a.step_inc = step;                                                          \
+      rc = launch_pack<T, NBV>(p, a, s);                                          \
+      a.one_out = nullptr;                                                        \
+      a.step_inc = nullptr;                                                       \
+      if (!rc) rc = launch_fused<T, NBV>(p, a, s);                                \
+      NNArgs<T> ar = a;                                                           \
+      ar.inst_k2_ptr = (const int32_t *)p->ik3p.ptr;                              \
+      ar.inst_k2_idx = (const int32_t *)p->ik3i.ptr;                              \
+      if (!rc) rc = launch_reduce<T>(p, ar, 1, nullptr, (T *)params_mut, (T *)m, \
                                      (T *)v, step, h, s, (T *)loss, 1);           \
     } else if (!p->causal && !p->weak) {                                          \
       /* four launches: the packing kernel also sets the unit seed scale */      \
@@ -2374,6 +2150,71 @@ int validate_nn(const tdx_nn_desc *d, int64_t *n_rows_out) {
   return TDX_OK;
 }
 
+// Whether a desc's network and options admit the fused training step
+// (k_fused): float, at most 32 hidden features and two H x H layers, one
+// output, at most two inputs, at most FUSE_MAX_SLOTS slots per group,
+// default tiles, no persistent forward, no causal or weak-form loss.
+bool fused_shape_ok(const tdx_nn_desc *d, int max_slots) {
+  return d->dtype == 0 && d->width <= 32 && d->n_layers - 2 <= 2 && d->d_in <= 2 &&
+         d->n_out == 1 && max_slots <= FUSE_MAX_SLOTS && d->tile_rows == 0 &&
+         d->persistent == 0 && d->causal_tol == 0.0 && d->weak_levels == 0;
+}
+
+// Every group's residual program rewritten as its own table pair (record
+// offsets relative to the pair, real values copied, unread words zero);
+// false when one exceeds FUSE_PI record or FUSE_PR real words.
+bool fused_programs(const tdx_nn_desc *d, std::vector<FusedProg> *out) {
+  const float *R = (const float *)d->reals;
+  out->clear();
+  for (int g = 0; g < d->n_groups; ++g) {
+    const int32_t *G = d->groups + (int64_t)g * TDX_GROUP_INTS;
+    FusedProg fp;
+    fp.n_eq = G[5];
+    std::vector<int32_t> &mi = fp.ints;
+    std::vector<float> &mr = fp.reals;
+    mi.assign((size_t)G[5] * TDX_EQ_INTS, 0);
+    auto real = [&](int32_t idx) {
+      mr.push_back(R[idx]);
+      return (int32_t)mr.size() - 1;
+    };
+    for (int q = 0; q < G[5]; ++q) {
+      const int32_t *E = d->ints + G[6] + q * TDX_EQ_INTS;
+      const size_t eo = (size_t)q * TDX_EQ_INTS;
+      for (int k = 0; k < TDX_EQ_INTS; ++k) mi[eo + k] = E[k];
+      for (int k = 4; k <= 6; ++k) mi[eo + k] = real(E[k]);
+      const size_t to = mi.size();
+      mi[eo + 1] = (int32_t)to;
+      mi.resize(to + (size_t)E[0] * TDX_TERM_INTS);
+      for (int t = 0; t < E[0]; ++t) {
+        const int32_t *Tm = d->ints + E[1] + t * TDX_TERM_INTS;
+        const size_t tt = to + (size_t)t * TDX_TERM_INTS;
+        for (int k = 0; k < TDX_TERM_INTS; ++k) mi[tt + k] = Tm[k];
+        if (Tm[2] == 0) mi[tt + 3] = real(Tm[3]);
+        const size_t fo = mi.size();
+        mi[tt + 1] = (int32_t)fo;
+        mi.resize(fo + (size_t)Tm[0] * TDX_FACTOR_INTS);
+        for (int f = 0; f < Tm[0]; ++f) {
+          const int32_t *F = d->ints + Tm[1] + f * TDX_FACTOR_INTS;
+          const size_t ff = fo + (size_t)f * TDX_FACTOR_INTS;
+          for (int k = 0; k < TDX_FACTOR_INTS; ++k) mi[ff + k] = F[k];
+          mi[ff + 5] = F[1] != 0 ? real(F[5]) : 0;
+          const size_t so = mi.size();
+          mi[ff + 4] = (int32_t)so;
+          mi.resize(so + (size_t)F[3] * TDX_STENCIL_INTS);
+          for (int k = 0; k < F[3]; ++k) {
+            const int32_t *S = d->ints + F[4] + k * TDX_STENCIL_INTS;
+            mi[so + 2 * k] = S[0];
+            mi[so + 2 * k + 1] = real(S[1]);
+          }
+        }
+      }
+    }
+    if (mi.size() > (size_t)FUSE_PI || mr.size() > (size_t)FUSE_PR) return false;
+    out->push_back(std::move(fp));
+  }
+  return true;
+}
+
 }  // namespace
 
 extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
@@ -2437,9 +2278,11 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
 
   // forward work: 64 entries per workgroup
   std::vector<int2> k1;
-  std::vector<std::vector<int>> inst_k1(d->n_inst), inst_k2(d->n_inst);
+  std::vector<std::vector<int>> inst_k1(d->n_inst), inst_k2(d->n_inst), inst_k3(d->n_inst);
+  std::vector<int> k1_first(d->n_groups);
   for (int g = 0; g < d->n_groups; ++g) {
     const int32_t *G = d->groups + (int64_t)g * TDX_GROUP_INTS;
+    k1_first[g] = (int)k1.size();
     for (int e = 0; e < G[1]; e += 16 * K1_WAVES) {
       inst_k1[G[0]].push_back((int)k1.size());
       k1.push_back(make_int2(g, e));
@@ -2525,6 +2368,67 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   if (ik1i.empty()) ik1i.push_back(0);
   if (ik2i.empty()) ik2i.push_back(0);
 
+  // fused training step for narrow float networks with one output and at
+  // most two inputs: work items of chunk_tiles 64-entry blocks (default 4)
+  // of one group, and each group's residual program as its own table pair
+  std::vector<int4> k3, prog_idx;
+  std::vector<int32_t> prog_ints, ik3p(1, 0), ik3i, prog_kind;
+  std::vector<float> prog_reals;
+  std::vector<FusedProg> progs;
+  p->k3_lds = (size_t)fused_lds_words(p->HP, p->NH, d->d_in, d->n_out, p->P) * ts;
+  bool fuse = fused_shape_ok(d, max_slots) && p->k2_wave_private && p->k3_lds <= LDS_MAX &&
+              fused_programs(d, &progs);
+  if (fuse) {
+    for (const FusedProg &fp : progs) {
+      prog_idx.push_back(make_int4((int)prog_ints.size(), (int)fp.ints.size(),
+                                   (int)prog_reals.size(), (int)fp.reals.size()));
+      prog_ints.insert(prog_ints.end(), fp.ints.begin(), fp.ints.end());
+      prog_reals.insert(prog_reals.end(), fp.reals.begin(), fp.reals.end());
+    }
+  }
+  if (fuse) {
+    const int C = d->chunk_tiles > 0 ? d->chunk_tiles : 4;
+    for (int g = 0; g < d->n_groups; ++g) {
+      const int32_t *G = d->groups + (int64_t)g * TDX_GROUP_INTS;
+      const int nblk = (G[1] + 63) / 64;
+      for (int j = 0; j < nblk; j += C) {
+        inst_k3[G[0]].push_back((int)k3.size());
+        k3.push_back(make_int4(g, 64 * j, std::min(G[1], 64 * (j + C)), k1_first[g] + j));
+      }
+    }
+    for (int i = 0; i < d->n_inst; ++i) {
+      for (int x : inst_k3[i]) ik3i.push_back(x);
+      ik3p.push_back((int32_t)ik3i.size());
+    }
+    p->fused = 1;
+    p->n_k3 = (int)k3.size();
+    // generated residual programs for large plans (TDX_NN_JIT: 0 never,
+    // 1 always); the interpreted kernel if compiling fails
+    const char *env = std::getenv("TDX_NN_JIT");
+    const bool jit = env ? std::atoi(env) != 0 : n_rows >= (1 << 20);
+    if (jit) {
+      std::vector<std::vector<int32_t>> keys;
+      std::vector<int> key_neq;
+      std::map<std::vector<int32_t>, int> kind_of;
+      for (const FusedProg &fp : progs) {
+        const std::vector<int32_t> key = nn_prog_key(fp);
+        auto it = kind_of.find(key);
+        if (it == kind_of.end()) {
+          it = kind_of.emplace(key, (int)keys.size()).first;
+          keys.push_back(key);
+          key_neq.push_back(fp.n_eq);
+        }
+        prog_kind.push_back(it->second);
+      }
+      const std::string src = nn_fused_jit_source(keys, key_neq, p->NB, p->NH);
+      if (nn_fused_jit_load(src, &p->fused_fn) != TDX_OK) {
+        p->fused_fn = nullptr;
+        prog_kind.clear();
+        clear_error();
+      }
+    }
+  }
+
   int r2 = TDX_OK;
   if (!r2) r2 = p->groups.upload(d->groups, (size_t)d->n_groups * TDX_GROUP_INTS * 4);
   if (!r2) r2 = p->ints.upload(d->ints, (size_t)std::max<int64_t>(d->n_ints, 1) * 4);
@@ -2536,6 +2440,16 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   if (!r2) r2 = p->ik1i.upload(ik1i.data(), ik1i.size() * 4);
   if (!r2) r2 = p->ik2p.upload(ik2p.data(), ik2p.size() * 4);
   if (!r2) r2 = p->ik2i.upload(ik2i.data(), ik2i.size() * 4);
+  if (!r2 && p->fused) {
+    r2 = p->k3_items.upload(k3.data(), k3.size() * sizeof(int4));
+    if (!r2) r2 = p->ik3p.upload(ik3p.data(), ik3p.size() * 4);
+    if (!r2) r2 = p->ik3i.upload(ik3i.data(), ik3i.size() * 4);
+    if (!r2) r2 = p->prog_idx.upload(prog_idx.data(), prog_idx.size() * sizeof(int4));
+    if (!r2) r2 = p->prog_ints.upload(prog_ints.data(), prog_ints.size() * 4);
+    if (!r2) r2 = p->prog_reals.upload(prog_reals.data(), prog_reals.size() * 4);
+    if (!r2 && !prog_kind.empty())
+      r2 = p->prog_kind.upload(prog_kind.data(), prog_kind.size() * 4);
+  }
   if (!r2 && p->persist) {
     // persistent forward: as many workgroups as the device keeps resident,
     // each taking a contiguous range of work items of about equal cost
@@ -2576,7 +2490,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   p->off_lp = off;
   off = align_up(off + (size_t)p->n_k1 * 2 * ts, 256);
   p->off_partial = off;
-  off = align_up(off + (size_t)p->n_k2 * p->P * ts, 256);
+  off = align_up(off + (size_t)std::max(p->n_k2, p->n_k3) * p->P * ts, 256);
   p->off_one = off;
   off = align_up(off + ts, 256);
   if (d->causal_tol != 0.0) {
@@ -2653,10 +2567,64 @@ extern "C" int tdx_nn_profile_read(tdx_nn_plan *p, double *ms, int *count) {
 extern "C" void tdx_nn_plan_destroy(tdx_nn_plan *p) {
   if (!p) return;
   (void)tdx_nn_profile(p, 0);
-  DevTable *t[] = {&p->groups, &p->ints, &p->reals, &p->data, &p->k1_items,
-                   &p->k2_items, &p->ik1p, &p->ik1i, &p->ik2p, &p->ik2i, &p->k1_bounds};
+  DevTable *t[] = {&p->groups,   &p->ints,     &p->reals,     &p->data,     &p->k1_items,
+                   &p->k2_items, &p->ik1p,     &p->ik1i,      &p->ik2p,     &p->ik2i,
+                   &p->k1_bounds, &p->k3_items, &p->ik3p,     &p->ik3i,     &p->prog_idx,
+                   &p->prog_ints, &p->prog_reals, &p->prog_kind};
   for (DevTable *x : t) x->release();
   delete p;
+}
+
+extern "C" int64_t tdx_nn_fused_source(const tdx_nn_desc *d, char *buf, int64_t cap,
+                                       int compile) {
+  TDX_REQUIRE(d && d->groups && d->ints && d->reals, "null argument");
+  TDX_REQUIRE(d->dtype == 0 || d->dtype == 1, "dtype must be 0 (f32) or 1 (f64)");
+  TDX_REQUIRE(d->d_in >= 1 && d->d_in <= MAXD && d->n_out >= 1 && d->n_out <= MAXO &&
+                  d->width >= 1 && d->width <= 128 && d->n_layers >= 2 && d->n_inst >= 1 &&
+                  d->n_groups >= 1,
+              "bad network shape");
+  int64_t n_rows = 0;
+  const int rc = validate_nn(d, &n_rows);
+  if (rc) return rc;
+  int max_slots = 1;
+  for (int g = 0; g < d->n_groups; ++g)
+    max_slots = std::max(max_slots, (int)d->groups[(int64_t)g * TDX_GROUP_INTS + 2]);
+  std::vector<FusedProg> progs;
+  if (!fused_shape_ok(d, max_slots)) {
+    set_error("the network or options leave the fused step out");
+    return 0;
+  }
+  if (!fused_programs(d, &progs)) {
+    set_error("a residual program exceeds the fused kernel's table size");
+    return 0;
+  }
+  std::vector<std::vector<int32_t>> keys;
+  std::vector<int> key_neq;
+  std::map<std::vector<int32_t>, int> kind_of;
+  for (const FusedProg &fp : progs) {
+    const std::vector<int32_t> key = nn_prog_key(fp);
+    if (kind_of.emplace(key, (int)keys.size()).second) {
+      keys.push_back(key);
+      key_neq.push_back(fp.n_eq);
+    }
+  }
+  const std::string src = nn_fused_jit_source(keys, key_neq, (d->width + 15) / 16, d->n_layers - 2);
+  if (compile) {
+    std::vector<char> code;
+    const int crc = nn_fused_jit_build(src, &code);
+    if (crc) return crc;
+  }
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>(src.size(), (size_t)(cap - 1));
+    std::memcpy(buf, src.data(), n);
+    buf[n] = '\0';
+  }
+  return (int64_t)src.size();
+}
+
+extern "C" int tdx_nn_step_kernel(const tdx_nn_plan *p) {
+  if (!p) return -1;
+  return p->fused ? (p->fused_fn ? 2 : 1) : 0;
 }
 
 extern "C" int64_t tdx_nn_param_count(const tdx_nn_plan *p) { return p ? p->P : -1; }

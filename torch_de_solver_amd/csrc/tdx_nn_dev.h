@@ -1,0 +1,708 @@
+// Device code of the 'NN' kernels that the runtime-compiled fused kernel
+// shares with the library: argument block, network image and layer
+// helpers, the residual program interpreter, and the fused training-step
+// body.  Include after tdx_dev.h (and, in the library, tdx.h for the table
+// record sizes TDX_*_INTS); self-contained otherwise, so that hipRTC
+// compiles this text as it is.
+#ifndef TDX_NN_DEV_H
+#define TDX_NN_DEV_H
+
+namespace tdx {
+
+constexpr int MAXD = 4;   // network inputs
+constexpr int MAXO = 4;   // network outputs
+constexpr int MAXF = 4;   // factors per term
+constexpr int K1_WAVES = 4;
+
+template <typename T>
+struct NNArgs {
+  // architecture
+  int d_in, n_out, H, NH, P;  // NH = hidden H x H layers
+  // tables
+  const int32_t *groups;
+  const int32_t *ints;
+  const T *reals;
+  const T *data;
+  // work lists
+  const int2 *k1_items;       // (group, entry start) per forward workgroup
+  const int4 *k2_items;       // (group, row start, row end, inst) per backward workgroup
+  const int32_t *inst_k1_ptr, *inst_k1_idx;
+  const int32_t *inst_k2_ptr, *inst_k2_idx;
+  int n_inst;
+  // buffers
+  const T *params;            // n_inst * P
+  T *packed;                  // n_inst packed network images
+  T *seeds;                   // n_rows * n_out
+  T *lp;                      // 2 per forward workgroup
+  T *partial;                 // P per backward workgroup
+  T *resid;
+  const T *dloss;
+  int acc_lds;                // backward accumulates in LDS
+  int tile_rows;
+  // seed multipliers of group 0's equations, [n_eq][N] (null: none)
+  const T *seed_mult;
+  int seed_mult_n;
+  // network values per seed row [n_rows][n_out]: u_out receives them
+  // (null: not kept); with u_in the forward kernel reads them instead of
+  // evaluating the network (null: evaluate)
+  T *u_out;
+  const T *u_in;
+  // persistent launches: work item counts, the largest slot count times
+  // n_out (LDS stride of the forward kernel), and the fused step's unit
+  // seed scale and Adam step counter (null: not touched)
+  int n_k1, n_k2, so_max;
+  const int32_t *k1_bounds;   // persistent forward: item range of each workgroup
+  T *one_out;
+  int64_t *step_inc;
+  // fused step: (group, entry start, entry end, forward item of the start)
+  // per workgroup, and each group's residual program as a self-contained
+  // table pair (ints offset, ints, reals offset, reals)
+  const int4 *k3_items;
+  const int4 *prog_idx;
+  const int32_t *prog_ints;
+  const T *prog_reals;
+  const int32_t *prog_kind;   // generated program of each group (run-time compiled kernel)
+};
+
+template <typename T>
+struct Layout {
+  int d_in, n_out, H, NH;
+  __device__ __host__ int oW1() const { return 0; }
+  __device__ __host__ int ob1() const { return H * d_in; }
+  __device__ __host__ int oWh(int j) const { return H * d_in + H + j * (H * H + H); }
+  __device__ __host__ int obh(int j) const { return oWh(j) + H * H; }
+  __device__ __host__ int oWL() const { return H * d_in + H + NH * (H * H + H); }
+  __device__ __host__ int obL() const { return oWL() + n_out * H; }
+  __device__ __host__ int count() const { return obL() + n_out; }
+};
+
+// ---------------------------------------------------------------------------
+// packed network image
+// ---------------------------------------------------------------------------
+// One image per instance, in the workspace, rebuilt from the parameters by
+// k_pack at the start of every forward pass (HP = 16 NB):
+//   [Wf: NH x HP*HP][small: SW][Wb: NH x HP*HP]
+// Wf and Wb hold each hidden layer's W_j as MFMA operand images, laid out so
+// that lane l reads the four values of k-steps i = 0..3 of block (xb, mb)
+// with one 128-bit load:
+//   Wf[j][ob][mb][lane][i] = W_j[16*ob + lane%16][16*mb + crow(lane/16, i)]
+//   Wb[j][ib][mb][lane][i] = W_j[16*mb + crow(lane/16, i)][16*ib + lane%16]
+// The small arrays are indexed by accumulator position p = 16 mb + 4 q + i,
+// which holds hidden feature f = 16 mb + crow(q, i), zero for f >= H:
+//   b1[HP] | W1[d_in][HP] | bh[NH][HP] | WL[n_out][HP] | bL[n_out, padded to 4]
+// so a lane's four features of one block are one 128-bit load, and padding
+// features evaluate to tanh(0) = 0 without a guard.  The forward kernel
+// stages the prefix without Wb in LDS, the backward kernel all of it.
+__host__ __device__ inline int small_words(int HP, int NH, int d_in, int n_out) {
+  return HP * (1 + d_in + NH + n_out) + ((n_out + 3) & ~3);
+}
+__host__ __device__ inline int image_words(int HP, int NH, int d_in, int n_out) {
+  return 2 * NH * HP * HP + small_words(HP, NH, d_in, n_out);
+}
+// Words of the LDS copy (in T units).
+__host__ __device__ inline int stage_words(int HP, int NH, int d_in, int n_out, bool with_wb) {
+  return NH * HP * HP * (with_wb ? 2 : 1) + small_words(HP, NH, d_in, n_out);
+}
+
+// ---------------------------------------------------------------------------
+// network view: the packed image in global memory or its copy in LDS
+// ---------------------------------------------------------------------------
+template <typename T, int NB>
+struct Net {
+  const T *Wf, *Wb;          // layer j image at Wf + j*HP*HP
+  const T *b1, *W1, *bh, *WL, *bL;  // small arrays, row stride HP
+  Layout<T> L;
+};
+
+template <typename T, int NB>
+__device__ __forceinline__ Net<T, NB> net_at(const NNArgs<T> &a, const T *base) {
+  constexpr int HP = 16 * NB;
+  Net<T, NB> n;
+  n.L = Layout<T>{a.d_in, a.n_out, a.H, a.NH};
+  n.Wf = base;
+  const T *sm = base + a.NH * HP * HP;
+  n.b1 = sm;
+  n.W1 = sm + HP;
+  n.bh = n.W1 + a.d_in * HP;
+  n.WL = n.bh + a.NH * HP;
+  n.bL = n.WL + a.n_out * HP;
+  n.Wb = sm + small_words(HP, a.NH, a.d_in, a.n_out);
+  return n;
+}
+
+template <typename T, int NB>
+__device__ __forceinline__ Net<T, NB> make_net(const NNArgs<T> &a, int inst) {
+  constexpr int HP = 16 * NB;
+  return net_at<T, NB>(a, a.packed + (int64_t)inst * image_words(HP, a.NH, a.d_in, a.n_out));
+}
+
+// Cooperative copy of one instance's image (without Wb unless with_wb) into
+// LDS at `dst`; returns the network view on the copy.  Ends with a
+// workgroup barrier.
+template <typename T, int NB>
+__device__ Net<T, NB> stage_net(const NNArgs<T> &a, int inst, T *dst, bool with_wb) {
+  constexpr int HP = 16 * NB;
+  typedef typename V4<T>::type v4;
+  const T *src = a.packed + (int64_t)inst * image_words(HP, a.NH, a.d_in, a.n_out);
+  const int n4 = stage_words(HP, a.NH, a.d_in, a.n_out, with_wb) / 4;
+  const v4 *s4 = reinterpret_cast<const v4 *>(src);
+  v4 *d4 = reinterpret_cast<v4 *>(dst);
+  for (int k = threadIdx.x; k < n4; k += blockDim.x) d4[k] = s4[k];
+  __syncthreads();
+  return net_at<T, NB>(a, dst);
+}
+
+// Four consecutive values of a small array at accumulator position p.
+template <typename T>
+__device__ __forceinline__ typename V4<T>::type ld4(const T *p) {
+  return *reinterpret_cast<const typename V4<T>::type *>(p);
+}
+
+// Layer 1 (VALU): act[mb][i] = tanh(b1[f] + sum_a x[a] W1[f][a]).
+template <typename T, int NB>
+__device__ __forceinline__ void layer_in(const Net<T, NB> &n, const T *x, int q,
+                                         T (&act)[NB][4]) {
+  constexpr int HP = 16 * NB;
+  typedef typename V4<T>::type v4;
+  const int d = n.L.d_in;
+#pragma unroll
+  for (int mb = 0; mb < NB; ++mb) {
+    v4 z = ld4(n.b1 + 16 * mb + 4 * q);
+#pragma unroll
+    for (int k = 0; k < MAXD; ++k)
+      if (k < d) {
+        const v4 w = ld4(n.W1 + k * HP + 16 * mb + 4 * q);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) z[i] = fma(x[k], w[i], z[i]);
+      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) act[mb][i] = dtanh(z[i]);
+  }
+}
+
+// acc[ob] += W_j (rows of block ob) x act, W given as an operand image.
+template <typename T, int NB>
+__device__ __forceinline__ void mma_layer(const T *img, int lane, const T (&b)[NB][4],
+                                          typename V4<T>::type (&acc)[NB]) {
+  typedef typename V4<T>::type v4;
+  const v4 *wp = reinterpret_cast<const v4 *>(img) + lane;
+#pragma unroll
+  for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+    for (int ob = 0; ob < NB; ++ob) {
+      const v4 w = wp[(ob * NB + mb) * 64];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[ob] = mma16(w[i], b[mb][i], acc[ob]);
+    }
+}
+
+// Hidden layer j (MFMA, the accumulator starts at the bias):
+// act <- tanh(b_j + W_j act).
+template <typename T, int NB>
+__device__ __forceinline__ void layer_hidden(const Net<T, NB> &n, int j, int lane,
+                                             T (&act)[NB][4]) {
+  constexpr int HP = 16 * NB;
+  typedef typename V4<T>::type v4;
+  const int q = lane >> 4;
+  v4 acc[NB];
+#pragma unroll
+  for (int ob = 0; ob < NB; ++ob) acc[ob] = ld4(n.bh + j * HP + 16 * ob + 4 * q);
+  mma_layer<T, NB>(n.Wf + j * HP * HP, lane, act, acc);
+#pragma unroll
+  for (int ob = 0; ob < NB; ++ob)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) act[ob][i] = dtanh(acc[ob][i]);
+}
+
+// Output layer (VALU + cross-quarter shuffle): u[o] for the lane's row.
+template <typename T, int NB>
+__device__ __forceinline__ T layer_out(const Net<T, NB> &n, int o, int q,
+                                       const T (&act)[NB][4]) {
+  constexpr int HP = 16 * NB;
+  typedef typename V4<T>::type v4;
+  T s = T(0);
+#pragma unroll
+  for (int mb = 0; mb < NB; ++mb) {
+    const v4 w = ld4(n.WL + o * HP + 16 * mb + 4 * q);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s = fma(act[mb][i], w[i], s);
+  }
+  s += __shfl_xor(s, 16);
+  s += __shfl_xor(s, 32);
+  return s + n.bL[o];
+}
+
+// Access to a residual program's tables by word index: records (i), real
+// values (r), per-entry data (d), the entry's network values (u) and its
+// seed accumulators (sd_add).  PtrProg reads them through pointers.
+template <typename T>
+struct PtrProg {
+  const int32_t *ints;
+  const T *reals;
+  const T *data;
+  const T *uv;
+  T *sd;
+  __device__ __forceinline__ int i(int k) const { return ints[k]; }
+  __device__ __forceinline__ T r(int k) const { return reals[k]; }
+  __device__ __forceinline__ T d(int64_t k) const { return data[k]; }
+  __device__ __forceinline__ T u(int k) const { return uv[k]; }
+  __device__ __forceinline__ void sd_add(int k, T v) const { sd[k] += v; }
+};
+
+// The residual program of one entry e of a group: for each equation
+// (records at word eq_i) the residual r = sum over terms of coefficient x
+// product of factor powers, each factor a stencil sum of network values
+// u(slot * n_out + var); r goes to resid[E[3] + e] when resid is given, the
+// loss terms lambda_d (r - target)^2 and lambda_n (r - target)^2 are added
+// to lsum and lnsum, and the seeds d/du(slot, var) of 2 w (r - target)
+// (times mult[eq * mult_n + e] when mult is given) are added through
+// sd_add (accumulators zeroed by the caller) through every factor of every
+// term: d(prod_j f_j^p_j)/d f_k = p_k f_k^(p_k-1) * prefix_k * suffix_k.
+template <typename T, typename A>
+__device__ __forceinline__ void entry_program(A &&g, int eq_i, int n_eq, int e, int n_out,
+                                              T &lsum, T &lnsum, T *resid, const T *mult,
+                                              int mult_n) {
+  // factor power x**p and its derivative (F: the factor record's word)
+  auto fpw = [&](int F, T x) {
+    return g.i(F + 1) == 0 ? ipow(x, g.i(F + 2)) : dpow(x, g.r(g.i(F + 5)));
+  };
+  auto dfpw = [&](int F, T x) {
+    if (g.i(F + 1) == 0) {
+      const int p = g.i(F + 2);
+      return p == 0 ? T(0) : T(p) * ipow(x, p - 1);
+    }
+    const T p = g.r(g.i(F + 5));
+    return p * dpow(x, p - T(1));
+  };
+  auto coeff = [&](int Tm) {
+    return g.i(Tm + 2) == 0 ? g.r(g.i(Tm + 3)) : g.d((int64_t)g.i(Tm + 3) + e);
+  };
+  auto stencil_sum = [&](int F) {
+    const int var = g.i(F), ns = g.i(F + 3), si = g.i(F + 4);
+    T gs = T(0);
+    for (int k = 0; k < ns; ++k) {
+      const int S = si + k * TDX_STENCIL_INTS;
+      gs = gs + g.u(g.i(S) * n_out + var) * g.r(g.i(S + 1));
+    }
+    return gs;
+  };
+  for (int eq = 0; eq < n_eq; ++eq) {
+    const int E = eq_i + eq * TDX_EQ_INTS;
+    const int n_terms = g.i(E), term_i = g.i(E + 1);
+    T r = T(0);
+    for (int t = 0; t < n_terms; ++t) {
+      const int Tm = term_i + t * TDX_TERM_INTS;
+      const int nf = g.i(Tm), fi = g.i(Tm + 1);
+      T der = T(1);
+      for (int f = 0; f < nf; ++f) {
+        const int F = fi + f * TDX_FACTOR_INTS;
+        der = der * fpw(F, stencil_sum(F));
+      }
+      const T tv = coeff(Tm) * der;
+      r = (t == 0) ? tv : r + tv;
+    }
+    const int tg = g.i(E + 2), ro = g.i(E + 3);
+    const T target = tg >= 0 ? g.d((int64_t)tg + e) : T(0);
+    if (ro >= 0 && resid) resid[ro + e] = r;
+    const T diff = r - target;
+    const T sq = diff * diff;
+    const T lam_d = g.r(g.i(E + 4));
+    lsum += lam_d * sq;
+    lnsum += g.r(g.i(E + 5)) * sq;
+    T dr = T(2) * g.r(g.i(E + 6)) * diff;
+    if (mult) dr *= mult[(int64_t)eq * mult_n + e];
+    for (int t = 0; t < n_terms; ++t) {
+      const int Tm = term_i + t * TDX_TERM_INTS;
+      const int nf = g.i(Tm), fi = g.i(Tm + 1);
+      T gsv[MAXF], pw[MAXF], pre[MAXF + 1], suf[MAXF + 1];
+#pragma unroll
+      for (int f = 0; f < MAXF; ++f) {
+        gsv[f] = T(0);
+        pw[f] = T(1);
+        if (f < nf) {
+          const int F = fi + f * TDX_FACTOR_INTS;
+          gsv[f] = stencil_sum(F);
+          pw[f] = fpw(F, gsv[f]);
+        }
+      }
+      pre[0] = T(1);
+#pragma unroll
+      for (int f = 0; f < MAXF; ++f) pre[f + 1] = pre[f] * pw[f];
+      suf[MAXF] = T(1);
+#pragma unroll
+      for (int f = MAXF - 1; f >= 0; --f) suf[f] = suf[f + 1] * pw[f];
+      const T cdr = coeff(Tm) * dr;
+#pragma unroll
+      for (int f = 0; f < MAXF; ++f) {
+        if (f < nf) {
+          const int F = fi + f * TDX_FACTOR_INTS;
+          const T gf = cdr * dfpw(F, gsv[f]) * pre[f] * suf[f + 1];
+          const int var = g.i(F), ns = g.i(F + 3), si = g.i(F + 4);
+          for (int k = 0; k < ns; ++k) {
+            const int S = si + k * TDX_STENCIL_INTS;
+            g.sd_add(g.i(S) * n_out + var, gf * g.r(g.i(S + 1)));
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// wavefront-private transposes
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wavefront-private transpose buffers are feature-major, [feature][row],
+// with a row stride of 20 words: the per-lane writes (16 rows x 4 quarters)
+// and the 128-bit operand reads (4 consecutive row positions per lane) both
+// touch 64 distinct banks.
+constexpr int WRS = 20;
+template <int HP>
+constexpr int wave_ls() { return WRS * HP / 16; }
+
+// ---------------------------------------------------------------------------
+// k_fused: forward, residual program, adjoint and weight gradients in one
+// pass (the fused training step of narrow float networks)
+// ---------------------------------------------------------------------------
+// A workgroup of two wavefronts takes a range of 64-entry blocks of one
+// group and walks it in rounds of 16 entries.  Wavefront w evaluates the
+// network at its own slots s = w, w + 2, ... (16 rows per slot, one MFMA
+// tile) and keeps every activation of those tiles in registers; the network
+// values go to LDS, where both wavefronts read all slots of the 16 entries.
+// Each wavefront then runs the residual program of the 16 entries itself
+// (its four lane quarters alike, so every lane of a row holds the row's
+// seeds) and back-propagates its own slots' seeds at once.  So the network
+// is evaluated once per step, not again inside the adjoint, and no seed
+// goes through HBM.  The residual program is a policy: the library's kernel
+// interprets the group's tables from an LDS copy (InterpResid); the kernel
+// compiled at plan creation runs straight-line code generated from them.
+// The per-entry arithmetic is the forward kernel's and the wavefront-private
+// backward's, operation for operation: values, seeds and loss pairs (the
+// forward kernel's pairwise tree over each 64-entry block) are the same
+// bits; the weight gradients add the same per-row terms in another order.
+// Requirements (checked at plan creation): one network output, at most two
+// inputs, at most 2 x 16 hidden features, at most two H x H layers, at most
+// FUSE_WAVES x FUSE_SPW slots per group.
+constexpr int FUSE_WAVES = 2;
+constexpr int FUSE_SPW = 4;                         // slots per wavefront
+constexpr int FUSE_MAX_SLOTS = FUSE_WAVES * FUSE_SPW;
+static_assert(FUSE_WAVES == 2, "k_fused picks a wavefront's seeds between two slot sets");
+constexpr int FUSE_PI = 512, FUSE_PR = 256;         // program words of one group
+
+// Words before the network values: the wavefronts' transpose buffers,
+// reused for the final gradient sum (P words).
+__host__ __device__ inline int fused_buf_words(int HP, int P) {
+  return (max(FUSE_WAVES * 2 * WRS * HP, P) + 3) & ~3;
+}
+// Words of the residual policy's LDS area (the interpreter's table copy and
+// seed rows; the generated program uses none of it).
+__host__ __device__ constexpr int fused_resid_words() {
+  return FUSE_PI + FUSE_PR + FUSE_WAVES * 16 * FUSE_MAX_SLOTS;
+}
+// All LDS words of the fused kernel: buffers, network values (two rounds),
+// the loss values of one block (loss, normalized), the residual policy's
+// area and the staged network image.
+__host__ __device__ inline int fused_lds_words(int HP, int NH, int d_in, int n_out, int P) {
+  return fused_buf_words(HP, P) + 2 * 16 * FUSE_MAX_SLOTS + 128 + fused_resid_words() +
+         stage_words(HP, NH, d_in, n_out, true);
+}
+
+// Residual policy of the library's fused kernel: the group's program tables
+// (a self-contained pair, record offsets relative to it) copied to LDS once
+// per workgroup and interpreted by entry_program; seeds gathered in an LDS
+// row per entry, then returned in registers.
+struct InterpResid {
+  const int32_t *pints;
+  const float *preals;
+  const float *data;
+  float *sd;
+  int n_eq;
+  // Copies the tables of group g; the caller's next workgroup barrier makes
+  // them visible.  area: fused_resid_words() words of LDS.
+  __device__ InterpResid(const NNArgs<float> &a, int g, int neq, float *area, int w) {
+    int32_t *pi_l = reinterpret_cast<int32_t *>(area);
+    float *pr_l = area + FUSE_PI;
+    const int4 pi = a.prog_idx[g];
+    for (int k = threadIdx.x; k < pi.y; k += blockDim.x) pi_l[k] = a.prog_ints[pi.x + k];
+    for (int k = threadIdx.x; k < pi.w; k += blockDim.x) pr_l[k] = a.prog_reals[pi.z + k];
+    pints = pi_l;
+    preals = pr_l;
+    data = a.data;
+    sd = area + FUSE_PI + FUSE_PR + w * 16 * FUSE_MAX_SLOTS;
+    n_eq = neq;
+  }
+  // Entry e (valid) of the row: loss values into ls/lns, the seeds of all
+  // slots into sdv (zero where none); u: the row's network values.
+  __device__ __forceinline__ void run(int e, bool valid, int row, const float *u,
+                                      float (&sdv)[FUSE_MAX_SLOTS], float &ls, float &lns) {
+    float *sdr = sd + row * FUSE_MAX_SLOTS;
+#pragma unroll
+    for (int s = 0; s < FUSE_MAX_SLOTS; ++s) sdr[s] = 0.0f;
+    if (valid)
+      entry_program<float>(PtrProg<float>{pints, preals, data, u, sdr}, 0, n_eq, e, 1, ls, lns,
+                           (float *)nullptr, (const float *)nullptr, 0);
+#pragma unroll
+    for (int s = 0; s < FUSE_MAX_SLOTS; ++s) sdv[s] = sdr[s];
+  }
+};
+
+// The fused kernel's body (launch: 64 FUSE_WAVES threads per workgroup,
+// fused_lds_words(...) words of dynamic LDS, one workgroup per work item)
+// with the residual policy R.
+template <int NB, int NH, typename R>
+__device__ __forceinline__ void fused_body(const NNArgs<float> &a, float *smem) {
+  typedef float T;
+  constexpr int HP = 16 * NB;
+  constexpr int FB = WRS * HP;
+  constexpr int NJ = NH > 0 ? NH : 1;
+  typedef typename V4<T>::type v4;
+  const int4 item = a.k3_items[blockIdx.x];
+  const int32_t *G = a.groups + item.x * TDX_GROUP_INTS;
+  const int n_slots = G[2], slot_base_i = G[3], slot_off_r = G[4], n_eq = G[5];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = lane & 15, q = lane >> 4;
+  const int d_in = a.d_in, H = a.H, P = a.P;
+  const Layout<T> L{d_in, 1, H, NH};
+
+  T *dbuf = smem + w * 2 * FB, *abuf = dbuf + FB;
+  T *u_all = smem + fused_buf_words(HP, P);               // [2][16][FUSE_MAX_SLOTS]
+  T *lbuf = u_all + 2 * 16 * FUSE_MAX_SLOTS;              // [2][64]
+  T *rarea = lbuf + 128;
+  T *stage = rarea + fused_resid_words();
+  R resid(a, item.x, n_eq, rarea, w);
+  const Net<T, NB> net = stage_net<T, NB>(a, G[0], stage, true);  // ends with a barrier
+  const T dl_scale = *a.dloss;
+
+  // gradient accumulators, as in the wavefront-private backward (one output
+  // and at most two inputs: the first and last layers on the VALU)
+  v4 dwacc[NJ][NB * NB];
+  T bhacc[NJ][NB];
+  T wlacc[NB][4], w1acc[NB][4][2], b1acc[NB][4];
+  T blacc = T(0);
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+#pragma unroll
+    for (int b = 0; b < NB * NB; ++b) dwacc[j][b] = v4{0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < NB; ++b) bhacc[j][b] = T(0);
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      wlacc[b][i] = T(0);
+      w1acc[b][i][0] = T(0);
+      w1acc[b][i][1] = T(0);
+      b1acc[b][i] = T(0);
+    }
+
+  const int e_end = item.z;
+  int blk = item.w;
+  int rnd = 0;
+  for (int e0 = item.y; e0 < e_end; e0 += 16, ++rnd) {
+    const int e = e0 + row;
+    const bool valid = e < e_end;
+    const int ec = valid ? e : e_end - 1;
+    T *ub = u_all + (rnd & 1) * 16 * FUSE_MAX_SLOTS;
+
+    // forward at the wavefront's slots, every activation kept
+    T act[FUSE_SPW][NH + 1][NB][4];
+    T x[FUSE_SPW][2];
+#pragma unroll
+    for (int k = 0; k < FUSE_SPW; ++k) {
+      const int s = w + FUSE_WAVES * k;
+      if (s < n_slots) {
+        const int64_t base = a.ints[slot_base_i + s];
+        T xs[MAXD];
+#pragma unroll
+        for (int c = 0; c < MAXD; ++c)
+          xs[c] = (c < d_in) ? a.data[base + (int64_t)ec * d_in + c] +
+                                   a.reals[slot_off_r + s * d_in + c]
+                             : T(0);
+        x[k][0] = xs[0];
+        x[k][1] = xs[1];
+        layer_in<T, NB>(net, xs, q, act[k][0]);
+#pragma unroll
+        for (int j = 1; j <= NH; ++j) {
+#pragma unroll
+          for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) act[k][j][mb][i] = act[k][j - 1][mb][i];
+          layer_hidden<T, NB>(net, j - 1, lane, act[k][j]);
+        }
+        const T u = layer_out<T, NB>(net, 0, q, act[k][NH]);
+        if (q == 0) ub[row * FUSE_MAX_SLOTS + s] = u;
+      }
+    }
+    __syncthreads();
+
+    // residual program of the 16 entries; loss values of the block
+    T sdv[FUSE_MAX_SLOTS];
+    T ls = T(0), lns = T(0);
+    resid.run(e, valid, row, ub + row * FUSE_MAX_SLOTS, sdv, ls, lns);
+    if (w == 0) {
+      if ((e0 & 63) == 0) {
+        lbuf[lane] = T(0);
+        lbuf[64 + lane] = T(0);
+      }
+      if (q == 0 && valid) {
+        lbuf[(e0 & 63) + row] = ls;
+        lbuf[64 + (e0 & 63) + row] = lns;
+      }
+      if (((e0 + 16) & 63) == 0 || e0 + 16 >= e_end) {
+        // the forward kernel's pairwise tree over the block's 64 entries
+        T v = lbuf[lane], vn = lbuf[64 + lane];
+#pragma unroll
+        for (int st = 32; st > 0; st >>= 1) {
+          v += __shfl_down(v, st);
+          vn += __shfl_down(vn, st);
+        }
+        if (lane == 0) {
+          a.lp[2 * blk] = v;
+          a.lp[2 * blk + 1] = vn;
+        }
+        ++blk;
+      }
+    }
+
+    // adjoint of the wavefront's slots
+#pragma unroll
+    for (int k = 0; k < FUSE_SPW; ++k) {
+      const int s = w + FUSE_WAVES * k;
+      if (s >= n_slots) continue;
+      const T sk = w == 0 ? sdv[FUSE_WAVES * k] : sdv[FUSE_WAVES * k + 1];
+      const T so = valid ? sk * dl_scale : T(0);
+      if (q == 0) blacc += so;
+      T dl[NB][4];
+#pragma unroll
+      for (int mb = 0; mb < NB; ++mb) {
+        const v4 wl = ld4(net.WL + 16 * mb + 4 * q);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          wlacc[mb][i] = fma(so, act[k][NH][mb][i], wlacc[mb][i]);
+          const T av = act[k][NH][mb][i];
+          dl[mb][i] = fma(wl[i], so, T(0)) * (T(1) - av * av);
+        }
+      }
+#pragma unroll
+      for (int j = NH; j >= 1; --j) {
+        const int jj = j - 1;
+#pragma unroll
+        for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int f = 16 * mb + crow<T>(q, i);
+            dbuf[f * WRS + row] = dl[mb][i];
+            abuf[f * WRS + row] = act[k][j - 1][mb][i];
+          }
+        wave_sync();
+        {
+          v4 av[NB], dv[NB];
+#pragma unroll
+          for (int xb = 0; xb < NB; ++xb) {
+            dv[xb] = *reinterpret_cast<const v4 *>(dbuf + (16 * xb + row) * WRS + 4 * q);
+            av[xb] = *reinterpret_cast<const v4 *>(abuf + (16 * xb + row) * WRS + 4 * q);
+          }
+#pragma unroll
+          for (int ob = 0; ob < NB; ++ob) {
+            bhacc[jj][ob] += (dv[ob][0] + dv[ob][1]) + (dv[ob][2] + dv[ob][3]);
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+              v4 c = dwacc[jj][ob * NB + ib];
+#pragma unroll
+              for (int kb = 0; kb < 4; ++kb) c = mma16(dv[ob][kb], av[ib][kb], c);
+              dwacc[jj][ob * NB + ib] = c;
+            }
+          }
+        }
+        wave_sync();
+        // delta_{j-1} = (W_jj^T delta_j) * (1 - act_{j-1}^2)
+        v4 c[NB];
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) c[ib] = v4{0, 0, 0, 0};
+        mma_layer<T, NB>(net.Wb + jj * HP * HP, lane, dl, c);
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const T av = act[k][j - 1][ib][i];
+            dl[ib][i] = c[ib][i] * (T(1) - av * av);
+          }
+      }
+      // first layer: per-lane sums over the rows, as the backward kernel
+#pragma unroll
+      for (int mb = 0; mb < NB; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const T dv = dl[mb][i];
+          b1acc[mb][i] += dv;
+          w1acc[mb][i][0] = fma(dv, x[k][0], w1acc[mb][i][0]);
+          w1acc[mb][i][1] = fma(dv, x[k][1], w1acc[mb][i][1]);
+        }
+    }
+  }
+
+  // fixed-order sum of the two wavefronts' gradients (the transpose buffers
+  // hold it), then the workgroup's partial
+  __syncthreads();
+  T *cp = smem;
+  for (int ww = 0; ww < FUSE_WAVES; ++ww) {
+    if (w == ww) {
+      const bool first = ww == 0;
+      auto put = [&](int pos, T v) { cp[pos] = first ? v : cp[pos] + v; };
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int out = 16 * ob + crow<T>(q, i);
+          if (out >= H) continue;
+#pragma unroll
+          for (int jj = 0; jj < NH; ++jj)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+              const int in = 16 * ib + row;
+              if (in < H) put(L.oWh(jj) + out * H + in, dwacc[jj][ob * NB + ib][i]);
+            }
+        }
+#pragma unroll
+      for (int ob = 0; ob < NB; ++ob)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int out = 16 * ob + crow<T>(q, i);
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            const T v = rowsum16(w1acc[ob][i][c]);
+            if (row == 0 && out < H && c < d_in) put(L.oW1() + out * d_in + c, v);
+          }
+          const T vb = rowsum16(b1acc[ob][i]);
+          if (row == 0 && out < H) put(L.ob1() + out, vb);
+          const T vl = rowsum16(wlacc[ob][i]);
+          if (row == 0 && out < H) put(L.oWL() + out, vl);
+        }
+#pragma unroll
+      for (int jj = 0; jj < NH; ++jj)
+#pragma unroll
+        for (int xb = 0; xb < NB; ++xb) {
+          T v = bhacc[jj][xb];
+          v += __shfl_xor(v, 16);
+          v += __shfl_xor(v, 32);
+          const int f = 16 * xb + row;
+          if (q == 0 && f < H) put(L.obh(jj) + f, v);
+        }
+      const T bl = rowsum16(blacc);
+      if (row == 0 && q == 0) put(L.obL(), bl);
+    }
+    __syncthreads();
+  }
+  T *out = a.partial + (int64_t)blockIdx.x * P;
+  for (int k = threadIdx.x; k < P; k += blockDim.x) out[k] = cp[k];
+}
+
+}  // namespace tdx
+
+#endif  // TDX_NN_DEV_H

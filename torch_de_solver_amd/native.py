@@ -29,7 +29,8 @@ EXPORTS = [
     'tdx_nn_plan_create', 'tdx_nn_plan_destroy', 'tdx_nn_param_count',
     'tdx_nn_workspace_bytes', 'tdx_nn_forward', 'tdx_nn_backward',
     'tdx_nn_row_count', 'tdx_nn_network_values', 'tdx_nn_forward_values', 'tdx_nn_jacobian',
-    'tdx_nn_train_step', 'tdx_nn_profile', 'tdx_nn_profile_read',
+    'tdx_nn_train_step', 'tdx_nn_profile', 'tdx_nn_profile_read', 'tdx_nn_fused_source',
+    'tdx_nn_step_kernel',
     'tdx_noalloc_scratch_bytes', 'tdx_nn_train_step_noalloc',
     'tdx_mat_plan_create', 'tdx_mat_plan_destroy', 'tdx_mat_workspace_bytes',
     'tdx_mat_forward', 'tdx_mat_backward', 'tdx_mat_train_step', 'tdx_mat_train_step_to',
@@ -134,6 +135,10 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.tdx_noalloc_scratch_bytes.restype = sz
     lib.tdx_noalloc_scratch_bytes.argtypes = []
     lib.tdx_nn_train_step_noalloc.argtypes = [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp]
+    lib.tdx_nn_fused_source.argtypes = [ctypes.POINTER(NNDesc), ctypes.c_char_p, i64,
+                                        ctypes.c_int]
+    lib.tdx_nn_fused_source.restype = i64
+    lib.tdx_nn_step_kernel.argtypes = [vp]
     lib.tdx_nn_profile.argtypes = [vp, ctypes.c_int]
     lib.tdx_nn_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int)]

@@ -187,6 +187,24 @@ class NNTables:
         self.n_inst = len(problems)
         self.causal = (float(prob.causal_tol), int(prob.causal_nt)) if prob.causal_tol else None
 
+    def fused_source(self, arch: MLPArch, compile: bool = False) -> str:
+        """Source of the fused training-step kernel with the residual
+        programs generated from these tables ('' when a plan of them has no
+        fused step; the reason is in tdx_last_error); ``compile`` also
+        compiles it for gfx950 (hipRTC, no device needed)."""
+        desc = self.desc(arch)
+        lib = native.lib()
+        n = int(lib.tdx_nn_fused_source(ctypes.byref(desc), None, 0, 0))
+        if n < 0:
+            native.check(n)
+        if n == 0:
+            return ''
+        buf = ctypes.create_string_buffer(n + 1)
+        rc = int(lib.tdx_nn_fused_source(ctypes.byref(desc), buf, n + 1, 1 if compile else 0))
+        if rc < 0:
+            native.check(rc)
+        return buf.value.decode()
+
     def desc(self, arch: MLPArch, tile_rows: int = 0, chunk_tiles: int = 0,
              persistent: int = 0) -> 'native.NNDesc':
         """The plan descriptor of include/tdx.h over these tables (which must
@@ -345,6 +363,15 @@ class NNEngine:
             native.ptr(loss) if loss is not None else None,
             native.ptr(self.workspace), native.stream_ptr()))
         self.fwd_gen += 1
+
+    @property
+    def step_kernel(self) -> str:
+        """The training step's kernels: 'separate' (forward, then adjoint),
+        'fused' (one kernel interpreting the residual programs) or
+        'fused-generated' (one kernel compiled with the plan's generated
+        residual programs)."""
+        k = int(native.lib().tdx_nn_step_kernel(self._plan))
+        return {0: 'separate', 1: 'fused', 2: 'fused-generated'}[k]
 
     def profile(self, capacity: int) -> None:
         """Time up to ``capacity`` launches per kernel kind with HIP events

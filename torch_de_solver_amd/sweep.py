@@ -23,8 +23,9 @@ class SweepSolver:
         as returned by the builders of ``workloads``; every net must have the
         same architecture.  ``autotune``: pick ``chunk_tiles`` by timing
         candidates (``autotune.tune_nn``); the choice is in ``self.tuning``.
-        ``persistent=1``: the forward kernel as a persistent grid
-        (``tdx_nn_desc.persistent``)."""
+        ``persistent``: ``tdx_nn_desc.persistent`` — 0 fuses forward and
+        adjoint into one kernel where the network allows it, -1 keeps them
+        separate, 1 runs the separate forward as a persistent grid."""
         self.device = torch.device(device or 'cuda')
         self.nets = [p[3] for p in problems]
         self.arch: MLPArch = mlp_arch(self.nets[0])
@@ -57,7 +58,7 @@ class SweepSolver:
         if autotune and chunk_tiles == 0:
             from .autotune import tune_nn
             chunk_tiles, self.tuning = tune_nn(compiled, self.arch, self.device, self.params,
-                                               tile_rows=tile_rows)
+                                               tile_rows=tile_rows, persistent=persistent)
         self.engine = NNEngine(compiled, self.arch, self.device, tile_rows, chunk_tiles,
                                persistent)
         self.exp_avg = torch.zeros_like(self.params)
