@@ -370,9 +370,9 @@ constexpr int wave_ls() { return WRS * HP / 16; }
 // pass (the fused training step of narrow float networks)
 // ---------------------------------------------------------------------------
 // A workgroup of two wavefronts takes a range of 64-entry blocks of one
-// group and walks it in rounds of 16 entries.  Wavefront w evaluates the
-// network at its own slots s = w, w + 2, ... (16 rows per slot, one MFMA
-// tile) and keeps every activation of those tiles in registers; the network
+// group and walks it in rounds of 16 entries.  In round r wavefront w
+// evaluates the network at its own slots s = w', w' + 2, ... with w' = w xor
+// (r mod 2) (16 rows per slot, one MFMA tile) and keeps every activation of those tiles in registers; the network
 // values go to LDS, where both wavefronts read all slots of the 16 entries.
 // Each wavefront then runs the residual program of the 16 entries itself
 // (its four lane quarters alike, so every lane of a row holds the row's
@@ -509,13 +509,17 @@ __device__ __forceinline__ void fused_body(const NNArgs<float> &a, float *smem) 
     const bool valid = e < e_end;
     const int ec = valid ? e : e_end - 1;
     T *ub = u_all + (rnd & 1) * 16 * FUSE_MAX_SLOTS;
+    // slot set of this wavefront in this round (w ^ round parity): with an
+    // odd slot count the larger set alternates between the two wavefronts,
+    // which evens out their work between the round barriers
+    const int ws = w ^ (rnd & 1);
 
     // forward at the wavefront's slots, every activation kept
     T act[FUSE_SPW][NH + 1][NB][4];
     T x[FUSE_SPW][2];
 #pragma unroll
     for (int k = 0; k < FUSE_SPW; ++k) {
-      const int s = w + FUSE_WAVES * k;
+      const int s = ws + FUSE_WAVES * k;
       if (s < n_slots) {
         const int64_t base = a.ints[slot_base_i + s];
         T xs[MAXD];
@@ -573,9 +577,9 @@ __device__ __forceinline__ void fused_body(const NNArgs<float> &a, float *smem) 
     // adjoint of the wavefront's slots
 #pragma unroll
     for (int k = 0; k < FUSE_SPW; ++k) {
-      const int s = w + FUSE_WAVES * k;
+      const int s = ws + FUSE_WAVES * k;
       if (s >= n_slots) continue;
-      const T sk = w == 0 ? sdv[FUSE_WAVES * k] : sdv[FUSE_WAVES * k + 1];
+      const T sk = ws == 0 ? sdv[FUSE_WAVES * k] : sdv[FUSE_WAVES * k + 1];
       const T so = valid ? sk * dl_scale : T(0);
       if (q == 0) blacc += so;
       T dl[NB][4];
