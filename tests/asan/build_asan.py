@@ -22,10 +22,13 @@ BUILD = os.path.join(HERE, '_build')
 LIB = os.path.join(BUILD, 'libtdx_asan.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ROCM_LIB = '/opt/rocm/lib'
+sys.path.insert(0, ROOT)
+from torch_de_solver_amd import build as product_build  # noqa: E402
+
 FLAGS = ['--offload-arch=gfx950', '-O1', '-g', '-std=c++17', '-fPIC', '-fno-omit-frame-pointer',
          '-Xarch_host', '-fsanitize=address', '-ffp-contract=off', '-Wno-unused-result',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
-         '-I', INCLUDE, '-I', CSRC]
+         '-I', INCLUDE, '-I', CSRC, '-I', product_build.GEN]
 
 
 def asan_runtime() -> str:
@@ -53,6 +56,8 @@ def up_to_date() -> bool:
 
 
 def build() -> str:
+    # the generated include of the embedded device headers (as the product build)
+    product_build.write_embed()
     if up_to_date():
         return LIB
     os.makedirs(BUILD, exist_ok=True)

@@ -101,6 +101,29 @@ def main():
                            derivative_points=dp, lambda_operator=1, lambda_bound=10)
         src = MatTables(prob, np.float32).specialized_source(1, compile=False)
         assert 'tdx_mat_jit_step' in src, name
+    # the generator of the fused 'NN' kernel's residual programs (host code
+    # only), on valid tables and on single-entry mutations of them
+    n_fused = 0
+    for name, (builder, layers) in problems.NN_CASES.items():
+        d, eq, b, kw = builder(api, dtype='float32')
+        prob = compile_nn(d.build('NN'), eq.equation_lst, b.build(d.variable_dict), 1,
+                          h=kw.get('h', 1e-3), lambda_operator=kw['lambda_operator'],
+                          lambda_bound=kw['lambda_bound'])
+        arch = MLPArch(d_in=layers[0], width=layers[1], n_out=layers[-1],
+                       n_layers=len(layers) - 1)
+        t = NNTables([prob, prob], arch, np.float32)
+        src = t.fused_source(arch, compile=False)
+        assert src == '' or 'tdx_fused_jit' in src, name
+        n_fused += 1 if src else 0
+        for _ in range(60):
+            i = int(rng.integers(t.ints.size))
+            old = int(t.ints[i])
+            t.ints[i] = int(rng.choice([-1, 0, 1, old + 1, old - 1, 1 << 20]))
+            desc = t.desc(arch)
+            rc = lib.tdx_nn_fused_source(ctypes.byref(desc), None, 0, 0)
+            assert rc >= 0 or rc in (-1, -2, -3, -4, -5), rc
+            t.ints[i] = old
+    assert n_fused >= 1
     # 'autograd'
     for name, (builder, layers) in problems.AD_CASES.items():
         d, eq, b, kw = builder(api, dtype='float64')
