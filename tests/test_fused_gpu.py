@@ -201,11 +201,14 @@ def _small(builder, layers, seed, **kw):
 
 @pytest.mark.parametrize('jit', ['0', '1'])
 @pytest.mark.parametrize('case', ['burgers_12x3', 'burgers_32x1', 'burgers_20x2',
-                                  'legendre_16x2', 'legendre_9x1', 'kdv_10x3'])
+                                  'legendre_16x2', 'legendre_9x1', 'kdv_10x3',
+                                  'legendre_opbc_16x2'])
 def test_fused_shapes_match_separate_kernels(case, jit, monkeypatch):
     """Every instantiated shape: one and two 16-column blocks (widths 9, 10,
     12, 16, 20, 32, padding features included), zero to two H x H layers,
-    one and two inputs, periodic KdV conditions; a grouped plan of three
+    one and two inputs, periodic KdV conditions, an operator condition
+    (one-sided stencil at t = 1) and per-entry coefficients (Legendre's
+    1 - t^2, -2t); a grouped plan of three
     instances of different sizes; interpreted and generated residual
     programs."""
     monkeypatch.setenv('TDX_NN_JIT', jit)
@@ -216,6 +219,7 @@ def test_fused_shapes_match_separate_kernels(case, jit, monkeypatch):
         'legendre_16x2': (problems.legendre_nn, [1, 16, 16, 1]),
         'legendre_9x1': (problems.legendre_nn, [1, 9, 1]),
         'kdv_10x3': (problems.kdv_nn, [2, 10, 10, 10, 1]),
+        'legendre_opbc_16x2': (problems.legendre_nn_opbc, [1, 16, 16, 1]),
     }[case]
     builder, layers = spec
     _print(f"[fused shape] {case}, TDX_NN_JIT={jit}")
