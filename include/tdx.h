@@ -138,8 +138,13 @@ typedef struct tdx_nn_desc {
    * HBM), so a step is three launches (packing, fused kernel, reduction with
    * the Adam update and the loss pairs); the loss pairs are the same bits
    * as with separate kernels, the gradient the same per-row terms summed in
-   * another order.  Otherwise, and with -1, separate forward and adjoint
-   * kernels, one workgroup per work item.  1: separate kernels with the
+   * another order.  Float networks of 48 to 128 hidden features (at most
+   * 64 slots per group, same other conditions) step without a forward
+   * kernel too: their block-owning adjoint, which evaluates the network
+   * again anyway, takes tiles of whole entries, runs the residual program
+   * on them and forms its own seeds (three launches; chunk_tiles counts
+   * such tiles, default 8).  Otherwise, and with -1, separate forward and
+   * adjoint kernels, one workgroup per work item.  1: separate kernels with the
    * persistent forward: when the network's forward image fits in LDS
    * and the plan has no causal or weak-form loss, the forward kernel runs
    * as a grid of CUs x resident workgroups, each walking a contiguous range
@@ -250,8 +255,9 @@ int tdx_nn_train_step_noalloc(const tdx_nn_plan *plan, void *params, void *exp_a
 int64_t tdx_nn_fused_source(const tdx_nn_desc *desc, char *buf, int64_t cap, int compile);
 /* Training-step kernels of a plan: 0 separate forward and adjoint kernels,
  * 1 the fused kernel interpreting the residual programs, 2 the fused kernel
- * compiled with the plan's generated residual programs; -1 for NULL.
- * Diagnostic; no counterpart in the reference. */
+ * compiled with the plan's generated residual programs, 3 the block-owning
+ * adjoint of wide networks forming its own seeds (no forward kernel); -1
+ * for NULL.  Diagnostic; no counterpart in the reference. */
 int tdx_nn_step_kernel(const tdx_nn_plan *plan);
 int tdx_nn_profile(tdx_nn_plan *plan, int capacity);
 int tdx_nn_profile_read(tdx_nn_plan *plan, double *ms, int *count);

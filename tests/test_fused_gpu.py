@@ -113,7 +113,12 @@ def _compare(probs, nets, chunk_tiles=0, expect='fused'):
         assert cf['forward'] == 0 and cf['backward'] == 1 and cf['update'] == 1
     else:
         assert cf == cs
-    assert torch.equal(lf, ls), (lf - ls).abs().max()
+    if expect == 'adjoint-seeds':
+        # the same per-entry loss values, summed per work item of whole-entry
+        # tiles instead of per 64-entry block
+        np.testing.assert_allclose(lf.numpy(), ls.numpy(), rtol=1e-6)
+    else:
+        assert torch.equal(lf, ls), (lf - ls).abs().max()
     # the seeds of every row (separate kernels; the fused kernel forms the
     # same bits) and the float64 / float32 CPU backward of them per instance
     u = sep.network_values(flat0)
@@ -234,6 +239,34 @@ def test_fused_shapes_match_separate_kernels(case, jit, monkeypatch):
         probs.append(prob)
         nets.append(net)
     _compare(probs, nets, expect='fused-generated' if jit == '1' else 'fused')
+
+
+@pytest.mark.parametrize('case', ['burgers_100x4', 'kdv_64x4', 'legendre_opbc_48x2'])
+def test_wide_step_forms_its_own_seeds(case):
+    """Wide float networks (48 to 128 features): the training step's
+    block-owning adjoint evaluates the network on tiles of whole entries,
+    runs the residual program there and forms its own seeds (no forward
+    kernel); against the separate kernels.  A grouped plan of three
+    instances of different sizes (7, 2 and 1 slots per group; tiles of 9,
+    32 and 64 entries, ragged ends)."""
+    builder, layers = {
+        'burgers_100x4': (problems.burgers_nn, [2, 100, 100, 100, 100, 1]),
+        'kdv_64x4': (problems.kdv_nn, [2, 64, 64, 64, 64, 1]),
+        'legendre_opbc_48x2': (problems.legendre_nn_opbc, [1, 48, 48, 1]),
+    }[case]
+    _print(f"[wide step] {case}")
+    probs, nets = [], []
+    for k, size in enumerate((None, 'mid', 'big')):
+        kw = {}
+        if size and builder in (problems.burgers_nn, problems.kdv_nn):
+            kw = {'nx': 23 if size == 'mid' else 41, 'nt': 11 if size == 'mid' else 17}
+        elif size:
+            kw = {'n': 37 if size == 'mid' else 150}
+        prob, net = _small(builder, layers, seed=20 + k, **kw)
+        probs.append(prob)
+        nets.append(net)
+    for chunk in (0, 1):
+        _compare(probs, nets, chunk_tiles=chunk, expect='adjoint-seeds')
 
 
 def _mixed(layers, seed):

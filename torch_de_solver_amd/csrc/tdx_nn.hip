@@ -946,8 +946,26 @@ __device__ __forceinline__ void mma_layer_fm(const T *img, int lane, const T *sr
 __host__ __device__ inline size_t block_lds_words(int HP, int NH) {
   return (size_t)(NH + 1) * HP * CTRS + (size_t)(MAXD + MAXO) * CTRS + 16;
 }
+// With the seeds formed in the kernel (FS): the tile's network values and
+// seeds, [64 rows][n_out] each, and the group's residual program (its
+// self-contained table pair).
+__host__ __device__ inline size_t block_fs_lds_words(int HP, int NH) {
+  return block_lds_words(HP, NH) + 2 * 64 * MAXO + FUSE_PI + FUSE_PR;
+}
 
-template <typename T, int NB, int NH>
+// FS = false: the adjoint of a forward pass whose seeds are in the
+// workspace; work item (group, row start, row end, instance) over the
+// group's slot-major rows.
+// FS = true (the training step without a separate forward kernel): work
+// item (group, entry start, entry end, instance); a 64-row tile holds
+// E = 64 / S whole entries of the group's S slots, entry-major (row
+// el * S + s: entry el, slot s; rows from E * S on are padding).  After the
+// forward recompute every row's network values go to LDS, wavefront 0 runs
+// the residual program of the tile's entries (one per lane; loss values
+// kept per lane over the item) and writes their seeds, which the adjoint
+// then reads from LDS; the item's loss pair (fixed-order lane sum) goes to
+// lp[item].
+template <typename T, int NB, int NH, bool FS>
 __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
   constexpr int HP = 16 * NB;
   constexpr int OBW = (NB + 3) / 4;  // output blocks owned per wavefront
@@ -961,11 +979,17 @@ __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
   T *xbuf = actb + (NH + 1) * HP * CTRS;   // [MAXD][CTRS] coordinates
   T *sbuf = xbuf + MAXD * CTRS;            // [MAXO][CTRS] seeds
   T *red = sbuf + MAXO * CTRS;             // 4 wavefront partials of the output bias
+  T *ubuf = red + 16;                      // FS: [64][n_out] network values
+  T *sdb = ubuf + 64 * MAXO;               // FS: [64][n_out] seeds
+  int32_t *pints = reinterpret_cast<int32_t *>(sdb + 64 * MAXO);  // FS: program
+  T *preals = reinterpret_cast<T *>(pints + FUSE_PI);
   const int4 item = a.k2_items[blockIdx.x];
   const int32_t *G = a.groups + item.x * TDX_GROUP_INTS;
-  const int n_entries = G[1];
+  const int n_entries = G[1], n_slots = G[2];
   const int slot_base_i = G[3], slot_off_r = G[4];
+  const int n_eq = G[5], eq_i = G[6];
   const int64_t row_base = G[7];
+  const int E = FS ? 64 / n_slots : 0;    // entries per tile
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row = lane & 15, q = lane >> 4;
   const int trow = 16 * w + row;  // this lane's row within the tile
@@ -989,6 +1013,14 @@ __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
 #pragma unroll
   for (int u = 0; u < OBW; ++u) sacc[u] = v4{0, 0, 0, 0};
   T blacc = T(0);
+  T lsum = T(0), lnsum = T(0);  // FS: loss values of wavefront 0's entry lanes
+  if constexpr (FS) {
+    // the group's residual program into LDS (visible after the tile loop's
+    // first barrier)
+    const int4 pi = a.prog_idx[item.x];
+    for (int k = threadIdx.x; k < pi.y; k += blockDim.x) pints[k] = a.prog_ints[pi.x + k];
+    for (int k = threadIdx.x; k < pi.w; k += blockDim.x) preals[k] = a.prog_reals[pi.z + k];
+  }
 
   auto layer_buf = [&](int j) { return actb + j * HP * CTRS; };
   // own-row store of a layer's values held as accumulator rows
@@ -1000,13 +1032,24 @@ __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
   };
 
   const int64_t r_begin = item.y, r_end = item.z;
-  for (int64_t t0 = r_begin; t0 < r_end; t0 += TR) {
-    const int64_t r = t0 + trow;
-    const bool valid = r < r_end;
-    const int64_t rc = valid ? r : r_begin;
-    const int lr = (int)(rc - row_base);
-    const int s = lr / n_entries;
-    const int e = lr - s * n_entries;
+  for (int64_t t0 = r_begin; t0 < r_end; t0 += (FS ? E : TR)) {
+    bool valid;
+    int s, e;
+    int64_t rc = 0;
+    if constexpr (FS) {
+      // t0: the tile's first entry; this lane's row is entry el, slot s
+      const int el = trow / n_slots;
+      s = trow - el * n_slots;
+      valid = el < E && t0 + el < r_end;
+      e = (int)(valid ? t0 + el : r_end - 1);
+    } else {
+      const int64_t r = t0 + trow;
+      valid = r < r_end;
+      rc = valid ? r : r_begin;
+      const int lr = (int)(rc - row_base);
+      s = lr / n_entries;
+      e = lr - s * n_entries;
+    }
     T x[MAXD];
     const int64_t base = a.ints[slot_base_i + s];
 #pragma unroll
@@ -1036,6 +1079,28 @@ __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
       store_rows(layer_buf(j), act);
     }
 
+    if constexpr (FS) {
+      // network values of the tile's rows, then the residual program of its
+      // entries (wavefront 0, one entry per lane) writes their seeds
+#pragma unroll
+      for (int o = 0; o < MAXO; ++o)
+        if (o < n_out) {
+          const T u = layer_out<T, NB>(net, o, q, act);
+          if (q == 0) ubuf[trow * n_out + o] = u;
+        }
+      __syncthreads();
+      if (w == 0 && lane < E) {
+        const int so_n = n_slots * n_out;
+        T *sdr = sdb + lane * so_n;
+        for (int k = 0; k < so_n; ++k) sdr[k] = T(0);
+        if (t0 + lane < r_end)
+          entry_program<T>(PtrProg<T>{pints, preals, a.data, ubuf + lane * so_n, sdr}, 0,
+                           n_eq, (int)(t0 + lane), n_out, lsum, lnsum, (T *)nullptr,
+                           (const T *)nullptr, 0);
+      }
+      __syncthreads();
+    }
+
     // seeds and the delta of the last hidden activation (act holds it)
     T dl[NB][4];
 #pragma unroll
@@ -1045,7 +1110,13 @@ __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
 #pragma unroll
     for (int o = 0; o < MAXO; ++o) {
       if (o < n_out) {
-        const T so = valid ? a.seeds[rc * n_out + o] * dl_scale : T(0);
+        T sd = T(0);
+        if constexpr (FS) {
+          sd = sdb[trow * n_out + o];
+        } else {
+          sd = a.seeds[rc * n_out + o];
+        }
+        const T so = valid ? sd * dl_scale : T(0);
         if (o == q) blacc += so;
         if (q == 0) sbuf[o * CTRS + trow] = so;
 #pragma unroll
@@ -1191,6 +1262,19 @@ __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
   if ((int)threadIdx.x < n_out) {
     const int o = threadIdx.x;
     out[L.obL() + o] = ((red[o] + red[4 + o]) + red[8 + o]) + red[12 + o];
+  }
+  if constexpr (FS) {
+    if (w == 0) {
+#pragma unroll
+      for (int st = 32; st > 0; st >>= 1) {
+        lsum += __shfl_down(lsum, st);
+        lnsum += __shfl_down(lnsum, st);
+      }
+      if (lane == 0) {
+        a.lp[2 * blockIdx.x] = lsum;
+        a.lp[2 * blockIdx.x + 1] = lnsum;
+      }
+    }
   }
 }
 
@@ -1649,6 +1733,12 @@ struct tdx_nn_plan {
   // the fused kernel compiled with the plan's generated residual programs
   // (null: the interpreted kernel)
   hipFunction_t fused_fn = nullptr;
+  // training step of wide float networks: the block-owning backward forms
+  // its own seeds (FS work items: tiles of whole entries)
+  int k2_fs = 0;
+  int n_k4 = 0;
+  size_t k4_lds = 0;
+  DevTable k4_items, ik4p, ik4i;
   int acc_lds;
   size_t k1_lds, k2_lds;
   DevTable groups, ints, reals, data, k1_items, k2_items, ik1p, ik1i, ik2p, ik2i;
@@ -1895,12 +1985,13 @@ __host__ __device__ constexpr int block_acc_regs(int NB, int NH) {
   return NH * ((NB + 3) / 4) * NB * 4;
 }
 
-template <typename T, int NB, int NH>
+template <typename T, int NB, int NH, bool FS = false>
 int launch_backward_c(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
   if constexpr (block_acc_regs(NB, NH) <= 192) {
-    raise_lds_limit({(const void *)k_backward_c<T, NB, NH>});
+    raise_lds_limit({(const void *)k_backward_c<T, NB, NH, FS>});
     ProfScope ps(p, 1, s);
-    hipLaunchKernelGGL((k_backward_c<T, NB, NH>), dim3(p->n_k2), dim3(256), p->k2_lds, s, a);
+    hipLaunchKernelGGL((k_backward_c<T, NB, NH, FS>), dim3(FS ? p->n_k4 : p->n_k2), dim3(256),
+                       FS ? p->k4_lds : p->k2_lds, s, a);
     TDX_TRY_HIP(hipGetLastError());
     return TDX_OK;
   } else {
@@ -1987,6 +2078,25 @@ int launch_fused(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
   return TDX_EUNSUPPORTED;
 }
 
+// The block-owning backward forming its own seeds (the training step of
+// wide float networks without a separate forward kernel); the items and
+// their lists are the FS work items.
+template <typename T, int NB>
+int launch_backward_fs(const tdx_nn_plan *p, NNArgs<T> a, hipStream_t s) {
+  a.k2_items = (const int4 *)p->k4_items.ptr;
+  if constexpr (sizeof(T) == 4 && NB >= 3) {
+    switch (p->NH) {
+      case 1: return launch_backward_c<T, NB, 1, true>(p, a, s);
+      case 2: return launch_backward_c<T, NB, 2, true>(p, a, s);
+      case 3: return launch_backward_c<T, NB, 3, true>(p, a, s);
+      case 4: return launch_backward_c<T, NB, 4, true>(p, a, s);
+      default: break;
+    }
+  }
+  set_error("no block-owning step for this shape");
+  return TDX_EUNSUPPORTED;
+}
+
 template <typename T>
 int launch_reduce(const tdx_nn_plan *p, const NNArgs<T> &a, int mode, T *dparams, T *params,
                   T *m, T *v, const int64_t *step, AdamHyper h, hipStream_t s,
@@ -2035,6 +2145,22 @@ int nn_dispatch(const tdx_nn_plan *p, int op, const void *params, void *params_m
       a.step_inc = nullptr;                                                       \
       if (!rc) rc = launch_backward<T, NBV>(p, a, s);                             \
       if (!rc) rc = launch_reduce<T>(p, a, 1, nullptr, (T *)params_mut, (T *)m,  \
+                                     (T *)v, step, h, s, (T *)loss, 1);           \
+    } else if (p->k2_fs) {                                                        \
+      /* three launches: packing (unit seed scale, step), the block-owning */    \
+      /* backward forming its own seeds and loss pairs, the reduction */         \
+      a.one_out = (T *)((char *)ws + p->off_one);                                 \
+      a.step_inc = step;                                                          \
+      rc = launch_pack<T, NBV>(p, a, s);                                          \
+      a.one_out = nullptr;                                                        \
+      a.step_inc = nullptr;                                                       \
+      if (!rc) rc = launch_backward_fs<T, NBV>(p, a, s);                          \
+      NNArgs<T> ar = a;                                                           \
+      ar.inst_k2_ptr = (const int32_t *)p->ik4p.ptr;                              \
+      ar.inst_k2_idx = (const int32_t *)p->ik4i.ptr;                              \
+      ar.inst_k1_ptr = (const int32_t *)p->ik4p.ptr;                              \
+      ar.inst_k1_idx = (const int32_t *)p->ik4i.ptr;                              \
+      if (!rc) rc = launch_reduce<T>(p, ar, 1, nullptr, (T *)params_mut, (T *)m, \
                                      (T *)v, step, h, s, (T *)loss, 1);           \
     } else if (p->fused) {                                                        \
       /* three launches: packing (unit seed scale, step), the fused */           \
@@ -2429,6 +2555,40 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
     }
   }
 
+  // wide float networks: the training step's block-owning backward forms
+  // its own seeds over tiles of whole entries (E = 64 / slots per tile),
+  // chunk_tiles tiles per work item (default 8)
+  std::vector<int4> k4;
+  std::vector<int32_t> ik4p(1, 0), ik4i;
+  if (p->k2_block && !p->fused && d->dtype == 0 && max_slots <= 64 && d->tile_rows == 0 &&
+      d->persistent == 0 && d->causal_tol == 0.0 && d->weak_levels == 0 &&
+      block_fs_lds_words(p->HP, p->NH) * ts <= LDS_MAX && fused_programs(d, &progs)) {
+    for (const FusedProg &fp : progs) {
+      prog_idx.push_back(make_int4((int)prog_ints.size(), (int)fp.ints.size(),
+                                   (int)prog_reals.size(), (int)fp.reals.size()));
+      prog_ints.insert(prog_ints.end(), fp.ints.begin(), fp.ints.end());
+      prog_reals.insert(prog_reals.end(), fp.reals.begin(), fp.reals.end());
+    }
+    std::vector<std::vector<int>> inst_k4(d->n_inst);
+    const int C = d->chunk_tiles > 0 ? d->chunk_tiles : 8;
+    for (int g = 0; g < d->n_groups; ++g) {
+      const int32_t *G = d->groups + (int64_t)g * TDX_GROUP_INTS;
+      const int E = 64 / G[2];
+      const int ntile = (G[1] + E - 1) / E;
+      for (int j = 0; j < ntile; j += C) {
+        inst_k4[G[0]].push_back((int)k4.size());
+        k4.push_back(make_int4(g, j * E, std::min(G[1], (j + C) * E), G[0]));
+      }
+    }
+    for (int i = 0; i < d->n_inst; ++i) {
+      for (int x : inst_k4[i]) ik4i.push_back(x);
+      ik4p.push_back((int32_t)ik4i.size());
+    }
+    p->k2_fs = 1;
+    p->n_k4 = (int)k4.size();
+    p->k4_lds = block_fs_lds_words(p->HP, p->NH) * ts;
+  }
+
   int r2 = TDX_OK;
   if (!r2) r2 = p->groups.upload(d->groups, (size_t)d->n_groups * TDX_GROUP_INTS * 4);
   if (!r2) r2 = p->ints.upload(d->ints, (size_t)std::max<int64_t>(d->n_ints, 1) * 4);
@@ -2440,6 +2600,14 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   if (!r2) r2 = p->ik1i.upload(ik1i.data(), ik1i.size() * 4);
   if (!r2) r2 = p->ik2p.upload(ik2p.data(), ik2p.size() * 4);
   if (!r2) r2 = p->ik2i.upload(ik2i.data(), ik2i.size() * 4);
+  if (!r2 && p->k2_fs) {
+    r2 = p->k4_items.upload(k4.data(), k4.size() * sizeof(int4));
+    if (!r2) r2 = p->ik4p.upload(ik4p.data(), ik4p.size() * 4);
+    if (!r2) r2 = p->ik4i.upload(ik4i.data(), ik4i.size() * 4);
+    if (!r2) r2 = p->prog_idx.upload(prog_idx.data(), prog_idx.size() * sizeof(int4));
+    if (!r2) r2 = p->prog_ints.upload(prog_ints.data(), prog_ints.size() * 4);
+    if (!r2) r2 = p->prog_reals.upload(prog_reals.data(), prog_reals.size() * 4);
+  }
   if (!r2 && p->fused) {
     r2 = p->k3_items.upload(k3.data(), k3.size() * sizeof(int4));
     if (!r2) r2 = p->ik3p.upload(ik3p.data(), ik3p.size() * 4);
@@ -2488,9 +2656,9 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   p->off_seeds = off;
   off = align_up(off + (size_t)n_rows * p->n_out * ts, 256);
   p->off_lp = off;
-  off = align_up(off + (size_t)p->n_k1 * 2 * ts, 256);
+  off = align_up(off + (size_t)std::max(p->n_k1, p->n_k4) * 2 * ts, 256);
   p->off_partial = off;
-  off = align_up(off + (size_t)std::max(p->n_k2, p->n_k3) * p->P * ts, 256);
+  off = align_up(off + (size_t)std::max({p->n_k2, p->n_k3, p->n_k4}) * p->P * ts, 256);
   p->off_one = off;
   off = align_up(off + ts, 256);
   if (d->causal_tol != 0.0) {
@@ -2570,7 +2738,8 @@ extern "C" void tdx_nn_plan_destroy(tdx_nn_plan *p) {
   DevTable *t[] = {&p->groups,   &p->ints,     &p->reals,     &p->data,     &p->k1_items,
                    &p->k2_items, &p->ik1p,     &p->ik1i,      &p->ik2p,     &p->ik2i,
                    &p->k1_bounds, &p->k3_items, &p->ik3p,     &p->ik3i,     &p->prog_idx,
-                   &p->prog_ints, &p->prog_reals, &p->prog_kind};
+                   &p->prog_ints, &p->prog_reals, &p->prog_kind, &p->k4_items, &p->ik4p,
+                   &p->ik4i};
   for (DevTable *x : t) x->release();
   delete p;
 }
@@ -2624,6 +2793,7 @@ extern "C" int64_t tdx_nn_fused_source(const tdx_nn_desc *d, char *buf, int64_t 
 
 extern "C" int tdx_nn_step_kernel(const tdx_nn_plan *p) {
   if (!p) return -1;
+  if (p->k2_fs) return 3;
   return p->fused ? (p->fused_fn ? 2 : 1) : 0;
 }
 
