@@ -242,22 +242,25 @@ int tdx_nn_train_step_noalloc(const tdx_nn_plan *plan, void *params, void *exp_a
  * resets the counts).  capacity 0 turns timing off.  Not for use under
  * graph capture.  No counterpart in the reference (measurement only). */
 #define TDX_NN_KERNEL_KINDS 3
-/* Source of the fused training-step kernel that a plan of `desc` compiles
- * at creation with its generated residual programs (float networks the
- * fused step serves; see tdx_nn_desc.persistent), copied into buf (cap
+/* Source of the training-step kernel that a plan of `desc` compiles at
+ * creation with its generated residual programs (float networks the fused
+ * step serves, or wide ones whose adjoint forms its own seeds; see
+ * tdx_nn_desc.persistent), copied into buf (cap
  * bytes, NUL-terminated; buf may be NULL); with compile != 0 also compiled
  * by hipRTC for gfx950 (no device needed).  Returns the source length, 0
  * when the plan has no fused step (reason in tdx_last_error) or a negative
  * error code.  Plans generate the programs when they hold at least 2^20
- * network rows, or always / never with the environment variable
- * TDX_NN_JIT=1 / 0, and fall back to the interpreted fused kernel if
- * compiling fails.  Diagnostic; no counterpart in the reference. */
+ * network rows (fused step) or 2^17 (wide networks), or always / never with
+ * the environment variable TDX_NN_JIT=1 / 0, and fall back to the library's
+ * interpreting kernel if compiling fails.  Diagnostic; no counterpart in the
+ * reference. */
 int64_t tdx_nn_fused_source(const tdx_nn_desc *desc, char *buf, int64_t cap, int compile);
 /* Training-step kernels of a plan: 0 separate forward and adjoint kernels,
  * 1 the fused kernel interpreting the residual programs, 2 the fused kernel
  * compiled with the plan's generated residual programs, 3 the block-owning
- * adjoint of wide networks forming its own seeds (no forward kernel); -1
- * for NULL.  Diagnostic; no counterpart in the reference. */
+ * adjoint of wide networks forming its own seeds (no forward kernel) with
+ * the interpreted programs, 4 the same compiled with generated programs;
+ * -1 for NULL.  Diagnostic; no counterpart in the reference. */
 int tdx_nn_step_kernel(const tdx_nn_plan *plan);
 int tdx_nn_profile(tdx_nn_plan *plan, int capacity);
 int tdx_nn_profile_read(tdx_nn_plan *plan, double *ms, int *count);

@@ -113,7 +113,7 @@ def _compare(probs, nets, chunk_tiles=0, expect='fused'):
         assert cf['forward'] == 0 and cf['backward'] == 1 and cf['update'] == 1
     else:
         assert cf == cs
-    if expect == 'adjoint-seeds':
+    if expect.startswith('adjoint-seeds'):
         # the same per-entry loss values, summed per work item of whole-entry
         # tiles instead of per 64-entry block
         np.testing.assert_allclose(lf.numpy(), ls.numpy(), rtol=1e-6)
@@ -241,20 +241,23 @@ def test_fused_shapes_match_separate_kernels(case, jit, monkeypatch):
     _compare(probs, nets, expect='fused-generated' if jit == '1' else 'fused')
 
 
+@pytest.mark.parametrize('jit', ['0', '1'])
 @pytest.mark.parametrize('case', ['burgers_100x4', 'kdv_64x4', 'legendre_opbc_48x2'])
-def test_wide_step_forms_its_own_seeds(case):
+def test_wide_step_forms_its_own_seeds(case, jit, monkeypatch):
     """Wide float networks (48 to 128 features): the training step's
     block-owning adjoint evaluates the network on tiles of whole entries,
     runs the residual program there and forms its own seeds (no forward
     kernel); against the separate kernels.  A grouped plan of three
     instances of different sizes (7, 2 and 1 slots per group; tiles of 9,
-    32 and 64 entries, ragged ends)."""
+    32 and 64 entries, ragged ends); interpreted and generated residual
+    programs."""
+    monkeypatch.setenv('TDX_NN_JIT', jit)
     builder, layers = {
         'burgers_100x4': (problems.burgers_nn, [2, 100, 100, 100, 100, 1]),
         'kdv_64x4': (problems.kdv_nn, [2, 64, 64, 64, 64, 1]),
         'legendre_opbc_48x2': (problems.legendre_nn_opbc, [1, 48, 48, 1]),
     }[case]
-    _print(f"[wide step] {case}")
+    _print(f"[wide step] {case}, TDX_NN_JIT={jit}")
     probs, nets = [], []
     for k, size in enumerate((None, 'mid', 'big')):
         kw = {}
@@ -266,7 +269,8 @@ def test_wide_step_forms_its_own_seeds(case):
         probs.append(prob)
         nets.append(net)
     for chunk in (0, 1):
-        _compare(probs, nets, chunk_tiles=chunk, expect='adjoint-seeds')
+        _compare(probs, nets, chunk_tiles=chunk,
+                 expect='adjoint-seeds-generated' if jit == '1' else 'adjoint-seeds')
 
 
 def _mixed(layers, seed):

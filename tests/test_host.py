@@ -299,16 +299,41 @@ def test_fused_step_kernel_generates_and_compiles():
     assert 'for (' not in op.replace('for (int s = 0; s < FUSE_MAX_SLOTS', '')
 
 
-def test_fused_step_kernel_not_for_double_or_wide_networks():
+def test_fused_step_kernel_not_for_double_or_many_slots():
     t, arch = _sweep_tables([0], np.float64)
     assert t.fused_source(arch) == ''
     assert 'fused step' in native.lib().tdx_last_error().decode()
+    # a narrow network with nine shifted copies per point (mixed derivative)
+    from torch_de_solver_amd.compiler import compile_nn
+    from torch_de_solver_amd.nn_engine import NNTables, mlp_arch
+    d = api.Domain()
+    d.variable('x', [0, 1], 9)
+    d.variable('t', [0, 1], 7)
+    b = api.Conditions()
+    b.dirichlet({'x': [0, 1], 't': 0}, value=0.0)
+    eq = api.Equation()
+    eq.add({'d2u/dx2': {'coeff': 1, 'term': [0, 0], 'pow': 1},
+            'd2u/dxdt': {'coeff': 0.5, 'term': [0, 1], 'pow': 1},
+            'd2u/dt2': {'coeff': 1, 'term': [1, 1], 'pow': 1}})
+    prob = compile_nn(d.build('NN'), eq.equation_lst, b.build(d.variable_dict), 1, h=0.001)
+    net = problems.make_mlp([2, 16, 16, 1])
+    assert NNTables([prob], mlp_arch(net), np.float32).fused_source(mlp_arch(net)) == ''
+
+
+def test_wide_step_kernel_generates_and_compiles():
+    """Wide float networks (here 2-100x4-1 on Burgers, two instances): the
+    block-owning kernel that forms its own seeds, with generated residual
+    programs, compiles for gfx950 (hipRTC, no device)."""
     from torch_de_solver_amd.compiler import compile_nn
     from torch_de_solver_amd.nn_engine import NNTables, mlp_arch
     d, eq, b, kw = problems.burgers_nn(api)
-    net = problems.make_mlp([2, 40, 40, 1])
     prob = compile_nn(d.build('NN'), eq.equation_lst, b.build(d.variable_dict), 1, h=0.001)
-    assert NNTables([prob], mlp_arch(net), np.float32).fused_source(mlp_arch(net)) == ''
+    net = problems.make_mlp([2, 100, 100, 100, 100, 1])
+    src = NNTables([prob, prob], mlp_arch(net), np.float32).fused_source(mlp_arch(net),
+                                                                        compile=True)
+    assert 'tdx_block_jit' in src and 'block_body<float, 7, 3, true' in src
+    gen = src[src.index('struct GenBlockResid'):]
+    assert gen.count('    case ') == 2  # the operator and the Dirichlet conditions
 
 
 def _ad_desc(name='wave_ad'):

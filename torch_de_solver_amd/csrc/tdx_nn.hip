@@ -896,386 +896,13 @@ k_fused(NNArgs<float> a) {
   fused_body<NB, NH, InterpResid>(a, reinterpret_cast<float *>(smem_raw));
 }
 
-// Block-owning backward for wide float networks (HP = 16 NB up to 128).
-// A workgroup of 4 wavefronts walks tiles of 64 rows, 16 per wavefront.
-// Each wavefront recomputes the forward pass of its 16 rows and stores every
-// layer's activations of the tile in feature-major LDS buffers
-// ([layer][feature][row], row stride 68: 128-bit operand loads and the
-// row-wise stores both free of bank conflicts); the deltas of the current
-// layer go to one more such buffer.  The weight gradient dW = delta^T act
-// over the 64 rows is computed block by block: wavefront w owns the output
-// blocks ob = w, w + 4, ... of every layer and keeps their sums in
-// accumulator registers over the whole work item, so no gradient is read
-// back or combined between wavefronts until the end.
-constexpr int CTRS = 68;  // row stride of the feature-major tile buffers
-
-// acc[ob] += W (rows of block ob) x b where b is this lane's row trow of a
-// feature-major LDS buffer (stride CTRS); one input block at a time, so only
-// that block's four operands and the NB accumulators are live.
-template <typename T, int NB>
-__device__ __forceinline__ void mma_layer_fm(const T *img, int lane, const T *src, int trow,
-                                             typename V4<T>::type (&acc)[NB]) {
-  typedef typename V4<T>::type v4;
-  const int q = lane >> 4;
-  const v4 *wp = reinterpret_cast<const v4 *>(img) + lane;
-  // two input blocks per round, the second block's weights loaded with the
-  // first's: their latency is paid once per pair
-  auto block = [&](int mb, const v4 (&w)[NB]) {
-    T b[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) b[i] = src[(16 * mb + crow<T>(q, i)) * CTRS + trow];
-#pragma unroll
-    for (int ob = 0; ob < NB; ++ob)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc[ob] = mma16(w[ob][i], b[i], acc[ob]);
-  };
-#pragma unroll 1
-  for (int mb = 0; mb < NB; mb += 2) {
-    v4 wa[NB], wb[NB];
-#pragma unroll
-    for (int ob = 0; ob < NB; ++ob) wa[ob] = wp[(ob * NB + mb) * 64];
-    if (mb + 1 < NB) {
-#pragma unroll
-      for (int ob = 0; ob < NB; ++ob) wb[ob] = wp[(ob * NB + mb + 1) * 64];
-    }
-    block(mb, wa);
-    if (mb + 1 < NB) block(mb + 1, wb);
-  }
-}
-
-__host__ __device__ inline size_t block_lds_words(int HP, int NH) {
-  return (size_t)(NH + 1) * HP * CTRS + (size_t)(MAXD + MAXO) * CTRS + 16;
-}
-// With the seeds formed in the kernel (FS): the tile's network values and
-// seeds, [64 rows][n_out] each, and the group's residual program (its
-// self-contained table pair).
-__host__ __device__ inline size_t block_fs_lds_words(int HP, int NH) {
-  return block_lds_words(HP, NH) + 2 * 64 * MAXO + FUSE_PI + FUSE_PR;
-}
-
-// FS = false: the adjoint of a forward pass whose seeds are in the
-// workspace; work item (group, row start, row end, instance) over the
-// group's slot-major rows.
-// FS = true (the training step without a separate forward kernel): work
-// item (group, entry start, entry end, instance); a 64-row tile holds
-// E = 64 / S whole entries of the group's S slots, entry-major (row
-// el * S + s: entry el, slot s; rows from E * S on are padding).  After the
-// forward recompute every row's network values go to LDS, wavefront 0 runs
-// the residual program of the tile's entries (one per lane; loss values
-// kept per lane over the item) and writes their seeds, which the adjoint
-// then reads from LDS; the item's loss pair (fixed-order lane sum) goes to
-// lp[item].
+// k_backward_c: the block-owning backward (body: block_body in
+// tdx_nn_dev.h) with the interpreted residual program where it forms its
+// own seeds.
 template <typename T, int NB, int NH, bool FS>
 __global__ __launch_bounds__(256) void k_backward_c(NNArgs<T> a) {
-  constexpr int HP = 16 * NB;
-  constexpr int OBW = (NB + 3) / 4;  // output blocks owned per wavefront
-  constexpr int TR = 64;
-  typedef typename V4<T>::type v4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T *smem = reinterpret_cast<T *>(smem_raw);
-  // [NH+1][HP][CTRS] activations; the delta of layer j replaces layer j's
-  // activations once their last readers are done (dW_{j+1} and delta_j)
-  T *actb = smem;
-  T *xbuf = actb + (NH + 1) * HP * CTRS;   // [MAXD][CTRS] coordinates
-  T *sbuf = xbuf + MAXD * CTRS;            // [MAXO][CTRS] seeds
-  T *red = sbuf + MAXO * CTRS;             // 4 wavefront partials of the output bias
-  T *ubuf = red + 16;                      // FS: [64][n_out] network values
-  T *sdb = ubuf + 64 * MAXO;               // FS: [64][n_out] seeds
-  int32_t *pints = reinterpret_cast<int32_t *>(sdb + 64 * MAXO);  // FS: program
-  T *preals = reinterpret_cast<T *>(pints + FUSE_PI);
-  const int4 item = a.k2_items[blockIdx.x];
-  const int32_t *G = a.groups + item.x * TDX_GROUP_INTS;
-  const int n_entries = G[1], n_slots = G[2];
-  const int slot_base_i = G[3], slot_off_r = G[4];
-  const int n_eq = G[5], eq_i = G[6];
-  const int64_t row_base = G[7];
-  const int E = FS ? 64 / n_slots : 0;    // entries per tile
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int row = lane & 15, q = lane >> 4;
-  const int trow = 16 * w + row;  // this lane's row within the tile
-  const int H = a.H, d_in = a.d_in, n_out = a.n_out, P = a.P;
-  const Net<T, NB> net = make_net<T, NB>(a, item.w);
-  const Layout<T> &L = net.L;
-  const T dl_scale = *a.dloss;
-  const int col_b1 = d_in, col_wl = d_in + 1;
-
-  v4 dwacc[NH][OBW][NB];
-  v4 sacc[OBW];
-  T bhacc[NH][OBW];
-#pragma unroll
-  for (int j = 0; j < NH; ++j)
-#pragma unroll
-    for (int u = 0; u < OBW; ++u) {
-      bhacc[j][u] = T(0);
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib) dwacc[j][u][ib] = v4{0, 0, 0, 0};
-    }
-#pragma unroll
-  for (int u = 0; u < OBW; ++u) sacc[u] = v4{0, 0, 0, 0};
-  T blacc = T(0);
-  T lsum = T(0), lnsum = T(0);  // FS: loss values of wavefront 0's entry lanes
-  if constexpr (FS) {
-    // the group's residual program into LDS (visible after the tile loop's
-    // first barrier)
-    const int4 pi = a.prog_idx[item.x];
-    for (int k = threadIdx.x; k < pi.y; k += blockDim.x) pints[k] = a.prog_ints[pi.x + k];
-    for (int k = threadIdx.x; k < pi.w; k += blockDim.x) preals[k] = a.prog_reals[pi.z + k];
-  }
-
-  auto layer_buf = [&](int j) { return actb + j * HP * CTRS; };
-  // own-row store of a layer's values held as accumulator rows
-  auto store_rows = [&](T *buf, const T (&v)[NB][4]) {
-#pragma unroll
-    for (int mb = 0; mb < NB; ++mb)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) buf[(16 * mb + crow<T>(q, i)) * CTRS + trow] = v[mb][i];
-  };
-
-  const int64_t r_begin = item.y, r_end = item.z;
-  for (int64_t t0 = r_begin; t0 < r_end; t0 += (FS ? E : TR)) {
-    bool valid;
-    int s, e;
-    int64_t rc = 0;
-    if constexpr (FS) {
-      // t0: the tile's first entry; this lane's row is entry el, slot s
-      const int el = trow / n_slots;
-      s = trow - el * n_slots;
-      valid = el < E && t0 + el < r_end;
-      e = (int)(valid ? t0 + el : r_end - 1);
-    } else {
-      const int64_t r = t0 + trow;
-      valid = r < r_end;
-      rc = valid ? r : r_begin;
-      const int lr = (int)(rc - row_base);
-      s = lr / n_entries;
-      e = lr - s * n_entries;
-    }
-    T x[MAXD];
-    const int64_t base = a.ints[slot_base_i + s];
-#pragma unroll
-    for (int k = 0; k < MAXD; ++k)
-      x[k] = (k < d_in) ? a.data[base + (int64_t)e * d_in + k] + a.reals[slot_off_r + s * d_in + k]
-                        : T(0);
-    if (q == 0) {
-#pragma unroll
-      for (int k = 0; k < MAXD; ++k) xbuf[k * CTRS + trow] = x[k];
-    }
-
-    // forward recompute of this wavefront's 16 rows, every layer to LDS
-    T act[NB][4];
-    layer_in<T, NB>(net, x, q, act);
-    store_rows(layer_buf(0), act);
-#pragma unroll 1
-    for (int j = 1; j <= NH; ++j) {
-      v4 acc[NB];
-#pragma unroll
-      for (int ob = 0; ob < NB; ++ob) acc[ob] = ld4(net.bh + (j - 1) * HP + 16 * ob + 4 * q);
-      wave_sync();  // the wavefront's own rows of layer j-1 are in LDS
-      mma_layer_fm<T, NB>(net.Wf + (j - 1) * HP * HP, lane, layer_buf(j - 1), trow, acc);
-#pragma unroll
-      for (int ob = 0; ob < NB; ++ob)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) act[ob][i] = dtanh(acc[ob][i]);
-      store_rows(layer_buf(j), act);
-    }
-
-    if constexpr (FS) {
-      // network values of the tile's rows, then the residual program of its
-      // entries (wavefront 0, one entry per lane) writes their seeds
-#pragma unroll
-      for (int o = 0; o < MAXO; ++o)
-        if (o < n_out) {
-          const T u = layer_out<T, NB>(net, o, q, act);
-          if (q == 0) ubuf[trow * n_out + o] = u;
-        }
-      __syncthreads();
-      if (w == 0 && lane < E) {
-        const int so_n = n_slots * n_out;
-        T *sdr = sdb + lane * so_n;
-        for (int k = 0; k < so_n; ++k) sdr[k] = T(0);
-        if (t0 + lane < r_end)
-          entry_program<T>(PtrProg<T>{pints, preals, a.data, ubuf + lane * so_n, sdr}, 0,
-                           n_eq, (int)(t0 + lane), n_out, lsum, lnsum, (T *)nullptr,
-                           (const T *)nullptr, 0);
-      }
-      __syncthreads();
-    }
-
-    // seeds and the delta of the last hidden activation (act holds it)
-    T dl[NB][4];
-#pragma unroll
-    for (int mb = 0; mb < NB; ++mb)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) dl[mb][i] = T(0);
-#pragma unroll
-    for (int o = 0; o < MAXO; ++o) {
-      if (o < n_out) {
-        T sd = T(0);
-        if constexpr (FS) {
-          sd = sdb[trow * n_out + o];
-        } else {
-          sd = a.seeds[rc * n_out + o];
-        }
-        const T so = valid ? sd * dl_scale : T(0);
-        if (o == q) blacc += so;
-        if (q == 0) sbuf[o * CTRS + trow] = so;
-#pragma unroll
-        for (int mb = 0; mb < NB; ++mb) {
-          const v4 wl = ld4(net.WL + o * HP + 16 * mb + 4 * q);
-#pragma unroll
-          for (int i = 0; i < 4; ++i) dl[mb][i] = fma(wl[i], so, dl[mb][i]);
-        }
-      }
-    }
-#pragma unroll
-    for (int mb = 0; mb < NB; ++mb)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) dl[mb][i] *= (T(1) - act[mb][i] * act[mb][i]);
-    __syncthreads();
-    // output layer W: columns col_wl + o of the owned small-term tiles,
-    // B = the seed column o (zero elsewhere)
-    {
-      const T *aN = layer_buf(NH);
-      const int o = row - col_wl;
-#pragma unroll
-      for (int u = 0; u < OBW; ++u) {
-        const int ob = w + 4 * u;
-        if (ob < NB) {
-#pragma unroll
-          for (int kk = 0; kk < TR / 16; ++kk) {
-            const v4 av = *reinterpret_cast<const v4 *>(aN + (16 * ob + row) * CTRS + 16 * kk + 4 * q);
-            const v4 sv = (o >= 0 && o < n_out)
-                              ? *reinterpret_cast<const v4 *>(sbuf + o * CTRS + 16 * kk + 4 * q)
-                              : v4{0, 0, 0, 0};
-#pragma unroll
-            for (int kb = 0; kb < 4; ++kb) sacc[u] = mma16(av[kb], sv[kb], sacc[u]);
-          }
-        }
-      }
-    }
-    __syncthreads();  // the last activations' readers are done
-
-#pragma unroll
-    for (int j = NH; j >= 1; --j) {
-      const int jj = j - 1;
-      const T *ain = layer_buf(j - 1);
-      T *dbuf = layer_buf(j);  // delta_j over act_j (read for the last time above)
-      store_rows(dbuf, dl);
-      __syncthreads();
-      // owned blocks of dW_jj over the 64 rows; hidden bias from the delta
-      // operand (each lane holds 4 rows of one feature per row group)
-#pragma unroll
-      for (int u = 0; u < OBW; ++u) {
-        const int ob = w + 4 * u;
-        if (ob < NB) {
-          // row groups outermost (not unrolled): only one group's operands
-          // are live; each block still sums its rows in the same order
-#pragma unroll 1
-          for (int kk = 0; kk < TR / 16; ++kk) {
-            const v4 dv = *reinterpret_cast<const v4 *>(dbuf + (16 * ob + row) * CTRS + 16 * kk + 4 * q);
-            bhacc[jj][u] += (dv[0] + dv[1]) + (dv[2] + dv[3]);
-#pragma unroll
-            for (int ib = 0; ib < NB; ++ib) {
-              const v4 av = *reinterpret_cast<const v4 *>(ain + (16 * ib + row) * CTRS + 16 * kk + 4 * q);
-              v4 c = dwacc[jj][u][ib];
-#pragma unroll
-              for (int kb = 0; kb < 4; ++kb) c = mma16(dv[kb], av[kb], c);
-              dwacc[jj][u][ib] = c;
-            }
-          }
-        }
-      }
-      // delta_{j-1} = (W_jj^T delta_j) * (1 - act_{j-1}^2) for this
-      // wavefront's rows; act_{j-1} of the own rows comes back from LDS
-      v4 c[NB];
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib) c[ib] = v4{0, 0, 0, 0};
-      mma_layer_fm<T, NB>(net.Wb + jj * HP * HP, lane, dbuf, trow, c);
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const T av = ain[(16 * ib + crow<T>(q, i)) * CTRS + trow];
-          dl[ib][i] = c[ib][i] * (T(1) - av * av);
-        }
-      __syncthreads();
-    }
-    // first layer: columns c < d_in of the small-term tiles (B = coordinate
-    // c) and column d_in (B = 1); delta_0 over act_0
-    T *dbuf = layer_buf(0);
-    store_rows(dbuf, dl);
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < OBW; ++u) {
-      const int ob = w + 4 * u;
-      if (ob < NB) {
-#pragma unroll
-        for (int kk = 0; kk < TR / 16; ++kk) {
-          const v4 dv = *reinterpret_cast<const v4 *>(dbuf + (16 * ob + row) * CTRS + 16 * kk + 4 * q);
-          const v4 xv = row < d_in
-                            ? *reinterpret_cast<const v4 *>(xbuf + row * CTRS + 16 * kk + 4 * q)
-                            : (row == col_b1 ? v4{1, 1, 1, 1} : v4{0, 0, 0, 0});
-#pragma unroll
-          for (int kb = 0; kb < 4; ++kb) sacc[u] = mma16(dv[kb], xv[kb], sacc[u]);
-        }
-      }
-    }
-    __syncthreads();
-  }
-
-  // each wavefront writes the blocks it owns; the output bias is the only
-  // sum over wavefronts (fixed order)
-  T *out = a.partial + (int64_t)blockIdx.x * P;
-#pragma unroll
-  for (int u = 0; u < OBW; ++u) {
-    const int ob = w + 4 * u;
-    if (ob < NB) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int of = 16 * ob + crow<T>(q, i);
-        if (of >= H) continue;
-#pragma unroll
-        for (int jj = 0; jj < NH; ++jj)
-#pragma unroll
-          for (int ib = 0; ib < NB; ++ib) {
-            const int in = 16 * ib + row;
-            if (in < H) out[L.oWh(jj) + of * H + in] = dwacc[jj][u][ib][i];
-          }
-        const T sv = sacc[u][i];
-        if (row < d_in) out[L.oW1() + of * d_in + row] = sv;
-        else if (row == col_b1) out[L.ob1() + of] = sv;
-        else if (row < col_wl + n_out) out[L.oWL() + (row - col_wl) * H + of] = sv;
-      }
-#pragma unroll
-      for (int jj = 0; jj < NH; ++jj) {
-        T v = bhacc[jj][u];
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        const int f = 16 * ob + row;
-        if (q == 0 && f < H) out[L.obh(jj) + f] = v;
-      }
-    }
-  }
-  const T bl = rowsum16(blacc);
-  if (row == 0) red[4 * w + q] = bl;
-  __syncthreads();
-  if ((int)threadIdx.x < n_out) {
-    const int o = threadIdx.x;
-    out[L.obL() + o] = ((red[o] + red[4 + o]) + red[8 + o]) + red[12 + o];
-  }
-  if constexpr (FS) {
-    if (w == 0) {
-#pragma unroll
-      for (int st = 32; st > 0; st >>= 1) {
-        lsum += __shfl_down(lsum, st);
-        lnsum += __shfl_down(lnsum, st);
-      }
-      if (lane == 0) {
-        a.lp[2 * blockIdx.x] = lsum;
-        a.lp[2 * blockIdx.x + 1] = lnsum;
-      }
-    }
-  }
+  block_body<T, NB, NH, FS, BlockInterp<T>>(a, reinterpret_cast<T *>(smem_raw));
 }
 
 // ---------------------------------------------------------------------------
@@ -1739,6 +1366,9 @@ struct tdx_nn_plan {
   int n_k4 = 0;
   size_t k4_lds = 0;
   DevTable k4_items, ik4p, ik4i;
+  // that kernel compiled with the plan's generated residual programs (null:
+  // interpreted)
+  hipFunction_t block_fn = nullptr;
   int acc_lds;
   size_t k1_lds, k2_lds;
   DevTable groups, ints, reals, data, k1_items, k2_items, ik1p, ik1i, ik2p, ik2i;
@@ -1988,6 +1618,14 @@ __host__ __device__ constexpr int block_acc_regs(int NB, int NH) {
 template <typename T, int NB, int NH, bool FS = false>
 int launch_backward_c(const tdx_nn_plan *p, const NNArgs<T> &a, hipStream_t s) {
   if constexpr (block_acc_regs(NB, NH) <= 192) {
+    if (FS && p->block_fn) {
+      ProfScope ps(p, 1, s);
+      NNArgs<T> arg = a;
+      void *args[] = {&arg};
+      TDX_TRY_HIP(hipModuleLaunchKernel(p->block_fn, p->n_k4, 1, 1, 256, 1, 1,
+                                        (unsigned)p->k4_lds, s, args, nullptr));
+      return TDX_OK;
+    }
     raise_lds_limit({(const void *)k_backward_c<T, NB, NH, FS>});
     ProfScope ps(p, 1, s);
     hipLaunchKernelGGL((k_backward_c<T, NB, NH, FS>), dim3(FS ? p->n_k4 : p->n_k2), dim3(256),
@@ -2547,7 +2185,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
         prog_kind.push_back(it->second);
       }
       const std::string src = nn_fused_jit_source(keys, key_neq, p->NB, p->NH);
-      if (nn_fused_jit_load(src, &p->fused_fn) != TDX_OK) {
+      if (nn_fused_jit_load(src, &p->fused_fn, "tdx_fused_jit") != TDX_OK) {
         p->fused_fn = nullptr;
         prog_kind.clear();
         clear_error();
@@ -2587,6 +2225,36 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
     p->k2_fs = 1;
     p->n_k4 = (int)k4.size();
     p->k4_lds = block_fs_lds_words(p->HP, p->NH) * ts;
+    // generated residual programs for plans of at least 2^17 network rows
+    // (TDX_NN_JIT: 0 never, 1 always); the interpreting kernel if compiling
+    // fails
+    const char *env = std::getenv("TDX_NN_JIT");
+    const bool jit = env ? std::atoi(env) != 0 : n_rows >= (1 << 17);
+    if (jit) {
+      std::vector<std::vector<int32_t>> keys;
+      std::vector<int> key_neq;
+      std::map<std::vector<int32_t>, int> kind_of;
+      for (const FusedProg &fp : progs) {
+        const std::vector<int32_t> key = nn_prog_key(fp);
+        auto it = kind_of.find(key);
+        if (it == kind_of.end()) {
+          it = kind_of.emplace(key, (int)keys.size()).first;
+          keys.push_back(key);
+          key_neq.push_back(fp.n_eq);
+        }
+        prog_kind.push_back(it->second);
+      }
+      const std::string src = nn_block_jit_source(keys, key_neq, p->NB, p->NH, d->n_out);
+      if (nn_fused_jit_load(src, &p->block_fn, "tdx_block_jit") != TDX_OK) {
+        p->block_fn = nullptr;
+        prog_kind.clear();
+        clear_error();
+      } else {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(p->block_fn),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipGetLastError();
+      }
+    }
   }
 
   int r2 = TDX_OK;
@@ -2607,6 +2275,8 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
     if (!r2) r2 = p->prog_idx.upload(prog_idx.data(), prog_idx.size() * sizeof(int4));
     if (!r2) r2 = p->prog_ints.upload(prog_ints.data(), prog_ints.size() * 4);
     if (!r2) r2 = p->prog_reals.upload(prog_reals.data(), prog_reals.size() * 4);
+    if (!r2 && !prog_kind.empty())
+      r2 = p->prog_kind.upload(prog_kind.data(), prog_kind.size() * 4);
   }
   if (!r2 && p->fused) {
     r2 = p->k3_items.upload(k3.data(), k3.size() * sizeof(int4));
@@ -2759,7 +2429,14 @@ extern "C" int64_t tdx_nn_fused_source(const tdx_nn_desc *d, char *buf, int64_t 
   for (int g = 0; g < d->n_groups; ++g)
     max_slots = std::max(max_slots, (int)d->groups[(int64_t)g * TDX_GROUP_INTS + 2]);
   std::vector<FusedProg> progs;
-  if (!fused_shape_ok(d, max_slots)) {
+  const int NB = (d->width + 15) / 16, NH = d->n_layers - 2;
+  // wide float networks: the block-owning kernel forming its own seeds
+  const bool wide = d->dtype == 0 && NB >= 3 && NB <= 8 && NH >= 1 && NH <= 4 &&
+                    block_acc_regs(NB, NH) <= 192 && d->d_in + 1 + d->n_out <= 16 &&
+                    max_slots <= 64 && d->tile_rows == 0 && d->persistent == 0 &&
+                    d->causal_tol == 0.0 && d->weak_levels == 0 &&
+                    block_fs_lds_words(16 * NB, NH) * 4 <= 160 * 1024;
+  if (!fused_shape_ok(d, max_slots) && !wide) {
     set_error("the network or options leave the fused step out");
     return 0;
   }
@@ -2777,7 +2454,8 @@ extern "C" int64_t tdx_nn_fused_source(const tdx_nn_desc *d, char *buf, int64_t 
       key_neq.push_back(fp.n_eq);
     }
   }
-  const std::string src = nn_fused_jit_source(keys, key_neq, (d->width + 15) / 16, d->n_layers - 2);
+  const std::string src = wide ? nn_block_jit_source(keys, key_neq, NB, NH, d->n_out)
+                               : nn_fused_jit_source(keys, key_neq, NB, NH);
   if (compile) {
     std::vector<char> code;
     const int crc = nn_fused_jit_build(src, &code);
@@ -2793,7 +2471,7 @@ extern "C" int64_t tdx_nn_fused_source(const tdx_nn_desc *d, char *buf, int64_t 
 
 extern "C" int tdx_nn_step_kernel(const tdx_nn_plan *p) {
   if (!p) return -1;
-  if (p->k2_fs) return 3;
+  if (p->k2_fs) return p->block_fn ? 4 : 3;
   return p->fused ? (p->fused_fn ? 2 : 1) : 0;
 }
 

@@ -30,13 +30,19 @@ std::vector<int32_t> nn_prog_key(const FusedProg &p);
 std::string nn_fused_jit_source(const std::vector<std::vector<int32_t>> &keys,
                                 const std::vector<int> &n_eq, int NB, int NH);
 
+// Source of the block-owning kernel that forms its own seeds (wide float
+// networks, NB blocks, NH H x H layers, n_out outputs) with one generated
+// residual function per structure in `keys`.
+std::string nn_block_jit_source(const std::vector<std::vector<int32_t>> &keys,
+                                const std::vector<int> &n_eq, int NB, int NH, int n_out);
+
 // Compiles `src` to a code object for `arch` (no device needed); returns 0
 // or a negative code with the compiler log in the error message.
 int nn_fused_jit_build(const std::string &src, std::vector<char> *code,
                        const std::string &arch = "gfx950");
 
 // Compiles and loads `src` on the current device once per process (cached
-// by device and source); returns 0 and the kernel, or a negative code.
-int nn_fused_jit_load(const std::string &src, hipFunction_t *fn);
+// by device and source); returns 0 and its kernel `name`, or a negative code.
+int nn_fused_jit_load(const std::string &src, hipFunction_t *fn, const char *name);
 
 }  // namespace tdx

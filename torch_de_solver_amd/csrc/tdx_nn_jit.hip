@@ -48,14 +48,16 @@ namespace {
 
 std::string num(int v) { return std::to_string(v); }
 
-// gs = sum over the factor's stencil points of u[slot] * w, in point order
+// gs = sum over the factor's stencil points of u[slot * n_out + var] * w,
+// in point order
 void emit_stencil_sum(std::ostringstream &o, const std::vector<int32_t> &k, int F,
-                      const std::string &dst) {
-  const int ns = k[F + 3], si = k[F + 4];
+                      const std::string &dst, int n_out) {
+  const int ns = k[F + 3], si = k[F + 4], var = k[F];
   o << dst << " = 0.0f;\n";
   for (int j = 0; j < ns; ++j) {
     const int S = si + j * TDX_STENCIL_INTS;
-    o << dst << " = " << dst << " + u[" << k[S] << "] * R[" << k[S + 1] << "];\n";
+    o << dst << " = " << dst << " + u[" << k[S] * n_out + var << "] * R[" << k[S + 1]
+      << "];\n";
   }
 }
 
@@ -80,8 +82,9 @@ std::string coeff_expr(const std::vector<int32_t> &k, int Tm) {
   return "data[(int64_t)I[" + num(Tm + 3) + "] + e]";
 }
 
-// entry_program of one structure, n_out = 1, as straight-line code
-void emit_program(std::ostringstream &o, const std::vector<int32_t> &k, int n_eq) {
+// entry_program of one structure as straight-line code (network values u,
+// seed accumulators sdv, both indexed slot * n_out + var)
+void emit_program(std::ostringstream &o, const std::vector<int32_t> &k, int n_eq, int n_out) {
   for (int q = 0; q < n_eq; ++q) {
     const int E = q * TDX_EQ_INTS;
     const int nT = k[E], ti = k[E + 1];
@@ -93,7 +96,7 @@ void emit_program(std::ostringstream &o, const std::vector<int32_t> &k, int n_eq
       for (int f = 0; f < nf; ++f) {
         const int F = fi + f * TDX_FACTOR_INTS;
         o << "{\nfloat gs;\n";
-        emit_stencil_sum(o, k, F, "gs");
+        emit_stencil_sum(o, k, F, "gs", n_out);
         o << "der = der * " << pow_expr(k, F, "gs") << ";\n}\n";
       }
       o << "const float tv = " << coeff_expr(k, Tm) << " * der;\n"
@@ -116,7 +119,7 @@ void emit_program(std::ostringstream &o, const std::vector<int32_t> &k, int n_eq
         if (f < nf) {
           const int F = fi + f * TDX_FACTOR_INTS;
           const std::string g = "gsv[" + num(f) + "]";
-          emit_stencil_sum(o, k, F, g);
+          emit_stencil_sum(o, k, F, g, n_out);
           o << "pw[" << f << "] = " << pow_expr(k, F, g) << ";\n";
         } else {
           o << "gsv[" << f << "] = 0.0f;\npw[" << f << "] = 1.0f;\n";
@@ -133,10 +136,10 @@ void emit_program(std::ostringstream &o, const std::vector<int32_t> &k, int n_eq
         const int F = fi + f * TDX_FACTOR_INTS;
         o << "{\nconst float gf = cdr * " << dpow_expr(k, F, "gsv[" + num(f) + "]") << " * pre["
           << f << "] * suf[" << f + 1 << "];\n";
-        const int ns = k[F + 3], si = k[F + 4];
+        const int ns = k[F + 3], si = k[F + 4], var = k[F];
         for (int j = 0; j < ns; ++j) {
           const int S = si + j * TDX_STENCIL_INTS;
-          o << "sdv[" << k[S] << "] += gf * R[" << k[S + 1] << "];\n";
+          o << "sdv[" << k[S] * n_out + var << "] += gf * R[" << k[S + 1] << "];\n";
         }
         o << "}\n";
       }
@@ -148,9 +151,10 @@ void emit_program(std::ostringstream &o, const std::vector<int32_t> &k, int n_eq
 
 }  // namespace
 
-std::string nn_fused_jit_source(const std::vector<std::vector<int32_t>> &keys,
-                                const std::vector<int> &n_eq, int NB, int NH) {
-  std::ostringstream o;
+namespace {
+
+// Shared prelude: integer types, the table record sizes, the device headers.
+void emit_prelude(std::ostringstream &o) {
   o << "typedef __hip_internal::int32_t int32_t;\n"
        "typedef __hip_internal::int64_t int64_t;\n"
     << "#define TDX_GROUP_INTS " << TDX_GROUP_INTS << "\n"
@@ -159,8 +163,16 @@ std::string nn_fused_jit_source(const std::vector<std::vector<int32_t>> &keys,
     << "#define TDX_FACTOR_INTS " << TDX_FACTOR_INTS << "\n"
     << "#define TDX_STENCIL_INTS " << TDX_STENCIL_INTS << "\n"
     << kTdxDevSrc << "\n"
-    << kTdxNnDevSrc << "\n"
-    << "namespace tdx {\n"
+    << kTdxNnDevSrc << "\n";
+}
+
+}  // namespace
+
+std::string nn_fused_jit_source(const std::vector<std::vector<int32_t>> &keys,
+                                const std::vector<int> &n_eq, int NB, int NH) {
+  std::ostringstream o;
+  emit_prelude(o);
+  o << "namespace tdx {\n"
        "// generated residual programs, one per structure (kind)\n"
        "struct GenResid {\n"
        "  const int32_t *I;\n"
@@ -183,7 +195,7 @@ std::string nn_fused_jit_source(const std::vector<std::vector<int32_t>> &keys,
        "    switch (kind) {\n";
   for (size_t i = 0; i < keys.size(); ++i) {
     o << "    case " << i << ": {\n";
-    emit_program(o, keys[i], n_eq[i]);
+    emit_program(o, keys[i], n_eq[i], 1);
     o << "    } break;\n";
   }
   o << "    default: break;\n"
@@ -197,6 +209,48 @@ std::string nn_fused_jit_source(const std::vector<std::vector<int32_t>> &keys,
        "  tdx::fused_body<"
     << NB << ", " << NH
     << ", tdx::GenResid>(a, reinterpret_cast<float *>(smem_raw));\n"
+       "}\n";
+  return o.str();
+}
+
+std::string nn_block_jit_source(const std::vector<std::vector<int32_t>> &keys,
+                                const std::vector<int> &n_eq, int NB, int NH, int n_out) {
+  std::ostringstream o;
+  emit_prelude(o);
+  o << "namespace tdx {\n"
+       "// generated residual programs, one per structure (kind); seeds added\n"
+       "// into the entry's LDS row sdv (zeroed by the caller)\n"
+       "struct GenBlockResid {\n"
+       "  const int32_t *I;\n"
+       "  const float *R;\n"
+       "  const float *data;\n"
+       "  int kind;\n"
+       "  __device__ GenBlockResid(const NNArgs<float> &a, int g, int32_t *, float *) {\n"
+       "    data = a.data;\n"
+       "    if (g < 0) return;\n"
+       "    const int4 pi = a.prog_idx[g];\n"
+       "    I = a.prog_ints + pi.x;\n"
+       "    R = a.prog_reals + pi.z;\n"
+       "    kind = a.prog_kind[g];\n"
+       "  }\n"
+       "  __device__ __forceinline__ void run(int, int e, int, const float *u, float *sdv,\n"
+       "                                      float &ls, float &lns) {\n"
+       "    switch (kind) {\n";
+  for (size_t i = 0; i < keys.size(); ++i) {
+    o << "    case " << i << ": {\n";
+    emit_program(o, keys[i], n_eq[i], n_out);
+    o << "    } break;\n";
+  }
+  o << "    default: break;\n"
+       "    }\n"
+       "  }\n"
+       "};\n"
+       "}  // namespace tdx\n"
+       "extern \"C\" __global__ __launch_bounds__(256) void tdx_block_jit(tdx::NNArgs<float> a) {\n"
+       "  extern __shared__ __align__(16) unsigned char smem_raw[];\n"
+       "  tdx::block_body<float, "
+    << NB << ", " << NH
+    << ", true, tdx::GenBlockResid>(a, reinterpret_cast<float *>(smem_raw));\n"
        "}\n";
   return o.str();
 }
@@ -229,7 +283,7 @@ int nn_fused_jit_build(const std::string &src, std::vector<char> *code, const st
   return TDX_OK;
 }
 
-int nn_fused_jit_load(const std::string &src, hipFunction_t *fn) {
+int nn_fused_jit_load(const std::string &src, hipFunction_t *fn, const char *name) {
   static std::mutex mu;
   static std::map<std::pair<int, std::string>, hipFunction_t> cache;
   int dev = 0;
@@ -259,7 +313,7 @@ int nn_fused_jit_load(const std::string &src, hipFunction_t *fn) {
     return TDX_ELAUNCH;
   }
   hipFunction_t f = nullptr;
-  if (hipModuleGetFunction(&f, mod, "tdx_fused_jit") != hipSuccess) {
+  if (hipModuleGetFunction(&f, mod, name) != hipSuccess) {
     (void)hipModuleUnload(mod);
     set_error("fused 'NN' kernel: entry point missing");
     return TDX_ELAUNCH;

@@ -369,10 +369,13 @@ class NNEngine:
         """The training step's kernels: 'separate' (forward, then adjoint),
         'fused' (one kernel interpreting the residual programs),
         'fused-generated' (one kernel compiled with the plan's generated
-        residual programs) or 'adjoint-seeds' (wide networks: the
-        block-owning adjoint forms its own seeds, no forward kernel)."""
+        residual programs), 'adjoint-seeds' (wide networks: the
+        block-owning adjoint forms its own seeds, no forward kernel) or
+        'adjoint-seeds-generated' (the same compiled with generated
+        programs)."""
         k = int(native.lib().tdx_nn_step_kernel(self._plan))
-        return {0: 'separate', 1: 'fused', 2: 'fused-generated', 3: 'adjoint-seeds'}[k]
+        return {0: 'separate', 1: 'fused', 2: 'fused-generated', 3: 'adjoint-seeds',
+                4: 'adjoint-seeds-generated'}[k]
 
     def profile(self, capacity: int) -> None:
         """Time up to ``capacity`` launches per kernel kind with HIP events
