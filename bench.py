@@ -312,13 +312,18 @@ def bench_nn(args):
     else:
         log(f"[bench] kernels: forward {fwd_ms:.3f} ms, backward {bwd_ms:.3f} ms, "
             f"update {upd_ms:.3f} ms")
-    kern = {'fused-generated': 'tdx_fused_jit', 'fused': 'k_fused'}.get(eng.step_kernel,
-                                                                         'k_backward_w')
+    kern = {'fused-generated': 'tdx_fused_jit', 'fused': 'k_fused',
+            'adjoint-seeds-generated': 'tdx_block_jit',
+            'adjoint-seeds': 'k_backward_c'}.get(eng.step_kernel, 'k_backward')
     kname = {'tdx_fused_jit': "tdx_fused_jit (forward + generated residual programs + adjoint "
                               "+ weight gradients, compiled at plan creation)",
              'k_fused': "k_fused (forward + interpreted residual programs + adjoint + weight "
                         "gradients)",
-             'k_backward_w': "k_backward (recompute + adjoint + weight gradients)"}[kern]
+             'tdx_block_jit': "tdx_block_jit (block-owning: forward + generated residual "
+                              "programs + adjoint + weight gradients, compiled at plan creation)",
+             'k_backward_c': "k_backward_c (block-owning: forward + interpreted residual "
+                             "programs + adjoint + weight gradients)",
+             'k_backward': "k_backward (recompute + adjoint + weight gradients)"}[kern]
     tname = f"{kern}_{args.workload}"
     out = {
         "metric": "train steps/s & collocation pts/s, fused+adjoint vs plain GPU, at equal rel-L2",
