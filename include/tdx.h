@@ -249,8 +249,8 @@ int tdx_nn_train_step_noalloc(const tdx_nn_plan *plan, void *params, void *exp_a
  * bytes, NUL-terminated; buf may be NULL); with compile != 0 also compiled
  * by hipRTC for gfx950 (no device needed).  Returns the source length, 0
  * when the plan has no fused step (reason in tdx_last_error) or a negative
- * error code.  Plans generate the programs when they hold at least 2^20
- * network rows (fused step) or 2^17 (wide networks), or always / never with
+ * error code.  Plans generate the programs when they hold at least 2^17
+ * network rows, or always / never with
  * the environment variable TDX_NN_JIT=1 / 0, and fall back to the library's
  * interpreting kernel if compiling fails.  Diagnostic; no counterpart in the
  * reference. */

@@ -20,7 +20,7 @@ What must agree:
 * two runs of the fused step: the same bits.
 The plan with the separate kernels is requested with persistent = -1.
 Which path ran is read from the launch profile (the fused step launches no
-forward kernel) and from tdx_nn_step_kernel: plans of at least 2^20 network
+forward kernel) and from tdx_nn_step_kernel: plans of at least 2^17 network
 rows compile the fused kernel with straight-line residual programs generated
 from their tables (hipRTC), smaller ones interpret the tables;
 TDX_NN_JIT=1 / 0 forces either.  The generated programs perform the

@@ -2166,10 +2166,11 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
     }
     p->fused = 1;
     p->n_k3 = (int)k3.size();
-    // generated residual programs for large plans (TDX_NN_JIT: 0 never,
-    // 1 always); the interpreted kernel if compiling fails
+    // generated residual programs for plans of at least 2^17 network rows
+    // (TDX_NN_JIT: 0 never, 1 always); the interpreted kernel if compiling
+    // fails
     const char *env = std::getenv("TDX_NN_JIT");
-    const bool jit = env ? std::atoi(env) != 0 : n_rows >= (1 << 20);
+    const bool jit = env ? std::atoi(env) != 0 : n_rows >= (1 << 17);
     if (jit) {
       std::vector<std::vector<int32_t>> keys;
       std::vector<int> key_neq;
