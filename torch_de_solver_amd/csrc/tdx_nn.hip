@@ -1105,6 +1105,34 @@ __global__ __launch_bounds__(256) void k_reduce(NNArgs<T> a, int mode, T *dparam
   }
 }
 
+// First stage of the gradient sum of plans with few instances (too few
+// parameters to keep enough loads in flight with one thread per parameter):
+// slice sl of instance i's work items -- a contiguous share of its list, in
+// list order -- summed per parameter into out[(sl * n_inst + i) * P + p];
+// k_reduce then sums the slices in order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_reduce_slices(NNArgs<T> a, int n_slices, T *out) {
+  const int64_t tot = (int64_t)a.n_inst * a.P;
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= tot) return;
+  const int sl = blockIdx.y;
+  const int inst = (int)(idx / a.P);
+  const int p = (int)(idx - (int64_t)inst * a.P);
+  const int k0 = a.inst_k2_ptr[inst], n = a.inst_k2_ptr[inst + 1] - k0;
+  int k = k0 + (int)((int64_t)n * sl / n_slices);
+  const int k1 = k0 + (int)((int64_t)n * (sl + 1) / n_slices);
+  T acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) acc[u] = T(0);
+  for (; k + 8 <= k1; k += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += a.partial[(int64_t)a.inst_k2_idx[k + u] * a.P + p];
+  }
+  for (; k < k1; ++k) acc[0] += a.partial[(int64_t)a.inst_k2_idx[k] * a.P + p];
+  out[((int64_t)sl * a.n_inst + inst) * a.P + p] =
+      ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_adam(int64_t n, T *p, const T *g, T *m, T *v,
                                               const int64_t *step, AdamHyper h, int vec) {
@@ -1360,6 +1388,11 @@ struct tdx_nn_plan {
   // the fused kernel compiled with the plan's generated residual programs
   // (null: the interpreted kernel)
   hipFunction_t fused_fn = nullptr;
+  // two-stage gradient sum (plans with few instances): slices per instance,
+  // their list (instance i: items s * n_inst + i) and workspace offset
+  int red_slices = 1;
+  DevTable ik5p, ik5i;
+  size_t off_red = 0;
   // training step of wide float networks: the block-owning backward forms
   // its own seeds (FS work items: tiles of whole entries)
   int k2_fs = 0;
@@ -1742,8 +1775,19 @@ int launch_reduce(const tdx_nn_plan *p, const NNArgs<T> &a, int mode, T *dparams
   const int64_t tot = (int64_t)p->n_inst * p->P;
   const int loss_blocks = loss ? p->n_inst : 0;
   ProfScope ps(p, 2, s);
+  NNArgs<T> ar = a;
+  if (p->red_slices > 1) {
+    // the slices' sums in the workspace (a.partial is workspace + off_partial)
+    T *sl = (T *)((char *)a.partial - p->off_partial + p->off_red);
+    hipLaunchKernelGGL((k_reduce_slices<T>), dim3((unsigned)((tot + 255) / 256), p->red_slices),
+                       dim3(256), 0, s, a, p->red_slices, sl);
+    TDX_TRY_HIP(hipGetLastError());
+    ar.partial = sl;
+    ar.inst_k2_ptr = (const int32_t *)p->ik5p.ptr;
+    ar.inst_k2_idx = (const int32_t *)p->ik5i.ptr;
+  }
   hipLaunchKernelGGL((k_reduce<T>), dim3((unsigned)((tot + 255) / 256) + loss_blocks), dim3(256),
-                     0, s, a, mode, dparams, params, m, v, step, h, loss, loss_blocks, step_done);
+                     0, s, ar, mode, dparams, params, m, v, step, h, loss, loss_blocks, step_done);
   TDX_TRY_HIP(hipGetLastError());
   return TDX_OK;
 }
@@ -2199,6 +2243,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   // chunk_tiles tiles per work item (default 8)
   std::vector<int4> k4;
   std::vector<int32_t> ik4p(1, 0), ik4i;
+  std::vector<int> inst_k4_size(d->n_inst, 0);
   if (p->k2_block && !p->fused && d->dtype == 0 && max_slots <= 64 && d->tile_rows == 0 &&
       d->persistent == 0 && d->causal_tol == 0.0 && d->weak_levels == 0 &&
       block_fs_lds_words(p->HP, p->NH) * ts <= LDS_MAX && fused_programs(d, &progs)) {
@@ -2222,6 +2267,7 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
     for (int i = 0; i < d->n_inst; ++i) {
       for (int x : inst_k4[i]) ik4i.push_back(x);
       ik4p.push_back((int32_t)ik4i.size());
+      inst_k4_size[i] = (int)inst_k4[i].size();
     }
     p->k2_fs = 1;
     p->n_k4 = (int)k4.size();
@@ -2258,7 +2304,29 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
     }
   }
 
+  // two-stage gradient sum when one thread per parameter of every instance
+  // is too few threads: enough slices per instance for about 2^20 threads,
+  // at least 8 work items per slice (an instance of few items is summed in
+  // one pass: a second launch would cost more than it saves)
+  std::vector<int32_t> ik5p, ik5i;
+  {
+    const int64_t np = (int64_t)d->n_inst * p->P;
+    int max_items = 0;
+    for (int i = 0; i < d->n_inst; ++i)
+      max_items = std::max({max_items, (int)inst_k2[i].size(), (int)inst_k3[i].size(),
+                            (int)inst_k4_size[i]});
+    const int sl = (int)std::min<int64_t>({64, ((1 << 20) + np - 1) / np, max_items / 8});
+    if (np < (1 << 19) && sl >= 2) {
+      p->red_slices = sl;
+      for (int i = 0; i <= d->n_inst; ++i) ik5p.push_back(i * p->red_slices);
+      for (int i = 0; i < d->n_inst; ++i)
+        for (int sl = 0; sl < p->red_slices; ++sl) ik5i.push_back(sl * d->n_inst + i);
+    }
+  }
+
   int r2 = TDX_OK;
+  if (!r2 && p->red_slices > 1) r2 = p->ik5p.upload(ik5p.data(), ik5p.size() * 4);
+  if (!r2 && p->red_slices > 1) r2 = p->ik5i.upload(ik5i.data(), ik5i.size() * 4);
   if (!r2) r2 = p->groups.upload(d->groups, (size_t)d->n_groups * TDX_GROUP_INTS * 4);
   if (!r2) r2 = p->ints.upload(d->ints, (size_t)std::max<int64_t>(d->n_ints, 1) * 4);
   if (!r2) r2 = p->reals.upload(d->reals, (size_t)d->n_reals * ts);
@@ -2332,6 +2400,8 @@ extern "C" int tdx_nn_plan_create(const tdx_nn_desc *d, tdx_nn_plan **out) {
   off = align_up(off + (size_t)std::max({p->n_k2, p->n_k3, p->n_k4}) * p->P * ts, 256);
   p->off_one = off;
   off = align_up(off + ts, 256);
+  p->off_red = off;
+  if (p->red_slices > 1) off = align_up(off + (size_t)p->red_slices * p->n_inst * p->P * ts, 256);
   if (d->causal_tol != 0.0) {
     const int32_t *G0 = d->groups;
     p->causal = 1;
@@ -2410,7 +2480,7 @@ extern "C" void tdx_nn_plan_destroy(tdx_nn_plan *p) {
                    &p->k2_items, &p->ik1p,     &p->ik1i,      &p->ik2p,     &p->ik2i,
                    &p->k1_bounds, &p->k3_items, &p->ik3p,     &p->ik3i,     &p->prog_idx,
                    &p->prog_ints, &p->prog_reals, &p->prog_kind, &p->k4_items, &p->ik4p,
-                   &p->ik4i};
+                   &p->ik4i, &p->ik5p, &p->ik5i};
   for (DevTable *x : t) x->release();
   delete p;
 }
