@@ -752,10 +752,10 @@ __global__ void k_mat_bc_adj(MatArgs<T> a) {
 // pairs b, b + G, b + 2G, ... (G = grid size) in a fixed order and writes one
 // pair to out[b]; a second launch with one workgroup adds those.
 template <typename T>
-__global__ void k_mat_loss(const T *lp, int n, T *loss) {
+__device__ __forceinline__ void mat_loss_block(const T *lp, int n, T *loss, int blk, int nblk) {
   __shared__ T red[2 * 256];
   T s0 = T(0), s1 = T(0);
-  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += 256 * gridDim.x) {
+  for (int k = blk * 256 + threadIdx.x; k < n; k += 256 * nblk) {
     s0 += lp[2 * k];
     s1 += lp[2 * k + 1];
   }
@@ -770,23 +770,33 @@ __global__ void k_mat_loss(const T *lp, int n, T *loss) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    loss[2 * blockIdx.x] = red[0];
-    loss[2 * blockIdx.x + 1] = red[256];
+    loss[2 * blk] = red[0];
+    loss[2 * blk + 1] = red[256];
   }
 }
 
-// The step's Adam scalars (t = *step + 1), computed once for every kernel of
-// the fused step: {1-b1, 1-b2, b2, lr/(1-b1^t), sqrt(1-b2^t), eps}.
 template <typename T>
-__global__ void k_mat_adam_prep(const int64_t *step, AdamHyper h, T *out) {
+__global__ void k_mat_loss(const T *lp, int n, T *loss) {
+  mat_loss_block(lp, n, loss, blockIdx.x, gridDim.x);
+}
+
+// The fused step's set-up in one launch: the adjoint's unit seed scale, the
+// Adam scalars of step t = *step + 1, computed once for every kernel of the
+// step ({1-b1, 1-b2, b2, lr/(1-b1^t), sqrt(1-b2^t), eps}), and the step
+// counter advanced to t (no later kernel of the fused step reads it).
+template <typename T>
+__global__ void k_mat_step_prep(T *one, int64_t *step, AdamHyper h, T *out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const AdamScalars<T> sc = adam_scalars<T>(*step + 1, h);
+    *one = T(1);
+    const int64_t t = *step + 1;
+    const AdamScalars<T> sc = adam_scalars<T>(t, h);
     out[0] = sc.w1;
     out[1] = sc.w2;
     out[2] = sc.b2;
     out[3] = sc.step_size;
     out[4] = sc.bc2s;
     out[5] = sc.eps;
+    *step = t;
   }
 }
 
@@ -795,7 +805,15 @@ __global__ void k_mat_adam_prep(const int64_t *step, AdamHyper h, T *out) {
 // generated kernel does not update; an empty rectangle covers the grid).
 template <typename T>
 __global__ __launch_bounds__(256) void k_mat_frame_adam(MatArgs<T> a, int r_lo, int r_hi,
-                                                        int c_lo, int c_hi) {
+                                                        int c_lo, int c_hi, const T *lp,
+                                                        int n_lp, T *loss) {
+  // with loss: the last workgroup forms the loss pair from the n_lp
+  // partial pairs (k_mat_loss in one workgroup) instead of updating points
+  if (loss && blockIdx.x == gridDim.x - 1) {
+    mat_loss_block(lp, n_lp, loss, 0, 1);
+    return;
+  }
+  const int n_upd = gridDim.x - (loss ? 1 : 0);
   const int n0 = a.n0, n1 = a.n1;
   const int64_t top = (int64_t)r_lo * n1, bot = (int64_t)(n0 - r_hi) * n1;
   const int wl = c_lo, wm = c_lo + (n1 - c_hi);
@@ -808,7 +826,7 @@ __global__ __launch_bounds__(256) void k_mat_frame_adam(MatArgs<T> a, int r_lo, 
   sc.step_size = a.adam[3];
   sc.bc2s = a.adam[4];
   sc.eps = a.adam[5];
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t stride = (int64_t)n_upd * blockDim.x;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot; idx += stride) {
     const int64_t e = idx / per, k = idx - e * per;
     int64_t row, col;
@@ -1206,25 +1224,29 @@ int mat_run(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream_t 
       TDX_TRY_HIP(hipGetLastError());
     }
   }
+  // loss partial pairs of the step's kernels
+  const int var = adj && a.fused && p->jit_fuse ? 2 : adj ? 1 : 0;
+  const int n = p->n_gen_wg + p->n_bc_wg + (p->jit_on ? p->j_waves[var] : 0);
+  const int g = std::min(256, (n + 4095) / 4096);
   if (adj && a.fused) {
-    // Adam on the points the generated kernel did not update
+    // Adam on the points the generated kernel did not update; its last
+    // workgroup also forms the loss pair when one workgroup sums them
     const bool in_jit = p->jit_on && p->jit_fuse;
     const int r_lo = in_jit ? p->jr_lo : 0, r_hi = in_jit ? p->jr_hi : 0;
     const int c_lo = in_jit ? p->jc_lo : 0, c_hi = in_jit ? p->jc_hi : 0;
     const int64_t per = (int64_t)p->d.n0 * p->d.n1 - (int64_t)(r_hi - r_lo) * (c_hi - c_lo);
     const int64_t tot = per * p->d.n_eq;
     if (tot > 0) {
+      T *lw = (loss && g == 1) ? loss : nullptr;
       const unsigned blocks =
           (unsigned)std::max<int64_t>(1, std::min<int64_t>((tot + 255) / 256, 16384));
-      hipLaunchKernelGGL((k_mat_frame_adam<T>), dim3(blocks), dim3(256), 0, s, a, r_lo, r_hi,
-                         c_lo, c_hi);
+      hipLaunchKernelGGL((k_mat_frame_adam<T>), dim3(blocks + (lw ? 1 : 0)), dim3(256), 0, s, a,
+                         r_lo, r_hi, c_lo, c_hi, (const T *)a.lp, n, lw);
       TDX_TRY_HIP(hipGetLastError());
+      if (lw) loss = nullptr;  // formed
     }
   }
   if (loss) {
-    const int var = adj && a.fused && p->jit_fuse ? 2 : adj ? 1 : 0;
-    const int n = p->n_gen_wg + p->n_bc_wg + (p->jit_on ? p->j_waves[var] : 0);
-    const int g = std::min(256, (n + 4095) / 4096);
     T *mid = a.lp_mid;
     if (g > 1) {
       hipLaunchKernelGGL((k_mat_loss<T>), dim3(g), dim3(256), 0, s, a.lp, n, mid);
@@ -1340,7 +1362,7 @@ int mat_run_nd(const tdx_mat_plan *p, MatArgs<T> a, bool adj, T *loss, hipStream
       b.n1 = N;
       const int64_t tot = gw;
       hipLaunchKernelGGL((k_mat_frame_adam<T>), dim3(nd_blocks(tot)), dim3(256), 0, s, b, 0, 0,
-                         0, 0);
+                         0, 0, (const T *)nullptr, 0, (T *)nullptr);
       TDX_TRY_HIP(hipGetLastError());
     }
   }
@@ -2083,9 +2105,8 @@ extern "C" int tdx_mat_train_step_to(const tdx_mat_plan *p, const void *u_in, vo
     a.m = (float *)m;
     a.v = (float *)v;
     a.adam = (const float *)(w + p->off_adam);
-    hipLaunchKernelGGL((k_mat_one<float>), dim3(1), dim3(64), 0, s, (float *)(w + p->off_one));
-    hipLaunchKernelGGL((k_mat_adam_prep<float>), dim3(1), dim3(64), 0, s, (const int64_t *)step,
-                       h, (float *)(w + p->off_adam));
+    hipLaunchKernelGGL((k_mat_step_prep<float>), dim3(1), dim3(64), 0, s,
+                       (float *)(w + p->off_one), step, h, (float *)(w + p->off_adam));
     rc = mat_dispatch<float>(p, a, true, (float *)loss, s);
   } else {
     MatArgs<double> a = mat_args<double>(p, u_in, ws);
@@ -2095,13 +2116,11 @@ extern "C" int tdx_mat_train_step_to(const tdx_mat_plan *p, const void *u_in, vo
     a.m = (double *)m;
     a.v = (double *)v;
     a.adam = (const double *)(w + p->off_adam);
-    hipLaunchKernelGGL((k_mat_one<double>), dim3(1), dim3(64), 0, s, (double *)(w + p->off_one));
-    hipLaunchKernelGGL((k_mat_adam_prep<double>), dim3(1), dim3(64), 0, s,
-                       (const int64_t *)step, h, (double *)(w + p->off_adam));
+    hipLaunchKernelGGL((k_mat_step_prep<double>), dim3(1), dim3(64), 0, s,
+                       (double *)(w + p->off_one), step, h, (double *)(w + p->off_adam));
     rc = mat_dispatch<double>(p, a, true, (double *)loss, s);
   }
-  if (rc) return rc;
-  return tdx_step_increment(step, stream);
+  return rc;
 }
 
 extern "C" int tdx_mat_train_step(const tdx_mat_plan *p, void *u, void *m, void *v,
