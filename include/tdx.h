@@ -508,11 +508,26 @@ int tdx_ad_backward(const tdx_ad_plan *plan, const void *params,
 int tdx_ad_jacobian(const tdx_ad_plan *plan, const void *params, void *jt,
                     tdx_stream_t stream);
 /* Fused step: forward, adjoint and torch.optim.Adam update (see
- * tdx_nn_train_step for step and loss). */
+ * tdx_nn_train_step for step and loss).  Float plans without causal or
+ * weak terms, with at most 16 jets, 128 hidden units, 4 H x H layers (3 at
+ * 113-128 units) and groups whose point sets times jets are at most 16 run
+ * it as three launches around one kernel that keeps a tile's jets on chip
+ * (forward, residual program, adjoint, weight gradients); other plans run
+ * the layer kernels of tdx_ad_forward/tdx_ad_backward and tdx_adam.  The
+ * two differ in the order of the float sums only.  Environment:
+ * TDX_AD_FUSED=0 keeps the layer kernels; TDX_AD_TILES sets the tiles per
+ * work item of the fused kernel (default: about one item per compute
+ * unit).  Replaces Closure._closure (tedeous/optimizers/closure.py:49-64)
+ * with loss.backward() through Derivative_autograd
+ * (tedeous/derivative.py:61-132). */
 int tdx_ad_train_step(const tdx_ad_plan *plan, void *params, void *exp_avg,
                       void *exp_avg_sq, int64_t *step, double lr, double beta1,
                       double beta2, double eps, void *loss, void *workspace,
                       tdx_stream_t stream);
+/* 1 when tdx_ad_train_step runs the fused kernel (k_ad_step), 0 when it
+ * runs the layer kernels, -1 for a null plan.  Diagnostic; no counterpart
+ * in the reference. */
+int tdx_ad_step_kernel(const tdx_ad_plan *plan);
 
 /* ===================================================================== */
 /* Optimizer                                                              */

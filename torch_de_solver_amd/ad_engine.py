@@ -228,6 +228,12 @@ class ADEngine:
             self.handle, native.ptr(params), native.ptr(dloss), native.ptr(dparams),
             native.ptr(self.workspace), native.stream_ptr()))
 
+    @property
+    def step_kernel(self) -> str:
+        """'fused' when train_step runs the one-kernel step (k_ad_step),
+        'layers' when it runs the layer kernels and the separate Adam pass."""
+        return 'fused' if native.lib().tdx_ad_step_kernel(self.handle) == 1 else 'layers'
+
     def train_step(self, params, exp_avg, exp_avg_sq, step, lr, beta1, beta2, eps, loss) -> None:
         self._check(params)
         native.check(native.lib().tdx_ad_train_step(

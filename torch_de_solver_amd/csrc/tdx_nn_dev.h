@@ -104,6 +104,61 @@ __host__ __device__ inline int stage_words(int HP, int NH, int d_in, int n_out, 
   return NH * HP * HP * (with_wb ? 2 : 1) + small_words(HP, NH, d_in, n_out);
 }
 
+// Hidden feature held at accumulator position p of a block.
+template <typename T>
+__device__ __forceinline__ int feature_at(int p) {
+  return (p & ~15) + crow<T>((p & 15) >> 2, p & 3);
+}
+
+// Word `off` of an instance's packed image, from its parameters W (torch
+// layout).
+template <typename T, int NB>
+__device__ __forceinline__ T image_value(const NNArgs<T> &a, const T *W, int off) {
+  constexpr int HP = 16 * NB;
+  const int NH = a.NH, H = a.H, d = a.d_in, no = a.n_out;
+  const int img = HP * HP;
+  const int SW = small_words(HP, NH, d, no);
+  const Layout<T> L{d, no, H, NH};
+  {
+    T v = T(0);
+    if (off < NH * img || off >= NH * img + SW) {
+      const int which = off < NH * img ? 0 : 1;
+      if (which) off -= NH * img + SW;
+      const int j = off / img;
+      int r = off - j * img;
+      const int i = r & 3;
+      const int lane = (r >> 2) & 63;
+      r >>= 8;
+      const int mb = r % NB, xb = r / NB;
+      int out, in;
+      if (which == 0) {
+        out = 16 * xb + (lane & 15);
+        in = 16 * mb + crow<T>(lane >> 4, i);
+      } else {
+        out = 16 * mb + crow<T>(lane >> 4, i);
+        in = 16 * xb + (lane & 15);
+      }
+      if (out < H && in < H) v = W[L.oWh(j) + out * H + in];
+    } else {
+      const int sp = off - NH * img;
+      const int blk = sp / HP, f = feature_at<T>(sp - blk * HP);
+      if (blk == 0) {
+        if (f < H) v = W[L.ob1() + f];
+      } else if (blk < 1 + d) {
+        if (f < H) v = W[L.oW1() + f * d + (blk - 1)];
+      } else if (blk < 1 + d + NH) {
+        if (f < H) v = W[L.obh(blk - 1 - d) + f];
+      } else if (blk < 1 + d + NH + no) {
+        if (f < H) v = W[L.oWL() + (blk - 1 - d - NH) * H + f];
+      } else {
+        const int o = sp - HP * (1 + d + NH + no);
+        if (o < no) v = W[L.obL() + o];
+      }
+    }
+    return v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // network view: the packed image in global memory or its copy in LDS
 // ---------------------------------------------------------------------------
