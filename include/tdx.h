@@ -524,10 +524,20 @@ int tdx_ad_train_step(const tdx_ad_plan *plan, void *params, void *exp_avg,
                       void *exp_avg_sq, int64_t *step, double lr, double beta1,
                       double beta2, double eps, void *loss, void *workspace,
                       tdx_stream_t stream);
-/* 1 when tdx_ad_train_step runs the fused kernel (k_ad_step), 0 when it
- * runs the layer kernels, -1 for a null plan.  Diagnostic; no counterpart
- * in the reference. */
+/* Kernels of tdx_ad_train_step: 0 the layer kernels, 1 the fused kernel
+ * interpreting the jet maps and residual programs from the tables (k_ad_step),
+ * 2 the fused kernel compiled at plan creation with the plan's generated jet
+ * maps and residual programs (tdx_ad_step_jit; plans of at least 2^17
+ * (point, jet) rows, TDX_AD_JIT=0/1 to force); -1 for a null plan.
+ * Diagnostic; no counterpart in the reference. */
 int tdx_ad_step_kernel(const tdx_ad_plan *plan);
+/* The generated source of a descriptor's fused step kernel (no device
+ * needed): writes up to cap - 1 characters and a terminating zero into buf
+ * (may be NULL) and returns the full length; with compile != 0 also
+ * compiles it for gfx950 (hipRTC) and returns the error code if that fails;
+ * 0 when the fused step does not serve the descriptor, a negative error code
+ * for an invalid one.  Diagnostic; no counterpart in the reference. */
+int64_t tdx_ad_step_source(const tdx_ad_desc *desc, char *buf, int64_t cap, int compile);
 
 /* ===================================================================== */
 /* Optimizer                                                              */

@@ -50,20 +50,26 @@ def _net(layers, seed=0):
     return net
 
 
-def _solution(d, eq, b, kw, net_dev, fused):
+def _solution(d, eq, b, kw, net_dev, fused, jit=None):
+    """A Solution whose plan is created with TDX_AD_FUSED = fused and, if
+    given, TDX_AD_JIT = jit (generated jet maps and residual programs)."""
     grid = d.build('autograd')
     pre = Operator_bcond_preproc(grid, eq.equation_lst,
                                  b.build(d.variable_dict)).set_strategy('autograd')
-    old = os.environ.get('TDX_AD_FUSED')
-    os.environ['TDX_AD_FUSED'] = '1' if fused else '0'
+    env = {'TDX_AD_FUSED': '1' if fused else '0'}
+    if jit is not None:
+        env['TDX_AD_JIT'] = str(int(jit))
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         sol = Solution(grid, pre, net_dev, 'autograd', None, kw['lambda_operator'],
                        kw['lambda_bound'])
     finally:
-        if old is None:
-            del os.environ['TDX_AD_FUSED']
-        else:
-            os.environ['TDX_AD_FUSED'] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
     return sol
 
 
@@ -98,13 +104,16 @@ def _oracle(d, eq, b, kw, net64):
     return float(loss), float(norm), grad.astype(np.float64)
 
 
+@pytest.mark.parametrize('jit', [0, 1])
 @pytest.mark.parametrize('name', list(problems.AD_CASES))
-def test_ad_step_against_float32_oracle(name):
-    _print(f"[ad step] {name}")
+def test_ad_step_against_float32_oracle(name, jit):
+    """Interpreted (jit 0) and generated (jit 1) jet maps and residual
+    programs."""
+    _print(f"[ad step] {name} jit {jit}")
     d, eq, b, kw, layers = _case(name)
     net_dev = _net(layers).to(torch.float32).to(DEV)
-    sol = _solution(d, eq, b, kw, net_dev, fused=True)
-    assert sol.engine.step_kernel == 'fused', name
+    sol = _solution(d, eq, b, kw, net_dev, fused=True, jit=jit)
+    assert sol.engine.step_kernel == ('fused-generated' if jit else 'fused'), name
     loss, grad = _step_gradient(sol, net_dev)
     rl, rn, rg = _oracle(d, eq, b, kw, _net(layers))
     rel = abs(loss[0] - rl) / rl
@@ -122,7 +131,7 @@ def test_ad_step_against_layer_kernels(name):
     res = []
     for fused in (True, False):
         net_dev = _net(layers).to(torch.float32).to(DEV)
-        sol = _solution(d, eq, b, kw, net_dev, fused=fused)
+        sol = _solution(d, eq, b, kw, net_dev, fused=fused, jit=0)
         assert sol.engine.step_kernel == ('fused' if fused else 'layers')
         res.append(_step_gradient(sol, net_dev))
     (lf, gf), (ll, gl) = res
@@ -136,14 +145,17 @@ def test_ad_step_against_layer_kernels(name):
 def test_ad_step_poisson(grid_res):
     """The bench workload's problem and network (2-100x5-1: 7 column blocks,
     four H x H layers, three points of five jets per wavefront), at 21^2 and
-    at the full 201^2 points: fused against the layer kernels, and two
-    fused steps from the same state bit for bit."""
+    at the full 201^2 points (where the plan compiles the generated jet
+    maps and residual programs by default): fused against the layer
+    kernels, and two fused steps from the same state bit for bit."""
     _print(f"[ad step] poisson {grid_res}")
     res = []
     for fused in (True, False):
         d, eq, b, net, kw = workloads.poisson_ad(grid_res)
         net_dev = net.to(torch.float32).to(DEV)
         sol = _solution(d, eq, b, kw, net_dev, fused=fused)
+        expect = ('fused-generated' if grid_res == 200 else 'fused') if fused else 'layers'
+        assert sol.engine.step_kernel == expect
         res.append(_step_gradient(sol, net_dev))
         if fused:
             again = _step_gradient(sol, net_dev)
@@ -156,13 +168,14 @@ def test_ad_step_poisson(grid_res):
 
 
 def test_ad_step_poisson_small_against_oracle():
-    """Poisson at 21^2 points on the 2-100x5-1 network against the float32
-    oracle."""
+    """Poisson at 21^2 points on the 2-100x5-1 network, generated kernel,
+    against the float32 oracle."""
     _print("[ad step] poisson 20 vs oracle")
     d, eq, b, net, kw = workloads.poisson_ad(20)
     net64 = net.to(torch.float64)
     net_dev = workloads.poisson_ad(20)[3].to(torch.float32).to(DEV)
-    sol = _solution(d, eq, b, kw, net_dev, fused=True)
+    sol = _solution(d, eq, b, kw, net_dev, fused=True, jit=1)
+    assert sol.engine.step_kernel == 'fused-generated'
     loss, grad = _step_gradient(sol, net_dev)
     rl, rn, rg = _oracle(d, eq, b, kw, net64)
     rel = abs(loss[0] - rl) / rl
@@ -185,7 +198,7 @@ def test_ad_step_model_train():
         model.verbose = False
         model.compile('autograd', lambda_operator=1, lambda_bound=100)
         if fused:
-            assert model.solution_cls.engine.step_kernel == 'fused'
+            assert model.solution_cls.engine.step_kernel in ('fused', 'fused-generated')
         model.train(tdx.Optimizer('Adam', {'lr': 1e-3}), 21, callbacks=None if fused else [object()])
         res.append(torch.cat([p.detach().reshape(-1) for p in net.parameters()]).cpu())
     np.testing.assert_allclose(res[0].numpy(), res[1].numpy(), rtol=1e-5, atol=1e-6)

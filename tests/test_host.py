@@ -336,6 +336,36 @@ def test_wide_step_kernel_generates_and_compiles():
     assert gen.count('    case ') == 2  # the operator and the Dirichlet conditions
 
 
+def test_ad_step_kernel_generates_and_compiles():
+    """The fused 'autograd' step kernel for the Poisson workload (jets (),
+    x, y, xx, yy; 2-100x5-1) with generated jet maps and residual programs
+    compiles for gfx950 (hipRTC, no device); the generated jet maps hold the
+    Faa di Bruno terms of the second derivatives; float64 plans and plans
+    with more than 16 rows per entry are left out."""
+    from torch_de_solver_amd import workloads as W
+    from torch_de_solver_amd.ad_engine import ad_step_source
+    from torch_de_solver_amd.compiler import compile_ad
+    from torch_de_solver_amd.nn_engine import mlp_arch
+    dom, eq, conds, net, kw = W.poisson_ad(20)
+    prob = compile_ad(dom.build('autograd'), eq.equation_lst, conds.build(dom.variable_dict), 1,
+                      lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
+    src = ad_step_source(prob, mlp_arch(net), compile=True)
+    assert 'tdx_ad_step_jit' in src and 'ad_step_body<7, 4, tdx::GenJets' in src
+    jets = src[src.index('struct GenJets'):src.index('struct GenADResid')]
+    assert 'float v = ((s.s2 * z1) * z1);' in jets and 'v = v + (s.s1 * z3);' in jets
+    gen = src[src.index('struct GenADResid'):]
+    assert gen.count('    case ') == len(prob.groups)
+    d, eq, b, kw = problems.heat3_ad(api, dtype='float32')
+    prob3 = compile_ad(d.build('autograd'), eq.equation_lst, b.build(d.variable_dict), 1,
+                       lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
+    assert ad_step_source(prob3, mlp_arch(problems.make_mlp([3, 20, 20, 1])))
+    d, eq, b, kw = problems.wave_ad(api, dtype='float64')
+    prob64 = compile_ad(d.build('autograd'), eq.equation_lst, b.build(d.variable_dict), 1,
+                        lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
+    assert ad_step_source(prob64, mlp_arch(problems.make_mlp([2, 12, 12, 1]))) == ''
+    assert 'fused step' in native.lib().tdx_last_error().decode()
+
+
 def _ad_desc(name='wave_ad'):
     from torch_de_solver_amd.ad_engine import ADTables
     from torch_de_solver_amd.compiler import compile_ad

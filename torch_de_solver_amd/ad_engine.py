@@ -170,6 +170,22 @@ class ADTables:
         return desc
 
 
+def ad_step_source(prob: 'ADProblem', arch: 'MLPArch', compile: bool = False) -> str:
+    """The generated source of the fused 'autograd' step kernel for a
+    problem (no device needed; with ``compile`` also compiled for gfx950),
+    '' when the fused step does not serve it."""
+    t = ADTables(prob, np.float32 if prob.dtype == torch.float32 else np.float64)
+    desc = t.desc(prob, arch)
+    lib = native.lib()
+    n = int(lib.tdx_ad_step_source(ctypes.byref(desc), None, 0, int(compile)))
+    native.check(min(n, 0))
+    if n == 0:
+        return ''
+    buf = ctypes.create_string_buffer(n + 1)
+    lib.tdx_ad_step_source(ctypes.byref(desc), buf, n + 1, 0)
+    return buf.value.decode()
+
+
 class ADEngine:
     """One native 'autograd' plan (one network instance)."""
 
@@ -230,9 +246,13 @@ class ADEngine:
 
     @property
     def step_kernel(self) -> str:
-        """'fused' when train_step runs the one-kernel step (k_ad_step),
-        'layers' when it runs the layer kernels and the separate Adam pass."""
-        return 'fused' if native.lib().tdx_ad_step_kernel(self.handle) == 1 else 'layers'
+        """'fused' when train_step runs the one-kernel step interpreting the
+        tables (k_ad_step), 'fused-generated' when it runs that kernel
+        compiled with the plan's generated jet maps and residual programs
+        (tdx_ad_step_jit), 'layers' for the layer kernels and the separate
+        Adam pass."""
+        k = native.lib().tdx_ad_step_kernel(self.handle)
+        return {1: 'fused', 2: 'fused-generated'}.get(k, 'layers')
 
     def train_step(self, params, exp_avg, exp_avg_sq, step, lr, beta1, beta2, eps, loss) -> None:
         self._check(params)

@@ -272,7 +272,7 @@ int nn_fused_jit_build(const std::string &src, std::vector<char> *code, const st
     std::string log(ls, '\0');
     if (ls) hiprtcGetProgramLog(prog, &log[0]);
     hiprtcDestroyProgram(&prog);
-    set_error("fused 'NN' kernel failed to compile: %.400s", log.c_str());
+    set_error("run-time compiled kernel failed to compile: %.400s", log.c_str());
     return TDX_ELAUNCH;
   }
   size_t cs = 0;
@@ -309,13 +309,13 @@ int nn_fused_jit_load(const std::string &src, hipFunction_t *fn, const char *nam
   if (rc) return rc;
   hipModule_t mod = nullptr;
   if (hipModuleLoadData(&mod, code.data()) != hipSuccess) {
-    set_error("hipModuleLoadData of the fused 'NN' kernel failed");
+    set_error("hipModuleLoadData of a run-time compiled kernel failed");
     return TDX_ELAUNCH;
   }
   hipFunction_t f = nullptr;
   if (hipModuleGetFunction(&f, mod, name) != hipSuccess) {
     (void)hipModuleUnload(mod);
-    set_error("fused 'NN' kernel: entry point missing");
+    set_error("run-time compiled kernel: entry point missing");
     return TDX_ELAUNCH;
   }
   cache[{dev, src}] = f;

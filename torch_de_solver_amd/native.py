@@ -38,7 +38,7 @@ EXPORTS = [
     'tdx_mat_specialized_points', 'tdx_mat_specialized_source',
     'tdx_ad_plan_create', 'tdx_ad_plan_destroy', 'tdx_ad_param_count',
     'tdx_ad_workspace_bytes', 'tdx_ad_forward', 'tdx_ad_backward', 'tdx_ad_train_step',
-    'tdx_ad_step_kernel',
+    'tdx_ad_step_kernel', 'tdx_ad_step_source',
     'tdx_ad_jacobian',
 ]
 
@@ -171,6 +171,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.tdx_ad_jacobian.argtypes = [vp, vp, vp, vp]
     lib.tdx_ad_train_step.argtypes = [vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]
     lib.tdx_ad_step_kernel.argtypes = [vp]
+    lib.tdx_ad_step_source.argtypes = [ctypes.POINTER(ADDesc), ctypes.c_char_p, i64, ctypes.c_int]
+    lib.tdx_ad_step_source.restype = i64
     lib.tdx_adam.argtypes = [ctypes.c_int, i64, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, vp]
     lib.tdx_step_increment.argtypes = [vp, vp]
 
