@@ -583,11 +583,26 @@ def bench_ad(args):
                    "jets": [list(j) for j in prob.jets], "parallelism": "single GPU"},
         "colloc_pts_per_s": N * 1e3 / ms,
         "roofline": {"bound": "mfma",
-                     "kernel": "fused train step: k_ad_fwd, k_ad_resid, k_ad_bwd, k_ad_wgrad, "
-                               "k_ad_wreduce, k_adam (linear jet maps on FP32 MFMA)",
+                     "kernel": {
+                         'fused-generated': "fused train step: k_ad_pack, tdx_ad_step_jit "
+                                            "(forward jets, generated residual programs and "
+                                            "jet maps, adjoint, weight gradients; compiled at "
+                                            "plan creation), k_ad_step_sum",
+                         'fused': "fused train step: k_ad_pack, k_ad_step (forward jets, "
+                                  "interpreted residual programs and jet maps, adjoint, weight "
+                                  "gradients), k_ad_step_sum",
+                     }.get(eng.step_kernel,
+                           "train step: k_ad_fwd, k_ad_resid, k_ad_bwd, k_ad_wgrad, "
+                           "k_ad_wreduce, k_adam (linear jet maps on FP32 MFMA)"),
                      "achieved": flops / (step_ms * 1e-3) / 1e12,
                      "peak": FP32_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": flops / (step_ms * 1e-3) / FP32_MFMA_PEAK, "traffic": None,
+                     "frac": flops / (step_ms * 1e-3) / FP32_MFMA_PEAK,
+                     "traffic": (profiled_traffic('tdx_ad_step_jit_poisson')
+                                 if eng.step_kernel == 'fused-generated' and args.grid_res == 200
+                                 else None),
+                     "traffic_source": "profiles/tdx_ad_step_jit_poisson_traffic.txt (rocprofv3 "
+                                       "FETCH_SIZE/WRITE_SIZE, copy-calibrated; the step kernel "
+                                       "of the 201x201 problem)",
                      "algorithmic_flops_per_launch": flops, "avg_launch_ms": step_ms},
         "forward_pass_ms": fwd_ms, "backward_pass_ms": bwd_ms,
     }
