@@ -432,9 +432,31 @@ int tdx_mat_train_step_to(const tdx_mat_plan *plan, const void *u_in,
  * times the product of its factors.
  * Factor (TDX_AD_FACTOR_INTS): (n_st, st_i, var, pow_kind, pow_ref):
  * (sum over its stencil entries of w * jet component) ** pow, pow the
- * integer pow_ref (kind 0) or reals[pow_ref] (kind 1).
+ * integer pow_ref (kind 0) or reals[pow_ref] (kind 1).  Kind 2 is a callable
+ * power, applied as the reference does to the running product of the term
+ * (tedeous/derivative.py:124-129: P = g(P * factor), P starting at 1, the
+ * other factors P = P * factor ** pow): pow_ref is the ints offset of g as a
+ * chain of one-argument operations, n then n (op, arg) pairs, applied to the
+ * value in order; op TDX_AD_OP_*, arg an index into reals (ADD..RDIV, POWR),
+ * an integer power (POWI) or unused.
  * Stencil entry (TDX_AD_STENCIL_INTS): (set, jet, weight_r).
  */
+#define TDX_AD_OP_ADD 1   /* v + c */
+#define TDX_AD_OP_SUB 2   /* v - c */
+#define TDX_AD_OP_RSUB 3  /* c - v */
+#define TDX_AD_OP_MUL 4   /* c * v */
+#define TDX_AD_OP_DIV 5   /* v / c */
+#define TDX_AD_OP_RDIV 6  /* c / v */
+#define TDX_AD_OP_POWI 7  /* v ** k, k the integer arg */
+#define TDX_AD_OP_POWR 8  /* v ** c */
+#define TDX_AD_OP_NEG 9
+#define TDX_AD_OP_SIN 10
+#define TDX_AD_OP_COS 11
+#define TDX_AD_OP_EXP 12
+#define TDX_AD_OP_LOG 13
+#define TDX_AD_OP_TANH 14
+#define TDX_AD_OP_SQRT 15
+#define TDX_AD_MAX_CHAIN 16  /* operations of one callable power */
 #define TDX_AD_MAX_ORDER 4
 #define TDX_AD_PART_INTS 5
 #define TDX_AD_GROUP_INTS 5

@@ -640,3 +640,36 @@ def heat3_ad(api, n=5, dtype='float64'):
 
 
 AD_CASES['heat3_ad'] = (heat3_ad, [3, 20, 20, 1])
+
+
+def sin_source_ad(api, n=10, dtype='float64'):
+    """Callable powers (tedeous/derivative.py:124-129: the callable applied
+    to the term's running product), after the reference's long-time heat
+    example (examples/examples_heat/example_heat_2d_long_time.py:70-100):
+    u_t - 0.01 u_xx + c(x, t) sin(3 u^2) + 0.5 (u u_x)^2 + exp(-u_x / 2) = 0,
+    the second product term written as [u, u_x] with pow [1, v -> v * v]
+    (the callable sees u * u_x), the last as u_x with pow v -> exp(-v / 2)."""
+    k = 3.0
+    d = api.Domain()
+    d.variable('x', [0, 1], n, dtype=dtype)
+    d.variable('t', [0, 0.5], n, dtype=dtype)
+    b = api.Conditions()
+    x = d.variable_dict['x']
+    b.dirichlet({'x': [0, 1], 't': 0}, value=torch.sin(np.pi * x))
+    b.dirichlet({'x': 0, 't': [0, 0.5]}, value=0)
+    b.dirichlet({'x': 1, 't': [0, 0.5]}, value=0)
+    eq = api.Equation()
+    eq.add({
+        'du/dt': {'coeff': 1, 'du/dt': [1], 'pow': 1, 'var': 0},
+        '-0.01 d2u/dx2': {'coeff': -0.01, 'd2u/dx2': [0, 0], 'pow': 1, 'var': 0},
+        'c sin(k u^2)': {'coeff': lambda g: 1 + g[:, 0:1] * g[:, 1:2], 'u': [None],
+                         'pow': lambda u: torch.sin(k * u ** 2), 'var': 0},
+        '(u u_x)^2': {'coeff': 0.5, 'u*u_x': [[None], [0]], 'pow': [1, lambda v: v * v],
+                      'var': [0, 0]},
+        'exp(-u_x/2)': {'coeff': 1, 'du/dx': [0], 'pow': lambda v: torch.exp(-v / 2),
+                        'var': 0},
+    })
+    return d, eq, b, dict(mode='autograd', lambda_operator=1, lambda_bound=10)
+
+
+AD_CASES['sin_source_ad'] = (sin_source_ad, [2, 16, 16, 1])

@@ -359,6 +359,13 @@ def test_ad_step_kernel_generates_and_compiles():
     prob3 = compile_ad(d.build('autograd'), eq.equation_lst, b.build(d.variable_dict), 1,
                        lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
     assert ad_step_source(prob3, mlp_arch(problems.make_mlp([3, 20, 20, 1])))
+    # callable powers: the generated residual function calls chain_term
+    d, eq, b, kw = problems.sin_source_ad(api, dtype='float32')
+    probc = compile_ad(d.build('autograd'), eq.equation_lst, b.build(d.variable_dict), 1,
+                       lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])
+    srcc = ad_step_source(probc, mlp_arch(problems.make_mlp([2, 16, 16, 1])), compile=True)
+    genc = srcc[srcc.index('struct GenADResid'):]
+    assert genc.count('chain_term(a.ints') == 6  # three terms: value and seeds
     d, eq, b, kw = problems.wave_ad(api, dtype='float64')
     prob64 = compile_ad(d.build('autograd'), eq.equation_lst, b.build(d.variable_dict), 1,
                         lambda_operator=kw['lambda_operator'], lambda_bound=kw['lambda_bound'])

@@ -158,6 +158,7 @@ AD_JAC_CASES = {
     'beam_ad': [1, 5, 5, 1],
     'system_ad': [1, 6, 6, 2],
     'mixed_ad': [2, 5, 5, 1],
+    'sin_source_ad': [2, 5, 5, 1],   # callable powers
 }
 
 

@@ -432,6 +432,33 @@ def test_ad_oracle_residual_and_conditions_closed_form():
                                rtol=1e-12, atol=1e-12)
 
 
+def test_ad_oracle_callable_pow_closed_form():
+    """Callable powers act on the term's running product
+    (tedeous/derivative.py:124-129): with the analytic stand-in network the
+    residual of sin_source_ad is u_t - 0.01 u_xx + (1 + x t) sin(3 u^2)
+    + 0.5 (u u_x)^2 + exp(-u_x / 2) -- the callable of [u, u_x] sees u * u_x,
+    not u_x alone -- and the engine's operation chains of the three callables
+    are the traced operations in order."""
+    from torch_de_solver_amd.compiler import AD_OP, callable_chain
+    net = _Analytic()
+    d, eq, b, kw = problems.sin_source_ad(api, n=6)
+    orc = O.ADOracle(_axes(d), eq.equation_lst, _conds(b))
+    _, _, op, *_ = orc.evaluate(net, kw['lambda_operator'], kw['lambda_bound'])
+    g = orc.grid.numpy()
+    x, t = g[:, 0], g[:, 1]
+    D = _u0_derivs(x, t)
+    want = D[(1,)] - 0.01 * D[(0, 0)] + (1 + x * t) * np.sin(3 * D[()] ** 2) \
+        + 0.5 * (D[()] * D[(0,)]) ** 2 + np.exp(-D[(0,)] / 2)
+    np.testing.assert_allclose(op.detach().numpy().reshape(-1), want, rtol=1e-12, atol=1e-12)
+    assert callable_chain(lambda u: torch.sin(3.0 * u ** 2)) == \
+        [(AD_OP['POWI'], 2.0), (AD_OP['MUL'], 3.0), (AD_OP['SIN'], 0.0)]
+    assert callable_chain(lambda v: torch.exp(-v / 2)) == \
+        [(AD_OP['NEG'], 0.0), (AD_OP['DIV'], 2.0), (AD_OP['EXP'], 0.0)]
+    assert callable_chain(lambda v: 1 - 2 / v) == [(AD_OP['RDIV'], 2.0), (AD_OP['RSUB'], 1.0)]
+    with pytest.raises(NotImplementedError):
+        callable_chain(lambda v: v * torch.sin(v))
+
+
 def test_ad_oracle_system_exact_solution():
     """u' = v, v' = -u is solved exactly by (cos t, -sin t): residuals vanish
     and the loss is the boundary misfit alone."""

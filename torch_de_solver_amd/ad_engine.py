@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import native
-from .compiler import ADProblem
+from .compiler import AD_OP, ADProblem
 from .nn_engine import MLPArch
 
 
@@ -103,7 +103,17 @@ class ADTables:
                         st_i = len(ints)
                         for (s, jet, w) in f.stencil:
                             ints += [s, jet_index[jet], add_real(w)]
-                        pref = int(f.pow_val) if f.pow_kind == 0 else add_real(f.pow_val)
+                        if f.pow_kind == 2:
+                            # callable power: n, then (op, arg) pairs; arg
+                            # the integer power (POWI) or a real's index
+                            pref = len(ints)
+                            ints.append(len(f.pow_val))
+                            for op, arg in f.pow_val:
+                                ints += [op, int(arg) if op == AD_OP['POWI'] else
+                                         (add_real(arg) if op <= AD_OP['RDIV'] or
+                                          op == AD_OP['POWR'] else 0)]
+                        else:
+                            pref = int(f.pow_val) if f.pow_kind == 0 else add_real(f.pow_val)
                         ints[fac_i + 5 * fi: fac_i + 5 * fi + 5] = [len(f.stencil), st_i, f.var,
                                                                      f.pow_kind, pref]
                     if t.coeff_array is not None:

@@ -60,10 +60,121 @@ def _pow_code(p) -> Tuple[int, float]:
     """(kind, value): kind 0 = integer power, 1 = real power."""
     if callable(p):
         raise NotImplementedError(
-            "a callable 'pow' is not supported by the native engine")
+            "a callable 'pow' is served in 'autograd' mode only")
     if float(p) == int(p):
         return 0, float(int(p))
     return 1, float(p)
+
+
+# one-argument operations of a callable power (the TDX_AD_OP_* codes)
+AD_OP = dict(ADD=1, SUB=2, RSUB=3, MUL=4, DIV=5, RDIV=6, POWI=7, POWR=8, NEG=9, SIN=10,
+             COS=11, EXP=12, LOG=13, TANH=14, SQRT=15)
+AD_MAX_CHAIN = 16
+_UNARY = {'sin': 'SIN', 'cos': 'COS', 'exp': 'EXP', 'log': 'LOG', 'tanh': 'TANH',
+          'sqrt': 'SQRT', 'neg': 'NEG'}
+
+
+def callable_chain(fn) -> List[Tuple[int, float]]:
+    """A callable power (applied by the reference to the term's running
+    product, tedeous/derivative.py:124-129) as the chain of one-argument
+    operations the 'autograd' kernels evaluate, [(op, arg)]: arg the real
+    constant (ADD..RDIV, POWR), the integer power (POWI) or 0.  The callable
+    is traced with torch.fx; it must apply, in sequence, +, -, *, / or **
+    with a number, unary minus, or torch.sin/cos/exp/log/tanh/sqrt to its
+    argument (e.g. lambda u: torch.sin(k * u ** 2)); anything else raises
+    NotImplementedError."""
+    import operator
+    import torch.fx as fx
+    try:
+        graph = fx.symbolic_trace(fn).graph
+    except Exception as exc:  # control flow, non-traceable callables
+        raise NotImplementedError(f"a callable 'pow' that torch.fx cannot trace: {exc}")
+    bad = NotImplementedError("a callable 'pow' must be a chain of one-argument operations "
+                              "(+, -, *, / or ** with a number, unary minus, "
+                              "torch.sin/cos/exp/log/tanh/sqrt)")
+    chain: List[Tuple[int, float]] = []
+    cur = None
+
+    def num(v):
+        if isinstance(v, bool) or not isinstance(v, (int, float)):
+            raise bad
+        return float(v)
+
+    for node in graph.nodes:
+        if node.op == 'placeholder':
+            if cur is not None:
+                raise bad
+            cur = node
+            continue
+        if node.op == 'output':
+            if node.args[0] is not cur:
+                raise bad
+            break
+        if node.op == 'call_method':
+            name = node.target
+        elif node.op == 'call_function':
+            name = getattr(node.target, '__name__', '')
+        else:
+            raise bad
+        args = list(node.args)
+        if node.kwargs:
+            raise bad
+        nodes = [a for a in args if isinstance(a, fx.Node)]
+        if any(a is not cur for a in nodes) or not nodes:
+            raise bad
+        tgt = node.target
+        if name in _UNARY or tgt in (operator.neg,):
+            if len(args) != 1:
+                raise bad
+            chain.append((AD_OP[_UNARY.get(name, 'NEG')], 0.0))
+        elif tgt in (operator.mul, torch.mul) or name == 'mul':
+            if len(nodes) == 2:
+                chain.append((AD_OP['POWI'], 2.0))      # u * u
+            else:
+                c = num(args[1] if args[0] is cur else args[0])
+                chain.append((AD_OP['MUL'], c))
+        elif tgt in (operator.add, torch.add) or name == 'add':
+            if len(nodes) == 2:
+                chain.append((AD_OP['MUL'], 2.0))       # u + u
+            else:
+                c = num(args[1] if args[0] is cur else args[0])
+                chain.append((AD_OP['ADD'], c))
+        elif tgt in (operator.sub, torch.sub) or name == 'sub':
+            if len(nodes) == 2:
+                raise bad
+            if args[0] is cur:
+                chain.append((AD_OP['SUB'], num(args[1])))
+            else:
+                chain.append((AD_OP['RSUB'], num(args[0])))
+        elif tgt in (operator.truediv, torch.div) or name in ('div', 'truediv'):
+            if len(nodes) == 2:
+                raise bad
+            if args[0] is cur:
+                chain.append((AD_OP['DIV'], num(args[1])))
+            else:
+                chain.append((AD_OP['RDIV'], num(args[0])))
+        elif tgt in (operator.pow, torch.pow) or name == 'pow':
+            if len(nodes) == 2 or args[0] is not cur:
+                raise bad
+            e = num(args[1])
+            if e == int(e) and abs(e) <= 64:
+                chain.append((AD_OP['POWI'], float(int(e))))
+            else:
+                chain.append((AD_OP['POWR'], e))
+        else:
+            raise bad
+        cur = node
+    if len(chain) > AD_MAX_CHAIN:
+        raise NotImplementedError(f"a callable 'pow' of more than {AD_MAX_CHAIN} operations")
+    return chain
+
+
+def _ad_pow_code(p):
+    """(kind, value) of an 'autograd' factor's power: 0 integer, 1 real,
+    2 callable (value: its operation chain)."""
+    if callable(p):
+        return 2, callable_chain(p)
+    return _pow_code(p)
 
 
 # ----------------------------------------------------------------------------
@@ -652,7 +763,7 @@ AD_MAX_ORDER = 4
 class ADFactor:
     var: int
     pow_kind: int
-    pow_val: float
+    pow_val: Any          # the power, or (kind 2) the operation chain of a callable
     stencil: List[Tuple[int, int, float]]      # (point set, jet, weight)
 
 
@@ -763,7 +874,7 @@ def _ad_terms(terms: List[Term], pts: torch.Tensor, set_k: int, sign: float = 1.
         for f in t.factors:
             if f.axes is not None and len(f.axes) > AD_MAX_ORDER:
                 raise NotImplementedError(f"derivative order above {AD_MAX_ORDER} in 'autograd' mode")
-            kind, pv = _pow_code(f.pow)
+            kind, pv = _ad_pow_code(f.pow)
             factors.append(ADFactor(var=f.var, pow_kind=kind, pow_val=pv,
                                     stencil=[(set_k, _jet_of(f), 1.0)]))
         if len(factors) > 4:

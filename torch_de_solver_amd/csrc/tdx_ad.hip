@@ -285,9 +285,16 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_resid(ResArgs<T> a) {
         const int32_t *Tr = a.ints + term_i + TDX_AD_TERM_INTS * t;
         const T c = Tr[2] == 0 ? a.reals[Tr[3]] : a.data[Tr[3] + e];
         T prod = T(1);
-        for (int f = 0; f < Tr[0]; ++f) {
-          const int32_t *F = a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f;
-          prod = prod * fpow(F[3], F[4], a.reals, factor_value(F));
+        if (term_has_chain(a.ints, Tr[1], Tr[0])) {
+          T fv[AD_MAXF], dfv[AD_MAXF];
+          for (int f = 0; f < AD_MAXF; ++f)
+            fv[f] = f < Tr[0] ? factor_value(a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f) : T(0);
+          prod = chain_term(a.ints, Tr[1], Tr[0], a.reals, fv, dfv);
+        } else {
+          for (int f = 0; f < Tr[0]; ++f) {
+            const int32_t *F = a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f;
+            prod = prod * fpow(F[3], F[4], a.reals, factor_value(F));
+          }
         }
         const T tv = c * prod;
         r = (t == 0) ? tv : r + tv;
@@ -302,18 +309,22 @@ __global__ __launch_bounds__(AD_THREADS) void k_ad_resid(ResArgs<T> a) {
         const int32_t *Tr = a.ints + term_i + TDX_AD_TERM_INTS * t;
         const int nf = Tr[0];
         const T c = Tr[2] == 0 ? a.reals[Tr[3]] : a.data[Tr[3] + e];
-        T fv[AD_MAXF], pw[AD_MAXF];
+        const bool chain = term_has_chain(a.ints, Tr[1], nf);
+        T fv[AD_MAXF], pw[AD_MAXF], dfv[AD_MAXF];
         for (int f = 0; f < AD_MAXF; ++f) {
+          fv[f] = T(0);
+          pw[f] = T(1);
           if (f < nf) {
             const int32_t *F = a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f;
             fv[f] = factor_value(F);
-            pw[f] = fpow(F[3], F[4], a.reals, fv[f]);
+            if (!chain) pw[f] = fpow(F[3], F[4], a.reals, fv[f]);
           }
         }
+        if (chain) (void)chain_term(a.ints, Tr[1], nf, a.reals, fv, dfv);
         for (int f = 0; f < nf; ++f) {
           const int32_t *F = a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f;
-          T dv = gq * c * fdpow(F[3], F[4], a.reals, fv[f]);
-          for (int f2 = 0; f2 < nf; ++f2)
+          T dv = chain ? gq * c * dfv[f] : gq * c * fdpow(F[3], F[4], a.reals, fv[f]);
+          for (int f2 = 0; f2 < nf && !chain; ++f2)
             if (f2 != f) dv = dv * pw[f2];
           const int n_st = F[0], st_i = F[1], var = F[2];
           for (int k = 0; k < n_st; ++k) {
@@ -822,20 +833,22 @@ __global__ void k_ad_jac(JacArgs<T> a) {
       const int32_t *Tr = a.ints + term_i + TDX_AD_TERM_INTS * tt;
       const int nf = Tr[0];
       const T c = Tr[2] == 0 ? a.reals[Tr[3]] : a.data[Tr[3] + e];
-      T fv[AD_MAXF], pw[AD_MAXF];
+      const bool chain = term_has_chain(a.ints, Tr[1], nf);
+      T fv[AD_MAXF], pw[AD_MAXF], dfv[AD_MAXF];
       for (int f = 0; f < AD_MAXF; ++f) {
         fv[f] = T(0);
         pw[f] = T(1);
         if (f < nf) {
           const int32_t *F = a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f;
           fv[f] = factor_value(F);
-          pw[f] = fpow(F[3], F[4], a.reals, fv[f]);
+          if (!chain) pw[f] = fpow(F[3], F[4], a.reals, fv[f]);
         }
       }
+      if (chain) (void)chain_term(a.ints, Tr[1], nf, a.reals, fv, dfv);
       for (int f = 0; f < nf; ++f) {
         const int32_t *F = a.ints + Tr[1] + TDX_AD_FACTOR_INTS * f;
-        T dv = c * fdpow(F[3], F[4], a.reals, fv[f]);
-        for (int f2 = 0; f2 < nf; ++f2)
+        T dv = chain ? c * dfv[f] : c * fdpow(F[3], F[4], a.reals, fv[f]);
+        for (int f2 = 0; f2 < nf && !chain; ++f2)
           if (f2 != f) dv = dv * pw[f2];
         for (int k = 0; k < F[0]; ++k) {
           const int32_t *S = a.ints + F[1] + TDX_AD_STENCIL_INTS * k;
@@ -1166,7 +1179,21 @@ int validate_ad(const tdx_ad_desc *d, std::vector<int64_t> *gstart_out, int *max
           const int nst = F[0], si = F[1], var = F[2];
           if (nst < 1 || !in_ints(si, (int64_t)TDX_AD_STENCIL_INTS * nst)) return fail("factor stencil");
           if (var < 0 || var >= d->n_out) return fail("factor variable");
-          if (F[3] == 1 ? !in_reals(F[4]) : F[3] != 0) return fail("factor power");
+          if (F[3] == 2) {
+            // callable power: a chain of one-argument operations
+            if (!in_ints(F[4], 1)) return fail("factor power chain");
+            const int n = I[F[4]];
+            if (n < 0 || n > TDX_AD_MAX_CHAIN || !in_ints(F[4] + 1, 2 * (int64_t)n))
+              return fail("factor power chain length");
+            for (int k = 0; k < n; ++k) {
+              const int op = I[F[4] + 1 + 2 * k], arg = I[F[4] + 2 + 2 * k];
+              if (op < TDX_AD_OP_ADD || op > TDX_AD_OP_SQRT) return fail("factor power chain op");
+              if ((op <= TDX_AD_OP_RDIV || op == TDX_AD_OP_POWR) && !in_reals(arg))
+                return fail("factor power chain constant");
+            }
+          } else if (F[3] == 1 ? !in_reals(F[4]) : F[3] != 0) {
+            return fail("factor power");
+          }
           for (int k = 0; k < nst; ++k) {
             const int32_t *S = I + si + TDX_AD_STENCIL_INTS * k;
             if (S[0] < 0 || S[0] >= ns || S[1] < 0 || S[1] >= J || !in_reals(S[2]))

@@ -63,6 +63,105 @@ __device__ __forceinline__ T fdpow(int kind, int ref, const T *reals, T x) {
   return p * dpow(x, p - T(1));
 }
 
+__device__ __forceinline__ float dsin(float x) { return sinf(x); }
+__device__ __forceinline__ double dsin(double x) { return sin(x); }
+__device__ __forceinline__ float dcos(float x) { return cosf(x); }
+__device__ __forceinline__ double dcos(double x) { return cos(x); }
+__device__ __forceinline__ float dexp(float x) { return expf(x); }
+__device__ __forceinline__ double dexp(double x) { return exp(x); }
+__device__ __forceinline__ float dlog(float x) { return logf(x); }
+__device__ __forceinline__ double dlog(double x) { return log(x); }
+__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+
+// A callable power: the chain of one-argument operations at ints[ref]
+// (n, then (op, arg) pairs) applied to x; v its value, d its derivative.
+template <typename T>
+__device__ __forceinline__ void chain_eval(const int32_t *I, int ref, const T *R, T x, T &v,
+                                           T &d) {
+  const int n = I[ref];
+  v = x;
+  d = T(1);
+  for (int k = 0; k < n; ++k) {
+    const int op = I[ref + 1 + 2 * k], arg = I[ref + 2 + 2 * k];
+    switch (op) {
+      case TDX_AD_OP_ADD: v = v + R[arg]; break;
+      case TDX_AD_OP_SUB: v = v - R[arg]; break;
+      case TDX_AD_OP_RSUB: v = R[arg] - v; d = -d; break;
+      case TDX_AD_OP_MUL: v = R[arg] * v; d = R[arg] * d; break;
+      case TDX_AD_OP_DIV: v = v / R[arg]; d = d / R[arg]; break;
+      case TDX_AD_OP_RDIV: {
+        const T nv = R[arg] / v;
+        d = -(nv / v) * d;
+        v = nv;
+      } break;
+      case TDX_AD_OP_POWI:
+        d = (arg == 0 ? T(0) : T(arg) * ipow(v, arg - 1)) * d;
+        v = ipow(v, arg);
+        break;
+      case TDX_AD_OP_POWR: {
+        const T p = R[arg];
+        d = (p * dpow(v, p - T(1))) * d;
+        v = dpow(v, p);
+      } break;
+      case TDX_AD_OP_NEG: v = -v; d = -d; break;
+      case TDX_AD_OP_SIN: d = dcos(v) * d; v = dsin(v); break;
+      case TDX_AD_OP_COS: d = -dsin(v) * d; v = dcos(v); break;
+      case TDX_AD_OP_EXP: v = dexp(v); d = v * d; break;
+      case TDX_AD_OP_LOG: d = d / v; v = dlog(v); break;
+      case TDX_AD_OP_TANH: v = dtanh(v); d = (T(1) - v * v) * d; break;
+      case TDX_AD_OP_SQRT: v = dsqrt(v); d = d / (T(2) * v); break;
+      default: break;
+    }
+  }
+}
+
+// Whether a term's factors include a callable power.
+__device__ __forceinline__ bool term_has_chain(const int32_t *I, int fi, int nf) {
+  bool c = false;
+  for (int f = 0; f < nf; ++f) c |= I[fi + TDX_AD_FACTOR_INTS * f + 3] == 2;
+  return c;
+}
+
+// A term with a callable power, as the reference forms it
+// (tedeous/derivative.py:124-129): the running product P = g(P * fv) at a
+// callable factor, P = P * fv ** p at the others, P starting at 1.  Returns
+// the final P and writes dP/dfv of every factor into dfv (zero beyond nf).
+template <typename T>
+__device__ __forceinline__ T chain_term(const int32_t *I, int fi, int nf, const T *R,
+                                        const T (&fv)[AD_MAXF], T (&dfv)[AD_MAXF]) {
+  T P = T(1);
+  T G[AD_MAXF], M[AD_MAXF];
+#pragma unroll
+  for (int f = 0; f < AD_MAXF; ++f) {
+    G[f] = T(0);
+    M[f] = T(1);
+    if (f < nf) {
+      const int F = fi + TDX_AD_FACTOR_INTS * f;
+      const int kind = I[F + 3], ref = I[F + 4];
+      if (kind == 2) {
+        T v, d;
+        chain_eval(I, ref, R, P * fv[f], v, d);
+        G[f] = d * P;
+        M[f] = d * fv[f];
+        P = v;
+      } else {
+        const T pw = fpow(kind, ref, R, fv[f]);
+        G[f] = P * fdpow(kind, ref, R, fv[f]);
+        M[f] = pw;
+        P = P * pw;
+      }
+    }
+  }
+  T g = T(1);
+#pragma unroll
+  for (int f = AD_MAXF - 1; f >= 0; --f) {
+    dfv[f] = g * G[f];
+    g = g * M[f];
+  }
+  return P;
+}
+
 // ---------------------------------------------------------------------------
 // fused training step (float): forward jets, residual program, adjoint and
 // weight gradients of a tile of whole entries in one workgroup
@@ -283,9 +382,17 @@ __device__ __forceinline__ void ad_entry(const ADStepArgs &a, const int32_t *G, 
       const int ck = ctab(a.ints, Tr + 2), cr = ctab(a.ints, Tr + 3);
       const float c = ck == 0 ? ctabf(a.reals, cr) : a.data[cr + e];
       float prod = 1.f;
-      for (int f = 0; f < nf; ++f) {
-        const int F = fi + TDX_AD_FACTOR_INTS * f;
-        prod = prod * fpow(ctab(a.ints, F + 3), ctab(a.ints, F + 4), a.reals, factor_value(F));
+      if (term_has_chain(a.ints, fi, nf)) {
+        float fv[AD_MAXF], dfv[AD_MAXF];
+#pragma unroll
+        for (int f = 0; f < AD_MAXF; ++f)
+          fv[f] = f < nf ? factor_value(fi + TDX_AD_FACTOR_INTS * f) : 0.f;
+        prod = chain_term(a.ints, fi, nf, a.reals, fv, dfv);
+      } else {
+        for (int f = 0; f < nf; ++f) {
+          const int F = fi + TDX_AD_FACTOR_INTS * f;
+          prod = prod * fpow(ctab(a.ints, F + 3), ctab(a.ints, F + 4), a.reals, factor_value(F));
+        }
       }
       const float tv = c * prod;
       r = (t == 0) ? tv : r + tv;
@@ -300,7 +407,8 @@ __device__ __forceinline__ void ad_entry(const ADStepArgs &a, const int32_t *G, 
       const int nf = ctab(a.ints, Tr), fi = ctab(a.ints, Tr + 1);
       const int ck = ctab(a.ints, Tr + 2), cr = ctab(a.ints, Tr + 3);
       const float c = ck == 0 ? ctabf(a.reals, cr) : a.data[cr + e];
-      float fv[AD_MAXF], pw[AD_MAXF];
+      float fv[AD_MAXF], pw[AD_MAXF], dfv[AD_MAXF];
+      const bool chain = term_has_chain(a.ints, fi, nf);
 #pragma unroll
       for (int f = 0; f < AD_MAXF; ++f) {
         fv[f] = 0.f;
@@ -308,17 +416,23 @@ __device__ __forceinline__ void ad_entry(const ADStepArgs &a, const int32_t *G, 
         if (f < nf) {
           const int F = fi + TDX_AD_FACTOR_INTS * f;
           fv[f] = factor_value(F);
-          pw[f] = fpow(ctab(a.ints, F + 3), ctab(a.ints, F + 4), a.reals, fv[f]);
+          if (!chain) pw[f] = fpow(ctab(a.ints, F + 3), ctab(a.ints, F + 4), a.reals, fv[f]);
         }
       }
+      if (chain) (void)chain_term(a.ints, fi, nf, a.reals, fv, dfv);
 #pragma unroll
       for (int f = 0; f < AD_MAXF; ++f) {
         if (f < nf) {
           const int F = fi + TDX_AD_FACTOR_INTS * f;
-          float dv = gq * c * fdpow(ctab(a.ints, F + 3), ctab(a.ints, F + 4), a.reals, fv[f]);
+          float dv;
+          if (chain) {
+            dv = gq * c * dfv[f];
+          } else {
+            dv = gq * c * fdpow(ctab(a.ints, F + 3), ctab(a.ints, F + 4), a.reals, fv[f]);
 #pragma unroll
-          for (int f2 = 0; f2 < AD_MAXF; ++f2)
-            if (f2 != f && f2 < nf) dv = dv * pw[f2];
+            for (int f2 = 0; f2 < AD_MAXF; ++f2)
+              if (f2 != f && f2 < nf) dv = dv * pw[f2];
+          }
           const int n_st = ctab(a.ints, F), st_i = ctab(a.ints, F + 1), var = ctab(a.ints, F + 2);
           for (int k = 0; k < n_st; ++k) {
             const int S = st_i + TDX_AD_STENCIL_INTS * k;

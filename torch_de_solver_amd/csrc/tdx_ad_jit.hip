@@ -15,6 +15,7 @@
 #include "tdx_ad_jit.h"
 
 #include <sstream>
+#include <utility>
 
 #include "tdx_common.h"
 // kTdxDevSrc, kTdxNnDevSrc, kTdxAdDevSrc: the texts of tdx_dev.h,
@@ -138,6 +139,25 @@ void emit_group(std::ostringstream &o, const AdJitSpec &sp, int g) {
       o << "        " << dst << (k == 0 ? " = " + t : " = " + dst + " + " + t) << ";\n";
     }
   };
+  auto has_chain = [&](int fi, int nf) {
+    for (int f = 0; f < nf; ++f)
+      if (I[fi + TDX_AD_FACTOR_INTS * f + 3] == 2) return true;
+    return false;
+  };
+  // a term with a callable power: the factor values into fvv, then the
+  // running product and its derivatives from the tables (chain_term)
+  auto chain_call = [&](int fi, int nf) {
+    o << "        float fvv[AD_MAXF], dfv[AD_MAXF];\n";
+    for (int f = 0; f < 4; ++f) {
+      if (f < nf) {
+        stencil(fi + TDX_AD_FACTOR_INTS * f, "fvv[" + num(f) + "]");
+      } else {
+        o << "        fvv[" << f << "] = 0.0f;\n";
+      }
+    }
+    o << "        const float pch = chain_term(a.ints, " << fi << ", " << nf
+      << ", a.reals, fvv, dfv);\n";
+  };
   for (int q = 0; q < neq; ++q) {
     const int E = eq_i + TDX_AD_EQ_INTS * q;
     const int nt = I[E], ti = I[E + 1];
@@ -146,11 +166,16 @@ void emit_group(std::ostringstream &o, const AdJitSpec &sp, int g) {
       const int Tr = ti + TDX_AD_TERM_INTS * t;
       const int nf = I[Tr], fi = I[Tr + 1];
       o << "      {\n        const float c = " << coeff(Tr) << ";\n        float prod = 1.0f;\n";
-      for (int f = 0; f < nf; ++f) {
-        const int F = fi + TDX_AD_FACTOR_INTS * f;
-        o << "        {\n        float fv;\n";
-        stencil(F, "fv");
-        o << "        prod = prod * " << pow_expr(I[F + 3], I[F + 4], "fv") << ";\n        }\n";
+      if (has_chain(fi, nf)) {
+        chain_call(fi, nf);
+        o << "        prod = pch;\n";
+      } else {
+        for (int f = 0; f < nf; ++f) {
+          const int F = fi + TDX_AD_FACTOR_INTS * f;
+          o << "        {\n        float fv;\n";
+          stencil(F, "fv");
+          o << "        prod = prod * " << pow_expr(I[F + 3], I[F + 4], "fv") << ";\n        }\n";
+        }
       }
       o << "        const float tv = c * prod;\n        " << (t == 0 ? "r = tv;" : "r = r + tv;")
         << "\n      }\n";
@@ -166,6 +191,22 @@ void emit_group(std::ostringstream &o, const AdJitSpec &sp, int g) {
       const int Tr = ti + TDX_AD_TERM_INTS * t;
       const int nf = I[Tr], fi = I[Tr + 1];
       o << "      {\n        const float c = " << coeff(Tr) << ";\n";
+      if (has_chain(fi, nf)) {
+        chain_call(fi, nf);
+        for (int f = 0; f < nf; ++f) {
+          const int F = fi + TDX_AD_FACTOR_INTS * f;
+          o << "        {\n          const float dv = gq * c * dfv[" << f << "];\n";
+          const int nst = I[F], sti = I[F + 1], var = I[F + 2];
+          for (int k = 0; k < nst; ++k) {
+            const int S = sti + TDX_AD_STENCIL_INTS * k;
+            o << "          SD[" << ((int64_t)I[S] * J + I[S + 1]) * no + var << "] += dv * R["
+              << I[S + 2] << "];\n";
+          }
+          o << "        }\n";
+        }
+        o << "      }\n";
+        continue;
+      }
       for (int f = 0; f < nf; ++f) {
         const int F = fi + TDX_AD_FACTOR_INTS * f;
         o << "        float fv" << f << ";\n";
@@ -211,6 +252,15 @@ std::string ad_step_jit_source(const AdJitSpec &sp) {
     << "#define TDX_AD_TERM_INTS " << TDX_AD_TERM_INTS << "\n"
     << "#define TDX_AD_FACTOR_INTS " << TDX_AD_FACTOR_INTS << "\n"
     << "#define TDX_AD_STENCIL_INTS " << TDX_AD_STENCIL_INTS << "\n"
+    << "#define TDX_AD_MAX_CHAIN " << TDX_AD_MAX_CHAIN << "\n";
+  const std::pair<const char *, int> ops[] = {
+      {"ADD", TDX_AD_OP_ADD},   {"SUB", TDX_AD_OP_SUB},   {"RSUB", TDX_AD_OP_RSUB},
+      {"MUL", TDX_AD_OP_MUL},   {"DIV", TDX_AD_OP_DIV},   {"RDIV", TDX_AD_OP_RDIV},
+      {"POWI", TDX_AD_OP_POWI}, {"POWR", TDX_AD_OP_POWR}, {"NEG", TDX_AD_OP_NEG},
+      {"SIN", TDX_AD_OP_SIN},   {"COS", TDX_AD_OP_COS},   {"EXP", TDX_AD_OP_EXP},
+      {"LOG", TDX_AD_OP_LOG},   {"TANH", TDX_AD_OP_TANH}, {"SQRT", TDX_AD_OP_SQRT}};
+  for (const auto &op : ops) o << "#define TDX_AD_OP_" << op.first << " " << op.second << "\n";
+  o
     << kTdxDevSrc << "\n"
     << kTdxNnDevSrc << "\n"
     << kTdxAdDevSrc << "\n";
