@@ -353,3 +353,40 @@ def test_schrodinger_matrix_generated_kernel():
     np.testing.assert_array_equal(out[1][2], out[0][2])
     np.testing.assert_array_equal(out[1][3], out[0][3])
     np.testing.assert_allclose(out[1][0], out[0][0], rtol=1e-6)
+
+
+def test_nn_callable_pow_against_oracle():
+    """Callable powers in 'NN' mode (tedeous/derivative.py:54-55: the
+    callable applied to the term's running product of stencil sums): the
+    sin_source problem (sin(3 u^2) with a coefficient function, (u u_x)^2 as
+    [u, u_x] with pow [1, v -> v * v], exp(-u_x / 2)) in float64 against
+    the oracle -- loss 1e-10, operator residual and gradient 1e-8 -- and the
+    plan runs the separate kernels (callables stay out of the fused
+    kernels' tables)."""
+    _print("[nn callable pow] sin_source float64")
+    d, eq, b, kw = problems.sin_source_ad(api)
+    h = 0.01
+    net = problems.make_mlp([2, 16, 16, 1], seed=3, dtype=torch.float64)
+    problems.randomize_biases(net, seed=4)
+    ref = problems.make_mlp([2, 16, 16, 1], seed=3, dtype=torch.float64)
+    ref.load_state_dict(net.state_dict())
+    orc = O.NNOracle(dict(d.variable_dict), eq.equation_lst, [dict(c) for c in b.conditions_lst],
+                     h=h)
+    lo, no, op, bv, tv, keys, lens = orc.evaluate(ref, 1, 10)
+    lo.backward()
+    g_ref = torch.cat([p.grad.reshape(-1) for p in ref.parameters()]).numpy()
+    grid = d.build('NN')
+    pre = Operator_bcond_preproc(grid, eq.equation_lst, b.build(d.variable_dict),
+                                 h=h).set_strategy('NN')
+    net = net.to(DEV)
+    sol = Solution(grid, pre, net, 'NN', None, 1, 10)
+    assert sol.engine.step_kernel == 'separate'
+    ln, _ = sol.evaluate()
+    ln.backward()
+    g = torch.cat([p.grad.reshape(-1) for p in net.parameters()]).cpu().numpy()
+    opn = sol.op.detach().cpu().numpy().reshape(-1)
+    opo = op.detach().numpy().reshape(-1)
+    _print(f"  loss {float(ln):.12g} (oracle {float(lo):.12g})")
+    np.testing.assert_allclose(float(ln), float(lo), rtol=1e-10)
+    np.testing.assert_allclose(opn, opo, rtol=1e-8, atol=1e-8 * np.abs(opo).max())
+    np.testing.assert_allclose(g, g_ref, rtol=1e-8, atol=1e-10 * np.abs(g_ref).max())

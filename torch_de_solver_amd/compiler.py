@@ -60,7 +60,7 @@ def _pow_code(p) -> Tuple[int, float]:
     """(kind, value): kind 0 = integer power, 1 = real power."""
     if callable(p):
         raise NotImplementedError(
-            "a callable 'pow' is served in 'autograd' mode only")
+            "a callable 'pow' is served in 'NN' and 'autograd' modes only")
     if float(p) == int(p):
         return 0, float(int(p))
     return 1, float(p)
@@ -170,8 +170,8 @@ def callable_chain(fn) -> List[Tuple[int, float]]:
 
 
 def _ad_pow_code(p):
-    """(kind, value) of an 'autograd' factor's power: 0 integer, 1 real,
-    2 callable (value: its operation chain)."""
+    """(kind, value) of an 'NN' or 'autograd' factor's power: 0 integer,
+    1 real, 2 callable (value: its operation chain)."""
     if callable(p):
         return 2, callable_chain(p)
     return _pow_code(p)
@@ -293,7 +293,7 @@ def _nn_terms(terms: List[Term], nvars: int, point_type: str, order: str,
                     slot_index[shift] = len(shifts)
                     shifts.append(shift)
                 st.append((slot_index[shift], w))
-            kind, pv = _pow_code(f.pow)
+            kind, pv = _ad_pow_code(f.pow)
             factors.append(NNFactor(var=f.var, pow_kind=kind, pow_val=pv,
                                     stencil=st))
         out_terms.append(NNTerm(coeff_scalar=cs, coeff_array=ca,

@@ -63,57 +63,12 @@ __device__ __forceinline__ T fdpow(int kind, int ref, const T *reals, T x) {
   return p * dpow(x, p - T(1));
 }
 
-__device__ __forceinline__ float dsin(float x) { return sinf(x); }
-__device__ __forceinline__ double dsin(double x) { return sin(x); }
-__device__ __forceinline__ float dcos(float x) { return cosf(x); }
-__device__ __forceinline__ double dcos(double x) { return cos(x); }
-__device__ __forceinline__ float dexp(float x) { return expf(x); }
-__device__ __forceinline__ double dexp(double x) { return exp(x); }
-__device__ __forceinline__ float dlog(float x) { return logf(x); }
-__device__ __forceinline__ double dlog(double x) { return log(x); }
-__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
-__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
-
-// A callable power: the chain of one-argument operations at ints[ref]
-// (n, then (op, arg) pairs) applied to x; v its value, d its derivative.
+// A callable power at ints[ref] of the plan's tables (chain_apply in
+// tdx_dev.h): v its value at x, d its derivative.
 template <typename T>
 __device__ __forceinline__ void chain_eval(const int32_t *I, int ref, const T *R, T x, T &v,
                                            T &d) {
-  const int n = I[ref];
-  v = x;
-  d = T(1);
-  for (int k = 0; k < n; ++k) {
-    const int op = I[ref + 1 + 2 * k], arg = I[ref + 2 + 2 * k];
-    switch (op) {
-      case TDX_AD_OP_ADD: v = v + R[arg]; break;
-      case TDX_AD_OP_SUB: v = v - R[arg]; break;
-      case TDX_AD_OP_RSUB: v = R[arg] - v; d = -d; break;
-      case TDX_AD_OP_MUL: v = R[arg] * v; d = R[arg] * d; break;
-      case TDX_AD_OP_DIV: v = v / R[arg]; d = d / R[arg]; break;
-      case TDX_AD_OP_RDIV: {
-        const T nv = R[arg] / v;
-        d = -(nv / v) * d;
-        v = nv;
-      } break;
-      case TDX_AD_OP_POWI:
-        d = (arg == 0 ? T(0) : T(arg) * ipow(v, arg - 1)) * d;
-        v = ipow(v, arg);
-        break;
-      case TDX_AD_OP_POWR: {
-        const T p = R[arg];
-        d = (p * dpow(v, p - T(1))) * d;
-        v = dpow(v, p);
-      } break;
-      case TDX_AD_OP_NEG: v = -v; d = -d; break;
-      case TDX_AD_OP_SIN: d = dcos(v) * d; v = dsin(v); break;
-      case TDX_AD_OP_COS: d = -dsin(v) * d; v = dcos(v); break;
-      case TDX_AD_OP_EXP: v = dexp(v); d = v * d; break;
-      case TDX_AD_OP_LOG: d = d / v; v = dlog(v); break;
-      case TDX_AD_OP_TANH: v = dtanh(v); d = (T(1) - v * v) * d; break;
-      case TDX_AD_OP_SQRT: v = dsqrt(v); d = d / (T(2) * v); break;
-      default: break;
-    }
-  }
+  chain_apply(PtrTab<T>{I, R}, ref, x, v, d);
 }
 
 // Whether a term's factors include a callable power.

@@ -89,6 +89,71 @@ __device__ __forceinline__ T ipow(T x, int p) {
   }
 }
 
+
+__device__ __forceinline__ float dsin(float x) { return sinf(x); }
+__device__ __forceinline__ double dsin(double x) { return sin(x); }
+__device__ __forceinline__ float dcos(float x) { return cosf(x); }
+__device__ __forceinline__ double dcos(double x) { return cos(x); }
+__device__ __forceinline__ float dexp(float x) { return expf(x); }
+__device__ __forceinline__ double dexp(double x) { return exp(x); }
+__device__ __forceinline__ float dlog(float x) { return logf(x); }
+__device__ __forceinline__ double dlog(double x) { return log(x); }
+__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+
+// Table access by word index: records (i) and real values (r).
+template <typename T>
+struct PtrTab {
+  const int32_t *I;
+  const T *R;
+  __device__ __forceinline__ int i(int k) const { return I[k]; }
+  __device__ __forceinline__ T r(int k) const { return R[k]; }
+};
+
+// A callable power of a residual program (the reference applies it to the
+// term's running product, tedeous/derivative.py:54-55, 128-129, 317-318):
+// the chain of one-argument operations at word ref of g's records -- n,
+// then n (op, arg) pairs, op TDX_AD_OP_*, arg a real's index or the integer
+// power -- applied to x in order; v its value, d its derivative (dual
+// numbers).
+template <typename T, typename G>
+__device__ __forceinline__ void chain_apply(const G &g, int ref, T x, T &v, T &d) {
+  const int n = g.i(ref);
+  v = x;
+  d = T(1);
+  for (int k = 0; k < n; ++k) {
+    const int op = g.i(ref + 1 + 2 * k), arg = g.i(ref + 2 + 2 * k);
+    switch (op) {
+      case TDX_AD_OP_ADD: v = v + g.r(arg); break;
+      case TDX_AD_OP_SUB: v = v - g.r(arg); break;
+      case TDX_AD_OP_RSUB: v = g.r(arg) - v; d = -d; break;
+      case TDX_AD_OP_MUL: v = g.r(arg) * v; d = g.r(arg) * d; break;
+      case TDX_AD_OP_DIV: v = v / g.r(arg); d = d / g.r(arg); break;
+      case TDX_AD_OP_RDIV: {
+        const T nv = g.r(arg) / v;
+        d = -(nv / v) * d;
+        v = nv;
+      } break;
+      case TDX_AD_OP_POWI:
+        d = (arg == 0 ? T(0) : T(arg) * ipow(v, arg - 1)) * d;
+        v = ipow(v, arg);
+        break;
+      case TDX_AD_OP_POWR: {
+        const T p = g.r(arg);
+        d = (p * dpow(v, p - T(1))) * d;
+        v = dpow(v, p);
+      } break;
+      case TDX_AD_OP_NEG: v = -v; d = -d; break;
+      case TDX_AD_OP_SIN: d = dcos(v) * d; v = dsin(v); break;
+      case TDX_AD_OP_COS: d = -dsin(v) * d; v = dcos(v); break;
+      case TDX_AD_OP_EXP: v = dexp(v); d = v * d; break;
+      case TDX_AD_OP_LOG: d = d / v; v = dlog(v); break;
+      case TDX_AD_OP_TANH: v = dtanh(v); d = (T(1) - v * v) * d; break;
+      case TDX_AD_OP_SQRT: v = dsqrt(v); d = d / (T(2) * v); break;
+      default: break;
+    }
+  }
+}
 }  // namespace tdx
 
 #endif  // TDX_DEV_H

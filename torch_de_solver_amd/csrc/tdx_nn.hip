@@ -1207,7 +1207,9 @@ __global__ void k_jac(NNArgs<T> a, T *jt, int64_t n_resid) {
       for (int tt = 0; tt < E[0]; ++tt) {
         const int32_t *Tm = a.ints + E[1] + tt * TDX_TERM_INTS;
         const int nf = Tm[0];
-        T gsv[MAXF], pw[MAXF], pre[MAXF + 1], suf[MAXF + 1];
+        const PtrTab<T> tab{a.ints, a.reals};
+        const bool chain = nn_term_has_chain(tab, Tm[1], nf);
+        T gsv[MAXF], pw[MAXF], pre[MAXF + 1], suf[MAXF + 1], dch[MAXF];
 #pragma unroll
         for (int f = 0; f < MAXF; ++f) {
           gsv[f] = T(0);
@@ -1220,9 +1222,10 @@ __global__ void k_jac(NNArgs<T> a, T *jt, int64_t n_resid) {
               gs = gs + uv[(S[0] * no + F[0]) * B + t] * a.reals[S[1]];
             }
             gsv[f] = gs;
-            pw[f] = fpow(a, F, gs);
+            if (!chain) pw[f] = fpow(a, F, gs);
           }
         }
+        if (chain) (void)nn_chain_term(tab, Tm[1], nf, gsv, dch);
         pre[0] = T(1);
 #pragma unroll
         for (int f = 0; f < MAXF; ++f) pre[f + 1] = pre[f] * pw[f];
@@ -1234,7 +1237,7 @@ __global__ void k_jac(NNArgs<T> a, T *jt, int64_t n_resid) {
         for (int f = 0; f < MAXF; ++f) {
           if (f < nf) {
             const int32_t *F = a.ints + Tm[1] + f * TDX_FACTOR_INTS;
-            const T g = c * dfpow(a, F, gsv[f]) * pre[f] * suf[f + 1];
+            const T g = chain ? c * dch[f] : c * dfpow(a, F, gsv[f]) * pre[f] * suf[f + 1];
             for (int k = 0; k < F[3]; ++k) {
               const int32_t *S = a.ints + F[4] + k * TDX_STENCIL_INTS;
               sd[(S[0] * no + F[0]) * B + t] += g * a.reals[S[1]];
@@ -1889,7 +1892,22 @@ int validate_nn(const tdx_nn_desc *d, int64_t *n_rows_out) {
         for (int f = 0; f < Tm[0]; ++f) {
           const int32_t *F = d->ints + Tm[1] + f * TDX_FACTOR_INTS;
           TDX_REQUIRE(F[0] >= 0 && F[0] < d->n_out, "factor variable out of range");
-          TDX_REQUIRE(F[1] == 0 || in_reals(F[5], 1), "power out of range");
+          if (F[1] == 2) {
+            // callable power: a chain of one-argument operations at F[2]
+            TDX_REQUIRE(in_ints(F[2], 1) && d->ints[F[2]] >= 0 &&
+                            d->ints[F[2]] <= TDX_AD_MAX_CHAIN &&
+                            in_ints(F[2] + 1, 2 * (int64_t)d->ints[F[2]]),
+                        "power chain out of range");
+            for (int k = 0; k < d->ints[F[2]]; ++k) {
+              const int op = d->ints[F[2] + 1 + 2 * k], arg = d->ints[F[2] + 2 + 2 * k];
+              TDX_REQUIRE(op >= TDX_AD_OP_ADD && op <= TDX_AD_OP_SQRT, "power chain op");
+              TDX_REQUIRE(!(op <= TDX_AD_OP_RDIV || op == TDX_AD_OP_POWR) || in_reals(arg, 1),
+                          "power chain constant out of range");
+            }
+          } else {
+            TDX_REQUIRE((F[1] == 0 || F[1] == 1) && (F[1] == 0 || in_reals(F[5], 1)),
+                        "power out of range");
+          }
           TDX_REQUIRE(F[3] >= 1 && in_ints(F[4], (int64_t)F[3] * TDX_STENCIL_INTS), "stencil");
           for (int k = 0; k < F[3]; ++k) {
             const int32_t *S = d->ints + F[4] + k * TDX_STENCIL_INTS;
@@ -1948,6 +1966,7 @@ bool fused_programs(const tdx_nn_desc *d, std::vector<FusedProg> *out) {
         mi.resize(fo + (size_t)Tm[0] * TDX_FACTOR_INTS);
         for (int f = 0; f < Tm[0]; ++f) {
           const int32_t *F = d->ints + Tm[1] + f * TDX_FACTOR_INTS;
+          if (F[1] == 2) return false;  // callable powers: the separate kernels
           const size_t ff = fo + (size_t)f * TDX_FACTOR_INTS;
           for (int k = 0; k < TDX_FACTOR_INTS; ++k) mi[ff + k] = F[k];
           mi[ff + 5] = F[1] != 0 ? real(F[5]) : 0;

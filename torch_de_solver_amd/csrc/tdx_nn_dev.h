@@ -313,7 +313,63 @@ struct PtrProg {
 // (times mult[eq * mult_n + e] when mult is given) are added through
 // sd_add (accumulators zeroed by the caller) through every factor of every
 // term: d(prod_j f_j^p_j)/d f_k = p_k f_k^(p_k-1) * prefix_k * suffix_k.
+// A term with a callable power (factor kind 2, its operation chain at word
+// F + 2 of the factor record): the reference's running product P = g(P * x)
+// at a callable factor, P = P * x ** p at the others, P from 1
+// (tedeous/derivative.py:54-55); returns P and writes dP/dx of each factor
+// into dx (zero beyond nf).
 template <typename T, typename A>
+__device__ __forceinline__ T nn_chain_term(const A &g, int fi, int nf, const T (&x)[MAXF],
+                                           T (&dx)[MAXF]) {
+  T P = T(1), G[MAXF], M[MAXF];
+#pragma unroll
+  for (int f = 0; f < MAXF; ++f) {
+    G[f] = T(0);
+    M[f] = T(1);
+    if (f < nf) {
+      const int F = fi + f * TDX_FACTOR_INTS;
+      const int kind = g.i(F + 1);
+      if (kind == 2) {
+        T v, d;
+        chain_apply(g, g.i(F + 2), P * x[f], v, d);
+        G[f] = d * P;
+        M[f] = d * x[f];
+        P = v;
+      } else {
+        T pw, dp;
+        if (kind == 0) {
+          const int p = g.i(F + 2);
+          pw = ipow(x[f], p);
+          dp = p == 0 ? T(0) : T(p) * ipow(x[f], p - 1);
+        } else {
+          const T p = g.r(g.i(F + 5));
+          pw = dpow(x[f], p);
+          dp = p * dpow(x[f], p - T(1));
+        }
+        G[f] = P * dp;
+        M[f] = pw;
+        P = P * pw;
+      }
+    }
+  }
+  T gg = T(1);
+#pragma unroll
+  for (int f = MAXF - 1; f >= 0; --f) {
+    dx[f] = gg * G[f];
+    gg = gg * M[f];
+  }
+  return P;
+}
+template <typename A>
+__device__ __forceinline__ bool nn_term_has_chain(const A &g, int fi, int nf) {
+  bool c = false;
+  for (int f = 0; f < nf; ++f) c |= g.i(fi + f * TDX_FACTOR_INTS + 1) == 2;
+  return c;
+}
+
+// CHAIN: whether terms may hold callable powers (the fused kernels' tables
+// never do: such plans run the separate kernels).
+template <typename T, typename A, bool CHAIN = true>
 __device__ __forceinline__ void entry_program(A &&g, int eq_i, int n_eq, int e, int n_out,
                                               T &lsum, T &lnsum, T *resid, const T *mult,
                                               int mult_n) {
@@ -349,9 +405,16 @@ __device__ __forceinline__ void entry_program(A &&g, int eq_i, int n_eq, int e, 
       const int Tm = term_i + t * TDX_TERM_INTS;
       const int nf = g.i(Tm), fi = g.i(Tm + 1);
       T der = T(1);
-      for (int f = 0; f < nf; ++f) {
-        const int F = fi + f * TDX_FACTOR_INTS;
-        der = der * fpw(F, stencil_sum(F));
+      if (CHAIN && nn_term_has_chain(g, fi, nf)) {
+        T x[MAXF], dx[MAXF];
+#pragma unroll
+        for (int f = 0; f < MAXF; ++f) x[f] = f < nf ? stencil_sum(fi + f * TDX_FACTOR_INTS) : T(0);
+        der = nn_chain_term(g, fi, nf, x, dx);
+      } else {
+        for (int f = 0; f < nf; ++f) {
+          const int F = fi + f * TDX_FACTOR_INTS;
+          der = der * fpw(F, stencil_sum(F));
+        }
       }
       const T tv = coeff(Tm) * der;
       r = (t == 0) ? tv : r + tv;
@@ -369,7 +432,8 @@ __device__ __forceinline__ void entry_program(A &&g, int eq_i, int n_eq, int e, 
     for (int t = 0; t < n_terms; ++t) {
       const int Tm = term_i + t * TDX_TERM_INTS;
       const int nf = g.i(Tm), fi = g.i(Tm + 1);
-      T gsv[MAXF], pw[MAXF], pre[MAXF + 1], suf[MAXF + 1];
+      T gsv[MAXF], pw[MAXF], pre[MAXF + 1], suf[MAXF + 1], dch[MAXF];
+      const bool chain = CHAIN && nn_term_has_chain(g, fi, nf);
 #pragma unroll
       for (int f = 0; f < MAXF; ++f) {
         gsv[f] = T(0);
@@ -377,9 +441,10 @@ __device__ __forceinline__ void entry_program(A &&g, int eq_i, int n_eq, int e, 
         if (f < nf) {
           const int F = fi + f * TDX_FACTOR_INTS;
           gsv[f] = stencil_sum(F);
-          pw[f] = fpw(F, gsv[f]);
+          if (!chain) pw[f] = fpw(F, gsv[f]);
         }
       }
+      if (chain) (void)nn_chain_term(g, fi, nf, gsv, dch);
       pre[0] = T(1);
 #pragma unroll
       for (int f = 0; f < MAXF; ++f) pre[f + 1] = pre[f] * pw[f];
@@ -391,7 +456,7 @@ __device__ __forceinline__ void entry_program(A &&g, int eq_i, int n_eq, int e, 
       for (int f = 0; f < MAXF; ++f) {
         if (f < nf) {
           const int F = fi + f * TDX_FACTOR_INTS;
-          const T gf = cdr * dfpw(F, gsv[f]) * pre[f] * suf[f + 1];
+          const T gf = chain ? cdr * dch[f] : cdr * dfpw(F, gsv[f]) * pre[f] * suf[f + 1];
           const int var = g.i(F), ns = g.i(F + 3), si = g.i(F + 4);
           for (int k = 0; k < ns; ++k) {
             const int S = si + k * TDX_STENCIL_INTS;
@@ -499,8 +564,9 @@ struct InterpResid {
 #pragma unroll
     for (int s = 0; s < FUSE_MAX_SLOTS; ++s) sdr[s] = 0.0f;
     if (valid)
-      entry_program<float>(PtrProg<float>{pints, preals, data, u, sdr}, 0, n_eq, e, 1, ls, lns,
-                           (float *)nullptr, (const float *)nullptr, 0);
+      entry_program<float, PtrProg<float>, false>(PtrProg<float>{pints, preals, data, u, sdr}, 0,
+                                                  n_eq, e, 1, ls, lns, (float *)nullptr,
+                                                  (const float *)nullptr, 0);
 #pragma unroll
     for (int s = 0; s < FUSE_MAX_SLOTS; ++s) sdv[s] = sdr[s];
   }
@@ -843,7 +909,7 @@ struct BlockInterp {
   // (zeroed by the caller); u: the entry's network values.
   __device__ __forceinline__ void run(int n_eq, int e, int n_out, const T *u, T *sd, T &ls,
                                       T &lns) {
-    entry_program<T>(PtrProg<T>{pints, preals, data, u, sd}, 0, n_eq, e, n_out, ls, lns,
+    entry_program<T, PtrProg<T>, false>(PtrProg<T>{pints, preals, data, u, sd}, 0, n_eq, e, n_out, ls, lns,
                      (T *)nullptr, (const T *)nullptr, 0);
   }
 };

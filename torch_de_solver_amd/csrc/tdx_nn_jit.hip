@@ -20,6 +20,7 @@
 #include <map>
 #include <mutex>
 #include <sstream>
+#include <utility>
 
 #include "tdx_common.h"
 #include "tdx_nn_dev.h"
@@ -161,8 +162,15 @@ void emit_prelude(std::ostringstream &o) {
     << "#define TDX_EQ_INTS " << TDX_EQ_INTS << "\n"
     << "#define TDX_TERM_INTS " << TDX_TERM_INTS << "\n"
     << "#define TDX_FACTOR_INTS " << TDX_FACTOR_INTS << "\n"
-    << "#define TDX_STENCIL_INTS " << TDX_STENCIL_INTS << "\n"
-    << kTdxDevSrc << "\n"
+    << "#define TDX_STENCIL_INTS " << TDX_STENCIL_INTS << "\n";
+  const std::pair<const char *, int> ops[] = {
+      {"ADD", TDX_AD_OP_ADD},   {"SUB", TDX_AD_OP_SUB},   {"RSUB", TDX_AD_OP_RSUB},
+      {"MUL", TDX_AD_OP_MUL},   {"DIV", TDX_AD_OP_DIV},   {"RDIV", TDX_AD_OP_RDIV},
+      {"POWI", TDX_AD_OP_POWI}, {"POWR", TDX_AD_OP_POWR}, {"NEG", TDX_AD_OP_NEG},
+      {"SIN", TDX_AD_OP_SIN},   {"COS", TDX_AD_OP_COS},   {"EXP", TDX_AD_OP_EXP},
+      {"LOG", TDX_AD_OP_LOG},   {"TANH", TDX_AD_OP_TANH}, {"SQRT", TDX_AD_OP_SQRT}};
+  for (const auto &op : ops) o << "#define TDX_AD_OP_" << op.first << " " << op.second << "\n";
+  o << kTdxDevSrc << "\n"
     << kTdxNnDevSrc << "\n";
 }
 

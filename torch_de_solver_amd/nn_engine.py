@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import native
-from .compiler import NNProblem
+from .compiler import AD_OP, NNProblem
 
 
 # ----------------------------------------------------------------------------
@@ -127,9 +127,19 @@ class NNTables:
                             st_i = len(ints)
                             for slot, w in f.stencil:
                                 ints.extend([slot, add_real(np_dtype(w))])
-                            pow_r = add_real(f.pow_val)
-                            rec = [f.var, f.pow_kind, int(f.pow_val) if f.pow_kind == 0 else 0,
-                                   len(f.stencil), st_i, pow_r]
+                            if f.pow_kind == 2:
+                                # callable power: n, then (op, arg) pairs at
+                                # the record's integer word
+                                pow_i, pow_r = len(ints), 0
+                                ints.append(len(f.pow_val))
+                                for op, arg in f.pow_val:
+                                    ints.extend([op, int(arg) if op == AD_OP['POWI'] else
+                                                 (add_real(np_dtype(arg)) if op <= AD_OP['RDIV']
+                                                  or op == AD_OP['POWR'] else 0)])
+                            else:
+                                pow_r = add_real(f.pow_val)
+                                pow_i = int(f.pow_val) if f.pow_kind == 0 else 0
+                            rec = [f.var, f.pow_kind, pow_i, len(f.stencil), st_i, pow_r]
                             ints[factor_i + fi * native.TDX_FACTOR_INTS:
                                  factor_i + (fi + 1) * native.TDX_FACTOR_INTS] = rec
                         if t.coeff_array is not None:
