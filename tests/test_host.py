@@ -150,9 +150,19 @@ def test_unsupported_features_raise():
     bc = b.build(d.variable_dict)
     bad = [dict(eq.equation_lst[0])]
     bad[0] = {k: dict(v) for k, v in bad[0].items()}
-    bad[0]['du/dt']['pow'] = lambda x: x
+    # a callable power that is not a chain of one-argument operations; any
+    # callable power in 'mat' mode
+    bad[0]['du/dt']['pow'] = lambda x: x * torch.sin(x)
     with pytest.raises(NotImplementedError):
         compile_nn(grid, bad, bc, 1)
+    bad[0]['du/dt']['pow'] = lambda x: x
+    compile_nn(grid, bad, bc, 1)
+    dm, eqm, bm, kwm = problems.wave_mat(api)
+    badm = [{k: dict(v) for k, v in eqm.equation_lst[0].items()}]
+    key = next(k for k, v in badm[0].items() if 'pow' in v)
+    badm[0][key]['pow'] = lambda x: x
+    with pytest.raises(NotImplementedError, match="'NN' and 'autograd'"):
+        compile_mat(dm.build('mat'), badm, bm.build(dm.variable_dict), 1)
     with pytest.raises(NotImplementedError):
         compile_nn(grid, eq.equation_lst, bc + [{'type': 'robin', 'bnd': bc[0]['bnd'],
                                                  'bop': None, 'bval': bc[0]['bval'], 'var': 0}], 1)
